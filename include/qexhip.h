@@ -1,0 +1,153 @@
+/*
+ * qexhip.h -- C ABI of the MI355X (gfx950) staggered Dslash / CG / multi-shift
+ * CG / fermion-force library.  This is the drop-in seam for QEX's solver
+ * backend switch (QEX src/solvers/solverBase.nim:6-7, dispatched in
+ * src/physics/stagSolve.nim:98-157), in the same place QEX plugs in QUDA
+ * (src/quda/qudaWrapperImpl.nim:171-273) and Grid (src/grid/gridStag.nim:5-83).
+ *
+ * Conventions (identical to QEX's):
+ *  - Site order: QEX's canonical single-rank Layout[1] order, even sites first:
+ *    idx = (lex + p*V)/2, lex = x + Lx*(y + Ly*(z + Lz*t)), p = (x+y+z+t)&1
+ *    (src/layout/qlayout.nim:108-136).  All lattice extents must be even.
+ *  - Host "canonical" arrays: color-vector fields are V x 3 complex doubles
+ *    (re,im interleaved) = 6 doubles/site; link fields are nl x V x 3x3
+ *    complex doubles, row-major = 18 doubles/site/link.
+ *  - Links are PHASED and BC-applied, exactly as QEX's Staggered.g holds them
+ *    (src/physics/stagD.nim:560-571, src/gauge/gaugeUtils.nim:133-140).
+ *    nl = 4: plain, g[mu];  nl = 8: HISQ form, g[2mu] fat (1 hop),
+ *    g[2mu+1] long (3 hops) (src/physics/stagD.nim:40-51,603-615).
+ *  - The operator solved by the parity solvers is QEX's
+ *    A = 4(m^2 - D_pq D_qp) (stagD2ee/stagD2oo, src/physics/stagD.nim:485-520).
+ *  - Parity: 0 even, 1 odd, 2 all.
+ *
+ * Error contract: every function returns 0 on success and a non-zero code on
+ * failure; qexhip_last_error() returns the message (thread-local).  QEX's shim
+ * maps a non-zero return to qexError (src/base/qexInternal.nim:39-48).
+ * Calls are synchronous with respect to the host (results are complete on
+ * return) unless stated otherwise, and not re-entrant, like QEX's backend
+ * calls (made on the master thread outside threads:, stagSolve.nim:396).
+ * Device pointers passed in must hold completed data.
+ */
+#ifndef QEXHIP_H
+#define QEXHIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qexhip_lattice qexhip_lattice;
+typedef struct qexhip_links qexhip_links;
+typedef struct qexhip_field qexhip_field;
+
+/* storage precisions */
+enum { QEXHIP_FP64 = 0, QEXHIP_FP32 = 1, QEXHIP_FP16 = 2, QEXHIP_BF16 = 3 };
+enum { QEXHIP_EVEN = 0, QEXHIP_ODD = 1, QEXHIP_ALL = 2 };
+
+/* Solver parameters: inputs and outputs mirror QEX's SolverParams
+   (src/solvers/solverBase.nim:10-28). */
+typedef struct {
+  /* inputs */
+  double r2req;           /* relative squared residual |b-Ax|^2/|b|^2 (QEX r2req) */
+  int maxits;             /* iteration cap (QEX maxits) */
+  int sloppy;             /* QEXHIP_FP64: plain FP64 CG (SloppyNone); FP32/FP16/BF16:
+                             FP32-arithmetic inner CG with this spinor/link storage and
+                             FP64 reliable updates (QEX sloppySolve, solverBase.nim:8-9) */
+  int sloppy_links;       /* link storage of the inner operator; -1 = same as sloppy */
+  double reliable_delta;  /* reliable update when |r| < delta*|r_last_update| (0.1) */
+  int reliable_max_its;   /* ... or after this many iterations (100) */
+  int use_prev_soln;      /* 1: the solution field holds the initial guess (QEX usePrevSoln) */
+  int verbosity;          /* QEX verbosity */
+  /* outputs */
+  int iterations;         /* QEX iterations */
+  int reliable_updates;   /* FP64 reliable updates performed (mixed mode) */
+  double r2;              /* final FP64 true |b-Ax|^2/|b|^2 */
+  double seconds;         /* wall time of the call */
+  double flops;           /* QEX flop count (nl*4*72+60)*nEven*its (stagSolve.nim:121) */
+} qexhip_solver_params;
+
+/* ---- library lifetime (QEX qexGlobalInitializers/Finalizers pattern,
+        src/quda/qudaWrapperImpl.nim:71-92) ---- */
+/* replaces qudaInit (qudaWrapperImpl.nim:71-84): selects the GPU, creates the stream */
+int qexhip_init(int device);
+/* replaces qudaFini (qudaWrapperImpl.nim:86-89) */
+int qexhip_finalize(void);
+const char *qexhip_last_error(void);
+/* wait for all work queued by the library */
+int qexhip_sync(void);
+/* the library's HIP stream (hipStream_t), for callers that order work against it */
+void *qexhip_stream(void);
+
+/* ---- lattice (replaces qudaSetup's layout, qudaWrapperImpl.nim:94-127) ---- */
+int qexhip_lattice_create(const int dims[4], qexhip_lattice **out);
+int qexhip_lattice_destroy(qexhip_lattice *lat);
+
+/* ---- links: device-resident copy of Staggered.g (newStag/newStag3,
+        src/physics/stagD.nim:573-615; repack as qudaWrapperImpl.nim:222-249) ---- */
+int qexhip_links_create(qexhip_lattice *lat, int nl, const double *canon, int canon_on_device,
+                        qexhip_links **out);
+/* new gauge generation in place (same nl) */
+int qexhip_links_update(qexhip_links *l, const double *canon, int canon_on_device);
+int qexhip_links_destroy(qexhip_links *l);
+
+/* ---- color-vector fields (QEX Field[V, ColorVector], src/field/fieldET.nim:17-22) ---- */
+int qexhip_field_create(qexhip_lattice *lat, int prec, qexhip_field **out);
+int qexhip_field_destroy(qexhip_field *f);
+/* canonical V x 6 doubles; only the sites of parity par are read / written */
+int qexhip_field_import(qexhip_field *f, int par, const double *canon, int canon_on_device);
+int qexhip_field_export(const qexhip_field *f, int par, double *canon, int canon_on_device);
+/* FP64-accumulated norm2 / redot on a parity (fieldET.nim:648-778) */
+int qexhip_field_norm2(const qexhip_field *f, int par, double *out);
+int qexhip_field_redot(const qexhip_field *x, const qexhip_field *y, int par, double *out);
+
+/* ---- operator (src/physics/stagD.nim) ----
+   All fields of one call share one precision; arithmetic is FP64 for FP64
+   fields and FP32 otherwise. */
+/* stagD2 (stagD.nim:383-448): r = a*r + b*x + (2D)x on parity par */
+int qexhip_stagD2(qexhip_links *l, int par, qexhip_field *r, const qexhip_field *x, double a, double b);
+/* stagD (stagD.nim:459-462): r = a*r + m*x + sc*D*x, D = (2D)/2 */
+int qexhip_stagD(qexhip_links *l, int par, qexhip_field *r, const qexhip_field *x, double m, double sc,
+                 double a);
+/* stagD2ee / stagD2oo (stagD.nim:485-520): r_p = 4*m2*x_p - (2D)_pq (2D)_qp x_p */
+int qexhip_stagD2xx(qexhip_links *l, int par, qexhip_field *r, const qexhip_field *x, double m2);
+
+/* ---- solvers ---- */
+/* solveEE / solveOO (src/physics/stagSolve.nim:330-427): solve A r = x on
+   parity par with A = 4(m^2 - D_pq D_qp); FP64 fields.  Outer defect-
+   correction loop as in solveXX; inner solve per sp->sloppy. */
+int qexhip_solve_xx(qexhip_links *l, int par, qexhip_field *r, const qexhip_field *x, double m,
+                    qexhip_solver_params *sp);
+/* QUDA-adapter-shaped host entry (qudaSolveEE/qudaSolveOO,
+   qudaWrapperImpl.nim:171-273): canonical host arrays of V x 6 doubles. */
+int qexhip_solve_ee_host(qexhip_links *l, double *r, const double *t, double m, qexhip_solver_params *sp);
+int qexhip_solve_oo_host(qexhip_links *l, double *r, const double *t, double m, qexhip_solver_params *sp);
+/* multi-shift CG on a parity (solveXXInternal multi-shift, stagSolve.nim:164-229,
+   and CgmState.solve, src/solvers/cgm.nim:68-246): solves
+   (A + sigma_k) x_k = b, k = 0..nshift-1, with sigma_0 treated as 0. */
+int qexhip_solve_multishift(qexhip_links *l, int par, qexhip_field *const *xs, const qexhip_field *b, double m,
+                            const double *sigmas, int nshift, qexhip_solver_params *sp);
+
+/* ---- fermion force (src/hmc/hmcAction.nim:669-792) ---- */
+/* stagForceSolve (hmcAction.nim:669-721): psi_e = (m^2 - D_eo D_oe)^-1 phi_e,
+   psi_o = (2D)_oe psi_e */
+int qexhip_force_solve(qexhip_links *l, qexhip_field *psi, const qexhip_field *phi, double m,
+                       qexhip_solver_params *sp);
+/* fermForce, unsmeared (hmcAction.nim:723-792): F_mu(x) = TAH(eps(x) psi(x) psi(x+mu)^+ U_mu(x)^+);
+   outer_only=1 returns eps(x) psi(x) psi(x+mu)^+ instead.  F: canonical 4 x V x 18 doubles. */
+int qexhip_force(qexhip_links *l, const qexhip_field *psi, double *F, int F_on_device, int outer_only);
+
+/* ---- synthetic inputs (no QEX counterpart; QEX's random fields use its own
+        RNG, src/rng).  Outputs are canonical FP64 arrays in device memory. ---- */
+int qexhip_gen_gauge(qexhip_lattice *lat, uint64_t seed, double *U_dev);
+int qexhip_gen_source(qexhip_lattice *lat, uint64_t seed, int stream, int par, double *x_dev);
+int qexhip_gen_phase_bc(qexhip_lattice *lat, double *U_dev);
+int qexhip_gen_hisq(qexhip_lattice *lat, uint64_t seed, const double *Uph_dev, double *g8_dev);
+
+/* ---- measurement (the in-module bench of stagD.nim:767-810) ----
+   average duration (ms, HIP events on the library stream) of one one-parity
+   hop r_e = (2D)_eo x_o with links and fields stored in precision prec. */
+int qexhip_time_hop(qexhip_links *l, int prec, int nrep, double *ms_per_hop);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

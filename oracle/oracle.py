@@ -1,0 +1,245 @@
+"""ctypes binding of the CPU oracle (oracle/qexoracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product package qex_amd
+never imports it.
+
+Fields are numpy complex128 arrays in QEX's canonical single-rank site order
+(even sites first): color vectors (V, 3), link fields (nl, V, 3, 3).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so not built (run `make -C oracle`)")
+        _LIB = ctypes.CDLL(path)
+        _declare(_LIB)
+    return _LIB
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _declare(l):
+    I, D, L, U64, U32 = ctypes.c_int, ctypes.c_double, ctypes.c_long, ctypes.c_uint64, ctypes.c_uint32
+    sig = {
+        "qo_volume": (L, [_ip]),
+        "qo_index": (L, [_ip, _ip]),
+        "qo_coords": (None, [_ip, L, _ip]),
+        "qo_philox4x32_10": (None, [ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32)]),
+        "qo_gauss2": (None, [U64, U32, U32, U32, U32, _dp]),
+        "qo_gauge_random": (None, [_ip, U64, _dp]),
+        "qo_gauge_unit": (None, [_ip, _dp]),
+        "qo_source_gauss": (None, [_ip, U64, I, I, _dp]),
+        "qo_set_bc": (None, [_ip, _dp]),
+        "qo_stag_phase": (None, [_ip, _dp]),
+        "qo_hisq_links": (None, [_ip, U64, _dp, _dp]),
+        "qo_stagD2": (None, [_ip, I, _dp, I, _dp, _dp, D, D]),
+        "qo_stagD": (None, [_ip, I, _dp, I, _dp, _dp, D, D, D]),
+        "qo_stagD2xx": (None, [_ip, I, _dp, I, _dp, _dp, D, _dp]),
+        "qo_D": (None, [_ip, I, _dp, _dp, _dp, D]),
+        "qo_Ddag": (None, [_ip, I, _dp, _dp, _dp, D]),
+        "qo_norm2": (D, [L, _dp]),
+        "qo_redot": (D, [L, _dp, _dp]),
+        "qo_cg": (I, [_ip, I, _dp, I, _dp, _dp, D, D, I, _dp]),
+        "qo_solveXX": (I, [_ip, I, _dp, I, _dp, _dp, D, D, I, _dp]),
+        "qo_cgm": (I, [_ip, I, _dp, I, _dp, _dp, D, _dp, I, D, I, _dp]),
+        "qo_ferm_force": (None, [_ip, _dp, _dp, _dp]),
+        "qo_ferm_outer": (None, [_ip, _dp, _dp]),
+        "qo_force_solve": (I, [_ip, I, _dp, _dp, D, D, I, _dp]),
+        "qo_su3_exp_i": (None, [_dp, D, _dp]),
+        "qo_su3_exp": (None, [_dp, _dp]),
+        "qo_mat_mul": (None, [_dp, _dp, _dp]),
+        "qo_project_tah": (None, [_dp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(l, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _L(L):
+    return (ctypes.c_int * 4)(*[int(v) for v in L])
+
+
+def _p(a):
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(_dp)
+
+
+def volume(L):
+    return int(np.prod(L))
+
+
+def coords(L, idx):
+    c = (ctypes.c_int * 4)()
+    lib().qo_coords(_L(L), int(idx), c)
+    return tuple(c)
+
+
+def index(L, x):
+    return int(lib().qo_index(_L(L), (ctypes.c_int * 4)(*x)))
+
+
+def all_coords(L):
+    """(V, 4) int array of coordinates of every canonical site index."""
+    V = volume(L)
+    lx = np.arange(V)
+    x = lx % L[0]
+    y = (lx // L[0]) % L[1]
+    z = (lx // (L[0] * L[1])) % L[2]
+    t = lx // (L[0] * L[1] * L[2])
+    p = (x + y + z + t) & 1
+    idx = (lx + p * V) // 2
+    out = np.empty((V, 4), dtype=np.int64)
+    out[idx] = np.stack([x, y, z, t], axis=1)
+    return out
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().qo_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
+def gauge_random(L, seed):
+    U = np.zeros((4, volume(L), 3, 3), dtype=np.complex128)
+    lib().qo_gauge_random(_L(L), seed, _p(U))
+    return U
+
+
+def gauge_unit(L):
+    U = np.zeros((4, volume(L), 3, 3), dtype=np.complex128)
+    lib().qo_gauge_unit(_L(L), _p(U))
+    return U
+
+
+def source_gauss(L, seed, stream=0, par=0):
+    x = np.zeros((volume(L), 3), dtype=np.complex128)
+    lib().qo_source_gauss(_L(L), seed, stream, par, _p(x))
+    return x
+
+
+def set_bc(L, U):
+    lib().qo_set_bc(_L(L), _p(U))
+
+
+def stag_phase(L, U):
+    lib().qo_stag_phase(_L(L), _p(U))
+
+
+def phased(L, U):
+    """setBC then stagPhase on a copy (QEX's rephase order, stagD.nim:106-109)."""
+    W = U.copy()
+    set_bc(L, W)
+    stag_phase(L, W)
+    return W
+
+
+def hisq_links(L, seed, Uph):
+    g8 = np.zeros((8, volume(L), 3, 3), dtype=np.complex128)
+    lib().qo_hisq_links(_L(L), seed, _p(np.ascontiguousarray(Uph)), _p(g8))
+    return g8
+
+
+def stagD2(L, g, par, r, x, a, b):
+    lib().qo_stagD2(_L(L), g.shape[0], _p(g), par, _p(r), _p(x), a, b)
+
+
+def stagD(L, g, par, r, x, m, sc=1.0, a=0.0):
+    lib().qo_stagD(_L(L), g.shape[0], _p(g), par, _p(r), _p(x), m, sc, a)
+
+
+def stagD2xx(L, g, par, x, m2):
+    r = np.zeros_like(x)
+    t = np.zeros_like(x)
+    lib().qo_stagD2xx(_L(L), g.shape[0], _p(g), par, _p(r), _p(x), m2, _p(t))
+    return r
+
+
+def D(L, g, x, m):
+    r = np.zeros_like(x)
+    lib().qo_D(_L(L), g.shape[0], _p(g), _p(r), _p(x), m)
+    return r
+
+
+def Ddag(L, g, x, m):
+    r = np.zeros_like(x)
+    lib().qo_Ddag(_L(L), g.shape[0], _p(g), _p(r), _p(x), m)
+    return r
+
+
+def cg(L, g, par, b, m, r2req, maxits=100000, x0=None):
+    x = np.zeros_like(b) if x0 is None else x0.copy()
+    r2 = ctypes.c_double()
+    its = lib().qo_cg(_L(L), g.shape[0], _p(g), par, _p(x), _p(b), m, r2req, maxits, ctypes.byref(r2))
+    return x, its, r2.value
+
+
+def solveXX(L, g, par, b, m, r2req, maxits=100000):
+    r = np.zeros_like(b)
+    tr = ctypes.c_double()
+    its = lib().qo_solveXX(_L(L), g.shape[0], _p(g), par, _p(r), _p(b), m, r2req, maxits, ctypes.byref(tr))
+    return r, its, tr.value
+
+
+def cgm(L, g, par, b, m, sigmas, r2req, maxits=100000):
+    sig = np.ascontiguousarray(sigmas, dtype=np.float64)
+    xs = np.zeros((len(sig),) + b.shape, dtype=np.complex128)
+    r2 = ctypes.c_double()
+    its = lib().qo_cgm(_L(L), g.shape[0], _p(g), par, _p(xs), _p(b), m, _p(sig), len(sig),
+                       r2req, maxits, ctypes.byref(r2))
+    return xs, its, r2.value
+
+
+def ferm_force(L, U, psi):
+    F = np.zeros((4, volume(L), 3, 3), dtype=np.complex128)
+    lib().qo_ferm_force(_L(L), _p(U), _p(psi), _p(F))
+    return F
+
+
+def ferm_outer(L, psi):
+    G = np.zeros((4, volume(L), 3, 3), dtype=np.complex128)
+    lib().qo_ferm_outer(_L(L), _p(psi), _p(G))
+    return G
+
+
+def force_solve(L, g, phi, m, r2req, maxits=100000):
+    psi = np.zeros_like(phi)
+    its = lib().qo_force_solve(_L(L), g.shape[0], _p(g), _p(phi), m, r2req, maxits, _p(psi))
+    return psi, its
+
+
+def su3_exp_i(H, s):
+    U = np.zeros((3, 3), dtype=np.complex128)
+    lib().qo_su3_exp_i(_p(np.ascontiguousarray(H, dtype=np.complex128)), s, _p(U))
+    return U
+
+
+def su3_exp(A):
+    U = np.zeros((3, 3), dtype=np.complex128)
+    lib().qo_su3_exp(_p(np.ascontiguousarray(A, dtype=np.complex128)), _p(U))
+    return U
+
+
+def project_tah(X):
+    R = np.zeros((3, 3), dtype=np.complex128)
+    lib().qo_project_tah(_p(np.ascontiguousarray(X, dtype=np.complex128)), _p(R))
+    return R
+
+
+def norm2(x):
+    return float(np.vdot(x, x).real)

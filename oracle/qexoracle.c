@@ -1,0 +1,575 @@
+/*
+ * qexoracle.c -- CPU restatement of QEX's staggered Dslash / CG / CGM / force.
+ *
+ * TEST INFRASTRUCTURE ONLY (the parity checker and the CPU baseline); see
+ * qexoracle.h for the conventions.  Every routine cites the QEX source it
+ * restates (paths relative to QEX's repository root).
+ *
+ * Parity status of each routine (what pins it, tests/test_oracle_*.py):
+ *  qo_philox4x32_10 : Random123 known-answer vectors.
+ *  qo_stagD2/qo_stagD/qo_D/qo_Ddag/qo_stagD2xx : closed-form point-source norms
+ *      (stagD.nim:735-752), free-field spectrum via FFT, dense 4^4 matrix
+ *      (anti-Hermiticity, eps-symmetry, spectrum m +- i lambda), gauge covariance.
+ *  qo_cg/qo_solveXX : dense LU solve on 4^4, FFT solution in the free field,
+ *      true-residual recomputation.
+ *  qo_cgm           : every shifted system's own residual, agreement with
+ *      independent single-shift CG.
+ *  qo_ferm_force    : central finite differences along random su(3)
+ *      directions, gauge covariance.
+ */
+#include "qexoracle.h"
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double complex cplx;
+
+/* ------------------------------------------------------------------ layout */
+
+long qo_volume(const int L[4]) { return (long)L[0] * L[1] * L[2] * L[3]; }
+
+/* layout/qlayout.nim:108-136 with innerGeom = 1, one rank */
+long qo_index(const int L[4], const int x[4]) {
+  long V = qo_volume(L);
+  long lex = x[0] + (long)L[0] * (x[1] + (long)L[1] * (x[2] + (long)L[2] * x[3]));
+  int p = (x[0] + x[1] + x[2] + x[3]) & 1;
+  return (lex + p * V) / 2;
+}
+
+void qo_coords(const int L[4], long idx, int x[4]) {
+  long Vh = qo_volume(L) / 2;
+  int p = idx >= Vh;
+  long i = idx - p * Vh;
+  long hx = L[0] / 2;
+  long xh = i % hx, r = i / hx;
+  x[1] = (int)(r % L[1]); r /= L[1];
+  x[2] = (int)(r % L[2]);
+  x[3] = (int)(r / L[2]);
+  x[0] = (int)(2 * xh + ((p + x[1] + x[2] + x[3]) & 1));
+}
+
+static inline long nbr(const int L[4], const int c[4], int mu, int d) {
+  int y[4] = {c[0], c[1], c[2], c[3]};
+  y[mu] = ((y[mu] + d) % L[mu] + L[mu]) % L[mu];
+  return qo_index(L, y);
+}
+
+static inline void par_range(const int L[4], int par, long *lo, long *hi) {
+  long V = qo_volume(L), Vh = V / 2;
+  *lo = (par == 1) ? Vh : 0;
+  *hi = (par == 0) ? Vh : V;
+}
+
+/* --------------------------------------------------------------- generator */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11). */
+void qo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int rnd = 0; rnd < 10; rnd++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* uniform in (0,1) from 53 bits of (a,b); Box-Muller pair */
+void qo_gauss2(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+               double g[2]) {
+  uint32_t ctr[4] = {c0, c1, c2, c3};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t o[4];
+  qo_philox4x32_10(ctr, key, o);
+  uint64_t a = (((uint64_t)o[0] << 32) | o[1]) >> 11;
+  uint64_t b = (((uint64_t)o[2] << 32) | o[3]) >> 11;
+  double u1 = ((double)a + 0.5) * 0x1.0p-53;
+  double u2 = ((double)b + 0.5) * 0x1.0p-53;
+  double rad = sqrt(-2.0 * log(u1));
+  double th = 6.283185307179586476925286766559 * u2;
+  g[0] = rad * cos(th);
+  g[1] = rad * sin(th);
+}
+
+/* ------------------------------------------------------------ 3x3 algebra */
+void qo_mat_mul(const double *a, const double *b, double *c) {
+  const cplx *A = (const cplx *)a, *B = (const cplx *)b;
+  cplx C[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      cplx s = 0;
+      for (int k = 0; k < 3; k++) s += A[3 * i + k] * B[3 * k + j];
+      C[3 * i + j] = s;
+    }
+  memcpy(c, C, sizeof C);
+}
+
+/* exp(A) for a general complex 3x3 A: scaling and squaring with a degree-16
+   Taylor polynomial (Horner form) */
+void qo_su3_exp(const double *a, double *u) {
+  const cplx *A = (const cplx *)a;
+  double nrm = 0;
+  for (int i = 0; i < 3; i++) {
+    double s = 0;
+    for (int j = 0; j < 3; j++) s += cabs(A[3 * i + j]);
+    if (s > nrm) nrm = s;
+  }
+  int sq = 0;
+  while (nrm > 0.5) { nrm *= 0.5; sq++; }
+  double sc = ldexp(1.0, -sq);
+  cplx B[9], T[9], W[9];
+  for (int i = 0; i < 9; i++) B[i] = A[i] * sc;
+  for (int i = 0; i < 9; i++) T[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  for (int n = 16; n >= 1; n--) {
+    qo_mat_mul((double *)B, (double *)T, (double *)W);
+    for (int i = 0; i < 9; i++) T[i] = W[i] / (double)n + ((i % 4 == 0) ? 1.0 : 0.0);
+  }
+  for (int s = 0; s < sq; s++) {
+    qo_mat_mul((double *)T, (double *)T, (double *)W);
+    memcpy(T, W, sizeof W);
+  }
+  memcpy(u, T, sizeof T);
+}
+
+/* U = exp(i*s*H) */
+void qo_su3_exp_i(const double *h, double s, double *u) {
+  const cplx *H = (const cplx *)h;
+  cplx A[9];
+  for (int i = 0; i < 9; i++) A[i] = I * s * H[i];
+  qo_su3_exp((double *)A, u);
+}
+
+/* maths/matrixFunctions.nim:414-419: r = (x - x^+)/2 - tr/3 */
+void qo_project_tah(const double *x, double *r) {
+  const cplx *X = (const cplx *)x;
+  cplx R[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) R[3 * i + j] = 0.5 * (X[3 * i + j] - conj(X[3 * j + i]));
+  cplx tr = (R[0] + R[4] + R[8]) / 3.0;
+  R[0] -= tr; R[4] -= tr; R[8] -= tr;
+  memcpy(r, R, sizeof R);
+}
+
+/* ------------------------------------------------------- synthetic inputs */
+enum { TAG_GAUGE = 0, TAG_SOURCE = 1, TAG_FATG = 2 };
+
+void qo_gauge_random(const int L[4], uint64_t seed, double *U) {
+  long V = qo_volume(L);
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < V; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    uint32_t lex = (uint32_t)(c[0] + (long)L[0] * (c[1] + (long)L[1] * (c[2] + (long)L[2] * c[3])));
+    for (int mu = 0; mu < 4; mu++) {
+      double g[10];
+      for (int j = 0; j < 5; j++) qo_gauss2(seed, lex, (uint32_t)mu, (uint32_t)j, TAG_GAUGE, g + 2 * j);
+      cplx H[9];
+      double tr = (g[0] + g[1] + g[2]) / 3.0;
+      H[0] = g[0] - tr; H[4] = g[1] - tr; H[8] = g[2] - tr;
+      H[1] = g[3] + I * g[4]; H[3] = conj(H[1]);
+      H[2] = g[5] + I * g[6]; H[6] = conj(H[2]);
+      H[5] = g[7] + I * g[8]; H[7] = conj(H[5]);
+      qo_su3_exp_i((double *)H, 0.5, U + ((long)mu * V + idx) * 18);
+    }
+  }
+}
+
+void qo_gauge_unit(const int L[4], double *U) {
+  long V = qo_volume(L);
+  memset(U, 0, sizeof(double) * 4 * V * 18);
+  for (long i = 0; i < 4 * V; i++) U[i * 18 + 0] = U[i * 18 + 8] = U[i * 18 + 16] = 1.0;
+}
+
+void qo_source_gauss(const int L[4], uint64_t seed, int stream, int par, double *x) {
+  long lo, hi;
+  par_range(L, par, &lo, &hi);
+  const double s = sqrt(0.5);
+#pragma omp parallel for schedule(static)
+  for (long idx = lo; idx < hi; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    uint32_t lex = (uint32_t)(c[0] + (long)L[0] * (c[1] + (long)L[1] * (c[2] + (long)L[2] * c[3])));
+    for (int j = 0; j < 3; j++) {
+      double g[2];
+      qo_gauss2(seed, lex, 64u + (uint32_t)stream, (uint32_t)j, TAG_SOURCE, g);
+      x[idx * 6 + 2 * j] = s * g[0];
+      x[idx * 6 + 2 * j + 1] = s * g[1];
+    }
+  }
+}
+
+/* gauge/gaugeUtils.nim:133-140 */
+void qo_set_bc(const int L[4], double *U) {
+  long V = qo_volume(L);
+  for (long idx = 0; idx < V; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    if (c[3] == L[3] - 1)
+      for (int e = 0; e < 18; e++) U[(3 * V + idx) * 18 + e] *= -1;
+  }
+}
+
+/* physics/stagD.nim:560-571, phases [8,9,11,0] */
+void qo_stag_phase(const int L[4], double *U) {
+  static const int ph[4] = {8, 9, 11, 0};
+  long V = qo_volume(L);
+  for (int mu = 0; mu < 4; mu++)
+    for (long idx = 0; idx < V; idx++) {
+      int c[4];
+      qo_coords(L, idx, c);
+      int s = 0;
+      for (int k = 0; k < 4; k++) s += (ph[mu] >> k) & c[k];
+      if (s & 1)
+        for (int e = 0; e < 18; e++) U[((long)mu * V + idx) * 18 + e] *= -1;
+    }
+}
+
+/* W = 0.8 U + 0.05 G ; Lg = -(1/24) U_mu(x) U_mu(x+mu) U_mu(x+2mu), from the
+   phased BC-applied U (gauge/fat7l.nim:145-154 long-link product, Naik
+   coefficient -1/24 from physics/hisqLinks.nim:17-28); interleaving
+   g[2mu]=fat, g[2mu+1]=long as newStag3 (physics/stagD.nim:603-615). */
+void qo_hisq_links(const int L[4], uint64_t seed, const double *Uph, double *g8) {
+  long V = qo_volume(L);
+  const double s = sqrt(0.5);
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < V; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    uint32_t lex = (uint32_t)(c[0] + (long)L[0] * (c[1] + (long)L[1] * (c[2] + (long)L[2] * c[3])));
+    for (int mu = 0; mu < 4; mu++) {
+      const double *u0 = Uph + ((long)mu * V + idx) * 18;
+      double *w = g8 + ((long)(2 * mu) * V + idx) * 18;
+      double *lg = g8 + ((long)(2 * mu + 1) * V + idx) * 18;
+      for (int j = 0; j < 9; j++) {
+        double g[2];
+        qo_gauss2(seed, lex, 16u + (uint32_t)mu, (uint32_t)j, TAG_FATG, g);
+        w[2 * j] = 0.8 * u0[2 * j] + 0.05 * s * g[0];
+        w[2 * j + 1] = 0.8 * u0[2 * j + 1] + 0.05 * s * g[1];
+      }
+      long i1 = nbr(L, c, mu, 1), i2 = nbr(L, c, mu, 2);
+      double t[18], t2[18];
+      qo_mat_mul(u0, Uph + ((long)mu * V + i1) * 18, t);
+      qo_mat_mul(t, Uph + ((long)mu * V + i2) * 18, t2);
+      for (int e = 0; e < 18; e++) lg[e] = (-1.0 / 24.0) * t2[e];
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- operator */
+static inline void mv_add(cplx *acc, const cplx *M, const cplx *v, double sgn) {
+  for (int i = 0; i < 3; i++) acc[i] += sgn * (M[3 * i] * v[0] + M[3 * i + 1] * v[1] + M[3 * i + 2] * v[2]);
+}
+static inline void amv_add(cplx *acc, const cplx *M, const cplx *v, double sgn) {
+  for (int i = 0; i < 3; i++)
+    acc[i] += sgn * (conj(M[i]) * v[0] + conj(M[3 + i]) * v[1] + conj(M[6 + i]) * v[2]);
+}
+static inline int hop_dir(int nl, int k) { return nl == 4 ? k : k >> 1; }
+static inline int hop_len(int nl, int k) { return (nl == 8 && (k & 1)) ? 3 : 1; }
+
+/* physics/stagD.nim:383-448 (stagD2): fwd and bwd interleaved per link */
+void qo_stagD2(const int L[4], int nl, const double *g, int par,
+               double *r, const double *x, double a, double b) {
+  long V = qo_volume(L), lo, hi;
+  par_range(L, par, &lo, &hi);
+  const cplx *G = (const cplx *)g, *X = (const cplx *)x;
+  cplx *R = (cplx *)r;
+#pragma omp parallel for schedule(static)
+  for (long idx = lo; idx < hi; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    cplx acc[3];
+    for (int i = 0; i < 3; i++)
+      acc[i] = (a == 0) ? b * X[3 * idx + i] : a * R[3 * idx + i] + b * X[3 * idx + i];
+    for (int k = 0; k < nl; k++) {
+      int mu = hop_dir(nl, k), len = hop_len(nl, k);
+      long jf = nbr(L, c, mu, len), jb = nbr(L, c, mu, -len);
+      mv_add(acc, G + ((long)k * V + idx) * 9, X + 3 * jf, 1.0);
+      amv_add(acc, G + ((long)k * V + jb) * 9, X + 3 * jb, -1.0);
+    }
+    for (int i = 0; i < 3; i++) R[3 * idx + i] = acc[i];
+  }
+}
+
+/* physics/stagD.nim:459-462 */
+void qo_stagD(const int L[4], int nl, const double *g, int par,
+              double *r, const double *x, double m, double sc, double a) {
+  qo_stagD2(L, nl, g, par, r, x, a / (0.5 * sc), m / (0.5 * sc));
+  long lo, hi;
+  par_range(L, par, &lo, &hi);
+  for (long i = lo * 6; i < hi * 6; i++) r[i] *= 0.5 * sc;
+}
+
+/* physics/stagD.nim:617-622 */
+void qo_D(const int L[4], int nl, const double *g, double *r, const double *x, double m) {
+  qo_stagD(L, nl, g, 0, r, x, m, 1.0, 0.0);
+  qo_stagD(L, nl, g, 1, r, x, m, 1.0, 0.0);
+}
+void qo_Ddag(const int L[4], int nl, const double *g, double *r, const double *x, double m) {
+  qo_stagD(L, nl, g, 0, r, x, m, -1.0, 0.0);
+  qo_stagD(L, nl, g, 1, r, x, m, -1.0, 0.0);
+}
+
+/* physics/stagD.nim:485-512 (stagD2xx): t = stagDP (rir:=0, all forward then all
+   backward, :234-271); r = stagDM with rir := 4*m2*x (fwd subtract, bwd add per
+   link, :312-347) */
+void qo_stagD2xx(const int L[4], int nl, const double *g, int par,
+                 double *r, const double *x, double m2, double *t) {
+  long V = qo_volume(L), lo, hi;
+  int q = 1 - par;
+  const cplx *G = (const cplx *)g, *X = (const cplx *)x;
+  cplx *R = (cplx *)r, *T = (cplx *)t;
+  par_range(L, q, &lo, &hi);
+#pragma omp parallel for schedule(static)
+  for (long idx = lo; idx < hi; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    cplx acc[3] = {0, 0, 0};
+    for (int k = 0; k < nl; k++) {
+      long jf = nbr(L, c, hop_dir(nl, k), hop_len(nl, k));
+      mv_add(acc, G + ((long)k * V + idx) * 9, X + 3 * jf, 1.0);
+    }
+    for (int k = 0; k < nl; k++) {
+      long jb = nbr(L, c, hop_dir(nl, k), -hop_len(nl, k));
+      amv_add(acc, G + ((long)k * V + jb) * 9, X + 3 * jb, -1.0);
+    }
+    for (int i = 0; i < 3; i++) T[3 * idx + i] = acc[i];
+  }
+  par_range(L, par, &lo, &hi);
+#pragma omp parallel for schedule(static)
+  for (long idx = lo; idx < hi; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    cplx acc[3];
+    for (int i = 0; i < 3; i++) acc[i] = (4.0 * m2) * X[3 * idx + i];
+    for (int k = 0; k < nl; k++) {
+      int mu = hop_dir(nl, k), len = hop_len(nl, k);
+      long jf = nbr(L, c, mu, len), jb = nbr(L, c, mu, -len);
+      mv_add(acc, G + ((long)k * V + idx) * 9, T + 3 * jf, -1.0);
+      amv_add(acc, G + ((long)k * V + jb) * 9, T + 3 * jb, 1.0);
+    }
+    for (int i = 0; i < 3; i++) R[3 * idx + i] = acc[i];
+  }
+}
+
+/* ------------------------------------------------------------ reductions */
+double qo_norm2(long n, const double *x) {
+  double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (long i = 0; i < 6 * n; i++) s += x[i] * x[i];
+  return s;
+}
+double qo_redot(long n, const double *x, const double *y) {
+  double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (long i = 0; i < 6 * n; i++) s += x[i] * y[i];
+  return s;
+}
+
+/* ---------------------------------------------------------------- solvers */
+/* solvers/cg.nim:55-272 with precon = cpNone (z aliases r, q aliases p),
+   starting from a fresh state (reset: b2=-1, r2old=rzold=1).  The operator
+   is stagD2xx on parity par (physics/stagSolve.nim:64-72). */
+int qo_cg(const int L[4], int nl, const double *g, int par,
+          double *x, const double *b, double m, double r2req, int maxits,
+          double *r2out) {
+  long V = qo_volume(L), lo, hi;
+  par_range(L, par, &lo, &hi);
+  long n = hi - lo;
+  double *r = calloc(6 * V, sizeof(double));
+  double *p = calloc(6 * V, sizeof(double));
+  double *Ap = calloc(6 * V, sizeof(double));
+  double *t = calloc(6 * V, sizeof(double));
+  const double m2 = m * m;
+  double *xs = x + 6 * lo, *rs = r + 6 * lo, *ps = p + 6 * lo, *Aps = Ap + 6 * lo;
+  const double *bs = b + 6 * lo;
+  double b2 = qo_norm2(n, bs), r2;
+  if (b2 == 0.0) {
+    memset(xs, 0, sizeof(double) * 6 * n);
+    r2 = 0.0;
+  } else {
+    qo_stagD2xx(L, nl, g, par, Ap, x, m2, t);
+    for (long i = 0; i < 6 * n; i++) rs[i] = bs[i] - Aps[i];
+    r2 = qo_norm2(n, rs);
+  }
+  double r2stop = r2req * b2;
+  int itn = 0;
+  double rzo = 1.0;
+  while (itn < maxits && r2 > r2stop) {
+    double rz = r2;
+    double beta = rz / rzo;
+    rzo = rz;
+    if (itn == 0)
+      memcpy(ps, rs, sizeof(double) * 6 * n);
+    else
+      for (long i = 0; i < 6 * n; i++) ps[i] = rs[i] + beta * ps[i];
+    itn++;
+    qo_stagD2xx(L, nl, g, par, Ap, p, m2, t);
+    double pAp = qo_redot(n, ps, Aps);
+    double alpha = rz / pAp;
+    for (long i = 0; i < 6 * n; i++) xs[i] += alpha * ps[i];
+    for (long i = 0; i < 6 * n; i++) rs[i] -= alpha * Aps[i];
+    r2 = qo_norm2(n, rs);
+  }
+  if (r2out) *r2out = r2;
+  free(r); free(p); free(Ap); free(t);
+  return itn;
+}
+
+/* physics/stagSolve.nim:330-421 (solveXX) with sloppySolve = SloppyNone:
+   outer defect-correction loop around solveXXInternal -> cg.  Returns the
+   total iterations; *true_r2_rel = |x - A r|^2/|x|^2 on parity par. */
+int qo_solveXX(const int L[4], int nl, const double *g, int par,
+               double *r, const double *x, double m, double r2req, int maxits,
+               double *true_r2_rel) {
+  long V = qo_volume(L), lo, hi;
+  par_range(L, par, &lo, &hi);
+  long n = hi - lo;
+  double *rho = calloc(6 * V, sizeof(double));
+  double *y = calloc(6 * V, sizeof(double));
+  double *t = calloc(6 * V, sizeof(double));
+  memset(r, 0, sizeof(double) * 6 * V);
+  memcpy(rho + 6 * lo, x + 6 * lo, sizeof(double) * 6 * n);
+  double b2 = qo_norm2(n, rho + 6 * lo);
+  double r2 = b2, r2stop = r2req * b2, r2prev = INFINITY;
+  int its = 0;
+  while (r2 > r2stop && r2 < r2prev) {
+    r2prev = r2;
+    int mx = maxits - its;
+    if (mx <= 0) break;
+    memset(y, 0, sizeof(double) * 6 * V);
+    double r2i;
+    its += qo_cg(L, nl, g, par, y, rho, m, r2stop / r2, mx, &r2i);
+    for (long i = 6 * lo; i < 6 * hi; i++) r[i] += y[i];
+    qo_stagD2xx(L, nl, g, par, rho, r, m * m, t);
+    for (long i = 6 * lo; i < 6 * hi; i++) rho[i] = x[i] - rho[i];
+    r2 = qo_norm2(n, rho + 6 * lo);
+  }
+  if (true_r2_rel) *true_r2_rel = (b2 > 0) ? r2 / b2 : 0.0;
+  free(rho); free(y); free(t);
+  return its;
+}
+
+/* solvers/cgm.nim:68-246 (fresh solve, b2 < 0 branch), shifted operators
+   A + sigma_k with A = stagD2xx(m^2) and sigma_0 forced to 0
+   (physics/stagSolve.nim:196-206). */
+int qo_cgm(const int L[4], int nl, const double *g, int par,
+           double *xs, const double *b, double m, const double *sigmas, int nshift,
+           double r2req, int maxits, double *r2out) {
+  long V = qo_volume(L), lo, hi;
+  par_range(L, par, &lo, &hi);
+  long n = hi - lo, o = 6 * lo;
+  double *r = calloc(6 * V, sizeof(double));
+  double *Ap = calloc(6 * V, sizeof(double));
+  double *t = calloc(6 * V, sizeof(double));
+  double *ps = calloc(6 * V * (size_t)nshift, sizeof(double));
+  double *sg = calloc(nshift, sizeof(double)), *zi = calloc(nshift, sizeof(double)),
+         *zim1 = calloc(nshift, sizeof(double));
+  for (int k = 0; k < nshift; k++) sg[k] = (k == 0) ? 0.0 : sigmas[k];
+  memcpy(r + o, b + o, sizeof(double) * 6 * n);
+  for (int k = 0; k < nshift; k++) memset(xs + 6 * V * k, 0, sizeof(double) * 6 * V);
+  double b2 = qo_norm2(n, b + o), r2 = b2, r2stop = r2req * b2;
+  int itn = 0;
+  if (r2 > r2stop) {
+    double r2i = r2, r2ip1 = 0, r2r = 0, alpha = 0, beta = 0, alphaim1 = -1.0, betaim1 = 0.0;
+    for (int k = 0; k < nshift; k++) {
+      zim1[k] = 1.0; zi[k] = 1.0;
+      memcpy(ps + 6 * V * k + o, r + o, sizeof(double) * 6 * n);
+    }
+    int continuing = 1;
+    while (continuing) {
+      for (int k = 0; k < nshift; k++) {
+        double *pk = ps + 6 * V * k, *xk = xs + 6 * V * k;
+        if (k == 0) {
+          qo_stagD2xx(L, nl, g, par, Ap, pk, m * m, t);
+          itn++;
+          double pAp = qo_redot(n, pk + o, Ap + o);
+          alpha = (pAp != 0.0) ? r2i / pAp : 0.0;
+          for (long i = o; i < o + 6 * n; i++) r[i] -= alpha * Ap[i];
+          for (long i = o; i < o + 6 * n; i++) xk[i] += alpha * pk[i];
+          r2ip1 = qo_norm2(n, r + o);
+          r2r = r2ip1;
+          beta = (r2i != 0.0) ? r2ip1 / r2i : 0.0;
+          continuing = (itn < maxits) && (r2r > r2stop);
+          if (continuing)
+            for (long i = o; i < o + 6 * n; i++) pk[i] = r[i] + beta * pk[i];
+        } else {
+          double zip1d = alpha * betaim1 * (zim1[k] - zi[k]);
+          zip1d += zim1[k] * alphaim1 * (1.0 + sg[k] * alpha);
+          double zip1 = (zip1d != 0.0) ? zi[k] * zim1[k] * alphaim1 / zip1d : 0.0;
+          double zr = (zi[k] != 0.0) ? zip1 / zi[k] : 0.0;
+          for (long i = o; i < o + 6 * n; i++) xk[i] += alpha * zr * pk[i];
+          if (continuing) {
+            for (long i = o; i < o + 6 * n; i++) pk[i] = zip1 * r[i] + beta * zr * zr * pk[i];
+            zim1[k] = zi[k];
+            zi[k] = zip1;
+          }
+        }
+      }
+      alphaim1 = alpha; betaim1 = beta; r2i = r2ip1;
+      r2 = r2r;
+    }
+  }
+  if (r2out) *r2out = r2;
+  free(r); free(Ap); free(t); free(ps); free(sg); free(zi); free(zim1);
+  return itn;
+}
+
+/* ------------------------------------------------------------------ force */
+/* hmc/hmcAction.nim:735-771: f1[mu](n) = psi(n) psi(n+mu)^+, times -1 on odd n */
+void qo_ferm_outer(const int L[4], const double *psi, double *G) {
+  long V = qo_volume(L), Vh = V / 2;
+  const cplx *P = (const cplx *)psi;
+  cplx *F = (cplx *)G;
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < V; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    double eps = (idx < Vh) ? 1.0 : -1.0;
+    for (int mu = 0; mu < 4; mu++) {
+      long j = nbr(L, c, mu, 1);
+      cplx *f = F + ((long)mu * V + idx) * 9;
+      for (int a = 0; a < 3; a++)
+        for (int bb = 0; bb < 3; bb++) f[3 * a + bb] = eps * P[3 * idx + a] * conj(P[3 * j + bb]);
+    }
+  }
+}
+
+/* hmc/hmcAction.nim:779-790: f = projectTAH(ff * U^+) with phased U */
+void qo_ferm_force(const int L[4], const double *U, const double *psi, double *F) {
+  long V = qo_volume(L);
+  qo_ferm_outer(L, psi, F);
+  const cplx *Uc = (const cplx *)U;
+  cplx *Fc = (cplx *)F;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < 4 * V; i++) {
+    cplx t[9];
+    const cplx *u = Uc + i * 9;
+    cplx *f = Fc + i * 9;
+    for (int a = 0; a < 3; a++)
+      for (int bb = 0; bb < 3; bb++) {
+        cplx s = 0;
+        for (int k = 0; k < 3; k++) s += f[3 * a + k] * conj(u[3 * bb + k]);
+        t[3 * a + bb] = s;
+      }
+    qo_project_tah((double *)t, (double *)f);
+  }
+}
+
+/* hmc/hmcAction.nim:669-721 (stagForceSolve) */
+int qo_force_solve(const int L[4], int nl, const double *g, const double *phi,
+                   double m, double r2req, int maxits, double *psi) {
+  long V = qo_volume(L), Vh = V / 2;
+  double *vphi = calloc(6 * V, sizeof(double));
+  double tr2;
+  int its = qo_solveXX(L, nl, g, 0, vphi, phi, m, r2req, maxits, &tr2);
+  for (long i = 0; i < 6 * Vh; i++) vphi[i] *= 4.0;
+  memset(psi, 0, sizeof(double) * 6 * V);
+  qo_stagD2(L, nl, g, 1, psi, vphi, 0.0, 0.0);
+  memcpy(psi, vphi, sizeof(double) * 6 * Vh);
+  free(vphi);
+  return its;
+}

@@ -1,0 +1,107 @@
+/*
+ * qexoracle.h -- CPU restatement of QEX's even-odd staggered Dslash, CG,
+ * multi-shift CG and staggered fermion force, in plain C (C99 + OpenMP).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * implementation in qex_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product path never does.
+ *
+ * Conventions (all restated from the reference, file:line into QEX's src/):
+ *  - Lattice L[4] = (Lx,Ly,Lz,Lt), all even.  lex = x + Lx*(y + Ly*(z + Lz*t)).
+ *    Canonical site index idx = (lex + p*V)/2 with p = (x+y+z+t)&1: even sites
+ *    first, then odd (layout/qlayout.nim:108-136 with innerGeom=1).
+ *  - Color vector: 3 complex = 6 doubles (re,im) per site.
+ *  - Link field k: V x 3x3 complex row-major = 18 doubles per site.
+ *    nlinks = 4 (plain, g[mu]) or 8 (HISQ: g[2mu]=fat hop 1, g[2mu+1]=long
+ *    hop 3; physics/stagD.nim:40-51,603-615).
+ *  - Links are passed PHASED and BC-applied, as QEX's Staggered object holds
+ *    them (physics/stagD.nim:560-571, gauge/gaugeUtils.nim:133-140).
+ *  - Parity argument: 0 even, 1 odd, 2 all.
+ */
+#ifndef QEXORACLE_H
+#define QEXORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- layout ---- */
+long qo_volume(const int L[4]);
+long qo_index(const int L[4], const int x[4]);
+void qo_coords(const int L[4], long idx, int x[4]);
+
+/* ---- counter-based generator (synthetic inputs) ---- */
+void qo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* two N(0,1) deviates from counter (c0,c1,c2,c3) and 64-bit seed */
+void qo_gauss2(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+               double g[2]);
+/* U_mu(x) = exp(i*0.5*H), H traceless Hermitian, unit Gaussian entries */
+void qo_gauge_random(const int L[4], uint64_t seed, double *U /*4*V*18*/);
+/* identity links */
+void qo_gauge_unit(const int L[4], double *U);
+/* complex Gaussian color vectors, E|z|^2=1 per component, on parity par */
+void qo_source_gauss(const int L[4], uint64_t seed, int stream, int par, double *x /*V*6*/);
+/* antiperiodic t boundary: flip g[3] on t=Lt-1 (gauge/gaugeUtils.nim:133-140) */
+void qo_set_bc(const int L[4], double *U);
+/* staggered phases, masks [8,9,11,0] (physics/stagD.nim:560-571) */
+void qo_stag_phase(const int L[4], double *U);
+/* HISQ-form links from phased U: W = 0.8U + 0.05G, Lg = -(1/24) U U U,
+   interleaved g8 = [W0,L0,W1,L1,W2,L2,W3,L3] (stagD.nim:603-615, fat7l.nim:145-154) */
+void qo_hisq_links(const int L[4], uint64_t seed, const double *Uph, double *g8);
+
+/* ---- operator ---- */
+/* r = a*r + b*x + sum_k [g_k(x) x(x+d_k) - g_k(x-d_k)^+ x(x-d_k)] on parity par
+   (physics/stagD.nim:383-448) */
+void qo_stagD2(const int L[4], int nl, const double *g, int par,
+               double *r, const double *x, double a, double b);
+/* r = a*r + m*x + sc*D*x with D = (1/2)(2D)  (stagD.nim:459-462) */
+void qo_stagD(const int L[4], int nl, const double *g, int par,
+              double *r, const double *x, double m, double sc, double a);
+/* r_p = 4*m2*x_p - (2D)_{p,q} (2D)_{q,p} x_p  (stagD.nim:485-520) */
+void qo_stagD2xx(const int L[4], int nl, const double *g, int par,
+                 double *r, const double *x, double m2, double *tscratch);
+/* D(m) = m + D and D^+(m) = m - D on all sites (stagD.nim:617-622) */
+void qo_D(const int L[4], int nl, const double *g, double *r, const double *x, double m);
+void qo_Ddag(const int L[4], int nl, const double *g, double *r, const double *x, double m);
+
+/* ---- reductions (FP64, fieldET.nim:648-778) ---- */
+double qo_norm2(long nsites, const double *x);
+double qo_redot(long nsites, const double *x, const double *y);
+
+/* ---- solvers ---- */
+/* CG (cpNone) on A = 4(m^2 - D_pq D_qp) (solvers/cg.nim:55-272), x starts at
+   the given value; returns iterations; *r2out = recursive r2. */
+int qo_cg(const int L[4], int nl, const double *g, int par,
+          double *x, const double *b, double m, double r2req, int maxits,
+          double *r2out);
+/* solveXX outer defect-correction loop (physics/stagSolve.nim:330-421) */
+int qo_solveXX(const int L[4], int nl, const double *g, int par,
+               double *r, const double *x, double m, double r2req, int maxits,
+               double *true_r2_rel);
+/* multi-shift CG (solvers/cgm.nim:68-246): solves (A + sigma_k) x_k = b,
+   sigma_0 = 0 for the base system; xs = nshift fields of V*6 doubles */
+int qo_cgm(const int L[4], int nl, const double *g, int par,
+           double *xs, const double *b, double m, const double *sigmas, int nshift,
+           double r2req, int maxits, double *r2out);
+
+/* ---- force ---- */
+/* F_mu(x) = TAH(eps(x) psi(x) psi(x+mu)^+ U_mu(x)^+) with phased U
+   (hmc/hmcAction.nim:723-792, unsmeared); F: 4*V*18 doubles */
+void qo_ferm_force(const int L[4], const double *U, const double *psi, double *F);
+/* outer product only: G_mu(x) = eps(x) psi(x) psi(x+mu)^+  */
+void qo_ferm_outer(const int L[4], const double *psi, double *G);
+/* psi = [Y_e ; (2D)_oe Y_e] with Y_e = (m^2 - D_eo D_oe)^{-1} phi_e
+   (hmc/hmcAction.nim:669-721) */
+int qo_force_solve(const int L[4], int nl, const double *g, const double *phi,
+                   double m, double r2req, int maxits, double *psi);
+
+/* ---- 3x3 helpers used by tests ---- */
+void qo_su3_exp_i(const double *H /*18, Hermitian*/, double s, double *U /*18*/);
+void qo_su3_exp(const double *A /*18, general*/, double *U /*18*/);
+void qo_mat_mul(const double *a, const double *b, double *c);
+void qo_project_tah(const double *x, double *r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

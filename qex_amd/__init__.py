@@ -1,0 +1,8 @@
+"""qex_amd: MI355X-native staggered Dslash / CG / multi-shift CG / fermion force
+behind QEX's solver-backend seam.  The compute path is the HIP library
+qex_amd/libqexhip.so (C ABI in include/qexhip.h); this package is its host-side
+mirror of QEX's Nim API."""
+from ._lib import ALL, BF16, EVEN, FP16, FP32, FP64, ODD, QexHipError, load  # noqa: F401
+from .stag import (D, Ddag, Field, Layout, SloppyType, SolverBackend, SolverParams,  # noqa: F401
+                   Staggered, fermForce, newLayout, newSolverParams, newStag, newStag3, solveEE,
+                   solveMulti, solveOO, stagD, stagD2, stagD2ee, stagD2oo, stagForceSolve, time_hop)

@@ -1,0 +1,116 @@
+"""ctypes binding of the qexhip C ABI (include/qexhip.h).
+
+The product path: every call goes to the HIP library built in-tree
+(qex_amd/libqexhip.so).  There is no CPU fallback; loading fails loudly
+when the library is missing.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqexhip.so")
+
+FP64, FP32, FP16, BF16 = 0, 1, 2, 3
+EVEN, ODD, ALL = 0, 1, 2
+
+# every symbol include/qexhip.h declares
+EXPORTS = [
+    "qexhip_init", "qexhip_finalize", "qexhip_last_error", "qexhip_sync", "qexhip_stream",
+    "qexhip_lattice_create", "qexhip_lattice_destroy",
+    "qexhip_links_create", "qexhip_links_update", "qexhip_links_destroy",
+    "qexhip_field_create", "qexhip_field_destroy", "qexhip_field_import", "qexhip_field_export",
+    "qexhip_field_norm2", "qexhip_field_redot",
+    "qexhip_stagD2", "qexhip_stagD", "qexhip_stagD2xx",
+    "qexhip_solve_xx", "qexhip_solve_ee_host", "qexhip_solve_oo_host", "qexhip_solve_multishift",
+    "qexhip_force_solve", "qexhip_force",
+    "qexhip_gen_gauge", "qexhip_gen_source", "qexhip_gen_phase_bc", "qexhip_gen_hisq",
+    "qexhip_time_hop",
+]
+
+
+class SolverParamsC(ctypes.Structure):
+    _fields_ = [
+        ("r2req", ctypes.c_double),
+        ("maxits", ctypes.c_int),
+        ("sloppy", ctypes.c_int),
+        ("sloppy_links", ctypes.c_int),
+        ("reliable_delta", ctypes.c_double),
+        ("reliable_max_its", ctypes.c_int),
+        ("use_prev_soln", ctypes.c_int),
+        ("verbosity", ctypes.c_int),
+        ("iterations", ctypes.c_int),
+        ("reliable_updates", ctypes.c_int),
+        ("r2", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+        ("flops", ctypes.c_double),
+    ]
+
+
+class QexHipError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the library and declare the ABI (no GPU call is made here)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise QexHipError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    vp, ip, dp, I, D = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), \
+        ctypes.c_int, ctypes.c_double
+    vpp = ctypes.POINTER(ctypes.c_void_p)
+    spp = ctypes.POINTER(SolverParamsC)
+    U64 = ctypes.c_uint64
+    sig = {
+        "qexhip_init": [I], "qexhip_finalize": [], "qexhip_sync": [],
+        "qexhip_lattice_create": [ip, vpp], "qexhip_lattice_destroy": [vp],
+        "qexhip_links_create": [vp, I, vp, I, vpp], "qexhip_links_update": [vp, vp, I],
+        "qexhip_links_destroy": [vp],
+        "qexhip_field_create": [vp, I, vpp], "qexhip_field_destroy": [vp],
+        "qexhip_field_import": [vp, I, vp, I], "qexhip_field_export": [vp, I, vp, I],
+        "qexhip_field_norm2": [vp, I, dp], "qexhip_field_redot": [vp, vp, I, dp],
+        "qexhip_stagD2": [vp, I, vp, vp, D, D], "qexhip_stagD": [vp, I, vp, vp, D, D, D],
+        "qexhip_stagD2xx": [vp, I, vp, vp, D],
+        "qexhip_solve_xx": [vp, I, vp, vp, D, spp],
+        "qexhip_solve_ee_host": [vp, vp, vp, D, spp], "qexhip_solve_oo_host": [vp, vp, vp, D, spp],
+        "qexhip_solve_multishift": [vp, I, vpp, vp, D, dp, I, spp],
+        "qexhip_force_solve": [vp, vp, vp, D, spp], "qexhip_force": [vp, vp, vp, I, I],
+        "qexhip_gen_gauge": [vp, U64, vp], "qexhip_gen_source": [vp, U64, I, I, vp],
+        "qexhip_gen_phase_bc": [vp, vp], "qexhip_gen_hisq": [vp, U64, vp, vp],
+        "qexhip_time_hop": [vp, I, I, dp],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    lib.qexhip_last_error.restype = ctypes.c_char_p
+    lib.qexhip_last_error.argtypes = []
+    lib.qexhip_stream.restype = ctypes.c_void_p
+    lib.qexhip_stream.argtypes = []
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call an ABI function; raise QexHipError with the library's message on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.qexhip_last_error().decode(errors="replace")
+        raise QexHipError(f"{name}: {msg}")
+    return rc
+
+
+_initialized = False
+
+
+def init(device=0):
+    global _initialized
+    if not _initialized:
+        call("qexhip_init", device)
+        _initialized = True

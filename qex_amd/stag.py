@@ -1,0 +1,368 @@
+"""Host-side mirror of QEX's staggered-fermion API over the qexhip C ABI.
+
+Names, argument meaning and error behaviour follow QEX's Nim procedures so
+that callers and tests read like QEX code:
+
+  Layout            ~ newLayout (src/layout/layoutX.nim:71-200), one rank
+  Field             ~ Field[V, ColorVector] (src/field/fieldET.nim:17-22)
+  Staggered         ~ Staggered[G,T] with se/so (src/physics/stagD.nim:15-35,68-75)
+  newStag/newStag3  ~ src/physics/stagD.nim:573-615
+  stagD2/stagD/stagD2ee/stagD2oo/D/Ddag ~ src/physics/stagD.nim:383-637
+  SolverParams      ~ src/solvers/solverBase.nim:10-63
+  solveEE/solveOO   ~ src/physics/stagSolve.nim:330-427
+  solveMulti        ~ multi-shift solveXXInternal, src/physics/stagSolve.nim:164-229
+  stagForceSolve/fermForce ~ src/hmc/hmcAction.nim:669-792
+
+Data live on the GPU in the library's layout; `Field.set`/`Field.get` move
+canonical arrays (V x 3 complex128, QEX's single-rank site order, even sites
+first) in from numpy (host) or torch (device) arrays.
+"""
+import ctypes
+import enum
+
+import numpy as np
+
+from . import _lib
+from ._lib import ALL, BF16, EVEN, FP16, FP32, FP64, ODD, QexHipError, SolverParamsC, call
+
+try:  # torch is plumbing for device arrays; numpy arrays work without it
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+_PAR = {"even": EVEN, "odd": ODD, "all": ALL, EVEN: EVEN, ODD: ODD, ALL: ALL}
+
+
+def _par(p):
+    try:
+        return _PAR[p]
+    except KeyError:
+        raise QexHipError(f"unknown subset {p!r}")
+
+
+def _ptr(a):
+    """(pointer, on_device) of a contiguous complex128/float64 numpy or torch array."""
+    if torch is not None and isinstance(a, torch.Tensor):
+        if not a.is_contiguous():
+            raise QexHipError("tensor must be contiguous")
+        if a.dtype not in (torch.complex128, torch.float64):
+            raise QexHipError("tensor must be complex128 or float64")
+        return ctypes.c_void_p(a.data_ptr()), int(a.is_cuda)
+    a = np.asarray(a)
+    if not a.flags["C_CONTIGUOUS"] or a.dtype not in (np.complex128, np.float64):
+        raise QexHipError("array must be C-contiguous complex128 or float64")
+    return ctypes.c_void_p(a.ctypes.data), 0
+
+
+class Layout:
+    """One-rank lattice (QEX newLayout).  All extents must be even."""
+
+    def __init__(self, lat):
+        _lib.init()
+        self.physGeom = tuple(int(v) for v in lat)
+        if len(self.physGeom) != 4:
+            raise QexHipError("4-d lattices only")
+        self.nSites = int(np.prod(self.physGeom))
+        self.nEven = self.nSites // 2
+        h = ctypes.c_void_p()
+        call("qexhip_lattice_create", (ctypes.c_int * 4)(*self.physGeom), ctypes.byref(h))
+        self.h = h
+
+    def ColorVector(self, prec=FP64):
+        return Field(self, prec)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._lib is not None:
+            _lib._lib.qexhip_lattice_destroy(self.h)
+            self.h = None
+
+
+def newLayout(lat):
+    return Layout(lat)
+
+
+class Field:
+    """Color-vector field on the GPU (both parities)."""
+
+    def __init__(self, lo, prec=FP64):
+        self.l = lo
+        self.prec = prec
+        h = ctypes.c_void_p()
+        call("qexhip_field_create", lo.h, prec, ctypes.byref(h))
+        self.h = h
+
+    def set(self, arr, sub="all"):
+        p, dev = _ptr(arr)
+        call("qexhip_field_import", self.h, _par(sub), p, dev)
+        return self
+
+    def get(self, sub="all", out=None, device=None):
+        """Canonical (V, 3) complex128 copy; numpy unless device is a torch device."""
+        V = self.l.nSites
+        if out is None:
+            if device is not None:
+                out = torch.zeros((V, 3), dtype=torch.complex128, device=device)
+            else:
+                out = np.zeros((V, 3), dtype=np.complex128)
+        p, dev = _ptr(out)
+        call("qexhip_field_export", self.h, _par(sub), p, dev)
+        return out
+
+    def norm2(self, sub="all"):
+        v = ctypes.c_double()
+        call("qexhip_field_norm2", self.h, _par(sub), ctypes.byref(v))
+        return v.value
+
+    def redot(self, other, sub="all"):
+        v = ctypes.c_double()
+        call("qexhip_field_redot", self.h, other.h, _par(sub), ctypes.byref(v))
+        return v.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._lib is not None:
+            _lib._lib.qexhip_field_destroy(self.h)
+            self.h = None
+
+
+class StaggeredD:
+    """One checkerboard of a Staggered operator (QEX StaggeredD: sub/subset)."""
+
+    def __init__(self, owner, sub):
+        self.owner = owner
+        self.sub = sub
+        self.par = _par(sub)
+
+
+class Staggered:
+    """QEX Staggered: links g (phased, BC-applied) plus even/odd descriptors.
+
+    g: canonical link array (nl, V, 3, 3) complex128, numpy or torch (device);
+    nl = 4 plain, 8 = [fat0, long0, fat1, long1, ...] (newStag3 interleave).
+    """
+
+    def __init__(self, lo, g):
+        self.l = lo
+        nl = int(g.shape[0])
+        if nl not in (4, 8):
+            raise QexHipError("Staggered: need 4 (plain) or 8 (fat+long) link fields")
+        if tuple(g.shape[1:]) != (lo.nSites, 3, 3):
+            raise QexHipError("Staggered: link array shape must be (nl, V, 3, 3)")
+        p, dev = _ptr(g)
+        h = ctypes.c_void_p()
+        call("qexhip_links_create", lo.h, nl, p, dev, ctypes.byref(h))
+        self.h = h
+        self.nl = nl
+        self.g = self  # QEX passes s.g alongside s.se/s.so; the handle carries the links
+        self.se = StaggeredD(self, "even")
+        self.so = StaggeredD(self, "odd")
+        self.sa = StaggeredD(self, "all")
+
+    def update(self, g):
+        """New gauge generation (same shape)."""
+        p, dev = _ptr(g)
+        call("qexhip_links_update", self.h, p, dev)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._lib is not None:
+            _lib._lib.qexhip_links_destroy(self.h)
+            self.h = None
+
+
+def newStag(lo, g):
+    """newStag (stagD.nim:573-592): plain links g[0..3]."""
+    return Staggered(lo, g)
+
+
+def newStag3(lo, g, g3):
+    """newStag3 (stagD.nim:603-615): interleave fat g[mu] and long g3[mu]."""
+    if torch is not None and isinstance(g, torch.Tensor):
+        gg = torch.stack([t for mu in range(4) for t in (g[mu], g3[mu])]).contiguous()
+    else:
+        gg = np.ascontiguousarray(np.stack([t for mu in range(4) for t in (g[mu], g3[mu])]))
+    return Staggered(lo, gg)
+
+
+def _sd(sd):
+    if isinstance(sd, StaggeredD):
+        return sd.owner, sd.par
+    raise QexHipError("expected a StaggeredD (s.se, s.so or s.sa)")
+
+
+def stagD2(sd, r, g, x, a, b):
+    """r = a*r + b*x + (2D)x on sd's subset (stagD.nim:383-448)."""
+    s, par = _sd(sd)
+    call("qexhip_stagD2", s.h, par, r.h, x.h, float(a), float(b))
+
+
+def stagD(sd, r, g, x, m, sc=1.0, a=0.0):
+    """r = a*r + m*x + sc*D*x (stagD.nim:459-462)."""
+    s, par = _sd(sd)
+    call("qexhip_stagD", s.h, par, r.h, x.h, float(m), float(sc), float(a))
+
+
+def stagD2ee(se, so, r, g, x, m2):
+    """r_e = 4*m2*x_e - (2D)_eo (2D)_oe x_e (stagD.nim:514-516)."""
+    call("qexhip_stagD2xx", se.owner.h, EVEN, r.h, x.h, float(m2))
+
+
+def stagD2oo(se, so, r, g, x, m2):
+    """r_o = 4*m2*x_o - (2D)_oe (2D)_eo x_o (stagD.nim:518-520)."""
+    call("qexhip_stagD2xx", se.owner.h, ODD, r.h, x.h, float(m2))
+
+
+def D(s, r, x, m):
+    """D(m) = m + D on all sites (stagD.nim:617-619)."""
+    stagD(s.se, r, s.g, x, m)
+    stagD(s.so, r, s.g, x, m)
+
+
+def Ddag(s, r, x, m):
+    """D^+(m) = m - D on all sites (stagD.nim:620-622)."""
+    stagD(s.se, r, s.g, x, m, -1.0)
+    stagD(s.so, r, s.g, x, m, -1.0)
+
+
+class SolverBackend(enum.IntEnum):
+    """solverBase.nim:6-7 plus the HIP backend this library provides."""
+    sbQex = 0
+    sbQuda = 1
+    sbGrid = 2
+    sbHip = 3
+
+
+class SloppyType(enum.IntEnum):
+    """solverBase.nim:8-9 (SloppyBF16 added: BF16 link/spinor storage)."""
+    SloppyNone = 0
+    SloppySingle = 1
+    SloppyHalf = 2
+    SloppyBF16 = 3
+
+
+_SLOPPY_PREC = {SloppyType.SloppyNone: FP64, SloppyType.SloppySingle: FP32,
+                SloppyType.SloppyHalf: FP16, SloppyType.SloppyBF16: BF16}
+
+
+class SolverParams:
+    """QEX SolverParams (solverBase.nim:10-63) with the HIP backend's extras."""
+
+    def __init__(self):
+        # inputs (defaults of solverBase.nim:49-59)
+        self.r2req = 1e-6
+        self.maxits = 50000
+        self.backend = SolverBackend.sbHip
+        self.sloppySolve = SloppyType.SloppyNone
+        self.sloppyLinks = None  # None: same storage as the spinors
+        self.usePrevSoln = False
+        self.verbosity = 1
+        self.subsetName = "all"
+        self.reliableDelta = 0.1
+        self.reliableMaxIts = 100
+        self.resetStats()
+
+    def resetStats(self):
+        self.calls = 0
+        self.iterations = 0
+        self.iterationsMax = 0
+        self.seconds = 0.0
+        self.flops = 0.0
+        self.r2 = 0.0
+        self.reliableUpdates = 0
+
+    @property
+    def finalIterations(self):
+        return self.iterations
+
+    def _c(self):
+        if self.backend != SolverBackend.sbHip:
+            raise QexHipError(f"backend {self.backend.name} is not provided by this library")
+        c = SolverParamsC()
+        c.r2req = self.r2req
+        c.maxits = self.maxits
+        c.sloppy = _SLOPPY_PREC[SloppyType(self.sloppySolve)]
+        c.sloppy_links = -1 if self.sloppyLinks is None else _SLOPPY_PREC[SloppyType(self.sloppyLinks)]
+        c.reliable_delta = self.reliableDelta
+        c.reliable_max_its = self.reliableMaxIts
+        c.use_prev_soln = int(self.usePrevSoln)
+        c.verbosity = self.verbosity
+        return c
+
+    def _add(self, c):
+        # addStats (solverBase.nim:75-83)
+        self.calls += 1
+        self.iterations += c.iterations
+        self.iterationsMax = max(self.iterationsMax, c.iterations)
+        self.seconds += c.seconds
+        self.flops += c.flops
+        self.r2 = c.r2
+        self.reliableUpdates += c.reliable_updates
+
+    def getStats(self):
+        """'its secs Gf/s r2' (solverBase.nim:85-111, single-call form)."""
+        s = max(1e-12, self.seconds)
+        return f"{self.iterations}  {s:.4g}s  {1e-9 * self.flops / s:.4g}Gf/s  {self.r2:.4g}"
+
+
+def newSolverParams():
+    return SolverParams()
+
+
+def _solve_xx(s, r, x, m, sp, par):
+    c = sp._c()
+    call("qexhip_solve_xx", s.h, par, r.h, x.h, float(m), ctypes.byref(c))
+    sp._add(c)
+    if sp.verbosity > 0:
+        print(("solveEE: " if par == EVEN else "solveOO: ") + sp.getStats(), flush=True)
+    return c
+
+
+def solveEE(s, r, x, m, sp):
+    """Solve 4(m^2 - D_eo D_oe) r = x on even sites (stagSolve.nim:423-424)."""
+    return _solve_xx(s, r, x, m, sp, EVEN)
+
+
+def solveOO(s, r, x, m, sp):
+    """Solve 4(m^2 - D_oe D_eo) r = x on odd sites (stagSolve.nim:426-427)."""
+    return _solve_xx(s, r, x, m, sp, ODD)
+
+
+def solveMulti(s, xs, b, m, sigmas, sp, subset="even"):
+    """Multi-shift CG: (4(m^2 - DD) + sigma_k) x_k = b, sigma_0 taken as 0
+    (stagSolve.nim:164-229, cgm.nim:68-246)."""
+    if len(xs) != len(sigmas):
+        raise QexHipError("solveMulti: need one solution field per shift")
+    c = sp._c()
+    arr = (ctypes.c_void_p * len(xs))(*[f.h.value for f in xs])
+    sig = (ctypes.c_double * len(sigmas))(*[float(v) for v in sigmas])
+    call("qexhip_solve_multishift", s.h, _par(subset), arr, b.h, float(m), sig, len(sigmas), ctypes.byref(c))
+    sp._add(c)
+    return c
+
+
+def stagForceSolve(s, psi, phi, mass, sp):
+    """psi_e = (m^2 - D_eo D_oe)^-1 phi_e ; psi_o = (2D)_oe psi_e (hmcAction.nim:669-721)."""
+    c = sp._c()
+    call("qexhip_force_solve", s.h, psi.h, phi.h, float(mass), ctypes.byref(c))
+    sp._add(c)
+    return c
+
+
+def fermForce(s, psi, out=None, outer_only=False, device=None):
+    """F_mu(x) = TAH(eps(x) psi(x) psi(x+mu)^+ U_mu(x)^+) (hmcAction.nim:723-792, unsmeared).
+
+    Returns a canonical (4, V, 3, 3) complex128 array (torch on `device` if given)."""
+    V = s.l.nSites
+    if out is None:
+        if device is not None:
+            out = torch.zeros((4, V, 3, 3), dtype=torch.complex128, device=device)
+        else:
+            out = np.zeros((4, V, 3, 3), dtype=np.complex128)
+    p, dev = _ptr(out)
+    call("qexhip_force", s.h, psi.h, p, dev, int(outer_only))
+    return out
+
+
+def time_hop(s, prec=FP64, nrep=100):
+    """Average ms of one one-parity hop (HIP events on the library stream)."""
+    v = ctypes.c_double()
+    call("qexhip_time_hop", s.h, prec, nrep, ctypes.byref(v))
+    return v.value

@@ -1,0 +1,148 @@
+"""GPU parity of the staggered hop (stagD2/stagD/stagD2xx/D/Ddag) against the
+CPU oracle, through the C ABI.  FP64: 1e-12 relative;
+FP32 arithmetic: 1e-5; FP16/BF16 storage: 4e-3 / 3e-2."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20240601
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def make_links(O, L, hisq):
+    U = O.phased(L, O.gauge_random(L, SEED))
+    return O.hisq_links(L, SEED + 1, U) if hisq else U
+
+
+def sd_of(s, par):
+    return {0: s.se, 1: s.so, 2: s.sa}[par]
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (6, 4, 8, 4), (8, 8, 8, 8)])
+@pytest.mark.parametrize("hisq", [False, True])
+def test_generators_match_oracle(oracle, gpu, L, hisq):
+    from qex_amd import gauge
+    O = oracle
+    lo = gpu.Layout(L)
+    Ug = gauge.random_gauge(lo, SEED)
+    Uo = O.gauge_random(L, SEED)
+    assert np.abs(Ug.cpu().numpy() - Uo).max() < 1e-13
+    gauge.phase_bc(lo, Ug)
+    Uph = O.phased(L, Uo)
+    assert np.abs(Ug.cpu().numpy() - Uph).max() < 1e-13
+    if hisq:
+        g8g = gauge.hisq_links(lo, SEED + 1, Ug).cpu().numpy()
+        assert np.abs(g8g - O.hisq_links(L, SEED + 1, Uph)).max() < 1e-13
+    for par in (0, 1, 2):
+        xg = gauge.source(lo, 77, stream=3, par=par).cpu().numpy()
+        assert np.abs(xg - O.source_gauss(L, 77, 3, par)).max() < 1e-14
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (6, 4, 8, 4), (8, 8, 8, 8)])
+@pytest.mark.parametrize("hisq", [False, True])
+@pytest.mark.parametrize("par", [0, 1, 2])
+@pytest.mark.parametrize("ab", [(0.0, 0.0), (0.0, 0.37), (1.5, -0.25)])
+def test_stagD2_fp64(oracle, gpu, L, hisq, par, ab):
+    O = oracle
+    g = make_links(O, L, hisq)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    x = O.source_gauss(L, 11, 0, 2)
+    r0 = O.source_gauss(L, 12, 1, 2)
+    fx = lo.ColorVector().set(x)
+    fr = lo.ColorVector().set(r0)
+    gpu.stagD2(sd_of(s, par), fr, s.g, fx, *ab)
+    got = fr.get()
+    want = r0.copy()
+    O.stagD2(L, g, par, want, x, *ab)
+    assert rel(got, want) < 1e-12
+
+
+@pytest.mark.parametrize("hisq", [False, True])
+@pytest.mark.parametrize("par", [0, 1])
+def test_stagD2xx_fp64(oracle, gpu, hisq, par):
+    O = oracle
+    L = (8, 8, 8, 8)
+    g = make_links(O, L, hisq)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    x = O.source_gauss(L, 21, 0, par)
+    fx = lo.ColorVector().set(x)
+    fr = lo.ColorVector()
+    (gpu.stagD2ee if par == 0 else gpu.stagD2oo)(s.se, s.so, fr, s.g, fx, 0.01)
+    got = fr.get(sub=par)
+    want = O.stagD2xx(L, g, par, x, 0.01)
+    assert rel(got, want) < 1e-12
+
+
+@pytest.mark.parametrize("hisq", [False, True])
+def test_D_Ddag_stagD(oracle, gpu, hisq):
+    O = oracle
+    L = (4, 6, 4, 8)
+    g = make_links(O, L, hisq)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    x = O.source_gauss(L, 31, 0, 2)
+    fx = lo.ColorVector().set(x)
+    fr = lo.ColorVector()
+    gpu.D(s, fr, fx, 0.3)
+    assert rel(fr.get(), O.D(L, g, x, 0.3)) < 1e-12
+    gpu.Ddag(s, fr, fx, 0.3)
+    assert rel(fr.get(), O.Ddag(L, g, x, 0.3)) < 1e-12
+    r0 = O.source_gauss(L, 32, 0, 2)
+    fr.set(r0)
+    gpu.stagD(s.so, fr, s.g, fx, 0.2, -1.0, 0.5)
+    want = r0.copy()
+    O.stagD(L, g, 1, want, x, 0.2, -1.0, 0.5)
+    assert rel(fr.get(), want) < 1e-12
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-5), (2, 4e-3), (3, 3e-2)])
+@pytest.mark.parametrize("hisq", [False, True])
+def test_stagD2_reduced_precision(oracle, gpu, prec, tol, hisq):
+    O = oracle
+    L = (8, 8, 8, 8)
+    g = make_links(O, L, hisq)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    x = O.source_gauss(L, 41, 0, 2)
+    fx = lo.ColorVector(prec).set(x)
+    fr = lo.ColorVector(prec)
+    gpu.stagD2(s.se, fr, s.g, fx, 0.0, 0.0)
+    want = np.zeros_like(x)
+    O.stagD2(L, g, 0, want, x, 0.0, 0.0)
+    got = fr.get(sub=0)
+    assert rel(got[: lo.nEven], want[: lo.nEven]) < tol
+
+
+@pytest.mark.parametrize("L", [(32, 32, 32, 32), (16, 16, 16, 16)])
+def test_large_volume_properties(gpu, L):
+    """At full size: anti-Hermiticity Re<y,Hx> = -Re<Hy,x> and linearity, FP64."""
+    from qex_amd import gauge
+    lo = gpu.Layout(L)
+    U = gauge.random_gauge(lo, SEED)
+    gauge.phase_bc(lo, U)
+    s = gpu.Staggered(lo, U)
+    x = lo.ColorVector().set(gauge.source(lo, 5, 0, 2))
+    y = lo.ColorVector().set(gauge.source(lo, 6, 0, 2))
+    Hx, Hy = lo.ColorVector(), lo.ColorVector()
+    gpu.stagD2(s.sa, Hx, s.g, x, 0.0, 0.0)
+    gpu.stagD2(s.sa, Hy, s.g, y, 0.0, 0.0)
+    a = y.redot(Hx)
+    b = Hy.redot(x)
+    scale = np.sqrt(y.norm2() * Hx.norm2())
+    assert abs(a + b) / scale < 1e-13
+    # linearity: H(x + 2y) = Hx + 2Hy
+    xa = x.get(device="cuda")
+    ya = y.get(device="cuda")
+    z = lo.ColorVector().set((xa + 2 * ya).contiguous())
+    Hz = lo.ColorVector()
+    gpu.stagD2(s.sa, Hz, s.g, z, 0.0, 0.0)
+    lhs = Hz.get(device="cuda")
+    rhs = Hx.get(device="cuda") + 2 * Hy.get(device="cuda")
+    import torch
+    assert float(torch.linalg.norm(lhs - rhs) / torch.linalg.norm(rhs)) < 1e-14

@@ -1,0 +1,160 @@
+"""GPU solvers and force against the CPU oracle, through the C ABI.
+
+FP64 CG: iteration count equal to the oracle's (+-1), solution within 1e-9
+relative, FP64 true residual below the request.  Mixed precision: the FP64
+true residual meets r2req and the solution agrees with the FP64 solve to the
+solver tolerance."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 777
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def links(O, L, hisq):
+    U = O.phased(L, O.gauge_random(L, SEED))
+    return O.hisq_links(L, SEED + 1, U) if hisq else U
+
+
+@pytest.mark.parametrize("hisq,m", [(False, 0.1), (True, 0.05)])
+@pytest.mark.parametrize("par", [0, 1])
+def test_solve_fp64_matches_oracle(oracle, gpu, hisq, m, par):
+    O = oracle
+    L = (8, 8, 8, 8)
+    g = links(O, L, hisq)
+    b = O.source_gauss(L, 5, 0, par)
+    want, its_o, tr_o = O.solveXX(L, g, par, b, m, 1e-20)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    fx = lo.ColorVector()
+    sp = gpu.SolverParams()
+    sp.r2req = 1e-20
+    sp.verbosity = 0
+    (gpu.solveEE if par == 0 else gpu.solveOO)(s, fx, fb, m, sp)
+    got = fx.get(sub=par)
+    assert abs(sp.iterations - its_o) <= 1, (sp.iterations, its_o)
+    assert sp.r2 <= 1e-20
+    sl = slice(0, lo.nEven) if par == 0 else slice(lo.nEven, lo.nSites)
+    assert rel(got[sl], want[sl]) < 1e-9
+
+
+@pytest.mark.parametrize("sloppy", [1, 2, 3])
+@pytest.mark.parametrize("hisq", [False, True])
+def test_mixed_precision_reaches_fp64_residual(oracle, gpu, sloppy, hisq):
+    O = oracle
+    L = (8, 8, 8, 8)
+    m = 0.05
+    g = links(O, L, hisq)
+    b = O.source_gauss(L, 9, 0, 0)
+    want, _, _ = O.solveXX(L, g, 0, b, m, 1e-24)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    fx = lo.ColorVector()
+    sp = gpu.SolverParams()
+    sp.r2req = 1e-20
+    sp.verbosity = 0
+    sp.sloppySolve = sloppy
+    gpu.solveEE(s, fx, fb, m, sp)
+    assert sp.r2 <= 1e-20, sp.r2
+    assert sp.reliableUpdates >= 1
+    got = fx.get(sub=0)[: lo.nEven]
+    # error vs FP64 solve bounded by the condition number times the residual
+    assert rel(got, want[: lo.nEven]) < 1e-6
+    # independent check of the true residual with the oracle operator
+    r = b - O.stagD2xx(L, g, 0, fx.get(), m * m)
+    assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20
+
+
+def test_multishift_matches_oracle(oracle, gpu):
+    O = oracle
+    L = (8, 8, 8, 8)
+    m = 0.1
+    g = links(O, L, False)
+    b = O.source_gauss(L, 13, 0, 0)
+    sig = [0.0, 0.01, 0.1, 1.0]
+    xs_o, its_o, _ = O.cgm(L, g, 0, b, m, sig, 1e-20)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    fxs = [lo.ColorVector() for _ in sig]
+    sp = gpu.SolverParams()
+    sp.r2req = 1e-20
+    sp.verbosity = 0
+    gpu.solveMulti(s, fxs, fb, m, sig, sp)
+    assert abs(sp.iterations - its_o) <= 1
+    ne = lo.nEven
+    for k, f in enumerate(fxs):
+        got = f.get(sub=0)[:ne]
+        assert rel(got, xs_o[k][:ne]) < 1e-9
+        # each shifted system's own residual (oracle operator)
+        r = b - O.stagD2xx(L, g, 0, f.get(), m * m) - sig[k] * f.get()
+        assert np.sqrt(O.norm2(r[:ne]) / O.norm2(b)) < 1e-9
+
+
+def test_force_matches_oracle(oracle, gpu):
+    O = oracle
+    L = (8, 8, 8, 8)
+    m = 0.05
+    U = O.phased(L, O.gauge_random(L, SEED))
+    phi = O.source_gauss(L, 17, 0, 0)
+    psi_o, _ = O.force_solve(L, U, phi, m, 1e-24)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, U)
+    fphi = lo.ColorVector().set(phi)
+    fpsi = lo.ColorVector()
+    sp = gpu.SolverParams()
+    sp.r2req = 1e-24
+    sp.verbosity = 0
+    gpu.stagForceSolve(s, fpsi, fphi, m, sp)
+    assert rel(fpsi.get(), psi_o) < 1e-9
+    # force from identical psi: FP64 parity 1e-12
+    fpsi.set(psi_o)
+    F = gpu.fermForce(s, fpsi)
+    assert rel(F, O.ferm_force(L, U, psi_o)) < 1e-12
+    G = gpu.fermForce(s, fpsi, outer_only=True)
+    assert rel(G, O.ferm_outer(L, psi_o)) < 1e-13
+
+
+def test_host_entry_point_like_quda_adapter(oracle, gpu):
+    """qexhip_solve_ee_host: canonical host arrays in/out (qudaSolveEE shape)."""
+    from qex_amd import _lib
+    O = oracle
+    L = (4, 4, 4, 8)
+    g = links(O, L, False)
+    b = O.source_gauss(L, 23, 0, 0)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    out = np.zeros_like(b)
+    c = _lib.SolverParamsC()
+    c.r2req, c.maxits, c.sloppy, c.sloppy_links = 1e-22, 10000, 0, -1
+    _lib.call("qexhip_solve_ee_host", s.h, ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(b.ctypes.data),
+              0.2, ctypes.byref(c))
+    want, its, _ = O.solveXX(L, g, 0, b, 0.2, 1e-22)
+    assert rel(out[: lo.nEven], want[: lo.nEven]) < 1e-9
+    assert c.r2 <= 1e-22
+
+
+def test_errors_are_reported(gpu):
+    from qex_amd import QexHipError
+    lo = gpu.Layout((4, 4, 4, 4))
+    with pytest.raises(QexHipError):
+        gpu.Layout((3, 4, 4, 4))
+    f32 = lo.ColorVector(1)
+    f64 = lo.ColorVector(0)
+    U = np.zeros((4, 256, 3, 3), dtype=np.complex128)
+    s = gpu.Staggered(lo, U)
+    with pytest.raises(QexHipError):
+        gpu.stagD2(s.se, f32, s.g, f64, 0.0, 0.0)
+    sp = gpu.SolverParams()
+    sp.backend = gpu.SolverBackend.sbQuda
+    with pytest.raises(QexHipError):
+        gpu.solveEE(s, f64, f64, 0.1, sp)
