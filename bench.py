@@ -1,0 +1,236 @@
+"""Benchmark: plain staggered even-odd Dslash GB/s and CG time-to-1e-10 residual,
+mixed precision vs FP64, on a 32^4 lattice on one MI355X.
+
+A "step" is one one-parity FP64 hop r_e = (2D)_eo x_o over the whole 32^4
+lattice (the dominant kernel of every solve).  `value` is its achieved
+algorithmic bandwidth: 1248 B per output site (8 FP64 links + input and
+output color vectors) x 524288 sites / average hop time, measured with HIP
+events on the library stream.  Inputs are synthetic (U = exp(i*0.5*H),
+complex-Gaussian sources, seeded Philox) and resident in HBM before timing.
+
+The same JSON line carries the CG comparison (FP64 CG vs FP32-inner CG with
+FP64 reliable updates, and the FP16/BF16-storage variants) at mass 0.01 to
+relative residual 1e-10 (r2req = 1e-20), each with its FP64 true residual,
+the hop table at 16^4/32^4/48^4 in all storage precisions, the roofline of
+the dominant kernel and the CPU baseline (the C oracle on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_GBS = 8000.0  # MI355X HBM3E
+LAT = (32, 32, 32, 32)
+MASS = 0.01
+R2REQ = 1e-20
+SEED = 20240601
+
+# compulsory reals per output site of one plain one-parity hop: 8 links x 18 + 2 x 6
+HOP_REALS = 8 * 18 + 2 * 6
+BYTES_PER_REAL = {0: 8, 1: 4, 2: 2, 3: 2}
+PREC_NAME = {0: "fp64", 1: "fp32", 2: "fp16", 3: "bf16"}
+# flops per output site of one hop: 8 complex 3x3 mat-vec (66) + 7*6 adds = 570
+HOP_FLOPS = 570
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cg", action="store_true", help="skip the CG comparison")
+    ap.add_argument("--no-table", action="store_true", help="skip the hop table")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
+    return ap.parse_args()
+
+
+def hop_gbs(prec, ms, vh):
+    return HOP_REALS * BYTES_PER_REAL[prec] * vh / (ms * 1e-3) / 1e9
+
+
+def load_traffic():
+    """HBM bytes per hop launch measured by rocprofv3 --pmc (committed summary)."""
+    path = os.path.join(ROOT, "profiles", "hop_fp64_32x32x32x32_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("traffic_bytes_per_launch")
+
+
+def cpu_baseline(budget_s):
+    """The C oracle (stagD2 hop and FP64 CG iterations) on the host cores, on a
+    bounded sample of the same 32^4 workload."""
+    import numpy as np
+    from oracle import oracle as O
+    O.lib()
+    L = LAT
+    U = O.phased(L, O.gauge_random(L, SEED))
+    x = O.source_gauss(L, 5, 0, 2)
+    r = np.zeros_like(x)
+    vh = O.volume(L) // 2
+    O.stagD2(L, U, 0, r, x, 0.0, 0.0)  # warm
+    n, t0 = 0, time.perf_counter()
+    while True:
+        O.stagD2(L, U, 0, r, x, 0.0, 0.0)
+        n += 1
+        if time.perf_counter() - t0 > budget_s * 0.5:
+            break
+    hop_s = (time.perf_counter() - t0) / n
+    # CG iterations: a capped solve, per-iteration time
+    b = O.source_gauss(L, 7, 0, 0)
+    t0 = time.perf_counter()
+    _, its, _ = O.cg(L, U, 0, b, MASS, R2REQ, maxits=max(5, int(budget_s * 0.5 / (2.2 * hop_s))))
+    cg_it_s = (time.perf_counter() - t0) / its
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": round(HOP_REALS * 8 * vh / hop_s / 1e9, 3),
+        "unit": "GB/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"oracle stagD2 FP64 one-parity hop at 32^4, {n} hops ({hop_s*1e3:.1f} ms each); "
+                  f"FP64 CG at 32^4 m={MASS}: {its} iterations, {cg_it_s*1e3:.1f} ms/iteration",
+        "hop_ms": round(hop_s * 1e3, 3),
+        "hop_gflops": round(HOP_FLOPS * vh / hop_s / 1e9, 3),
+        "cg_ms_per_iteration": round(cg_it_s * 1e3, 3),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+    import torch
+    if args.copy_calib_mb:
+        n = args.copy_calib_mb << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "launches": 5}), flush=True)
+        return
+    import qex_amd as Q
+    from qex_amd import gauge
+    from qex_amd import _lib
+    _lib.init(0)
+
+    lo = Q.Layout(LAT)
+    vh = lo.nEven
+    U = gauge.random_gauge(lo, SEED)
+    gauge.phase_bc(lo, U)
+    s = Q.Staggered(lo, U)
+    del U
+    torch.cuda.synchronize()
+
+    # ---- timed region: K FP64 hops (events on the library stream) ----
+    Q.time_hop(s, 0, max(1, args.warmup))  # untimed warmup
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms = Q.time_hop(s, 0, args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = hop_gbs(0, ms, vh)
+
+    out = {
+        "metric": "Dslash GB/s + CG time-to-1e-10 residual, mixed vs FP64, 32^4 on 1 MI355X",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "wall_s_timed_region": round(wall, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Philox: U=exp(i*0.5*H) SU(3) links with staggered phases and "
+                "antiperiodic t; complex-Gaussian color vectors)",
+        "config": {"workload": "plain staggered even-odd one-parity Dslash hop, FP64, 32^4 "
+                               "(+ CG m=0.01 to |r|/|b|<=1e-10, FP64 vs mixed)",
+                   "lattice": list(LAT), "mass": MASS, "r2req": R2REQ,
+                   "bytes_per_site": HOP_REALS * 8, "sites_per_launch": vh},
+        "gflops": round(HOP_FLOPS * vh / (ms * 1e-3) / 1e9, 2),
+    }
+    out["roofline"] = {"bound": "hbm", "achieved": round(value, 2), "peak": PEAK_GBS, "unit": "GB/s",
+                       "frac": round(value / PEAK_GBS, 4), "traffic": load_traffic(),
+                       "kernel": "k_hop<P64,P64,P64,P64,double2,4,0,0>",
+                       "algorithmic_bytes_per_launch": HOP_REALS * 8 * vh,
+                       "avg_launch_ms": round(ms, 5)}
+
+    # ---- hop table (16^4, 32^4, 48^4 x storage precisions) ----
+    if not args.no_table:
+        table = {}
+        for L in [(16, 16, 16, 16), (32, 32, 32, 32), (48, 48, 48, 48)]:
+            lo2 = lo if L == LAT else Q.Layout(L)
+            s2 = s
+            if L != LAT:
+                U2 = gauge.random_gauge(lo2, SEED)
+                gauge.phase_bc(lo2, U2)
+                s2 = Q.Staggered(lo2, U2)
+                del U2
+            key = "x".join(map(str, L))
+            table[key] = {}
+            for prec in (0, 1, 2, 3):
+                Q.time_hop(s2, prec, 5)
+                n = 200 if L[0] <= 32 else 60
+                m_ = Q.time_hop(s2, prec, n)
+                table[key][PREC_NAME[prec]] = {"ms": round(m_, 5),
+                                               "GB_s": round(hop_gbs(prec, m_, lo2.nEven), 1),
+                                               "GFLOP_s": round(HOP_FLOPS * lo2.nEven / (m_ * 1e-3) / 1e9, 1)}
+            del s2
+            torch.cuda.empty_cache()
+        out["hop_table"] = table
+
+    # ---- CG: FP64 vs mixed (reliable updates every 100 its or at 0.1x drop) ----
+    if not args.no_cg:
+        b = lo.ColorVector().set(gauge.source(lo, 7, 0, 0))
+        x = lo.ColorVector()
+        cg = {}
+        variants = [("fp64", Q.SloppyType.SloppyNone), ("mixed_fp32", Q.SloppyType.SloppySingle),
+                    ("mixed_fp16", Q.SloppyType.SloppyHalf), ("mixed_bf16", Q.SloppyType.SloppyBF16)]
+        for name, sl in variants:
+            best = None
+            for rep in range(2):  # first call packs the sloppy links
+                sp = Q.SolverParams()
+                sp.r2req = R2REQ
+                sp.maxits = 20000
+                sp.verbosity = 0
+                sp.sloppySolve = sl
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                Q.solveEE(s, x, b, MASS, sp)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                if best is None or dt < best["seconds"]:
+                    best = {"seconds": round(dt, 5), "iterations": sp.iterations,
+                            "reliable_updates": sp.reliableUpdates,
+                            "true_rel_resid": float(sp.r2) ** 0.5, "true_r2_rel": sp.r2,
+                            "qex_gflops": round(1e-9 * sp.flops / dt, 1)}
+            cg[name] = best
+        base = cg["fp64"]["seconds"]
+        for name in cg:
+            cg[name]["speedup_vs_fp64"] = round(base / cg[name]["seconds"], 3)
+        out["cg"] = cg
+
+    # ---- CPU baseline (oracle on host cores) ----
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
