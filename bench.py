@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--no-cg", action="store_true", help="skip the CG comparison")
     ap.add_argument("--no-table", action="store_true", help="skip the hop table")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
+    ap.add_argument("--cg-only", default="", help="comma-separated CG variants to run")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -197,9 +198,17 @@ def main():
         b = lo.ColorVector().set(gauge.source(lo, 7, 0, 0))
         x = lo.ColorVector()
         cg = {}
-        variants = [("fp64", Q.SloppyType.SloppyNone), ("mixed_fp32", Q.SloppyType.SloppySingle),
-                    ("mixed_fp16", Q.SloppyType.SloppyHalf), ("mixed_bf16", Q.SloppyType.SloppyBF16)]
-        for name, sl in variants:
+        S = Q.SloppyType
+        # (name, spinor storage, link storage, link reals)
+        variants = [("fp64", S.SloppyNone, None, 18), ("fp64_r12", S.SloppyNone, None, 12),
+                    ("mixed_fp32", S.SloppySingle, None, 18), ("mixed_fp32_r12", S.SloppySingle, None, 12),
+                    ("mixed_fp16", S.SloppyHalf, None, 18), ("mixed_fp16_r12", S.SloppyHalf, None, 12),
+                    ("mixed_fp16links_fp32vec_r12", S.SloppySingle, S.SloppyHalf, 12),
+                    ("mixed_bf16", S.SloppyBF16, None, 18),
+                    ("mixed_bf16links_fp32vec_r12", S.SloppySingle, S.SloppyBF16, 12)]
+        if args.cg_only:
+            variants = [v for v in variants if v[0] in args.cg_only.split(",")]
+        for name, sl, sll, rc in variants:
             best = None
             for rep in range(2):  # first call packs the sloppy links
                 sp = Q.SolverParams()
@@ -207,6 +216,8 @@ def main():
                 sp.maxits = 20000
                 sp.verbosity = 0
                 sp.sloppySolve = sl
+                sp.sloppyLinks = sll
+                sp.recon = rc
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 Q.solveEE(s, x, b, MASS, sp)
@@ -218,9 +229,14 @@ def main():
                             "true_rel_resid": float(sp.r2) ** 0.5, "true_r2_rel": sp.r2,
                             "qex_gflops": round(1e-9 * sp.flops / dt, 1)}
             cg[name] = best
-        base = cg["fp64"]["seconds"]
-        for name in cg:
-            cg[name]["speedup_vs_fp64"] = round(base / cg[name]["seconds"], 3)
+        if "fp64" in cg:
+            base = cg["fp64"]["seconds"]
+            for name in cg:
+                cg[name]["speedup_vs_fp64"] = round(base / cg[name]["seconds"], 3)
+            conv = {k: v for k, v in cg.items() if k != "fp64" and v["true_r2_rel"] <= R2REQ}
+            if conv:
+                best = min(conv, key=lambda k: conv[k]["seconds"])
+                out["cg_best_mixed"] = {"variant": best, "speedup_vs_fp64": cg[best]["speedup_vs_fp64"]}
         out["cg"] = cg
 
     # ---- CPU baseline (oracle on host cores) ----

@@ -53,6 +53,8 @@ typedef struct {
                              FP32-arithmetic inner CG with this spinor/link storage and
                              FP64 reliable updates (QEX sloppySolve, solverBase.nim:8-9) */
   int sloppy_links;       /* link storage of the inner operator; -1 = same as sloppy */
+  int recon;              /* inner-operator links: 18 reals, or 12 (rows 0-1 stored, row 2
+                             rebuilt; plain phased-SU(3) links only); 0 = 18 */
   double reliable_delta;  /* reliable update when |r| < delta*|r_last_update| (0.1) */
   int reliable_max_its;   /* ... or after this many iterations (100) */
   int use_prev_soln;      /* 1: the solution field holds the initial guess (QEX usePrevSoln) */
@@ -144,8 +146,12 @@ int qexhip_gen_hisq(qexhip_lattice *lat, uint64_t seed, const double *Uph_dev, d
 
 /* ---- measurement (the in-module bench of stagD.nim:767-810) ----
    average duration (ms, HIP events on the library stream) of one one-parity
-   hop r_e = (2D)_eo x_o with links and fields stored in precision prec. */
-int qexhip_time_hop(qexhip_links *l, int prec, int nrep, double *ms_per_hop);
+   hop r_e = (2D)_eo x_o with links and fields stored in precision prec and
+   links of recon (18 or 12) reals. */
+int qexhip_time_hop(qexhip_links *l, int prec, int recon, int nrep, double *ms_per_hop);
+/* whether the links are phased SU(3) matrices, so 12-real storage is exact
+   (1) or not (0) */
+int qexhip_links_recon_ok(qexhip_links *l, int *ok);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libqexhip.so")
+LIB_PATH = os.environ.get("QEXHIP_LIB") or os.path.join(_HERE, "libqexhip.so")
 
 FP64, FP32, FP16, BF16 = 0, 1, 2, 3
 EVEN, ODD, ALL = 0, 1, 2
@@ -24,7 +24,7 @@ EXPORTS = [
     "qexhip_solve_xx", "qexhip_solve_ee_host", "qexhip_solve_oo_host", "qexhip_solve_multishift",
     "qexhip_force_solve", "qexhip_force",
     "qexhip_gen_gauge", "qexhip_gen_source", "qexhip_gen_phase_bc", "qexhip_gen_hisq",
-    "qexhip_time_hop",
+    "qexhip_time_hop", "qexhip_links_recon_ok",
 ]
 
 
@@ -34,6 +34,7 @@ class SolverParamsC(ctypes.Structure):
         ("maxits", ctypes.c_int),
         ("sloppy", ctypes.c_int),
         ("sloppy_links", ctypes.c_int),
+        ("recon", ctypes.c_int),
         ("reliable_delta", ctypes.c_double),
         ("reliable_max_its", ctypes.c_int),
         ("use_prev_soln", ctypes.c_int),
@@ -82,7 +83,7 @@ def load(path=LIB_PATH):
         "qexhip_force_solve": [vp, vp, vp, D, spp], "qexhip_force": [vp, vp, vp, I, I],
         "qexhip_gen_gauge": [vp, U64, vp], "qexhip_gen_source": [vp, U64, I, I, vp],
         "qexhip_gen_phase_bc": [vp, vp], "qexhip_gen_hisq": [vp, U64, vp, vp],
-        "qexhip_time_hop": [vp, I, I, dp],
+        "qexhip_time_hop": [vp, I, I, I, dp], "qexhip_links_recon_ok": [vp, ip],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["kernels.hip", "gen.hip", "api.hip"]
+SOURCES = ["dslash.hip", "kernels.hip", "gen.hip", "api.hip"]
 OUT = os.path.join(HERE, "libqexhip.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -29,12 +29,12 @@ FLAGS = [
 ]
 
 
-def build(verbose=True, jobs=3):
+def build(verbose=True, jobs=4, out=OUT, defines=()):
     objs = []
     procs = []
     for src in SOURCES:
-        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
-        cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        obj = out + "." + src.replace(".hip", ".o")
+        cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -43,14 +43,14 @@ def build(verbose=True, jobs=3):
             _wait(procs)
             procs = []
     _wait(procs)
-    link = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", OUT + ".tmp"] + objs
+    link = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-Wl,-z,defs", "-o", out + ".tmp"] + objs
     if verbose:
         print(" ".join(link), flush=True)
     subprocess.run(link, check=True)
-    os.replace(OUT + ".tmp", OUT)
+    os.replace(out + ".tmp", out)
     for o in objs:
         os.remove(o)
-    return OUT
+    return out
 
 
 def _wait(procs):

@@ -22,6 +22,7 @@ thread_local std::string g_err;
 hipStream_t g_stream = nullptr;
 int g_device = -1;
 int g_block = 256;
+int g_remap = 1;
 
 #define HIPCHK(x)                                                                                  \
   do {                                                                                             \
@@ -102,8 +103,9 @@ struct Work {
 struct qexhip_links {
   qexhip_lattice *lat;
   int nl;
-  double *canon = nullptr;  // device canonical FP64 copy, nl*V*18
-  void *st[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *canon = nullptr;             // device canonical FP64 copy, nl*V*18
+  char *st[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+  int recon_ok = -1;                   // -1 unknown, 0 no, 1 phased SU(3)
   Work *ws = nullptr;
 };
 
@@ -111,16 +113,38 @@ namespace {
 
 size_t vec64_bytes(const qexhip_lattice *lat) { return (size_t)3 * lat->g.stride * 16; }
 
-const void *links_store(qexhip_links *l, int prec) {
-  if (!l->st[prec]) {
-    const qexhip_lattice *lat = l->lat;
-    size_t bytes = (size_t)l->nl * 2 * 9 * lat->g.stride * word_bytes(prec);
-    l->st[prec] = dalloc<void>(bytes);
-    launch_fill(l->st[prec], bytes, g_stream);
-    launch_pack_links(prec, 0, lat->g, l->nl, l->canon, l->st[prec], g_stream);
+bool recon_allowed(qexhip_links *l) {
+  if (l->nl != 4) return false;
+  if (l->recon_ok < 0) {
+    unsigned *d = dalloc<unsigned>(16);
+    launch_fill(d, 16, g_stream);
+    launch_recon_check(l->lat->g, l->canon, d, g_stream);
+    check_launch();
+    unsigned bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, d, 4, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    (void)hipFree(d);
+    float dev;
+    memcpy(&dev, &bits, 4);
+    l->recon_ok = (dev < 1e-10f) ? 1 : 0;
+  }
+  return l->recon_ok == 1;
+}
+
+long lkblk_bytes(const qexhip_links *l, int prec, int rc) { return link_block_bytes(prec, rc, l->lat->g.stride); }
+
+const char *links_store(qexhip_links *l, int prec, int rc = 18) {
+  require(rc == 18 || rc == 12, "recon must be 18 or 12");
+  if (rc == 12) require(recon_allowed(l), "12-real links need plain (nl=4) phased SU(3) links");
+  const int ri = (rc == 12);
+  if (!l->st[prec][ri]) {
+    const size_t bytes = (size_t)l->nl * 2 * lkblk_bytes(l, prec, rc);
+    l->st[prec][ri] = dalloc<char>(bytes);
+    launch_fill(l->st[prec][ri], bytes, g_stream);
+    launch_pack_links(prec, rc, l->lat->g, l->nl, l->canon, l->st[prec][ri], g_stream);
     check_launch();
   }
-  return l->st[prec];
+  return l->st[prec][ri];
 }
 
 Work *work(qexhip_links *l) {
@@ -163,13 +187,14 @@ void pull(Scal *Sh, const Scal *S) {
   HIPCHK(hipStreamSynchronize(g_stream));
 }
 
-// One hop launch: out = ca*A + cb*B + chop*(2D)in on output parity par.
-void hop(qexhip_links *l, int pl, int pi, int po, int pa, int par, int mode, int redm, const void *in, void *out,
-         const void *a, const void *b, double ca, double cb, double chop, int ca_m2, int pred, Scal *S) {
+// One hop launch: out = ca*A + cb*B + chop*(2D)in on output parity par, with
+// links and vectors in precision prec (arithmetic FP64 for FP64, else FP32).
+void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const void *in, void *out, const void *a,
+         const void *b, double ca, double cb, double chop, int ca_m2, int pred, Scal *S) {
   HopArgs h;
   h.g = l->lat->g;
-  h.links = links_store(l, pl);
-  h.lkblk = 9 * h.g.stride;
+  h.links = links_store(l, prec, rc);
+  h.lkblk = lkblk_bytes(l, prec, rc);
   h.in = in;
   h.out = out;
   h.a = a;
@@ -180,18 +205,19 @@ void hop(qexhip_links *l, int pl, int pi, int po, int pa, int par, int mode, int
   h.ca_m2 = ca_m2;
   h.par = par;
   h.pred = pred;
+  h.remap = g_remap;
   h.S = S;
   h.red = l->lat->red;
-  const bool dbl = (pl == P64);
-  launch_hop(pl, pi, po, pa, dbl, l->nl, mode, redm, h, g_stream, g_block);
+  HopKey k{prec, rc, prec, prec, prec};
+  launch_hop(k, l->nl, mode, redm, h, g_stream, g_block);
 }
 
 // A = 4(m2 - D_pq D_qp) applied to x (FP64, own parity par): out = cx*x + cb*b - sgn...
 // Computes out = b - A x  (when with_b) with |out|^2 reduced per redm.
 void resid64(qexhip_links *l, int par, const double *x, const double *b, double *out, Scal *S, int redm,
              double *t64) {
-  hop(l, P64, P64, P64, P64, 1 - par, 0, 0, x, t64, nullptr, nullptr, 0, 0, 1.0, 0, 0, S);
-  hop(l, P64, P64, P64, P64, par, 2, redm, t64, out, x, b, -1.0, 1.0, 1.0, 1, 0, S);
+  hop(l, P64, 18, 1 - par, 0, 0, x, t64, nullptr, nullptr, 0, 0, 1.0, 0, 0, S);
+  hop(l, P64, 18, par, 2, redm, t64, out, x, b, -1.0, 1.0, 1.0, 1, 0, S);
 }
 
 double flops_per_its(const qexhip_links *l) {
@@ -200,20 +226,25 @@ double flops_per_its(const qexhip_links *l) {
 
 const int kChunk = 16;
 
-// CG on A = 4(m^2 - D_pq D_qp), parity par (solvers/cg.nim:55-272, cpNone).
-// FP64 mode follows QEX's recurrence exactly.  Mixed mode runs the same
-// recurrence in FP32 arithmetic on sloppy storage with FP64 reliable updates
-// (true residual, solution accumulation in FP64).  x: FP64 in/out.
+// CG on A = 4(m^2 - D_pq D_qp), parity par (solvers/cg.nim:55-272, cpNone),
+// two fused kernels per iteration (dslash.hip).  FP64 mode follows QEX's
+// recurrence (alpha from exact norms; beta from a one-step extrapolation of
+// |r|^2 computed in FP64, the stop test confirmed on the exact norm).  Mixed
+// mode runs it in FP32 arithmetic on sloppy storage with FP64 reliable updates
+// (true residual, FP64 solution accumulation).  x: FP64 in/out.
 void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, double r2req, int maxits,
-              int sloppy, int slinks, double delta, int relmax, int *its_out, int *nrel_out) {
+              int sloppy, int slinks, int recon, double delta, int relmax, int *its_out, int *nrel_out) {
   Work *w = work(l);
   qexhip_lattice *lat = l->lat;
   const Geom &g = lat->g;
   Scal *S = w->S, *Sh = w->Sh;
+  const int rc = (recon == 12) ? 12 : 18;
+  const bool mixed = (sloppy != P64);
+  const int ps = sloppy, pl = mixed ? ((slinks < 0) ? sloppy : slinks) : P64;
   memset(Sh, 0, sizeof(Scal));
   Sh->m2 = m * m;
   Sh->maxits = maxits;
-  Sh->mixed = (sloppy != P64);
+  Sh->mixed = mixed;
   Sh->delta2 = delta * delta;
   Sh->relmax = relmax > 0 ? relmax : (1 << 30);
   push(S, Sh);
@@ -231,52 +262,51 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   Sh->b2 = b2;
   Sh->r2stop = r2req * b2;
   push(S, Sh);
-  if (!Sh->mixed) {
-    // r = b - A x ; r2 = |r|^2 ; p = 0 (cg.nim:145-153)
+  CgArgs a;
+  a.g = g;
+  a.links = links_store(l, pl, rc);
+  a.lkblk = lkblk_bytes(l, pl, rc);
+  a.par = par;
+  a.remap = g_remap;
+  a.S = S;
+  a.red = lat->red;
+  if (!mixed) {
+    a.x = x; a.r = w->r64; a.p = w->p64; a.Ap = w->Ap64; a.t = w->t64;
+    launch_fill(w->Ap64, vec64_bytes(lat), g_stream);
+    // r = b - A x ; r2 = |r|^2 (cg.nim:145-153)
     resid64(l, par, x, b, w->r64, S, 3, w->t64);
     pull(Sh, S);
     Sh->r2 = Sh->tmp[0];
+    Sh->r2prev = Sh->r2;
     Sh->done = !(Sh->r2 > Sh->r2stop) || maxits <= 0;
     push(S, Sh);
-    while (!Sh->done) {
-      for (int k = 0; k < kChunk; k++) {
-        launch_pupdate(P64, P64, g, w->p64, w->r64, S, g_stream);
-        hop(l, P64, P64, P64, P64, 1 - par, 0, 0, w->p64, w->t64, nullptr, nullptr, 0, 0, 1.0, 0, 1, S);
-        hop(l, P64, P64, P64, P64, par, 1, 1, w->t64, w->Ap64, w->p64, nullptr, 1.0, 0, -1.0, 1, 1, S);
-        launch_xr(P64, P64, P64, P64, g, x, w->p64, w->r64, w->Ap64, S, lat->red, g_stream);
-      }
-      check_launch();
-      pull(Sh, S);
-    }
-    *its_out = Sh->its;
-    return;
-  }
-  // mixed precision with reliable updates
-  const int ss = sloppy, sl = (slinks < 0) ? sloppy : slinks;
-  launch_fill(w->xs, (size_t)3 * g.stride * 8, g_stream);
-  resid64(l, par, x, b, w->r64, S, 2, w->t64);  // epi_reliable: sigma, r2, done
-  launch_rs_from_true(P32, g, w->rs, w->r64, S, g_stream);
-  check_launch();
-  pull(Sh, S);
-  while (!Sh->done) {
-    for (int k = 0; k < kChunk; k++) {
-      launch_pupdate(ss, P32, g, w->ps, w->rs, S, g_stream);
-      hop(l, sl, ss, ss, ss, 1 - par, 0, 0, w->ps, w->ts, nullptr, nullptr, 0, 0, 1.0, 0, 1, S);
-      hop(l, sl, ss, P32, ss, par, 1, 1, w->ts, w->Aps, w->ps, nullptr, 1.0, 0, -1.0, 1, 1, S);
-      launch_xr(P32, ss, P32, P32, g, w->xs, w->ps, w->rs, w->Aps, S, lat->red, g_stream);
-    }
+  } else {
+    a.x = w->xs; a.r = w->rs; a.p = w->ps; a.Ap = w->Aps; a.t = w->ts;
+    const size_t b32 = (size_t)3 * g.stride * 8;
+    launch_fill(w->xs, b32, g_stream);
+    launch_fill(w->Aps, b32, g_stream);
+    resid64(l, par, x, b, w->r64, S, 2, w->t64);  // epi_reliable: sigma, r2, done
+    launch_rs_from_true(g, w->rs, w->r64, S, g_stream);
     check_launch();
     pull(Sh, S);
-    if (Sh->need_rel) {
-      launch_xfold(g, x, w->xs, S, g_stream);
-      resid64(l, par, x, b, w->r64, S, 2, w->t64);
-      launch_rs_from_true(P32, g, w->rs, w->r64, S, g_stream);
+  }
+  while (!Sh->done) {
+    for (int k = 0; k < kChunk; k++) launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, g_block);
+    check_launch();
+    pull(Sh, S);
+    if (Sh->done || Sh->need_rel) {
+      launch_cg_flush(ps, a, g_stream);  // apply the pending x/r update; exact |r|^2
+      if (mixed) {
+        launch_xfold(g, x, w->xs, S, g_stream);
+        resid64(l, par, x, b, w->r64, S, 2, w->t64);
+        launch_rs_from_true(g, w->rs, w->r64, S, g_stream);
+      }
       check_launch();
       pull(Sh, S);
     }
   }
   *its_out = Sh->its;
-  *nrel_out = Sh->nrel - 1;
+  *nrel_out = mixed ? Sh->nrel - 1 : 0;
 }
 
 // solveXX (src/physics/stagSolve.nim:330-421): outer defect-correction loop.
@@ -304,8 +334,8 @@ void solve_xx(qexhip_links *l, int par, double *r, const double *x, double m, qe
     if (mx <= 0) break;
     launch_fill(w->y, vb, g_stream);
     int it = 0, nr = 0;
-    cg_solve(l, par, w->y, w->rho, m, r2stop / r2, mx, sp->sloppy, sp->sloppy_links, sp->reliable_delta,
-             sp->reliable_max_its, &it, &nr);
+    cg_solve(l, par, w->y, w->rho, m, r2stop / r2, mx, sp->sloppy, sp->sloppy_links, sp->recon,
+             sp->reliable_delta, sp->reliable_max_its, &it, &nr);
     its += it;
     nrel += nr;
     launch_axpby64(g, 1.0, w->y, 1.0, r, g_stream);  // r[par] += y
@@ -379,8 +409,8 @@ void solve_ms(qexhip_links *l, int par, double *const *xs, const double *b, doub
     HIPCHK(hipMemcpyAsync(w->msp[k], w->r64, vb, hipMemcpyDeviceToDevice, g_stream));  // ps[m] := r
   while (Th->continuing) {
     for (int c = 0; c < kChunk; c++) {
-      hop(l, P64, P64, P64, P64, 1 - par, 0, 0, w->msp[0], w->t64, nullptr, nullptr, 0, 0, 1.0, 0, 1, S);
-      hop(l, P64, P64, P64, P64, par, 1, 1, w->t64, w->Ap64, w->msp[0], nullptr, 1.0, 0, -1.0, 1, 1, S);
+      hop(l, P64, 18, 1 - par, 0, 0, w->msp[0], w->t64, nullptr, nullptr, 0, 0, 1.0, 0, 1, S);
+      hop(l, P64, 18, par, 1, 1, w->t64, w->Ap64, w->msp[0], nullptr, 1.0, 0, -1.0, 1, 1, S);
       launch_cgm_alpha(T, S, g_stream);
       launch_cgm_xr(g, nshift, w->r64, w->Ap64, w->d_x, w->d_p, T, S, lat->red, g_stream);
       launch_cgm_p(g, nshift, w->r64, w->d_p, T, g_stream);
@@ -447,8 +477,9 @@ int qexhip_init(int device) {
   g_device = device;
   if (const char *b = getenv("QEXHIP_BLOCK")) {
     int v = atoi(b);
-    if (v >= 64 && v <= 1024 && v % 64 == 0) g_block = v;
+    if (v >= 64 && v <= 256 && v % 64 == 0) g_block = v;
   }
+  if (const char *r = getenv("QEXHIP_XCD_REMAP")) g_remap = atoi(r) != 0;
   API_END
 }
 
@@ -522,7 +553,7 @@ int qexhip_links_create(qexhip_lattice *lat, int nl, const double *canon, int on
   l->canon = dalloc<double>(bytes);
   HIPCHK(hipMemcpyAsync(l->canon, canon, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         g_stream));
-  links_store(l, P64);
+  links_store(l, P64, 18);
   HIPCHK(hipStreamSynchronize(g_stream));
   *out = l;
   API_END
@@ -535,8 +566,17 @@ int qexhip_links_update(qexhip_links *l, const double *canon, int on_device) {
   const size_t bytes = (size_t)l->nl * l->lat->V * 18 * sizeof(double);
   HIPCHK(hipMemcpyAsync(l->canon, canon, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         g_stream));
+  l->recon_ok = -1;
   for (int p = 0; p < 4; p++)
-    if (l->st[p]) launch_pack_links(p, 0, l->lat->g, l->nl, l->canon, l->st[p], g_stream);
+    for (int ri = 0; ri < 2; ri++) {
+      if (!l->st[p][ri]) continue;
+      if (ri == 1 && !recon_allowed(l)) {
+        (void)hipFree(l->st[p][ri]);
+        l->st[p][ri] = nullptr;
+        continue;
+      }
+      launch_pack_links(p, ri ? 12 : 18, l->lat->g, l->nl, l->canon, l->st[p][ri], g_stream);
+    }
   check_launch();
   HIPCHK(hipStreamSynchronize(g_stream));
   API_END
@@ -548,7 +588,8 @@ int qexhip_links_destroy(qexhip_links *l) {
   HIPCHK(hipStreamSynchronize(g_stream));
   (void)hipFree(l->canon);
   for (int p = 0; p < 4; p++)
-    if (l->st[p]) (void)hipFree(l->st[p]);
+    for (int ri = 0; ri < 2; ri++)
+      if (l->st[p][ri]) (void)hipFree(l->st[p][ri]);
   free_work(l->ws);
   delete l;
   API_END
@@ -643,10 +684,9 @@ static void dslash_core(qexhip_links *l, int par, qexhip_field *r, const qexhip_
   for (int p = 0; p < 2; p++) {
     if (par != 2 && par != p) continue;
     if (ca == 0.0)
-      hop(l, pr, pr, pr, pr, p, 1, 0, x->par(1 - p), r->par(p), x->par(p), nullptr, cb, 0, chop, 0, 0, nullptr);
+      hop(l, pr, 18, p, 1, 0, x->par(1 - p), r->par(p), x->par(p), nullptr, cb, 0, chop, 0, 0, nullptr);
     else
-      hop(l, pr, pr, pr, pr, p, 2, 0, x->par(1 - p), r->par(p), r->par(p), x->par(p), ca, cb, chop, 0, 0,
-          nullptr);
+      hop(l, pr, 18, p, 2, 0, x->par(1 - p), r->par(p), r->par(p), x->par(p), ca, cb, chop, 0, 0, nullptr);
     check_launch();
   }
   HIPCHK(hipStreamSynchronize(g_stream));
@@ -678,8 +718,8 @@ int qexhip_stagD2xx(qexhip_links *l, int par, qexhip_field *r, const qexhip_fiel
   const int pr = r->prec;
   void *t = (pr == P64) ? (void *)w->t64 : w->ts;
   // t = (2D)_qp x_p  (stagDP, rir := 0) ; r = 4 m2 x - (2D)_pq t  (stagDM)
-  hop(l, pr, pr, pr, pr, 1 - par, 0, 0, x->par(par), t, nullptr, nullptr, 0, 0, 1.0, 0, 0, nullptr);
-  hop(l, pr, pr, pr, pr, par, 1, 0, t, r->par(par), x->par(par), nullptr, 4.0 * m2, 0, -1.0, 0, 0, nullptr);
+  hop(l, pr, 18, 1 - par, 0, 0, x->par(par), t, nullptr, nullptr, 0, 0, 1.0, 0, 0, nullptr);
+  hop(l, pr, 18, par, 1, 0, t, r->par(par), x->par(par), nullptr, 4.0 * m2, 0, -1.0, 0, 0, nullptr);
   check_launch();
   HIPCHK(hipStreamSynchronize(g_stream));
   API_END
@@ -696,11 +736,13 @@ int qexhip_solve_xx(qexhip_links *l, int par, qexhip_field *r, const qexhip_fiel
   require(r->prec == P64 && x->prec == P64, "solver fields must be FP64");
   require(sp->sloppy >= 0 && sp->sloppy <= 3, "bad sloppy precision");
   require(sp->sloppy_links >= -1 && sp->sloppy_links <= 3, "bad sloppy link precision");
+  require(sp->recon == 0 || sp->recon == 18 || sp->recon == 12, "recon must be 0, 18 or 12");
   if (sp->sloppy != P64) {
     const int sl = sp->sloppy_links < 0 ? sp->sloppy : sp->sloppy_links;
     require(sl != P64, "mixed solve needs sloppy links below FP64");
     require(sl == sp->sloppy || sp->sloppy == P32, "sloppy link/spinor storage combination not supported");
   }
+  if (sp->recon == 12) require(recon_allowed(l), "12-real links need plain (nl=4) phased SU(3) links");
   solve_xx(l, par, (double *)r->par(par), (const double *)x->par(par), m, sp);
   API_END
 }
@@ -761,7 +803,7 @@ int qexhip_force_solve(qexhip_links *l, qexhip_field *psi, const qexhip_field *p
   // solveEE(varphi, phi) ; varphi.even *= 4 ; psi.odd = (2D)_oe varphi ; psi.even = varphi
   solve_xx(l, 0, (double *)psi->par(0), (const double *)phi->par(0), m, sp);
   launch_axpby64(g, 0.0, (const double *)psi->par(0), 4.0, (double *)psi->par(0), g_stream);
-  hop(l, P64, P64, P64, P64, 1, 0, 0, psi->par(0), psi->par(1), nullptr, nullptr, 0, 0, 1.0, 0, 0, nullptr);
+  hop(l, P64, 18, 1, 0, 0, psi->par(0), psi->par(1), nullptr, nullptr, 0, 0, 1.0, 0, 0, nullptr);
   check_launch();
   HIPCHK(hipStreamSynchronize(g_stream));
   API_END
@@ -778,7 +820,7 @@ int qexhip_force(qexhip_links *l, const qexhip_field *psi, double *F, int F_on_d
   const size_t bytes = (size_t)4 * lat->V * 18 * sizeof(double);
   double *dF = F;
   if (!F_on_device) dF = dalloc<double>(bytes);
-  launch_force(lat->g, (const double *)links_store(l, P64), 9 * lat->g.stride, (const double *)psi->par(0),
+  launch_force(lat->g, links_store(l, P64, 18), lkblk_bytes(l, P64, 18), (const double *)psi->par(0),
                (const double *)psi->par(1), dF, outer_only, g_stream);
   check_launch();
   if (!F_on_device) {
@@ -828,7 +870,15 @@ int qexhip_gen_hisq(qexhip_lattice *lat, uint64_t seed, const double *Uph, doubl
   API_END
 }
 
-int qexhip_time_hop(qexhip_links *l, int prec, int nrep, double *ms) {
+int qexhip_links_recon_ok(qexhip_links *l, int *ok) {
+  API_BEGIN
+  check_init();
+  require(l && ok, "null argument");
+  *ok = recon_allowed(l) ? 1 : 0;
+  API_END
+}
+
+int qexhip_time_hop(qexhip_links *l, int prec, int recon, int nrep, double *ms) {
   API_BEGIN
   check_init();
   require(l && ms && nrep > 0, "bad arguments");
@@ -838,14 +888,15 @@ int qexhip_time_hop(qexhip_links *l, int prec, int nrep, double *ms) {
   if (qexhip_field_create(lat, prec, &x) || qexhip_field_create(lat, prec, &r)) throw std::runtime_error(g_err);
   launch_source_gauss(lat->g, 99, 0, 2, lat->stage, g_stream);
   import_field(x, 2, lat->stage, 1);
+  const int rc = (recon == 12) ? 12 : 18;
   for (int i = 0; i < 3; i++)
-    hop(l, prec, prec, prec, prec, 0, 0, 0, x->par(1), r->par(0), nullptr, nullptr, 0, 0, 1.0, 0, 0, nullptr);
+    hop(l, prec, rc, 0, 0, 0, x->par(1), r->par(0), nullptr, nullptr, 0, 0, 1.0, 0, 0, nullptr);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, g_stream));
   for (int i = 0; i < nrep; i++)
-    hop(l, prec, prec, prec, prec, 0, 0, 0, x->par(1), r->par(0), nullptr, nullptr, 0, 0, 1.0, 0, 0, nullptr);
+    hop(l, prec, rc, 0, 0, 0, x->par(1), r->par(0), nullptr, nullptr, 0, 0, 1.0, 0, 0, nullptr);
   HIPCHK(hipEventRecord(e1, g_stream));
   HIPCHK(hipEventSynchronize(e1));
   float t = 0;

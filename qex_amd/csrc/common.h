@@ -12,6 +12,11 @@
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 
+// software-pipeline depth of the stencil, in links (see dslash.hip)
+#ifndef QH_PIPE
+#define QH_PIPE -2
+#endif
+
 namespace qh {
 
 // ------------------------------------------------------------ precision tags
@@ -43,26 +48,31 @@ __device__ __forceinline__ uint32_t f_to_bf16(float f) {
 }
 
 // load one complex (as accumulation type A) from storage word
+// element offsets are 32-bit (fields are < 4 GiB): scalar base + vector offset
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
 template <int P, class A> __device__ __forceinline__ A ldc(const void *base, long off);
 template <> __device__ __forceinline__ double2 ldc<P64, double2>(const void *b, long o) {
-  return ((const double2 *)b)[o];
+  const d2v v = *(const d2v *)((const char *)b + (uint32_t)o * 16u);
+  return make_double2(v.x, v.y);
 }
 template <> __device__ __forceinline__ float2 ldc<P32, float2>(const void *b, long o) {
-  return ((const float2 *)b)[o];
+  const f2v v = *(const f2v *)((const char *)b + (uint32_t)o * 8u);
+  return make_float2(v.x, v.y);
 }
 template <> __device__ __forceinline__ double2 ldc<P32, double2>(const void *b, long o) {
-  float2 f = ((const float2 *)b)[o];
-  return make_double2(f.x, f.y);
+  const f2v v = *(const f2v *)((const char *)b + (uint32_t)o * 8u);
+  return make_double2(v.x, v.y);
 }
 template <> __device__ __forceinline__ float2 ldc<P64, float2>(const void *b, long o) {
-  double2 d = ((const double2 *)b)[o];
-  return make_float2((float)d.x, (float)d.y);
+  const d2v v = *(const d2v *)((const char *)b + (uint32_t)o * 16u);
+  return make_float2((float)v.x, (float)v.y);
 }
 template <> __device__ __forceinline__ float2 ldc<P16, float2>(const void *b, long o) {
-  return __half22float2(((const __half2 *)b)[o]);
+  return __half22float2(*(const __half2 *)((const char *)b + (uint32_t)o * 4u));
 }
 template <> __device__ __forceinline__ float2 ldc<PBF, float2>(const void *b, long o) {
-  uint32_t w = ((const uint32_t *)b)[o];
+  const uint32_t w = *(const uint32_t *)((const char *)b + (uint32_t)o * 4u);
   return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
 }
 
@@ -180,6 +190,182 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* 16*NV 
     }
   }
   __syncthreads();
+}
+
+// Reduction workspace shared by consecutive kernels on one stream.
+struct RedWs {
+  double *partials;   // [4][maxblocks]
+  unsigned *counter;
+  int maxblocks;
+};
+
+__device__ __forceinline__ double ld_coherent(const double *p) {
+  unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double((long long)u);
+}
+
+// Block partials -> global partials; the last block to arrive sums all partials
+// in a fixed order (deterministic) and returns true (result valid in thread 0).
+template <int NV>
+__device__ __forceinline__ bool grid_finish(double (&v)[NV], double *lds, const RedWs &red) {
+  __shared__ int last;
+  const int nb = gridDim.x * gridDim.y;
+  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+  block_sum<NV>(v, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) red.partials[(long)k * red.maxblocks + bid] = v[k];
+    __threadfence();
+    unsigned t = atomicAdd(red.counter, 1u);
+    last = (t == (unsigned)(nb - 1));
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) s[k] = 0.0;
+  for (int j = threadIdx.x; j < nb; j += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < NV; k++) s[k] += ld_coherent(red.partials + (long)k * red.maxblocks + j);
+  block_sum<NV>(s, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = s[k];
+    atomicExch(red.counter, 0u);
+  }
+  return true;
+}
+
+// XCD-aware block order: the hardware deals workgroups round-robin over the 8
+// XCDs (block b runs on XCD b%8); this remap gives each XCD one contiguous run
+// of sites, so stencil neighbours (y, z, t rows) are read through the same L2.
+__device__ __forceinline__ long xcd_block(int remap) {
+  const long b = blockIdx.x;
+  if (!remap) return b;
+  const long nb = gridDim.x, q = nb >> 3, r = nb & 7;
+  const long xcd = b & 7, idx = b >> 3;
+  return xcd * q + (xcd < r ? xcd : r) + idx;
+}
+
+// ------------------------------------------------------------- link storage
+// One link = RC reals (18, or 12 = first two rows) in precision PL, stored as
+// 16-byte planes plus one narrower remainder plane:
+//   block (k, parity): plane p (0 <= p < nfull) at byte (p*stride + i)*16,
+//                      remainder plane at byte nfull*16*stride + i*rem.
+// Rows 0,1 of a 12-real link hold the phased link; row 2 is rebuilt as
+// sign * conj(row0 x row1), the sign being the staggered phase times the
+// antiperiodic-t boundary sign of that link (phased SU(3) = sign * SU(3)).
+template <int PL, int RC> struct LFmt {
+  static constexpr int bpr = (PL == P64) ? 8 : (PL == P32) ? 4 : 2;
+  static constexpr int bytes = RC * bpr;
+  static constexpr int nfull = bytes / 16;
+  static constexpr int rem = bytes % 16;
+  static constexpr int words = bytes / 4;
+};
+
+inline long link_block_bytes(int prec, int rc, long stride) {
+  const int bpr = (prec == P64) ? 8 : (prec == P32) ? 4 : 2;
+  return (long)rc * bpr * stride;
+}
+
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+
+template <int PL, int RC>
+__device__ __forceinline__ void st_link_words(char *blk, long stride, long i, const uint32_t (&w)[LFmt<PL, RC>::words]) {
+  using F = LFmt<PL, RC>;
+#pragma unroll
+  for (int p = 0; p < F::nfull; p++) {
+    u4v v;
+    v.x = w[4 * p]; v.y = w[4 * p + 1]; v.z = w[4 * p + 2]; v.w = w[4 * p + 3];
+    *(u4v *)(blk + ((long)p * stride + i) * 16) = v;
+  }
+  if (F::rem == 8) {
+    u2v v;
+    v.x = w[4 * F::nfull]; v.y = w[4 * F::nfull + 1];
+    *(u2v *)(blk + (long)F::nfull * 16 * stride + i * 8) = v;
+  } else if (F::rem == 4) {
+    *(unsigned *)(blk + (long)F::nfull * 16 * stride + i * 4) = w[4 * F::nfull];
+  }
+}
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <class T, bool NT> __device__ __forceinline__ T ldv(const char *p) {
+  return NT ? __builtin_nontemporal_load((const T *)p) : *(const T *)p;
+}
+
+// Load one link (LFmt layout, see above) straight into 3x3 complex form,
+// rebuilding row 2 for 12-real storage: row2_k = sgn*conj(row0 x row1)_k.
+template <int PL, int RC, bool NT, class A>
+__device__ __forceinline__ void ld_link(const char *blk, long stride, long i, A (&U)[9], typename cx<A>::R sgn) {
+  using F = LFmt<PL, RC>;
+  const uint32_t s16 = (uint32_t)stride * 16u, i16 = (uint32_t)i * 16u;
+  if constexpr (PL == P64) {
+#pragma unroll
+    for (int c = 0; c < RC / 2; c++) {
+      const d2v v = ldv<d2v, NT>(blk + (uint32_t)(c * s16 + i16));
+      U[c] = mk<A>(v.x, v.y);
+    }
+  } else if constexpr (PL == P32) {
+#pragma unroll
+    for (int p = 0; p < F::nfull; p++) {
+      const f4v v = ldv<f4v, NT>(blk + (uint32_t)(p * s16 + i16));
+      U[2 * p] = mk<A>(v.x, v.y);
+      U[2 * p + 1] = mk<A>(v.z, v.w);
+    }
+    if constexpr (F::rem == 8) {
+      const f2v v = ldv<f2v, NT>(blk + (uint32_t)(F::nfull * s16 + (uint32_t)i * 8u));
+      U[8] = mk<A>(v.x, v.y);
+    }
+  } else {
+    uint32_t w[F::words];
+#pragma unroll
+    for (int p = 0; p < F::nfull; p++) {
+      const u4v v = ldv<u4v, NT>(blk + (uint32_t)(p * s16 + i16));
+      w[4 * p] = v.x; w[4 * p + 1] = v.y; w[4 * p + 2] = v.z; w[4 * p + 3] = v.w;
+    }
+    if constexpr (F::rem == 8) {
+      const u2v v = ldv<u2v, NT>(blk + (uint32_t)(F::nfull * s16 + (uint32_t)i * 8u));
+      w[4 * F::nfull] = v.x; w[4 * F::nfull + 1] = v.y;
+    } else if constexpr (F::rem == 4) {
+      w[4 * F::nfull] = ldv<unsigned, NT>(blk + (uint32_t)(F::nfull * s16 + (uint32_t)i * 4u));
+    }
+#pragma unroll
+    for (int c = 0; c < RC / 2; c++) {
+      if constexpr (PL == P16) {
+        __half2 h;
+        __builtin_memcpy(&h, &w[c], 4);
+        const float2 f = __half22float2(h);
+        U[c] = mk<A>(f.x, f.y);
+      } else {
+        U[c] = mk<A>(__uint_as_float(w[c] << 16), __uint_as_float(w[c] & 0xffff0000u));
+      }
+    }
+  }
+  if constexpr (RC == 12) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const A a1 = U[(k + 1) % 3], a2 = U[(k + 2) % 3], b1 = U[3 + (k + 1) % 3], b2 = U[3 + (k + 2) % 3];
+      const typename cx<A>::R re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
+      const typename cx<A>::R im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
+      U[6 + k] = mk<A>(sgn * re, -sgn * im);
+    }
+  }
+}
+
+// staggered phase (QEX masks [8,9,11,0], physics/stagD.nim:560-571) of a link
+// in direction mu at site s, times the antiperiodic-t sign of the forward
+// (fwd=1, link leaving s) or backward (fwd=0, link arriving at s) link.
+__device__ __forceinline__ float link_sign(const Geom &g, const Site &s, int mu, int fwd) {
+  int e = 0;
+  if (mu == 0) e = s.t;
+  else if (mu == 1) e = s.x + s.t;
+  else if (mu == 2) e = s.x + s.y + s.t;
+  else e = fwd ? (s.t == g.L[3] - 1) : (s.t == 0);
+  return (e & 1) ? -1.0f : 1.0f;
 }
 
 }  // namespace qh

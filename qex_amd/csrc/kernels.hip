@@ -1,231 +1,12 @@
-// Device kernels of the qexhip library: the one-parity staggered hop (plain and
-// HISQ-form), layout conversion, CG/CGM vector updates with fused FP64
-// reductions, and the fermion-force outer product.
-//
-// The hop computes, for every site x of the output parity p,
-//   H(x) = sum_k [ U_k(x) in(x + l_k mu_k) - U_k(x - l_k mu_k)^+ in(x - l_k mu_k) ]
-// i.e. QEX's (2D) stencil (physics/stagD.nim:383-448; nl=4: l_k=1, mu_k=k;
-// nl=8: k=2mu fat with l=1, k=2mu+1 long with l=3, stagD.nim:40-51), and
-//   out = ca*A + cb*B + chop*H
-// with optional FP64 reductions of the result fused into the epilogue.
+// Device kernels of the qexhip library besides the stencil: layout conversion,
+// vector kernels with fused FP64 reductions, the reliable-update helpers, the
+// multi-shift CG updates and the fermion-force outer product.
 #include <type_traits>
 #include <algorithm>
 #include <stdio.h>
 #include "internal.h"
 
 namespace qh {
-
-// -------------------------------------------------------------- reductions
-__device__ __forceinline__ double ld_coherent(const double *p) {
-  unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)u);
-}
-
-// Block partials -> global partials; the last block to arrive sums all partials
-// in a fixed order (deterministic) and returns true (result valid in thread 0).
-template <int NV>
-__device__ __forceinline__ bool grid_finish(double (&v)[NV], double *lds, const RedWs &red) {
-  __shared__ int last;
-  const int nb = gridDim.x * gridDim.y;
-  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-  block_sum<NV>(v, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) red.partials[(long)k * red.maxblocks + bid] = v[k];
-    __threadfence();
-    unsigned t = atomicAdd(red.counter, 1u);
-    last = (t == (unsigned)(nb - 1));
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double s[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) s[k] = 0.0;
-  for (int j = threadIdx.x; j < nb; j += blockDim.x)
-#pragma unroll
-    for (int k = 0; k < NV; k++) s[k] += ld_coherent(red.partials + (long)k * red.maxblocks + j);
-  block_sum<NV>(s, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = s[k];
-    atomicExch(red.counter, 0u);
-  }
-  return true;
-}
-
-// ---------------------------------------------------------------- epilogues
-// after p.Ap (solvers/cg.nim:206-208): alpha = rz/pAp
-__device__ void epi_alpha(Scal *S, double pAp) {
-  S->pAp = pAp;
-  S->alpha = S->r2 / pAp;
-  S->first = 0;
-  S->pscale = 1.0;
-}
-
-// after r2 = |r|^2 (cg.nim:209-213, loop test :174)
-__device__ void epi_r2(Scal *S, double r2new) {
-  S->r2old = S->r2;
-  S->r2 = r2new;
-  S->its += 1;
-  S->since += 1;
-  S->beta = r2new / S->r2old;
-  if (S->mixed) {
-    S->need_rel = (r2new < S->delta2 * S->maxr2) || (S->since >= S->relmax) ||
-                  (r2new <= S->r2stop_s) || (S->its >= S->maxits);
-  } else {
-    S->done = (S->its >= S->maxits) || (r2new <= S->r2stop);
-  }
-}
-
-// after the FP64 true residual r = b - A x (reliable update)
-__device__ void epi_reliable(Scal *S, double s, double nreal) {
-  S->r2true = s;
-  const double sig_old = S->sigma;
-  const double sig_new = (s > 0.0) ? sqrt(nreal / s) : 1.0;
-  const double r2old_unscaled = S->r2old / (sig_old * sig_old);
-  S->beta = (r2old_unscaled > 0.0) ? s / r2old_unscaled : 0.0;
-  S->pscale = sig_new / sig_old;
-  S->sigma = sig_new;
-  S->r2 = s * sig_new * sig_new;
-  S->r2stop_s = S->r2stop * sig_new * sig_new;
-  S->maxr2 = S->r2;
-  S->since = 0;
-  S->need_rel = 0;
-  S->nrel += 1;
-  S->done = (s <= S->r2stop) || (S->its >= S->maxits);
-}
-
-__device__ __forceinline__ bool skip_pred(int pred, const Scal *S) {
-  if (pred == 1) return (S->done | S->need_rel) != 0;
-  if (pred == 2) return S->need_rel == 0;
-  return false;
-}
-
-// -------------------------------------------------------------------- hop
-template <int PL, class A>
-__device__ __forceinline__ void load_link(const void *base, long off, long stride, A (&U)[9]) {
-#pragma unroll
-  for (int c = 0; c < 9; c++) U[c] = ldc<PL, A>(base, off + c * stride);
-}
-
-template <int PL, int PI, int PO, int PA, class A, int NL, int MODE, int RED>
-__global__ void __launch_bounds__(256) k_hop(HopArgs P) {
-  using R = typename cx<A>::R;
-  __shared__ double lds[16 * 2];
-  if (P.S && skip_pred(P.pred, P.S)) return;
-  const Geom &g = P.g;
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  double red = 0.0;
-  if (i < g.Vh) {
-    const Site s = decode(g, i, P.par);
-    A af[3], ab[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) { af[c] = mk<A>(0, 0); ab[c] = mk<A>(0, 0); }
-#pragma unroll
-    for (int k = 0; k < NL; k++) {
-      const int mu = (NL == 4) ? k : (k >> 1);
-      const int len = (NL == 8 && (k & 1)) ? 3 : 1;
-      const long jf = nbr(g, s, mu, len);
-      const long jb = nbr(g, s, mu, -len);
-      A U[9], v[3];
-      load_link<PL, A>(P.links, (long)(2 * k + P.par) * P.lkblk + i, g.stride, U);
-#pragma unroll
-      for (int c = 0; c < 3; c++) v[c] = ldc<PI, A>(P.in, c * g.stride + jf);
-#pragma unroll
-      for (int r = 0; r < 3; r++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) cmac(af[r], U[3 * r + c], v[c]);
-      load_link<PL, A>(P.links, (long)(2 * k + 1 - P.par) * P.lkblk + jb, g.stride, U);
-#pragma unroll
-      for (int c = 0; c < 3; c++) v[c] = ldc<PI, A>(P.in, c * g.stride + jb);
-#pragma unroll
-      for (int r = 0; r < 3; r++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) cjmac(ab[r], U[3 * c + r], v[c]);
-    }
-    const R chop = (R)P.chop;
-    A res[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) res[c] = mk<A>(chop * (af[c].x - ab[c].x), chop * (af[c].y - ab[c].y));
-    A av[3];
-    if (MODE >= 1) {
-      double ca = P.ca;
-      if (P.ca_m2) ca *= 4.0 * P.S->m2;
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        av[c] = ldc<PA, A>(P.a, c * g.stride + i);
-        res[c].x += (R)ca * av[c].x;
-        res[c].y += (R)ca * av[c].y;
-      }
-    }
-    if (MODE == 2) {
-      const R cb = (R)P.cb;
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        A bv = ldc<PA, A>(P.b, c * g.stride + i);
-        res[c].x += cb * bv.x;
-        res[c].y += cb * bv.y;
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) stc<PO, A>(P.out, c * g.stride + i, res[c]);
-    if (RED == 1) {
-#pragma unroll
-      for (int c = 0; c < 3; c++) red += (double)av[c].x * (double)res[c].x + (double)av[c].y * (double)res[c].y;
-    } else if (RED >= 2) {
-#pragma unroll
-      for (int c = 0; c < 3; c++) red += (double)res[c].x * (double)res[c].x + (double)res[c].y * (double)res[c].y;
-    }
-  }
-  if (RED) {
-    double v[1] = {red};
-    if (grid_finish<1>(v, lds, P.red) && threadIdx.x == 0) {
-      if (RED == 1) epi_alpha(P.S, v[0]);
-      else if (RED == 2) epi_reliable(P.S, v[0], 6.0 * (double)g.Vh);
-      else P.S->tmp[0] = v[0];
-    }
-  }
-}
-
-template <int PL, int PI, int PO, int PA, class A, int NL>
-static void hop_nl(int mode, int redm, const HopArgs &a, hipStream_t st, int block) {
-  dim3 grid((unsigned)((a.g.Vh + block - 1) / block));
-#define QH_HOP(M, R) \
-  if (mode == M && redm == R) { hipLaunchKernelGGL((k_hop<PL, PI, PO, PA, A, NL, M, R>), grid, dim3(block), 0, st, a); return; }
-  QH_HOP(0, 0) QH_HOP(1, 0) QH_HOP(1, 1) QH_HOP(2, 0) QH_HOP(2, 2) QH_HOP(2, 3)
-#undef QH_HOP
-  fprintf(stderr, "qexhip: unsupported hop mode %d/%d\n", mode, redm);
-}
-
-template <int PL, int PI, int PO, int PA, class A>
-static void hop_t(int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block) {
-  if (nl == 4) hop_nl<PL, PI, PO, PA, A, 4>(mode, redm, a, st, block);
-  else hop_nl<PL, PI, PO, PA, A, 8>(mode, redm, a, st, block);
-}
-
-void launch_hop(int PL, int PI, int PO, int PA, bool dbl, int nl, int mode, int redm, const HopArgs &a,
-                hipStream_t st, int block) {
-  const int key = (PL << 12) | (PI << 8) | (PO << 4) | PA;
-#define K4(a_, b_, c_, d_) (((a_) << 12) | ((b_) << 8) | ((c_) << 4) | (d_))
-  if (dbl) {
-    if (key == K4(P64, P64, P64, P64)) return hop_t<P64, P64, P64, P64, double2>(nl, mode, redm, a, st, block);
-  } else {
-    switch (key) {
-      case K4(P32, P32, P32, P32): return hop_t<P32, P32, P32, P32, float2>(nl, mode, redm, a, st, block);
-      case K4(P16, P16, P16, P16): return hop_t<P16, P16, P16, P16, float2>(nl, mode, redm, a, st, block);
-      case K4(P16, P16, P32, P16): return hop_t<P16, P16, P32, P16, float2>(nl, mode, redm, a, st, block);
-      case K4(PBF, PBF, PBF, PBF): return hop_t<PBF, PBF, PBF, PBF, float2>(nl, mode, redm, a, st, block);
-      case K4(PBF, PBF, P32, PBF): return hop_t<PBF, PBF, P32, PBF, float2>(nl, mode, redm, a, st, block);
-      case K4(P16, P32, P32, P32): return hop_t<P16, P32, P32, P32, float2>(nl, mode, redm, a, st, block);
-      case K4(PBF, P32, P32, P32): return hop_t<PBF, P32, P32, P32, float2>(nl, mode, redm, a, st, block);
-      default: break;
-    }
-  }
-#undef K4
-  fprintf(stderr, "qexhip: unsupported hop precision combination %x (dbl=%d)\n", key, (int)dbl);
-}
 
 // ---------------------------------------------------------- layout packing
 template <int PO>
@@ -257,25 +38,6 @@ __global__ void k_unpack_vec(Geom g, const void *in, double2 *canon) {
   }
 }
 
-template <int PO>
-__global__ void k_pack_links(Geom g, int nl, const double2 *canon, void *out) {
-  const long V = 2 * g.Vh;
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
-  if (idx >= V) return;
-  const int par = idx >= g.Vh;
-  const long i = idx - par * g.Vh;
-  using A = typename std::conditional<PO == P64, double2, float2>::type;
-  const long base = (long)(2 * k + par) * 9 * g.stride + i;
-#pragma unroll
-  for (int e = 0; e < 9; e++) {
-    double2 d = canon[((long)k * V + idx) * 9 + e];
-    A v;
-    v.x = d.x; v.y = d.y;
-    stc<PO, A>(out, base + e * g.stride, v);
-  }
-}
-
 static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 void launch_pack_vec(int prec, const Geom &g, const double *canon, void *out, hipStream_t st) {
@@ -297,20 +59,6 @@ void launch_unpack_vec(int prec, const Geom &g, const void *in, double *canon, h
     case P32: hipLaunchKernelGGL(k_unpack_vec<P32>, dim3(nblk(g.Vh, b)), dim3(b), 0, st, g, in, c); break;
     case P16: hipLaunchKernelGGL(k_unpack_vec<P16>, dim3(nblk(g.Vh, b)), dim3(b), 0, st, g, in, c); break;
     case PBF: hipLaunchKernelGGL(k_unpack_vec<PBF>, dim3(nblk(g.Vh, b)), dim3(b), 0, st, g, in, c); break;
-  }
-}
-
-void launch_pack_links(int prec, int recon, const Geom &g, int nl, const double *canon, void *out,
-                       hipStream_t st) {
-  (void)recon;
-  const int b = 256;
-  dim3 grid(nblk(2 * g.Vh, b), nl);
-  const double2 *c = (const double2 *)canon;
-  switch (prec) {
-    case P64: hipLaunchKernelGGL(k_pack_links<P64>, grid, dim3(b), 0, st, g, nl, c, out); break;
-    case P32: hipLaunchKernelGGL(k_pack_links<P32>, grid, dim3(b), 0, st, g, nl, c, out); break;
-    case P16: hipLaunchKernelGGL(k_pack_links<P16>, grid, dim3(b), 0, st, g, nl, c, out); break;
-    case PBF: hipLaunchKernelGGL(k_pack_links<PBF>, grid, dim3(b), 0, st, g, nl, c, out); break;
   }
 }
 
@@ -387,80 +135,6 @@ void launch_redot(int prec, const Geom &g, const void *x, const void *y, Scal *S
   }
 }
 
-// p = r + beta*p (cg.nim:186-193; first iteration p = r)
-template <int PP, int PR, class A>
-__global__ void k_pupdate(Geom g, void *p, const void *r, Scal *S) {
-  if ((S->done | S->need_rel) != 0) return;
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.Vh) return;
-  using R = typename cx<A>::R;
-  if (S->first) {
-#pragma unroll
-    for (int c = 0; c < 3; c++) stc<PP, A>(p, c * g.stride + i, ldc<PR, A>(r, c * g.stride + i));
-    return;
-  }
-  const R beta = (R)(S->beta * S->pscale);
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    A rv = ldc<PR, A>(r, c * g.stride + i), pv = ldc<PP, A>(p, c * g.stride + i);
-    pv.x = rv.x + beta * pv.x;
-    pv.y = rv.y + beta * pv.y;
-    stc<PP, A>(p, c * g.stride + i, pv);
-  }
-}
-
-// x += alpha p ; r -= alpha Ap ; r2 = |r|^2  (cg.nim:208-213)
-template <int PX, int PP, int PR, int PAp, class A>
-__global__ void __launch_bounds__(256) k_xr(Geom g, void *x, const void *p, void *r, const void *Ap, Scal *S,
-                                            RedWs red) {
-  __shared__ double lds[16];
-  if ((S->done | S->need_rel) != 0) return;
-  using R = typename cx<A>::R;
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  double s = 0.0;
-  if (i < g.Vh) {
-    const R alpha = (R)S->alpha;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const long o = c * g.stride + i;
-      A pv = ldc<PP, A>(p, o), xv = ldc<PX, A>(x, o), rv = ldc<PR, A>(r, o), av = ldc<PAp, A>(Ap, o);
-      xv.x += alpha * pv.x; xv.y += alpha * pv.y;
-      rv.x -= alpha * av.x; rv.y -= alpha * av.y;
-      stc<PX, A>(x, o, xv);
-      stc<PR, A>(r, o, rv);
-      s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
-    }
-  }
-  double v[1] = {s};
-  if (grid_finish<1>(v, lds, red) && threadIdx.x == 0) epi_r2(S, v[0]);
-}
-
-void launch_pupdate(int PP, int PR, const Geom &g, void *p, const void *r, Scal *S, hipStream_t st) {
-  dim3 grid(nblk(g.Vh, 256)), blk(256);
-  if (PP == P64 && PR == P64) { hipLaunchKernelGGL((k_pupdate<P64, P64, double2>), grid, blk, 0, st, g, p, r, S); return; }
-  if (PR != P32) { fprintf(stderr, "qexhip: pupdate r precision\n"); return; }
-  switch (PP) {
-    case P32: hipLaunchKernelGGL((k_pupdate<P32, P32, float2>), grid, blk, 0, st, g, p, r, S); break;
-    case P16: hipLaunchKernelGGL((k_pupdate<P16, P32, float2>), grid, blk, 0, st, g, p, r, S); break;
-    case PBF: hipLaunchKernelGGL((k_pupdate<PBF, P32, float2>), grid, blk, 0, st, g, p, r, S); break;
-  }
-}
-
-void launch_xr(int PX, int PP, int PR, int PAp, const Geom &g, void *x, const void *p, void *r, const void *Ap,
-               Scal *S, RedWs red, hipStream_t st) {
-  dim3 grid(nblk(g.Vh, 256)), blk(256);
-  if (PX == P64 && PP == P64) {
-    hipLaunchKernelGGL((k_xr<P64, P64, P64, P64, double2>), grid, blk, 0, st, g, x, p, r, Ap, S, red);
-    return;
-  }
-  (void)PX; (void)PR; (void)PAp;
-  switch (PP) {
-    case P32: hipLaunchKernelGGL((k_xr<P32, P32, P32, P32, float2>), grid, blk, 0, st, g, x, p, r, Ap, S, red); break;
-    case P16: hipLaunchKernelGGL((k_xr<P32, P16, P32, P32, float2>), grid, blk, 0, st, g, x, p, r, Ap, S, red); break;
-    case PBF: hipLaunchKernelGGL((k_xr<P32, PBF, P32, P32, float2>), grid, blk, 0, st, g, x, p, r, Ap, S, red); break;
-  }
-}
-
 // reliable update, part 1: x += xs/sigma ; xs = 0   (FP64 solution accumulation)
 __global__ void k_xfold(Geom g, double2 *x, float2 *xs, Scal *S) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -491,8 +165,7 @@ __global__ void k_rs_from_true(Geom g, float2 *rs, const double2 *rt, Scal *S) {
   }
 }
 
-void launch_rs_from_true(int PR, const Geom &g, void *rs, const double *rtrue, Scal *S, hipStream_t st) {
-  (void)PR;
+void launch_rs_from_true(const Geom &g, void *rs, const double *rtrue, Scal *S, hipStream_t st) {
   hipLaunchKernelGGL(k_rs_from_true, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, (float2 *)rs,
                      (const double2 *)rtrue, S);
 }
@@ -505,7 +178,7 @@ __global__ void k_scal_init(Scal *S, int mode) {
   if (threadIdx.x != 0) return;
   if (mode == 0) {  // start of a CG solve: fresh state (cg.nim:21-27)
     S->its = 0; S->since = 0; S->done = 0; S->need_rel = 0; S->first = 1; S->nrel = 0;
-    S->r2old = 1.0; S->beta = 0.0; S->alpha = 0.0; S->pscale = 1.0; S->sigma = 1.0;
+    S->r2prev = 1.0; S->beta = 0.0; S->alpha = 0.0; S->pscale = 1.0; S->sigma = 1.0;
   } else if (mode == 1) {  // force a reliable update now
     S->need_rel = 1;
   }
@@ -627,7 +300,7 @@ void launch_cgm_p(const Geom &g, int nshift, const double *r, double **p, ShiftS
 // F_mu(x) = TAH(eps(x) psi(x) psi(x+mu)^+ U_mu(x)^+)   (hmc/hmcAction.nim:735-790,
 // unsmeared; TAH = maths/matrixFunctions.nim:414-419).  One thread per
 // (canonical site, mu); output in QEX canonical order [mu][site][3][3].
-__global__ void __launch_bounds__(256) k_force(Geom g, const double2 *links, long lkblk, const double2 *pe,
+__global__ void __launch_bounds__(256) k_force(Geom g, const char *links, long lkblk, const double2 *pe,
                                                const double2 *po, double2 *F, int outer_only) {
   const long V = 2 * g.Vh;
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -659,7 +332,7 @@ __global__ void __launch_bounds__(256) k_force(Geom g, const double2 *links, lon
     return;
   }
   double2 U[9];
-  load_link<P64, double2>(links, (long)(2 * mu + par) * lkblk + i, g.stride, U);
+  ld_link<P64, 18, false, double2>(links + (long)(2 * mu + par) * lkblk, g.stride, i, U, 1.0);
   double2 T[9];
 #pragma unroll
   for (int r = 0; r < 3; r++)
@@ -690,10 +363,10 @@ __global__ void __launch_bounds__(256) k_force(Geom g, const double2 *links, lon
   for (int e = 0; e < 9; e++) out[e] = Rm[e];
 }
 
-void launch_force(const Geom &g, const double *links64, long lkblk, const double *psi_e, const double *psi_o,
+void launch_force(const Geom &g, const char *links64, long lkblk, const double *psi_e, const double *psi_o,
                   double *F, int outer_only, hipStream_t st) {
   dim3 grid(nblk(2 * g.Vh, 256), 4);
-  hipLaunchKernelGGL(k_force, grid, dim3(256), 0, st, g, (const double2 *)links64, lkblk, (const double2 *)psi_e,
+  hipLaunchKernelGGL(k_force, grid, dim3(256), 0, st, g, links64, lkblk, (const double2 *)psi_e,
                      (const double2 *)psi_o, (double2 *)F, outer_only);
 }
 

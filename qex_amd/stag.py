@@ -252,6 +252,7 @@ class SolverParams:
         self.backend = SolverBackend.sbHip
         self.sloppySolve = SloppyType.SloppyNone
         self.sloppyLinks = None  # None: same storage as the spinors
+        self.recon = 18          # 12: 12-real links in the inner operator (plain SU(3) links)
         self.usePrevSoln = False
         self.verbosity = 1
         self.subsetName = "all"
@@ -280,6 +281,7 @@ class SolverParams:
         c.maxits = self.maxits
         c.sloppy = _SLOPPY_PREC[SloppyType(self.sloppySolve)]
         c.sloppy_links = -1 if self.sloppyLinks is None else _SLOPPY_PREC[SloppyType(self.sloppyLinks)]
+        c.recon = int(self.recon)
         c.reliable_delta = self.reliableDelta
         c.reliable_max_its = self.reliableMaxIts
         c.use_prev_soln = int(self.usePrevSoln)
@@ -361,8 +363,15 @@ def fermForce(s, psi, out=None, outer_only=False, device=None):
     return out
 
 
-def time_hop(s, prec=FP64, nrep=100):
+def time_hop(s, prec=FP64, nrep=100, recon=18):
     """Average ms of one one-parity hop (HIP events on the library stream)."""
     v = ctypes.c_double()
-    call("qexhip_time_hop", s.h, prec, nrep, ctypes.byref(v))
+    call("qexhip_time_hop", s.h, prec, recon, nrep, ctypes.byref(v))
     return v.value
+
+
+def recon_ok(s):
+    """True when the links are phased SU(3) matrices (12-real storage exact)."""
+    v = ctypes.c_int()
+    call("qexhip_links_recon_ok", s.h, ctypes.byref(v))
+    return bool(v.value)

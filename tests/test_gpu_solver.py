@@ -158,3 +158,57 @@ def test_errors_are_reported(gpu):
     sp.backend = gpu.SolverBackend.sbQuda
     with pytest.raises(QexHipError):
         gpu.solveEE(s, f64, f64, 0.1, sp)
+
+
+@pytest.mark.parametrize("sloppy,recon", [(0, 12), (1, 12), (2, 12), (2, 18)])
+def test_recon12_and_precisions(oracle, gpu, sloppy, recon):
+    """12-real links (row 2 rebuilt from rows 0-1 with the phase/BC sign) in
+    the inner operator: FP64 CG keeps the oracle's iteration count."""
+    O = oracle
+    L = (8, 8, 8, 8)
+    m = 0.1
+    g = links(O, L, False)
+    b = O.source_gauss(L, 29, 0, 0)
+    want, its_o, _ = O.solveXX(L, g, 0, b, m, 1e-20)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    assert gpu.stag.recon_ok(s)
+    fb = lo.ColorVector().set(b)
+    fx = lo.ColorVector()
+    sp = gpu.SolverParams()
+    sp.r2req, sp.verbosity, sp.sloppySolve, sp.recon = 1e-20, 0, sloppy, recon
+    gpu.solveEE(s, fx, fb, m, sp)
+    assert sp.r2 <= 1e-20
+    if sloppy == 0:
+        assert abs(sp.iterations - its_o) <= 1
+    assert rel(fx.get(sub=0)[: lo.nEven], want[: lo.nEven]) < 1e-8
+
+
+def test_recon_refused_for_hisq_and_unphased(oracle, gpu):
+    from qex_amd import QexHipError
+    O = oracle
+    L = (8, 8, 8, 8)
+    U = O.phased(L, O.gauge_random(L, SEED))
+    lo = gpu.Layout(L)
+    s8 = gpu.Staggered(lo, O.hisq_links(L, SEED + 1, U))
+    assert not gpu.stag.recon_ok(s8)
+    s4 = gpu.Staggered(lo, O.gauge_random(L, SEED))  # no phases: sign check fails
+    assert not gpu.stag.recon_ok(s4)
+    fb = lo.ColorVector().set(O.source_gauss(L, 3, 0, 0))
+    fx = lo.ColorVector()
+    sp = gpu.SolverParams()
+    sp.recon, sp.verbosity = 12, 0
+    with pytest.raises(QexHipError):
+        gpu.solveEE(s4, fx, fb, 0.1, sp)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+@pytest.mark.parametrize("recon", [18, 12])
+def test_time_hop_runs(gpu, prec, recon):
+    from qex_amd import gauge
+    lo = gpu.Layout((16, 16, 16, 16))
+    U = gauge.random_gauge(lo, 3)
+    gauge.phase_bc(lo, U)
+    s = gpu.Staggered(lo, U)
+    ms = gpu.time_hop(s, prec, 5, recon=recon)
+    assert 0 < ms < 10
