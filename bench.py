@@ -31,8 +31,8 @@ SEED = 20240601
 
 # compulsory reals per output site of one plain one-parity hop: 8 links x 18 + 2 x 6
 HOP_REALS = 8 * 18 + 2 * 6
-BYTES_PER_REAL = {0: 8, 1: 4, 2: 2, 3: 2}
-PREC_NAME = {0: "fp64", 1: "fp32", 2: "fp16", 3: "bf16"}
+BYTES_PER_REAL = {0: 8, 1: 4, 2: 2, 3: 2, 4: 2}
+PREC_NAME = {0: "fp64", 1: "fp32", 2: "fp16", 3: "bf16", 4: "q16links_fp16vec"}
 # flops per output site of one hop: 8 complex 3x3 mat-vec (66) + 7*6 adds = 570
 HOP_FLOPS = 570
 
@@ -182,7 +182,7 @@ def main():
                 del U2
             key = "x".join(map(str, L))
             table[key] = {}
-            for prec in (0, 1, 2, 3):
+            for prec in (0, 1, 2, 3, 4):
                 Q.time_hop(s2, prec, 5)
                 n = 200 if L[0] <= 32 else 60
                 m_ = Q.time_hop(s2, prec, n)
@@ -204,6 +204,9 @@ def main():
                     ("mixed_fp32", S.SloppySingle, None, 18), ("mixed_fp32_r12", S.SloppySingle, None, 12),
                     ("mixed_fp16", S.SloppyHalf, None, 18), ("mixed_fp16_r12", S.SloppyHalf, None, 12),
                     ("mixed_fp16links_fp32vec_r12", S.SloppySingle, S.SloppyHalf, 12),
+                    ("mixed_q16links_fp16vec", S.SloppyHalf, S.SloppyHalfFixed, 18),
+                    ("mixed_q16links_fp16vec_r12", S.SloppyHalf, S.SloppyHalfFixed, 12),
+                    ("mixed_q16links_fp32vec_r12", S.SloppySingle, S.SloppyHalfFixed, 12),
                     ("mixed_bf16", S.SloppyBF16, None, 18),
                     ("mixed_bf16links_fp32vec_r12", S.SloppySingle, S.SloppyBF16, 12)]
         if args.cg_only:

@@ -39,8 +39,10 @@ typedef struct qexhip_lattice qexhip_lattice;
 typedef struct qexhip_links qexhip_links;
 typedef struct qexhip_field qexhip_field;
 
-/* storage precisions */
-enum { QEXHIP_FP64 = 0, QEXHIP_FP32 = 1, QEXHIP_FP16 = 2, QEXHIP_BF16 = 3 };
+/* storage precisions.  QEXHIP_Q16 (links only): 16-bit fixed point with one
+   scale per link field, value = int16 * max|entry|/32767; used with FP16
+   color vectors. */
+enum { QEXHIP_FP64 = 0, QEXHIP_FP32 = 1, QEXHIP_FP16 = 2, QEXHIP_BF16 = 3, QEXHIP_Q16 = 4 };
 enum { QEXHIP_EVEN = 0, QEXHIP_ODD = 1, QEXHIP_ALL = 2 };
 
 /* Solver parameters: inputs and outputs mirror QEX's SolverParams
@@ -52,7 +54,8 @@ typedef struct {
   int sloppy;             /* QEXHIP_FP64: plain FP64 CG (SloppyNone); FP32/FP16/BF16:
                              FP32-arithmetic inner CG with this spinor/link storage and
                              FP64 reliable updates (QEX sloppySolve, solverBase.nim:8-9) */
-  int sloppy_links;       /* link storage of the inner operator; -1 = same as sloppy */
+  int sloppy_links;       /* link storage of the inner operator; -1 = same as sloppy;
+                             QEXHIP_Q16 needs sloppy = FP16 or FP32 */
   int recon;              /* inner-operator links: 18 reals, or 12 (rows 0-1 stored, row 2
                              rebuilt; plain phased-SU(3) links only); 0 = 18 */
   double reliable_delta;  /* reliable update when |r| < delta*|r_last_update| (0.1) */
@@ -146,8 +149,9 @@ int qexhip_gen_hisq(qexhip_lattice *lat, uint64_t seed, const double *Uph_dev, d
 
 /* ---- measurement (the in-module bench of stagD.nim:767-810) ----
    average duration (ms, HIP events on the library stream) of one one-parity
-   hop r_e = (2D)_eo x_o with links and fields stored in precision prec and
-   links of recon (18 or 12) reals. */
+   hop r_e = (2D)_eo x_o with links and fields stored in precision prec
+   (QEXHIP_Q16: fixed-point links, FP16 fields) and links of recon (18 or 12)
+   reals. */
 int qexhip_time_hop(qexhip_links *l, int prec, int recon, int nrep, double *ms_per_hop);
 /* whether the links are phased SU(3) matrices, so 12-real storage is exact
    (1) or not (0) */

@@ -10,7 +10,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("QEXHIP_LIB") or os.path.join(_HERE, "libqexhip.so")
 
-FP64, FP32, FP16, BF16 = 0, 1, 2, 3
+FP64, FP32, FP16, BF16, Q16 = 0, 1, 2, 3, 4
 EVEN, ODD, ALL = 0, 1, 2
 
 # every symbol include/qexhip.h declares

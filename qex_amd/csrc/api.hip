@@ -104,8 +104,10 @@ struct qexhip_links {
   qexhip_lattice *lat;
   int nl;
   double *canon = nullptr;             // device canonical FP64 copy, nl*V*18
-  char *st[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+  char *st[5][2] = {};                 // [precision][12-real?]
   int recon_ok = -1;                   // -1 unknown, 0 no, 1 phased SU(3)
+  float qs[8] = {};                    // PQ16 scale/32767 per link field
+  bool qs_ok = false;
   Work *ws = nullptr;
 };
 
@@ -133,15 +135,35 @@ bool recon_allowed(qexhip_links *l) {
 
 long lkblk_bytes(const qexhip_links *l, int prec, int rc) { return link_block_bytes(prec, rc, l->lat->g.stride); }
 
+void link_scales(qexhip_links *l) {
+  if (l->qs_ok) return;
+  unsigned *d = dalloc<unsigned>(64);
+  launch_fill(d, 64, g_stream);
+  launch_link_maxabs(l->lat->g, l->nl, l->canon, d, g_stream);
+  check_launch();
+  unsigned bits[8] = {};
+  HIPCHK(hipMemcpyAsync(bits, d, sizeof(unsigned) * l->nl, hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  (void)hipFree(d);
+  for (int k = 0; k < l->nl; k++) {
+    float m;
+    memcpy(&m, &bits[k], 4);
+    l->qs[k] = m / 32767.0f;
+  }
+  l->qs_ok = true;
+}
+
 const char *links_store(qexhip_links *l, int prec, int rc = 18) {
   require(rc == 18 || rc == 12, "recon must be 18 or 12");
+  require(prec >= 0 && prec <= PQ16, "bad link precision");
+  if (prec == PQ16) link_scales(l);
   if (rc == 12) require(recon_allowed(l), "12-real links need plain (nl=4) phased SU(3) links");
   const int ri = (rc == 12);
   if (!l->st[prec][ri]) {
     const size_t bytes = (size_t)l->nl * 2 * lkblk_bytes(l, prec, rc);
     l->st[prec][ri] = dalloc<char>(bytes);
     launch_fill(l->st[prec][ri], bytes, g_stream);
-    launch_pack_links(prec, rc, l->lat->g, l->nl, l->canon, l->st[prec][ri], g_stream);
+    launch_pack_links(prec, rc, l->lat->g, l->nl, l->canon, l->st[prec][ri], l->qs, g_stream);
     check_launch();
   }
   return l->st[prec][ri];
@@ -195,6 +217,7 @@ void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const v
   h.g = l->lat->g;
   h.links = links_store(l, prec, rc);
   h.lkblk = lkblk_bytes(l, prec, rc);
+  for (int k = 0; k < 8; k++) h.qs[k] = l->qs[k];
   h.in = in;
   h.out = out;
   h.a = a;
@@ -208,7 +231,8 @@ void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const v
   h.remap = g_remap;
   h.S = S;
   h.red = l->lat->red;
-  HopKey k{prec, rc, prec, prec, prec};
+  const int pv = (prec == PQ16) ? P16 : prec;  // 16-bit fixed-point links go with FP16 vectors
+  HopKey k{prec, rc, pv, pv, pv};
   launch_hop(k, l->nl, mode, redm, h, g_stream, g_block);
 }
 
@@ -266,6 +290,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   a.g = g;
   a.links = links_store(l, pl, rc);
   a.lkblk = lkblk_bytes(l, pl, rc);
+  for (int k = 0; k < 8; k++) a.qs[k] = l->qs[k];
   a.par = par;
   a.remap = g_remap;
   a.S = S;
@@ -514,8 +539,6 @@ int qexhip_lattice_create(const int dims[4], qexhip_lattice **out) {
   lat->g.stride = (lat->g.Vh + 255) / 256 * 256;
   lat->red.maxblocks = (int)((lat->g.stride + 63) / 64) + 64;
   lat->red.partials = dalloc<double>(sizeof(double) * 4 * lat->red.maxblocks);
-  lat->red.counter = dalloc<unsigned>(256);
-  launch_fill(lat->red.counter, 256, g_stream);
   lat->S = dalloc<Scal>(sizeof(Scal));
   HIPCHK(hipHostMalloc((void **)&lat->Sh, sizeof(Scal), hipHostMallocDefault));
   memset(lat->Sh, 0, sizeof(Scal));
@@ -531,7 +554,6 @@ int qexhip_lattice_destroy(qexhip_lattice *lat) {
   if (!lat) return 0;
   HIPCHK(hipStreamSynchronize(g_stream));
   (void)hipFree(lat->red.partials);
-  (void)hipFree(lat->red.counter);
   (void)hipFree(lat->S);
   (void)hipHostFree(lat->Sh);
   (void)hipFree(lat->stage);
@@ -567,7 +589,9 @@ int qexhip_links_update(qexhip_links *l, const double *canon, int on_device) {
   HIPCHK(hipMemcpyAsync(l->canon, canon, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         g_stream));
   l->recon_ok = -1;
-  for (int p = 0; p < 4; p++)
+  l->qs_ok = false;
+  if (l->st[PQ16][0] || l->st[PQ16][1]) link_scales(l);
+  for (int p = 0; p <= PQ16; p++)
     for (int ri = 0; ri < 2; ri++) {
       if (!l->st[p][ri]) continue;
       if (ri == 1 && !recon_allowed(l)) {
@@ -575,7 +599,7 @@ int qexhip_links_update(qexhip_links *l, const double *canon, int on_device) {
         l->st[p][ri] = nullptr;
         continue;
       }
-      launch_pack_links(p, ri ? 12 : 18, l->lat->g, l->nl, l->canon, l->st[p][ri], g_stream);
+      launch_pack_links(p, ri ? 12 : 18, l->lat->g, l->nl, l->canon, l->st[p][ri], l->qs, g_stream);
     }
   check_launch();
   HIPCHK(hipStreamSynchronize(g_stream));
@@ -587,7 +611,7 @@ int qexhip_links_destroy(qexhip_links *l) {
   if (!l) return 0;
   HIPCHK(hipStreamSynchronize(g_stream));
   (void)hipFree(l->canon);
-  for (int p = 0; p < 4; p++)
+  for (int p = 0; p <= PQ16; p++)
     for (int ri = 0; ri < 2; ri++)
       if (l->st[p][ri]) (void)hipFree(l->st[p][ri]);
   free_work(l->ws);
@@ -735,12 +759,13 @@ int qexhip_solve_xx(qexhip_links *l, int par, qexhip_field *r, const qexhip_fiel
   require(par == 0 || par == 1, "solve parity must be 0 (even) or 1 (odd)");
   require(r->prec == P64 && x->prec == P64, "solver fields must be FP64");
   require(sp->sloppy >= 0 && sp->sloppy <= 3, "bad sloppy precision");
-  require(sp->sloppy_links >= -1 && sp->sloppy_links <= 3, "bad sloppy link precision");
+  require(sp->sloppy_links >= -1 && sp->sloppy_links <= PQ16, "bad sloppy link precision");
   require(sp->recon == 0 || sp->recon == 18 || sp->recon == 12, "recon must be 0, 18 or 12");
   if (sp->sloppy != P64) {
     const int sl = sp->sloppy_links < 0 ? sp->sloppy : sp->sloppy_links;
     require(sl != P64, "mixed solve needs sloppy links below FP64");
-    require(sl == sp->sloppy || sp->sloppy == P32, "sloppy link/spinor storage combination not supported");
+    require(sl == sp->sloppy || sp->sloppy == P32 || (sl == PQ16 && sp->sloppy == P16),
+            "sloppy link/spinor storage combination not supported");
   }
   if (sp->recon == 12) require(recon_allowed(l), "12-real links need plain (nl=4) phased SU(3) links");
   solve_xx(l, par, (double *)r->par(par), (const double *)x->par(par), m, sp);
@@ -882,10 +907,11 @@ int qexhip_time_hop(qexhip_links *l, int prec, int recon, int nrep, double *ms) 
   API_BEGIN
   check_init();
   require(l && ms && nrep > 0, "bad arguments");
-  require(prec >= 0 && prec <= 3, "bad precision");
+  require(prec >= 0 && prec <= PQ16, "bad precision");
   qexhip_lattice *lat = l->lat;
   qexhip_field *x = nullptr, *r = nullptr;
-  if (qexhip_field_create(lat, prec, &x) || qexhip_field_create(lat, prec, &r)) throw std::runtime_error(g_err);
+  const int pv = (prec == PQ16) ? P16 : prec;
+  if (qexhip_field_create(lat, pv, &x) || qexhip_field_create(lat, pv, &r)) throw std::runtime_error(g_err);
   launch_source_gauss(lat->g, 99, 0, 2, lat->stage, g_stream);
   import_field(x, 2, lat->stage, 1);
   const int rc = (recon == 12) ? 12 : 18;

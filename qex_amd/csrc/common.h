@@ -20,7 +20,11 @@
 namespace qh {
 
 // ------------------------------------------------------------ precision tags
-enum Prec : int { P64 = 0, P32 = 1, P16 = 2, PBF = 3 };
+// P64/P32/P16/PBF: IEEE double, single, half, bfloat16.  PQ16 (links only):
+// 16-bit fixed point, value = int16 * scale/32767 with one scale per link
+// field (the largest |re|,|im| of that field), i.e. absolute precision
+// scale/32767 instead of FP16's 11-bit relative precision.
+enum Prec : int { P64 = 0, P32 = 1, P16 = 2, PBF = 3, PQ16 = 4 };
 
 struct c64 { double re, im; };
 struct c32 { float re, im; };
@@ -195,47 +199,20 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* 16*NV 
 // Reduction workspace shared by consecutive kernels on one stream.
 struct RedWs {
   double *partials;   // [4][maxblocks]
-  unsigned *counter;
   int maxblocks;
 };
 
-__device__ __forceinline__ double ld_coherent(const double *p) {
-  unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)u);
-}
-
-// Block partials -> global partials; the last block to arrive sums all partials
-// in a fixed order (deterministic) and returns true (result valid in thread 0).
+// Block partial sums of a reduction: the block's total of each of the NV
+// values goes to partials[k*maxblocks + block]; a one-block finish kernel
+// (reduce.hip) then adds the partials in a fixed order (deterministic, and no
+// grid-wide fence in the producing kernel).
 template <int NV>
-__device__ __forceinline__ bool grid_finish(double (&v)[NV], double *lds, const RedWs &red) {
-  __shared__ int last;
-  const int nb = gridDim.x * gridDim.y;
+__device__ __forceinline__ void block_partials(double (&v)[NV], double *lds, const RedWs &red) {
   const int bid = blockIdx.y * gridDim.x + blockIdx.x;
   block_sum<NV>(v, lds);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
     for (int k = 0; k < NV; k++) red.partials[(long)k * red.maxblocks + bid] = v[k];
-    __threadfence();
-    unsigned t = atomicAdd(red.counter, 1u);
-    last = (t == (unsigned)(nb - 1));
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double s[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) s[k] = 0.0;
-  for (int j = threadIdx.x; j < nb; j += blockDim.x)
-#pragma unroll
-    for (int k = 0; k < NV; k++) s[k] += ld_coherent(red.partials + (long)k * red.maxblocks + j);
-  block_sum<NV>(s, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = s[k];
-    atomicExch(red.counter, 0u);
-  }
-  return true;
 }
 
 // XCD-aware block order: the hardware deals workgroups round-robin over the 8
@@ -300,7 +277,8 @@ template <class T, bool NT> __device__ __forceinline__ T ldv(const char *p) {
 // Load one link (LFmt layout, see above) straight into 3x3 complex form,
 // rebuilding row 2 for 12-real storage: row2_k = sgn*conj(row0 x row1)_k.
 template <int PL, int RC, bool NT, class A>
-__device__ __forceinline__ void ld_link(const char *blk, long stride, long i, A (&U)[9], typename cx<A>::R sgn) {
+__device__ __forceinline__ void ld_link(const char *blk, long stride, long i, A (&U)[9], typename cx<A>::R sgn,
+                                        float qscale = 0.0f) {
   using F = LFmt<PL, RC>;
   const uint32_t s16 = (uint32_t)stride * 16u, i16 = (uint32_t)i * 16u;
   if constexpr (PL == P64) {
@@ -340,6 +318,8 @@ __device__ __forceinline__ void ld_link(const char *blk, long stride, long i, A 
         __builtin_memcpy(&h, &w[c], 4);
         const float2 f = __half22float2(h);
         U[c] = mk<A>(f.x, f.y);
+      } else if constexpr (PL == PQ16) {
+        U[c] = mk<A>((float)(int16_t)(w[c] & 0xffffu) * qscale, (float)(int16_t)(w[c] >> 16) * qscale);
       } else {
         U[c] = mk<A>(__uint_as_float(w[c] << 16), __uint_as_float(w[c] & 0xffff0000u));
       }

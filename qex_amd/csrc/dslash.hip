@@ -25,14 +25,14 @@ template <class A> struct Leg {
 };
 
 template <int PL, int RC, int NL, class A, class LD>
-__device__ __forceinline__ void leg_load(const Geom &g, const char *links, long lkblk, int par, long i, const Site &s,
-                                         LD &ld, int j, Leg<A> &L) {
+__device__ __forceinline__ void leg_load(const Geom &g, const char *links, long lkblk, const float *qs, int par,
+                                         long i, const Site &s, LD &ld, int j, Leg<A> &L) {
   const int k = j >> 1, fwd = !(j & 1);
   const int mu = (NL == 4) ? k : (k >> 1);
   const int len = (NL == 8 && (k & 1)) ? 3 : 1;
   const long n = nbr(g, s, mu, fwd ? len : -len);
   ld_link<PL, RC, true, A>(links + (long)(2 * k + (fwd ? par : 1 - par)) * lkblk, g.stride, fwd ? i : n, L.U,
-                           (RT<A>)(RC == 12 ? link_sign(g, s, mu, fwd) : 1.0f));
+                           (RT<A>)(RC == 12 ? link_sign(g, s, mu, fwd) : 1.0f), PL == PQ16 ? qs[k] : 0.0f);
   ld(n, L.v);
 }
 
@@ -61,8 +61,8 @@ __device__ __forceinline__ void mem_fence_compiler() { asm volatile("" ::: "memo
 // compiler barrier after each leg).  QH_PIPE = -1 / -2: a rolled loop over
 // hops / links, which bounds the registers a wave holds.
 template <int PL, int RC, int NL, class A, class LD>
-__device__ __forceinline__ void stencil(const Geom &g, const char *links, long lkblk, int par, long i,
-                                        const Site &s, LD &&ld, A (&res)[3]) {
+__device__ __forceinline__ void stencil(const Geom &g, const char *links, long lkblk, const float *qs, int par,
+                                        long i, const Site &s, LD &&ld, A (&res)[3]) {
   constexpr int NJ = 2 * NL;
   constexpr int D = QH_PIPE;
   A af[3], ab[3];
@@ -72,8 +72,8 @@ __device__ __forceinline__ void stencil(const Geom &g, const char *links, long l
 #pragma unroll 1
     for (int j = 0; j < NJ; j += 2) {
       Leg<A> L0, L1;
-      leg_load<PL, RC, NL, A>(g, links, lkblk, par, i, s, ld, j, L0);
-      leg_load<PL, RC, NL, A>(g, links, lkblk, par, i, s, ld, j + 1, L1);
+      leg_load<PL, RC, NL, A>(g, links, lkblk, qs, par, i, s, ld, j, L0);
+      leg_load<PL, RC, NL, A>(g, links, lkblk, qs, par, i, s, ld, j + 1, L1);
       leg_fma<A>(j, L0, af, ab);
       leg_fma<A>(j + 1, L1, af, ab);
     }
@@ -81,17 +81,17 @@ __device__ __forceinline__ void stencil(const Geom &g, const char *links, long l
 #pragma unroll 1
     for (int j = 0; j < NJ; j++) {
       Leg<A> L0;
-      leg_load<PL, RC, NL, A>(g, links, lkblk, par, i, s, ld, j, L0);
+      leg_load<PL, RC, NL, A>(g, links, lkblk, qs, par, i, s, ld, j, L0);
       leg_fma<A>(j, L0, af, ab);
     }
   } else {
     constexpr int DD = (D < 0) ? 0 : D;
     Leg<A> L[DD + 1];
 #pragma unroll
-    for (int j = 0; j < DD; j++) leg_load<PL, RC, NL, A>(g, links, lkblk, par, i, s, ld, j, L[j]);
+    for (int j = 0; j < DD; j++) leg_load<PL, RC, NL, A>(g, links, lkblk, qs, par, i, s, ld, j, L[j]);
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-      if (j + DD < NJ) leg_load<PL, RC, NL, A>(g, links, lkblk, par, i, s, ld, j + DD, L[(j + DD) % (DD + 1)]);
+      if (j + DD < NJ) leg_load<PL, RC, NL, A>(g, links, lkblk, qs, par, i, s, ld, j + DD, L[(j + DD) % (DD + 1)]);
       leg_fma<A>(j, L[j % (DD + 1)], af, ab);
       mem_fence_compiler();
     }
@@ -112,59 +112,6 @@ template <int P, class A> __device__ __forceinline__ A rt_store(A v) {
   return v;
 }
 
-// ------------------------------------------------------------- epilogues
-__device__ void epi_reliable(Scal *S, double s, double nreal) {
-  // FP64 true residual r = b - A x just computed: restart the sloppy
-  // recursion on sigma*r with beta = |r_true|^2/|r_prev|^2 (unscaled).
-  S->r2true = s;
-  const double sig_old = S->sigma;
-  const double sig_new = (s > 0.0) ? sqrt(nreal / s) : 1.0;
-  const double r2old_unscaled = S->r2prev / (sig_old * sig_old);
-  S->beta = (r2old_unscaled > 0.0) ? s / r2old_unscaled : 0.0;
-  S->pscale = sig_new / sig_old;
-  S->sigma = sig_new;
-  S->r2 = s * sig_new * sig_new;
-  S->r2stop_s = S->r2stop * sig_new * sig_new;
-  S->maxr2 = S->r2;
-  S->alpha = 0.0;
-  S->since = 0;
-  S->need_rel = 0;
-  S->nrel += 1;
-  S->done = (s <= S->r2stop) || (S->its >= S->maxits);
-}
-
-// After iteration n of the fused CG (solvers/cg.nim:174-214 recurrence):
-// rr = |r_{n-1}|^2 exact, pAp = <p_n,Ap_n>, rAp = <r_{n-1},Ap_n>, ApAp = |Ap_n|^2.
-//   alpha_n = rr/pAp ; |r_n|^2 = rr - 2 alpha rAp + alpha^2 ApAp ; beta_{n+1} = |r_n|^2/rr
-__device__ void epi_cg2(Scal *S, double pAp, double rAp, double ApAp, double rr) {
-  const double alpha = rr / pAp;
-  double r2n = rr - 2.0 * alpha * rAp + alpha * alpha * ApAp;
-  if (!(r2n > 0.0)) r2n = 0.0;
-  S->r2prev = rr;
-  S->pAp = pAp;
-  S->alpha = alpha;
-  S->r2 = r2n;
-  S->beta = r2n / rr;
-  S->first = 0;
-  S->pscale = 1.0;
-  S->its += 1;
-  S->since += 1;
-  if (S->mixed) {
-    S->need_rel = (r2n < S->delta2 * S->maxr2) || (S->since >= S->relmax) || (r2n <= S->r2stop_s) ||
-                  (S->its >= S->maxits);
-  } else {
-    S->done = (S->its >= S->maxits) || (r2n <= S->r2stop);
-  }
-}
-
-// after the pending x/r update was applied: exact |r_n|^2
-__device__ void epi_flush(Scal *S, double rr) {
-  S->r2 = rr;
-  S->alpha = 0.0;
-  S->beta = (S->r2prev > 0.0) ? rr / S->r2prev : 0.0;
-  if (!S->mixed) S->done = (S->its >= S->maxits) || (rr <= S->r2stop);
-}
-
 // ------------------------------------------------------------ generic hop
 template <int PL, int RC, int PI, int PO, int PA, class A, int NL, int MODE, int RED>
 __global__ void __launch_bounds__(256) k_hop(HopArgs P) {
@@ -181,7 +128,7 @@ __global__ void __launch_bounds__(256) k_hop(HopArgs P) {
 #pragma unroll
       for (int c = 0; c < 3; c++) v[c] = ldc<PI, A>(P.in, c * g.stride + j);
     };
-    stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.par, i, s, ld, res);
+    stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, P.par, i, s, ld, res);
     const R chop = (R)P.chop;
 #pragma unroll
     for (int c = 0; c < 3; c++) res[c] = mk<A>(chop * res[c].x, chop * res[c].y);
@@ -217,11 +164,7 @@ __global__ void __launch_bounds__(256) k_hop(HopArgs P) {
   }
   if (RED) {
     double v[1] = {red};
-    if (grid_finish<1>(v, lds, P.red) && threadIdx.x == 0) {
-      if (RED == 1) P.S->pAp = v[0];
-      else if (RED == 2) epi_reliable(P.S, v[0], 6.0 * (double)g.Vh);
-      else P.S->tmp[0] = v[0];
-    }
+    block_partials<1>(v, lds, P.red);
   }
 }
 
@@ -231,6 +174,10 @@ static bool hop_modes(int mode, int redm, const HopArgs &a, hipStream_t st, int 
 #define QH_HOP(M, R)                                                                                 \
   if (mode == M && redm == R) {                                                                      \
     hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R>), grid, dim3(block), 0, st, a);       \
+    if (R) {                                                                                         \
+      FinArgs f{a.red, (int)grid.x, a.S, nullptr, 0, 6.0 * (double)a.g.Vh, a.pred};                  \
+      launch_finish(R == 1 ? EPI_PAP : R == 2 ? EPI_RELIABLE : EPI_TMP, f, st);                      \
+    }                                                                                                \
     return true;                                                                                     \
   }
   QH_HOP(0, 0) QH_HOP(1, 0) QH_HOP(2, 0)
@@ -247,8 +194,17 @@ static bool hop_prec(int rc, int nl, int mode, int redm, const HopArgs &a, hipSt
   return false;
 }
 
+template <class A>
+static bool hop_q16(int rc, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block) {
+  if (rc == 18 && nl == 4) return hop_modes<PQ16, 18, P16, P16, P16, A, 4>(mode, redm, a, st, block);
+  if (rc == 18 && nl == 8) return hop_modes<PQ16, 18, P16, P16, P16, A, 8>(mode, redm, a, st, block);
+  if (rc == 12 && nl == 4) return hop_modes<PQ16, 12, P16, P16, P16, A, 4>(mode, redm, a, st, block);
+  return false;
+}
+
 void launch_hop(const HopKey &k, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block) {
   bool ok = false;
+  if (k.pl == PQ16 && k.pi == P16 && k.po == P16 && k.pa == P16) ok = hop_q16<float2>(k.rc, nl, mode, redm, a, st, block);
   if (k.pi == k.pl && k.po == k.pl && k.pa == k.pl) {
     switch (k.pl) {
       case P64: ok = hop_prec<P64, double2>(k.rc, nl, mode, redm, a, st, block); break;
@@ -305,7 +261,7 @@ __global__ void __launch_bounds__(256) k_cg1(CgArgs P) {
     }
   };
   A res[3];
-  stencil<PL, RC, NL, A>(g, P.links, P.lkblk, q, i, s, ld, res);
+  stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, q, i, s, ld, res);
 #pragma unroll
   for (int c = 0; c < 3; c++) stc<PS, A>(P.t, c * g.stride + i, res[c]);
 }
@@ -322,23 +278,31 @@ __global__ void __launch_bounds__(256) k_cg2(CgArgs P) {
   if (i < g.Vh) {
     const bool first = S->first != 0;
     const R alpha = (R)S->alpha, beta = (R)(S->beta * S->pscale), m4 = (R)(4.0 * S->m2);
+    // own-site vectors first: their loads overlap the stencil's
+    A rv[3], av[3], pv[3], xv[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const long off = c * g.stride + i;
+      rv[c] = ldc<PR, A>(P.r, off);
+      av[c] = ldc<PS, A>(P.Ap, off);
+      pv[c] = first ? mk<A>(0, 0) : ldc<PS, A>(P.p, off);
+      xv[c] = ldc<PX, A>(P.x, off);
+    }
     const Site s = decode(g, i, P.par);
     auto ld = [&](long j, A(&v)[3]) {
 #pragma unroll
       for (int c = 0; c < 3; c++) v[c] = ldc<PS, A>(P.t, c * g.stride + j);
     };
     A H[3];
-    stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.par, i, s, ld, H);
+    stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, P.par, i, s, ld, H);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const long off = c * g.stride + i;
-      A rn, pn, pv, av;
-      p_next<PS, PR, A>(P, off, alpha, beta, first, rn, pn, pv, av);
-      A xv = ldc<PX, A>(P.x, off);
-      xv.x += alpha * pv.x;
-      xv.y += alpha * pv.y;
+      const A rn = mk<A>(rv[c].x - alpha * av[c].x, rv[c].y - alpha * av[c].y);
+      const A pn = rt_store<PS, A>(first ? rn : mk<A>(rn.x + beta * pv[c].x, rn.y + beta * pv[c].y));
+      const A xn = mk<A>(xv[c].x + alpha * pv[c].x, xv[c].y + alpha * pv[c].y);
       const A apn = rt_store<PS, A>(mk<A>(m4 * pn.x - H[c].x, m4 * pn.y - H[c].y));
-      stc<PX, A>(P.x, off, xv);
+      stc<PX, A>(P.x, off, xn);
       stc<PR, A>(P.r, off, rn);
       stc<PS, A>(P.p, off, pn);
       stc<PS, A>(P.Ap, off, apn);
@@ -348,7 +312,7 @@ __global__ void __launch_bounds__(256) k_cg2(CgArgs P) {
       red[3] += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
     }
   }
-  if (grid_finish<4>(red, lds, P.red) && threadIdx.x == 0) epi_cg2(S, red[0], red[1], red[2], red[3]);
+  block_partials<4>(red, lds, P.red);
 }
 
 // x += alpha p ; r -= alpha Ap ; |r|^2 (the pending update of the last iteration)
@@ -374,7 +338,7 @@ __global__ void __launch_bounds__(256) k_cg_flush(CgArgs P) {
     }
   }
   double v[1] = {rr};
-  if (grid_finish<1>(v, lds, P.red) && threadIdx.x == 0) epi_flush(S, v[0]);
+  block_partials<1>(v, lds, P.red);
 }
 
 template <int PL, int RC, int PS, int NL>
@@ -387,6 +351,8 @@ static void cg_iter_t(const CgArgs &a, hipStream_t st, int block) {
     hipLaunchKernelGGL((k_cg1<PL, RC, PS, P32, P32, float2, NL>), grid, blk, 0, st, a);
     hipLaunchKernelGGL((k_cg2<PL, RC, PS, P32, P32, float2, NL>), grid, blk, 0, st, a);
   }
+  FinArgs f{a.red, (int)grid.x, a.S, nullptr, 0, 0.0, 1};
+  launch_finish(EPI_CG2, f, st);
 }
 
 template <int PL, int PS>
@@ -405,6 +371,8 @@ void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t
   else if (pl == P16 && ps == P32) ok = cg_iter_rc<P16, P32>(rc, nl, a, st, block);
   else if (pl == PBF && ps == PBF) ok = cg_iter_rc<PBF, PBF>(rc, nl, a, st, block);
   else if (pl == PBF && ps == P32) ok = cg_iter_rc<PBF, P32>(rc, nl, a, st, block);
+  else if (pl == PQ16 && ps == P16) ok = cg_iter_rc<PQ16, P16>(rc, nl, a, st, block);
+  else if (pl == PQ16 && ps == P32) ok = cg_iter_rc<PQ16, P32>(rc, nl, a, st, block);
   if (!ok) fprintf(stderr, "qexhip: no CG instantiation for pl=%d rc=%d ps=%d nl=%d\n", pl, rc, ps, nl);
 }
 
@@ -416,11 +384,15 @@ void launch_cg_flush(int ps, const CgArgs &a, hipStream_t st) {
     case P16: hipLaunchKernelGGL((k_cg_flush<P16, P32, P32, float2>), grid, blk, 0, st, a); break;
     case PBF: hipLaunchKernelGGL((k_cg_flush<PBF, P32, P32, float2>), grid, blk, 0, st, a); break;
   }
+  FinArgs f{a.red, (int)grid.x, a.S, nullptr, 0, 0.0, 0};
+  launch_finish(EPI_FLUSH, f, st);
 }
 
 // ------------------------------------------------------------ link packing
+struct QScale { float inv[8]; };  // PQ16: 32767/scale per link field
+
 template <int PL, int RC>
-__global__ void k_pack_links(Geom g, int nl, const double2 *canon, char *out, long lkblk) {
+__global__ void k_pack_links(Geom g, int nl, const double2 *canon, char *out, long lkblk, QScale q) {
   using F = LFmt<PL, RC>;
   const long V = 2 * g.Vh;
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -440,6 +412,11 @@ __global__ void k_pack_links(Geom g, int nl, const double2 *canon, char *out, lo
     } else if (PL == P16) {
       __half2 h = __floats2half2_rn((float)d.x, (float)d.y);
       __builtin_memcpy(&w[c], &h, 4);
+    } else if (PL == PQ16) {
+      const double f = (double)q.inv[k];
+      const int re = (int)fmax(-32767.0, fmin(32767.0, rint(d.x * f)));
+      const int im = (int)fmax(-32767.0, fmin(32767.0, rint(d.y * f)));
+      w[c] = ((uint32_t)re & 0xffffu) | ((uint32_t)im << 16);
     } else {
       w[c] = f_to_bf16((float)d.x) | (f_to_bf16((float)d.y) << 16);
     }
@@ -448,22 +425,46 @@ __global__ void k_pack_links(Geom g, int nl, const double2 *canon, char *out, lo
 }
 
 template <int PL>
-static void pack_rc(int rc, const Geom &g, int nl, const double *canon, char *out, hipStream_t st) {
+static void pack_rc(int rc, const Geom &g, int nl, const double *canon, char *out, const QScale &q, hipStream_t st) {
   dim3 grid((unsigned)((2 * g.Vh + 255) / 256), nl);
   const long blk = link_block_bytes(PL, rc, g.stride);
   if (rc == 12)
-    hipLaunchKernelGGL((k_pack_links<PL, 12>), grid, dim3(256), 0, st, g, nl, (const double2 *)canon, out, blk);
+    hipLaunchKernelGGL((k_pack_links<PL, 12>), grid, dim3(256), 0, st, g, nl, (const double2 *)canon, out, blk, q);
   else
-    hipLaunchKernelGGL((k_pack_links<PL, 18>), grid, dim3(256), 0, st, g, nl, (const double2 *)canon, out, blk);
+    hipLaunchKernelGGL((k_pack_links<PL, 18>), grid, dim3(256), 0, st, g, nl, (const double2 *)canon, out, blk, q);
 }
 
-void launch_pack_links(int prec, int rc, const Geom &g, int nl, const double *canon, char *out, hipStream_t st) {
+void launch_pack_links(int prec, int rc, const Geom &g, int nl, const double *canon, char *out, const float *qs,
+                       hipStream_t st) {
+  QScale q;
+  for (int k = 0; k < 8; k++) q.inv[k] = (qs && k < nl && qs[k] > 0.0f) ? 1.0f / qs[k] : 0.0f;
   switch (prec) {
-    case P64: pack_rc<P64>(rc, g, nl, canon, out, st); break;
-    case P32: pack_rc<P32>(rc, g, nl, canon, out, st); break;
-    case P16: pack_rc<P16>(rc, g, nl, canon, out, st); break;
-    case PBF: pack_rc<PBF>(rc, g, nl, canon, out, st); break;
+    case P64: pack_rc<P64>(rc, g, nl, canon, out, q, st); break;
+    case P32: pack_rc<P32>(rc, g, nl, canon, out, q, st); break;
+    case P16: pack_rc<P16>(rc, g, nl, canon, out, q, st); break;
+    case PBF: pack_rc<PBF>(rc, g, nl, canon, out, q, st); break;
+    case PQ16: pack_rc<PQ16>(rc, g, nl, canon, out, q, st); break;
   }
+}
+
+// largest |re|, |im| of each link field (for the PQ16 scale)
+__global__ void k_link_maxabs(Geom g, int nl, const double2 *canon, unsigned *maxbits) {
+  const long V = 2 * g.Vh;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  float m = 0.0f;
+  if (idx < V) {
+    const double2 *u = canon + ((long)k * V + idx) * 9;
+    for (int e = 0; e < 9; e++) m = fmaxf(m, fmaxf(fabsf((float)u[e].x), fabsf((float)u[e].y)));
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(maxbits + k, __float_as_uint(m));
+}
+
+void launch_link_maxabs(const Geom &g, int nl, const double *canon, unsigned *maxbits, hipStream_t st) {
+  dim3 grid((unsigned)((2 * g.Vh + 255) / 256), nl);
+  hipLaunchKernelGGL(k_link_maxabs, grid, dim3(256), 0, st, g, nl, (const double2 *)canon, maxbits);
 }
 
 // Largest |row2 - sign*conj(row0 x row1)| over all plain links: 12-real

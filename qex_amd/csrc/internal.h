@@ -45,6 +45,7 @@ struct HopArgs {
   Geom g;
   const char *links;  // link blocks [k][parity], LFmt layout
   long lkblk;         // bytes per (k, parity) block
+  float qs[8];        // PQ16 links: value = int16 * qs[k]
   const void *in;     // other-parity input
   void *out;          // own-parity output
   const void *a;      // own-parity term A (precision PA)
@@ -63,6 +64,7 @@ struct CgArgs {
   Geom g;
   const char *links;
   long lkblk;
+  float qs[8];        // PQ16 links: value = int16 * qs[k]
   int par;            // solve parity; the first kernel writes the other parity
   int remap;
   void *x, *r, *p, *Ap, *t;
@@ -70,15 +72,33 @@ struct CgArgs {
   RedWs red;
 };
 
+// Finish step of a reduction (reduce.hip): sum the block partials of the
+// kernel just launched and apply one epilogue to the solver scalars.
+enum Epi { EPI_TMP = 0, EPI_PAP = 1, EPI_RELIABLE = 2, EPI_CG2 = 3, EPI_FLUSH = 4, EPI_CGM = 5 };
+struct FinArgs {
+  RedWs red;
+  int nb;             // blocks of the producing kernel
+  Scal *S;
+  ShiftScal *T;
+  int slot;           // EPI_TMP: S->tmp[slot]
+  double nreal;       // EPI_RELIABLE: real components of one parity vector
+  int pred;           // 0 none; 1 skip when done|need_rel; 3 skip unless T->continuing
+};
+
 struct HopKey {       // precision/format of one hop instantiation
   int pl, rc, pi, po, pa;
 };
+
+// reduce.hip
+void launch_finish(int epi, const FinArgs &f, hipStream_t st);
 
 // dslash.hip
 void launch_hop(const HopKey &k, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block);
 void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block);
 void launch_cg_flush(int ps, const CgArgs &a, hipStream_t st);
-void launch_pack_links(int prec, int rc, const Geom &g, int nl, const double *canon, char *out, hipStream_t st);
+void launch_pack_links(int prec, int rc, const Geom &g, int nl, const double *canon, char *out, const float *qs,
+                       hipStream_t st);
+void launch_link_maxabs(const Geom &g, int nl, const double *canon, unsigned *maxbits, hipStream_t st);
 void launch_recon_check(const Geom &g, const double *canon, unsigned *maxdev_bits, hipStream_t st);
 
 // kernels.hip

@@ -111,7 +111,12 @@ __global__ void __launch_bounds__(256) k_reduce(Geom g, const void *x, const voi
     }
   }
   double v[1] = {s};
-  if (grid_finish<1>(v, lds, red) && threadIdx.x == 0) S->tmp[slot] = v[0];
+  block_partials<1>(v, lds, red);
+}
+
+static void finish_tmp(unsigned nb, Scal *S, int slot, RedWs red, hipStream_t st) {
+  FinArgs f{red, (int)nb, S, nullptr, slot, 0.0, 0};
+  launch_finish(EPI_TMP, f, st);
 }
 
 void launch_norm2(int prec, const Geom &g, const void *x, Scal *S, int slot, RedWs red, hipStream_t st) {
@@ -122,6 +127,7 @@ void launch_norm2(int prec, const Geom &g, const void *x, Scal *S, int slot, Red
     case P16: hipLaunchKernelGGL((k_reduce<P16, 0>), grid, dim3(256), 0, st, g, x, x, S, slot, red); break;
     case PBF: hipLaunchKernelGGL((k_reduce<PBF, 0>), grid, dim3(256), 0, st, g, x, x, S, slot, red); break;
   }
+  finish_tmp(grid.x, S, slot, red, st);
 }
 
 void launch_redot(int prec, const Geom &g, const void *x, const void *y, Scal *S, int slot, RedWs red,
@@ -133,6 +139,7 @@ void launch_redot(int prec, const Geom &g, const void *x, const void *y, Scal *S
     case P16: hipLaunchKernelGGL((k_reduce<P16, 1>), grid, dim3(256), 0, st, g, x, y, S, slot, red); break;
     case PBF: hipLaunchKernelGGL((k_reduce<PBF, 1>), grid, dim3(256), 0, st, g, x, y, S, slot, red); break;
   }
+  finish_tmp(grid.x, S, slot, red, st);
 }
 
 // reliable update, part 1: x += xs/sigma ; xs = 0   (FP64 solution accumulation)
@@ -248,21 +255,7 @@ __global__ void __launch_bounds__(256) k_cgm_xr(Geom g, int nshift, double2 *r, 
     }
   }
   double v[1] = {s};
-  if (grid_finish<1>(v, lds, red) && threadIdx.x == 0) {
-    const double r2ip1 = v[0];
-    T->r2ip1 = r2ip1;
-    T->its += 1;
-    const double beta = (T->r2i != 0.0) ? r2ip1 / T->r2i : 0.0;
-    T->beta = beta;
-    T->continuing = (T->its < T->maxits) && (r2ip1 > T->r2stop);
-    S->done = !T->continuing;
-    T->pc1[0] = 1.0;
-    T->pc2[0] = beta;
-    for (int k = 1; k < T->nshift; k++) T->pc2[k] *= beta;
-    T->alphaim1 = T->alpha;
-    T->betaim1 = beta;
-    T->r2i = r2ip1;
-  }
+  block_partials<1>(v, lds, red);
 }
 
 // p_k = pc1[k] r + pc2[k] p_k (only while continuing, cgm.nim:186,198-199)
@@ -289,6 +282,8 @@ void launch_cgm_xr(const Geom &g, int nshift, double *r, const double *Ap, doubl
                    Scal *S, RedWs red, hipStream_t st) {
   hipLaunchKernelGGL(k_cgm_xr, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (double2 *)r,
                      (const double2 *)Ap, (double2 *const *)x, (double2 *const *)p, T, S, red);
+  FinArgs f{red, (int)nblk(g.Vh, 256), S, T, 0, 0.0, 3};
+  launch_finish(EPI_CGM, f, st);
 }
 
 void launch_cgm_p(const Geom &g, int nshift, const double *r, double **p, ShiftScal *T, hipStream_t st) {

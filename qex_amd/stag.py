@@ -23,7 +23,7 @@ import enum
 import numpy as np
 
 from . import _lib
-from ._lib import ALL, BF16, EVEN, FP16, FP32, FP64, ODD, QexHipError, SolverParamsC, call
+from ._lib import ALL, BF16, EVEN, FP16, FP32, FP64, ODD, Q16, QexHipError, SolverParamsC, call
 
 try:  # torch is plumbing for device arrays; numpy arrays work without it
     import torch
@@ -41,12 +41,17 @@ def _par(p):
 
 
 def _ptr(a):
-    """(pointer, on_device) of a contiguous complex128/float64 numpy or torch array."""
+    """(pointer, on_device) of a contiguous complex128/float64 numpy or torch array.
+
+    The library works on its own HIP stream and requires device data to be
+    complete when a call starts, so torch's pending work is waited for here."""
     if torch is not None and isinstance(a, torch.Tensor):
         if not a.is_contiguous():
             raise QexHipError("tensor must be contiguous")
         if a.dtype not in (torch.complex128, torch.float64):
             raise QexHipError("tensor must be complex128 or float64")
+        if a.is_cuda:
+            torch.cuda.current_stream(a.device).synchronize()
         return ctypes.c_void_p(a.data_ptr()), int(a.is_cuda)
     a = np.asarray(a)
     if not a.flags["C_CONTIGUOUS"] or a.dtype not in (np.complex128, np.float64):
@@ -231,15 +236,18 @@ class SolverBackend(enum.IntEnum):
 
 
 class SloppyType(enum.IntEnum):
-    """solverBase.nim:8-9 (SloppyBF16 added: BF16 link/spinor storage)."""
+    """solverBase.nim:8-9, plus BF16 storage and 16-bit fixed-point links
+    (SloppyHalfFixed: valid for sloppyLinks only, with FP16 or FP32 vectors)."""
     SloppyNone = 0
     SloppySingle = 1
     SloppyHalf = 2
     SloppyBF16 = 3
+    SloppyHalfFixed = 4
 
 
 _SLOPPY_PREC = {SloppyType.SloppyNone: FP64, SloppyType.SloppySingle: FP32,
-                SloppyType.SloppyHalf: FP16, SloppyType.SloppyBF16: BF16}
+                SloppyType.SloppyHalf: FP16, SloppyType.SloppyBF16: BF16,
+                SloppyType.SloppyHalfFixed: Q16}
 
 
 class SolverParams:

@@ -160,8 +160,9 @@ def test_errors_are_reported(gpu):
         gpu.solveEE(s, f64, f64, 0.1, sp)
 
 
-@pytest.mark.parametrize("sloppy,recon", [(0, 12), (1, 12), (2, 12), (2, 18)])
-def test_recon12_and_precisions(oracle, gpu, sloppy, recon):
+@pytest.mark.parametrize("sloppy,slinks,recon", [(0, None, 12), (1, None, 12), (2, None, 12), (2, None, 18),
+                                                  (2, 4, 18), (2, 4, 12), (1, 4, 12), (1, 3, 12)])
+def test_recon12_and_precisions(oracle, gpu, sloppy, slinks, recon):
     """12-real links (row 2 rebuilt from rows 0-1 with the phase/BC sign) in
     the inner operator: FP64 CG keeps the oracle's iteration count."""
     O = oracle
@@ -177,6 +178,7 @@ def test_recon12_and_precisions(oracle, gpu, sloppy, recon):
     fx = lo.ColorVector()
     sp = gpu.SolverParams()
     sp.r2req, sp.verbosity, sp.sloppySolve, sp.recon = 1e-20, 0, sloppy, recon
+    sp.sloppyLinks = slinks
     gpu.solveEE(s, fx, fb, m, sp)
     assert sp.r2 <= 1e-20
     if sloppy == 0:
@@ -202,7 +204,7 @@ def test_recon_refused_for_hisq_and_unphased(oracle, gpu):
         gpu.solveEE(s4, fx, fb, 0.1, sp)
 
 
-@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("recon", [18, 12])
 def test_time_hop_runs(gpu, prec, recon):
     from qex_amd import gauge
@@ -212,3 +214,23 @@ def test_time_hop_runs(gpu, prec, recon):
     s = gpu.Staggered(lo, U)
     ms = gpu.time_hop(s, prec, 5, recon=recon)
     assert 0 < ms < 10
+
+
+def test_q16_link_scale_hisq(oracle, gpu):
+    """16-bit fixed-point links with per-field scales (fat ~1, long ~1/24):
+    HISQ-form mixed solve with Q16 links and FP16 vectors meets the FP64 residual."""
+    O = oracle
+    L = (8, 8, 8, 8)
+    m = 0.05
+    g = links(O, L, True)
+    b = O.source_gauss(L, 31, 0, 0)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    fx = lo.ColorVector()
+    sp = gpu.SolverParams()
+    sp.r2req, sp.verbosity, sp.sloppySolve, sp.sloppyLinks = 1e-20, 0, 2, 4
+    gpu.solveEE(s, fx, fb, m, sp)
+    assert sp.r2 <= 1e-20
+    r = b - O.stagD2xx(L, g, 0, fx.get(), m * m)
+    assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20
