@@ -103,6 +103,9 @@ int qexhip_field_export(const qexhip_field *f, int par, double *canon, int canon
 /* FP64-accumulated norm2 / redot on a parity (fieldET.nim:648-778) */
 int qexhip_field_norm2(const qexhip_field *f, int par, double *out);
 int qexhip_field_redot(const qexhip_field *x, const qexhip_field *y, int par, double *out);
+/* y = a*x + b*y on a parity (QEX field expressions such as r.odd += b/m,
+   stagD.nim:633-637); FP64 fields */
+int qexhip_field_axpby(const qexhip_field *x, qexhip_field *y, int par, double a, double b);
 
 /* ---- operator (src/physics/stagD.nim) ----
    All fields of one call share one precision; arithmetic is FP64 for FP64

@@ -4,5 +4,6 @@ qex_amd/libqexhip.so (C ABI in include/qexhip.h); this package is its host-side
 mirror of QEX's Nim API."""
 from ._lib import ALL, BF16, EVEN, FP16, FP32, FP64, ODD, QexHipError, load  # noqa: F401
 from .stag import (D, Ddag, Field, Layout, SloppyType, SolverBackend, SolverParams,  # noqa: F401
-                   Staggered, fermForce, newLayout, newSolverParams, newStag, newStag3, solveEE,
-                   solveMulti, solveOO, stagD, stagD2, stagD2ee, stagD2oo, stagForceSolve, time_hop)
+                   Staggered, eoReconstruct, eoReduce, fermForce, newLayout, newSolverParams, newStag,
+                   newStag3, solve, solveEE, solveMulti, solveMultiMass, solveOO, stagD, stagD2, stagD2ee,
+                   stagD2oo, stagForceSolve, time_hop)

@@ -19,7 +19,7 @@ EXPORTS = [
     "qexhip_lattice_create", "qexhip_lattice_destroy",
     "qexhip_links_create", "qexhip_links_update", "qexhip_links_destroy",
     "qexhip_field_create", "qexhip_field_destroy", "qexhip_field_import", "qexhip_field_export",
-    "qexhip_field_norm2", "qexhip_field_redot",
+    "qexhip_field_norm2", "qexhip_field_redot", "qexhip_field_axpby",
     "qexhip_stagD2", "qexhip_stagD", "qexhip_stagD2xx",
     "qexhip_solve_xx", "qexhip_solve_ee_host", "qexhip_solve_oo_host", "qexhip_solve_multishift",
     "qexhip_force_solve", "qexhip_force",
@@ -75,6 +75,7 @@ def load(path=LIB_PATH):
         "qexhip_field_create": [vp, I, vpp], "qexhip_field_destroy": [vp],
         "qexhip_field_import": [vp, I, vp, I], "qexhip_field_export": [vp, I, vp, I],
         "qexhip_field_norm2": [vp, I, dp], "qexhip_field_redot": [vp, vp, I, dp],
+        "qexhip_field_axpby": [vp, vp, I, D, D],
         "qexhip_stagD2": [vp, I, vp, vp, D, D], "qexhip_stagD": [vp, I, vp, vp, D, D, D],
         "qexhip_stagD2xx": [vp, I, vp, vp, D],
         "qexhip_solve_xx": [vp, I, vp, vp, D, spp],

@@ -697,6 +697,21 @@ int qexhip_field_redot(const qexhip_field *x, const qexhip_field *y, int par, do
   API_END
 }
 
+int qexhip_field_axpby(const qexhip_field *x, qexhip_field *y, int par, double a, double b) {
+  API_BEGIN
+  check_init();
+  require(x && y && x->lat == y->lat, "bad arguments");
+  require(x->prec == P64 && y->prec == P64, "axpby fields must be FP64");
+  require(par >= 0 && par <= 2, "bad parity");
+  for (int p = 0; p < 2; p++) {
+    if (par != 2 && par != p) continue;
+    launch_axpby64(x->lat->g, a, (const double *)x->par(p), b, (double *)y->par(p), g_stream);
+  }
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
 // r = ca*r + cb*x + chop*(2D)x on parity par (general hop entry)
 static void dslash_core(qexhip_links *l, int par, qexhip_field *r, const qexhip_field *x, double ca, double cb,
                         double chop) {

@@ -118,6 +118,17 @@ class Field:
         call("qexhip_field_norm2", self.h, _par(sub), ctypes.byref(v))
         return v.value
 
+    def axpby(self, a, x, b=1.0, sub="all"):
+        """self = a*x + b*self on a subset (FP64 fields)."""
+        call("qexhip_field_axpby", x.h, self.h, _par(sub), float(a), float(b))
+        return self
+
+    def copy_from(self, x, sub="all"):
+        return self.axpby(1.0, x, 0.0, sub)
+
+    def zero(self, sub="all"):
+        return self.axpby(0.0, self, 0.0, sub)
+
     def redot(self, other, sub="all"):
         v = ctypes.c_double()
         call("qexhip_field_redot", self.h, other.h, _par(sub), ctypes.byref(v))
@@ -225,6 +236,17 @@ def Ddag(s, r, x, m):
     """D^+(m) = m - D on all sites (stagD.nim:620-622)."""
     stagD(s.se, r, s.g, x, m, -1.0)
     stagD(s.so, r, s.g, x, m, -1.0)
+
+
+def eoReduce(s, r, b, m):
+    """r.even = (D^+ b).even (stagD.nim:626-632)."""
+    stagD(s.se, r, s.g, b, m, -1.0)
+
+
+def eoReconstruct(s, r, b, m):
+    """r.odd = (b.odd - D_oe r.even)/m (stagD.nim:633-637)."""
+    stagD(s.so, r, s.g, r, 0.0, -1.0 / m)
+    r.axpby(1.0 / m, b, 1.0, "odd")
 
 
 class SolverBackend(enum.IntEnum):
@@ -346,6 +368,153 @@ def solveMulti(s, xs, b, m, sigmas, sp, subset="even"):
     call("qexhip_solve_multishift", s.h, _par(subset), arr, b.h, float(m), sig, len(sigmas), ctypes.byref(c))
     sp._add(c)
     return c
+
+
+def _sub_solve(s, x, b, m, sp, r2req, par):
+    sub = SolverParams()
+    sub.__dict__.update({k: v for k, v in sp.__dict__.items() if not k.startswith("_")})
+    sub.resetStats()
+    sub.r2req = r2req
+    sub.maxits = sp.maxits - sp.iterations
+    sub.verbosity = sp.verbosity - 1
+    _solve_xx(s, x, b, m, sub, par)
+    sp.iterations += sub.iterations
+    sp.seconds += sub.seconds
+    sp.flops += sub.flops
+    sp.reliableUpdates += sub.reliableUpdates
+
+
+def _solve_recon_r(s, x, b, m, sp, b2e, b2o):
+    """solveReconR (stagSolve.nim:238-278): right-preconditioned."""
+    lo = s.l
+    b2 = b2e + b2o
+    r2stop = sp.r2req * b2
+    r2stop2 = 0.5 * r2stop
+    r2stope = r2stop - b2o if b2o <= r2stop2 else r2stop2
+    y = lo.ColorVector()
+    if b2e > r2stope:
+        _sub_solve(s, y, b, m, sp, r2stope / b2e, EVEN)
+        y.axpby(0.0, y, 4.0, "even")
+        Ddag(s, x, y, m)
+        return
+    r2stopo = r2stop - b2e if b2e <= r2stop2 else r2stop2
+    if b2o > r2stopo:
+        _sub_solve(s, y, b, m, sp, r2stopo / b2o, ODD)
+        y.axpby(0.0, y, 4.0, "odd")
+        Ddag(s, x, y, m)
+
+
+def _solve_recon_l(s, x, b, m, sp, b2e, b2o):
+    """solveReconL (stagSolve.nim:281-310): D^+ reduce, even solve, odd reconstruct."""
+    lo = s.l
+    d = lo.ColorVector()
+    Ddag(s, d, b, m)
+    x.zero()
+    d2e = d.norm2("even")
+    r2req = 0.99 * sp.r2req * (b2e + b2o) * m * m / d2e
+    _sub_solve(s, x, d, m, sp, r2req, EVEN)
+    x.axpby(0.0, x, 4.0, "even")
+    eoReconstruct(s, x, b, m)
+
+
+def solve(s, x, b, m, sp):
+    """Full-lattice D(m) x = b (stagSolve.nim:469-557): even-odd preconditioned
+    inner solves (solveInner, :313-326) inside an FP64 true-residual loop."""
+    lo = s.l
+    b2 = b.norm2()
+    r2stop = sp.r2req * b2
+    r = lo.ColorVector()
+    if sp.usePrevSoln:
+        D(s, r, x, m)
+        r.axpby(1.0, b, -1.0)
+    else:
+        x.zero()
+        r.copy_from(b)
+    r2e, r2o = r.norm2("even"), r.norm2("odd")
+    r2 = r2e + r2o
+    y = lo.ColorVector()
+    its0 = sp.iterations
+    inner = SolverParams()
+    inner.__dict__.update({k: v for k, v in sp.__dict__.items() if not k.startswith("_")})
+    inner.resetStats()
+    while r2 > r2stop:
+        inner.maxits = sp.maxits - inner.iterations
+        if inner.maxits <= 0:
+            break
+        inner.r2req = r2stop / r2
+        y.zero()
+        rs = r2stop / r2 * (r2e + r2o)
+        rs2 = 0.5 * rs
+        r2stope = rs - r2o if r2o <= rs2 else rs2
+        r2stopo = rs - r2e if r2e <= rs2 else rs2
+        if r2e <= r2stope or r2o <= r2stopo or m == 0.0:
+            _solve_recon_r(s, y, r, m, inner, r2e, r2o)
+        else:
+            _solve_recon_l(s, y, r, m, inner, r2e, r2o)
+        x.axpby(1.0, y, 1.0)
+        D(s, r, x, m)
+        r.axpby(1.0, b, -1.0)
+        r2e, r2o = r.norm2("even"), r.norm2("odd")
+        r2 = r2e + r2o
+    sp.calls += 1
+    sp.iterations = its0 + inner.iterations
+    sp.seconds += inner.seconds
+    sp.flops += inner.flops
+    sp.reliableUpdates += inner.reliableUpdates
+    sp.r2 = r2 / b2 if b2 > 0 else 0.0
+    return sp
+
+
+def solveMultiMass(s, xs, b, ms, sp):
+    """Multi-mass full-lattice solve (stagSolve.nim:560-668): CGM on one parity
+    with shifts 4(m_k^2 - m_0^2), outer FP64 residual loop on m_0, then
+    x_k = D^+(m_k) x_k."""
+    lo = s.l
+    if len(xs) != len(ms):
+        raise QexHipError("solveMultiMass: need one solution field per mass")
+    b2, b2e, b2o = b.norm2(), b.norm2("even"), b.norm2("odd")
+    r = lo.ColorVector().copy_from(b)
+    r2 = b2e + b2o
+    r2stop = sp.r2req * b2
+    shifts = [0.0] + [4.0 * (mk * mk - ms[0] * ms[0]) for mk in ms[1:]]
+    for f in xs:
+        f.zero()
+    ys = [lo.ColorVector() for _ in ms]
+    xt = lo.ColorVector()
+    its = 0
+    while r2 > r2stop:
+        inner = SolverParams()
+        inner.__dict__.update({k: v for k, v in sp.__dict__.items() if not k.startswith("_")})
+        inner.resetStats()
+        inner.maxits = sp.maxits - its
+        if inner.maxits <= 0:
+            break
+        req = r2stop
+        rs2 = 0.5 * req
+        r2stope = req - b2o if b2o <= rs2 else rs2
+        r2stopo = req - b2e if b2e <= rs2 else rs2
+        if b2e > r2stope:
+            inner.r2req, sub = r2stope / b2e, "even"
+        elif b2o > r2stopo:
+            inner.r2req, sub = r2stopo / b2o, "odd"
+        else:
+            break
+        solveMulti(s, ys, r, ms[0], shifts, inner, subset=sub)
+        its += inner.iterations
+        for f, y in zip(xs, ys):
+            f.axpby(4.0, y, 1.0, sub)
+        Ddag(s, xt, xs[0], ms[0])
+        D(s, r, xt, ms[0])
+        r.axpby(1.0, b, -1.0)
+        b2e, b2o = r.norm2("even"), r.norm2("odd")
+        r2 = b2e + b2o
+    for k, f in enumerate(xs):
+        Ddag(s, xt, f, ms[k])
+        f.copy_from(xt)
+    sp.calls += 1
+    sp.iterations += its
+    sp.r2 = r2 / b2 if b2 > 0 else 0.0
+    return sp
 
 
 def stagForceSolve(s, psi, phi, mass, sp):
