@@ -143,6 +143,20 @@ int qexhip_force_solve(qexhip_links *l, qexhip_field *psi, const qexhip_field *p
    outer_only=1 returns eps(x) psi(x) psi(x+mu)^+ instead.  F: canonical 4 x V x 18 doubles. */
 int qexhip_force(qexhip_links *l, const qexhip_field *psi, double *F, int F_on_device, int outer_only);
 
+/* ---- gauge field for HMC (FP64 canonical arrays U[4][V][3][3] in device
+        memory; the pieces config-5's leapfrog needs around the fermion force) ---- */
+/* sum_{x,mu<nu} Re tr U_P; QEX's Wilson action is -(beta/3) times this
+   (gauge/gaugeAction.nim:397-427) */
+int qexhip_plaquette(qexhip_lattice *lat, const double *U_dev, double *sum);
+/* F_mu(x) = (2 beta/3) TAH(U_mu(x) * staples), so that dS_W/dw = -1/2 Re tr(X F) for
+   U -> exp(wX)U, the convention of qexhip_force (gauge/gaugeAction.nim:765-781) */
+int qexhip_wilson_force(qexhip_lattice *lat, const double *U_dev, double beta, double *F_dev);
+/* traceless anti-Hermitian Gaussian momenta, density exp(tr P^2/2)
+   (gauge/gaugeUtils.nim:1474-1500) */
+int qexhip_random_momenta(qexhip_lattice *lat, uint64_t seed, double *P_dev);
+/* U <- exp(eps P) U (gauge/gaugeUtils.nim:453-475) */
+int qexhip_link_update(qexhip_lattice *lat, const double *P_dev, double eps, double *U_dev);
+
 /* ---- synthetic inputs (no QEX counterpart; QEX's random fields use its own
         RNG, src/rng).  Outputs are canonical FP64 arrays in device memory. ---- */
 int qexhip_gen_gauge(qexhip_lattice *lat, uint64_t seed, double *U_dev);

@@ -58,6 +58,10 @@ def _declare(l):
         "qo_ferm_force": (None, [_ip, _dp, _dp, _dp]),
         "qo_ferm_outer": (None, [_ip, _dp, _dp]),
         "qo_force_solve": (I, [_ip, I, _dp, _dp, D, D, I, _dp]),
+        "qo_plaquette": (D, [_ip, _dp]),
+        "qo_wilson_force": (None, [_ip, _dp, D, _dp]),
+        "qo_random_momenta": (None, [_ip, U64, _dp]),
+        "qo_link_update": (None, [_ip, _dp, D, _dp]),
         "qo_su3_exp_i": (None, [_dp, D, _dp]),
         "qo_su3_exp": (None, [_dp, _dp]),
         "qo_mat_mul": (None, [_dp, _dp, _dp]),
@@ -243,3 +247,24 @@ def project_tah(X):
 
 def norm2(x):
     return float(np.vdot(x, x).real)
+
+
+def plaquette(L, U):
+    return float(lib().qo_plaquette(_L(L), _p(U)))
+
+
+def wilson_force(L, U, beta):
+    F = np.zeros_like(U)
+    lib().qo_wilson_force(_L(L), _p(U), beta, _p(F))
+    return F
+
+
+def random_momenta(L, seed):
+    P = np.zeros((4, volume(L), 3, 3), dtype=np.complex128)
+    lib().qo_random_momenta(_L(L), seed, _p(P))
+    return P
+
+
+def link_update(L, P, eps, U):
+    """U <- exp(eps P) U in place."""
+    lib().qo_link_update(_L(L), _p(P), eps, _p(U))

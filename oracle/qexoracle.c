@@ -155,7 +155,7 @@ void qo_project_tah(const double *x, double *r) {
 }
 
 /* ------------------------------------------------------- synthetic inputs */
-enum { TAG_GAUGE = 0, TAG_SOURCE = 1, TAG_FATG = 2 };
+enum { TAG_GAUGE = 0, TAG_SOURCE = 1, TAG_FATG = 2, TAG_MOM = 3 };
 
 void qo_gauge_random(const int L[4], uint64_t seed, double *U) {
   long V = qo_volume(L);
@@ -572,4 +572,109 @@ int qo_force_solve(const int L[4], int nl, const double *g, const double *phi,
   memcpy(psi, vphi, sizeof(double) * 6 * Vh);
   free(vphi);
   return its;
+}
+
+/* ------------------------------------------------------ gauge field (HMC) */
+static inline const cplx *lnk(const double *U, long V, int mu, long idx) {
+  return (const cplx *)(U + ((long)mu * V + idx) * 18);
+}
+static void m3mul(const cplx *a, const cplx *b, cplx *c) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      cplx t = 0;
+      for (int k = 0; k < 3; k++) t += a[3 * i + k] * b[3 * k + j];
+      c[3 * i + j] = t;
+    }
+}
+static void m3dag(const cplx *a, cplx *r) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) r[3 * i + j] = conj(a[3 * j + i]);
+}
+
+/* gauge/gaugeAction.nim:397-427 (plaquette term) */
+double qo_plaquette(const int L[4], const double *U) {
+  long V = qo_volume(L);
+  double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (long idx = 0; idx < V; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    for (int mu = 0; mu < 4; mu++)
+      for (int nu = mu + 1; nu < 4; nu++) {
+        cplx a[9], b[9];
+        m3mul(lnk(U, V, mu, idx), lnk(U, V, nu, nbr(L, c, mu, 1)), a);
+        m3mul(lnk(U, V, nu, idx), lnk(U, V, mu, nbr(L, c, nu, 1)), b);
+        for (int e = 0; e < 9; e++) s += creal(a[e] * conj(b[e]));
+      }
+  }
+  return s;
+}
+
+/* derivative of -(beta/3) sum Re tr U_P (gauge/gaugeAction.nim:765-781) */
+void qo_wilson_force(const int L[4], const double *U, double beta, double *F) {
+  long V = qo_volume(L);
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < V; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    for (int mu = 0; mu < 4; mu++) {
+      cplx A[9] = {0};
+      for (int nu = 0; nu < 4; nu++) {
+        if (nu == mu) continue;
+        int cm[4] = {c[0], c[1], c[2], c[3]};
+        cm[nu] = (cm[nu] - 1 + L[nu]) % L[nu];
+        long xp = nbr(L, c, mu, 1), xpn = nbr(L, c, nu, 1), xmn = nbr(L, c, nu, -1), xpmn = nbr(L, cm, mu, 1);
+        cplx t1[9], t2[9], d[9], up[9], lo[9];
+        m3dag(lnk(U, V, mu, xpn), d);
+        m3mul(lnk(U, V, nu, xp), d, t1);
+        m3dag(lnk(U, V, nu, idx), d);
+        m3mul(t1, d, up);
+        m3dag(lnk(U, V, nu, xpmn), t1);
+        m3dag(lnk(U, V, mu, xmn), d);
+        m3mul(t1, d, t2);
+        m3mul(t2, lnk(U, V, nu, xmn), lo);
+        for (int e = 0; e < 9; e++) A[e] += up[e] + lo[e];
+      }
+      cplx T[9], R[9];
+      m3mul(lnk(U, V, mu, idx), A, T);
+      qo_project_tah((double *)T, (double *)R);
+      cplx *f = (cplx *)(F + ((long)mu * V + idx) * 18);
+      for (int e = 0; e < 9; e++) f[e] = (2.0 * beta / 3.0) * R[e];
+    }
+  }
+}
+
+void qo_random_momenta(const int L[4], uint64_t seed, double *P) {
+  long V = qo_volume(L);
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < V; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    uint32_t lex = (uint32_t)(c[0] + (long)L[0] * (c[1] + (long)L[1] * (c[2] + (long)L[2] * c[3])));
+    for (int mu = 0; mu < 4; mu++) {
+      double r[10];
+      for (int j = 0; j < 5; j++) qo_gauss2(seed, lex, 32u + (uint32_t)mu, (uint32_t)j, TAG_MOM, r + 2 * j);
+      const double s = sqrt(0.5), m = (r[0] + r[1] + r[2]) / 3.0;
+      cplx H[9];
+      H[0] = r[0] - m; H[4] = r[1] - m; H[8] = r[2] - m;
+      H[1] = s * r[3] + I * s * r[4]; H[3] = conj(H[1]);
+      H[2] = s * r[5] + I * s * r[6]; H[6] = conj(H[2]);
+      H[5] = s * r[7] + I * s * r[8]; H[7] = conj(H[5]);
+      cplx *p = (cplx *)(P + ((long)mu * V + idx) * 18);
+      for (int e = 0; e < 9; e++) p[e] = I * H[e];
+    }
+  }
+}
+
+void qo_link_update(const int L[4], const double *P, double eps, double *U) {
+  long V = qo_volume(L);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < 4 * V; i++) {
+    cplx A[9], E[9], T[9];
+    const cplx *p = (const cplx *)(P + i * 18);
+    for (int e = 0; e < 9; e++) A[e] = eps * p[e];
+    qo_su3_exp((double *)A, (double *)E);
+    m3mul(E, (const cplx *)(U + i * 18), T);
+    memcpy(U + i * 18, T, sizeof T);
+  }
 }

@@ -95,6 +95,17 @@ void qo_ferm_outer(const int L[4], const double *psi, double *G);
 int qo_force_solve(const int L[4], int nl, const double *g, const double *phi,
                    double m, double r2req, int maxits, double *psi);
 
+/* ---- gauge field pieces of the staggered HMC ---- */
+/* sum_{x,mu<nu} Re tr U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ ; QEX's Wilson
+   action is -(beta/3) times this (gauge/gaugeAction.nim:397-427) */
+double qo_plaquette(const int L[4], const double *U);
+/* F = (2 beta/3) TAH(U * staples): dS/dw = -1/2 Re tr(X F) for U -> exp(wX)U */
+void qo_wilson_force(const int L[4], const double *U, double beta, double *F);
+/* P = i H, H traceless Hermitian, density exp(-tr H^2/2) (gaugeUtils.nim:1474-1500) */
+void qo_random_momenta(const int L[4], uint64_t seed, double *P);
+/* U <- exp(eps P) U (gaugeUtils.nim:453-475) */
+void qo_link_update(const int L[4], const double *P, double eps, double *U);
+
 /* ---- 3x3 helpers used by tests ---- */
 void qo_su3_exp_i(const double *H /*18, Hermitian*/, double s, double *U /*18*/);
 void qo_su3_exp(const double *A /*18, general*/, double *U /*18*/);

@@ -25,6 +25,7 @@ EXPORTS = [
     "qexhip_force_solve", "qexhip_force",
     "qexhip_gen_gauge", "qexhip_gen_source", "qexhip_gen_phase_bc", "qexhip_gen_hisq",
     "qexhip_time_hop", "qexhip_links_recon_ok",
+    "qexhip_plaquette", "qexhip_wilson_force", "qexhip_random_momenta", "qexhip_link_update",
 ]
 
 
@@ -85,6 +86,8 @@ def load(path=LIB_PATH):
         "qexhip_gen_gauge": [vp, U64, vp], "qexhip_gen_source": [vp, U64, I, I, vp],
         "qexhip_gen_phase_bc": [vp, vp], "qexhip_gen_hisq": [vp, U64, vp, vp],
         "qexhip_time_hop": [vp, I, I, I, dp], "qexhip_links_recon_ok": [vp, ip],
+        "qexhip_plaquette": [vp, vp, dp], "qexhip_wilson_force": [vp, vp, D, vp],
+        "qexhip_random_momenta": [vp, U64, vp], "qexhip_link_update": [vp, vp, D, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -910,6 +910,47 @@ int qexhip_gen_hisq(qexhip_lattice *lat, uint64_t seed, const double *Uph, doubl
   API_END
 }
 
+int qexhip_plaquette(qexhip_lattice *lat, const double *U, double *sum) {
+  API_BEGIN
+  check_init();
+  require(lat && U && sum, "null argument");
+  launch_plaquette(lat->g, U, lat->S, 0, lat->red, g_stream);
+  check_launch();
+  pull(lat->Sh, lat->S);
+  *sum = lat->Sh->tmp[0];
+  API_END
+}
+
+int qexhip_wilson_force(qexhip_lattice *lat, const double *U, double beta, double *F) {
+  API_BEGIN
+  check_init();
+  require(lat && U && F, "null argument");
+  launch_wilson_force(lat->g, U, beta, F, g_stream);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+int qexhip_random_momenta(qexhip_lattice *lat, uint64_t seed, double *P) {
+  API_BEGIN
+  check_init();
+  require(lat && P, "null argument");
+  launch_random_momenta(lat->g, seed, P, g_stream);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+int qexhip_link_update(qexhip_lattice *lat, const double *P, double eps, double *U) {
+  API_BEGIN
+  check_init();
+  require(lat && P && U, "null argument");
+  launch_link_update(lat->g, P, eps, U, g_stream);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
 int qexhip_links_recon_ok(qexhip_links *l, int *ok) {
   API_BEGIN
   check_init();

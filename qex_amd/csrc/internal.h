@@ -119,6 +119,12 @@ void launch_cgm_alpha(ShiftScal *T, Scal *S, hipStream_t st);
 void launch_force(const Geom &g, const char *links64, long lkblk, const double *psi_e, const double *psi_o,
                   double *F, int outer_only, hipStream_t st);
 
+// gauge.hip
+void launch_plaquette(const Geom &g, const double *U, Scal *S, int slot, RedWs red, hipStream_t st);
+void launch_wilson_force(const Geom &g, const double *U, double beta, double *F, hipStream_t st);
+void launch_random_momenta(const Geom &g, uint64_t seed, double *P, hipStream_t st);
+void launch_link_update(const Geom &g, const double *P, double eps, double *U, hipStream_t st);
+
 // gen.hip
 void launch_gauge_random(const Geom &g, uint64_t seed, double *canon, hipStream_t st);
 void launch_source_gauss(const Geom &g, uint64_t seed, int stream, int par, double *canon, hipStream_t st);

@@ -1,0 +1,153 @@
+"""Gauge-field kernels of the staggered HMC and the torch autograd binding of the
+fermion action, through the C ABI, against the CPU oracle and against what the
+mathematics fixes:
+
+ - plaquette, Wilson force, momenta and link update equal the oracle's;
+ - the leapfrog energy violation scales as dt^2 and the integrator is
+   reversible (forces and action are mutually consistent);
+ - the total MD force equals a central finite difference of the GPU action;
+ - U.grad of the autograd action equals a finite difference of the action.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+pytestmark = pytest.mark.gpu
+
+SEED = 31
+
+
+def _gpu_links(O, L, seed):
+    import torch
+    return torch.from_numpy(O.gauge_random(L, seed)).cuda()
+
+
+def test_plaquette_matches_oracle(oracle, gpu):
+    from qex_amd import hmc
+    L = (8, 8, 4, 6)
+    lo = gpu.Layout(L)
+    U = oracle.gauge_random(L, SEED)
+    got = hmc.plaquette(lo, _gpu_links(oracle, L, SEED))
+    want = oracle.plaquette(L, U)
+    assert abs(got - want) <= 1e-13 * abs(want)
+    import torch
+    unit = torch.from_numpy(oracle.gauge_unit(L)).cuda()
+    assert hmc.plaquette(lo, unit) == 18.0 * lo.nSites
+
+
+def test_wilson_force_matches_oracle(oracle, gpu):
+    from qex_amd import hmc
+    L = (8, 4, 6, 8)
+    lo = gpu.Layout(L)
+    U = oracle.gauge_random(L, SEED)
+    F = hmc.wilson_force(lo, _gpu_links(oracle, L, SEED), 5.6).cpu().numpy()
+    want = oracle.wilson_force(L, U, 5.6)
+    assert np.abs(F - want).max() <= 1e-13 * np.abs(want).max()
+
+
+def test_momenta_and_link_update_match_oracle(oracle, gpu):
+    from qex_amd import hmc
+    L = (8, 8, 8, 4)
+    lo = gpu.Layout(L)
+    P = hmc.random_momenta(lo, 99).cpu().numpy()
+    Pw = oracle.random_momenta(L, 99)
+    assert np.abs(P - Pw).max() < 1e-15
+    U = oracle.gauge_random(L, SEED)
+    Ug = _gpu_links(oracle, L, SEED)
+    import torch
+    hmc.link_update(lo, torch.from_numpy(Pw).cuda(), 0.07, Ug)
+    oracle.link_update(L, Pw, 0.07, U)
+    assert np.abs(Ug.cpu().numpy() - U).max() < 1e-14
+
+
+def _hmc(gpu, L, mass=0.2, beta=5.6):
+    from qex_amd import hmc, gauge
+    lo = gpu.Layout(L)
+    U = gauge.unit_gauge(lo, device="cuda")
+    return hmc.StaggeredHMC(lo, U, beta, mass, r2req_force=1e-26, r2req_action=1e-26)
+
+
+def test_leapfrog_dH_scales_as_dt2(gpu):
+    import torch
+    L = (8, 8, 8, 8)
+    dH = []
+    for n in (6, 12):
+        h = _hmc(gpu, L)
+        from qex_amd import hmc
+        P = hmc.random_momenta(h.lo, 5)
+        h.heatbath(6)
+        H0 = h.hamiltonian(P)
+        P = h.leapfrog(P, 0.5, n)
+        dH.append(h.hamiltonian(P) - H0)
+        torch.cuda.synchronize()
+    ratio = dH[0] / dH[1]
+    assert abs(dH[0]) > 1e-4, dH
+    assert 3.2 < ratio < 4.8, dH
+
+
+def test_leapfrog_reversible(gpu):
+    h = _hmc(gpu, (8, 8, 8, 8))
+    dU, dP = h.reversibility(11, tau=0.5, nsteps=8)
+    assert dU < 1e-10 and dP < 1e-9, (dU, dP)
+
+
+def test_md_force_matches_action_difference(gpu):
+    """-2 Pdot is dS/dw along U_mu(x) -> exp(w X) U_mu(x); compare with a
+    central difference of S_g + S_f evaluated by the GPU."""
+    import torch
+    from qex_amd import hmc
+    L = (8, 8, 8, 8)
+    h = _hmc(gpu, L, mass=0.3)
+    # move off the unit configuration so the gauge force is non-zero
+    hmc.link_update(h.lo, hmc.random_momenta(h.lo, 3), 0.3, h.U)
+    h._links_changed()
+    h.heatbath(4)
+    F = (-2.0 * h.pdot()).cpu().numpy()
+    rng = np.random.default_rng(2)
+    U0 = h.U.clone()
+    step = 1e-4
+    for trial in range(3):
+        mu, site = int(rng.integers(4)), int(rng.integers(h.lo.nSites))
+        A = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        X = 0.5 * (A - A.conj().T)
+        X -= np.trace(X) / 3 * np.eye(3)
+        S = []
+        for w in (step, -step):
+            h.U.copy_(U0)
+            h.U[mu, site] = torch.from_numpy(scipy.linalg.expm(w * X)).cuda() @ U0[mu, site]
+            h._links_changed()
+            S.append(h.gauge_action() + h.fermion_action())
+        fd = (S[0] - S[1]) / (2 * step)
+        an = -0.5 * np.trace(X @ F[mu, site]).real
+        assert abs(fd - an) < 1e-5 * max(1.0, abs(an)), (trial, fd, an)
+    h.U.copy_(U0)
+    h._links_changed()
+
+
+def test_autograd_action_gradient(oracle, gpu):
+    import torch
+    from qex_amd import autograd, gauge
+    L = (8, 8, 4, 4)
+    lo = gpu.Layout(L)
+    U = torch.from_numpy(oracle.phased(L, oracle.gauge_random(L, SEED))).cuda()
+    phi = gauge.source(lo, 8, stream=0, par=0)
+    m = 0.25
+    Ur = U.clone().requires_grad_(True)
+    S = autograd.staggered_action(Ur, phi, m, lo)
+    S.backward()
+    grad = Ur.grad
+    # the value is phi_e^+ (m^2 - D_eo D_oe)^-1 phi_e
+    want_y, _, _ = oracle.solveXX(L, U.cpu().numpy(), 0, phi.cpu().numpy(), m, 1e-28)
+    Sw = 4.0 * np.vdot(phi.cpu().numpy()[: lo.nEven], want_y[: lo.nEven]).real
+    assert abs(float(S.detach()) - Sw) < 1e-10 * abs(Sw)
+    gen = torch.Generator(device="cpu").manual_seed(1)
+    h = 1e-5
+    for trial in range(3):
+        dU = torch.complex(torch.randn(U.shape, generator=gen, dtype=torch.float64),
+                           torch.randn(U.shape, generator=gen, dtype=torch.float64)).cuda()
+        dU = dU * 0.01
+        Sp = float(autograd.staggered_action(U + h * dU, phi, m, lo))
+        Sm = float(autograd.staggered_action(U - h * dU, phi, m, lo))
+        fd = (Sp - Sm) / (2 * h)
+        an = float(torch.sum((dU.conj() * grad).real))
+        assert abs(fd - an) < 1e-6 * max(1.0, abs(an)), (trial, fd, an)

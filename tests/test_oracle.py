@@ -393,3 +393,78 @@ def test_force_gauge_covariance(oracle):
     # F is traceless anti-Hermitian
     assert np.abs(F + np.conj(np.swapaxes(F, -1, -2))).max() < 1e-13
     assert np.abs(np.trace(F, axis1=-2, axis2=-1)).max() < 1e-13
+
+
+# ------------------------------------------------------------ HMC pieces
+def _staple_np(oracle, L, U, mu, site):
+    """Sum of the six staples at (mu, site), computed independently in numpy."""
+    C = _coords_np(L)
+    c = C[site]
+
+    def idx(d):
+        y = (c + np.array(d)) % np.array(L)
+        return int(_index_np(L, y[None])[0])
+
+    A = np.zeros((3, 3), complex)
+    e = np.eye(4, dtype=int)
+    for nu in range(4):
+        if nu == mu:
+            continue
+        dag = lambda m: m.conj().T
+        A += U[nu, idx(e[mu])] @ dag(U[mu, idx(e[nu])]) @ dag(U[nu, site])
+        A += dag(U[nu, idx(e[mu] - e[nu])]) @ dag(U[mu, idx(-e[nu])]) @ U[nu, idx(-e[nu])]
+    return A
+
+
+def test_plaquette_closed_forms(oracle):
+    L = (4, 4, 4, 6)
+    assert oracle.plaquette(L, oracle.gauge_unit(L)) == 18 * oracle.volume(L)
+    U = oracle.gauge_random(L, SEED)
+    g = _random_su3_field(oracle, L, 3)
+    Ug = _transform(L, U, g, [(k, 1) for k in range(4)])
+    assert abs(oracle.plaquette(L, Ug) - oracle.plaquette(L, U)) < 1e-10
+
+
+def test_wilson_force_finite_differences(oracle):
+    """dS/dw = -1/2 Re tr(X F) for S = -(beta/3) sum Re tr U_P; the action change
+    is evaluated locally through the staple sum (exact up to rounding)."""
+    L = (4, 4, 4, 4)
+    beta = 5.6
+    U = oracle.gauge_random(L, SEED)
+    F = oracle.wilson_force(L, U, beta)
+    assert np.abs(F + np.conj(np.swapaxes(F, -1, -2))).max() < 1e-13
+    rng = np.random.default_rng(11)
+    h = 1e-5
+    for trial in range(8):
+        mu, site = int(rng.integers(4)), int(rng.integers(oracle.volume(L)))
+        X = _rand_su3_alg(rng)
+        A = _staple_np(oracle, L, U, mu, site)
+        S = lambda W: -(beta / 3) * np.trace(W @ A).real
+        fd = (S(scipy.linalg.expm(h * X) @ U[mu, site]) - S(scipy.linalg.expm(-h * X) @ U[mu, site])) / (2 * h)
+        an = -0.5 * np.trace(X @ F[mu, site]).real
+        assert abs(fd - an) <= 1e-8 * max(1.0, abs(an))
+    # and the staple route agrees with the whole-lattice plaquette difference
+    mu, site = 2, 77
+    X = _rand_su3_alg(rng)
+    Up = U.copy()
+    Up[mu, site] = scipy.linalg.expm(1e-3 * X) @ U[mu, site]
+    dfull = -(beta / 3) * (oracle.plaquette(L, Up) - oracle.plaquette(L, U))
+    A = _staple_np(oracle, L, U, mu, site)
+    dloc = -(beta / 3) * (np.trace(Up[mu, site] @ A).real - np.trace(U[mu, site] @ A).real)
+    assert abs(dfull - dloc) < 1e-9
+
+
+def test_momenta_and_link_update(oracle):
+    L = (4, 4, 4, 4)
+    P = oracle.random_momenta(L, 9)
+    assert np.abs(P + np.conj(np.swapaxes(P, -1, -2))).max() == 0.0
+    assert np.abs(np.trace(P, axis1=-2, axis2=-1)).max() < 1e-14
+    # density exp(tr P^2/2): 8 real degrees of freedom of unit variance per link
+    kin = -0.5 * np.trace(P @ P, axis1=-2, axis2=-1).real
+    assert abs(kin.mean() - 4.0) < 0.2
+    U = oracle.gauge_random(L, 3)
+    W = U.copy()
+    oracle.link_update(L, P, 0.05, W)
+    for k in (0, 100, 1000):
+        mu, s = divmod(k, oracle.volume(L))
+        assert np.abs(W[mu, s] - scipy.linalg.expm(0.05 * P[mu, s]) @ U[mu, s]).max() < 1e-14
