@@ -28,6 +28,10 @@ from . import stag as Q
 from ._lib import call
 
 
+# mdevolve's default 2MN parameter (listed in algorithms/integrator.nim:60-61)
+OMELYAN_LAMBDA = 0.1931833275037836
+
+
 def _vp(t):
     return ctypes.c_void_p(t.data_ptr())
 
@@ -65,7 +69,11 @@ def kinetic(P):
 
 
 class StaggeredHMC:
-    def __init__(self, lo, U, beta, mass, r2req_force=1e-20, r2req_action=1e-24, sloppy=Q.SloppyType.SloppyNone):
+    def __init__(self, lo, U, beta, mass, r2req_force=1e-20, r2req_action=1e-24, sloppy=Q.SloppyType.SloppyNone,
+                 integrator="leapfrog", lam=OMELYAN_LAMBDA):
+        if integrator not in ("leapfrog", "2MN"):
+            raise ValueError("integrator must be 'leapfrog' or '2MN'")
+        self.integrator, self.lam = integrator, float(lam)
         self.lo = lo
         self.U = U.contiguous()
         self.beta = float(beta)
@@ -129,18 +137,42 @@ class StaggeredHMC:
             P = P + (dt if i < nsteps - 1 else 0.5 * dt) * self.pdot()
         return P
 
-    def trajectory(self, seed, tau=1.0, nsteps=20, rng=None):
-        """One HMC trajectory with Metropolis; returns a dict of diagnostics."""
+    def omelyan2mn(self, P, tau, nsteps, lam=OMELYAN_LAMBDA):
+        """Omelyan's second-order minimum-norm integrator, steps of
+        V(lam dt) T(dt/2) V((1-2 lam) dt) T(dt/2) V(lam dt) with adjacent V
+        updates merged (mdevolve's 2MN, selected by the name "2MN[,lam]" in
+        algorithms/integrator.nim:32-34); lam = 0 is the position leapfrog."""
+        dt = tau / nsteps
+        if lam != 0.0:
+            P = P + (lam * dt) * self.pdot()
+        for i in range(nsteps):
+            link_update(self.lo, P, 0.5 * dt, self.U)
+            self._links_changed()
+            P = P + ((1.0 - 2.0 * lam) * dt) * self.pdot()
+            link_update(self.lo, P, 0.5 * dt, self.U)
+            self._links_changed()
+            if lam != 0.0:
+                P = P + ((2.0 * lam if i < nsteps - 1 else lam) * dt) * self.pdot()
+        return P
+
+    def integrate(self, P, tau, nsteps):
+        if self.integrator == "leapfrog":
+            return self.leapfrog(P, tau, nsteps)
+        return self.omelyan2mn(P, tau, nsteps, self.lam)
+
+    def trajectory(self, seed, tau=1.0, nsteps=20, rng=None, metropolis=True):
+        """One HMC trajectory, with a Metropolis test unless metropolis is False
+        (thermalization); returns a dict of diagnostics."""
         t0 = time.perf_counter()
         U0 = self.U.clone()
         P = random_momenta(self.lo, seed)
         self.heatbath(seed + 1)
         H0 = self.hamiltonian(P)
-        P = self.leapfrog(P, tau, nsteps)
+        P = self.integrate(P, tau, nsteps)
         H1 = self.hamiltonian(P)
         dH = H1 - H0
         u = (rng or np.random.default_rng(seed)).random()
-        accept = bool(u < np.exp(-dH))
+        accept = bool(u < np.exp(-dH)) if metropolis else True
         if not accept:
             self.U.copy_(U0)
             self._links_changed()
@@ -153,10 +185,169 @@ class StaggeredHMC:
         U0 = self.U.clone()
         P0 = random_momenta(self.lo, seed)
         self.heatbath(seed + 1)
-        P1 = self.leapfrog(P0.clone(), tau, nsteps)
-        P2 = self.leapfrog(-P1, tau, nsteps)
+        P1 = self.integrate(P0.clone(), tau, nsteps)
+        P2 = self.integrate(-P1, tau, nsteps)
         dU = float(torch.max(torch.abs(self.U - U0)))
         dP = float(torch.max(torch.abs(P2 + P0)))
         self.U.copy_(U0)
         self._links_changed()
         return dU, dP
+
+
+# ------------------------------------------------------------------ checks
+def _coords(L, idx):
+    """Coordinates of canonical site index idx (even sites first)."""
+    V = int(np.prod(L))
+    p = 1 if idx >= V // 2 else 0
+    for lex in (2 * idx - p * V, 2 * idx - p * V + 1):
+        c = [lex % L[0], (lex // L[0]) % L[1], (lex // (L[0] * L[1])) % L[2], lex // (L[0] * L[1] * L[2])]
+        if (sum(c) & 1) == p:
+            return c
+    raise ValueError(idx)
+
+
+def _index(L, c):
+    V = int(np.prod(L))
+    c = [int(c[k]) % L[k] for k in range(4)]
+    lex = c[0] + L[0] * (c[1] + L[1] * (c[2] + L[2] * c[3]))
+    return (lex + (sum(c) & 1) * V) // 2
+
+
+def _staples(L, U, mu, site):
+    """Sum A of the six staples of U_mu(site): sum over plaquettes through the link
+    of Re tr U_P = Re tr(U_mu(site) A)."""
+    c = _coords(L, site)
+
+    def at(nu, d):
+        y = list(c)
+        for k, v in d:
+            y[k] += v
+        return _index(L, y)
+
+    A = np.zeros((3, 3), complex)
+    for nu in range(4):
+        if nu == mu:
+            continue
+        m = lambda k, s: U[k, s].cpu().numpy()
+        A += m(nu, at(nu, [(mu, 1)])) @ m(mu, at(nu, [(nu, 1)])).conj().T @ m(nu, site).conj().T
+        A += (m(nu, at(nu, [(mu, 1), (nu, -1)])).conj().T @ m(mu, at(nu, [(nu, -1)])).conj().T
+              @ m(nu, at(nu, [(nu, -1)])))
+    return A
+
+
+def _su3_alg(rng):
+    a = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+    x = 0.5 * (a - a.conj().T)
+    return x - np.trace(x) / 3 * np.eye(3)
+
+
+def force_fd_check(lo, Uph, phi, mass, nlinks=64, seed=1, h=1e-5, r2req=1e-26, beta=None, U=None):
+    """Finite-difference check of the fermion force (and, with beta and plain links
+    U, of the Wilson gauge force) on nlinks seeded links.
+
+    For U_mu(x) -> exp(w X) U_mu(x), X a random su(3) element, compares
+    -1/2 Re tr(X F_mu(x)) with a central difference of the action.  The fermion
+    action difference uses the exact identity
+        S(U') - S(U) = -Y'^+ (M' - M) Y,   M = m^2 - D_eo D_oe,  Y = M^-1 phi,
+    so the comparison is limited by the solver residual and not by h; the gauge
+    action is linear in each link, so its difference is -(beta/3) Re tr((U'-U) A)
+    with A the staple sum.  Returns (worst relative error of the fermion force,
+    worst relative error of the gauge force or None, seconds)."""
+    import scipy.linalg
+
+    t0 = time.perf_counter()
+    L = lo.physGeom
+    s = Q.Staggered(lo, Uph)
+    sp = Q.SolverParams()
+    sp.r2req, sp.verbosity, sp.maxits = r2req, 0, 100000
+    psi = lo.ColorVector()
+    fphi = lo.ColorVector().set(phi)
+    Q.stagForceSolve(s, psi, fphi, mass, sp)
+    F = Q.fermForce(s, psi, device=Uph.device)
+    Fg = wilson_force(lo, U, beta) if beta is not None else None
+    Up = Uph.clone()
+    sp_ = Q.Staggered(lo, Up)
+    a, b, y = lo.ColorVector(), lo.ColorVector(), lo.ColorVector()
+    m2 = mass * mass
+    Q.stagD2ee(s.se, s.so, a, s.g, psi, m2)  # 4 M Y
+    rng = np.random.default_rng(seed)
+    worst_f, worst_g = 0.0, 0.0
+    for _ in range(nlinks):
+        mu, site = int(rng.integers(4)), int(rng.integers(lo.nSites))
+        X = _su3_alg(rng)
+        dS = []
+        for sg in (1.0, -1.0):
+            E = torch.from_numpy(scipy.linalg.expm(sg * h * X)).to(Uph.device)
+            Up[mu, site] = E @ Uph[mu, site]
+            sp_.update(Up)
+            Q.solveEE(sp_, y, fphi, mass, sp)  # y = (4 M')^-1 phi
+            Q.stagD2ee(sp_.se, sp_.so, b, sp_.g, psi, m2)  # 4 M' Y
+            b.axpby(-1.0, a, 1.0, "even")
+            dS.append(-y.redot(b, "even"))
+        Up[mu, site] = Uph[mu, site]
+        fd = (dS[0] - dS[1]) / (2 * h)
+        an = -0.5 * np.trace(X @ F[mu, site].cpu().numpy()).real
+        worst_f = max(worst_f, abs(fd - an) / max(abs(an), 1e-3))
+        if Fg is not None:
+            A = _staples(L, U, mu, site)
+            u = U[mu, site].cpu().numpy()
+            dU = (scipy.linalg.expm(h * X) - scipy.linalg.expm(-h * X)) @ u
+            fdg = -(beta / 3.0) * np.trace(dU @ A).real / (2 * h)
+            ang = -0.5 * np.trace(X @ Fg[mu, site].cpu().numpy()).real
+            worst_g = max(worst_g, abs(fdg - ang) / max(abs(ang), 1e-3))
+    torch.cuda.synchronize()
+    return worst_f, (worst_g if Fg is not None else None), time.perf_counter() - t0
+
+
+def main(argv=None):
+    """Plain-staggered HMC run in the manner of staghmc.nim (cold start, Metropolis),
+    one JSON line per trajectory; --fd-links N first runs force_fd_check."""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--lat", type=int, nargs=4, default=[8, 8, 8, 8])
+    ap.add_argument("--beta", type=float, default=5.6)
+    ap.add_argument("--mass", type=float, default=0.05)
+    ap.add_argument("--tau", type=float, default=1.0)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--trajs", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--frsq", type=float, default=1e-20, help="force solve r2req")
+    ap.add_argument("--arsq", type=float, default=1e-24, help="action solve r2req")
+    ap.add_argument("--sloppy", type=int, default=0, help="SloppyType of the force solves")
+    ap.add_argument("--fd-links", type=int, default=0)
+    ap.add_argument("--reversibility", action="store_true")
+    ap.add_argument("--integrator", default="leapfrog", help="leapfrog or 2MN[,lambda]")
+    ap.add_argument("--warmup", type=int, default=0, help="trajectories without Metropolis first")
+    args = ap.parse_args(argv)
+
+    from ._lib import init
+    init(0)
+    lo = Q.Layout(tuple(args.lat))
+    if args.fd_links:
+        U = G.random_gauge(lo, args.seed)
+        Uph = G.phase_bc(lo, U.clone())
+        phi = G.source(lo, args.seed + 1, stream=0, par=0)
+        wf, wg, sec = force_fd_check(lo, Uph, phi, args.mass, nlinks=args.fd_links, seed=args.seed,
+                                     beta=args.beta, U=U)
+        print(json.dumps({"check": "force_fd", "lat": args.lat, "mass": args.mass, "links": args.fd_links,
+                          "fermion_worst_rel": wf, "gauge_worst_rel": wg, "seconds": round(sec, 2)}), flush=True)
+    name, *rest = args.integrator.split(",")
+    h = StaggeredHMC(lo, G.unit_gauge(lo), args.beta, args.mass, r2req_force=args.frsq,
+                     r2req_action=args.arsq, sloppy=Q.SloppyType(args.sloppy), integrator=name,
+                     lam=float(rest[0]) if rest else OMELYAN_LAMBDA)
+    rng = np.random.default_rng(args.seed)
+    for n in range(args.warmup + args.trajs):
+        its0 = h.solver_its
+        warm = n < args.warmup
+        r = h.trajectory(1000 * (n + 1) + args.seed, args.tau, args.steps, rng=rng, metropolis=not warm)
+        r.update({"traj": n + 1 - args.warmup, "warmup": warm, "solver_its": h.solver_its - its0})
+        if args.reversibility:
+            dU, dP = h.reversibility(777 + n, args.tau, args.steps)
+            r.update({"rev_dU": dU, "rev_dP": dP})
+        print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main()

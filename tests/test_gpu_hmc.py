@@ -60,33 +60,42 @@ def test_momenta_and_link_update_match_oracle(oracle, gpu):
     assert np.abs(Ug.cpu().numpy() - U).max() < 1e-14
 
 
-def _hmc(gpu, L, mass=0.2, beta=5.6):
+def _hmc(gpu, L, mass=0.2, beta=5.6, integrator="leapfrog"):
     from qex_amd import hmc, gauge
     lo = gpu.Layout(L)
     U = gauge.unit_gauge(lo, device="cuda")
-    return hmc.StaggeredHMC(lo, U, beta, mass, r2req_force=1e-26, r2req_action=1e-26)
+    return hmc.StaggeredHMC(lo, U, beta, mass, r2req_force=1e-26, r2req_action=1e-26, integrator=integrator)
 
 
-def test_leapfrog_dH_scales_as_dt2(gpu):
+def _dH(gpu, L, integrator, n):
     import torch
+    from qex_amd import hmc
+    h = _hmc(gpu, L, integrator=integrator)
+    P = hmc.random_momenta(h.lo, 5)
+    h.heatbath(6)
+    H0 = h.hamiltonian(P)
+    P = h.integrate(P, 0.5, n)
+    torch.cuda.synchronize()
+    return h.hamiltonian(P) - H0
+
+
+@pytest.mark.parametrize("integrator", ["leapfrog", "2MN"])
+def test_dH_scales_as_dt2(gpu, integrator):
     L = (8, 8, 8, 8)
-    dH = []
-    for n in (6, 12):
-        h = _hmc(gpu, L)
-        from qex_amd import hmc
-        P = hmc.random_momenta(h.lo, 5)
-        h.heatbath(6)
-        H0 = h.hamiltonian(P)
-        P = h.leapfrog(P, 0.5, n)
-        dH.append(h.hamiltonian(P) - H0)
-        torch.cuda.synchronize()
+    dH = [_dH(gpu, L, integrator, n) for n in (6, 12)]
     ratio = dH[0] / dH[1]
-    assert abs(dH[0]) > 1e-4, dH
+    assert abs(dH[0]) > 1e-5, dH
     assert 3.2 < ratio < 4.8, dH
 
 
-def test_leapfrog_reversible(gpu):
-    h = _hmc(gpu, (8, 8, 8, 8))
+def test_omelyan_beats_leapfrog(gpu):
+    L = (8, 8, 8, 8)
+    assert abs(_dH(gpu, L, "2MN", 6)) < 0.5 * abs(_dH(gpu, L, "leapfrog", 6))
+
+
+@pytest.mark.parametrize("integrator", ["leapfrog", "2MN"])
+def test_integrator_reversible(gpu, integrator):
+    h = _hmc(gpu, (8, 8, 8, 8), integrator=integrator)
     dU, dP = h.reversibility(11, tau=0.5, nsteps=8)
     assert dU < 1e-10 and dP < 1e-9, (dU, dP)
 
@@ -151,3 +160,16 @@ def test_autograd_action_gradient(oracle, gpu):
         fd = (Sp - Sm) / (2 * h)
         an = float(torch.sum((dU.conj() * grad).real))
         assert abs(fd - an) < 1e-6 * max(1.0, abs(an)), (trial, fd, an)
+
+
+def test_force_fd_check_exact_identity(gpu):
+    """Fermion and gauge forces against finite differences on 16 seeded links
+    (the fermion side through S(U') - S(U) = -Y'^+ (M' - M) Y, limited by the
+    solver residual only)."""
+    from qex_amd import gauge, hmc
+    lo = gpu.Layout((8, 8, 8, 8))
+    U = gauge.random_gauge(lo, 5)
+    Uph = gauge.phase_bc(lo, U.clone())
+    phi = gauge.source(lo, 6, stream=0, par=0)
+    wf, wg, _ = hmc.force_fd_check(lo, Uph, phi, 0.1, nlinks=16, seed=3, beta=5.6, U=U)
+    assert wf < 1e-8 and wg < 1e-8, (wf, wg)
