@@ -33,6 +33,8 @@ SEED = 20240601
 HOP_REALS = 8 * 18 + 2 * 6
 BYTES_PER_REAL = {0: 8, 1: 4, 2: 2, 3: 2, 4: 2}
 PREC_NAME = {0: "fp64", 1: "fp32", 2: "fp16", 3: "bf16", 4: "q16links_fp16vec"}
+# HISQ form: 8 fat + 8 long links + 2 vectors
+HISQ_REALS = 16 * 18 + 2 * 6
 # flops per output site of one hop: 8 complex 3x3 mat-vec (66) + 7*6 adds = 570
 HOP_FLOPS = 570
 
@@ -47,6 +49,8 @@ def parse():
     ap.add_argument("--no-table", action="store_true", help="skip the hop table")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--cg-only", default="", help="comma-separated CG variants to run")
+    ap.add_argument("--hisq-cg", action="store_true",
+                    help="add the HISQ-form 32^3x64 m=0.005 storage-precision study")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -102,6 +106,86 @@ def cpu_baseline(budget_s):
         "hop_gflops": round(HOP_FLOPS * vh / hop_s / 1e9, 3),
         "cg_ms_per_iteration": round(cg_it_s * 1e3, 3),
     }
+
+
+def _solve_stats(Q, torch, s, x, b, mass, r2req, maxits, sloppy, sloppy_links=None, recon=18):
+    sp = Q.SolverParams()
+    sp.r2req, sp.maxits, sp.verbosity = r2req, maxits, 0
+    sp.sloppySolve, sp.sloppyLinks, sp.recon = sloppy, sloppy_links, recon
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Q.solveEE(s, x, b, mass, sp)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"seconds": round(dt, 5), "iterations": sp.iterations, "reliable_updates": sp.reliableUpdates,
+            "true_rel_resid": float(sp.r2) ** 0.5, "qex_gflops": round(1e-9 * sp.flops / dt, 1)}
+
+
+def hisq_study():
+    """HISQ-form operator (W = 0.8U + 0.05G fat, L = -(1/24)UUU long links) at
+    32^3x64, m = 0.005: one-parity hop bandwidth per storage precision, and CG
+    to |r|/|b| <= 1e-10 with FP64, FP32, FP16, BF16 and 16-bit fixed-point link
+    storage (FP32 arithmetic, FP64 reliable updates), each with its FP64 true
+    residual and its solution error against the FP64 solve (and against an FP64
+    solve to 1e-13)."""
+    import torch
+    import qex_amd as Q
+    from qex_amd import gauge
+    L, m = (32, 32, 32, 64), 0.005
+    lo = Q.Layout(L)
+    U = gauge.random_gauge(lo, SEED)
+    gauge.phase_bc(lo, U)
+    g8 = gauge.hisq_links(lo, SEED + 1, U)
+    del U
+    s = Q.Staggered(lo, g8)
+    del g8
+    torch.cuda.empty_cache()
+    out = {"lattice": list(L), "mass": m, "r2req": R2REQ, "hop": {}}
+    for prec in (0, 1, 2, 3, 4):
+        Q.time_hop(s, prec, 5)
+        ms = Q.time_hop(s, prec, 100)
+        out["hop"][PREC_NAME[prec]] = {"ms": round(ms, 5),
+                                       "GB_s": round(HISQ_REALS * BYTES_PER_REAL[prec] * lo.nEven / (ms * 1e-3) / 1e9, 1)}
+    b = lo.ColorVector().set(gauge.source(lo, 7, 0, 0))
+    xt = lo.ColorVector()
+    S = Q.SloppyType
+    tight = _solve_stats(Q, torch, s, xt, b, m, 1e-26, 200000, S.SloppyNone)
+    out["fp64_tight"] = tight
+    x64 = lo.ColorVector()
+    x = lo.ColorVector()
+    e = lo.ColorVector()
+    n64 = None
+    cg = {}
+    variants = [("fp64", S.SloppyNone, None), ("mixed_fp32", S.SloppySingle, None),
+                ("mixed_fp16", S.SloppyHalf, None), ("mixed_bf16", S.SloppyBF16, None),
+                ("mixed_q16links_fp16vec", S.SloppyHalf, S.SloppyHalfFixed),
+                ("mixed_fp16links_fp32vec", S.SloppySingle, S.SloppyHalf),
+                ("mixed_bf16links_fp32vec", S.SloppySingle, S.SloppyBF16)]
+    nt = xt.norm2("even")
+    for name, sl, sll in variants:
+        best = None
+        for rep in range(2):  # first call packs the sloppy links
+            dst = x64 if name == "fp64" else x
+            r = _solve_stats(Q, torch, s, dst, b, m, R2REQ, 100000, sl, sll)
+            if best is None or r["seconds"] < best["seconds"]:
+                best = r
+            if r["seconds"] > 10.0:
+                break
+        if name == "fp64":
+            n64 = x64.norm2("even")
+            dst = x64
+        e.copy_from(dst, "even")
+        e.axpby(-1.0, x64, 1.0, "even")
+        best["err_vs_fp64"] = (e.norm2("even") / n64) ** 0.5
+        e.copy_from(dst, "even")
+        e.axpby(-1.0, xt, 1.0, "even")
+        best["err_vs_fp64_tight"] = (e.norm2("even") / nt) ** 0.5
+        cg[name] = best
+    base = cg["fp64"]["seconds"]
+    for name in cg:
+        cg[name]["speedup_vs_fp64"] = round(base / cg[name]["seconds"], 3)
+    out["cg"] = cg
+    return out
 
 
 def main():
@@ -216,7 +300,7 @@ def main():
             for rep in range(2):  # first call packs the sloppy links
                 sp = Q.SolverParams()
                 sp.r2req = R2REQ
-                sp.maxits = 20000
+                sp.maxits = 100000
                 sp.verbosity = 0
                 sp.sloppySolve = sl
                 sp.sloppyLinks = sll
@@ -241,6 +325,11 @@ def main():
                 best = min(conv, key=lambda k: conv[k]["seconds"])
                 out["cg_best_mixed"] = {"variant": best, "speedup_vs_fp64": cg[best]["speedup_vs_fp64"]}
         out["cg"] = cg
+
+    if args.hisq_cg:
+        del s
+        torch.cuda.empty_cache()
+        out["hisq_cg"] = hisq_study()
 
     # ---- CPU baseline (oracle on host cores) ----
     if not args.no_cpu:

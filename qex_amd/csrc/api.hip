@@ -312,6 +312,8 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
     launch_fill(w->Aps, b32, g_stream);
     resid64(l, par, x, b, w->r64, S, 2, w->t64);  // epi_reliable: sigma, r2, done
     launch_rs_from_true(g, w->rs, w->r64, S, g_stream);
+    // best FP64 iterate so far (the FP64-only p vector is free in mixed mode)
+    HIPCHK(hipMemcpyAsync(w->p64, x, vec64_bytes(lat), hipMemcpyDeviceToDevice, g_stream));
     check_launch();
     pull(Sh, S);
   }
@@ -328,6 +330,17 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
       }
       check_launch();
       pull(Sh, S);
+      if (mixed && Sh->rollback) {
+        // the sloppy recursion diverged: back to the best iterate, whose true
+        // residual epi_reliable accepts again, restarting p = r
+        HIPCHK(hipMemcpyAsync(x, w->p64, vec64_bytes(lat), hipMemcpyDeviceToDevice, g_stream));
+        resid64(l, par, x, b, w->r64, S, 2, w->t64);
+        launch_rs_from_true(g, w->rs, w->r64, S, g_stream);
+        check_launch();
+        pull(Sh, S);
+      } else if (mixed && Sh->improved) {
+        HIPCHK(hipMemcpyAsync(w->p64, x, vec64_bytes(lat), hipMemcpyDeviceToDevice, g_stream));
+      }
     }
   }
   *its_out = Sh->its;

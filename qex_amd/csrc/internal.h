@@ -25,8 +25,20 @@ struct Scal {
   double tmp[4];    // generic reduction results
   int its, maxits, since, relmax;
   int done, need_rel, first, nrel;
-  int mixed, pad0, pad1, pad2;
+  int mixed;
+  int restart;       // mixed: next reliable update restarts the recursion (p = r)
+  int rollback;      // mixed: the true residual grew; host returns x to the last good iterate
+  int resinc;        // mixed: reliable updates whose true residual grew
+  int improved;      // mixed: the last reliable update reached a new best residual
+  int pad0;
+  double r2rel_last; // mixed: best true residual^2 at a reliable update
 };
+
+// mixed mode gives up after this many reliable updates that failed to reduce
+// the true residual
+#define QH_MAX_RES_INCREASE 20
+// growth of the true residual^2 over the best one that counts as divergence
+#define QH_RES_GROWTH 10.0
 
 #define QH_MAXSHIFT 64
 struct ShiftScal {

@@ -186,6 +186,7 @@ __global__ void k_scal_init(Scal *S, int mode) {
   if (mode == 0) {  // start of a CG solve: fresh state (cg.nim:21-27)
     S->its = 0; S->since = 0; S->done = 0; S->need_rel = 0; S->first = 1; S->nrel = 0;
     S->r2prev = 1.0; S->beta = 0.0; S->alpha = 0.0; S->pscale = 1.0; S->sigma = 1.0;
+    S->restart = 0; S->rollback = 0; S->resinc = 0; S->improved = 0; S->r2rel_last = __builtin_inf();
   } else if (mode == 1) {  // force a reliable update now
     S->need_rel = 1;
   }

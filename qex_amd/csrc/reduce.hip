@@ -8,8 +8,25 @@ namespace qh {
 
 // FP64 true residual r = b - A x just computed (reliable update): restart the
 // sloppy recursion on sigma*r with beta = |r_true|^2/|r_prev|^2 (unscaled).
+// A true residual^2 more than QH_RES_GROWTH times the best one so far means
+// the sloppy recursion diverged: ask the host to return x to the best iterate
+// (the next call sees that residual again and accepts it), then restart the
+// recursion from p = r.  After QH_MAX_RES_INCREASE such events the solve stops
+// and reports the residual it has.  (CG residual norms are not monotone, so
+// smaller increases are accepted.)
 __device__ void epi_reliable(Scal *S, double s, double nreal) {
   S->r2true = s;
+  if (!(s <= QH_RES_GROWTH * S->r2rel_last)) {
+    S->rollback = 1;
+    S->restart = 1;
+    S->resinc += 1;
+    S->need_rel = 0;
+    S->done = S->resinc > QH_MAX_RES_INCREASE || S->its >= S->maxits;
+    return;
+  }
+  S->rollback = 0;
+  S->improved = (s <= S->r2rel_last);
+  if (S->improved) S->r2rel_last = s;
   const double sig_old = S->sigma;
   const double sig_new = (s > 0.0) ? sqrt(nreal / s) : 1.0;
   const double r2old_unscaled = S->r2prev / (sig_old * sig_old);
@@ -23,7 +40,12 @@ __device__ void epi_reliable(Scal *S, double s, double nreal) {
   S->since = 0;
   S->need_rel = 0;
   S->nrel += 1;
-  S->done = (s <= S->r2stop) || (S->its >= S->maxits);
+  if (S->restart) {
+    S->first = 1;
+    S->beta = 0.0;
+    S->restart = 0;
+  }
+  S->done = (s <= S->r2stop) || (S->its >= S->maxits) || (S->resinc > QH_MAX_RES_INCREASE);
 }
 
 // After iteration n of the fused CG (solvers/cg.nim:174-214 recurrence):
@@ -32,6 +54,20 @@ __device__ void epi_reliable(Scal *S, double s, double nreal) {
 __device__ void epi_cg2(Scal *S, double pAp, double rAp, double ApAp, double rr) {
   const double alpha = rr / pAp;
   double r2n = rr - 2.0 * alpha * rAp + alpha * alpha * ApAp;
+  if (S->mixed && !(pAp > 0.0 && isfinite(r2n))) {
+    // breakdown of the sloppy recursion (rounding made p.Ap non-positive):
+    // drop this step and restart from the true residual
+    S->r2prev = rr;
+    S->alpha = 0.0;
+    S->r2 = rr;
+    S->first = 0;
+    S->pscale = 1.0;
+    S->its += 1;
+    S->since += 1;
+    S->restart = 1;
+    S->need_rel = 1;
+    return;
+  }
   if (!(r2n > 0.0)) r2n = 0.0;
   S->r2prev = rr;
   S->pAp = pAp;

@@ -234,3 +234,30 @@ def test_q16_link_scale_hisq(oracle, gpu):
     assert sp.r2 <= 1e-20
     r = b - O.stagD2xx(L, g, 0, fx.get(), m * m)
     assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20
+
+
+@pytest.mark.parametrize("maxits", [400, 100000])
+def test_bf16_small_mass_hisq_stays_finite_and_converges(oracle, gpu, maxits):
+    """BF16 links and vectors perturb the operator at ~4e-3; at small mass the
+    sloppy recursion can break down.  The solver must never return a grown or
+    non-finite residual: with few iterations the true residual is below the
+    starting one, with enough it meets the FP64 request."""
+    O = oracle
+    L = (8, 8, 8, 8)
+    m = 0.005
+    g = links(O, L, True)
+    b = O.source_gauss(L, 41, 0, 0)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    fx = lo.ColorVector()
+    sp = gpu.SolverParams()
+    sp.r2req, sp.verbosity, sp.sloppySolve, sp.maxits = 1e-20, 0, 3, maxits
+    gpu.solveEE(s, fx, fb, m, sp)
+    r = b - O.stagD2xx(L, g, 0, fx.get(), m * m)
+    r2 = O.norm2(r[: lo.nEven]) / O.norm2(b)
+    assert np.isfinite(r2) and abs(r2 - sp.r2) <= 1e-6 * r2 + 1e-30
+    if maxits < 1000:
+        assert r2 < 1.0
+    else:
+        assert r2 <= 1.0001e-20
