@@ -134,6 +134,32 @@ int qexhip_solve_oo_host(qexhip_links *l, double *r, const double *t, double m, 
 int qexhip_solve_multishift(qexhip_links *l, int par, qexhip_field *const *xs, const qexhip_field *b, double m,
                             const double *sigmas, int nshift, qexhip_solver_params *sp);
 
+/* ---- batched solves (no single QEX counterpart: each system is one QEX solve,
+        run together on one GPU so that one launch per solver stage serves all
+        of them).  flags: QEXHIP_BATCH_GRAPH replays the iterations from a HIP
+        graph; QEXHIP_BATCH_TUNE picks the launch shape by timing candidates
+        (cached per shape); QEXHIP_BATCH_NOFREEZE keeps updating shifted
+        systems that already converged (QEX's cgm.nim behaviour). ---- */
+#define QEXHIP_BATCH_GRAPH 1
+#define QEXHIP_BATCH_TUNE 2
+#define QEXHIP_BATCH_NOFREEZE 4
+/* n independent FP64 CG solves A_k x_k = b_k, A_k = 4(m_k^2 - D_pq D_qp) with
+   links[k] (lattices may differ; all plain or all HISQ form), each as
+   qexhip_solve_xx's inner CG from x = 0 (solvers/cg.nim:55-272); sp[k] gives
+   r2req and maxits and receives iterations and the FP64 true residual. */
+int qexhip_solve_batch(int n, qexhip_links *const *links, int par, qexhip_field *const *x,
+                       qexhip_field *const *b, const double *masses, qexhip_solver_params *sp, int flags);
+/* multi-shift CG (src/solvers/cgm.nim:68-246) for nrhs right-hand sides on one
+   set of links: (A + sigma_k) x[j*nshift+k] = b[j], sigma_0 treated as 0; each
+   link load serves up to 8 right-hand sides.  sp[j] per right-hand side. */
+int qexhip_solve_multishift_batch(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs,
+                                  qexhip_field *const *b, double m, const double *sigmas, int nshift,
+                                  qexhip_solver_params *sp, int flags);
+/* launch-shape autotuner (no QEX counterpart): on/off (default on unless
+   QEXHIP_TUNE=0 or QEXHIP_BLOCK is set), and a text table of the shapes chosen */
+int qexhip_set_autotune(int on);
+int qexhip_autotune_report(char *buf, int len);
+
 /* ---- fermion force (src/hmc/hmcAction.nim:669-792) ---- */
 /* stagForceSolve (hmcAction.nim:669-721): psi_e = (m^2 - D_eo D_oe)^-1 phi_e,
    psi_o = (2D)_oe psi_e */
@@ -144,7 +170,7 @@ int qexhip_force_solve(qexhip_links *l, qexhip_field *psi, const qexhip_field *p
 int qexhip_force(qexhip_links *l, const qexhip_field *psi, double *F, int F_on_device, int outer_only);
 
 /* ---- gauge field for HMC (FP64 canonical arrays U[4][V][3][3] in device
-        memory; the pieces config-5's leapfrog needs around the fermion force) ---- */
+        memory; the pieces a staggered HMC needs around the fermion force) ---- */
 /* sum_{x,mu<nu} Re tr U_P; QEX's Wilson action is -(beta/3) times this
    (gauge/gaugeAction.nim:397-427) */
 int qexhip_plaquette(qexhip_lattice *lat, const double *U_dev, double *sum);

@@ -26,7 +26,11 @@ EXPORTS = [
     "qexhip_gen_gauge", "qexhip_gen_source", "qexhip_gen_phase_bc", "qexhip_gen_hisq",
     "qexhip_time_hop", "qexhip_links_recon_ok",
     "qexhip_plaquette", "qexhip_wilson_force", "qexhip_random_momenta", "qexhip_link_update",
+    "qexhip_solve_batch", "qexhip_solve_multishift_batch", "qexhip_set_autotune", "qexhip_autotune_report",
 ]
+
+# qexhip_solve_batch / qexhip_solve_multishift_batch flags
+BATCH_GRAPH, BATCH_TUNE, BATCH_NOFREEZE = 1, 2, 4
 
 
 class SolverParamsC(ctypes.Structure):
@@ -88,6 +92,9 @@ def load(path=LIB_PATH):
         "qexhip_time_hop": [vp, I, I, I, dp], "qexhip_links_recon_ok": [vp, ip],
         "qexhip_plaquette": [vp, vp, dp], "qexhip_wilson_force": [vp, vp, D, vp],
         "qexhip_random_momenta": [vp, U64, vp], "qexhip_link_update": [vp, vp, D, vp],
+        "qexhip_solve_batch": [I, vpp, I, vpp, vpp, dp, spp, I],
+        "qexhip_solve_multishift_batch": [vp, I, I, vpp, vpp, D, dp, I, spp, I],
+        "qexhip_set_autotune": [I], "qexhip_autotune_report": [ctypes.c_char_p, I],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

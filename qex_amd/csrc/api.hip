@@ -7,6 +7,10 @@
 #include <stdlib.h>
 #include <string.h>
 #include <chrono>
+#include <algorithm>
+#include <climits>
+#include <functional>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -23,6 +27,8 @@ hipStream_t g_stream = nullptr;
 int g_device = -1;
 int g_block = 256;
 int g_remap = 1;
+bool g_block_forced = false;  // QEXHIP_BLOCK set: no autotuning
+bool g_autotune = true;
 
 #define HIPCHK(x)                                                                                  \
   do {                                                                                             \
@@ -211,6 +217,60 @@ void pull(Scal *Sh, const Scal *S) {
 
 // One hop launch: out = ca*A + cb*B + chop*(2D)in on output parity par, with
 // links and vectors in precision prec (arithmetic FP64 for FP64, else FP32).
+// ---------------------------------------------------------------- autotuner
+// Launch shape (block size, XCD-aware block order) per kernel family, lattice
+// and storage format, chosen by timing every candidate with HIP events on
+// scratch data the first time a shape is needed, and kept for the process.
+// Reductions sum block partials, so the chosen block size fixes the summation
+// order: results are reproducible within a process and across processes that
+// choose the same shape (set QEXHIP_BLOCK to pin it).
+struct Tuned {
+  int block, remap;
+  double us;
+};
+std::map<std::string, Tuned> g_tune;
+
+std::string geom_key(const Geom &g) {
+  char b[96];
+  snprintf(b, sizeof b, "%dx%dx%dx%d", g.L[0], g.L[1], g.L[2], g.L[3]);
+  return b;
+}
+
+double time_us(const std::function<void()> &f, int warm, int reps) {
+  for (int i = 0; i < warm; i++) f();
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, g_stream));
+  for (int i = 0; i < reps; i++) f();
+  HIPCHK(hipEventRecord(e1, g_stream));
+  HIPCHK(hipEventSynchronize(e1));
+  check_launch();
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 1e3 * (double)t / reps;
+}
+
+// f(block, remap) runs the kernel sequence once
+Tuned tune(const std::string &key, const std::function<void(int, int)> &f, bool with_remap = true) {
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) return it->second;
+  if (g_block_forced || !g_autotune) return Tuned{g_block, g_remap, 0.0};
+  Tuned best{256, 1, 1e300};
+  for (int block : {64, 128, 256})
+    for (int remap : {1, 0}) {
+      if (!with_remap && remap == 0) continue;
+      const double us = time_us([&] { f(block, remap); }, 2, 8);
+      if (us < best.us) best = Tuned{block, remap, us};
+    }
+  g_tune[key] = best;
+  return best;
+}
+
+Tuned tuned_hop(qexhip_links *l, int prec, int rc);
+
 void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const void *in, void *out, const void *a,
          const void *b, double ca, double cb, double chop, int ca_m2, int pred, Scal *S) {
   HopArgs h;
@@ -228,12 +288,75 @@ void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const v
   h.ca_m2 = ca_m2;
   h.par = par;
   h.pred = pred;
-  h.remap = g_remap;
+  const Tuned tu = tuned_hop(l, prec, rc);
+  h.remap = tu.remap;
   h.S = S;
   h.red = l->lat->red;
   const int pv = (prec == PQ16) ? P16 : prec;  // 16-bit fixed-point links go with FP16 vectors
   HopKey k{prec, rc, pv, pv, pv};
-  launch_hop(k, l->nl, mode, redm, h, g_stream, g_block);
+  launch_hop(k, l->nl, mode, redm, h, g_stream, tu.block);
+}
+
+Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
+  char kb[64];
+  snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d ", l->nl, prec, rc);
+  const std::string key = kb + geom_key(l->lat->g);
+  if (g_tune.count(key) || g_block_forced || !g_autotune) return tune(key, nullptr);
+  const size_t vb = vec64_bytes(l->lat);
+  char *in = dalloc<char>(vb), *out = dalloc<char>(vb);
+  launch_fill(in, vb, g_stream);
+  HopArgs h{};
+  h.g = l->lat->g;
+  h.links = links_store(l, prec, rc);
+  h.lkblk = lkblk_bytes(l, prec, rc);
+  for (int k = 0; k < 8; k++) h.qs[k] = l->qs[k];
+  h.in = in;
+  h.out = out;
+  h.chop = 1.0;
+  h.red = l->lat->red;
+  const int pv = (prec == PQ16) ? P16 : prec;
+  const HopKey hk{prec, rc, pv, pv, pv};
+  const Tuned t = tune(key, [&](int block, int remap) {
+    h.remap = remap;
+    launch_hop(hk, l->nl, 0, 0, h, g_stream, block);
+  });
+  HIPCHK(hipStreamSynchronize(g_stream));
+  (void)hipFree(in);
+  (void)hipFree(out);
+  return t;
+}
+
+// fused CG iteration: scratch vectors and device scalars that never converge
+Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, const CgArgs &real) {
+  char kb[64];
+  snprintf(kb, sizeof kb, "cg nl=%d links=%d rc=%d vec=%d ", l->nl, pl, rc, ps);
+  const std::string key = kb + geom_key(l->lat->g);
+  if (g_tune.count(key) || g_block_forced || !g_autotune) return tune(key, nullptr);
+  const size_t vb = vec64_bytes(l->lat);
+  CgArgs a = real;
+  void *v[5];
+  for (int k = 0; k < 5; k++) {
+    v[k] = dalloc<char>(vb);
+    launch_fill(v[k], vb, g_stream);
+  }
+  a.x = v[0]; a.r = v[1]; a.p = v[2]; a.Ap = v[3]; a.t = v[4];
+  Scal *S = dalloc<Scal>(sizeof(Scal));
+  Scal h;
+  memset(&h, 0, sizeof h);
+  h.r2stop = -1.0;
+  h.r2stop_s = -1.0;
+  h.maxits = INT_MAX;
+  h.sigma = h.pscale = 1.0;
+  HIPCHK(hipMemcpyAsync(S, &h, sizeof h, hipMemcpyHostToDevice, g_stream));
+  a.S = S;
+  const Tuned t = tune(key, [&](int block, int remap) {
+    a.remap = remap;
+    launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, block);
+  });
+  HIPCHK(hipStreamSynchronize(g_stream));
+  for (int k = 0; k < 5; k++) (void)hipFree(v[k]);
+  (void)hipFree(S);
+  return t;
 }
 
 // A = 4(m2 - D_pq D_qp) applied to x (FP64, own parity par): out = cx*x + cb*b - sgn...
@@ -292,9 +415,10 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   a.lkblk = lkblk_bytes(l, pl, rc);
   for (int k = 0; k < 8; k++) a.qs[k] = l->qs[k];
   a.par = par;
-  a.remap = g_remap;
   a.S = S;
   a.red = lat->red;
+  const Tuned tu = tuned_cg(l, pl, rc, ps, a);
+  a.remap = tu.remap;
   if (!mixed) {
     a.x = x; a.r = w->r64; a.p = w->p64; a.Ap = w->Ap64; a.t = w->t64;
     launch_fill(w->Ap64, vec64_bytes(lat), g_stream);
@@ -318,7 +442,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
     pull(Sh, S);
   }
   while (!Sh->done) {
-    for (int k = 0; k < kChunk; k++) launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, g_block);
+    for (int k = 0; k < kChunk; k++) launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, tu.block);
     check_launch();
     pull(Sh, S);
     if (Sh->done || Sh->need_rel) {
@@ -497,6 +621,422 @@ void check_same_lat(const qexhip_links *l, const qexhip_field *f) {
   require(l && f && l->lat == f->lat, "field and links belong to different lattices");
 }
 
+// ------------------------------------------------------------ batched CG
+// Independent FP64 CG solves (the fused iteration of cg_solve) on systems of
+// different lattices, links and masses, one launch per stage for all of them
+// (batch.hip).  The workspace (vectors, device scalars, block table, graph)
+// is kept for the next call with the same links and parity.
+struct BatchWs {
+  std::vector<qexhip_links *> links;
+  int par = 0, nl = 4, n = 0;
+  std::vector<double *> x, r, p, Ap, t, b;
+  Scal *S = nullptr, *Sh = nullptr, *Sx = nullptr;
+  double *partials = nullptr;
+  long maxblocks = 0, nblocks = 0;
+  BatchItem *T = nullptr, *Tx = nullptr;  // device tables: solve, tuning scratch
+  int block = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~BatchWs() {
+    for (auto *v : {&x, &r, &p, &Ap, &t, &b})
+      for (double *q : *v) (void)hipFree(q);
+    (void)hipFree(S);
+    (void)hipHostFree(Sh);
+    (void)hipFree(Sx);
+    (void)hipFree(partials);
+    (void)hipFree(T);
+    (void)hipFree(Tx);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+BatchWs *g_batch = nullptr;
+
+void drop_batch() {
+  if (g_batch) {
+    HIPCHK(hipStreamSynchronize(g_stream));
+    delete g_batch;
+    g_batch = nullptr;
+  }
+}
+
+// block table for block size `block`, scalars S (device array); returns the
+// number of blocks of the batched grid
+long batch_table(BatchWs *w, int block, Scal *S, BatchItem *dst) {
+  std::vector<BatchItem> T(w->n);
+  long b0 = 0;
+  for (int k = 0; k < w->n; k++) {
+    qexhip_links *l = w->links[k];
+    const Geom &g = l->lat->g;
+    BatchItem &it = T[k];
+    memset(&it, 0, sizeof it);
+    it.a.g = g;
+    it.a.links = links_store(l, P64, 18);
+    it.a.lkblk = lkblk_bytes(l, P64, 18);
+    it.a.par = w->par;
+    it.a.remap = 0;
+    it.a.x = w->x[k]; it.a.r = w->r[k]; it.a.p = w->p[k]; it.a.Ap = w->Ap[k]; it.a.t = w->t[k];
+    it.a.S = S + k;
+    it.a.red = RedWs{w->partials, (int)w->maxblocks};
+    it.blk0 = b0;
+    it.nblk = (g.Vh + block - 1) / block;
+    const RedWs slice{w->partials + b0, (int)w->maxblocks};
+    it.fin = FinArgs{slice, (int)it.nblk, S + k, nullptr, 0, 0.0, 1};
+    it.finf = FinArgs{slice, (int)it.nblk, S + k, nullptr, 0, 0.0, 0};
+    b0 += it.nblk;
+  }
+  HIPCHK(hipMemcpyAsync(dst, T.data(), sizeof(BatchItem) * w->n, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return b0;
+}
+
+BatchWs *get_batch(int n, qexhip_links *const *ls, int par) {
+  if (g_batch && g_batch->n == n && g_batch->par == par &&
+      std::equal(ls, ls + n, g_batch->links.begin()))
+    return g_batch;
+  drop_batch();
+  BatchWs *w = new BatchWs();
+  g_batch = w;
+  w->n = n;
+  w->par = par;
+  w->links.assign(ls, ls + n);
+  w->nl = ls[0]->nl;
+  long mb = 0;
+  for (int k = 0; k < n; k++) {
+    const size_t vb = vec64_bytes(ls[k]->lat);
+    for (auto *v : {&w->x, &w->r, &w->p, &w->Ap, &w->t, &w->b}) {
+      v->push_back(dalloc<double>(vb));
+      launch_fill(v->back(), vb, g_stream);
+    }
+    mb += (ls[k]->lat->g.Vh + 63) / 64;
+  }
+  w->maxblocks = mb;
+  w->partials = dalloc<double>(sizeof(double) * 4 * mb);
+  w->S = dalloc<Scal>(sizeof(Scal) * n);
+  w->Sx = dalloc<Scal>(sizeof(Scal) * n);
+  HIPCHK(hipHostMalloc((void **)&w->Sh, sizeof(Scal) * n, hipHostMallocDefault));
+  w->T = dalloc<BatchItem>(sizeof(BatchItem) * n);
+  w->Tx = dalloc<BatchItem>(sizeof(BatchItem) * n);
+  return w;
+}
+
+void batch_chunk(BatchWs *w, const BatchItem *T) {
+  for (int k = 0; k < kChunk; k++) launch_bcg_iter(T, w->n, w->nblocks, w->nl, w->block, g_stream);
+  launch_bcg_flush(T, w->n, w->nblocks, w->block, g_stream);
+}
+
+void batch_shape(BatchWs *w, bool autotune) {
+  int block = 256;
+  if (autotune && !g_block_forced) {
+    std::string key = "batch-cg nl=" + std::to_string(w->nl);
+    for (auto *l : w->links) key += " " + geom_key(l->lat->g);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      block = it->second.block;
+    } else {
+      // scratch scalars that never converge
+      std::vector<Scal> h(w->n);
+      memset(h.data(), 0, sizeof(Scal) * w->n);
+      for (auto &q : h) { q.r2stop = -1.0; q.maxits = INT_MAX; q.sigma = q.pscale = 1.0; q.flushed = 1; }
+      HIPCHK(hipMemcpyAsync(w->Sx, h.data(), sizeof(Scal) * w->n, hipMemcpyHostToDevice, g_stream));
+      Tuned best{256, 0, 1e300};
+      for (int b : {64, 128, 256}) {
+        const long nb = batch_table(w, b, w->Sx, w->Tx);
+        const double us = time_us([&] { launch_bcg_iter(w->Tx, w->n, nb, w->nl, b, g_stream); }, 2, 8);
+        if (us < best.us) best = Tuned{b, 0, us};
+      }
+      g_tune[key] = best;
+      block = best.block;
+    }
+  } else if (g_block_forced) {
+    block = g_block;
+  }
+  if (block != w->block || !w->exec) {
+    w->block = block;
+    w->nblocks = batch_table(w, block, w->S, w->T);
+    if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
+    if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
+  }
+}
+
+void solve_batch(int n, qexhip_links *const *ls, int par, qexhip_field *const *xf, qexhip_field *const *bf,
+                 const double *ms, qexhip_solver_params *sp, int flags) {
+  auto t0 = std::chrono::steady_clock::now();
+  BatchWs *w = get_batch(n, ls, par);
+  batch_shape(w, (flags & QEXHIP_BATCH_TUNE) != 0);
+  const bool graph = (flags & QEXHIP_BATCH_GRAPH) != 0;
+  if (graph && !w->exec) {
+    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    batch_chunk(w, w->T);
+    HIPCHK(hipStreamEndCapture(g_stream, &w->graph));
+    HIPCHK(hipGraphInstantiate(&w->exec, w->graph, nullptr, nullptr, 0));
+  }
+  // x = 0, r = b, Ap = 0 ; scalars as cg_solve with x = 0 (cg.nim:138-153)
+  for (int k = 0; k < n; k++) {
+    const size_t vb = vec64_bytes(ls[k]->lat);
+    HIPCHK(hipMemcpyAsync(w->b[k], bf[k]->par(par), vb, hipMemcpyDeviceToDevice, g_stream));
+    HIPCHK(hipMemcpyAsync(w->r[k], w->b[k], vb, hipMemcpyDeviceToDevice, g_stream));
+    launch_fill(w->x[k], vb, g_stream);
+    launch_fill(w->Ap[k], vb, g_stream);
+    Scal &h = w->Sh[k];
+    memset(&h, 0, sizeof h);
+    h.m2 = ms[k] * ms[k];
+    h.maxits = sp[k].maxits;
+    HIPCHK(hipMemcpyAsync(w->S + k, &h, sizeof h, hipMemcpyHostToDevice, g_stream));
+    launch_scal_init(w->S + k, g_stream, 0);
+    launch_norm2(P64, ls[k]->lat->g, w->b[k], w->S + k, 1, ls[k]->lat->red, g_stream);
+  }
+  check_launch();
+  HIPCHK(hipMemcpyAsync(w->Sh, w->S, sizeof(Scal) * n, hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  std::vector<double> b2(n);
+  for (int k = 0; k < n; k++) {
+    Scal &h = w->Sh[k];
+    b2[k] = h.tmp[1];
+    h.b2 = b2[k];
+    h.r2stop = sp[k].r2req * b2[k];
+    h.r2 = h.r2prev = b2[k];
+    h.done = !(b2[k] > h.r2stop) || sp[k].maxits <= 0;
+    h.flushed = h.done;
+  }
+  HIPCHK(hipMemcpyAsync(w->S, w->Sh, sizeof(Scal) * n, hipMemcpyHostToDevice, g_stream));
+  for (;;) {
+    bool all = true;
+    for (int k = 0; k < n; k++) all = all && w->Sh[k].done && w->Sh[k].flushed;
+    if (all) break;
+    if (graph) HIPCHK(hipGraphLaunch(w->exec, g_stream));
+    else batch_chunk(w, w->T);
+    check_launch();
+    HIPCHK(hipMemcpyAsync(w->Sh, w->S, sizeof(Scal) * n, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+  }
+  // FP64 true residuals, then the solutions out
+  for (int k = 0; k < n; k++) {
+    resid64(ls[k], par, w->x[k], w->b[k], w->r[k], w->S + k, 3, w->t[k]);
+    HIPCHK(hipMemcpyAsync(xf[k]->par(par), w->x[k], vec64_bytes(ls[k]->lat), hipMemcpyDeviceToDevice, g_stream));
+  }
+  check_launch();
+  HIPCHK(hipMemcpyAsync(w->Sh, w->S, sizeof(Scal) * n, hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int k = 0; k < n; k++) {
+    sp[k].iterations = w->Sh[k].its;
+    sp[k].reliable_updates = 0;
+    sp[k].r2 = (b2[k] > 0) ? w->Sh[k].tmp[0] / b2[k] : 0.0;
+    sp[k].flops = flops_per_its(ls[k]) * w->Sh[k].its;
+    sp[k].seconds = sec;
+  }
+}
+
+// ------------------------------------------------ batched multi-shift CG
+// nrhs right-hand sides, one set of links (batch.hip (2)); the right-hand
+// sides run in groups of 8/4/2/1 that share each link load.
+struct MsbWs {
+  qexhip_links *l = nullptr;
+  int par = 0, nrhs = 0, nshift = 0;
+  std::vector<double *> x, p, r, Ap, t;
+  double **dx = nullptr, **dp = nullptr;
+  MsbScal *T = nullptr, *Th = nullptr, *Tx = nullptr;
+  RedWs red{};
+  std::vector<std::pair<int, int>> groups;  // (first rhs, size)
+  int block = 0;
+  double m2 = -1.0;  // mass^2 baked into the captured graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~MsbWs() {
+    for (auto *v : {&x, &p, &r, &Ap, &t})
+      for (double *q : *v) (void)hipFree(q);
+    (void)hipFree(dx);
+    (void)hipFree(dp);
+    (void)hipFree(T);
+    (void)hipHostFree(Th);
+    (void)hipFree(Tx);
+    (void)hipFree(red.partials);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+MsbWs *g_msb = nullptr;
+
+void drop_msb() {
+  if (g_msb) {
+    HIPCHK(hipStreamSynchronize(g_stream));
+    delete g_msb;
+    g_msb = nullptr;
+  }
+}
+
+MsbWs *get_msb(qexhip_links *l, int par, int nrhs, int nshift) {
+  if (g_msb && g_msb->l == l && g_msb->par == par && g_msb->nrhs == nrhs && g_msb->nshift == nshift) return g_msb;
+  drop_msb();
+  MsbWs *w = new MsbWs();
+  g_msb = w;
+  w->l = l;
+  w->par = par;
+  w->nrhs = nrhs;
+  w->nshift = nshift;
+  const size_t vb = vec64_bytes(l->lat);
+  for (int k = 0; k < nrhs * nshift; k++) {
+    w->x.push_back(dalloc<double>(vb));
+    w->p.push_back(dalloc<double>(vb));
+  }
+  for (int j = 0; j < nrhs; j++)
+    for (auto *v : {&w->r, &w->Ap, &w->t}) {
+      v->push_back(dalloc<double>(vb));
+      launch_fill(v->back(), vb, g_stream);
+    }
+  w->dx = dalloc<double *>(sizeof(double *) * nrhs * nshift);
+  w->dp = dalloc<double *>(sizeof(double *) * nrhs * nshift);
+  HIPCHK(hipMemcpyAsync(w->dx, w->x.data(), sizeof(double *) * nrhs * nshift, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(w->dp, w->p.data(), sizeof(double *) * nrhs * nshift, hipMemcpyHostToDevice, g_stream));
+  w->T = dalloc<MsbScal>(sizeof(MsbScal) * nrhs);
+  w->Tx = dalloc<MsbScal>(sizeof(MsbScal) * nrhs);
+  HIPCHK(hipHostMalloc((void **)&w->Th, sizeof(MsbScal) * nrhs, hipHostMallocDefault));
+  w->red.maxblocks = (int)((l->lat->g.Vh + 63) / 64) + 64;
+  w->red.partials = dalloc<double>(sizeof(double) * 3 * QH_MAXRHS * w->red.maxblocks);
+  for (int j = 0; j < nrhs;) {
+    int g = 8;
+    while (g > nrhs - j) g >>= 1;
+    w->groups.push_back({j, g});
+    j += g;
+  }
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return w;
+}
+
+MsbPtrs msb_ptrs(MsbWs *w, int j0, int nr) {
+  MsbPtrs P;
+  memset(&P, 0, sizeof P);
+  for (int j = 0; j < nr; j++) {
+    P.p0[j] = w->p[(j0 + j) * w->nshift];
+    P.t[j] = w->t[j0 + j];
+    P.Ap[j] = w->Ap[j0 + j];
+    P.r[j] = w->r[j0 + j];
+  }
+  P.x = w->dx + j0 * w->nshift;
+  P.p = w->dp + j0 * w->nshift;
+  return P;
+}
+
+void msb_iter(MsbWs *w, MsbScal *T, double m2, int block) {
+  qexhip_links *l = w->l;
+  const char *links = links_store(l, P64, 18);
+  const long lkblk = lkblk_bytes(l, P64, 18);
+  for (auto &gr : w->groups)
+    launch_msb_iter(l->lat->g, links, lkblk, l->nl, w->par, gr.second, w->nshift, m2,
+                    msb_ptrs(w, gr.first, gr.second), T + gr.first, w->red, block, g_stream);
+}
+
+void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexhip_field *const *bf, double m,
+               const double *sigmas, int nshift, qexhip_solver_params *sp, int flags) {
+  auto t0 = std::chrono::steady_clock::now();
+  MsbWs *w = get_msb(l, par, nrhs, nshift);
+  links_store(l, P64, 18);  // packed before any capture (it may allocate)
+  qexhip_lattice *lat = l->lat;
+  const size_t vb = vec64_bytes(lat);
+  const double m2 = m * m;
+  // launch shape
+  int block = g_block_forced ? g_block : 256;
+  if ((flags & QEXHIP_BATCH_TUNE) && !g_block_forced) {
+    const std::string key = "msb nl=" + std::to_string(l->nl) + " nrhs=" + std::to_string(nrhs) +
+                            " nshift=" + std::to_string(nshift) + " " + geom_key(lat->g);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      block = it->second.block;
+    } else {
+      std::vector<MsbScal> h(nrhs);
+      memset(h.data(), 0, sizeof(MsbScal) * nrhs);
+      for (auto &q : h) {
+        q.continuing = 1;
+        q.maxits = INT_MAX;
+        q.r2stop = -1.0;
+        q.nshift = nshift;
+        for (int k = 0; k < nshift; k++) { q.active[k] = 1; q.zi[k] = q.zim1[k] = 1.0; }
+      }
+      HIPCHK(hipMemcpyAsync(w->Tx, h.data(), sizeof(MsbScal) * nrhs, hipMemcpyHostToDevice, g_stream));
+      for (int k = 0; k < nrhs * nshift; k++) {
+        launch_fill(w->x[k], vb, g_stream);
+        launch_fill(w->p[k], vb, g_stream);
+      }
+      Tuned best{256, 1, 1e300};
+      for (int b : {64, 128, 256}) {
+        const double us = time_us([&] { msb_iter(w, w->Tx, m2, b); }, 1, 4);
+        if (us < best.us) best = Tuned{b, 1, us};
+      }
+      g_tune[key] = best;
+      block = best.block;
+    }
+  }
+  const bool graph = (flags & QEXHIP_BATCH_GRAPH) != 0;
+  if (block != w->block || m2 != w->m2) {
+    w->block = block;
+    w->m2 = m2;
+    if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
+    if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
+  }
+  // r := b ; p_k := b ; x_k := 0 (cgm.nim:122-158)
+  std::vector<double> b2(nrhs);
+  for (int j = 0; j < nrhs; j++) {
+    HIPCHK(hipMemcpyAsync(w->r[j], bf[j]->par(par), vb, hipMemcpyDeviceToDevice, g_stream));
+    for (int k = 0; k < nshift; k++) {
+      HIPCHK(hipMemcpyAsync(w->p[j * nshift + k], w->r[j], vb, hipMemcpyDeviceToDevice, g_stream));
+      launch_fill(w->x[j * nshift + k], vb, g_stream);
+    }
+    launch_norm2(P64, lat->g, w->r[j], lat->S, 1, lat->red, g_stream);
+    check_launch();
+    pull(lat->Sh, lat->S);
+    b2[j] = lat->Sh->tmp[1];
+    MsbScal &h = w->Th[j];
+    memset(&h, 0, sizeof h);
+    h.r2i = b2[j];
+    h.r2stop = sp[j].r2req * b2[j];
+    h.alphaim1 = -1.0;
+    h.betaim1 = 0.0;
+    h.maxits = sp[j].maxits;
+    h.nshift = nshift;
+    h.freeze = (flags & QEXHIP_BATCH_NOFREEZE) ? 0 : 1;
+    h.continuing = (b2[j] > h.r2stop) && sp[j].maxits > 0;
+    for (int k = 0; k < nshift; k++) {
+      h.sg[k] = (k == 0) ? 0.0 : sigmas[k];
+      h.zi[k] = h.zim1[k] = 1.0;
+      h.active[k] = h.continuing;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(w->T, w->Th, sizeof(MsbScal) * nrhs, hipMemcpyHostToDevice, g_stream));
+  if (graph && !w->exec) {
+    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block);
+    HIPCHK(hipStreamEndCapture(g_stream, &w->graph));
+    HIPCHK(hipGraphInstantiate(&w->exec, w->graph, nullptr, nullptr, 0));
+  }
+  for (;;) {
+    bool any = false;
+    for (int j = 0; j < nrhs; j++) any = any || w->Th[j].continuing;
+    if (!any) break;
+    if (graph) {
+      HIPCHK(hipGraphLaunch(w->exec, g_stream));
+    } else {
+      for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block);
+    }
+    check_launch();
+    HIPCHK(hipMemcpyAsync(w->Th, w->T, sizeof(MsbScal) * nrhs, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+  }
+  for (int j = 0; j < nrhs; j++)
+    for (int k = 0; k < nshift; k++)
+      HIPCHK(hipMemcpyAsync(xs[j * nshift + k]->par(par), w->x[j * nshift + k], vb, hipMemcpyDeviceToDevice,
+                            g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int j = 0; j < nrhs; j++) {
+    sp[j].iterations = w->Th[j].its;
+    sp[j].reliable_updates = 0;
+    sp[j].r2 = (b2[j] > 0) ? w->Th[j].r2i / b2[j] : 0.0;
+    sp[j].flops = flops_per_its(l) * w->Th[j].its + (double)(4 * lat->g.Vh + 15) * w->Th[j].its * nshift;
+    sp[j].seconds = sec;
+  }
+}
+
 }  // namespace
 
 // ================================================================ C ABI
@@ -515,8 +1055,12 @@ int qexhip_init(int device) {
   g_device = device;
   if (const char *b = getenv("QEXHIP_BLOCK")) {
     int v = atoi(b);
-    if (v >= 64 && v <= 256 && v % 64 == 0) g_block = v;
+    if (v >= 64 && v <= 256 && v % 64 == 0) {
+      g_block = v;
+      g_block_forced = true;
+    }
   }
+  if (const char *t = getenv("QEXHIP_TUNE")) g_autotune = atoi(t) != 0;
   if (const char *r = getenv("QEXHIP_XCD_REMAP")) g_remap = atoi(r) != 0;
   API_END
 }
@@ -524,6 +1068,8 @@ int qexhip_init(int device) {
 int qexhip_finalize(void) {
   API_BEGIN
   if (g_stream) {
+    drop_batch();
+    drop_msb();
     HIPCHK(hipStreamSynchronize(g_stream));
     HIPCHK(hipStreamDestroy(g_stream));
     g_stream = nullptr;
@@ -623,6 +1169,8 @@ int qexhip_links_destroy(qexhip_links *l) {
   API_BEGIN
   if (!l) return 0;
   HIPCHK(hipStreamSynchronize(g_stream));
+  if (g_batch && std::find(g_batch->links.begin(), g_batch->links.end(), l) != g_batch->links.end()) drop_batch();
+  if (g_msb && g_msb->l == l) drop_msb();
   (void)hipFree(l->canon);
   for (int p = 0; p <= PQ16; p++)
     for (int ri = 0; ri < 2; ri++)
@@ -969,6 +1517,63 @@ int qexhip_links_recon_ok(qexhip_links *l, int *ok) {
   check_init();
   require(l && ok, "null argument");
   *ok = recon_allowed(l) ? 1 : 0;
+  API_END
+}
+
+int qexhip_solve_batch(int n, qexhip_links *const *links, int par, qexhip_field *const *x,
+                       qexhip_field *const *b, const double *masses, qexhip_solver_params *sp, int flags) {
+  API_BEGIN
+  check_init();
+  require(n >= 1 && n <= QH_MAXBATCH, "solve_batch: n out of range");
+  require(links && x && b && masses && sp, "null argument");
+  require(par == 0 || par == 1, "solve_batch: parity must be 0 or 1");
+  for (int k = 0; k < n; k++) {
+    require(links[k] && links[k]->nl == links[0]->nl, "solve_batch: all systems need the same link count");
+    check_same_lat(links[k], x[k]);
+    check_same_lat(links[k], b[k]);
+    require(x[k]->prec == P64 && b[k]->prec == P64, "solve_batch fields must be FP64");
+  }
+  solve_batch(n, links, par, x, b, masses, sp, flags);
+  API_END
+}
+
+int qexhip_solve_multishift_batch(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs,
+                                  qexhip_field *const *b, double m, const double *sigmas, int nshift,
+                                  qexhip_solver_params *sp, int flags) {
+  API_BEGIN
+  check_init();
+  require(l && xs && b && sigmas && sp, "null argument");
+  require(par == 0 || par == 1, "multishift parity must be 0 or 1");
+  require(nrhs >= 1 && nrhs <= 64, "multishift batch: nrhs out of range");
+  require(nshift >= 1 && nshift <= QH_MAXSHIFT, "multishift: nshift out of range");
+  for (int j = 0; j < nrhs; j++) {
+    check_same_lat(l, b[j]);
+    require(b[j]->prec == P64, "multishift fields must be FP64");
+  }
+  for (int k = 0; k < nrhs * nshift; k++) {
+    check_same_lat(l, xs[k]);
+    require(xs[k]->prec == P64, "multishift fields must be FP64");
+  }
+  solve_msb(l, par, nrhs, xs, b, m, sigmas, nshift, sp, flags);
+  API_END
+}
+
+int qexhip_set_autotune(int on) {
+  g_autotune = on != 0;
+  return 0;
+}
+
+int qexhip_autotune_report(char *buf, int len) {
+  API_BEGIN
+  require(buf && len > 0, "bad buffer");
+  std::string out;
+  for (auto &kv : g_tune) {
+    char line[256];
+    snprintf(line, sizeof line, "%s: block=%d remap=%d us=%.2f\n", kv.first.c_str(), kv.second.block,
+             kv.second.remap, kv.second.us);
+    out += line;
+  }
+  snprintf(buf, (size_t)len, "%s", out.c_str());
   API_END
 }
 

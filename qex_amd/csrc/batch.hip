@@ -1,0 +1,335 @@
+// Batched solves on one GPU, FP64.
+//
+// (1) Independent CG solves (solvers/cg.nim:55-272 each) on lattices of
+//     different sizes, each with its own links, mass and source, advanced
+//     together: every CG stage is ONE launch over all systems, whose blocks
+//     are dealt to the systems through a table (BatchItem), and the
+//     per-system dot products finish in one launch with one block per system.
+//     Converged systems drop out block by block; the host captures a chunk of
+//     iterations in a hipGraph and replays it.
+//
+// (2) Multi-shift CG (solvers/cgm.nim:68-246) for up to QH_MAXRHS right-hand
+//     sides on one set of links.  Per iteration: t_j = (2D)_oe p_j and
+//     Ap_j = 4 m^2 p_j - (2D)_eo t_j for all j in two multi-RHS hops that load
+//     each link once for every right-hand side; alpha_j and the zeta recurrence
+//     of every shift from the fused dot products <p,Ap>, <r,Ap>, |Ap|^2 (the
+//     new |r|^2 extrapolated in one step, as in the fused CG); then ONE pass
+//     updates r_j, every x_jk and every p_jk (x_k += alpha_k p_k and
+//     p_k = zeta r + beta_k p_k share the read of p_k), reducing the exact
+//     |r_j|^2 for the next alpha and the stopping test.  Shifts whose residual
+//     zeta_k^2 |r|^2 met the request stop being updated (optional).
+#include <stdio.h>
+#include "stencil.h"
+#include "finish.h"
+
+namespace qh {
+
+// ------------------------------------------------------- (1) CG batch
+__device__ __forceinline__ int batch_item(const BatchItem *T, int n, long b) {
+  int k = 0;
+  while (k + 1 < n && T[k + 1].blk0 <= b) k++;
+  return k;
+}
+
+template <int NL>
+__global__ void __launch_bounds__(256) k_bcg1(const BatchItem *T, int n) {
+  const long b = blockIdx.x;
+  const BatchItem &it = T[batch_item(T, n, b)];
+  cg1_body<P64, 18, P64, P64, P64, double2, NL>(it.a, b - it.blk0);
+}
+
+template <int NL>
+__global__ void __launch_bounds__(256) k_bcg2(const BatchItem *T, int n) {
+  __shared__ double lds[16 * 4];
+  const long b = blockIdx.x;
+  const BatchItem &it = T[batch_item(T, n, b)];
+  cg2_body<P64, 18, P64, P64, P64, double2, NL>(it.a, b - it.blk0, lds, b);
+}
+
+__global__ void __launch_bounds__(1024) k_bfin_cg2(const BatchItem *T) {
+  __shared__ double lds[16 * 4];
+  finish_body<4, EPI_CG2>(T[blockIdx.x].fin, lds);
+}
+
+// apply the pending update of systems that just converged (once each)
+__global__ void __launch_bounds__(256) k_bflush(const BatchItem *T, int n) {
+  __shared__ double lds[16];
+  const long b = blockIdx.x;
+  const BatchItem &it = T[batch_item(T, n, b)];
+  const Scal *S = it.a.S;
+  if (!S->done || S->flushed) return;
+  cg_flush_body<P64, P64, P64, double2>(it.a, b - it.blk0, lds, b);
+}
+
+__global__ void __launch_bounds__(1024) k_bfin_flush(const BatchItem *T) {
+  __shared__ double lds[16];
+  const BatchItem &it = T[blockIdx.x];
+  Scal *S = it.a.S;
+  const bool run = S->done && !S->flushed;  // read by every thread before any write
+  __syncthreads();
+  if (!run) return;
+  finish_body<1, EPI_FLUSH>(it.finf, lds);
+  if (threadIdx.x == 0) S->flushed = 1;
+}
+
+void launch_bcg_iter(const BatchItem *T, int n, long nblocks, int nl, int block, hipStream_t st) {
+  const dim3 grid((unsigned)nblocks), blk(block);
+  if (nl == 8) {
+    hipLaunchKernelGGL(k_bcg1<8>, grid, blk, 0, st, T, n);
+    hipLaunchKernelGGL(k_bcg2<8>, grid, blk, 0, st, T, n);
+  } else {
+    hipLaunchKernelGGL(k_bcg1<4>, grid, blk, 0, st, T, n);
+    hipLaunchKernelGGL(k_bcg2<4>, grid, blk, 0, st, T, n);
+  }
+  hipLaunchKernelGGL(k_bfin_cg2, dim3(n), dim3(1024), 0, st, T);
+}
+
+void launch_bcg_flush(const BatchItem *T, int n, long nblocks, int block, hipStream_t st) {
+  hipLaunchKernelGGL(k_bflush, dim3((unsigned)nblocks), dim3(block), 0, st, T, n);
+  hipLaunchKernelGGL(k_bfin_flush, dim3(n), dim3(1024), 0, st, T);
+}
+
+// --------------------------------------------- (2) multi-shift, multi-RHS
+// acc -= conj(a)^T-row product: acc -= conj(a) * b
+__device__ __forceinline__ void cjmsub(double2 &acc, double2 a, double2 b) {
+  acc.x -= a.x * b.x + a.y * b.y;
+  acc.y -= a.x * b.y - a.y * b.x;
+}
+
+// MODE 0: t_j = (2D) p0_j on parity q (x = site of parity q).
+// MODE 1: Ap_j = m4 p0_j - (2D) t_j on parity q with the partial sums of
+//         <p0_j, Ap_j>, <r_j, Ap_j>, |Ap_j|^2 (slots 3j, 3j+1, 3j+2).
+// Each link is loaded once and applied to the NR right-hand sides.
+template <int NL, int NR, int MODE>
+__global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, long lkblk, int q, double m4,
+                                                 MsbPtrs P, const MsbScal *T, RedWs red) {
+  __shared__ double lds[16 * 3 * NR];
+  const long i = xcd_block(1) * blockDim.x + threadIdx.x;
+  bool act[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) act[r] = T[r].continuing != 0;
+  double2 acc[NR][3];
+#pragma unroll
+  for (int r = 0; r < NR; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[r][c] = make_double2(0.0, 0.0);
+  const bool in = i < g.Vh;
+  if (in) {
+    const Site s = decode(g, i, q);
+#pragma unroll 1
+    for (int j = 0; j < 2 * NL; j++) {
+      const int k = j >> 1, fwd = !(j & 1);
+      const int mu = (NL == 4) ? k : (k >> 1);
+      const int len = (NL == 8 && (k & 1)) ? 3 : 1;
+      const long n = nbr(g, s, mu, fwd ? len : -len);
+      double2 U[9];
+      ld_link<P64, 18, true, double2>(links + (long)(2 * k + (fwd ? q : 1 - q)) * lkblk, g.stride, fwd ? i : n, U,
+                                      1.0);
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        if (!act[r]) continue;
+        const double2 *src = (const double2 *)(MODE == 0 ? P.p0[r] : P.t[r]);
+        double2 v[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] = src[c * g.stride + n];
+        if (fwd) {
+#pragma unroll
+          for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) cmac(acc[r][a], U[3 * a + c], v[c]);
+        } else {
+#pragma unroll
+          for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) cjmsub(acc[r][a], U[3 * c + a], v[c]);
+        }
+      }
+    }
+  }
+  if constexpr (MODE == 0) {
+    if (in)
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        if (!act[r]) continue;
+        double2 *dst = (double2 *)P.t[r];
+#pragma unroll
+        for (int c = 0; c < 3; c++) dst[c * g.stride + i] = acc[r][c];
+      }
+  } else {
+    double sums[3 * NR];
+#pragma unroll
+    for (int k = 0; k < 3 * NR; k++) sums[k] = 0.0;
+    if (in)
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        if (!act[r]) continue;
+        const double2 *pp = (const double2 *)P.p0[r], *rr = (const double2 *)P.r[r];
+        double2 *dst = (double2 *)P.Ap[r];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const long o = c * g.stride + i;
+          const double2 pv = pp[o], rv = rr[o];
+          const double2 av = make_double2(m4 * pv.x - acc[r][c].x, m4 * pv.y - acc[r][c].y);
+          dst[o] = av;
+          sums[3 * r] += pv.x * av.x + pv.y * av.y;
+          sums[3 * r + 1] += rv.x * av.x + rv.y * av.y;
+          sums[3 * r + 2] += av.x * av.x + av.y * av.y;
+        }
+      }
+    block_partials<3 * NR>(sums, lds, red);
+  }
+}
+
+// After the MODE-1 hop: alpha_j, the extrapolated |r_j|^2, beta_j and the
+// update coefficients of every shift (cgm.nim:163-199).
+template <int NR>
+__global__ void __launch_bounds__(1024) k_msb_fin1(RedWs red, int nb, MsbScal *T) {
+  __shared__ double lds[16 * 3 * NR];
+  double s[3 * NR];
+#pragma unroll
+  for (int k = 0; k < 3 * NR; k++) s[k] = 0.0;
+  for (int j = threadIdx.x; j < nb; j += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 3 * NR; k++) s[k] += red.partials[(long)k * red.maxblocks + j];
+  block_sum<3 * NR>(s, lds);
+  if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    MsbScal *t = T + r;
+    if (!t->continuing) continue;
+    const double pAp = s[3 * r], rAp = s[3 * r + 1], ApAp = s[3 * r + 2];
+    const double alpha = (pAp != 0.0) ? t->r2i / pAp : 0.0;
+    double r2n = t->r2i - 2.0 * alpha * rAp + alpha * alpha * ApAp;
+    if (!(r2n > 0.0)) r2n = 0.0;
+    const double beta = (t->r2i != 0.0) ? r2n / t->r2i : 0.0;
+    t->alpha = alpha;
+    t->beta = beta;
+    t->xc[0] = alpha;
+    t->pc1[0] = 1.0;
+    t->pc2[0] = beta;
+    for (int k = 1; k < t->nshift; k++) {
+      if (!t->active[k]) continue;
+      double zip1d = alpha * t->betaim1 * (t->zim1[k] - t->zi[k]);
+      zip1d += t->zim1[k] * t->alphaim1 * (1.0 + t->sg[k] * alpha);
+      const double zip1 = (zip1d != 0.0) ? t->zi[k] * t->zim1[k] * t->alphaim1 / zip1d : 0.0;
+      const double zr = (t->zi[k] != 0.0) ? zip1 / t->zi[k] : 0.0;
+      t->xc[k] = alpha * zr;
+      t->pc1[k] = zip1;
+      t->pc2[k] = beta * zr * zr;
+      t->zim1[k] = t->zi[k];
+      t->zi[k] = zip1;
+    }
+    t->alphaim1 = alpha;
+    t->betaim1 = beta;
+    t->its += 1;
+  }
+}
+
+// r_j -= alpha_j Ap_j ; for every active shift x_jk += xc p_jk, p_jk = pc1 r_j + pc2 p_jk ;
+// partial sums of the exact |r_j|^2.
+template <int NR>
+__global__ void __launch_bounds__(256) k_msb_update(Geom g, int nshift, MsbPtrs P, const MsbScal *T, RedWs red) {
+  __shared__ double lds[16 * NR];
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double rr[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) rr[r] = 0.0;
+  if (i < g.Vh) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const MsbScal *t = T + r;
+      if (!t->continuing) continue;
+      const double alpha = t->alpha;
+      double2 rn[3];
+      double2 *rv = (double2 *)P.r[r];
+      const double2 *av = (const double2 *)P.Ap[r];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const long o = c * g.stride + i;
+        const double2 a = av[o], x = rv[o];
+        rn[c] = make_double2(x.x - alpha * a.x, x.y - alpha * a.y);
+        rv[o] = rn[c];
+        rr[r] += rn[c].x * rn[c].x + rn[c].y * rn[c].y;
+      }
+#pragma unroll 1
+      for (int k = 0; k < nshift; k++) {
+        if (!t->active[k]) continue;
+        const double xc = t->xc[k], c1 = t->pc1[k], c2 = t->pc2[k];
+        double2 *xk = (double2 *)P.x[r * nshift + k], *pk = (double2 *)P.p[r * nshift + k];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const long o = c * g.stride + i;
+          double2 xv = xk[o];
+          const double2 pv = pk[o];
+          xv.x += xc * pv.x;
+          xv.y += xc * pv.y;
+          xk[o] = xv;
+          pk[o] = make_double2(c1 * rn[c].x + c2 * pv.x, c1 * rn[c].y + c2 * pv.y);
+        }
+      }
+    }
+  }
+  block_partials<NR>(rr, lds, red);
+}
+
+// exact |r_j|^2: next alpha, stopping test, shift freezing (cgm.nim:176-186)
+template <int NR>
+__global__ void __launch_bounds__(1024) k_msb_fin2(RedWs red, int nb, MsbScal *T) {
+  __shared__ double lds[16 * NR];
+  double s[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) s[k] = 0.0;
+  for (int j = threadIdx.x; j < nb; j += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < NR; k++) s[k] += red.partials[(long)k * red.maxblocks + j];
+  block_sum<NR>(s, lds);
+  if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    MsbScal *t = T + r;
+    if (!t->continuing) continue;
+    const double r2 = s[r];
+    t->r2i = r2;
+    t->continuing = (t->its < t->maxits) && (r2 > t->r2stop);
+    if (!t->continuing) {
+      for (int k = 0; k < t->nshift; k++) t->active[k] = 0;
+    } else if (t->freeze) {
+      for (int k = 1; k < t->nshift; k++)
+        if (t->active[k] && t->zi[k] * t->zi[k] * r2 <= t->r2stop) t->active[k] = 0;
+    }
+  }
+}
+
+template <int NL, int NR>
+static void msb_iter_t(const Geom &g, const char *links, long lkblk, int par, int nshift, double m2,
+                       const MsbPtrs &P, MsbScal *T, RedWs red, int block, hipStream_t st) {
+  const unsigned nb = (unsigned)((g.Vh + block - 1) / block);
+  const dim3 grid(nb), blk(block);
+  hipLaunchKernelGGL((k_mrhs_hop<NL, NR, 0>), grid, blk, 0, st, g, links, lkblk, 1 - par, 0.0, P, T, red);
+  hipLaunchKernelGGL((k_mrhs_hop<NL, NR, 1>), grid, blk, 0, st, g, links, lkblk, par, 4.0 * m2, P, T, red);
+  hipLaunchKernelGGL((k_msb_fin1<NR>), dim3(1), dim3(1024), 0, st, red, (int)nb, T);
+  hipLaunchKernelGGL((k_msb_update<NR>), dim3((unsigned)((g.Vh + 255) / 256)), dim3(256), 0, st, g, nshift, P, T,
+                     red);
+  hipLaunchKernelGGL((k_msb_fin2<NR>), dim3(1), dim3(1024), 0, st, red, (int)((g.Vh + 255) / 256), T);
+}
+
+template <int NL>
+static bool msb_nr(int nr, const Geom &g, const char *links, long lkblk, int par, int nshift, double m2,
+                   const MsbPtrs &P, MsbScal *T, RedWs red, int block, hipStream_t st) {
+  switch (nr) {
+    case 1: msb_iter_t<NL, 1>(g, links, lkblk, par, nshift, m2, P, T, red, block, st); return true;
+    case 2: msb_iter_t<NL, 2>(g, links, lkblk, par, nshift, m2, P, T, red, block, st); return true;
+    case 4: msb_iter_t<NL, 4>(g, links, lkblk, par, nshift, m2, P, T, red, block, st); return true;
+    case 8: msb_iter_t<NL, 8>(g, links, lkblk, par, nshift, m2, P, T, red, block, st); return true;
+  }
+  return false;
+}
+
+void launch_msb_iter(const Geom &g, const char *links64, long lkblk, int nl, int par, int nr, int nshift,
+                     double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, hipStream_t st) {
+  const bool ok = (nl == 8) ? msb_nr<8>(nr, g, links64, lkblk, par, nshift, m2, P, T, red, block, st)
+                            : msb_nr<4>(nr, g, links64, lkblk, par, nshift, m2, P, T, red, block, st);
+  if (!ok) fprintf(stderr, "qexhip: no multi-RHS instantiation for nr=%d nl=%d\n", nr, nl);
+}
+
+}  // namespace qh

@@ -207,12 +207,16 @@ struct RedWs {
 // (reduce.hip) then adds the partials in a fixed order (deterministic, and no
 // grid-wide fence in the producing kernel).
 template <int NV>
-__device__ __forceinline__ void block_partials(double (&v)[NV], double *lds, const RedWs &red) {
-  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+__device__ __forceinline__ void block_partials_at(double (&v)[NV], double *lds, const RedWs &red, long bid) {
   block_sum<NV>(v, lds);
   if (threadIdx.x == 0)
 #pragma unroll
     for (int k = 0; k < NV; k++) red.partials[(long)k * red.maxblocks + bid] = v[k];
+}
+
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double *lds, const RedWs &red) {
+  block_partials_at<NV>(v, lds, red, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // XCD-aware block order: the hardware deals workgroups round-robin over the 8

@@ -30,7 +30,7 @@ struct Scal {
   int rollback;      // mixed: the true residual grew; host returns x to the last good iterate
   int resinc;        // mixed: reliable updates whose true residual grew
   int improved;      // mixed: the last reliable update reached a new best residual
-  int pad0;
+  int flushed;       // batched FP64 CG: the pending update of a converged system was applied
   double r2rel_last; // mixed: best true residual^2 at a reliable update
 };
 
@@ -97,6 +97,38 @@ struct FinArgs {
   int pred;           // 0 none; 1 skip when done|need_rel; 3 skip unless T->continuing
 };
 
+// One system of a batched FP64 CG (batch.hip): blocks [blk0, blk0+nblk) of
+// the batched grid belong to it.
+struct BatchItem {
+  CgArgs a;           // its CG arguments (a.red: the batch partials, slot = grid block)
+  FinArgs fin;        // finish of the iteration dot products (its partials slice)
+  FinArgs finf;       // finish of the flush
+  long blk0, nblk;
+};
+#define QH_MAXBATCH 64
+
+// One right-hand side of the batched multi-shift CG (batch.hip); shift 0 is
+// the base system A = 4(m^2 - D_eo D_oe), shift k > 0 adds sg[k].
+struct MsbScal {
+  double r2i, r2stop, alpha, beta, alphaim1, betaim1;
+  int its, maxits, continuing, nshift;
+  int freeze;                  // stop updating shifts whose residual met r2stop
+  int pad;
+  double sg[QH_MAXSHIFT], zi[QH_MAXSHIFT], zim1[QH_MAXSHIFT];
+  double xc[QH_MAXSHIFT];      // x_k += xc[k] p_k (old p_k)
+  double pc1[QH_MAXSHIFT];     // p_k = pc1[k] r_new + pc2[k] p_k
+  double pc2[QH_MAXSHIFT];
+  int active[QH_MAXSHIFT];
+};
+#define QH_MAXRHS 8
+struct MsbPtrs {              // one group of nr right-hand sides
+  const double *p0[QH_MAXRHS];  // base search vectors (own parity)
+  double *t[QH_MAXRHS];         // (2D) p0 on the other parity
+  double *Ap[QH_MAXRHS];
+  double *r[QH_MAXRHS];
+  double **x, **p;              // device arrays [nr*nshift]: shifted solutions and search vectors
+};
+
 struct HopKey {       // precision/format of one hop instantiation
   int pl, rc, pi, po, pa;
 };
@@ -130,6 +162,12 @@ void launch_cgm_p(const Geom &g, int nshift, const double *r, double **p, ShiftS
 void launch_cgm_alpha(ShiftScal *T, Scal *S, hipStream_t st);
 void launch_force(const Geom &g, const char *links64, long lkblk, const double *psi_e, const double *psi_o,
                   double *F, int outer_only, hipStream_t st);
+
+// batch.hip
+void launch_bcg_iter(const BatchItem *T, int n, long nblocks, int nl, int block, hipStream_t st);
+void launch_bcg_flush(const BatchItem *T, int n, long nblocks, int block, hipStream_t st);
+void launch_msb_iter(const Geom &g, const char *links64, long lkblk, int nl, int par, int nr, int nshift,
+                     double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, hipStream_t st);
 
 // gauge.hip
 void launch_plaquette(const Geom &g, const double *U, Scal *S, int slot, RedWs red, hipStream_t st);

@@ -2,133 +2,14 @@
 // kernel launched just before it (stream order makes them visible) in a fixed
 // order, so results are bitwise reproducible, then applies the solver
 // epilogue to the device-resident scalars.
-#include "internal.h"
+#include "finish.h"
 
 namespace qh {
-
-// FP64 true residual r = b - A x just computed (reliable update): restart the
-// sloppy recursion on sigma*r with beta = |r_true|^2/|r_prev|^2 (unscaled).
-// A true residual^2 more than QH_RES_GROWTH times the best one so far means
-// the sloppy recursion diverged: ask the host to return x to the best iterate
-// (the next call sees that residual again and accepts it), then restart the
-// recursion from p = r.  After QH_MAX_RES_INCREASE such events the solve stops
-// and reports the residual it has.  (CG residual norms are not monotone, so
-// smaller increases are accepted.)
-__device__ void epi_reliable(Scal *S, double s, double nreal) {
-  S->r2true = s;
-  if (!(s <= QH_RES_GROWTH * S->r2rel_last)) {
-    S->rollback = 1;
-    S->restart = 1;
-    S->resinc += 1;
-    S->need_rel = 0;
-    S->done = S->resinc > QH_MAX_RES_INCREASE || S->its >= S->maxits;
-    return;
-  }
-  S->rollback = 0;
-  S->improved = (s <= S->r2rel_last);
-  if (S->improved) S->r2rel_last = s;
-  const double sig_old = S->sigma;
-  const double sig_new = (s > 0.0) ? sqrt(nreal / s) : 1.0;
-  const double r2old_unscaled = S->r2prev / (sig_old * sig_old);
-  S->beta = (r2old_unscaled > 0.0) ? s / r2old_unscaled : 0.0;
-  S->pscale = sig_new / sig_old;
-  S->sigma = sig_new;
-  S->r2 = s * sig_new * sig_new;
-  S->r2stop_s = S->r2stop * sig_new * sig_new;
-  S->maxr2 = S->r2;
-  S->alpha = 0.0;
-  S->since = 0;
-  S->need_rel = 0;
-  S->nrel += 1;
-  if (S->restart) {
-    S->first = 1;
-    S->beta = 0.0;
-    S->restart = 0;
-  }
-  S->done = (s <= S->r2stop) || (S->its >= S->maxits) || (S->resinc > QH_MAX_RES_INCREASE);
-}
-
-// After iteration n of the fused CG (solvers/cg.nim:174-214 recurrence):
-// rr = |r_{n-1}|^2 exact, pAp = <p_n,Ap_n>, rAp = <r_{n-1},Ap_n>, ApAp = |Ap_n|^2.
-//   alpha_n = rr/pAp ; |r_n|^2 = rr - 2 alpha rAp + alpha^2 ApAp ; beta_{n+1} = |r_n|^2/rr
-__device__ void epi_cg2(Scal *S, double pAp, double rAp, double ApAp, double rr) {
-  const double alpha = rr / pAp;
-  double r2n = rr - 2.0 * alpha * rAp + alpha * alpha * ApAp;
-  if (S->mixed && !(pAp > 0.0 && isfinite(r2n))) {
-    // breakdown of the sloppy recursion (rounding made p.Ap non-positive):
-    // drop this step and restart from the true residual
-    S->r2prev = rr;
-    S->alpha = 0.0;
-    S->r2 = rr;
-    S->first = 0;
-    S->pscale = 1.0;
-    S->its += 1;
-    S->since += 1;
-    S->restart = 1;
-    S->need_rel = 1;
-    return;
-  }
-  if (!(r2n > 0.0)) r2n = 0.0;
-  S->r2prev = rr;
-  S->pAp = pAp;
-  S->alpha = alpha;
-  S->r2 = r2n;
-  S->beta = r2n / rr;
-  S->first = 0;
-  S->pscale = 1.0;
-  S->its += 1;
-  S->since += 1;
-  if (S->mixed) {
-    S->need_rel = (r2n < S->delta2 * S->maxr2) || (S->since >= S->relmax) || (r2n <= S->r2stop_s) ||
-                  (S->its >= S->maxits);
-  } else {
-    S->done = (S->its >= S->maxits) || (r2n <= S->r2stop);
-  }
-}
-
-// after the pending x/r update was applied: exact |r_n|^2
-__device__ void epi_flush(Scal *S, double rr) {
-  S->r2 = rr;
-  S->alpha = 0.0;
-  S->beta = (S->r2prev > 0.0) ? rr / S->r2prev : 0.0;
-  if (!S->mixed) S->done = (S->its >= S->maxits) || (rr <= S->r2stop);
-}
-
-// multi-shift CG after r -= alpha Ap and |r|^2 (solvers/cgm.nim:176-186,200-202)
-__device__ void epi_cgm(ShiftScal *T, Scal *S, double r2ip1) {
-  T->r2ip1 = r2ip1;
-  T->its += 1;
-  const double beta = (T->r2i != 0.0) ? r2ip1 / T->r2i : 0.0;
-  T->beta = beta;
-  T->continuing = (T->its < T->maxits) && (r2ip1 > T->r2stop);
-  S->done = !T->continuing;
-  T->pc1[0] = 1.0;
-  T->pc2[0] = beta;
-  for (int k = 1; k < T->nshift; k++) T->pc2[k] *= beta;
-  T->alphaim1 = T->alpha;
-  T->betaim1 = beta;
-  T->r2i = r2ip1;
-}
 
 template <int NV, int EPI>
 __global__ void __launch_bounds__(1024) k_finish(FinArgs F) {
   __shared__ double lds[16 * NV];
-  if (F.pred == 1 && (F.S->done | F.S->need_rel)) return;
-  if (F.pred == 3 && !F.T->continuing) return;
-  double s[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) s[k] = 0.0;
-  for (int j = threadIdx.x; j < F.nb; j += blockDim.x)
-#pragma unroll
-    for (int k = 0; k < NV; k++) s[k] += F.red.partials[(long)k * F.red.maxblocks + j];
-  block_sum<NV>(s, lds);
-  if (threadIdx.x != 0) return;
-  if (EPI == EPI_TMP) F.S->tmp[F.slot] = s[0];
-  else if (EPI == EPI_PAP) F.S->pAp = s[0];
-  else if (EPI == EPI_RELIABLE) epi_reliable(F.S, s[0], F.nreal);
-  else if (EPI == EPI_CG2) epi_cg2(F.S, s[0], s[1 % NV], s[2 % NV], s[3 % NV]);
-  else if (EPI == EPI_FLUSH) epi_flush(F.S, s[0]);
-  else if (EPI == EPI_CGM) epi_cgm(F.T, F.S, s[0]);
+  finish_body<NV, EPI>(F, lds);
 }
 
 void launch_finish(int epi, const FinArgs &f, hipStream_t st) {

@@ -370,6 +370,64 @@ def solveMulti(s, xs, b, m, sigmas, sp, subset="even"):
     return c
 
 
+def _vec(ptrs):
+    return (ctypes.c_void_p * len(ptrs))(*[p.h.value for p in ptrs])
+
+
+def solveBatch(ss, xs, bs, masses, sps, subset="even", graph=True, tune=True):
+    """Independent FP64 CG solves 4(m_k^2 - DD) x_k = b_k on one parity for
+    systems with their own links (lattices may differ), advanced together on
+    the GPU, one launch per solver stage for all of them (each system is the
+    CG of solveEE/solveOO from x = 0, cg.nim:55-272).  sps: one SolverParams
+    per system (r2req, maxits in; iterations and FP64 true residual out)."""
+    n = len(ss)
+    if not (len(xs) == len(bs) == len(masses) == len(sps) == n):
+        raise QexHipError("solveBatch: one field, mass and SolverParams per system")
+    cs = (_lib.SolverParamsC * n)(*[sp._c() for sp in sps])
+    flags = (_lib.BATCH_GRAPH if graph else 0) | (_lib.BATCH_TUNE if tune else 0)
+    call("qexhip_solve_batch", n, _vec(ss), _par(subset), _vec(xs), _vec(bs),
+         (ctypes.c_double * n)(*[float(m) for m in masses]), cs, flags)
+    for sp, c in zip(sps, cs):
+        sp._add(c)
+    return sps
+
+
+def solveMultiBatch(s, xs, bs, m, sigmas, sps, subset="even", graph=True, tune=True, freeze=True):
+    """Multi-shift CG for several right-hand sides on one set of links:
+    (4(m^2 - DD) + sigma_k) xs[j][k] = bs[j], sigma_0 taken as 0 (cgm.nim:68-246
+    per right-hand side); each link load serves up to 8 right-hand sides.
+    freeze=False keeps updating shifted systems that already converged, as
+    cgm.nim does."""
+    nr, ns = len(bs), len(sigmas)
+    if len(xs) != nr or any(len(row) != ns for row in xs) or len(sps) != nr:
+        raise QexHipError("solveMultiBatch: xs must be nrhs lists of nshift fields, one SolverParams per rhs")
+    cs = (_lib.SolverParamsC * nr)(*[sp._c() for sp in sps])
+    flags = (_lib.BATCH_GRAPH if graph else 0) | (_lib.BATCH_TUNE if tune else 0) | \
+        (0 if freeze else _lib.BATCH_NOFREEZE)
+    call("qexhip_solve_multishift_batch", s.h, _par(subset), nr, _vec([f for row in xs for f in row]), _vec(bs),
+         float(m), (ctypes.c_double * ns)(*[float(v) for v in sigmas]), ns, cs, flags)
+    for sp, c in zip(sps, cs):
+        sp._add(c)
+    return sps
+
+
+def set_autotune(on=True):
+    """Launch-shape autotuning (on by default; QEXHIP_BLOCK pins the block size)."""
+    call("qexhip_set_autotune", int(bool(on)))
+
+
+def autotune_report():
+    """{kernel family and shape: (block, xcd_remap, microseconds)} chosen so far."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    call("qexhip_autotune_report", buf, len(buf))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        key, _, rest = line.rpartition(": ")
+        f = dict(kv.split("=") for kv in rest.split())
+        out[key] = (int(f["block"]), int(f["remap"]), float(f["us"]))
+    return out
+
+
 def _sub_solve(s, x, b, m, sp, r2req, par):
     sub = SolverParams()
     sub.__dict__.update({k: v for k, v in sp.__dict__.items() if not k.startswith("_")})
