@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--cg-only", default="", help="comma-separated CG variants to run")
     ap.add_argument("--hisq-cg", action="store_true",
                     help="add the HISQ-form 32^3x64 m=0.005 storage-precision study")
+    ap.add_argument("--batched", action="store_true",
+                    help="add the batched-solve study (multi-shift x multi-RHS, uneven CG batch)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -185,6 +187,132 @@ def hisq_study():
     for name in cg:
         cg[name]["speedup_vs_fp64"] = round(base / cg[name]["seconds"], 3)
     out["cg"] = cg
+    return out
+
+
+def batched_study():
+    """Batched solves against one-at-a-time solves, FP64, |r|/|b| <= 1e-10.
+
+    (a) HISQ-form 24^3x48: 8 complex-Gaussian sources x 12 shifts,
+        (-4 D_eo D_oe + sigma_i) x_i = b with sigma_i log-spaced over [1e-4, 1]
+        (QEX normalization 4(m^2 - D_eo D_oe) + s_k with 4m^2 = sigma_0,
+        s_k = sigma_k - sigma_0): one batched multi-shift solve (each link load
+        serves the 8 sources; hipGraph replay; autotuned block) vs 8 calls of
+        the single-source multi-shift solver.
+    (b) 16 plain solves, 4 each on 8^4, 12^4, 16^4 and 16^3x32, distinct
+        seeds, masses uniform in [0.01, 0.1]: one batched solve vs 16 calls of
+        solveEE (same FP64 CG)."""
+    import numpy as np
+    import torch
+    import qex_amd as Q
+    from qex_amd import gauge
+
+    def sp_new(r2req=R2REQ):
+        sp = Q.SolverParams()
+        sp.r2req, sp.verbosity, sp.maxits = r2req, 0, 100000
+        return sp
+
+    def timed(f):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    out = {}
+    # ---- (a)
+    L = (24, 24, 24, 48)
+    lo = Q.Layout(L)
+    U = gauge.random_gauge(lo, SEED + 7)
+    gauge.phase_bc(lo, U)
+    g8 = gauge.hisq_links(lo, SEED + 8, U)
+    del U
+    s = Q.Staggered(lo, g8)
+    del g8
+    torch.cuda.empty_cache()
+    sig = list(np.logspace(-4, 0, 12))
+    m = 0.5 * sig[0] ** 0.5
+    shifts = [0.0] + [v - sig[0] for v in sig[1:]]
+    nr = 8
+    bs = [lo.ColorVector().set(gauge.source(lo, 900 + j, 0, 0)) for j in range(nr)]
+    xs = [[lo.ColorVector() for _ in shifts] for _ in range(nr)]
+    res = {"lattice": list(L), "nrhs": nr, "nshift": len(shifts), "sigmas": sig, "r2req": R2REQ}
+    for name, kw in [("batched_graph_tuned", dict(graph=True, tune=True)),
+                     ("batched_graph_tuned_nofreeze", dict(graph=True, tune=True, freeze=False)),
+                     ("batched_nograph_block256", dict(graph=False, tune=False))]:
+        best = None
+        for rep in range(2):
+            sps = [sp_new() for _ in range(nr)]
+            t = timed(lambda: Q.solveMultiBatch(s, xs, bs, m, shifts, sps, **kw))
+            if best is None or t < best[0]:
+                best = (t, sps)
+        t, sps = best
+        res[name] = {"seconds": round(t, 4), "iterations": [sp.iterations for sp in sps],
+                     "max_r2": max(sp.r2 for sp in sps)}
+    # one-at-a-time: the single-source solver, 8 calls
+    ys = [lo.ColorVector() for _ in shifts]
+    sp1 = [sp_new() for _ in range(nr)]
+
+    def one_by_one():
+        for j in range(nr):
+            Q.solveMulti(s, ys, bs[j], m, shifts, sp1[j])
+    timed(lambda: Q.solveMulti(s, ys, bs[0], m, shifts, sp_new()))  # warm
+    t1 = timed(one_by_one)
+    res["one_at_a_time"] = {"seconds": round(t1, 4), "iterations": [sp.iterations for sp in sp1]}
+    # agreement of the batched and single solutions (source 7)
+    e = lo.ColorVector()
+    errs = []
+    for k in range(len(shifts)):
+        e.copy_from(xs[7][k], "even")
+        e.axpby(-1.0, ys[k], 1.0, "even")
+        errs.append((e.norm2("even") / ys[k].norm2("even")) ** 0.5)
+    res["max_rel_diff_vs_single"] = max(errs)
+    for name in ("batched_graph_tuned", "batched_graph_tuned_nofreeze", "batched_nograph_block256"):
+        res[name]["speedup_vs_one_at_a_time"] = round(t1 / res[name]["seconds"], 3)
+    out["multishift_multirhs"] = res
+    del s, bs, xs, ys, e
+    torch.cuda.empty_cache()
+
+    # ---- (b)
+    lats = [(8, 8, 8, 8)] * 4 + [(12, 12, 12, 12)] * 4 + [(16, 16, 16, 16)] * 4 + [(16, 16, 16, 32)] * 4
+    rng = np.random.default_rng(SEED)
+    masses = list(rng.uniform(0.01, 0.1, len(lats)))
+    los, ss, bs, xs = [], [], [], []
+    for k, Lk in enumerate(lats):
+        lk = Q.Layout(Lk)
+        Uk = gauge.random_gauge(lk, SEED + 100 + k)
+        gauge.phase_bc(lk, Uk)
+        los.append(lk)
+        ss.append(Q.Staggered(lk, Uk))
+        bs.append(lk.ColorVector().set(gauge.source(lk, 300 + k, 0, 0)))
+        xs.append(lk.ColorVector())
+    res = {"lattices": [list(v) for v in lats], "masses": masses, "r2req": R2REQ}
+    for name, kw in [("batched_graph_tuned", dict(graph=True, tune=True)),
+                     ("batched_nograph_block256", dict(graph=False, tune=False))]:
+        best = None
+        for rep in range(3):
+            sps = [sp_new() for _ in lats]
+            t = timed(lambda: Q.solveBatch(ss, xs, bs, masses, sps, **kw))
+            if best is None or t < best[0]:
+                best = (t, sps)
+        t, sps = best
+        res[name] = {"seconds": round(t, 5), "iterations": [sp.iterations for sp in sps],
+                     "max_true_r2": max(sp.r2 for sp in sps)}
+    y = [lk.ColorVector() for lk in los]
+
+    def one_by_one_cg():
+        for k in range(len(lats)):
+            Q.solveEE(ss[k], y[k], bs[k], masses[k], sp1b[k])
+    sp1b = [sp_new() for _ in lats]
+    one_by_one_cg()  # warm (tuning, link packing)
+    sp1b = [sp_new() for _ in lats]
+    t1 = timed(one_by_one_cg)
+    res["one_at_a_time"] = {"seconds": round(t1, 5), "iterations": [sp.iterations for sp in sp1b],
+                            "max_true_r2": max(sp.r2 for sp in sp1b)}
+    for name in ("batched_graph_tuned", "batched_nograph_block256"):
+        res[name]["speedup_vs_one_at_a_time"] = round(t1 / res[name]["seconds"], 3)
+    out["uneven_cg_batch"] = res
+    out["autotune"] = {k: {"block": v[0], "xcd_remap": v[1], "us": v[2]} for k, v in Q.autotune_report().items()}
     return out
 
 
@@ -326,10 +454,13 @@ def main():
                 out["cg_best_mixed"] = {"variant": best, "speedup_vs_fp64": cg[best]["speedup_vs_fp64"]}
         out["cg"] = cg
 
-    if args.hisq_cg:
+    if args.hisq_cg or args.batched:
         del s
         torch.cuda.empty_cache()
+    if args.hisq_cg:
         out["hisq_cg"] = hisq_study()
+    if args.batched:
+        out["batched"] = batched_study()
 
     # ---- CPU baseline (oracle on host cores) ----
     if not args.no_cpu:

@@ -256,7 +256,8 @@ def test_bf16_small_mass_hisq_stays_finite_and_converges(oracle, gpu, maxits):
     gpu.solveEE(s, fx, fb, m, sp)
     r = b - O.stagD2xx(L, g, 0, fx.get(), m * m)
     r2 = O.norm2(r[: lo.nEven]) / O.norm2(b)
-    assert np.isfinite(r2) and abs(r2 - sp.r2) <= 1e-6 * r2 + 1e-30
+    # both are FP64 b - A x: at |r|/|b| ~ 1e-10 they agree to rounding (~1e-5 relative)
+    assert np.isfinite(r2) and abs(r2 - sp.r2) <= 1e-3 * r2 + 1e-30
     if maxits < 1000:
         assert r2 < 1.0
     else:
