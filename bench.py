@@ -107,7 +107,57 @@ def cpu_baseline(budget_s):
         "hop_ms": round(hop_s * 1e3, 3),
         "hop_gflops": round(HOP_FLOPS * vh / hop_s / 1e9, 3),
         "cg_ms_per_iteration": round(cg_it_s * 1e3, 3),
+        "lattice_8x8x8x8": cpu_small_case(),
     }
+
+
+def cpu_small_case():
+    """The CPU-sized case: 8^4, m = 0.1, (m^2 - D_eo D_oe) x = b to |r|/|b| <= 1e-10
+    with the C oracle; hop GFLOP/s and CG iterations."""
+    import numpy as np
+    from oracle import oracle as O
+    L = (8, 8, 8, 8)
+    U = O.phased(L, O.gauge_random(L, SEED))
+    x = O.source_gauss(L, 5, 0, 2)
+    r = np.zeros_like(x)
+    vh = O.volume(L) // 2
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 0.5:
+        O.stagD2(L, U, 0, r, x, 0.0, 0.0)
+        n += 1
+    hop_s = (time.perf_counter() - t0) / n
+    b = O.source_gauss(L, 7, 0, 0)
+    t0 = time.perf_counter()
+    _, its, _ = O.cg(L, U, 0, b, 0.1, R2REQ)
+    cg_s = time.perf_counter() - t0
+    return {"mass": 0.1, "hop_gflops": round(HOP_FLOPS * vh / hop_s / 1e9, 3), "hop_us": round(hop_s * 1e6, 2),
+            "cg_iterations": its, "cg_seconds": round(cg_s, 4)}
+
+
+def gpu_small_case():
+    """The same 8^4, m = 0.1 solve and hop on the GPU."""
+    import torch
+    import qex_amd as Q
+    from qex_amd import gauge
+    lo = Q.Layout((8, 8, 8, 8))
+    U = gauge.random_gauge(lo, SEED)
+    gauge.phase_bc(lo, U)
+    s = Q.Staggered(lo, U)
+    ms = Q.time_hop(s, 0, 200)
+    b = lo.ColorVector().set(gauge.source(lo, 7, 0, 0))
+    x = lo.ColorVector()
+    sp = Q.SolverParams()
+    sp.r2req, sp.verbosity = R2REQ, 0
+    Q.solveEE(s, x, b, 0.1, sp)
+    sp = Q.SolverParams()
+    sp.r2req, sp.verbosity = R2REQ, 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Q.solveEE(s, x, b, 0.1, sp)
+    torch.cuda.synchronize()
+    return {"mass": 0.1, "hop_gflops": round(HOP_FLOPS * lo.nEven / (ms * 1e-3) / 1e9, 2), "hop_us": round(ms * 1e3, 2),
+            "cg_iterations": sp.iterations, "cg_seconds": round(time.perf_counter() - t0, 5),
+            "true_rel_resid": float(sp.r2) ** 0.5}
 
 
 def _solve_stats(Q, torch, s, x, b, mass, r2req, maxits, sloppy, sloppy_links=None, recon=18):
@@ -465,6 +515,7 @@ def main():
     # ---- CPU baseline (oracle on host cores) ----
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["gpu_8x8x8x8"] = gpu_small_case()
     else:
         out["cpu_baseline"] = None
 

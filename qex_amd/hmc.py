@@ -235,26 +235,55 @@ def _staples(L, U, mu, site):
     return A
 
 
+def pseudofermion(lo, s, mass, seed):
+    """phi = (D^+(m) eta)_e with eta complex Gaussian on all sites (hmcAction.nim:794-818);
+    canonical (V, 3) torch tensor, zero on odd sites."""
+    eta = lo.ColorVector().set(G.source(lo, seed, stream=7, par=2))
+    phi = lo.ColorVector()
+    Q.Ddag(s, phi, eta, mass)
+    phi.zero("odd")
+    return phi.get(device="cuda")
+
+
 def _su3_alg(rng):
     a = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
     x = 0.5 * (a - a.conj().T)
     return x - np.trace(x) / 3 * np.eye(3)
 
 
-def force_fd_check(lo, Uph, phi, mass, nlinks=64, seed=1, h=1e-5, r2req=1e-26, beta=None, U=None):
+def _expm1_su3(A):
+    """exp(A) - 1 for a small 3x3 matrix, by its power series (no cancellation)."""
+    out = np.zeros((3, 3), complex)
+    term = np.eye(3, dtype=complex)
+    for k in range(1, 12):
+        term = term @ A / k
+        out += term
+    return out
+
+
+def force_fd_check(lo, Uph, phi, mass, nlinks=64, seed=1, h=1e-5, r2req=1e-28, beta=None, U=None, details=None,
+                   richardson=False):
     """Finite-difference check of the fermion force (and, with beta and plain links
     U, of the Wilson gauge force) on nlinks seeded links.
 
     For U_mu(x) -> exp(w X) U_mu(x), X a random su(3) element, compares
-    -1/2 Re tr(X F_mu(x)) with a central difference of the action.  The fermion
-    action difference uses the exact identity
-        S(U') - S(U) = -Y'^+ (M' - M) Y,   M = m^2 - D_eo D_oe,  Y = M^-1 phi,
-    so the comparison is limited by the solver residual and not by h; the gauge
-    action is linear in each link, so its difference is -(beta/3) Re tr((U'-U) A)
-    with A the staple sum.  Returns (worst relative error of the fermion force,
-    worst relative error of the gauge force or None, seconds)."""
-    import scipy.linalg
+    -1/2 Re tr(X F_mu(x)) with the central difference (S(+h) - S(-h))/2h.  The
+    fermion action difference uses the exact identity
+        S(U') - S(U) = -Y'^+ (M' - M) Y,   M = m^2 - D_eo D_oe,  Y = M^-1 phi.
+    (M' - M) Y is applied directly through the link difference dU (the
+    operator is linear in the links: (2D')(2D') - (2D)(2D) = (2D)(2D_dU) +
+    (2D_dU)(2D')), so no O(1) quantities are subtracted; Y' = Y + d with
+    M' d = phi - M' Y solved as a correction, so the error of Y cancels from
+    Y'.  The gauge action is linear in each link: its difference is
+    -(beta/3) Re tr(dU A) with A the staple sum.
 
+    richardson=True also takes the central difference at 2h and reports
+    (4 D(h) - D(2h))/3, which removes the O(h^2) truncation term.
+
+    Returns (worst relative error of the fermion force, worst relative error
+    of the gauge force or None, seconds); relative to max(|analytic|, 1e-3).
+    details, if a list, receives (mu, site, finite difference, analytic,
+    1/2 |X| |F_mu(x)|) per link."""
     t0 = time.perf_counter()
     L = lo.physGeom
     s = Q.Staggered(lo, Uph)
@@ -266,33 +295,56 @@ def force_fd_check(lo, Uph, phi, mass, nlinks=64, seed=1, h=1e-5, r2req=1e-26, b
     F = Q.fermForce(s, psi, device=Uph.device)
     Fg = wilson_force(lo, U, beta) if beta is not None else None
     Up = Uph.clone()
-    sp_ = Q.Staggered(lo, Up)
-    a, b, y = lo.ColorVector(), lo.ColorVector(), lo.ColorVector()
+    s_p = Q.Staggered(lo, Up)
+    Z = torch.zeros_like(Uph)
+    s_d = Q.Staggered(lo, Z)
+    a, t1, t2, u1, u2, rhs, dd = (lo.ColorVector() for _ in range(7))
     m2 = mass * mass
     Q.stagD2ee(s.se, s.so, a, s.g, psi, m2)  # 4 M Y
+    spd = Q.SolverParams()
+    spd.r2req, spd.verbosity, spd.maxits = 1e-18, 0, 100000
     rng = np.random.default_rng(seed)
     worst_f, worst_g = 0.0, 0.0
     for _ in range(nlinks):
         mu, site = int(rng.integers(4)), int(rng.integers(lo.nSites))
         X = _su3_alg(rng)
+        u0 = Uph[mu, site].cpu().numpy()
         dS = []
-        for sg in (1.0, -1.0):
-            E = torch.from_numpy(scipy.linalg.expm(sg * h * X)).to(Uph.device)
-            Up[mu, site] = E @ Uph[mu, site]
-            sp_.update(Up)
-            Q.solveEE(sp_, y, fphi, mass, sp)  # y = (4 M')^-1 phi
-            Q.stagD2ee(sp_.se, sp_.so, b, sp_.g, psi, m2)  # 4 M' Y
-            b.axpby(-1.0, a, 1.0, "even")
-            dS.append(-y.redot(b, "even"))
+        steps = (h, -h, 2 * h, -2 * h) if richardson else (h, -h)
+        for w in steps:
+            dU = _expm1_su3(w * X) @ u0
+            Up[mu, site] = Uph[mu, site] + torch.from_numpy(dU).to(Uph.device)
+            s_p.update(Up)
+            Z[mu, site] = torch.from_numpy(dU).to(Uph.device)
+            s_d.update(Z)
+            Q.stagD2(s_p.so, t1, s_p.g, psi, 0.0, 0.0)  # (2D')_oe Y
+            Q.stagD2(s_d.se, u1, s_d.g, t1, 0.0, 0.0)  # (2D_dU)_eo (2D')_oe Y
+            Q.stagD2(s_d.so, t2, s_d.g, psi, 0.0, 0.0)  # (2D_dU)_oe Y
+            Q.stagD2(s.se, u2, s.g, t2, 0.0, 0.0)  # (2D)_eo (2D_dU)_oe Y
+            u1.axpby(1.0, u2, 1.0, "even")  # -4 (M' - M) Y
+            rhs.copy_from(a, "even")
+            rhs.axpby(4.0, fphi, -1.0, "even")
+            rhs.axpby(1.0, u1, 1.0, "even")  # 4 (phi - M' Y)
+            Q.solveEE(s_p, dd, rhs, mass, spd)  # d = M'^-1 (phi - M' Y)
+            dd.axpby(1.0, psi, 1.0, "even")  # Y' = Y + d
+            dS.append(0.25 * dd.redot(u1, "even"))
+            Z[mu, site] = 0.0
         Up[mu, site] = Uph[mu, site]
         fd = (dS[0] - dS[1]) / (2 * h)
-        an = -0.5 * np.trace(X @ F[mu, site].cpu().numpy()).real
+        if richardson:
+            fd = (4.0 * fd - (dS[2] - dS[3]) / (4 * h)) / 3.0
+        Fx = F[mu, site].cpu().numpy()
+        an = -0.5 * np.trace(X @ Fx).real
         worst_f = max(worst_f, abs(fd - an) / max(abs(an), 1e-3))
+        if details is not None:
+            details.append((mu, site, fd, an, 0.5 * np.linalg.norm(X) * np.linalg.norm(Fx)))
         if Fg is not None:
             A = _staples(L, U, mu, site)
             u = U[mu, site].cpu().numpy()
-            dU = (scipy.linalg.expm(h * X) - scipy.linalg.expm(-h * X)) @ u
-            fdg = -(beta / 3.0) * np.trace(dU @ A).real / (2 * h)
+            def gd(hh):
+                dUg = (_expm1_su3(hh * X) - _expm1_su3(-hh * X)) @ u
+                return -(beta / 3.0) * np.trace(dUg @ A).real / (2 * hh)
+            fdg = (4.0 * gd(h) - gd(2 * h)) / 3.0 if richardson else gd(h)
             ang = -0.5 * np.trace(X @ Fg[mu, site].cpu().numpy()).real
             worst_g = max(worst_g, abs(fdg - ang) / max(abs(ang), 1e-3))
     torch.cuda.synchronize()
@@ -328,7 +380,7 @@ def main(argv=None):
     if args.fd_links:
         U = G.random_gauge(lo, args.seed)
         Uph = G.phase_bc(lo, U.clone())
-        phi = G.source(lo, args.seed + 1, stream=0, par=0)
+        phi = pseudofermion(lo, Q.Staggered(lo, Uph), args.mass, args.seed + 1)
         wf, wg, sec = force_fd_check(lo, Uph, phi, args.mass, nlinks=args.fd_links, seed=args.seed,
                                      beta=args.beta, U=U)
         print(json.dumps({"check": "force_fd", "lat": args.lat, "mass": args.mass, "links": args.fd_links,
