@@ -362,7 +362,7 @@ def batched_study():
     for name in ("batched_graph_tuned", "batched_nograph_block256"):
         res[name]["speedup_vs_one_at_a_time"] = round(t1 / res[name]["seconds"], 3)
     out["uneven_cg_batch"] = res
-    out["autotune"] = {k: {"block": v[0], "xcd_remap": v[1], "us": v[2]} for k, v in Q.autotune_report().items()}
+    out["autotune"] = Q.autotune_report()
     return out
 
 
@@ -519,6 +519,7 @@ def main():
     else:
         out["cpu_baseline"] = None
 
+    out["autotune"] = Q.autotune_report()
     print(json.dumps(out), flush=True)
 
 

@@ -159,6 +159,9 @@ int qexhip_solve_multishift_batch(qexhip_links *l, int par, int nrhs, qexhip_fie
    QEXHIP_TUNE=0 or QEXHIP_BLOCK is set), and a text table of the shapes chosen */
 int qexhip_set_autotune(int on);
 int qexhip_autotune_report(char *buf, int len);
+/* CG iteration form: 2 = two stencil kernels forming p on the fly, 3 = a
+   streaming update kernel plus two stencil kernels; 0 = chosen by the tuner */
+int qexhip_set_cg_form(int form);
 
 /* ---- fermion force (src/hmc/hmcAction.nim:669-792) ---- */
 /* stagForceSolve (hmcAction.nim:669-721): psi_e = (m^2 - D_eo D_oe)^-1 phi_e,

@@ -6,4 +6,5 @@ from ._lib import ALL, BF16, EVEN, FP16, FP32, FP64, ODD, QexHipError, load  # n
 from .stag import (D, Ddag, Field, Layout, SloppyType, SolverBackend, SolverParams,  # noqa: F401
                    Staggered, eoReconstruct, eoReduce, fermForce, newLayout, newSolverParams, newStag,
                    newStag3, solve, solveBatch, solveEE, solveMulti, solveMultiBatch, solveMultiMass, solveOO,
-                   stagD, stagD2, stagD2ee, stagD2oo, stagForceSolve, time_hop, autotune_report, set_autotune)
+                   stagD, stagD2, stagD2ee, stagD2oo, stagForceSolve, time_hop, autotune_report, set_autotune,
+                   set_cg_form)

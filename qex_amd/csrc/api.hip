@@ -227,6 +227,7 @@ void pull(Scal *Sh, const Scal *S) {
 struct Tuned {
   int block, remap;
   double us;
+  int form = 2;  // CG iteration: 2 or 3 kernels (dslash.hip)
 };
 std::map<std::string, Tuned> g_tune;
 
@@ -253,18 +254,25 @@ double time_us(const std::function<void()> &f, int warm, int reps) {
   return 1e3 * (double)t / reps;
 }
 
-// f(block, remap) runs the kernel sequence once
-Tuned tune(const std::string &key, const std::function<void(int, int)> &f, bool with_remap = true) {
+int g_cg_form = 0;  // QEXHIP_CG_FORM=2/3 pins the CG iteration form; 0: tuned
+
+// f(block, remap, form) runs the kernel sequence once; forms lists the
+// iteration forms to try (the hop has one)
+Tuned tune(const std::string &key, const std::function<void(int, int, int)> &f,
+           std::initializer_list<int> forms = {2}) {
   auto it = g_tune.find(key);
   if (it != g_tune.end()) return it->second;
-  if (g_block_forced || !g_autotune) return Tuned{g_block, g_remap, 0.0};
-  Tuned best{256, 1, 1e300};
-  for (int block : {64, 128, 256})
-    for (int remap : {1, 0}) {
-      if (!with_remap && remap == 0) continue;
-      const double us = time_us([&] { f(block, remap); }, 2, 8);
-      if (us < best.us) best = Tuned{block, remap, us};
-    }
+  const int fixed_form = g_cg_form ? g_cg_form : 2;
+  if (g_block_forced || !g_autotune) return Tuned{g_block, g_remap, 0.0, fixed_form};
+  Tuned best{256, 1, 1e300, fixed_form};
+  for (int form : forms) {
+    if (g_cg_form && form != g_cg_form) continue;
+    for (int block : {64, 128, 256})
+      for (int remap : {1, 0}) {
+        const double us = time_us([&] { f(block, remap, form); }, 2, 8);
+        if (us < best.us) best = Tuned{block, remap, us, form};
+      }
+  }
   g_tune[key] = best;
   return best;
 }
@@ -316,7 +324,7 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
   h.red = l->lat->red;
   const int pv = (prec == PQ16) ? P16 : prec;
   const HopKey hk{prec, rc, pv, pv, pv};
-  const Tuned t = tune(key, [&](int block, int remap) {
+  const Tuned t = tune(key, [&](int block, int remap, int) {
     h.remap = remap;
     launch_hop(hk, l->nl, 0, 0, h, g_stream, block);
   });
@@ -349,10 +357,13 @@ Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, const CgArgs &real) {
   h.sigma = h.pscale = 1.0;
   HIPCHK(hipMemcpyAsync(S, &h, sizeof h, hipMemcpyHostToDevice, g_stream));
   a.S = S;
-  const Tuned t = tune(key, [&](int block, int remap) {
-    a.remap = remap;
-    launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, block);
-  });
+  const Tuned t = tune(
+      key,
+      [&](int block, int remap, int form) {
+        a.remap = remap;
+        launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, block, form);
+      },
+      {2, 3});
   HIPCHK(hipStreamSynchronize(g_stream));
   for (int k = 0; k < 5; k++) (void)hipFree(v[k]);
   (void)hipFree(S);
@@ -417,7 +428,8 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   a.par = par;
   a.S = S;
   a.red = lat->red;
-  const Tuned tu = tuned_cg(l, pl, rc, ps, a);
+  Tuned tu = tuned_cg(l, pl, rc, ps, a);
+  if (g_cg_form) tu.form = g_cg_form;
   a.remap = tu.remap;
   if (!mixed) {
     a.x = x; a.r = w->r64; a.p = w->p64; a.Ap = w->Ap64; a.t = w->t64;
@@ -442,7 +454,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
     pull(Sh, S);
   }
   while (!Sh->done) {
-    for (int k = 0; k < kChunk; k++) launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, tu.block);
+    for (int k = 0; k < kChunk; k++) launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, tu.block, tu.form);
     check_launch();
     pull(Sh, S);
     if (Sh->done || Sh->need_rel) {
@@ -1061,6 +1073,10 @@ int qexhip_init(int device) {
     }
   }
   if (const char *t = getenv("QEXHIP_TUNE")) g_autotune = atoi(t) != 0;
+  if (const char *f = getenv("QEXHIP_CG_FORM")) {
+    const int v = atoi(f);
+    if (v == 2 || v == 3) g_cg_form = v;
+  }
   if (const char *r = getenv("QEXHIP_XCD_REMAP")) g_remap = atoi(r) != 0;
   API_END
 }
@@ -1563,14 +1579,21 @@ int qexhip_set_autotune(int on) {
   return 0;
 }
 
+int qexhip_set_cg_form(int form) {
+  API_BEGIN
+  require(form == 0 || form == 2 || form == 3, "cg form must be 0 (tuned), 2 or 3");
+  g_cg_form = form;
+  API_END
+}
+
 int qexhip_autotune_report(char *buf, int len) {
   API_BEGIN
   require(buf && len > 0, "bad buffer");
   std::string out;
   for (auto &kv : g_tune) {
     char line[256];
-    snprintf(line, sizeof line, "%s: block=%d remap=%d us=%.2f\n", kv.first.c_str(), kv.second.block,
-             kv.second.remap, kv.second.us);
+    snprintf(line, sizeof line, "%s: block=%d remap=%d form=%d us=%.2f\n", kv.first.c_str(),
+             kv.second.block, kv.second.remap, kv.second.form, kv.second.us);
     out += line;
   }
   snprintf(buf, (size_t)len, "%s", out.c_str());

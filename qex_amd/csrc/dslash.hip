@@ -136,43 +136,73 @@ __global__ void __launch_bounds__(256) k_cg2(CgArgs P) {
 }
 
 template <int PS, int PR, int PX, class A>
+__global__ void __launch_bounds__(256) k_cg0(CgArgs P) {
+  cg0_body<PS, PR, PX, A>(P, blockIdx.x);
+}
+
+template <int PL, int RC, int PS, class A, int NL>
+__global__ void __launch_bounds__(256) k_cg1b(CgArgs P) {
+  cg1b_body<PL, RC, PS, A, NL>(P, xcd_block(P.remap));
+}
+
+template <int PL, int RC, int PS, int PR, class A, int NL>
+__global__ void __launch_bounds__(256) k_cg2b(CgArgs P) {
+  __shared__ double lds[16 * 4];
+  cg2b_body<PL, RC, PS, PR, A, NL>(P, xcd_block(P.remap), lds, blockIdx.x);
+}
+
+template <int PS, int PR, int PX, class A>
 __global__ void __launch_bounds__(256) k_cg_flush(CgArgs P) {
   __shared__ double lds[16];
   cg_flush_body<PS, PR, PX, A>(P, blockIdx.x, lds, blockIdx.x);
 }
 
 template <int PL, int RC, int PS, int NL>
-static void cg_iter_t(const CgArgs &a, hipStream_t st, int block) {
+static void cg_iter_t(const CgArgs &a, hipStream_t st, int block, int form) {
   dim3 grid((unsigned)((a.g.Vh + block - 1) / block)), blk(block);
   if constexpr (PL == P64) {
-    hipLaunchKernelGGL((k_cg1<PL, RC, P64, P64, P64, double2, NL>), grid, blk, 0, st, a);
-    hipLaunchKernelGGL((k_cg2<PL, RC, P64, P64, P64, double2, NL>), grid, blk, 0, st, a);
+    if (form == 3) {
+      hipLaunchKernelGGL((k_cg0<P64, P64, P64, double2>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg1b<PL, RC, P64, double2, NL>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg2b<PL, RC, P64, P64, double2, NL>), grid, blk, 0, st, a);
+    } else {
+      hipLaunchKernelGGL((k_cg1<PL, RC, P64, P64, P64, double2, NL>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg2<PL, RC, P64, P64, P64, double2, NL>), grid, blk, 0, st, a);
+    }
   } else {
-    hipLaunchKernelGGL((k_cg1<PL, RC, PS, P32, P32, float2, NL>), grid, blk, 0, st, a);
-    hipLaunchKernelGGL((k_cg2<PL, RC, PS, P32, P32, float2, NL>), grid, blk, 0, st, a);
+    if (form == 3) {
+      hipLaunchKernelGGL((k_cg0<PS, P32, P32, float2>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg1b<PL, RC, PS, float2, NL>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg2b<PL, RC, PS, P32, float2, NL>), grid, blk, 0, st, a);
+    } else {
+      hipLaunchKernelGGL((k_cg1<PL, RC, PS, P32, P32, float2, NL>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg2<PL, RC, PS, P32, P32, float2, NL>), grid, blk, 0, st, a);
+    }
   }
   FinArgs f{a.red, (int)grid.x, a.S, nullptr, 0, 0.0, 1};
   launch_finish(EPI_CG2, f, st);
 }
 
 template <int PL, int PS>
-static bool cg_iter_rc(int rc, int nl, const CgArgs &a, hipStream_t st, int block) {
-  if (rc == 18 && nl == 4) { cg_iter_t<PL, 18, PS, 4>(a, st, block); return true; }
-  if (rc == 18 && nl == 8) { cg_iter_t<PL, 18, PS, 8>(a, st, block); return true; }
-  if (rc == 12 && nl == 4) { cg_iter_t<PL, 12, PS, 4>(a, st, block); return true; }
+static bool cg_iter_rc(int rc, int nl, const CgArgs &a, hipStream_t st, int block, int form) {
+  if (rc == 18 && nl == 4) { cg_iter_t<PL, 18, PS, 4>(a, st, block, form); return true; }
+  if (rc == 18 && nl == 8) { cg_iter_t<PL, 18, PS, 8>(a, st, block, form); return true; }
+  if (rc == 12 && nl == 4) { cg_iter_t<PL, 12, PS, 4>(a, st, block, form); return true; }
   return false;
 }
 
-void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block) {
+// form 2: two stencil kernels (p formed on the fly); form 3: a streaming update
+// kernel plus two stencil kernels that gather one vector each
+void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form) {
   bool ok = false;
-  if (pl == P64 && ps == P64) ok = cg_iter_rc<P64, P64>(rc, nl, a, st, block);
-  else if (pl == P32 && ps == P32) ok = cg_iter_rc<P32, P32>(rc, nl, a, st, block);
-  else if (pl == P16 && ps == P16) ok = cg_iter_rc<P16, P16>(rc, nl, a, st, block);
-  else if (pl == P16 && ps == P32) ok = cg_iter_rc<P16, P32>(rc, nl, a, st, block);
-  else if (pl == PBF && ps == PBF) ok = cg_iter_rc<PBF, PBF>(rc, nl, a, st, block);
-  else if (pl == PBF && ps == P32) ok = cg_iter_rc<PBF, P32>(rc, nl, a, st, block);
-  else if (pl == PQ16 && ps == P16) ok = cg_iter_rc<PQ16, P16>(rc, nl, a, st, block);
-  else if (pl == PQ16 && ps == P32) ok = cg_iter_rc<PQ16, P32>(rc, nl, a, st, block);
+  if (pl == P64 && ps == P64) ok = cg_iter_rc<P64, P64>(rc, nl, a, st, block, form);
+  else if (pl == P32 && ps == P32) ok = cg_iter_rc<P32, P32>(rc, nl, a, st, block, form);
+  else if (pl == P16 && ps == P16) ok = cg_iter_rc<P16, P16>(rc, nl, a, st, block, form);
+  else if (pl == P16 && ps == P32) ok = cg_iter_rc<P16, P32>(rc, nl, a, st, block, form);
+  else if (pl == PBF && ps == PBF) ok = cg_iter_rc<PBF, PBF>(rc, nl, a, st, block, form);
+  else if (pl == PBF && ps == P32) ok = cg_iter_rc<PBF, P32>(rc, nl, a, st, block, form);
+  else if (pl == PQ16 && ps == P16) ok = cg_iter_rc<PQ16, P16>(rc, nl, a, st, block, form);
+  else if (pl == PQ16 && ps == P32) ok = cg_iter_rc<PQ16, P32>(rc, nl, a, st, block, form);
   if (!ok) fprintf(stderr, "qexhip: no CG instantiation for pl=%d rc=%d ps=%d nl=%d\n", pl, rc, ps, nl);
 }
 

@@ -138,7 +138,7 @@ void launch_finish(int epi, const FinArgs &f, hipStream_t st);
 
 // dslash.hip
 void launch_hop(const HopKey &k, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block);
-void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block);
+void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form);
 void launch_cg_flush(int ps, const CgArgs &a, hipStream_t st);
 void launch_pack_links(int prec, int rc, const Geom &g, int nl, const double *canon, char *out, const float *qs,
                        hipStream_t st);

@@ -201,6 +201,92 @@ __device__ __forceinline__ void cg2_body(const CgArgs &P, long blk, double *lds,
   block_partials_at<4>(red, lds, P.red, pidx);
 }
 
+// Three-kernel form of the same iteration: (0) own parity, streaming: apply
+// the pending update and store p_n = r_{n-1} + beta p_{n-1}; (1) other parity:
+// t = (2D) p_n gathering p_n alone; (2) own parity: Ap_n = 4 m^2 p_n - (2D) t
+// and the four dot products.  Same algorithmic bytes as the two-kernel form,
+// but the stencil kernels gather one vector instead of forming p_n from three
+// at every neighbour.
+template <int PS, int PR, int PX, class A>
+__device__ __forceinline__ void cg0_body(const CgArgs &P, long blk) {
+  using R = RT<A>;
+  const Scal *S = P.S;
+  if (S->done | S->need_rel) return;
+  const Geom &g = P.g;
+  const long i = blk * blockDim.x + threadIdx.x;
+  if (i >= g.Vh) return;
+  const bool first = S->first != 0;
+  const R alpha = (R)S->alpha, beta = (R)(S->beta * S->pscale);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const long off = c * g.stride + i;
+    const A rv = ldc<PR, A>(P.r, off), av = ldc<PS, A>(P.Ap, off);
+    const A pv = first ? mk<A>(0, 0) : ldc<PS, A>(P.p, off);
+    const A xv = ldc<PX, A>(P.x, off);
+    const A rn = mk<A>(rv.x - alpha * av.x, rv.y - alpha * av.y);
+    const A pn = rt_store<PS, A>(first ? rn : mk<A>(rn.x + beta * pv.x, rn.y + beta * pv.y));
+    stc<PX, A>(P.x, off, mk<A>(xv.x + alpha * pv.x, xv.y + alpha * pv.y));
+    stc<PR, A>(P.r, off, rn);
+    stc<PS, A>(P.p, off, pn);
+  }
+}
+
+template <int PL, int RC, int PS, class A, int NL>
+__device__ __forceinline__ void cg1b_body(const CgArgs &P, long blk) {
+  const Scal *S = P.S;
+  if (S->done | S->need_rel) return;
+  const Geom &g = P.g;
+  const long i = blk * blockDim.x + threadIdx.x;
+  if (i >= g.Vh) return;
+  const int q = 1 - P.par;
+  const Site s = decode(g, i, q);
+  auto ld = [&](long j, A(&v)[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) v[c] = ldc<PS, A>(P.p, c * g.stride + j);
+  };
+  A res[3];
+  stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, q, i, s, ld, res);
+#pragma unroll
+  for (int c = 0; c < 3; c++) stc<PS, A>(P.t, c * g.stride + i, res[c]);
+}
+
+template <int PL, int RC, int PS, int PR, class A, int NL>
+__device__ __forceinline__ void cg2b_body(const CgArgs &P, long blk, double *lds, long pidx) {
+  using R = RT<A>;
+  Scal *S = P.S;
+  if (S->done | S->need_rel) return;
+  const Geom &g = P.g;
+  const long i = blk * blockDim.x + threadIdx.x;
+  double red[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < g.Vh) {
+    const R m4 = (R)(4.0 * S->m2);
+    A rv[3], pv[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const long off = c * g.stride + i;
+      rv[c] = ldc<PR, A>(P.r, off);
+      pv[c] = ldc<PS, A>(P.p, off);
+    }
+    const Site s = decode(g, i, P.par);
+    auto ld = [&](long j, A(&v)[3]) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[c] = ldc<PS, A>(P.t, c * g.stride + j);
+    };
+    A H[3];
+    stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, P.par, i, s, ld, H);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const A apn = rt_store<PS, A>(mk<A>(m4 * pv[c].x - H[c].x, m4 * pv[c].y - H[c].y));
+      stc<PS, A>(P.Ap, c * g.stride + i, apn);
+      red[0] += (double)pv[c].x * (double)apn.x + (double)pv[c].y * (double)apn.y;
+      red[1] += (double)rv[c].x * (double)apn.x + (double)rv[c].y * (double)apn.y;
+      red[2] += (double)apn.x * (double)apn.x + (double)apn.y * (double)apn.y;
+      red[3] += (double)rv[c].x * (double)rv[c].x + (double)rv[c].y * (double)rv[c].y;
+    }
+  }
+  block_partials_at<4>(red, lds, P.red, pidx);
+}
+
 // x += alpha p ; r -= alpha Ap ; |r|^2 (the pending update of the last iteration)
 template <int PS, int PR, int PX, class A>
 __device__ __forceinline__ void cg_flush_body(const CgArgs &P, long blk, double *lds, long pidx) {

@@ -416,15 +416,22 @@ def set_autotune(on=True):
     call("qexhip_set_autotune", int(bool(on)))
 
 
+def set_cg_form(form=0):
+    """Pin the CG iteration form (2 or 3 kernels per iteration); 0 lets the tuner choose."""
+    call("qexhip_set_cg_form", int(form))
+
+
 def autotune_report():
-    """{kernel family and shape: (block, xcd_remap, microseconds)} chosen so far."""
+    """{kernel family and shape: {"block", "remap", "form", "us"}} chosen so far
+    (form: 2- or 3-kernel CG iteration; us: time of one timed unit)."""
     buf = ctypes.create_string_buffer(1 << 16)
     call("qexhip_autotune_report", buf, len(buf))
     out = {}
     for line in buf.value.decode().splitlines():
         key, _, rest = line.rpartition(": ")
         f = dict(kv.split("=") for kv in rest.split())
-        out[key] = (int(f["block"]), int(f["remap"]), float(f["us"]))
+        out[key] = {"block": int(f["block"]), "remap": int(f["remap"]), "form": int(f["form"]),
+                    "us": float(f["us"])}
     return out
 
 

@@ -126,5 +126,6 @@ def test_multishift_batch_eight_rhs_matches_single(oracle, gpu):
 def test_autotune_report_lists_shapes(gpu):
     rep = gpu.autotune_report()
     assert isinstance(rep, dict)
-    for key, (block, remap, us) in rep.items():
-        assert block in (64, 128, 256) and remap in (0, 1) and us >= 0.0, key
+    for key, v in rep.items():
+        assert v["block"] in (64, 128, 256) and v["remap"] in (0, 1) and v["form"] in (2, 3), key
+        assert v["us"] >= 0.0, key

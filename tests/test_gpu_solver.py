@@ -262,3 +262,31 @@ def test_bf16_small_mass_hisq_stays_finite_and_converges(oracle, gpu, maxits):
         assert r2 < 1.0
     else:
         assert r2 <= 1.0001e-20
+
+
+@pytest.mark.parametrize("form", [2, 3])
+@pytest.mark.parametrize("sloppy,slinks,recon", [(0, None, 18), (2, 4, 12), (1, None, 18)])
+def test_cg_iteration_forms_agree(oracle, gpu, form, sloppy, slinks, recon):
+    """Both CG iteration forms (p formed on the fly in the stencil kernel, or
+    stored by a streaming kernel first) give the oracle's answer."""
+    O = oracle
+    L = (8, 8, 8, 8)
+    m = 0.05
+    g = links(O, L, False)
+    b = O.source_gauss(L, 19, 0, 0)
+    want, its_o, _ = O.solveXX(L, g, 0, b, m, 1e-20)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    fx = lo.ColorVector()
+    sp = gpu.SolverParams()
+    sp.r2req, sp.verbosity, sp.sloppySolve, sp.sloppyLinks, sp.recon = 1e-20, 0, sloppy, slinks, recon
+    gpu.set_cg_form(form)
+    try:
+        gpu.solveEE(s, fx, fb, m, sp)
+    finally:
+        gpu.set_cg_form(0)
+    assert sp.r2 <= 1e-20
+    if sloppy == 0:
+        assert abs(sp.iterations - its_o) <= 1
+    assert rel(fx.get(sub=0)[: lo.nEven], want[: lo.nEven]) < 1e-8
