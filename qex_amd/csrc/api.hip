@@ -228,6 +228,7 @@ struct Tuned {
   int block, remap;
   double us;
   int form = 2;  // CG iteration: 2 or 3 kernels (dslash.hip)
+  int cap = 0;   // hop: persistent form with at most cap workgroups (0: one pass)
 };
 std::map<std::string, Tuned> g_tune;
 
@@ -256,22 +257,24 @@ double time_us(const std::function<void()> &f, int warm, int reps) {
 
 int g_cg_form = 0;  // QEXHIP_CG_FORM=2/3 pins the CG iteration form; 0: tuned
 
-// f(block, remap, form) runs the kernel sequence once; forms lists the
-// iteration forms to try (the hop has one)
-Tuned tune(const std::string &key, const std::function<void(int, int, int)> &f,
-           std::initializer_list<int> forms = {2}) {
+// f(t) runs the kernel sequence once with shape t; the candidates are every
+// block size x block order x listed form x listed workgroup cap
+Tuned tune(const std::string &key, const std::function<void(const Tuned &)> &f,
+           std::initializer_list<int> forms = {2}, std::initializer_list<int> caps = {0}) {
   auto it = g_tune.find(key);
   if (it != g_tune.end()) return it->second;
   const int fixed_form = g_cg_form ? g_cg_form : 2;
-  if (g_block_forced || !g_autotune) return Tuned{g_block, g_remap, 0.0, fixed_form};
-  Tuned best{256, 1, 1e300, fixed_form};
+  if (g_block_forced || !g_autotune) return Tuned{g_block, g_remap, 0.0, fixed_form, 0};
+  Tuned best{256, 1, 1e300, fixed_form, 0};
   for (int form : forms) {
     if (g_cg_form && form != g_cg_form) continue;
-    for (int block : {64, 128, 256})
-      for (int remap : {1, 0}) {
-        const double us = time_us([&] { f(block, remap, form); }, 2, 8);
-        if (us < best.us) best = Tuned{block, remap, us, form};
-      }
+    for (int cap : caps)
+      for (int block : {64, 128, 256})
+        for (int remap : {1, 0}) {
+          const Tuned t{block, remap, 0.0, form, cap};
+          const double us = time_us([&] { f(t); }, 2, 8);
+          if (us < best.us) best = Tuned{block, remap, us, form, cap};
+        }
   }
   g_tune[key] = best;
   return best;
@@ -302,7 +305,7 @@ void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const v
   h.red = l->lat->red;
   const int pv = (prec == PQ16) ? P16 : prec;  // 16-bit fixed-point links go with FP16 vectors
   HopKey k{prec, rc, pv, pv, pv};
-  launch_hop(k, l->nl, mode, redm, h, g_stream, tu.block);
+  launch_hop(k, l->nl, mode, redm, h, g_stream, tu.block, tu.cap);
 }
 
 Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
@@ -324,10 +327,13 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
   h.red = l->lat->red;
   const int pv = (prec == PQ16) ? P16 : prec;
   const HopKey hk{prec, rc, pv, pv, pv};
-  const Tuned t = tune(key, [&](int block, int remap, int) {
-    h.remap = remap;
-    launch_hop(hk, l->nl, 0, 0, h, g_stream, block);
-  });
+  const Tuned t = tune(
+      key,
+      [&](const Tuned &c) {
+        h.remap = c.remap;
+        launch_hop(hk, l->nl, 0, 0, h, g_stream, c.block, c.cap);
+      },
+      {2}, {0, 1024, 2048});
   HIPCHK(hipStreamSynchronize(g_stream));
   (void)hipFree(in);
   (void)hipFree(out);
@@ -359,9 +365,9 @@ Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, const CgArgs &real) {
   a.S = S;
   const Tuned t = tune(
       key,
-      [&](int block, int remap, int form) {
-        a.remap = remap;
-        launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, block, form);
+      [&](const Tuned &c) {
+        a.remap = c.remap;
+        launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, c.block, c.form);
       },
       {2, 3});
   HIPCHK(hipStreamSynchronize(g_stream));
@@ -1592,8 +1598,8 @@ int qexhip_autotune_report(char *buf, int len) {
   std::string out;
   for (auto &kv : g_tune) {
     char line[256];
-    snprintf(line, sizeof line, "%s: block=%d remap=%d form=%d us=%.2f\n", kv.first.c_str(),
-             kv.second.block, kv.second.remap, kv.second.form, kv.second.us);
+    snprintf(line, sizeof line, "%s: block=%d remap=%d form=%d cap=%d us=%.2f\n", kv.first.c_str(),
+             kv.second.block, kv.second.remap, kv.second.form, kv.second.cap, kv.second.us);
     out += line;
   }
   snprintf(buf, (size_t)len, "%s", out.c_str());

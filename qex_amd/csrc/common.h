@@ -230,6 +230,33 @@ __device__ __forceinline__ long xcd_block(int remap) {
   return xcd * q + (xcd < r ? xcd : r) + idx;
 }
 
+// Logical blocks served by this workgroup.  The launch has gridDim.x blocks,
+// a multiple of 8 when remap is set, and may be smaller than the nblk logical
+// blocks (persistent form): with remap, XCD x (= blockIdx.x % 8) owns the
+// contiguous logical range [x*Q, (x+1)*Q), Q = ceil(nblk/8), which its
+// gridDim.x/8 workgroups walk with stride gridDim.x/8; without remap the
+// blocks stride over all logical blocks.  With gridDim.x = nblk rounded up to
+// 8 every workgroup serves at most one logical block (the xcd_block order).
+struct BlockIter {
+  long cur, end, step;
+};
+__device__ __forceinline__ BlockIter block_iter(long nblk, int remap) {
+  const long G = gridDim.x;
+  if (!remap) return BlockIter{(long)blockIdx.x, nblk, G};
+  const long per = G >> 3, x = blockIdx.x & 7, lane = blockIdx.x >> 3;
+  const long Q = (nblk + 7) >> 3;
+  const long end = (x + 1) * Q < nblk ? (x + 1) * Q : nblk;
+  return BlockIter{x * Q + lane, end, per};
+}
+
+// grid for nblk logical blocks: capped at cap workgroups when cap > 0,
+// rounded up to a multiple of 8 for the XCD-aware order
+inline unsigned grid_blocks(long nblk, int remap, int cap) {
+  long G = (cap > 0 && cap < nblk) ? cap : nblk;
+  if (remap) G = (G + 7) & ~7L;
+  return (unsigned)(G > 0 ? G : 1);
+}
+
 // ------------------------------------------------------------- link storage
 // One link = RC reals (18, or 12 = first two rows) in precision PL, stored as
 // 16-byte planes plus one narrower remainder plane:

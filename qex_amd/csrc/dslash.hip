@@ -23,9 +23,12 @@ __global__ void __launch_bounds__(256) k_hop(HopArgs P) {
   __shared__ double lds[16];
   if (P.pred && (P.S->done | P.S->need_rel)) return;
   const Geom &g = P.g;
-  const long i = xcd_block(P.remap) * blockDim.x + threadIdx.x;
   double red = 0.0;
-  if (i < g.Vh) {
+  const BlockIter it = block_iter((g.Vh + blockDim.x - 1) / blockDim.x, P.remap);
+#pragma unroll 1
+  for (long lb = it.cur; lb < it.end; lb += it.step) {
+    const long i = lb * blockDim.x + threadIdx.x;
+    if (i >= g.Vh) continue;
     const Site s = decode(g, i, P.par);
     A res[3];
     auto ld = [&](long j, A(&v)[3]) {
@@ -73,8 +76,8 @@ __global__ void __launch_bounds__(256) k_hop(HopArgs P) {
 }
 
 template <int PL, int RC, int PI, int PO, int PA, class A, int NL>
-static bool hop_modes(int mode, int redm, const HopArgs &a, hipStream_t st, int block) {
-  dim3 grid((unsigned)((a.g.Vh + block - 1) / block));
+static bool hop_modes(int mode, int redm, const HopArgs &a, hipStream_t st, int block, int cap) {
+  dim3 grid(grid_blocks((a.g.Vh + block - 1) / block, a.remap, cap));
 #define QH_HOP(M, R)                                                                                 \
   if (mode == M && redm == R) {                                                                      \
     hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R>), grid, dim3(block), 0, st, a);       \
@@ -91,30 +94,32 @@ static bool hop_modes(int mode, int redm, const HopArgs &a, hipStream_t st, int 
 }
 
 template <int PL, class A>
-static bool hop_prec(int rc, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block) {
-  if (rc == 18 && nl == 4) return hop_modes<PL, 18, PL, PL, PL, A, 4>(mode, redm, a, st, block);
-  if (rc == 18 && nl == 8) return hop_modes<PL, 18, PL, PL, PL, A, 8>(mode, redm, a, st, block);
-  if (rc == 12 && nl == 4) return hop_modes<PL, 12, PL, PL, PL, A, 4>(mode, redm, a, st, block);
+static bool hop_prec(int rc, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block, int cap) {
+  if (rc == 18 && nl == 4) return hop_modes<PL, 18, PL, PL, PL, A, 4>(mode, redm, a, st, block, cap);
+  if (rc == 18 && nl == 8) return hop_modes<PL, 18, PL, PL, PL, A, 8>(mode, redm, a, st, block, cap);
+  if (rc == 12 && nl == 4) return hop_modes<PL, 12, PL, PL, PL, A, 4>(mode, redm, a, st, block, cap);
   return false;
 }
 
 template <class A>
-static bool hop_q16(int rc, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block) {
-  if (rc == 18 && nl == 4) return hop_modes<PQ16, 18, P16, P16, P16, A, 4>(mode, redm, a, st, block);
-  if (rc == 18 && nl == 8) return hop_modes<PQ16, 18, P16, P16, P16, A, 8>(mode, redm, a, st, block);
-  if (rc == 12 && nl == 4) return hop_modes<PQ16, 12, P16, P16, P16, A, 4>(mode, redm, a, st, block);
+static bool hop_q16(int rc, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block, int cap) {
+  if (rc == 18 && nl == 4) return hop_modes<PQ16, 18, P16, P16, P16, A, 4>(mode, redm, a, st, block, cap);
+  if (rc == 18 && nl == 8) return hop_modes<PQ16, 18, P16, P16, P16, A, 8>(mode, redm, a, st, block, cap);
+  if (rc == 12 && nl == 4) return hop_modes<PQ16, 12, P16, P16, P16, A, 4>(mode, redm, a, st, block, cap);
   return false;
 }
 
-void launch_hop(const HopKey &k, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block) {
+// cap > 0: persistent form with at most cap workgroups (block_iter)
+void launch_hop(const HopKey &k, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block, int cap) {
   bool ok = false;
-  if (k.pl == PQ16 && k.pi == P16 && k.po == P16 && k.pa == P16) ok = hop_q16<float2>(k.rc, nl, mode, redm, a, st, block);
+  if (k.pl == PQ16 && k.pi == P16 && k.po == P16 && k.pa == P16)
+    ok = hop_q16<float2>(k.rc, nl, mode, redm, a, st, block, cap);
   if (k.pi == k.pl && k.po == k.pl && k.pa == k.pl) {
     switch (k.pl) {
-      case P64: ok = hop_prec<P64, double2>(k.rc, nl, mode, redm, a, st, block); break;
-      case P32: ok = hop_prec<P32, float2>(k.rc, nl, mode, redm, a, st, block); break;
-      case P16: ok = hop_prec<P16, float2>(k.rc, nl, mode, redm, a, st, block); break;
-      case PBF: ok = hop_prec<PBF, float2>(k.rc, nl, mode, redm, a, st, block); break;
+      case P64: ok = hop_prec<P64, double2>(k.rc, nl, mode, redm, a, st, block, cap); break;
+      case P32: ok = hop_prec<P32, float2>(k.rc, nl, mode, redm, a, st, block, cap); break;
+      case P16: ok = hop_prec<P16, float2>(k.rc, nl, mode, redm, a, st, block, cap); break;
+      case PBF: ok = hop_prec<PBF, float2>(k.rc, nl, mode, redm, a, st, block, cap); break;
     }
   }
   if (!ok)

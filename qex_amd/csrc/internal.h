@@ -137,7 +137,7 @@ struct HopKey {       // precision/format of one hop instantiation
 void launch_finish(int epi, const FinArgs &f, hipStream_t st);
 
 // dslash.hip
-void launch_hop(const HopKey &k, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block);
+void launch_hop(const HopKey &k, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block, int cap);
 void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form);
 void launch_cg_flush(int ps, const CgArgs &a, hipStream_t st);
 void launch_pack_links(int prec, int rc, const Geom &g, int nl, const double *canon, char *out, const float *qs,

@@ -422,8 +422,9 @@ def set_cg_form(form=0):
 
 
 def autotune_report():
-    """{kernel family and shape: {"block", "remap", "form", "us"}} chosen so far
-    (form: 2- or 3-kernel CG iteration; us: time of one timed unit)."""
+    """{kernel family and shape: {"block", "remap", "form", "cap", "us"}} chosen so
+    far (form: 2- or 3-kernel CG iteration; cap: workgroup cap of the persistent
+    hop, 0 = one pass; us: time of one timed unit)."""
     buf = ctypes.create_string_buffer(1 << 16)
     call("qexhip_autotune_report", buf, len(buf))
     out = {}
@@ -431,7 +432,7 @@ def autotune_report():
         key, _, rest = line.rpartition(": ")
         f = dict(kv.split("=") for kv in rest.split())
         out[key] = {"block": int(f["block"]), "remap": int(f["remap"]), "form": int(f["form"]),
-                    "us": float(f["us"])}
+                    "cap": int(f["cap"]), "us": float(f["us"])}
     return out
 
 
