@@ -455,6 +455,18 @@ def main():
             torch.cuda.empty_cache()
         out["hop_table"] = table
 
+    # ---- multi-RHS hop at 32^4: links loaded once for n fields ----
+    if not args.no_table:
+        mr = {}
+        for prec in (0, 1):
+            for n in (1, 2, 4, 8):
+                Q.time_hop_multi(s, prec, n, 3)
+                m_ = Q.time_hop_multi(s, prec, n, 50)
+                b = BYTES_PER_REAL[prec] * (8 * 18 + n * 12) * vh
+                mr[f"{PREC_NAME[prec]}_n{n}"] = {"ms": round(m_, 5), "ms_per_rhs": round(m_ / n, 5),
+                                                "GB_s": round(b / (m_ * 1e-3) / 1e9, 1)}
+        out["multi_rhs_hop"] = mr
+
     # ---- CG: FP64 vs mixed (reliable updates every 100 its or at 0.1x drop) ----
     if not args.no_cg:
         b = lo.ColorVector().set(gauge.source(lo, 7, 0, 0))

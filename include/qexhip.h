@@ -117,6 +117,11 @@ int qexhip_stagD(qexhip_links *l, int par, qexhip_field *r, const qexhip_field *
                  double a);
 /* stagD2ee / stagD2oo (stagD.nim:485-520): r_p = 4*m2*x_p - (2D)_pq (2D)_qp x_p */
 int qexhip_stagD2xx(qexhip_links *l, int par, qexhip_field *r, const qexhip_field *x, double m2);
+/* multi-RHS stagD2 (stagDPN/stagDN, stagD.nim:132-199,451-456): r_j = a*r_j + b*x_j + (2D)x_j
+   for j < n on parity par, each link loaded once for up to 8 fields; FP64 or FP32 fields.
+   QEX's stagDN(sd, r, g, x, m) is a = 0, b = m. */
+int qexhip_stagD2_multi(qexhip_links *l, int par, int n, qexhip_field *const *r, const qexhip_field *const *x,
+                        double a, double b);
 
 /* ---- solvers ---- */
 /* solveEE / solveOO (src/physics/stagSolve.nim:330-427): solve A r = x on
@@ -199,6 +204,9 @@ int qexhip_gen_hisq(qexhip_lattice *lat, uint64_t seed, const double *Uph_dev, d
    (QEXHIP_Q16: fixed-point links, FP16 fields) and links of recon (18 or 12)
    reals. */
 int qexhip_time_hop(qexhip_links *l, int prec, int recon, int nrep, double *ms_per_hop);
+/* the same for the multi-RHS hop over nrhs fields (FP64 or FP32, 18-real links):
+   ms for one application to all nrhs fields */
+int qexhip_time_hop_multi(qexhip_links *l, int prec, int nrhs, int nrep, double *ms);
 /* whether the links are phased SU(3) matrices, so 12-real storage is exact
    (1) or not (0) */
 int qexhip_links_recon_ok(qexhip_links *l, int *ok);

@@ -1321,6 +1321,72 @@ int qexhip_stagD2(qexhip_links *l, int par, qexhip_field *r, const qexhip_field 
   API_END
 }
 
+// r_j = a r_j + b x_j + (2D) x_j, j < n, links loaded once per group of up to 8
+static void dslash_multi(qexhip_links *l, int par, int n, qexhip_field *const *r, const qexhip_field *const *x,
+                         double ca, double cb) {
+  require(n >= 1 && r && x, "stagD2_multi: bad arguments");
+  const int pr = r[0]->prec;
+  require(pr == P64 || pr == P32, "stagD2_multi: FP64 or FP32 fields");
+  require(par >= 0 && par <= 2, "bad parity");
+  for (int j = 0; j < n; j++) {
+    require(r[j] && x[j], "null field");
+    check_same_lat(l, r[j]);
+    check_same_lat(l, x[j]);
+    require(r[j]->prec == pr && x[j]->prec == pr, "stagD2_multi: all fields must share one precision");
+  }
+  const char *links = links_store(l, pr, 18);
+  const long lkblk = lkblk_bytes(l, pr, 18);
+  for (int p = 0; p < 2; p++) {
+    if (par != 2 && par != p) continue;
+    for (int j0 = 0; j0 < n;) {
+      int gsz = QH_MAXRHS;
+      while (gsz > n - j0) gsz >>= 1;
+      MultiPtrs P;
+      memset(&P, 0, sizeof P);
+      for (int j = 0; j < gsz; j++) {
+        P.in[j] = x[j0 + j]->par(1 - p);
+        P.own[j] = x[j0 + j]->par(p);
+        P.out[j] = r[j0 + j]->par(p);
+      }
+      launch_hop_multi(pr, l->nl, gsz, l->lat->g, links, lkblk, p, ca, cb, P, 256, 0, g_stream);
+      j0 += gsz;
+    }
+    check_launch();
+  }
+}
+
+int qexhip_stagD2_multi(qexhip_links *l, int par, int n, qexhip_field *const *r, const qexhip_field *const *x,
+                        double a, double b) {
+  API_BEGIN
+  check_init();
+  dslash_multi(l, par, n, r, x, a, b);
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+int qexhip_time_hop_multi(qexhip_links *l, int prec, int nrhs, int nrep, double *ms) {
+  API_BEGIN
+  check_init();
+  require(l && ms && nrep > 0 && nrhs >= 1 && nrhs <= 64, "bad arguments");
+  require(prec == P64 || prec == P32, "multi-RHS hop: FP64 or FP32");
+  qexhip_lattice *lat = l->lat;
+  std::vector<qexhip_field *> x(nrhs), r(nrhs);
+  for (int j = 0; j < nrhs; j++) {
+    if (qexhip_field_create(lat, prec, &x[j]) || qexhip_field_create(lat, prec, &r[j]))
+      throw std::runtime_error(g_err);
+    launch_source_gauss(lat->g, 99 + j, 0, 2, lat->stage, g_stream);
+    import_field(x[j], 2, lat->stage, 1);
+  }
+  for (int i = 0; i < 3; i++) dslash_multi(l, 0, nrhs, r.data(), x.data(), 0.0, 0.0);
+  const double us = time_us([&] { dslash_multi(l, 0, nrhs, r.data(), x.data(), 0.0, 0.0); }, 0, nrep);
+  *ms = us * 1e-3;
+  for (int j = 0; j < nrhs; j++) {
+    qexhip_field_destroy(x[j]);
+    qexhip_field_destroy(r[j]);
+  }
+  API_END
+}
+
 int qexhip_stagD(qexhip_links *l, int par, qexhip_field *r, const qexhip_field *x, double m, double sc, double a) {
   API_BEGIN
   check_init();

@@ -180,6 +180,101 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
   }
 }
 
+// Multi-RHS stagD2 (QEX's stagDPN/stagDN, stagD.nim:132-199,451-456): on parity
+// par, r_j = a r_j + b x_j + (2D) x_j for NR fields, each link loaded once for
+// all of them.  FP64 (A = double2, P64 links) or FP32 (float2, P32 links).
+template <int PL, class A, int NL, int NR>
+__global__ void __launch_bounds__(256) k_hop_multi(Geom g, const char *links, long lkblk, int par, double ca,
+                                                  double cb, MultiPtrs P) {
+  using R = RT<A>;
+  const long nblk = (g.Vh + blockDim.x - 1) / blockDim.x;
+  const BlockIter it = block_iter(nblk, 1);
+#pragma unroll 1
+  for (long lb = it.cur; lb < it.end; lb += it.step) {
+    const long i = lb * blockDim.x + threadIdx.x;
+    if (i >= g.Vh) continue;
+    const Site s = decode(g, i, par);
+    A acc[NR][3];
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) acc[r][c] = mk<A>(0, 0);
+#pragma unroll 1
+    for (int j = 0; j < 2 * NL; j++) {
+      const int k = j >> 1, fwd = !(j & 1);
+      const int mu = (NL == 4) ? k : (k >> 1);
+      const int len = (NL == 8 && (k & 1)) ? 3 : 1;
+      const long n = nbr(g, s, mu, fwd ? len : -len);
+      A U[9];
+      ld_link<PL, 18, true, A>(links + (long)(2 * k + (fwd ? par : 1 - par)) * lkblk, g.stride, fwd ? i : n, U,
+                               (R)1.0);
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        A v[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] = ldc<PL, A>(P.in[r], c * g.stride + n);
+        if (fwd) {
+#pragma unroll
+          for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) cmac(acc[r][a], U[3 * a + c], v[c]);
+        } else {
+#pragma unroll
+          for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              acc[r][a].x -= U[3 * c + a].x * v[c].x + U[3 * c + a].y * v[c].y;
+              acc[r][a].y -= U[3 * c + a].x * v[c].y - U[3 * c + a].y * v[c].x;
+            }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const long o = c * g.stride + i;
+        A v = acc[r][c];
+        if (cb != 0.0) {
+          const A x = ldc<PL, A>(P.own[r], o);
+          v.x += (R)cb * x.x;
+          v.y += (R)cb * x.y;
+        }
+        if (ca != 0.0) {
+          const A y = ldc<PL, A>(P.out[r], o);
+          v.x += (R)ca * y.x;
+          v.y += (R)ca * y.y;
+        }
+        stc<PL, A>(P.out[r], o, v);
+      }
+  }
+}
+
+template <int PL, class A, int NL>
+static bool hop_multi_nr(int nr, const Geom &g, const char *links, long lkblk, int par, double ca, double cb,
+                         const MultiPtrs &P, int block, int cap, hipStream_t st) {
+  const dim3 grid(grid_blocks((g.Vh + block - 1) / block, 1, cap)), blk(block);
+  switch (nr) {
+    case 1: hipLaunchKernelGGL((k_hop_multi<PL, A, NL, 1>), grid, blk, 0, st, g, links, lkblk, par, ca, cb, P); return true;
+    case 2: hipLaunchKernelGGL((k_hop_multi<PL, A, NL, 2>), grid, blk, 0, st, g, links, lkblk, par, ca, cb, P); return true;
+    case 4: hipLaunchKernelGGL((k_hop_multi<PL, A, NL, 4>), grid, blk, 0, st, g, links, lkblk, par, ca, cb, P); return true;
+    case 8: hipLaunchKernelGGL((k_hop_multi<PL, A, NL, 8>), grid, blk, 0, st, g, links, lkblk, par, ca, cb, P); return true;
+  }
+  return false;
+}
+
+void launch_hop_multi(int prec, int nl, int nr, const Geom &g, const char *links, long lkblk, int par, double ca,
+                      double cb, const MultiPtrs &P, int block, int cap, hipStream_t st) {
+  bool ok = false;
+  if (prec == P64)
+    ok = (nl == 8) ? hop_multi_nr<P64, double2, 8>(nr, g, links, lkblk, par, ca, cb, P, block, cap, st)
+                   : hop_multi_nr<P64, double2, 4>(nr, g, links, lkblk, par, ca, cb, P, block, cap, st);
+  else if (prec == P32)
+    ok = (nl == 8) ? hop_multi_nr<P32, float2, 8>(nr, g, links, lkblk, par, ca, cb, P, block, cap, st)
+                   : hop_multi_nr<P32, float2, 4>(nr, g, links, lkblk, par, ca, cb, P, block, cap, st);
+  if (!ok) fprintf(stderr, "qexhip: no multi-RHS hop for prec=%d nl=%d nr=%d\n", prec, nl, nr);
+}
+
 // After the MODE-1 hop: alpha_j, the extrapolated |r_j|^2, beta_j and the
 // update coefficients of every shift (cgm.nim:163-199).
 template <int NR>

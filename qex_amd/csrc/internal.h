@@ -129,6 +129,12 @@ struct MsbPtrs {              // one group of nr right-hand sides
   double **x, **p;              // device arrays [nr*nshift]: shifted solutions and search vectors
 };
 
+struct MultiPtrs {     // one group of up to QH_MAXRHS fields of the multi-RHS stagD2
+  const void *in[QH_MAXRHS];   // other-parity inputs
+  const void *own[QH_MAXRHS];  // own-parity x
+  void *out[QH_MAXRHS];        // own-parity r
+};
+
 struct HopKey {       // precision/format of one hop instantiation
   int pl, rc, pi, po, pa;
 };
@@ -166,6 +172,8 @@ void launch_force(const Geom &g, const char *links64, long lkblk, const double *
 // batch.hip
 void launch_bcg_iter(const BatchItem *T, int n, long nblocks, int nl, int block, hipStream_t st);
 void launch_bcg_flush(const BatchItem *T, int n, long nblocks, int block, hipStream_t st);
+void launch_hop_multi(int prec, int nl, int nr, const Geom &g, const char *links, long lkblk, int par, double ca,
+                      double cb, const MultiPtrs &P, int block, int cap, hipStream_t st);
 void launch_msb_iter(const Geom &g, const char *links64, long lkblk, int nl, int par, int nr, int nshift,
                      double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, hipStream_t st);
 

@@ -216,6 +216,23 @@ def stagD(sd, r, g, x, m, sc=1.0, a=0.0):
     call("qexhip_stagD", s.h, par, r.h, x.h, float(m), float(sc), float(a))
 
 
+def stagD2multi(sd, rs, g, xs, a, b):
+    """r_j = a*r_j + b*x_j + (2D)x_j for every field pair on sd's subset, each link
+    loaded once for up to 8 fields (the multi-RHS form of stagD2)."""
+    s, par = _sd(sd)
+    if len(rs) != len(xs):
+        raise QexHipError("stagD2multi: one output per input")
+    call("qexhip_stagD2_multi", s.h, par, len(rs), _vec(rs), _vec(xs), float(a), float(b))
+
+
+def stagDN(sds, rs, g, xs, m, sc=1.0):
+    """QEX's stagDN (stagD.nim:451-456): r_i = m*x_i + (2D)x_i for several fields.
+    As in QEX, sc is accepted and not applied (the 0.5*sc scaling there is
+    commented out)."""
+    sd = sds[0] if isinstance(sds, (list, tuple)) else sds
+    stagD2multi(sd, rs, g, xs, 0.0, m)
+
+
 def stagD2ee(se, so, r, g, x, m2):
     """r_e = 4*m2*x_e - (2D)_eo (2D)_oe x_e (stagD.nim:514-516)."""
     call("qexhip_stagD2xx", se.owner.h, EVEN, r.h, x.h, float(m2))
@@ -610,6 +627,13 @@ def time_hop(s, prec=FP64, nrep=100, recon=18):
     """Average ms of one one-parity hop (HIP events on the library stream)."""
     v = ctypes.c_double()
     call("qexhip_time_hop", s.h, prec, recon, nrep, ctypes.byref(v))
+    return v.value
+
+
+def time_hop_multi(s, prec=FP64, nrhs=8, nrep=100):
+    """Average ms of one multi-RHS hop over nrhs fields (HIP events)."""
+    v = ctypes.c_double()
+    call("qexhip_time_hop_multi", s.h, prec, nrhs, nrep, ctypes.byref(v))
     return v.value
 
 
