@@ -430,6 +430,10 @@ def main():
                        "kernel": "k_hop<P64,P64,P64,P64,double2,4,0,0>",
                        "algorithmic_bytes_per_launch": HOP_REALS * 8 * vh,
                        "avg_launch_ms": round(ms, 5)}
+    # practical ceiling: a grid-stride non-temporal read kernel on the same device
+    sr = Q.stream_read_gbs(2048, 10)
+    out["roofline"]["stream_read_ceiling_GBs"] = round(sr, 1)
+    out["roofline"]["frac_of_stream_ceiling"] = round(value / sr, 4)
 
     # ---- hop table (16^4, 32^4, 48^4 x storage precisions) ----
     if not args.no_table:

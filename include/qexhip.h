@@ -207,6 +207,9 @@ int qexhip_time_hop(qexhip_links *l, int prec, int recon, int nrep, double *ms_p
 /* the same for the multi-RHS hop over nrhs fields (FP64 or FP32, 18-real links):
    ms for one application to all nrhs fields */
 int qexhip_time_hop_multi(qexhip_links *l, int prec, int nrhs, int nrep, double *ms);
+/* device read bandwidth (GB/s) of a grid-stride non-temporal 16-byte-load kernel over
+   mbytes MiB: the practical ceiling a streaming kernel reaches (no QEX counterpart) */
+int qexhip_stream_read(long mbytes, int nrep, double *gbs);
 /* whether the links are phased SU(3) matrices, so 12-real storage is exact
    (1) or not (0) */
 int qexhip_links_recon_ok(qexhip_links *l, int *ok);

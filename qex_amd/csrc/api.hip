@@ -1364,6 +1364,25 @@ int qexhip_stagD2_multi(qexhip_links *l, int par, int n, qexhip_field *const *r,
   API_END
 }
 
+int qexhip_stream_read(long mbytes, int nrep, double *gbs) {
+  API_BEGIN
+  check_init();
+  require(mbytes >= 16 && mbytes <= (1L << 16) && nrep > 0 && gbs, "bad arguments");
+  const long n16 = (mbytes << 20) / 16;
+  const int blocks = 2048;
+  char *in = dalloc<char>((size_t)n16 * 16);
+  double *out = dalloc<double>(sizeof(double) * blocks * 256);
+  launch_fill_pattern(in, (size_t)n16 * 16, g_stream);  // non-zero data, as real fields
+  double best = 1e300;
+  for (int rep = 0; rep < 3; rep++)
+    best = std::min(best, time_us([&] { launch_stream_read(in, n16, out, blocks, g_stream); }, 1, nrep));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  (void)hipFree(in);
+  (void)hipFree(out);
+  *gbs = (double)n16 * 16 / (best * 1e-6) / 1e9;
+  API_END
+}
+
 int qexhip_time_hop_multi(qexhip_links *l, int prec, int nrhs, int nrep, double *ms) {
   API_BEGIN
   check_init();

@@ -155,6 +155,8 @@ void launch_recon_check(const Geom &g, const double *canon, unsigned *maxdev_bit
 void launch_pack_vec(int prec, const Geom &g, const double *canon, void *out, hipStream_t st);
 void launch_unpack_vec(int prec, const Geom &g, const void *in, double *canon, hipStream_t st);
 void launch_fill(void *p, size_t bytes, hipStream_t st);
+void launch_stream_read(const void *in, long n16, double *out, int blocks, hipStream_t st);
+void launch_fill_pattern(void *p, size_t bytes, hipStream_t st);
 void launch_axpby64(const Geom &g, double a, const double *x, double b, double *y, hipStream_t st);
 void launch_norm2(int prec, const Geom &g, const void *x, Scal *S, int slot, RedWs red, hipStream_t st);
 void launch_redot(int prec, const Geom &g, const void *x, const void *y, Scal *S, int slot, RedWs red,

@@ -76,6 +76,34 @@ void launch_fill(void *p, size_t bytes, hipStream_t st) {
   hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, st, (uint4 *)p, n16);
 }
 
+// Streaming-read reference: grid-stride non-temporal 16-byte loads, one store
+// per thread (out: gridDim.x*blockDim.x doubles).  Measures the read bandwidth
+// a kernel can reach on this device, the practical ceiling of the hop.
+__global__ void __launch_bounds__(256) k_stream_read(const double2 *in, long n16, double *out) {
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  const d2v *p = (const d2v *)in;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  d2v acc = {0.0, 0.0};
+  for (long i = t; i < n16; i += (long)gridDim.x * blockDim.x) acc += __builtin_nontemporal_load(p + i);
+  out[t] = acc.x + acc.y;
+}
+
+// non-trivial bit patterns (a 64-bit multiplicative hash of the index)
+__global__ void k_fill_pattern(uint64_t *p, long n8) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += (long)gridDim.x * blockDim.x)
+    p[i] = ((uint64_t)i + 1) * 0x9E3779B97F4A7C15ull;
+}
+
+void launch_fill_pattern(void *p, size_t bytes, hipStream_t st) {
+  const long n8 = (long)(bytes / 8);
+  if (n8 == 0) return;
+  hipLaunchKernelGGL(k_fill_pattern, dim3(4096), dim3(256), 0, st, (uint64_t *)p, n8);
+}
+
+void launch_stream_read(const void *in, long n16, double *out, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_stream_read, dim3(blocks), dim3(256), 0, st, (const double2 *)in, n16, out);
+}
+
 __global__ void k_axpby64(Geom g, double a, const double2 *x, double b, double2 *y) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.Vh) return;

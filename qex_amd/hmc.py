@@ -70,7 +70,7 @@ def kinetic(P):
 
 class StaggeredHMC:
     def __init__(self, lo, U, beta, mass, r2req_force=1e-20, r2req_action=1e-24, sloppy=Q.SloppyType.SloppyNone,
-                 integrator="leapfrog", lam=OMELYAN_LAMBDA):
+                 integrator="leapfrog", lam=OMELYAN_LAMBDA, sloppy_links=None, recon=18):
         if integrator not in ("leapfrog", "2MN"):
             raise ValueError("integrator must be 'leapfrog' or '2MN'")
         self.integrator, self.lam = integrator, float(lam)
@@ -85,7 +85,7 @@ class StaggeredHMC:
         self.psi = lo.ColorVector()
         self.y = lo.ColorVector()
         self.r2f, self.r2a = r2req_force, r2req_action
-        self.sloppy = sloppy
+        self.sloppy, self.sloppy_links, self.recon = sloppy, sloppy_links, recon
         self.solver_its = 0
         self.Ff = torch.empty_like(self.U)
         self.Fg = torch.empty_like(self.U)
@@ -93,7 +93,7 @@ class StaggeredHMC:
     def _sp(self, r2req):
         sp = Q.SolverParams()
         sp.r2req, sp.verbosity, sp.maxits = r2req, 0, 100000
-        sp.sloppySolve = self.sloppy
+        sp.sloppySolve, sp.sloppyLinks, sp.recon = self.sloppy, self.sloppy_links, self.recon
         return sp
 
     def _links_changed(self):
@@ -367,7 +367,11 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--frsq", type=float, default=1e-20, help="force solve r2req")
     ap.add_argument("--arsq", type=float, default=1e-24, help="action solve r2req")
-    ap.add_argument("--sloppy", type=int, default=0, help="SloppyType of the force solves")
+    ap.add_argument("--sloppy", type=int, default=0,
+                    help="SloppyType of the solves (vectors): 0 FP64, 1 FP32, 2 FP16, 3 BF16")
+    ap.add_argument("--sloppy-links", type=int, default=-1,
+                    help="SloppyType of the sloppy links (-1: as the vectors; 4: 16-bit fixed point)")
+    ap.add_argument("--recon", type=int, default=18, help="reals per sloppy link (18 or 12)")
     ap.add_argument("--fd-links", type=int, default=0)
     ap.add_argument("--reversibility", action="store_true")
     ap.add_argument("--integrator", default="leapfrog", help="leapfrog or 2MN[,lambda]")
@@ -388,7 +392,9 @@ def main(argv=None):
     name, *rest = args.integrator.split(",")
     h = StaggeredHMC(lo, G.unit_gauge(lo), args.beta, args.mass, r2req_force=args.frsq,
                      r2req_action=args.arsq, sloppy=Q.SloppyType(args.sloppy), integrator=name,
-                     lam=float(rest[0]) if rest else OMELYAN_LAMBDA)
+                     lam=float(rest[0]) if rest else OMELYAN_LAMBDA,
+                     sloppy_links=None if args.sloppy_links < 0 else Q.SloppyType(args.sloppy_links),
+                     recon=args.recon)
     rng = np.random.default_rng(args.seed)
     for n in range(args.warmup + args.trajs):
         its0 = h.solver_its

@@ -637,6 +637,14 @@ def time_hop_multi(s, prec=FP64, nrhs=8, nrep=100):
     return v.value
 
 
+def stream_read_gbs(mbytes=2048, nrep=10):
+    """Read bandwidth (GB/s) of the library's grid-stride streaming-read kernel:
+    the practical ceiling for the bandwidth-bound kernels."""
+    v = ctypes.c_double()
+    call("qexhip_stream_read", int(mbytes), int(nrep), ctypes.byref(v))
+    return v.value
+
+
 def recon_ok(s):
     """True when the links are phased SU(3) matrices (12-real storage exact)."""
     v = ctypes.c_int()

@@ -198,3 +198,8 @@ def test_time_hop_multi_runs(gpu, nrhs):
     gauge.phase_bc(lo, U)
     s = gpu.Staggered(lo, U)
     assert gpu.time_hop_multi(s, 0, nrhs, 5) > 0.0
+
+
+def test_stream_read_probe(gpu):
+    gbs = gpu.stream_read_gbs(256, 5)
+    assert 500.0 < gbs < 8000.0
