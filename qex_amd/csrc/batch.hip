@@ -276,10 +276,13 @@ void launch_hop_multi(int prec, int nl, int nr, const Geom &g, const char *links
 }
 
 // After the MODE-1 hop: alpha_j, the extrapolated |r_j|^2, beta_j and the
-// update coefficients of every shift (cgm.nim:163-199).
+// update coefficients of every shift (cgm.nim:163-199).  One thread per
+// (right-hand side, shift): every thread reads the old per-source scalars,
+// then (after a barrier) shift 0's thread writes the new ones.
 template <int NR>
 __global__ void __launch_bounds__(1024) k_msb_fin1(RedWs red, int nb, MsbScal *T) {
   __shared__ double lds[16 * 3 * NR];
+  __shared__ double tot[3 * NR];
   double s[3 * NR];
 #pragma unroll
   for (int k = 0; k < 3 * NR; k++) s[k] = 0.0;
@@ -287,36 +290,45 @@ __global__ void __launch_bounds__(1024) k_msb_fin1(RedWs red, int nb, MsbScal *T
 #pragma unroll
     for (int k = 0; k < 3 * NR; k++) s[k] += red.partials[(long)k * red.maxblocks + j];
   block_sum<3 * NR>(s, lds);
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x == 0)
 #pragma unroll
-  for (int r = 0; r < NR; r++) {
-    MsbScal *t = T + r;
-    if (!t->continuing) continue;
-    const double pAp = s[3 * r], rAp = s[3 * r + 1], ApAp = s[3 * r + 2];
-    const double alpha = (pAp != 0.0) ? t->r2i / pAp : 0.0;
+    for (int k = 0; k < 3 * NR; k++) tot[k] = s[k];
+  __syncthreads();
+  const int r = threadIdx.x / QH_MAXSHIFT, k = threadIdx.x % QH_MAXSHIFT;
+  const bool mine = r < NR;
+  MsbScal *t = T + (mine ? r : 0);
+  const bool cont = mine && t->continuing && k < t->nshift;
+  double alpha = 0.0, beta = 0.0, aim1 = 0.0, bim1 = 0.0;
+  if (cont) {
+    const double pAp = tot[3 * r], rAp = tot[3 * r + 1], ApAp = tot[3 * r + 2];
+    alpha = (pAp != 0.0) ? t->r2i / pAp : 0.0;
     double r2n = t->r2i - 2.0 * alpha * rAp + alpha * alpha * ApAp;
     if (!(r2n > 0.0)) r2n = 0.0;
-    const double beta = (t->r2i != 0.0) ? r2n / t->r2i : 0.0;
+    beta = (t->r2i != 0.0) ? r2n / t->r2i : 0.0;
+    aim1 = t->alphaim1;
+    bim1 = t->betaim1;
+  }
+  __syncthreads();
+  if (!cont) return;
+  if (k == 0) {
     t->alpha = alpha;
     t->beta = beta;
     t->xc[0] = alpha;
     t->pc1[0] = 1.0;
     t->pc2[0] = beta;
-    for (int k = 1; k < t->nshift; k++) {
-      if (!t->active[k]) continue;
-      double zip1d = alpha * t->betaim1 * (t->zim1[k] - t->zi[k]);
-      zip1d += t->zim1[k] * t->alphaim1 * (1.0 + t->sg[k] * alpha);
-      const double zip1 = (zip1d != 0.0) ? t->zi[k] * t->zim1[k] * t->alphaim1 / zip1d : 0.0;
-      const double zr = (t->zi[k] != 0.0) ? zip1 / t->zi[k] : 0.0;
-      t->xc[k] = alpha * zr;
-      t->pc1[k] = zip1;
-      t->pc2[k] = beta * zr * zr;
-      t->zim1[k] = t->zi[k];
-      t->zi[k] = zip1;
-    }
     t->alphaim1 = alpha;
     t->betaim1 = beta;
     t->its += 1;
+  } else if (t->active[k]) {
+    double zip1d = alpha * bim1 * (t->zim1[k] - t->zi[k]);
+    zip1d += t->zim1[k] * aim1 * (1.0 + t->sg[k] * alpha);
+    const double zip1 = (zip1d != 0.0) ? t->zi[k] * t->zim1[k] * aim1 / zip1d : 0.0;
+    const double zr = (t->zi[k] != 0.0) ? zip1 / t->zi[k] : 0.0;
+    t->xc[k] = alpha * zr;
+    t->pc1[k] = zip1;
+    t->pc2[k] = beta * zr * zr;
+    t->zim1[k] = t->zi[k];
+    t->zi[k] = zip1;
   }
 }
 
@@ -367,10 +379,12 @@ __global__ void __launch_bounds__(256) k_msb_update(Geom g, int nshift, MsbPtrs 
   block_partials<NR>(rr, lds, red);
 }
 
-// exact |r_j|^2: next alpha, stopping test, shift freezing (cgm.nim:176-186)
+// exact |r_j|^2: next alpha, stopping test, shift freezing (cgm.nim:176-186);
+// one thread per (right-hand side, shift), reads before the barrier, writes after
 template <int NR>
 __global__ void __launch_bounds__(1024) k_msb_fin2(RedWs red, int nb, MsbScal *T) {
   __shared__ double lds[16 * NR];
+  __shared__ double tot[NR];
   double s[NR];
 #pragma unroll
   for (int k = 0; k < NR; k++) s[k] = 0.0;
@@ -378,20 +392,27 @@ __global__ void __launch_bounds__(1024) k_msb_fin2(RedWs red, int nb, MsbScal *T
 #pragma unroll
     for (int k = 0; k < NR; k++) s[k] += red.partials[(long)k * red.maxblocks + j];
   block_sum<NR>(s, lds);
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x == 0)
 #pragma unroll
-  for (int r = 0; r < NR; r++) {
-    MsbScal *t = T + r;
-    if (!t->continuing) continue;
-    const double r2 = s[r];
+    for (int k = 0; k < NR; k++) tot[k] = s[k];
+  __syncthreads();
+  const int r = threadIdx.x / QH_MAXSHIFT, k = threadIdx.x % QH_MAXSHIFT;
+  const bool mine = r < NR;
+  MsbScal *t = T + (mine ? r : 0);
+  const bool was = mine && t->continuing && k < t->nshift;
+  bool cont = false, freeze_k = false;
+  double r2 = 0.0;
+  if (was) {
+    r2 = tot[r];
+    cont = (t->its < t->maxits) && (r2 > t->r2stop);
+    freeze_k = !cont || (t->freeze && k >= 1 && t->zi[k] * t->zi[k] * r2 <= t->r2stop);
+  }
+  __syncthreads();
+  if (!was) return;
+  if (freeze_k) t->active[k] = 0;
+  if (k == 0) {
     t->r2i = r2;
-    t->continuing = (t->its < t->maxits) && (r2 > t->r2stop);
-    if (!t->continuing) {
-      for (int k = 0; k < t->nshift; k++) t->active[k] = 0;
-    } else if (t->freeze) {
-      for (int k = 1; k < t->nshift; k++)
-        if (t->active[k] && t->zi[k] * t->zi[k] * r2 <= t->r2stop) t->active[k] = 0;
-    }
+    t->continuing = cont;
   }
 }
 
