@@ -485,6 +485,7 @@ def main():
                     ("mixed_q16links_fp16vec", S.SloppyHalf, S.SloppyHalfFixed, 18),
                     ("mixed_q16links_fp16vec_r12", S.SloppyHalf, S.SloppyHalfFixed, 12),
                     ("mixed_q16links_fp32vec_r12", S.SloppySingle, S.SloppyHalfFixed, 12),
+
                     ("mixed_bf16", S.SloppyBF16, None, 18),
                     ("mixed_bf16links_fp32vec_r12", S.SloppySingle, S.SloppyBF16, 12)]
         if args.cg_only:

@@ -43,6 +43,9 @@ typedef struct qexhip_field qexhip_field;
    scale per link field, value = int16 * max|entry|/32767; used with FP16
    color vectors. */
 enum { QEXHIP_FP64 = 0, QEXHIP_FP32 = 1, QEXHIP_FP16 = 2, QEXHIP_BF16 = 3, QEXHIP_Q16 = 4 };
+/* solver_params.sloppy only: FP16 storage for every sloppy vector, the residual
+   and the solution increment included (QEXHIP_FP16 keeps those two in FP32) */
+enum { QEXHIP_FP16_ALL = 5 };
 enum { QEXHIP_EVEN = 0, QEXHIP_ODD = 1, QEXHIP_ALL = 2 };
 
 /* Solver parameters: inputs and outputs mirror QEX's SolverParams

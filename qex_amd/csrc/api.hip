@@ -341,9 +341,9 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
 }
 
 // fused CG iteration: scratch vectors and device scalars that never converge
-Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, const CgArgs &real) {
-  char kb[64];
-  snprintf(kb, sizeof kb, "cg nl=%d links=%d rc=%d vec=%d ", l->nl, pl, rc, ps);
+Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, int rx, const CgArgs &real) {
+  char kb[80];
+  snprintf(kb, sizeof kb, "cg nl=%d links=%d rc=%d vec=%d rx=%d ", l->nl, pl, rc, ps, rx);
   const std::string key = kb + geom_key(l->lat->g);
   if (g_tune.count(key) || g_block_forced || !g_autotune) return tune(key, nullptr);
   const size_t vb = vec64_bytes(l->lat);
@@ -367,7 +367,7 @@ Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, const CgArgs &real) {
       key,
       [&](const Tuned &c) {
         a.remap = c.remap;
-        launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, c.block, c.form);
+        launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, c.block, c.form, rx);
       },
       {2, 3});
   HIPCHK(hipStreamSynchronize(g_stream));
@@ -404,6 +404,9 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   Scal *S = w->S, *Sh = w->Sh;
   const int rc = (recon == 12) ? 12 : 18;
   const bool mixed = (sloppy != P64);
+  // QEXHIP_FP16_ALL: FP16 search vectors and FP16 residual / solution increment
+  const int rx = (sloppy == QEXHIP_FP16_ALL) ? P16 : P32;
+  if (sloppy == QEXHIP_FP16_ALL) sloppy = P16;
   const int ps = sloppy, pl = mixed ? ((slinks < 0) ? sloppy : slinks) : P64;
   memset(Sh, 0, sizeof(Scal));
   Sh->m2 = m * m;
@@ -434,7 +437,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   a.par = par;
   a.S = S;
   a.red = lat->red;
-  Tuned tu = tuned_cg(l, pl, rc, ps, a);
+  Tuned tu = tuned_cg(l, pl, rc, ps, rx, a);
   if (g_cg_form) tu.form = g_cg_form;
   a.remap = tu.remap;
   if (!mixed) {
@@ -453,22 +456,22 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
     launch_fill(w->xs, b32, g_stream);
     launch_fill(w->Aps, b32, g_stream);
     resid64(l, par, x, b, w->r64, S, 2, w->t64);  // epi_reliable: sigma, r2, done
-    launch_rs_from_true(g, w->rs, w->r64, S, g_stream);
+    launch_rs_from_true(g, w->rs, w->r64, S, rx, g_stream);
     // best FP64 iterate so far (the FP64-only p vector is free in mixed mode)
     HIPCHK(hipMemcpyAsync(w->p64, x, vec64_bytes(lat), hipMemcpyDeviceToDevice, g_stream));
     check_launch();
     pull(Sh, S);
   }
   while (!Sh->done) {
-    for (int k = 0; k < kChunk; k++) launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, tu.block, tu.form);
+    for (int k = 0; k < kChunk; k++) launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, tu.block, tu.form, rx);
     check_launch();
     pull(Sh, S);
     if (Sh->done || Sh->need_rel) {
-      launch_cg_flush(ps, a, g_stream);  // apply the pending x/r update; exact |r|^2
+      launch_cg_flush(ps, rx, a, g_stream);  // apply the pending x/r update; exact |r|^2
       if (mixed) {
-        launch_xfold(g, x, w->xs, S, g_stream);
+        launch_xfold(g, x, w->xs, S, rx, g_stream);
         resid64(l, par, x, b, w->r64, S, 2, w->t64);
-        launch_rs_from_true(g, w->rs, w->r64, S, g_stream);
+        launch_rs_from_true(g, w->rs, w->r64, S, rx, g_stream);
       }
       check_launch();
       pull(Sh, S);
@@ -477,7 +480,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
         // residual epi_reliable accepts again, restarting p = r
         HIPCHK(hipMemcpyAsync(x, w->p64, vec64_bytes(lat), hipMemcpyDeviceToDevice, g_stream));
         resid64(l, par, x, b, w->r64, S, 2, w->t64);
-        launch_rs_from_true(g, w->rs, w->r64, S, g_stream);
+        launch_rs_from_true(g, w->rs, w->r64, S, rx, g_stream);
         check_launch();
         pull(Sh, S);
       } else if (mixed && Sh->improved) {
@@ -637,6 +640,23 @@ void export_field(const qexhip_field *f, int par, double *canon, int on_device) 
 
 void check_same_lat(const qexhip_links *l, const qexhip_field *f) {
   require(l && f && l->lat == f->lat, "field and links belong to different lattices");
+}
+
+void check_solver_params(qexhip_links *l, const qexhip_solver_params *sp) {
+  require(sp != nullptr, "null solver params");
+  require((sp->sloppy >= 0 && sp->sloppy <= 3) || sp->sloppy == QEXHIP_FP16_ALL, "bad sloppy precision");
+  require(sp->sloppy_links >= -1 && sp->sloppy_links <= PQ16, "bad sloppy link precision");
+  require(sp->recon == 0 || sp->recon == 18 || sp->recon == 12, "recon must be 0, 18 or 12");
+  if (sp->sloppy != P64) {
+    const int sv = (sp->sloppy == QEXHIP_FP16_ALL) ? P16 : sp->sloppy;
+    const int sl = sp->sloppy_links < 0 ? sv : sp->sloppy_links;
+    require(sl != P64, "mixed solve needs sloppy links below FP64");
+    require(sl == sv || sv == P32 || (sl == PQ16 && sv == P16),
+            "sloppy link/spinor storage combination not supported");
+    if (sp->sloppy == QEXHIP_FP16_ALL)
+      require(sl == P16 || sl == PQ16, "FP16_ALL needs FP16 or Q16 sloppy links");
+  }
+  if (sp->recon == 12) require(recon_allowed(l), "12-real links need plain (nl=4) phased SU(3) links");
 }
 
 // ------------------------------------------------------------ batched CG
@@ -1441,16 +1461,7 @@ int qexhip_solve_xx(qexhip_links *l, int par, qexhip_field *r, const qexhip_fiel
   require(sp != nullptr, "null solver params");
   require(par == 0 || par == 1, "solve parity must be 0 (even) or 1 (odd)");
   require(r->prec == P64 && x->prec == P64, "solver fields must be FP64");
-  require(sp->sloppy >= 0 && sp->sloppy <= 3, "bad sloppy precision");
-  require(sp->sloppy_links >= -1 && sp->sloppy_links <= PQ16, "bad sloppy link precision");
-  require(sp->recon == 0 || sp->recon == 18 || sp->recon == 12, "recon must be 0, 18 or 12");
-  if (sp->sloppy != P64) {
-    const int sl = sp->sloppy_links < 0 ? sp->sloppy : sp->sloppy_links;
-    require(sl != P64, "mixed solve needs sloppy links below FP64");
-    require(sl == sp->sloppy || sp->sloppy == P32 || (sl == PQ16 && sp->sloppy == P16),
-            "sloppy link/spinor storage combination not supported");
-  }
-  if (sp->recon == 12) require(recon_allowed(l), "12-real links need plain (nl=4) phased SU(3) links");
+  check_solver_params(l, sp);
   solve_xx(l, par, (double *)r->par(par), (const double *)x->par(par), m, sp);
   API_END
 }
@@ -1505,7 +1516,7 @@ int qexhip_force_solve(qexhip_links *l, qexhip_field *psi, const qexhip_field *p
   check_same_lat(l, psi);
   check_same_lat(l, phi);
   require(psi->prec == P64 && phi->prec == P64, "force fields must be FP64");
-  require(sp != nullptr, "null solver params");
+  check_solver_params(l, sp);
   const Geom &g = l->lat->g;
   launch_fill(psi->data, 2 * psi->par_bytes, g_stream);
   // solveEE(varphi, phi) ; varphi.even *= 4 ; psi.odd = (2D)_oe varphi ; psi.even = varphi

@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(256) k_cg_flush(CgArgs P) {
   cg_flush_body<PS, PR, PX, A>(P, blockIdx.x, lds, blockIdx.x);
 }
 
-template <int PL, int RC, int PS, int NL>
+template <int PL, int RC, int PS, int PRX, int NL>
 static void cg_iter_t(const CgArgs &a, hipStream_t st, int block, int form) {
   dim3 grid((unsigned)((a.g.Vh + block - 1) / block)), blk(block);
   if constexpr (PL == P64) {
@@ -176,31 +176,35 @@ static void cg_iter_t(const CgArgs &a, hipStream_t st, int block, int form) {
     }
   } else {
     if (form == 3) {
-      hipLaunchKernelGGL((k_cg0<PS, P32, P32, float2>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg0<PS, PRX, PRX, float2>), grid, blk, 0, st, a);
       hipLaunchKernelGGL((k_cg1b<PL, RC, PS, float2, NL>), grid, blk, 0, st, a);
-      hipLaunchKernelGGL((k_cg2b<PL, RC, PS, P32, float2, NL>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg2b<PL, RC, PS, PRX, float2, NL>), grid, blk, 0, st, a);
     } else {
-      hipLaunchKernelGGL((k_cg1<PL, RC, PS, P32, P32, float2, NL>), grid, blk, 0, st, a);
-      hipLaunchKernelGGL((k_cg2<PL, RC, PS, P32, P32, float2, NL>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg1<PL, RC, PS, PRX, PRX, float2, NL>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg2<PL, RC, PS, PRX, PRX, float2, NL>), grid, blk, 0, st, a);
     }
   }
   FinArgs f{a.red, (int)grid.x, a.S, nullptr, 0, 0.0, 1};
   launch_finish(EPI_CG2, f, st);
 }
 
-template <int PL, int PS>
+template <int PL, int PS, int PRX = P32>
 static bool cg_iter_rc(int rc, int nl, const CgArgs &a, hipStream_t st, int block, int form) {
-  if (rc == 18 && nl == 4) { cg_iter_t<PL, 18, PS, 4>(a, st, block, form); return true; }
-  if (rc == 18 && nl == 8) { cg_iter_t<PL, 18, PS, 8>(a, st, block, form); return true; }
-  if (rc == 12 && nl == 4) { cg_iter_t<PL, 12, PS, 4>(a, st, block, form); return true; }
+  if (rc == 18 && nl == 4) { cg_iter_t<PL, 18, PS, PRX, 4>(a, st, block, form); return true; }
+  if (rc == 18 && nl == 8) { cg_iter_t<PL, 18, PS, PRX, 8>(a, st, block, form); return true; }
+  if (rc == 12 && nl == 4) { cg_iter_t<PL, 12, PS, PRX, 4>(a, st, block, form); return true; }
   return false;
 }
 
 // form 2: two stencil kernels (p formed on the fly); form 3: a streaming update
-// kernel plus two stencil kernels that gather one vector each
-void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form) {
+// kernel plus two stencil kernels that gather one vector each.  rx: storage of
+// the sloppy residual and solution increment (P32, or P16 with FP16 vectors).
+void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form, int rx) {
   bool ok = false;
-  if (pl == P64 && ps == P64) ok = cg_iter_rc<P64, P64>(rc, nl, a, st, block, form);
+  if (rx == P16 && ps == P16) {
+    if (pl == P16) ok = cg_iter_rc<P16, P16, P16>(rc, nl, a, st, block, form);
+    else if (pl == PQ16) ok = cg_iter_rc<PQ16, P16, P16>(rc, nl, a, st, block, form);
+  } else if (pl == P64 && ps == P64) ok = cg_iter_rc<P64, P64>(rc, nl, a, st, block, form);
   else if (pl == P32 && ps == P32) ok = cg_iter_rc<P32, P32>(rc, nl, a, st, block, form);
   else if (pl == P16 && ps == P16) ok = cg_iter_rc<P16, P16>(rc, nl, a, st, block, form);
   else if (pl == P16 && ps == P32) ok = cg_iter_rc<P16, P32>(rc, nl, a, st, block, form);
@@ -208,16 +212,21 @@ void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t
   else if (pl == PBF && ps == P32) ok = cg_iter_rc<PBF, P32>(rc, nl, a, st, block, form);
   else if (pl == PQ16 && ps == P16) ok = cg_iter_rc<PQ16, P16>(rc, nl, a, st, block, form);
   else if (pl == PQ16 && ps == P32) ok = cg_iter_rc<PQ16, P32>(rc, nl, a, st, block, form);
-  if (!ok) fprintf(stderr, "qexhip: no CG instantiation for pl=%d rc=%d ps=%d nl=%d\n", pl, rc, ps, nl);
+  if (!ok)
+    fprintf(stderr, "qexhip: no CG instantiation for pl=%d rc=%d ps=%d rx=%d nl=%d\n", pl, rc, ps, rx, nl);
 }
 
-void launch_cg_flush(int ps, const CgArgs &a, hipStream_t st) {
+void launch_cg_flush(int ps, int rx, const CgArgs &a, hipStream_t st) {
   dim3 grid((unsigned)((a.g.Vh + 255) / 256)), blk(256);
-  switch (ps) {
-    case P64: hipLaunchKernelGGL((k_cg_flush<P64, P64, P64, double2>), grid, blk, 0, st, a); break;
-    case P32: hipLaunchKernelGGL((k_cg_flush<P32, P32, P32, float2>), grid, blk, 0, st, a); break;
-    case P16: hipLaunchKernelGGL((k_cg_flush<P16, P32, P32, float2>), grid, blk, 0, st, a); break;
-    case PBF: hipLaunchKernelGGL((k_cg_flush<PBF, P32, P32, float2>), grid, blk, 0, st, a); break;
+  if (rx == P16 && ps == P16) {
+    hipLaunchKernelGGL((k_cg_flush<P16, P16, P16, float2>), grid, blk, 0, st, a);
+  } else {
+    switch (ps) {
+      case P64: hipLaunchKernelGGL((k_cg_flush<P64, P64, P64, double2>), grid, blk, 0, st, a); break;
+      case P32: hipLaunchKernelGGL((k_cg_flush<P32, P32, P32, float2>), grid, blk, 0, st, a); break;
+      case P16: hipLaunchKernelGGL((k_cg_flush<P16, P32, P32, float2>), grid, blk, 0, st, a); break;
+      case PBF: hipLaunchKernelGGL((k_cg_flush<PBF, P32, P32, float2>), grid, blk, 0, st, a); break;
+    }
   }
   FinArgs f{a.red, (int)grid.x, a.S, nullptr, 0, 0.0, 0};
   launch_finish(EPI_FLUSH, f, st);

@@ -144,8 +144,9 @@ void launch_finish(int epi, const FinArgs &f, hipStream_t st);
 
 // dslash.hip
 void launch_hop(const HopKey &k, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block, int cap);
-void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form);
-void launch_cg_flush(int ps, const CgArgs &a, hipStream_t st);
+void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form,
+                    int rx = P32);
+void launch_cg_flush(int ps, int rx, const CgArgs &a, hipStream_t st);
 void launch_pack_links(int prec, int rc, const Geom &g, int nl, const double *canon, char *out, const float *qs,
                        hipStream_t st);
 void launch_link_maxabs(const Geom &g, int nl, const double *canon, unsigned *maxbits, hipStream_t st);
@@ -161,8 +162,8 @@ void launch_axpby64(const Geom &g, double a, const double *x, double b, double *
 void launch_norm2(int prec, const Geom &g, const void *x, Scal *S, int slot, RedWs red, hipStream_t st);
 void launch_redot(int prec, const Geom &g, const void *x, const void *y, Scal *S, int slot, RedWs red,
                   hipStream_t st);
-void launch_rs_from_true(const Geom &g, void *rs, const double *rtrue, Scal *S, hipStream_t st);
-void launch_xfold(const Geom &g, double *x, void *xs, Scal *S, hipStream_t st);
+void launch_rs_from_true(const Geom &g, void *rs, const double *rtrue, Scal *S, int pr, hipStream_t st);
+void launch_xfold(const Geom &g, double *x, void *xs, Scal *S, int px, hipStream_t st);
 void launch_scal_init(Scal *S, hipStream_t st, int mode);
 void launch_cgm_xr(const Geom &g, int nshift, double *r, const double *Ap, double **x, double **p,
                    ShiftScal *T, Scal *S, RedWs red, hipStream_t st);

@@ -170,25 +170,29 @@ void launch_redot(int prec, const Geom &g, const void *x, const void *y, Scal *S
   finish_tmp(grid.x, S, slot, red, st);
 }
 
-// reliable update, part 1: x += xs/sigma ; xs = 0   (FP64 solution accumulation)
-__global__ void k_xfold(Geom g, double2 *x, float2 *xs, Scal *S) {
+// reliable update, part 1: x += xs/sigma ; xs = 0   (FP64 solution accumulation;
+// xs stored in PX = FP32 or FP16)
+template <int PX>
+__global__ void k_xfold(Geom g, double2 *x, void *xs, Scal *S) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.Vh) return;
   const double is = 1.0 / S->sigma;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const long o = c * g.stride + i;
-    float2 s = xs[o];
+    const float2 s = ldc<PX, float2>(xs, o);
     double2 v = x[o];
     v.x += is * (double)s.x;
     v.y += is * (double)s.y;
     x[o] = v;
-    xs[o] = make_float2(0.f, 0.f);
+    stc<PX, float2>(xs, o, make_float2(0.f, 0.f));
   }
 }
 
-// reliable update, last part: rs = sigma * r_true (sloppy copy of the true residual)
-__global__ void k_rs_from_true(Geom g, float2 *rs, const double2 *rt, Scal *S) {
+// reliable update, last part: rs = sigma * r_true (sloppy copy of the true residual,
+// stored in PR = FP32 or FP16)
+template <int PR>
+__global__ void k_rs_from_true(Geom g, void *rs, const double2 *rt, Scal *S) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.Vh) return;
   const double sg = S->sigma;
@@ -196,17 +200,24 @@ __global__ void k_rs_from_true(Geom g, float2 *rs, const double2 *rt, Scal *S) {
   for (int c = 0; c < 3; c++) {
     const long o = c * g.stride + i;
     double2 v = rt[o];
-    rs[o] = make_float2((float)(sg * v.x), (float)(sg * v.y));
+    stc<PR, float2>(rs, o, make_float2((float)(sg * v.x), (float)(sg * v.y)));
   }
 }
 
-void launch_rs_from_true(const Geom &g, void *rs, const double *rtrue, Scal *S, hipStream_t st) {
-  hipLaunchKernelGGL(k_rs_from_true, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, (float2 *)rs,
-                     (const double2 *)rtrue, S);
+void launch_rs_from_true(const Geom &g, void *rs, const double *rtrue, Scal *S, int pr, hipStream_t st) {
+  if (pr == P16)
+    hipLaunchKernelGGL(k_rs_from_true<P16>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, rs, (const double2 *)rtrue,
+                       S);
+  else
+    hipLaunchKernelGGL(k_rs_from_true<P32>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, rs, (const double2 *)rtrue,
+                       S);
 }
 
-void launch_xfold(const Geom &g, double *x, void *xs, Scal *S, hipStream_t st) {
-  hipLaunchKernelGGL(k_xfold, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, (double2 *)x, (float2 *)xs, S);
+void launch_xfold(const Geom &g, double *x, void *xs, Scal *S, int px, hipStream_t st) {
+  if (px == P16)
+    hipLaunchKernelGGL(k_xfold<P16>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, (double2 *)x, xs, S);
+  else
+    hipLaunchKernelGGL(k_xfold<P32>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, (double2 *)x, xs, S);
 }
 
 __global__ void k_scal_init(Scal *S, int mode) {

@@ -184,7 +184,7 @@ __device__ __forceinline__ void cg2_body(const CgArgs &P, long blk, double *lds,
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const long off = c * g.stride + i;
-      const A rn = mk<A>(rv[c].x - alpha * av[c].x, rv[c].y - alpha * av[c].y);
+      const A rn = rt_store<PR, A>(mk<A>(rv[c].x - alpha * av[c].x, rv[c].y - alpha * av[c].y));
       const A pn = rt_store<PS, A>(first ? rn : mk<A>(rn.x + beta * pv[c].x, rn.y + beta * pv[c].y));
       const A xn = mk<A>(xv[c].x + alpha * pv[c].x, xv[c].y + alpha * pv[c].y);
       const A apn = rt_store<PS, A>(mk<A>(m4 * pn.x - H[c].x, m4 * pn.y - H[c].y));

@@ -282,11 +282,12 @@ class SloppyType(enum.IntEnum):
     SloppyHalf = 2
     SloppyBF16 = 3
     SloppyHalfFixed = 4
+    SloppyHalfAll = 5  # FP16 for every sloppy vector, residual and solution increment included
 
 
 _SLOPPY_PREC = {SloppyType.SloppyNone: FP64, SloppyType.SloppySingle: FP32,
                 SloppyType.SloppyHalf: FP16, SloppyType.SloppyBF16: BF16,
-                SloppyType.SloppyHalfFixed: Q16}
+                SloppyType.SloppyHalfFixed: Q16, SloppyType.SloppyHalfAll: 5}
 
 
 class SolverParams:

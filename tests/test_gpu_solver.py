@@ -290,3 +290,25 @@ def test_cg_iteration_forms_agree(oracle, gpu, form, sloppy, slinks, recon):
     if sloppy == 0:
         assert abs(sp.iterations - its_o) <= 1
     assert rel(fx.get(sub=0)[: lo.nEven], want[: lo.nEven]) < 1e-8
+
+
+@pytest.mark.parametrize("slinks,recon,hisq", [(None, 18, False), (4, 12, False), (4, 18, True)])
+def test_fp16_all_storage_reaches_fp64_residual(oracle, gpu, slinks, recon, hisq):
+    """Sloppy residual and solution increment in FP16 too (SloppyHalfAll)."""
+    O = oracle
+    L = (8, 8, 8, 8)
+    m = 0.05
+    g = links(O, L, hisq)
+    b = O.source_gauss(L, 23, 0, 0)
+    want, _, _ = O.solveXX(L, g, 0, b, m, 1e-24)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    fx = lo.ColorVector()
+    sp = gpu.SolverParams()
+    sp.r2req, sp.verbosity, sp.sloppySolve, sp.sloppyLinks, sp.recon = 1e-20, 0, 5, slinks, recon
+    gpu.solveEE(s, fx, fb, m, sp)
+    assert sp.r2 <= 1e-20
+    r = b - O.stagD2xx(L, g, 0, fx.get(), m * m)
+    assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20
+    assert rel(fx.get(sub=0)[: lo.nEven], want[: lo.nEven]) < 1e-6
