@@ -170,6 +170,9 @@ int qexhip_autotune_report(char *buf, int len);
 /* CG iteration form: 2 = two stencil kernels forming p on the fly, 3 = a
    streaming update kernel plus two stencil kernels; 0 = chosen by the tuner */
 int qexhip_set_cg_form(int form);
+/* replay each chunk of CG iterations from a HIP graph (1, default; QEXHIP_CG_GRAPH=0
+   at init turns it off) or launch kernel by kernel (0) */
+int qexhip_set_cg_graph(int on);
 
 /* ---- fermion force (src/hmc/hmcAction.nim:669-792) ---- */
 /* stagForceSolve (hmcAction.nim:669-721): psi_e = (m^2 - D_eo D_oe)^-1 phi_e,

@@ -7,4 +7,5 @@ from .stag import (D, Ddag, Field, Layout, SloppyType, SolverBackend, SolverPara
                    Staggered, eoReconstruct, eoReduce, fermForce, newLayout, newSolverParams, newStag,
                    newStag3, solve, solveBatch, solveEE, solveMulti, solveMultiBatch, solveMultiMass, solveOO,
                    stagD, stagD2, stagD2ee, stagD2oo, stagForceSolve, time_hop, autotune_report, set_autotune,
-                   set_cg_form, stagD2multi, stagDN, time_hop_multi, stream_read_gbs)
+                   set_cg_form, set_cg_graph, stagD2multi, stagDN, time_hop_multi,
+                   stream_read_gbs)

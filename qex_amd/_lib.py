@@ -28,6 +28,7 @@ EXPORTS = [
     "qexhip_plaquette", "qexhip_wilson_force", "qexhip_random_momenta", "qexhip_link_update",
     "qexhip_solve_batch", "qexhip_solve_multishift_batch", "qexhip_set_autotune", "qexhip_autotune_report",
     "qexhip_set_cg_form", "qexhip_stagD2_multi", "qexhip_time_hop_multi", "qexhip_stream_read",
+    "qexhip_set_cg_graph",
 ]
 
 # qexhip_solve_batch / qexhip_solve_multishift_batch flags
@@ -96,7 +97,7 @@ def load(path=LIB_PATH):
         "qexhip_solve_batch": [I, vpp, I, vpp, vpp, dp, spp, I],
         "qexhip_solve_multishift_batch": [vp, I, I, vpp, vpp, D, dp, I, spp, I],
         "qexhip_set_autotune": [I], "qexhip_autotune_report": [ctypes.c_char_p, I],
-        "qexhip_set_cg_form": [I],
+        "qexhip_set_cg_form": [I], "qexhip_set_cg_graph": [I],
         "qexhip_stagD2_multi": [vp, I, I, vpp, vpp, D, D], "qexhip_time_hop_multi": [vp, I, I, I, dp],
         "qexhip_stream_read": [ctypes.c_long, I, dp],
     }

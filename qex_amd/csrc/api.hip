@@ -29,6 +29,7 @@ int g_block = 256;
 int g_remap = 1;
 bool g_block_forced = false;  // QEXHIP_BLOCK set: no autotuning
 bool g_autotune = true;
+bool g_cg_graph = true;       // QEXHIP_CG_GRAPH=0: launch CG chunks kernel by kernel
 
 #define HIPCHK(x)                                                                                  \
   do {                                                                                             \
@@ -462,8 +463,31 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
     check_launch();
     pull(Sh, S);
   }
-  while (!Sh->done) {
+  // The chunk of iterations is captured once per solve and replayed: one graph
+  // launch instead of ~4*kChunk kernel launches, so the host never limits the
+  // short sloppy iterations (the kernels read every scalar from S).
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  auto chunk = [&] {
     for (int k = 0; k < kChunk; k++) launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, tu.block, tu.form, rx);
+  };
+  if (g_cg_graph && !Sh->done) {
+    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    chunk();
+    HIPCHK(hipStreamEndCapture(g_stream, &graph));
+    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  struct GraphGuard {
+    hipGraph_t &g;
+    hipGraphExec_t &e;
+    ~GraphGuard() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } guard{graph, exec};
+  while (!Sh->done) {
+    if (exec) HIPCHK(hipGraphLaunch(exec, g_stream));
+    else chunk();
     check_launch();
     pull(Sh, S);
     if (Sh->done || Sh->need_rel) {
@@ -1099,6 +1123,7 @@ int qexhip_init(int device) {
     }
   }
   if (const char *t = getenv("QEXHIP_TUNE")) g_autotune = atoi(t) != 0;
+  if (const char *c = getenv("QEXHIP_CG_GRAPH")) g_cg_graph = atoi(c) != 0;
   if (const char *f = getenv("QEXHIP_CG_FORM")) {
     const int v = atoi(f);
     if (v == 2 || v == 3) g_cg_form = v;
@@ -1678,6 +1703,11 @@ int qexhip_solve_multishift_batch(qexhip_links *l, int par, int nrhs, qexhip_fie
 
 int qexhip_set_autotune(int on) {
   g_autotune = on != 0;
+  return 0;
+}
+
+int qexhip_set_cg_graph(int on) {
+  g_cg_graph = on != 0;
   return 0;
 }
 

@@ -434,6 +434,12 @@ def set_autotune(on=True):
     call("qexhip_set_autotune", int(bool(on)))
 
 
+def set_cg_graph(on=True):
+    """Replay each chunk of CG iterations from a HIP graph (default) or launch
+    kernel by kernel."""
+    call("qexhip_set_cg_graph", int(bool(on)))
+
+
 def set_cg_form(form=0):
     """Pin the CG iteration form (2 or 3 kernels per iteration); 0 lets the tuner choose."""
     call("qexhip_set_cg_form", int(form))

@@ -312,3 +312,30 @@ def test_fp16_all_storage_reaches_fp64_residual(oracle, gpu, slinks, recon, hisq
     r = b - O.stagD2xx(L, g, 0, fx.get(), m * m)
     assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20
     assert rel(fx.get(sub=0)[: lo.nEven], want[: lo.nEven]) < 1e-6
+
+
+@pytest.mark.parametrize("sloppy,slinks,recon", [(0, None, 18), (2, 4, 12)])
+def test_cg_graph_replay_matches_eager(oracle, gpu, sloppy, slinks, recon):
+    """The CG chunks replayed from a HIP graph (with reliable updates and
+    synchronisations between replays) give bit-identical iterates to the
+    kernel-by-kernel launches."""
+    O = oracle
+    L = (8, 8, 8, 8)
+    g = links(O, L, False)
+    b = O.source_gauss(L, 29, 0, 0)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    out = []
+    for graph in (True, False):
+        fx = lo.ColorVector()
+        sp = gpu.SolverParams()
+        sp.r2req, sp.verbosity, sp.sloppySolve, sp.sloppyLinks, sp.recon = 1e-20, 0, sloppy, slinks, recon
+        gpu.set_cg_graph(graph)
+        try:
+            gpu.solveEE(s, fx, fb, 0.03, sp)
+        finally:
+            gpu.set_cg_graph(True)
+        out.append((fx.get(), sp.iterations, sp.r2))
+    assert out[0][1] == out[1][1] and out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0])
