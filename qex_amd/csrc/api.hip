@@ -258,6 +258,23 @@ double time_us(const std::function<void()> &f, int warm, int reps) {
 
 int g_cg_form = 0;  // QEXHIP_CG_FORM=2/3 pins the CG iteration form; 0: tuned
 
+// Capture the launches of f on the library stream into an instantiated graph.
+// A failure inside f ends the capture before rethrowing, so the stream is
+// never left capturing.
+void capture(const std::function<void()> &f, hipGraph_t *graph, hipGraphExec_t *exec) {
+  HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+  try {
+    f();
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(g_stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HIPCHK(hipStreamEndCapture(g_stream, graph));
+  HIPCHK(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+}
+
 // f(t) runs the kernel sequence once with shape t; the candidates are every
 // block size x block order x listed form x listed workgroup cap
 Tuned tune(const std::string &key, const std::function<void(const Tuned &)> &f,
@@ -472,10 +489,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
     for (int k = 0; k < kChunk; k++) launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, tu.block, tu.form, rx);
   };
   if (g_cg_graph && !Sh->done) {
-    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
-    chunk();
-    HIPCHK(hipStreamEndCapture(g_stream, &graph));
-    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    capture(chunk, &graph, &exec);
   }
   struct GraphGuard {
     hipGraph_t &g;
@@ -696,7 +710,7 @@ struct BatchWs {
   double *partials = nullptr;
   long maxblocks = 0, nblocks = 0;
   BatchItem *T = nullptr, *Tx = nullptr;  // device tables: solve, tuning scratch
-  int block = 0;
+  int block = 0, form = 2;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~BatchWs() {
@@ -783,18 +797,19 @@ BatchWs *get_batch(int n, qexhip_links *const *ls, int par) {
 }
 
 void batch_chunk(BatchWs *w, const BatchItem *T) {
-  for (int k = 0; k < kChunk; k++) launch_bcg_iter(T, w->n, w->nblocks, w->nl, w->block, g_stream);
+  for (int k = 0; k < kChunk; k++) launch_bcg_iter(T, w->n, w->nblocks, w->nl, w->block, g_stream, w->form);
   launch_bcg_flush(T, w->n, w->nblocks, w->block, g_stream);
 }
 
 void batch_shape(BatchWs *w, bool autotune) {
-  int block = 256;
+  int block = 256, form = g_cg_form ? g_cg_form : 2;
   if (autotune && !g_block_forced) {
     std::string key = "batch-cg nl=" + std::to_string(w->nl);
     for (auto *l : w->links) key += " " + geom_key(l->lat->g);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
       block = it->second.block;
+      form = it->second.form;
     } else {
       // scratch scalars that never converge
       std::vector<Scal> h(w->n);
@@ -802,19 +817,24 @@ void batch_shape(BatchWs *w, bool autotune) {
       for (auto &q : h) { q.r2stop = -1.0; q.maxits = INT_MAX; q.sigma = q.pscale = 1.0; q.flushed = 1; }
       HIPCHK(hipMemcpyAsync(w->Sx, h.data(), sizeof(Scal) * w->n, hipMemcpyHostToDevice, g_stream));
       Tuned best{256, 0, 1e300};
-      for (int b : {64, 128, 256}) {
-        const long nb = batch_table(w, b, w->Sx, w->Tx);
-        const double us = time_us([&] { launch_bcg_iter(w->Tx, w->n, nb, w->nl, b, g_stream); }, 2, 8);
-        if (us < best.us) best = Tuned{b, 0, us};
+      for (int f : {2, 3}) {
+        if (g_cg_form && f != g_cg_form) continue;
+        for (int b : {64, 128, 256}) {
+          const long nb = batch_table(w, b, w->Sx, w->Tx);
+          const double us = time_us([&] { launch_bcg_iter(w->Tx, w->n, nb, w->nl, b, g_stream, f); }, 2, 8);
+          if (us < best.us) best = Tuned{b, 0, us, f};
+        }
       }
       g_tune[key] = best;
       block = best.block;
+      form = best.form;
     }
   } else if (g_block_forced) {
     block = g_block;
   }
-  if (block != w->block || !w->exec) {
+  if (block != w->block || form != w->form || !w->exec) {
     w->block = block;
+    w->form = form;
     w->nblocks = batch_table(w, block, w->S, w->T);
     if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
     if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
@@ -828,10 +848,7 @@ void solve_batch(int n, qexhip_links *const *ls, int par, qexhip_field *const *x
   batch_shape(w, (flags & QEXHIP_BATCH_TUNE) != 0);
   const bool graph = (flags & QEXHIP_BATCH_GRAPH) != 0;
   if (graph && !w->exec) {
-    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
-    batch_chunk(w, w->T);
-    HIPCHK(hipStreamEndCapture(g_stream, &w->graph));
-    HIPCHK(hipGraphInstantiate(&w->exec, w->graph, nullptr, nullptr, 0));
+    capture([&] { batch_chunk(w, w->T); }, &w->graph, &w->exec);
   }
   // x = 0, r = b, Ap = 0 ; scalars as cg_solve with x = 0 (cg.nim:138-153)
   for (int k = 0; k < n; k++) {
@@ -1066,10 +1083,7 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
   }
   HIPCHK(hipMemcpyAsync(w->T, w->Th, sizeof(MsbScal) * nrhs, hipMemcpyHostToDevice, g_stream));
   if (graph && !w->exec) {
-    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
-    for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block);
-    HIPCHK(hipStreamEndCapture(g_stream, &w->graph));
-    HIPCHK(hipGraphInstantiate(&w->exec, w->graph, nullptr, nullptr, 0));
+    capture([&] { for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block); }, &w->graph, &w->exec);
   }
   for (;;) {
     bool any = false;
@@ -1102,6 +1116,17 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
 }  // namespace
 
 // ================================================================ C ABI
+extern "C" int qexhip_field_destroy(qexhip_field *f);
+
+namespace {
+struct FieldGuard {  // destroys a field created inside an entry point on every exit
+  qexhip_field *&f;
+  ~FieldGuard() {
+    if (f) qexhip_field_destroy(f);
+  }
+};
+}  // namespace
+
 extern "C" {
 
 const char *qexhip_last_error(void) { return g_err.c_str(); }
@@ -1434,7 +1459,14 @@ int qexhip_time_hop_multi(qexhip_links *l, int prec, int nrhs, int nrep, double 
   require(l && ms && nrep > 0 && nrhs >= 1 && nrhs <= 64, "bad arguments");
   require(prec == P64 || prec == P32, "multi-RHS hop: FP64 or FP32");
   qexhip_lattice *lat = l->lat;
-  std::vector<qexhip_field *> x(nrhs), r(nrhs);
+  std::vector<qexhip_field *> x(nrhs, nullptr), r(nrhs, nullptr);
+  struct Guard {
+    std::vector<qexhip_field *> &a, &b;
+    ~Guard() {
+      for (auto *f : a) qexhip_field_destroy(f);
+      for (auto *f : b) qexhip_field_destroy(f);
+    }
+  } guard{x, r};
   for (int j = 0; j < nrhs; j++) {
     if (qexhip_field_create(lat, prec, &x[j]) || qexhip_field_create(lat, prec, &r[j]))
       throw std::runtime_error(g_err);
@@ -1444,10 +1476,6 @@ int qexhip_time_hop_multi(qexhip_links *l, int prec, int nrhs, int nrep, double 
   for (int i = 0; i < 3; i++) dslash_multi(l, 0, nrhs, r.data(), x.data(), 0.0, 0.0);
   const double us = time_us([&] { dslash_multi(l, 0, nrhs, r.data(), x.data(), 0.0, 0.0); }, 0, nrep);
   *ms = us * 1e-3;
-  for (int j = 0; j < nrhs; j++) {
-    qexhip_field_destroy(x[j]);
-    qexhip_field_destroy(r[j]);
-  }
   API_END
 }
 
@@ -1496,15 +1524,12 @@ static int solve_host(qexhip_links *l, int par, double *r, const double *t, doub
   check_init();
   require(l && r && t && sp, "null argument");
   qexhip_field *fr = nullptr, *fx = nullptr;
+  FieldGuard gr{fr}, gx{fx};
   if (qexhip_field_create(l->lat, P64, &fr) || qexhip_field_create(l->lat, P64, &fx))
     throw std::runtime_error(g_err);
   import_field(fx, par, t, 0);
-  int rc = qexhip_solve_xx(l, par, fr, fx, m, sp);
-  if (rc == 0) export_field(fr, par, r, 0);
-  std::string e = g_err;
-  qexhip_field_destroy(fr);
-  qexhip_field_destroy(fx);
-  if (rc) throw std::runtime_error(e);
+  if (qexhip_solve_xx(l, par, fr, fx, m, sp)) throw std::runtime_error(g_err);
+  export_field(fr, par, r, 0);
   API_END
 }
 
@@ -1739,6 +1764,7 @@ int qexhip_time_hop(qexhip_links *l, int prec, int recon, int nrep, double *ms) 
   require(prec >= 0 && prec <= PQ16, "bad precision");
   qexhip_lattice *lat = l->lat;
   qexhip_field *x = nullptr, *r = nullptr;
+  FieldGuard gx{x}, gr{r};
   const int pv = (prec == PQ16) ? P16 : prec;
   if (qexhip_field_create(lat, pv, &x) || qexhip_field_create(lat, pv, &r)) throw std::runtime_error(g_err);
   launch_source_gauss(lat->g, 99, 0, 2, lat->stage, g_stream);
@@ -1759,8 +1785,6 @@ int qexhip_time_hop(qexhip_links *l, int prec, int recon, int nrep, double *ms) 
   *ms = (double)t / nrep;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  qexhip_field_destroy(x);
-  qexhip_field_destroy(r);
   API_END
 }
 

@@ -46,6 +46,28 @@ __global__ void __launch_bounds__(256) k_bcg2(const BatchItem *T, int n) {
   cg2_body<P64, 18, P64, P64, P64, double2, NL>(it.a, b - it.blk0, lds, b);
 }
 
+// three-kernel form (stencil.h): streaming update, then one-vector gathers
+__global__ void __launch_bounds__(256) k_bcg0(const BatchItem *T, int n) {
+  const long b = blockIdx.x;
+  const BatchItem &it = T[batch_item(T, n, b)];
+  cg0_body<P64, P64, P64, double2>(it.a, b - it.blk0);
+}
+
+template <int NL>
+__global__ void __launch_bounds__(256) k_bcg1b(const BatchItem *T, int n) {
+  const long b = blockIdx.x;
+  const BatchItem &it = T[batch_item(T, n, b)];
+  cg1b_body<P64, 18, P64, double2, NL>(it.a, b - it.blk0);
+}
+
+template <int NL>
+__global__ void __launch_bounds__(256) k_bcg2b(const BatchItem *T, int n) {
+  __shared__ double lds[16 * 4];
+  const long b = blockIdx.x;
+  const BatchItem &it = T[batch_item(T, n, b)];
+  cg2b_body<P64, 18, P64, P64, double2, NL>(it.a, b - it.blk0, lds, b);
+}
+
 __global__ void __launch_bounds__(1024) k_bfin_cg2(const BatchItem *T) {
   __shared__ double lds[16 * 4];
   finish_body<4, EPI_CG2>(T[blockIdx.x].fin, lds);
@@ -72,9 +94,18 @@ __global__ void __launch_bounds__(1024) k_bfin_flush(const BatchItem *T) {
   if (threadIdx.x == 0) S->flushed = 1;
 }
 
-void launch_bcg_iter(const BatchItem *T, int n, long nblocks, int nl, int block, hipStream_t st) {
+void launch_bcg_iter(const BatchItem *T, int n, long nblocks, int nl, int block, hipStream_t st, int form) {
   const dim3 grid((unsigned)nblocks), blk(block);
-  if (nl == 8) {
+  if (form == 3) {
+    hipLaunchKernelGGL(k_bcg0, grid, blk, 0, st, T, n);
+    if (nl == 8) {
+      hipLaunchKernelGGL(k_bcg1b<8>, grid, blk, 0, st, T, n);
+      hipLaunchKernelGGL(k_bcg2b<8>, grid, blk, 0, st, T, n);
+    } else {
+      hipLaunchKernelGGL(k_bcg1b<4>, grid, blk, 0, st, T, n);
+      hipLaunchKernelGGL(k_bcg2b<4>, grid, blk, 0, st, T, n);
+    }
+  } else if (nl == 8) {
     hipLaunchKernelGGL(k_bcg1<8>, grid, blk, 0, st, T, n);
     hipLaunchKernelGGL(k_bcg2<8>, grid, blk, 0, st, T, n);
   } else {

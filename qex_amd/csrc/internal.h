@@ -173,7 +173,7 @@ void launch_force(const Geom &g, const char *links64, long lkblk, const double *
                   double *F, int outer_only, hipStream_t st);
 
 // batch.hip
-void launch_bcg_iter(const BatchItem *T, int n, long nblocks, int nl, int block, hipStream_t st);
+void launch_bcg_iter(const BatchItem *T, int n, long nblocks, int nl, int block, hipStream_t st, int form);
 void launch_bcg_flush(const BatchItem *T, int n, long nblocks, int block, hipStream_t st);
 void launch_hop_multi(int prec, int nl, int nr, const Geom &g, const char *links, long lkblk, int par, double ca,
                       double cb, const MultiPtrs &P, int block, int cap, hipStream_t st);
