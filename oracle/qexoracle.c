@@ -356,18 +356,25 @@ void qo_stagD2xx(const int L[4], int nl, const double *g, int par,
 }
 
 /* ------------------------------------------------------------ reductions */
-double qo_norm2(long n, const double *x) {
+/* Deterministic: fixed chunks summed in parallel, then the chunk sums in
+   order, so results do not depend on the thread count or on the order in
+   which threads finish (OpenMP's reduction clause fixes neither). */
+#define QO_NCHUNK 64
+static double chunked_dot(long m, const double *x, const double *y) {
+  double part[QO_NCHUNK];
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < QO_NCHUNK; k++) {
+    const long lo = m * k / QO_NCHUNK, hi = m * (k + 1) / QO_NCHUNK;
+    double s = 0;
+    for (long i = lo; i < hi; i++) s += x[i] * y[i];
+    part[k] = s;
+  }
   double s = 0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-  for (long i = 0; i < 6 * n; i++) s += x[i] * x[i];
+  for (int k = 0; k < QO_NCHUNK; k++) s += part[k];
   return s;
 }
-double qo_redot(long n, const double *x, const double *y) {
-  double s = 0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-  for (long i = 0; i < 6 * n; i++) s += x[i] * y[i];
-  return s;
-}
+double qo_norm2(long n, const double *x) { return chunked_dot(6 * n, x, x); }
+double qo_redot(long n, const double *x, const double *y) { return chunked_dot(6 * n, x, y); }
 
 /* ---------------------------------------------------------------- solvers */
 /* solvers/cg.nim:55-272 with precon = cpNone (z aliases r, q aliases p),
@@ -594,19 +601,25 @@ static void m3dag(const cplx *a, cplx *r) {
 /* gauge/gaugeAction.nim:397-427 (plaquette term) */
 double qo_plaquette(const int L[4], const double *U) {
   long V = qo_volume(L);
-  double s = 0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-  for (long idx = 0; idx < V; idx++) {
-    int c[4];
-    qo_coords(L, idx, c);
-    for (int mu = 0; mu < 4; mu++)
-      for (int nu = mu + 1; nu < 4; nu++) {
-        cplx a[9], b[9];
-        m3mul(lnk(U, V, mu, idx), lnk(U, V, nu, nbr(L, c, mu, 1)), a);
-        m3mul(lnk(U, V, nu, idx), lnk(U, V, mu, nbr(L, c, nu, 1)), b);
-        for (int e = 0; e < 9; e++) s += creal(a[e] * conj(b[e]));
-      }
+  double part[QO_NCHUNK];  /* deterministic chunked sum, as chunked_dot */
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < QO_NCHUNK; k++) {
+    double s = 0;
+    for (long idx = V * k / QO_NCHUNK; idx < V * (k + 1) / QO_NCHUNK; idx++) {
+      int c[4];
+      qo_coords(L, idx, c);
+      for (int mu = 0; mu < 4; mu++)
+        for (int nu = mu + 1; nu < 4; nu++) {
+          cplx a[9], b[9];
+          m3mul(lnk(U, V, mu, idx), lnk(U, V, nu, nbr(L, c, mu, 1)), a);
+          m3mul(lnk(U, V, nu, idx), lnk(U, V, mu, nbr(L, c, nu, 1)), b);
+          for (int e = 0; e < 9; e++) s += creal(a[e] * conj(b[e]));
+        }
+    }
+    part[k] = s;
   }
+  double s = 0;
+  for (int k = 0; k < QO_NCHUNK; k++) s += part[k];
   return s;
 }
 

@@ -39,6 +39,20 @@ def main():
     out["force_phi"] = phi
     out["force_psi"] = psi
     out["force_F"] = O.ferm_force(L, U, psi)
+    # multi-shift CG on the plain operator
+    sig = np.array([0.0, 0.05, 0.5])
+    bm = O.source_gauss(L, 8, 0, 0)
+    xs, its, _ = O.cgm(L, U, 0, bm, 0.1, sig, 1e-22)
+    out["cgm_sigmas"], out["cgm_b"], out["cgm_x"], out["cgm_its"] = sig, bm, xs, np.array(its)
+    # gauge-field pieces of the HMC, on the unphased links
+    Uu = O.gauge_random(L, seed)
+    out["plaquette"] = np.array(O.plaquette(L, Uu))
+    out["wilson_force_b5.6"] = O.wilson_force(L, Uu, 5.6)
+    P = O.random_momenta(L, seed + 2)
+    out["momenta"] = P
+    W = Uu.copy()
+    O.link_update(L, P, 0.1, W)
+    out["link_update_0.1"] = W
     np.savez_compressed(os.path.join(HERE, "golden_4x4x4x4.npz"), **out)
     # U itself is regenerated from the seed by the generator under test
     print("wrote golden_4x4x4x4.npz")

@@ -33,6 +33,15 @@ def test_oracle_reproduces_golden(oracle, gold):
         sol, its, _ = oracle.solveXX(L, g, 0, gold[f"{name}_cg_b"], 0.2, 1e-22)
         assert rel(sol, gold[f"{name}_cg_x"]) < 1e-12 and its == int(gold[f"{name}_cg_its"])
     assert rel(oracle.ferm_force(L, U, gold["force_psi"]), gold["force_F"]) < 1e-14
+    xs, its, _ = oracle.cgm(L, U, 0, gold["cgm_b"], 0.1, gold["cgm_sigmas"], 1e-22)
+    assert rel(xs, gold["cgm_x"]) < 1e-12 and its == int(gold["cgm_its"])
+    Uu = oracle.gauge_random(L, seed)
+    assert abs(oracle.plaquette(L, Uu) - float(gold["plaquette"])) < 1e-12 * abs(float(gold["plaquette"]))
+    assert rel(oracle.wilson_force(L, Uu, 5.6), gold["wilson_force_b5.6"]) < 1e-14
+    P = oracle.random_momenta(L, seed + 2)
+    assert np.array_equal(P, gold["momenta"])
+    oracle.link_update(L, P, 0.1, Uu)
+    assert rel(Uu, gold["link_update_0.1"]) < 1e-14
 
 
 @pytest.mark.gpu
@@ -68,3 +77,28 @@ def test_gpu_reproduces_golden(gpu, gold):
     F = gpu.fermForce(s, psi, device="cuda")
     torch.cuda.synchronize()
     assert rel(F.cpu().numpy(), gold["force_F"]) < 1e-12
+    # multi-shift, single and batched
+    sig = [float(v) for v in gold["cgm_sigmas"]]
+    bm = lo.ColorVector().set(gold["cgm_b"])
+    ys = [lo.ColorVector() for _ in sig]
+    sp = gpu.SolverParams()
+    sp.r2req, sp.verbosity = 1e-22, 0
+    gpu.solveMulti(s, ys, bm, 0.1, sig, sp)
+    assert abs(sp.iterations - int(gold["cgm_its"])) <= 1
+    for k in range(len(sig)):
+        assert rel(ys[k].get()[:ne], gold["cgm_x"][k][:ne]) < 1e-10
+    zs = [[lo.ColorVector() for _ in sig]]
+    sps = [gpu.SolverParams()]
+    sps[0].r2req, sps[0].verbosity = 1e-22, 0
+    gpu.solveMultiBatch(s, zs, [bm], 0.1, sig, sps, freeze=False)
+    for k in range(len(sig)):
+        assert rel(zs[0][k].get()[:ne], gold["cgm_x"][k][:ne]) < 1e-10
+    # gauge-field pieces
+    from qex_amd import hmc
+    Uu = gauge.random_gauge(lo, seed)
+    assert abs(hmc.plaquette(lo, Uu) - float(gold["plaquette"])) < 1e-12 * abs(float(gold["plaquette"]))
+    assert rel(hmc.wilson_force(lo, Uu, 5.6).cpu().numpy(), gold["wilson_force_b5.6"]) < 1e-12
+    P = hmc.random_momenta(lo, seed + 2)
+    assert np.abs(P.cpu().numpy() - gold["momenta"]).max() < 1e-15
+    hmc.link_update(lo, P, 0.1, Uu)
+    assert rel(Uu.cpu().numpy(), gold["link_update_0.1"]) < 1e-13
