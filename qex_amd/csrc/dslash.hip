@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(256) k_hop(HopArgs P) {
       }
     }
 #pragma unroll
-    for (int c = 0; c < 3; c++) stc<PO, A>(P.out, c * g.stride + i, res[c]);
+    for (int c = 0; c < 3; c++) stc_out<PO, A>(P.out, c * g.stride + i, res[c]);
     if (RED == 1) {
 #pragma unroll
       for (int c = 0; c < 3; c++) red += (double)av[c].x * (double)res[c].x + (double)av[c].y * (double)res[c].y;

@@ -247,7 +247,7 @@ __device__ __forceinline__ void cg1b_body(const CgArgs &P, long blk) {
   A res[3];
   stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, q, i, s, ld, res);
 #pragma unroll
-  for (int c = 0; c < 3; c++) stc<PS, A>(P.t, c * g.stride + i, res[c]);
+  for (int c = 0; c < 3; c++) stc_out<PS, A>(P.t, c * g.stride + i, res[c]);
 }
 
 template <int PL, int RC, int PS, int PR, class A, int NL>
@@ -277,7 +277,7 @@ __device__ __forceinline__ void cg2b_body(const CgArgs &P, long blk, double *lds
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const A apn = rt_store<PS, A>(mk<A>(m4 * pv[c].x - H[c].x, m4 * pv[c].y - H[c].y));
-      stc<PS, A>(P.Ap, c * g.stride + i, apn);
+      stc_out<PS, A>(P.Ap, c * g.stride + i, apn);
       red[0] += (double)pv[c].x * (double)apn.x + (double)pv[c].y * (double)apn.y;
       red[1] += (double)rv[c].x * (double)apn.x + (double)rv[c].y * (double)apn.y;
       red[2] += (double)apn.x * (double)apn.x + (double)apn.y * (double)apn.y;
