@@ -13,9 +13,12 @@
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 
-// software-pipeline depth of the stencil, in links (see dslash.hip)
+// Stencil loop form (stencil.h): >= 0 software-pipeline depth in links;
+// -1 / -2 rolled loop over hops / links with separate forward and backward
+// accumulators; -3 rolled loop over links with one accumulator (fewest
+// registers: 100 VGPRs for the FP64 hop; fastest measured).
 #ifndef QH_PIPE
-#define QH_PIPE -2
+#define QH_PIPE -3
 #endif
 
 namespace qh {

@@ -17,8 +17,11 @@ namespace qh {
 
 
 // ------------------------------------------------------------ generic hop
+#ifndef QH_HOP_WAVES
+#define QH_HOP_WAVES 1  // minimum waves per SIMD asked of the register allocator
+#endif
 template <int PL, int RC, int PI, int PO, int PA, class A, int NL, int MODE, int RED>
-__global__ void __launch_bounds__(256) k_hop(HopArgs P) {
+__global__ void __launch_bounds__(256, QH_HOP_WAVES) k_hop(HopArgs P) {
   using R = RT<A>;
   __shared__ double lds[16];
   if (P.pred && (P.S->done | P.S->need_rel)) return;

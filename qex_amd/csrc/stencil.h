@@ -50,8 +50,9 @@ __device__ __forceinline__ void mem_fence_compiler() { asm volatile("" ::: "memo
 // res = sum_k [U_k(x) v(x+l mu) - U_k(x-l mu)^+ v(x-l mu)], v from the loader.
 // QH_PIPE >= 0: the 2*NL links are unrolled and software-pipelined QH_PIPE
 // legs deep (loads of leg j+QH_PIPE issued before the FMAs of leg j, a
-// compiler barrier after each leg).  QH_PIPE = -1 / -2: a rolled loop over
-// hops / links, which bounds the registers a wave holds.
+// compiler barrier after each leg).  QH_PIPE = -1 / -2 / -3: a rolled loop over
+// hops / links / links with a single accumulator, which bounds the registers
+// a wave holds.
 template <int PL, int RC, int NL, class A, class LD>
 __device__ __forceinline__ void stencil(const Geom &g, const char *links, long lkblk, const float *qs, int par,
                                         long i, const Site &s, LD &&ld, A (&res)[3]) {
@@ -60,7 +61,27 @@ __device__ __forceinline__ void stencil(const Geom &g, const char *links, long l
   A af[3], ab[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) { af[c] = mk<A>(0, 0); ab[c] = mk<A>(0, 0); }
-  if constexpr (D == -1) {  // rolled loop, one hop (two links) per trip
+  if constexpr (D == -3) {  // rolled loop, one link per trip, one accumulator
+#pragma unroll 1
+    for (int j = 0; j < NJ; j++) {
+      Leg<A> L0;
+      leg_load<PL, RC, NL, A>(g, links, lkblk, qs, par, i, s, ld, j, L0);
+      if (!(j & 1)) {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) cmac(af[r], L0.U[3 * r + c], L0.v[c]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            af[r].x -= L0.U[3 * c + r].x * L0.v[c].x + L0.U[3 * c + r].y * L0.v[c].y;
+            af[r].y -= L0.U[3 * c + r].x * L0.v[c].y - L0.U[3 * c + r].y * L0.v[c].x;
+          }
+      }
+    }
+  } else if constexpr (D == -1) {  // rolled loop, one hop (two links) per trip
 #pragma unroll 1
     for (int j = 0; j < NJ; j += 2) {
       Leg<A> L0, L1;
