@@ -61,7 +61,30 @@ __device__ __forceinline__ void stencil(const Geom &g, const char *links, long l
   A af[3], ab[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) { af[c] = mk<A>(0, 0); ab[c] = mk<A>(0, 0); }
-  if constexpr (D == -3) {  // rolled loop, one link per trip, one accumulator
+  if constexpr (D == -4) {  // rolled loop, one accumulator, next link loaded ahead
+    Leg<A> L0;
+    leg_load<PL, RC, NL, A>(g, links, lkblk, qs, par, i, s, ld, 0, L0);
+#pragma unroll 1
+    for (int j = 0; j < NJ; j++) {
+      Leg<A> L1;
+      if (j + 1 < NJ) leg_load<PL, RC, NL, A>(g, links, lkblk, qs, par, i, s, ld, j + 1, L1);
+      if (!(j & 1)) {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) cmac(af[r], L0.U[3 * r + c], L0.v[c]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            af[r].x -= L0.U[3 * c + r].x * L0.v[c].x + L0.U[3 * c + r].y * L0.v[c].y;
+            af[r].y -= L0.U[3 * c + r].x * L0.v[c].y - L0.U[3 * c + r].y * L0.v[c].x;
+          }
+      }
+      L0 = L1;
+    }
+  } else if constexpr (D == -3) {  // rolled loop, one link per trip, one accumulator
 #pragma unroll 1
     for (int j = 0; j < NJ; j++) {
       Leg<A> L0;
