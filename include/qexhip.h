@@ -199,6 +199,36 @@ int qexhip_link_update(qexhip_lattice *lat, const double *P_dev, double eps, dou
 
 /* ---- synthetic inputs (no QEX counterpart; QEX's random fields use its own
         RNG, src/rng).  Outputs are canonical FP64 arrays in device memory. ---- */
+/* ---- HISQ link construction (device arrays, canonical [mu][site][3][3] complex) ---- */
+
+/* Fat7lCoefs, gauge/fat7l.nim:5-11 */
+typedef struct {
+  double one_link, three_staple, five_staple, seven_staple, lepage;
+} qexhip_fat7_coefs;
+/* HisqCoefs without the projection policy (Cayley-Hamilton), physics/hisqLinks.nim:4-8 */
+typedef struct {
+  qexhip_fat7_coefs fat7first, fat7second;
+  double naik;
+} qexhip_hisq_coefs;
+/* HisqCoefs.init defaults, physics/hisqLinks.nim:17-28 (setHisqFat7 :10-15) */
+void qexhip_hisq_coefs_init(qexhip_hisq_coefs *c);
+/* makeImpLinks, gauge/fat7l.nim:71-159: fl = fat7 path sum of gf; when ll is not
+   NULL and naik != 0, ll_mu(x) = naik gl_mu(x) gl_mu(x+mu) gl_mu(x+2mu) with
+   gl = gf_long, or gf when gf_long is NULL.  Outputs must not alias gf. */
+int qexhip_make_imp_links(qexhip_lattice *lat, double *fl_dev, const double *gf_dev, const qexhip_fat7_coefs *coef,
+                          double *ll_dev, const double *gf_long_dev, double naik);
+/* projectU, maths/matrixFunctions.nim:337-352: u = x (x^+ x + 1e-20)^{-1/2} on n 3x3 matrices */
+int qexhip_project_u(long n, double *u_dev, const double *x_dev);
+/* HisqCoefs.smear, physics/hisqLinks.nim:39-51: fat7(first) -> projectU ->
+   fat7(second) and Naik long links; c = NULL takes the defaults.  g is the
+   phased, BC-applied gauge field (tests/examples/testStagProp.nim:24-33). */
+int qexhip_hisq_smear(qexhip_lattice *lat, const qexhip_hisq_coefs *c, const double *g_dev, double *fl_dev,
+                      double *ll_dev);
+/* timing: ms[0] per smear with default coefficients; ms[1..3] k_fat3, k_fat5,
+   k_fatf of the first level, ms[4..6] of the second */
+int qexhip_time_hisq_smear(qexhip_lattice *lat, const double *g_dev, double *fl_dev, double *ll_dev, int nrep,
+                           double *ms);
+
 int qexhip_gen_gauge(qexhip_lattice *lat, uint64_t seed, double *U_dev);
 int qexhip_gen_source(qexhip_lattice *lat, uint64_t seed, int stream, int par, double *x_dev);
 int qexhip_gen_phase_bc(qexhip_lattice *lat, double *U_dev);

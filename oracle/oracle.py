@@ -62,6 +62,9 @@ def _declare(l):
         "qo_wilson_force": (None, [_ip, _dp, D, _dp]),
         "qo_random_momenta": (None, [_ip, U64, _dp]),
         "qo_link_update": (None, [_ip, _dp, D, _dp]),
+        "qo_make_imp_links": (None, [_ip, _dp, _dp, _dp, D, _dp, _dp]),
+        "qo_project_u": (None, [L, _dp, _dp]),
+        "qo_hisq_smear": (None, [_ip, _dp, _dp, D, _dp, _dp, _dp]),
         "qo_su3_exp_i": (None, [_dp, D, _dp]),
         "qo_su3_exp": (None, [_dp, _dp]),
         "qo_mat_mul": (None, [_dp, _dp, _dp]),
@@ -268,3 +271,48 @@ def random_momenta(L, seed):
 def link_update(L, P, eps, U):
     """U <- exp(eps P) U in place."""
     lib().qo_link_update(_L(L), _p(P), eps, _p(U))
+
+
+def fat7_coefs(one_link=0.0, three_staple=0.0, five_staple=0.0, seven_staple=0.0, lepage=0.0):
+    """Fat7lCoefs (gauge/fat7l.nim:5-11) as the 5-vector the oracle takes."""
+    return np.array([one_link, three_staple, five_staple, seven_staple, lepage], dtype=np.float64)
+
+
+def hisq_fat7(f7lf, naik):
+    """setHisqFat7 (physics/hisqLinks.nim:10-15)."""
+    return fat7_coefs((1.0 + 3.0 * f7lf + naik) / 8.0, -1.0 / 16.0, 1.0 / 64.0, -1.0 / 384.0, -f7lf / 16.0)
+
+
+def hisq_coefs():
+    """HisqCoefs.init defaults (physics/hisqLinks.nim:17-28): (first, second, naik)."""
+    return hisq_fat7(0.0, 0.0), hisq_fat7(2.0, 1.0), -1.0 / 24.0
+
+
+def make_imp_links(L, gf, coef, naik=0.0, gl=None):
+    """makeImpLinks (gauge/fat7l.nim:71-159): returns (fl, ll); ll is None
+    when naik == 0."""
+    gf = np.ascontiguousarray(gf, dtype=np.complex128)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    fl = np.empty_like(gf)
+    ll = np.empty_like(gf) if naik != 0.0 else None
+    glp = _p(np.ascontiguousarray(gl, dtype=np.complex128)) if gl is not None else None
+    lib().qo_make_imp_links(_L(L), _p(gf), _p(c), glp, float(naik), _p(fl), _p(ll) if ll is not None else None)
+    return fl, ll
+
+
+def project_u(X):
+    """projectU (maths/matrixFunctions.nim:337-352) on every 3x3 matrix of X."""
+    X = np.ascontiguousarray(X, dtype=np.complex128)
+    out = np.empty_like(X)
+    lib().qo_project_u(X.size // 9, _p(X), _p(out))
+    return out
+
+
+def hisq_smear(L, g, coefs=None):
+    """HisqCoefs.smear (physics/hisqLinks.nim:39-51): returns (fl, ll)."""
+    first, second, naik = coefs if coefs is not None else hisq_coefs()
+    g = np.ascontiguousarray(g, dtype=np.complex128)
+    fl, ll = np.empty_like(g), np.empty_like(g)
+    lib().qo_hisq_smear(_L(L), _p(np.ascontiguousarray(first, dtype=np.float64)),
+                        _p(np.ascontiguousarray(second, dtype=np.float64)), float(naik), _p(g), _p(fl), _p(ll))
+    return fl, ll

@@ -691,3 +691,162 @@ void qo_link_update(const int L[4], const double *P, double eps, double *U) {
     memcpy(U + i * 18, T, sizeof T);
   }
 }
+
+/* --------------------------------------------------- HISQ link construction */
+
+/* Generalized staple of a matrix field K in the (mu,nu) plane,
+   gauge/fat7l.nim:24-67 (computeGenStaple):
+     S_nu[K](x) = U_nu(x) K(x+nu) U_nu(x+mu)^+ + U_nu(x-nu)^+ K(x-nu) U_nu(x-nu+mu) */
+static void gen_staple(const int L[4], const double *U, int mu, int nu, const cplx *K, cplx *out) {
+  long V = qo_volume(L);
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < V; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    int cm[4] = {c[0], c[1], c[2], c[3]};
+    cm[nu] = (cm[nu] - 1 + L[nu]) % L[nu];
+    long xpn = nbr(L, c, nu, 1), xpm = nbr(L, c, mu, 1), xmn = nbr(L, c, nu, -1), xmnpm = nbr(L, cm, mu, 1);
+    cplx t[9], d[9], a[9], b[9];
+    m3mul(lnk(U, V, nu, idx), K + xpn * 9, t);
+    m3dag(lnk(U, V, nu, xpm), d);
+    m3mul(t, d, a);
+    m3dag(lnk(U, V, nu, xmn), d);
+    m3mul(d, K + xmn * 9, t);
+    m3mul(t, lnk(U, V, nu, xmnpm), b);
+    for (int e = 0; e < 9; e++) out[idx * 9 + e] = a[e] + b[e];
+  }
+}
+
+static void axpy_field(long n, double a, const cplx *x, cplx *y) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; i++) y[i] += a * x[i];
+}
+
+/* makeImpLinks, gauge/fat7l.nim:71-159.  coef = {oneLink, threeStaple,
+   fiveStaple, sevenStaple, lepage}.  The Lepage term is the double staple in
+   one plane, whose two retracing paths per direction are cancelled by moving
+   6*lepage out of the one-link term (fat7l.nim:96-97); 5- and 7-link staples
+   nest the 3-link staple in the remaining directions (fat7l.nim:130-141).
+   Long links: ll = naik * gl_mu(x) gl_mu(x+mu) gl_mu(x+2mu) (fat7l.nim:145-154);
+   skipped when ll is NULL or naik == 0; gl = gf when NULL. */
+void qo_make_imp_links(const int L[4], const double *gf, const double coef[5], const double *gl, double naik,
+                       double *fl, double *ll) {
+  long V = qo_volume(L);
+  const double c3 = coef[1], c5 = coef[2], c7 = coef[3], cL = coef[4];
+  const double c1 = coef[0] - 6.0 * cL;
+  const int have5 = (c5 != 0.0) || (c7 != 0.0) || (cL != 0.0);
+  const int have3 = (c3 != 0.0) || have5;
+  cplx *st = malloc(sizeof(cplx) * 9 * V), *t5 = malloc(sizeof(cplx) * 9 * V), *tmp = malloc(sizeof(cplx) * 9 * V);
+  for (int dir = 0; dir < 4; dir++) {
+    cplx *f = (cplx *)(fl + (long)dir * V * 18);
+    const cplx *u = (const cplx *)(gf + (long)dir * V * 18);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < 9 * V; i++) f[i] = c1 * u[i];
+    if (!have3) continue;
+    for (int nu = 0; nu < 4; nu++) {
+      if (nu == dir) continue;
+      gen_staple(L, gf, dir, nu, u, st);
+      axpy_field(9 * V, c3, st, f);
+      if (cL != 0.0) {
+        gen_staple(L, gf, dir, nu, st, tmp);
+        axpy_field(9 * V, cL, tmp, f);
+      }
+      if (c5 != 0.0 || c7 != 0.0)
+        for (int rho = 0; rho < 4; rho++) {
+          if (rho == dir || rho == nu) continue;
+          gen_staple(L, gf, dir, rho, st, t5);
+          axpy_field(9 * V, c5, t5, f);
+          if (c7 != 0.0)
+            for (int sig = 0; sig < 4; sig++) {
+              if (sig == dir || sig == nu || sig == rho) continue;
+              gen_staple(L, gf, dir, sig, t5, tmp);
+              axpy_field(9 * V, c7, tmp, f);
+            }
+        }
+    }
+  }
+  free(st);
+  free(t5);
+  free(tmp);
+  if (ll && naik != 0.0) {
+    if (!gl) gl = gf;
+#pragma omp parallel for schedule(static)
+    for (long idx = 0; idx < V; idx++) {
+      int c[4];
+      qo_coords(L, idx, c);
+      for (int mu = 0; mu < 4; mu++) {
+        cplx t[9], t2[9];
+        m3mul(lnk(gl, V, mu, idx), lnk(gl, V, mu, nbr(L, c, mu, 1)), t);
+        m3mul(t, lnk(gl, V, mu, nbr(L, c, mu, 2)), t2);
+        cplx *o = (cplx *)(ll + ((long)mu * V + idx) * 18);
+        for (int e = 0; e < 9; e++) o[e] = naik * t2[e];
+      }
+    }
+  }
+}
+
+/* Eigenvalues of a Hermitian 3x3 matrix by the trigonometric solution of its
+   characteristic cubic (O. K. Smith, CACM 4 (1961) 168): with q = tr/3,
+   B = H - q, p = sqrt(tr B^2 / 6), r = det(B/p)/2, the eigenvalues are
+   q + 2p cos(acos(r)/3 + 2 pi k/3). */
+static void herm3_eigs(const cplx *h, double ev[3]) {
+  const double q = (creal(h[0]) + creal(h[4]) + creal(h[8])) / 3.0;
+  const double b0 = creal(h[0]) - q, b1 = creal(h[4]) - q, b2 = creal(h[8]) - q;
+  const double o01 = creal(h[1] * conj(h[1])), o02 = creal(h[2] * conj(h[2])), o12 = creal(h[5] * conj(h[5]));
+  const double p2 = (b0 * b0 + b1 * b1 + b2 * b2 + 2.0 * (o01 + o02 + o12)) / 6.0;
+  if (p2 <= 0.0) {
+    ev[0] = ev[1] = ev[2] = q;
+    return;
+  }
+  const double p = sqrt(p2);
+  /* det B for Hermitian B: b0 b1 b2 + 2 Re(h01 h12 h20) - b0|h12|^2 - b1|h02|^2 - b2|h01|^2 */
+  const double detb = b0 * b1 * b2 + 2.0 * creal(h[1] * h[5] * h[6]) - b0 * o12 - b1 * o02 - b2 * o01;
+  double r = detb / (2.0 * p2 * p);
+  r = r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r);
+  const double phi = acos(r) / 3.0;
+  ev[0] = q + 2.0 * p * cos(phi);
+  ev[2] = q + 2.0 * p * cos(phi + 2.0943951023931954923);
+  ev[1] = 3.0 * q - ev[0] - ev[2];
+}
+
+/* projectU by Cayley-Hamilton, maths/matrixFunctions.nim:337-352 with
+   rsqrtPHM3 (:165-219): u = x (x^+ x + eps)^{-1/2}, eps = 1e-20.  The inverse
+   square root of the Hermitian H = x^+ x + eps is the quadratic
+   f0 + f1 H + f2 H^2 that equals 1/s_i at the eigenvalues s_i^2; with
+   u1 = s0+s1+s2, v = s0 s1 + s1 s2 + s2 s0, w = s0 s1 s2 and d = w (u1 v - w):
+   f0 = (u1 v^2 - v w - u1^2 w)/d, f1 = -(u1 tr H + w)/d, f2 = u1/d. */
+void qo_project_u(long n, const double *x, double *u) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; i++) {
+    const cplx *X = (const cplx *)(x + i * 18);
+    cplx Xd[9], H[9], H2[9], R[9];
+    m3dag(X, Xd);
+    m3mul(Xd, X, H);
+    for (int k = 0; k < 3; k++) H[4 * k] += 1e-20;
+    double ev[3];
+    herm3_eigs(H, ev);
+    const double s0 = sqrt(fabs(ev[0])), s1 = sqrt(fabs(ev[1])), s2 = sqrt(fabs(ev[2]));
+    const double u1 = s0 + s1 + s2, v = s0 * s1 + s1 * s2 + s2 * s0, w = s0 * s1 * s2;
+    const double tr = creal(H[0]) + creal(H[4]) + creal(H[8]);
+    const double di = 1.0 / (w * (u1 * v - w));
+    const double f0 = (u1 * v * v - v * w - u1 * u1 * w) * di, f1 = -(u1 * tr + w) * di, f2 = u1 * di;
+    m3mul(H, H, H2);
+    for (int e = 0; e < 9; e++) R[e] = f1 * H[e] + f2 * H2[e];
+    for (int k = 0; k < 3; k++) R[4 * k] += f0;
+    m3mul(X, R, (cplx *)(u + i * 18));
+  }
+}
+
+/* HisqCoefs.smear, physics/hisqLinks.nim:39-51: first fat7 level (no Naik),
+   unitary projection, second fat7 level with Naik long links from the
+   projected links.  first/second as coef of qo_make_imp_links. */
+void qo_hisq_smear(const int L[4], const double first[5], const double second[5], double naik, const double *g,
+                   double *fl, double *ll) {
+  long V = qo_volume(L);
+  double *t1 = malloc(sizeof(double) * 72 * V), *t2 = malloc(sizeof(double) * 72 * V);
+  qo_make_imp_links(L, g, first, NULL, 0.0, t1, NULL);
+  qo_project_u(4 * V, t1, t2);
+  qo_make_imp_links(L, t2, second, t2, naik, fl, ll);
+  free(t1);
+  free(t2);
+}

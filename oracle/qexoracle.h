@@ -106,6 +106,18 @@ void qo_random_momenta(const int L[4], uint64_t seed, double *P);
 /* U <- exp(eps P) U (gaugeUtils.nim:453-475) */
 void qo_link_update(const int L[4], const double *P, double eps, double *U);
 
+/* ---- HISQ link construction (gauge/fat7l.nim, physics/hisqLinks.nim) ---- */
+/* makeImpLinks: coef = {oneLink, threeStaple, fiveStaple, sevenStaple, lepage};
+   fl, gf: 4*V*18; ll = naik * gl gl gl (gl = gf when NULL), skipped when ll is
+   NULL or naik == 0 */
+void qo_make_imp_links(const int L[4], const double *gf, const double coef[5], const double *gl, double naik,
+                       double *fl, double *ll);
+/* u = x (x^+ x + 1e-20)^{-1/2} for n 3x3 matrices (projectU, Cayley-Hamilton) */
+void qo_project_u(long n, const double *x, double *u);
+/* HisqCoefs.smear: fat7(first) -> projectU -> fat7(second) + Naik */
+void qo_hisq_smear(const int L[4], const double first[5], const double second[5], double naik, const double *g,
+                   double *fl, double *ll);
+
 /* ---- 3x3 helpers used by tests ---- */
 void qo_su3_exp_i(const double *H /*18, Hermitian*/, double s, double *U /*18*/);
 void qo_su3_exp(const double *A /*18, general*/, double *U /*18*/);

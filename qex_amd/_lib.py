@@ -29,6 +29,8 @@ EXPORTS = [
     "qexhip_solve_batch", "qexhip_solve_multishift_batch", "qexhip_set_autotune", "qexhip_autotune_report",
     "qexhip_set_cg_form", "qexhip_stagD2_multi", "qexhip_time_hop_multi", "qexhip_stream_read",
     "qexhip_set_cg_graph",
+    "qexhip_hisq_coefs_init", "qexhip_make_imp_links", "qexhip_project_u", "qexhip_hisq_smear",
+    "qexhip_time_hisq_smear",
 ]
 
 # qexhip_solve_batch / qexhip_solve_multishift_batch flags
@@ -52,6 +54,14 @@ class SolverParamsC(ctypes.Structure):
         ("seconds", ctypes.c_double),
         ("flops", ctypes.c_double),
     ]
+
+
+class Fat7CoefsC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("one_link", "three_staple", "five_staple", "seven_staple", "lepage")]
+
+
+class HisqCoefsC(ctypes.Structure):
+    _fields_ = [("fat7first", Fat7CoefsC), ("fat7second", Fat7CoefsC), ("naik", ctypes.c_double)]
 
 
 class QexHipError(RuntimeError):
@@ -100,11 +110,17 @@ def load(path=LIB_PATH):
         "qexhip_set_cg_form": [I], "qexhip_set_cg_graph": [I],
         "qexhip_stagD2_multi": [vp, I, I, vpp, vpp, D, D], "qexhip_time_hop_multi": [vp, I, I, I, dp],
         "qexhip_stream_read": [ctypes.c_long, I, dp],
+        "qexhip_make_imp_links": [vp, vp, vp, ctypes.POINTER(Fat7CoefsC), vp, vp, D],
+        "qexhip_project_u": [ctypes.c_long, vp, vp],
+        "qexhip_hisq_smear": [vp, ctypes.POINTER(HisqCoefsC), vp, vp, vp],
+        "qexhip_time_hisq_smear": [vp, vp, vp, vp, I, dp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = ctypes.c_int
+    lib.qexhip_hisq_coefs_init.restype = None
+    lib.qexhip_hisq_coefs_init.argtypes = [ctypes.POINTER(HisqCoefsC)]
     lib.qexhip_last_error.restype = ctypes.c_char_p
     lib.qexhip_last_error.argtypes = []
     lib.qexhip_stream.restype = ctypes.c_void_p

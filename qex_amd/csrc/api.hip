@@ -84,6 +84,7 @@ struct qexhip_lattice {
   Scal *S = nullptr;   // generic device scalars (API reductions)
   Scal *Sh = nullptr;  // pinned host mirror
   double *stage = nullptr;  // canonical staging buffer (V*6 doubles)
+  SmearWs smear;            // HISQ link construction workspace (allocated on first use)
 };
 
 struct qexhip_field {
@@ -1677,6 +1678,117 @@ int qexhip_link_update(qexhip_lattice *lat, const double *P, double eps, double 
   launch_link_update(lat->g, P, eps, U, g_stream);
   check_launch();
   HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+static void fat7_array(const qexhip_fat7_coefs *c, double out[5]) {
+  out[0] = c->one_link;
+  out[1] = c->three_staple;
+  out[2] = c->five_staple;
+  out[3] = c->seven_staple;
+  out[4] = c->lepage;
+}
+
+static void hisq_fat7(qexhip_fat7_coefs *c, double f7lf, double naik) {
+  c->one_link = (1.0 + 3.0 * f7lf + naik) / 8.0;
+  c->three_staple = -1.0 / 16.0;
+  c->five_staple = 1.0 / 64.0;
+  c->seven_staple = -1.0 / 384.0;
+  c->lepage = -f7lf / 16.0;
+}
+
+void qexhip_hisq_coefs_init(qexhip_hisq_coefs *c) {
+  if (!c) return;
+  hisq_fat7(&c->fat7first, 0.0, 0.0);
+  hisq_fat7(&c->fat7second, 2.0, 1.0);
+  c->naik = -1.0 / 24.0;
+}
+
+int qexhip_make_imp_links(qexhip_lattice *lat, double *fl, const double *gf, const qexhip_fat7_coefs *coef,
+                          double *ll, const double *gf_long, double naik) {
+  API_BEGIN
+  check_init();
+  require(lat && fl && gf && coef, "null argument");
+  require(fl != gf && (!ll || (ll != gf && ll != fl)), "outputs must not alias the input links");
+  if (ll && naik != 0.0)
+    for (int d = 0; d < 4; d++) require(lat->g.L[d] >= 3, "long links need extents >= 3");
+  double c[5];
+  fat7_array(coef, c);
+  launch_make_imp_links(lat->g, lat->smear, fl, gf, c, ll, gf_long, naik, g_remap, g_stream, nullptr);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+int qexhip_project_u(long n, double *u, const double *x) {
+  API_BEGIN
+  check_init();
+  require(u && x && n >= 0, "bad arguments");
+  if (n > 0) launch_project_u(n, u, x, g_stream);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+int qexhip_hisq_smear(qexhip_lattice *lat, const qexhip_hisq_coefs *c, const double *g, double *fl, double *ll) {
+  API_BEGIN
+  check_init();
+  require(lat && g && fl && ll, "null argument");
+  require(fl != g && ll != g && fl != ll, "outputs must not alias the input links");
+  for (int d = 0; d < 4; d++) require(lat->g.L[d] >= 3, "HISQ links need extents >= 3");
+  qexhip_hisq_coefs def;
+  if (!c) {
+    qexhip_hisq_coefs_init(&def);
+    c = &def;
+  }
+  double c1[5], c2[5];
+  fat7_array(&c->fat7first, c1);
+  fat7_array(&c->fat7second, c2);
+  launch_hisq_smear(lat->g, lat->smear, c1, c2, c->naik, g, fl, ll, g_remap, g_stream, nullptr);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+int qexhip_time_hisq_smear(qexhip_lattice *lat, const double *g, double *fl, double *ll, int nrep, double *ms) {
+  API_BEGIN
+  check_init();
+  require(lat && g && fl && ll && ms && nrep > 0, "bad arguments");
+  qexhip_hisq_coefs c;
+  qexhip_hisq_coefs_init(&c);
+  double c1[5], c2[5];
+  fat7_array(&c.fat7first, c1);
+  fat7_array(&c.fat7second, c2);
+  SmearTimes tm[2];
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (auto &t : tm)
+    for (auto &e : t.ev) HIPCHK(hipEventCreate(&e));
+  launch_hisq_smear(lat->g, lat->smear, c1, c2, c.naik, g, fl, ll, g_remap, g_stream, nullptr);
+  HIPCHK(hipEventRecord(e0, g_stream));
+  for (int i = 0; i < nrep; i++)
+    launch_hisq_smear(lat->g, lat->smear, c1, c2, c.naik, g, fl, ll, g_remap, g_stream, nullptr);
+  HIPCHK(hipEventRecord(e1, g_stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  ms[0] = (double)t / nrep;
+  for (int k = 1; k < 7; k++) ms[k] = 0.0;
+  for (int i = 0; i < nrep; i++) {
+    launch_hisq_smear(lat->g, lat->smear, c1, c2, c.naik, g, fl, ll, g_remap, g_stream, tm);
+    HIPCHK(hipStreamSynchronize(g_stream));
+    for (int lv = 0; lv < 2; lv++)
+      for (int k = 0; k < 3; k++) {
+        HIPCHK(hipEventElapsedTime(&t, tm[lv].ev[k], tm[lv].ev[k + 1]));
+        ms[1 + 3 * lv + k] += (double)t / nrep;
+      }
+  }
+  check_launch();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (auto &tt : tm)
+    for (auto &e : tt.ev) (void)hipEventDestroy(e);
   API_END
 }
 

@@ -186,6 +186,27 @@ void launch_wilson_force(const Geom &g, const double *U, double beta, double *F,
 void launch_random_momenta(const Geom &g, uint64_t seed, double *P, hipStream_t st);
 void launch_link_update(const Geom &g, const double *P, double eps, double *U, hipStream_t st);
 
+// smear.hip: HISQ link construction.  Workspace of 36 component-major link
+// fields (input copy, 3-link staples, summed 5-link staples, first-level fat
+// links, projected links), kept between calls on one lattice.
+struct SmearWs {
+  double2 *buf = nullptr;
+  long V = 0;
+  SmearWs() = default;
+  SmearWs(const SmearWs &) = delete;
+  SmearWs &operator=(const SmearWs &) = delete;
+  ~SmearWs();
+};
+// events around k_fat3, k_fat5 and k_fatf of one fat7 level
+struct SmearTimes {
+  hipEvent_t ev[4];
+};
+void launch_make_imp_links(const Geom &g, SmearWs &w, double *fl, const double *gf, const double coef[5], double *ll,
+                           const double *gl, double naik, int remap, hipStream_t st, SmearTimes *tm);
+void launch_project_u(long n, double *u, const double *x, hipStream_t st);
+void launch_hisq_smear(const Geom &g, SmearWs &w, const double first[5], const double second[5], double naik,
+                       const double *gin, double *fl, double *ll, int remap, hipStream_t st, SmearTimes *tm);
+
 // gen.hip
 void launch_gauge_random(const Geom &g, uint64_t seed, double *canon, hipStream_t st);
 void launch_source_gauss(const Geom &g, uint64_t seed, int stream, int par, double *canon, hipStream_t st);

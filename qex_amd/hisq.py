@@ -1,0 +1,136 @@
+"""HISQ link construction on the GPU, mirroring QEX's Nim API:
+
+- ``Fat7lCoefs`` (gauge/fat7l.nim:5-11) and ``setHisqFat7``
+  (physics/hisqLinks.nim:10-15);
+- ``HisqCoefs`` with ``init`` (physics/hisqLinks.nim:17-28) and ``smear``
+  (:39-51): first fat7 level, unitary projection, second fat7 level with the
+  Naik long links;
+- ``makeImpLinks(fl, gf, coef, ll, gfLong, naik)`` (gauge/fat7l.nim:71-159);
+- ``projectU`` (maths/matrixFunctions.nim:337-352, Cayley-Hamilton).
+
+Link fields are torch complex128 tensors (4, nSites, 3, 3) on the GPU in
+QEX's canonical single-rank site order, as produced by ``qex_amd.gauge``.
+Every call runs the HIP kernels of qex_amd/libqexhip.so.
+"""
+import ctypes
+from dataclasses import dataclass, field
+
+import torch
+
+from ._lib import Fat7CoefsC, HisqCoefsC, call, load
+from .stag import newStag3
+
+
+@dataclass
+class Fat7lCoefs:
+    oneLink: float = 0.0
+    threeStaple: float = 0.0
+    fiveStaple: float = 0.0
+    sevenStaple: float = 0.0
+    lepage: float = 0.0
+
+    def setHisqFat7(self, f7lf, naik):
+        """setHisqFat7 (physics/hisqLinks.nim:10-15)."""
+        self.oneLink = (1.0 + 3.0 * f7lf + naik) / 8.0
+        self.threeStaple = -1.0 / 16.0
+        self.fiveStaple = 1.0 / 64.0
+        self.sevenStaple = -1.0 / 384.0
+        self.lepage = -f7lf / 16.0
+        return self
+
+    def _c(self):
+        return Fat7CoefsC(self.oneLink, self.threeStaple, self.fiveStaple, self.sevenStaple, self.lepage)
+
+
+@dataclass
+class HisqCoefs:
+    fat7first: Fat7lCoefs = field(default_factory=Fat7lCoefs)
+    fat7second: Fat7lCoefs = field(default_factory=Fat7lCoefs)
+    naik: float = 0.0
+
+    def init(self):
+        """HisqCoefs.init defaults (physics/hisqLinks.nim:17-28); the unitary
+        projection is Cayley-Hamilton, the reference's default policy."""
+        self.fat7first.setHisqFat7(0.0, 0.0)
+        self.fat7second.setHisqFat7(2.0, 1.0)
+        self.naik = -1.0 / 24.0
+        return self
+
+    def _c(self):
+        return HisqCoefsC(self.fat7first._c(), self.fat7second._c(), float(self.naik))
+
+    def smear(self, lo, g, fl=None, ll=None):
+        """HisqCoefs.smear (physics/hisqLinks.nim:39-51): fat and long links
+        from the phased, BC-applied gauge field g.  Returns (fl, ll)."""
+        _check(lo, g)
+        fl = torch.empty_like(g) if fl is None else fl
+        ll = torch.empty_like(g) if ll is None else ll
+        _check(lo, fl)
+        _check(lo, ll)
+        c = self._c()
+        torch.cuda.synchronize()
+        call("qexhip_hisq_smear", lo.h, ctypes.byref(c), _p(g), _p(fl), _p(ll))
+        return fl, ll
+
+
+def newHISQ():
+    """HisqCoefs with the reference's default coefficients."""
+    return HisqCoefs().init()
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _check(lo, t):
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.complex128 and t.is_contiguous()
+            and tuple(t.shape) == (4, lo.nSites, 3, 3)):
+        raise ValueError("link fields must be contiguous complex128 CUDA tensors of shape (4, nSites, 3, 3)")
+
+
+def makeImpLinks(lo, fl, gf, coef, ll=None, gfLong=None, naik=0.0):
+    """makeImpLinks (gauge/fat7l.nim:71-159): fl = fat7 path sum of gf with
+    coefficients coef (Fat7lCoefs); if ll is given and naik != 0,
+    ll_mu(x) = naik gl_mu(x) gl_mu(x+mu) gl_mu(x+2mu), gl = gfLong or gf."""
+    for t in (fl, gf) + tuple(t for t in (ll, gfLong) if t is not None):
+        _check(lo, t)
+    c = coef._c()
+    torch.cuda.synchronize()
+    call("qexhip_make_imp_links", lo.h, _p(fl), _p(gf), ctypes.byref(c),
+         _p(ll) if ll is not None else None, _p(gfLong) if gfLong is not None else None, float(naik))
+    return fl, ll
+
+
+def projectU(x, out=None):
+    """projectU (maths/matrixFunctions.nim:337-352) on every 3x3 matrix of x:
+    x (x^+ x)^{-1/2}."""
+    if not (x.is_cuda and x.dtype == torch.complex128 and x.is_contiguous() and x.shape[-2:] == (3, 3)):
+        raise ValueError("projectU takes a contiguous complex128 CUDA tensor of 3x3 matrices")
+    out = torch.empty_like(x) if out is None else out
+    torch.cuda.synchronize()
+    call("qexhip_project_u", ctypes.c_long(x.numel() // 9), _p(out), _p(x))
+    return out
+
+
+def hisqStag(lo, g, coefs=None):
+    """Smear g (phased, BC-applied) and build the HISQ Staggered operator from
+    the fat and long links (newStag3, physics/stagD.nim:603-615)."""
+    fl, ll = (coefs or newHISQ()).smear(lo, g)
+    return newStag3(lo, fl, ll), fl, ll
+
+
+def time_smear(lo, g, nrep=10):
+    """Milliseconds per HISQ smear with default coefficients, and per kernel
+    (k_fat3, k_fat5, k_fatf of the first, then the second level)."""
+    fl, ll = torch.empty_like(g), torch.empty_like(g)
+    ms = (ctypes.c_double * 7)()
+    torch.cuda.synchronize()
+    call("qexhip_time_hisq_smear", lo.h, _p(g), _p(fl), _p(ll), int(nrep), ms)
+    return list(ms)
+
+
+def default_coefs_c():
+    """The library's own HisqCoefs defaults (qexhip_hisq_coefs_init)."""
+    c = HisqCoefsC()
+    load().qexhip_hisq_coefs_init(ctypes.byref(c))
+    return c
