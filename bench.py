@@ -53,6 +53,8 @@ def parse():
                     help="add the HISQ-form 32^3x64 m=0.005 storage-precision study")
     ap.add_argument("--batched", action="store_true",
                     help="add the batched-solve study (multi-shift x multi-RHS, uneven CG batch)")
+    ap.add_argument("--hisq-links", action="store_true",
+                    help="only run the HISQ link-construction study (32^3x64 smearing)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -240,6 +242,59 @@ def hisq_study():
     return out
 
 
+# HISQ link construction at 32^3 x 64, per site of one fat7 level: the
+# compulsory matrices each kernel reads and writes (144 B each in FP64) and its
+# flops (a 3x3 complex product is 198 flops, a generalized staple 4 of them
+# plus one matrix add, 810).
+SMEAR_LAT = (32, 32, 32, 64)
+SMEAR_MATS = {"k_fat3": 4 + 12, "k_fat5": 12 + 4 + 12, "k_fatf": 4 + 12 + 12 + 4}
+SMEAR_FLOPS = {"k_fat3": 12 * 810, "k_fat5": 24 * 810 + 12 * 18,
+               # first level: 7-link staples only; second level adds the Lepage term
+               "k_fatf": (4 * (3 * (36 + 36 + 846) + 18), 4 * (3 * (36 + 36 + 846 + 846) + 18))}
+PEAK_FP64_TFLOPS = 78.6
+
+
+def hisq_links_study(cpu=True):
+    """HISQ smearing (fat7 -> projectU -> fat7 + Naik) of a phased random
+    32^3x64 field: time per smear and per kernel, with each kernel's
+    algorithmic GB/s and TFLOP/s; the oracle's time on a bounded CPU sample."""
+    import torch
+    import qex_amd as Q
+    from qex_amd import gauge, hisq
+    lo = Q.Layout(SMEAR_LAT)
+    V = lo.nSites
+    U = gauge.phase_bc(lo, gauge.random_gauge(lo, SEED))
+    ms = hisq.time_smear(lo, U, nrep=5)
+    res = {"lattice": list(SMEAR_LAT), "ms_per_smear": round(ms[0], 3), "kernels": {}}
+    for lv in range(2):
+        for k, name in enumerate(("k_fat3", "k_fat5", "k_fatf")):
+            t = ms[1 + 3 * lv + k] * 1e-3
+            fl = SMEAR_FLOPS[name][lv] if isinstance(SMEAR_FLOPS[name], tuple) else SMEAR_FLOPS[name]
+            gbs = SMEAR_MATS[name] * 144 * V / t / 1e9
+            res["kernels"][f"level{lv + 1}_{name}"] = {
+                "ms": round(t * 1e3, 4), "bytes_per_site": SMEAR_MATS[name] * 144, "flops_per_site": fl,
+                "GB_s": round(gbs, 1), "TFLOP_s": round(fl * V / t / 1e12, 2),
+                "frac_hbm_peak": round(gbs / PEAK_GBS, 4),
+                "frac_fp64_peak": round(fl * V / t / 1e12 / PEAK_FP64_TFLOPS, 4)}
+    del U
+    torch.cuda.empty_cache()
+    if cpu:
+        from oracle import oracle as O
+        Lc = (8, 8, 8, 8)
+        Uc = O.phased(Lc, O.gauge_random(Lc, SEED))
+        O.hisq_smear(Lc, Uc)
+        t0 = time.perf_counter()
+        n = 0
+        while n < 3 or time.perf_counter() - t0 < 2.0:
+            O.hisq_smear(Lc, Uc)
+            n += 1
+        dt = (time.perf_counter() - t0) / n
+        res["cpu_oracle_8x8x8x8"] = {"ms_per_smear": round(dt * 1e3, 2), "cores": os.cpu_count(),
+                                     "us_per_site": round(dt * 1e6 / O.volume(Lc), 3)}
+        res["gpu_us_per_site"] = round(ms[0] * 1e3 / V, 5)
+    return res
+
+
 def batched_study():
     """Batched solves against one-at-a-time solves, FP64, |r|/|b| <= 1e-10.
 
@@ -386,6 +441,9 @@ def main():
     from qex_amd import gauge
     from qex_amd import _lib
     _lib.init(0)
+    if args.hisq_links:
+        print(json.dumps({"hisq_links": hisq_links_study(cpu=not args.no_cpu)}), flush=True)
+        return
 
     lo = Q.Layout(LAT)
     vh = lo.nEven
@@ -520,6 +578,9 @@ def main():
                 best = min(conv, key=lambda k: conv[k]["seconds"])
                 out["cg_best_mixed"] = {"variant": best, "speedup_vs_fp64": cg[best]["speedup_vs_fp64"]}
         out["cg"] = cg
+
+    if not args.no_table:
+        out["hisq_links"] = hisq_links_study(cpu=not args.no_cpu)
 
     if args.hisq_cg or args.batched:
         del s

@@ -130,16 +130,18 @@ struct Fat7 {
   double c1, c3, c5, c7, cL;  // c1 already shifted by -6 cL
 };
 
-// Blocks walk the sites in XCD-contiguous ranges so that the neighbour gathers
-// of one range share an L2.
-__device__ __forceinline__ long site_of(long V, int remap) {
-  return xcd_block(remap) * blockDim.x + threadIdx.x;
-}
+// A workgroup covers 64 consecutive sites with one wavefront per direction mu
+// (per sigma in k_fat5): its waves gather overlapping links of the same
+// neighbourhood at the same time, so those loads hit in the CU's L1 and the
+// XCD's L2 instead of returning in a later pass over the lattice.  Workgroups
+// walk the sites in XCD-contiguous ranges.
+__device__ __forceinline__ long site_of(int remap) { return xcd_block(remap) * 64 + (threadIdx.x & 63); }
+__device__ __forceinline__ int wave_of() { return threadIdx.x >> 6; }
 
 __global__ void __launch_bounds__(256) k_fat3(Geom g, SmearFields f, int remap) {
   const long V = 2 * g.Vh;
-  const long x = site_of(V, remap);
-  const int mu = blockIdx.y;
+  const long x = site_of(remap);
+  const int mu = wave_of();
   if (x >= V) return;
   int c[4];
   uint32_t lex;
@@ -151,10 +153,10 @@ __global__ void __launch_bounds__(256) k_fat3(Geom g, SmearFields f, int remap) 
   }
 }
 
-__global__ void __launch_bounds__(256) k_fat5(Geom g, SmearFields f, int remap) {
+__global__ void __launch_bounds__(192) k_fat5(Geom g, SmearFields f, int remap) {
   const long V = 2 * g.Vh;
-  const long x = site_of(V, remap);
-  const int mu = blockIdx.y, ks = blockIdx.z;
+  const long x = site_of(remap);
+  const int mu = blockIdx.y, ks = wave_of();
   if (x >= V) return;
   const int sig = other(mu, ks);
   int nu = -1, rho = -1;
@@ -179,8 +181,8 @@ __global__ void __launch_bounds__(256) k_fat5(Geom g, SmearFields f, int remap) 
 template <int HAVE>
 __global__ void __launch_bounds__(256) k_fatf(Geom g, SmearFields f, Fat7 c, int remap) {
   const long V = 2 * g.Vh;
-  const long x = site_of(V, remap);
-  const int mu = blockIdx.y;
+  const long x = site_of(remap);
+  const int mu = wave_of();
   if (x >= V) return;
   M3 acc = ldf(f.U[mu], x);
 #pragma unroll
@@ -210,8 +212,8 @@ __global__ void __launch_bounds__(256) k_fatf(Geom g, SmearFields f, Fat7 c, int
 __global__ void __launch_bounds__(256) k_naik(Geom g, MF W0, MF W1, MF W2, MF W3, MF o0, MF o1, MF o2, MF o3,
                                               double naik, int remap) {
   const long V = 2 * g.Vh;
-  const long x = site_of(V, remap);
-  const int mu = blockIdx.y;
+  const long x = site_of(remap);
+  const int mu = wave_of();
   if (x >= V) return;
   const MF W = mu == 0 ? W0 : (mu == 1 ? W1 : (mu == 2 ? W2 : W3));
   const MF o = mu == 0 ? o0 : (mu == 1 ? o1 : (mu == 2 ? o2 : o3));
@@ -319,6 +321,12 @@ static void evrec(hipEvent_t e, hipStream_t st) {
 
 static unsigned nblk(long V) { return (unsigned)((V + 255) / 256); }
 
+// workgroups of 64 sites, a multiple of 8 when they are remapped to XCDs
+static unsigned site_groups(long V, int remap) {
+  const unsigned n = (unsigned)((V + 63) / 64);
+  return remap ? (n + 7) & ~7u : n;
+}
+
 // one fat7 level: input links U (4 views), output fat links out (4 views)
 static void fat7_level(const Geom &g, SmearWs &w, const MF U[4], const MF out[4], const double coef[5], int remap,
                        hipStream_t st, SmearTimes *tm) {
@@ -336,25 +344,24 @@ static void fat7_level(const Geom &g, SmearWs &w, const MF U[4], const MF out[4]
   const Fat7 c{coef[0] - 6.0 * cL, coef[1], coef[2], coef[3], cL};
   const bool have5 = c.c5 != 0.0 || c.c7 != 0.0 || c.cL != 0.0;
   const bool have3 = c.c3 != 0.0 || have5;
-  const unsigned nb = remap ? (unsigned)((nblk(V) + 7) & ~7u) : nblk(V);
+  const unsigned nb = site_groups(V, remap);
   if (tm) evrec(tm->ev[0], st);
-  if (have3) hipLaunchKernelGGL(k_fat3, dim3(nb, 4), dim3(256), 0, st, g, f, remap);
+  if (have3) hipLaunchKernelGGL(k_fat3, dim3(nb), dim3(256), 0, st, g, f, remap);
   if (tm) evrec(tm->ev[1], st);
-  if (c.c5 != 0.0 || c.c7 != 0.0) hipLaunchKernelGGL(k_fat5, dim3(nb, 4, 3), dim3(256), 0, st, g, f, remap);
+  if (c.c5 != 0.0 || c.c7 != 0.0) hipLaunchKernelGGL(k_fat5, dim3(nb, 4), dim3(192), 0, st, g, f, remap);
   if (tm) evrec(tm->ev[2], st);
   if (have5)
-    hipLaunchKernelGGL((k_fatf<2>), dim3(nb, 4), dim3(256), 0, st, g, f, c, remap);
+    hipLaunchKernelGGL((k_fatf<2>), dim3(nb), dim3(256), 0, st, g, f, c, remap);
   else if (have3)
-    hipLaunchKernelGGL((k_fatf<1>), dim3(nb, 4), dim3(256), 0, st, g, f, c, remap);
+    hipLaunchKernelGGL((k_fatf<1>), dim3(nb), dim3(256), 0, st, g, f, c, remap);
   else
-    hipLaunchKernelGGL((k_fatf<0>), dim3(nb, 4), dim3(256), 0, st, g, f, c, remap);
+    hipLaunchKernelGGL((k_fatf<0>), dim3(nb), dim3(256), 0, st, g, f, c, remap);
   if (tm) evrec(tm->ev[3], st);
 }
 
 static void naik_links(const Geom &g, const MF W[4], const MF o[4], double naik, int remap, hipStream_t st) {
   const long V = 2 * g.Vh;
-  const unsigned nb = remap ? (unsigned)((nblk(V) + 7) & ~7u) : nblk(V);
-  hipLaunchKernelGGL(k_naik, dim3(nb, 4), dim3(256), 0, st, g, W[0], W[1], W[2], W[3], o[0], o[1], o[2], o[3], naik,
+  hipLaunchKernelGGL(k_naik, dim3(site_groups(V, remap)), dim3(256), 0, st, g, W[0], W[1], W[2], W[3], o[0], o[1], o[2], o[3], naik,
                      remap);
 }
 
