@@ -65,6 +65,10 @@ def _declare(l):
         "qo_make_imp_links": (None, [_ip, _dp, _dp, _dp, D, _dp, _dp]),
         "qo_project_u": (None, [L, _dp, _dp]),
         "qo_hisq_smear": (None, [_ip, _dp, _dp, D, _dp, _dp, _dp]),
+        "qo_fat7l_deriv": (None, [_ip, _dp, _dp, _dp, _dp, _dp, _dp, D, _dp]),
+        "qo_project_u_deriv": (None, [L, _dp, _dp, _dp, _dp]),
+        "qo_hisq_smear_deriv": (None, [_ip, _dp, _dp, D, _dp, _dp, _dp, _dp]),
+        "qo_hisq_outer": (None, [_ip, _dp, _dp]),
         "qo_su3_exp_i": (None, [_dp, D, _dp]),
         "qo_su3_exp": (None, [_dp, _dp]),
         "qo_mat_mul": (None, [_dp, _dp, _dp]),
@@ -316,3 +320,49 @@ def hisq_smear(L, g, coefs=None):
     lib().qo_hisq_smear(_L(L), _p(np.ascontiguousarray(first, dtype=np.float64)),
                         _p(np.ascontiguousarray(second, dtype=np.float64)), float(naik), _p(g), _p(fl), _p(ll))
     return fl, ll
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def fat7l_deriv(L, gf, mid, coef, naik=0.0, gl=None, llmid=None):
+    """fat7lDeriv (gauge/fat7lderiv.nim:122-475): (dS/dgf, dS/dgl) given
+    mid = dS/dfl and llmid = dS/dll.  The Naik part is returned separately
+    (None without llmid or naik)."""
+    gf, mid = _c(gf), _c(mid)
+    d = np.zeros_like(gf)
+    lld = np.zeros_like(gf) if (llmid is not None and naik != 0.0) else None
+    glp = _p(_c(gl)) if gl is not None else None
+    llm = _p(_c(llmid)) if lld is not None else None
+    lib().qo_fat7l_deriv(_L(L), _p(gf), _p(mid), _p(np.ascontiguousarray(coef, dtype=np.float64)), _p(d),
+                         glp, llm, float(naik), _p(lld) if lld is not None else None)
+    return d, lld
+
+
+def project_u_deriv(U, X, C):
+    """projectUderiv (maths/matrixFunctions.nim:361-392): dS/dX given U =
+    projectU(X) and C = dS/dU."""
+    U, X, C = _c(U), _c(X), _c(C)
+    out = np.empty_like(X)
+    lib().qo_project_u_deriv(X.size // 9, _p(U), _p(X), _p(C), _p(out))
+    return out
+
+
+def hisq_smear_deriv(L, g, dfl, dll, coefs=None):
+    """smearGetForce (gauge/hisqsmear.nim:54-91): dS/dg for (fl, ll) = smear(g)."""
+    first, second, naik = coefs if coefs is not None else hisq_coefs()
+    g, dfl, dll = _c(g), _c(dfl), _c(dll)
+    dg = np.zeros_like(g)
+    lib().qo_hisq_smear_deriv(_L(L), _p(np.ascontiguousarray(first, dtype=np.float64)),
+                              _p(np.ascontiguousarray(second, dtype=np.float64)), float(naik), _p(g), _p(dfl),
+                              _p(dll), _p(dg))
+    return dg
+
+
+def hisq_outer(L, psi):
+    """(8, V, 3, 3): eps psi psi(x+mu)^+ at [2mu] and eps psi psi(x+3mu)^+ at [2mu+1]."""
+    psi = _c(psi)
+    G = np.empty((8, volume(L), 3, 3), dtype=np.complex128)
+    lib().qo_hisq_outer(_L(L), _p(psi), _p(G))
+    return G

@@ -850,3 +850,274 @@ void qo_hisq_smear(const int L[4], const double first[5], const double second[5]
   free(t1);
   free(t2);
 }
+
+/* ------------------------------------------- HISQ smearing chain rule (force) */
+/* Derivatives are with respect to the complex link entries in QEX's pairing
+   redot(A, B) = Re tr(A^+ B) (gauge/fat7lderiv.nim, maths/matrixFunctions.nim:
+   361-392): given mid = dS/dF for F = f(U), deriv += dS/dU, i.e.
+   sum redot(deriv, dU) = sum redot(mid, dF) to first order. */
+
+/* Derivative of the generalized staple S_lam[K] (plane mu,lam) with respect to
+   its side links U_lam, chain C, accumulated with weight a into D (direction
+   lam): the four link occurrences of the upper and lower staples,
+     C(y) U(y+mu) K(y+lam)^+ + K(y) U(y+mu) C(y+lam)^+
+   + C(y-mu)^+ U(y-mu) K(y-mu+lam) + K(y-mu)^+ U(y-mu) C(y-mu+lam),  U = U_lam.
+   The derivative with respect to K is S_lam[C] itself (the staple map is its
+   own adjoint in this pairing). */
+static void side_deriv(const int L[4], const double *U, int mu, int lam, const cplx *C, const cplx *K, double a,
+                       cplx *D) {
+  if (a == 0.0) return;
+  long V = qo_volume(L);
+#pragma omp parallel for schedule(static)
+  for (long y = 0; y < V; y++) {
+    int c[4];
+    qo_coords(L, y, c);
+    int cm[4] = {c[0], c[1], c[2], c[3]};
+    cm[mu] = (cm[mu] - 1 + L[mu]) % L[mu];
+    long ypm = nbr(L, c, mu, 1), ypl = nbr(L, c, lam, 1), ymm = qo_index(L, cm), ymmpl = nbr(L, cm, lam, 1);
+    cplx t[9], d[9], r[9], acc[9] = {0};
+    /* C(y) U(y+mu) K(y+lam)^+ */
+    m3mul(C + y * 9, lnk(U, V, lam, ypm), t);
+    m3dag(K + ypl * 9, d);
+    m3mul(t, d, r);
+    for (int e = 0; e < 9; e++) acc[e] += r[e];
+    /* K(y) U(y+mu) C(y+lam)^+ */
+    m3mul(K + y * 9, lnk(U, V, lam, ypm), t);
+    m3dag(C + ypl * 9, d);
+    m3mul(t, d, r);
+    for (int e = 0; e < 9; e++) acc[e] += r[e];
+    /* C(y-mu)^+ U(y-mu) K(y-mu+lam) */
+    m3dag(C + ymm * 9, d);
+    m3mul(d, lnk(U, V, lam, ymm), t);
+    m3mul(t, K + ymmpl * 9, r);
+    for (int e = 0; e < 9; e++) acc[e] += r[e];
+    /* K(y-mu)^+ U(y-mu) C(y-mu+lam) */
+    m3dag(K + ymm * 9, d);
+    m3mul(d, lnk(U, V, lam, ymm), t);
+    m3mul(t, C + ymmpl * 9, r);
+    for (int e = 0; e < 9; e++) D[y * 9 + e] += a * (acc[e] + r[e]);
+  }
+}
+
+static int third_dir(int a, int b, int c) {
+  for (int d = 0; d < 4; d++)
+    if (d != a && d != b && d != c) return d;
+  return -1;
+}
+
+/* fat7lDeriv, gauge/fat7lderiv.nim:122-475: the chain rule of makeImpLinks.
+   Per direction mu the forward pass is rebuilt (S_nu = S_nu[U_mu], T_sig =
+   S_rho[S_nu] + S_nu[S_rho] for {nu,rho} = rest of {mu,sig}) and run
+   backwards: chains CT_sig = c5 C + c7 S_sig[C] and CS_nu = c3 C + cL S_nu[C]
+   + sum_{sig != nu} S_rho[CT_sig]; deriv_mu gets (c1 - 6 cL) C + sum_nu
+   S_nu[CS_nu]; each side link U_lam gets the side derivatives of every staple
+   in the (mu,lam) plane.  Naik (fat7lderiv.nim:436-475): for
+   ll = naik gl(x) gl(x+mu) gl(x+2mu), llderiv_mu(y) += naik [C(y) (gl(y+mu)
+   gl(y+2mu))^+ + gl(y-mu)^+ C(y-mu) gl(y+mu)^+ + (gl(y-2mu) gl(y-mu))^+ C(y-2mu)]. */
+void qo_fat7l_deriv(const int L[4], const double *gf, const double *mid, const double coef[5], double *deriv,
+                    const double *gl, const double *llmid, double naik, double *llderiv) {
+  long V = qo_volume(L);
+  const double c3 = coef[1], c5 = coef[2], c7 = coef[3], cL = coef[4];
+  const double c1 = coef[0] - 6.0 * cL;
+  cplx *S[4], *T[4], *CS[4], *CT[4], *tmp = malloc(sizeof(cplx) * 9 * V);
+  for (int k = 0; k < 4; k++) {
+    S[k] = calloc(9 * V, sizeof(cplx));
+    T[k] = calloc(9 * V, sizeof(cplx));
+    CS[k] = calloc(9 * V, sizeof(cplx));
+    CT[k] = calloc(9 * V, sizeof(cplx));
+  }
+  for (int mu = 0; mu < 4; mu++) {
+    const cplx *u = (const cplx *)(gf + (long)mu * V * 18);
+    const cplx *C = (const cplx *)(mid + (long)mu * V * 18);
+    cplx *Dm = (cplx *)(deriv + (long)mu * V * 18);
+    axpy_field(9 * V, c1, C, Dm);
+    for (int nu = 0; nu < 4; nu++)
+      if (nu != mu) gen_staple(L, gf, mu, nu, u, S[nu]);
+    for (int sig = 0; sig < 4; sig++) {
+      if (sig == mu) continue;
+      int nu = -1, rho = -1;
+      for (int d = 0; d < 4; d++)
+        if (d != mu && d != sig) {
+          if (nu < 0) nu = d; else rho = d;
+        }
+      gen_staple(L, gf, mu, rho, S[nu], T[sig]);
+      gen_staple(L, gf, mu, nu, S[rho], tmp);
+      axpy_field(9 * V, 1.0, tmp, T[sig]);
+      /* CT_sig = c5 C + c7 S_sig[C] */
+      gen_staple(L, gf, mu, sig, C, CT[sig]);
+      for (long i = 0; i < 9 * V; i++) CT[sig][i] = c7 * CT[sig][i] + c5 * C[i];
+    }
+    for (int nu = 0; nu < 4; nu++) {
+      if (nu == mu) continue;
+      gen_staple(L, gf, mu, nu, C, CS[nu]);
+      for (long i = 0; i < 9 * V; i++) CS[nu][i] = cL * CS[nu][i] + c3 * C[i];
+      for (int sig = 0; sig < 4; sig++) {
+        if (sig == mu || sig == nu) continue;
+        gen_staple(L, gf, mu, third_dir(mu, nu, sig), CT[sig], tmp);
+        axpy_field(9 * V, 1.0, tmp, CS[nu]);
+      }
+    }
+    for (int nu = 0; nu < 4; nu++) {
+      if (nu == mu) continue;
+      gen_staple(L, gf, mu, nu, CS[nu], tmp);
+      axpy_field(9 * V, 1.0, tmp, Dm);
+    }
+    for (int lam = 0; lam < 4; lam++) {
+      if (lam == mu) continue;
+      cplx *Dl = (cplx *)(deriv + (long)lam * V * 18);
+      side_deriv(L, gf, mu, lam, CS[lam], u, 1.0, Dl);     /* S_lam[U_mu] */
+      side_deriv(L, gf, mu, lam, C, S[lam], cL, Dl);       /* S_lam[S_lam] (Lepage) */
+      side_deriv(L, gf, mu, lam, C, T[lam], c7, Dl);       /* S_lam[T_lam] (7-link) */
+      for (int sig = 0; sig < 4; sig++) {                   /* S_lam[S_other] inside T_sig */
+        if (sig == mu || sig == lam) continue;
+        side_deriv(L, gf, mu, lam, CT[sig], S[third_dir(mu, lam, sig)], 1.0, Dl);
+      }
+    }
+  }
+  for (int k = 0; k < 4; k++) {
+    free(S[k]); free(T[k]); free(CS[k]); free(CT[k]);
+  }
+  free(tmp);
+  if (llderiv && llmid && naik != 0.0) {
+    if (!gl) gl = gf;
+#pragma omp parallel for schedule(static)
+    for (long y = 0; y < V; y++) {
+      int c[4];
+      qo_coords(L, y, c);
+      for (int mu = 0; mu < 4; mu++) {
+        const cplx *C = (const cplx *)(llmid + (long)mu * V * 18);
+        long p1 = nbr(L, c, mu, 1), p2 = nbr(L, c, mu, 2), m1 = nbr(L, c, mu, -1), m2 = nbr(L, c, mu, -2);
+        cplx a[9], b[9], t[9], r[9], acc[9];
+        /* C(y) (gl(y+mu) gl(y+2mu))^+ */
+        m3mul(lnk(gl, V, mu, p1), lnk(gl, V, mu, p2), a);
+        m3dag(a, b);
+        m3mul(C + y * 9, b, acc);
+        /* gl(y-mu)^+ C(y-mu) gl(y+mu)^+ */
+        m3dag(lnk(gl, V, mu, m1), a);
+        m3mul(a, C + m1 * 9, t);
+        m3dag(lnk(gl, V, mu, p1), b);
+        m3mul(t, b, r);
+        for (int e = 0; e < 9; e++) acc[e] += r[e];
+        /* (gl(y-2mu) gl(y-mu))^+ C(y-2mu) */
+        m3mul(lnk(gl, V, mu, m2), lnk(gl, V, mu, m1), a);
+        m3dag(a, b);
+        m3mul(b, C + m2 * 9, r);
+        cplx *o = (cplx *)(llderiv + ((long)mu * V + y) * 18);
+        for (int e = 0; e < 9; e++) o[e] += naik * (acc[e] + r[e]);
+      }
+    }
+  }
+}
+
+static void m3inv(const cplx *a, cplx *r) {
+  cplx c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+  cplx det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+  cplx id = 1.0 / det;
+  r[0] = c00 * id; r[3] = c01 * id; r[6] = c02 * id;
+  r[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+  r[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+  r[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+  r[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+  r[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+  r[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
+/* Solves A X + X A = C for Hermitian positive A (maths/projUderiv.nim:96,
+   sylsolve) in closed form: with p(l) = l^3 + a2 l^2 + a1 l + a0 the
+   characteristic polynomial of A, Cayley-Hamilton (p(A) = 0) applied to
+   A^k X = X (-A)^k + sum_j A^j C (-A)^{k-1-j} gives
+   X = -[a1 C + a2 (AC - CA) + C A^2 - A C A + A^2 C] p(-A)^{-1}. */
+static void sylsolve3(const cplx *A, const cplx *C, cplx *X) {
+  cplx A2[9], AC[9], CA[9], CA2[9], ACA[9], A2C[9], A3[9], N[9], P[9], Pi[9];
+  m3mul(A, A, A2);
+  m3mul(A, C, AC);
+  m3mul(C, A, CA);
+  m3mul(C, A2, CA2);
+  m3mul(AC, A, ACA);
+  m3mul(A2, C, A2C);
+  m3mul(A2, A, A3);
+  const cplx tr = A[0] + A[4] + A[8], tr2 = A2[0] + A2[4] + A2[8];
+  const cplx det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                   A[2] * (A[3] * A[7] - A[4] * A[6]);
+  const cplx a2 = -tr, a1 = 0.5 * (tr * tr - tr2), a0 = -det;
+  for (int e = 0; e < 9; e++) {
+    N[e] = a1 * C[e] + a2 * (AC[e] - CA[e]) + CA2[e] - ACA[e] + A2C[e];
+    P[e] = -A3[e] + a2 * A2[e] - a1 * A[e];
+  }
+  for (int k = 0; k < 3; k++) P[4 * k] += a0;
+  m3inv(P, Pi);
+  m3mul(N, Pi, X);
+  for (int e = 0; e < 9; e++) X[e] = -X[e];
+}
+
+/* projectUderiv, maths/matrixFunctions.nim:361-392: for u = x z,
+   z = (x^+ x)^{-1/2}, y = z^{-1}, chain c = dS/du:
+   dS/dx = c z - x (t + t^+) with y t + t y = u^+ c z. */
+void qo_project_u_deriv(long n, const double *u, const double *x, const double *chain, double *r) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; i++) {
+    const cplx *X = (const cplx *)(x + i * 18), *Uu = (const cplx *)(u + i * 18), *C = (const cplx *)(chain + i * 18);
+    cplx Xd[9], H[9], H2[9], Z[9], Y[9], R[9], Ud[9], t1[9], t2[9], s[9], xs[9];
+    m3dag(X, Xd);
+    m3mul(Xd, X, H);
+    for (int k = 0; k < 3; k++) H[4 * k] += 1e-20;
+    double ev[3];
+    herm3_eigs(H, ev);
+    const double s0 = sqrt(fabs(ev[0])), s1 = sqrt(fabs(ev[1])), s2 = sqrt(fabs(ev[2]));
+    const double u1 = s0 + s1 + s2, v = s0 * s1 + s1 * s2 + s2 * s0, w = s0 * s1 * s2;
+    const double tr = creal(H[0]) + creal(H[4]) + creal(H[8]);
+    const double di = 1.0 / (w * (u1 * v - w));
+    const double f0 = (u1 * v * v - v * w - u1 * u1 * w) * di, f1 = -(u1 * tr + w) * di, f2 = u1 * di;
+    m3mul(H, H, H2);
+    for (int e = 0; e < 9; e++) Z[e] = f1 * H[e] + f2 * H2[e];
+    for (int k = 0; k < 3; k++) Z[4 * k] += f0;
+    m3inv(Z, Y);
+    m3mul(C, Z, R);
+    m3dag(Uu, Ud);
+    m3mul(Ud, R, t1);
+    sylsolve3(Y, t1, t2);
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) s[3 * a + b] = t2[3 * a + b] + conj(t2[3 * b + a]);
+    m3mul(X, s, xs);
+    cplx *o = (cplx *)(r + i * 18);
+    for (int e = 0; e < 9; e++) o[e] = R[e] - xs[e];
+  }
+}
+
+/* HisqCoefs.smearGetForce, gauge/hisqsmear.nim:54-91: dS/dg for fl, ll =
+   smear(g), given dfl = dS/dfl and dll = dS/dll: second-level fat7 and Naik
+   chain to the projected links w, projectUderiv to the first-level links v,
+   first-level chain to g.  dg is accumulated. */
+void qo_hisq_smear_deriv(const int L[4], const double first[5], const double second[5], double naik, const double *g,
+                         const double *dfl, const double *dll, double *dg) {
+  long V = qo_volume(L);
+  double *v = malloc(sizeof(double) * 72 * V), *w = malloc(sizeof(double) * 72 * V);
+  double *t = calloc(72 * V, sizeof(double)), *t2 = malloc(sizeof(double) * 72 * V);
+  qo_make_imp_links(L, g, first, NULL, 0.0, v, NULL);
+  qo_project_u(4 * V, v, w);
+  qo_fat7l_deriv(L, w, dfl, second, t, w, dll, naik, t);
+  qo_project_u_deriv(4 * V, w, v, t, t2);
+  qo_fat7l_deriv(L, g, t2, first, dg, NULL, NULL, 0.0, NULL);
+  free(v); free(w); free(t); free(t2);
+}
+
+/* Outer products of the HISQ fermion force (hmc/hmcAction.nim:735-771 with
+   the 3-hop term of physics/stagD.nim:640-715): G[2mu](x) = eps(x) psi(x)
+   psi(x+mu)^+, G[2mu+1](x) = eps(x) psi(x) psi(x+3mu)^+, eps = -1 on odd x. */
+void qo_hisq_outer(const int L[4], const double *psi, double *G) {
+  long V = qo_volume(L), Vh = V / 2;
+  const cplx *P = (const cplx *)psi;
+  cplx *F = (cplx *)G;
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < V; idx++) {
+    int c[4];
+    qo_coords(L, idx, c);
+    double eps = (idx < Vh) ? 1.0 : -1.0;
+    for (int k = 0; k < 8; k++) {
+      long j = nbr(L, c, k >> 1, (k & 1) ? 3 : 1);
+      cplx *f = F + ((long)k * V + idx) * 9;
+      for (int a = 0; a < 3; a++)
+        for (int bb = 0; bb < 3; bb++) f[3 * a + bb] = eps * P[3 * idx + a] * conj(P[3 * j + bb]);
+    }
+  }
+}

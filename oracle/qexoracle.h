@@ -118,6 +118,19 @@ void qo_project_u(long n, const double *x, double *u);
 void qo_hisq_smear(const int L[4], const double first[5], const double second[5], double naik, const double *g,
                    double *fl, double *ll);
 
+/* ---- HISQ smearing chain rule (derivatives in the redot pairing Re tr(A^+ B)) ---- */
+/* fatl7Deriv: deriv += dS/dgf given mid = dS/dfl; llderiv += dS/dgl given llmid = dS/dll
+   (Naik part skipped when llderiv or llmid is NULL or naik == 0; gl = gf when NULL) */
+void qo_fat7l_deriv(const int L[4], const double *gf, const double *mid, const double coef[5], double *deriv,
+                    const double *gl, const double *llmid, double naik, double *llderiv);
+/* projectUderiv on n matrices: r = dS/dx given u = projectU(x) and chain = dS/du */
+void qo_project_u_deriv(long n, const double *u, const double *x, const double *chain, double *r);
+/* smearGetForce: dg += dS/dg given dfl = dS/dfl, dll = dS/dll for (fl, ll) = smear(g) */
+void qo_hisq_smear_deriv(const int L[4], const double first[5], const double second[5], double naik, const double *g,
+                         const double *dfl, const double *dll, double *dg);
+/* HISQ force outer products: G[2mu] = eps psi psi(x+mu)^+, G[2mu+1] = eps psi psi(x+3mu)^+ */
+void qo_hisq_outer(const int L[4], const double *psi, double *G);
+
 /* ---- 3x3 helpers used by tests ---- */
 void qo_su3_exp_i(const double *H /*18, Hermitian*/, double s, double *U /*18*/);
 void qo_su3_exp(const double *A /*18, general*/, double *U /*18*/);

@@ -183,3 +183,120 @@ def test_hisq_smear_gauge_covariance(oracle):
     for mu in range(4):
         assert rel(flg[mu], G @ fl[mu] @ dag(G[fwd(mu, 1)])) < 1e-10
         assert rel(llg[mu], G @ ll[mu] @ dag(G[fwd(mu, 3)])) < 1e-10
+
+
+# --------------------------------------------------------------- chain rule
+# A fat7 level is a polynomial of degree <= 7 in the links, so the 9-point
+# central difference (exact through degree 8) gives its directional
+# derivative exactly up to rounding; projectU and the full smearing are
+# smooth and the same stencil with a small step leaves O(h^8) truncation.
+W9 = np.array([1 / 280, -4 / 105, 1 / 5, -4 / 5, 0, 4 / 5, -1 / 5, 4 / 105, -1 / 280])
+
+
+def redot(a, b):
+    return float(np.real(np.sum(np.conj(a) * b)))
+
+
+def _crand(rng, *shape):
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+def fd9(f, h):
+    return float(np.dot(W9, [f(k * h) for k in range(-4, 5)]) / h)
+
+
+def test_fat7_deriv_against_exact_differences(oracle):
+    L = (4, 4, 4, 6)
+    V = oracle.volume(L)
+    rng = np.random.default_rng(SEED)
+    U = oracle.gauge_random(L, SEED + 6)
+    C, dU = _crand(rng, 4, V, 3, 3), _crand(rng, 4, V, 3, 3)
+    for name, c in list(TERMS.items()) + [("hisq second", tuple(oracle.hisq_coefs()[1]))]:
+        coef = oracle.fat7_coefs(*c)
+        fd = fd9(lambda t: redot(C, oracle.make_imp_links(L, U + t * dU, coef)[0]), 1e-2)
+        d, _ = oracle.fat7l_deriv(L, U, C, coef)
+        assert abs(redot(d, dU) - fd) < 1e-12 * abs(fd), name
+    # Naik long links from a separate field
+    W = oracle.gauge_random(L, SEED + 7)
+    Cl = _crand(rng, 4, V, 3, 3)
+    fd = fd9(lambda t: redot(Cl, oracle.make_imp_links(L, U, oracle.fat7_coefs(1.0), naik=-1 / 24,
+                                                       gl=W + t * dU)[1]), 1e-2)
+    _, lld = oracle.fat7l_deriv(L, U, np.zeros_like(U), oracle.fat7_coefs(1.0), naik=-1 / 24, gl=W, llmid=Cl)
+    assert abs(redot(lld, dU) - fd) < 1e-12 * abs(fd)
+    # one link at a time: the derivative's entries, not only a projection of them
+    for mu, site in [(0, 3), (3, V - 1), (2, V // 2)]:
+        E = np.zeros_like(U)
+        E[mu, site] = _crand(rng, 3, 3)
+        coef = oracle.fat7_coefs(*TERMS["all"])
+        fd = fd9(lambda t: redot(C, oracle.make_imp_links(L, U + t * E, coef)[0]), 1e-2)
+        d, _ = oracle.fat7l_deriv(L, U, C, coef)
+        assert abs(redot(d, E) - fd) < 1e-11 * abs(fd)
+
+
+def test_project_u_deriv_against_differences(oracle):
+    rng = np.random.default_rng(SEED + 1)
+    X = np.eye(3) + 0.3 * _crand(rng, 64, 3, 3)
+    C, dX = _crand(rng, 64, 3, 3), _crand(rng, 64, 3, 3)
+    fd = fd9(lambda t: redot(C, oracle.project_u(X + t * dX)), 3e-4)
+    r = oracle.project_u_deriv(oracle.project_u(X), X, C)
+    assert abs(redot(r, dX) - fd) < 1e-10 * abs(fd)
+
+
+def test_hisq_smear_deriv_against_differences(oracle):
+    L = (4, 4, 4, 6)
+    V = oracle.volume(L)
+    rng = np.random.default_rng(SEED + 2)
+    U = oracle.phased(L, oracle.gauge_random(L, SEED + 8))
+    Cf, Cl, dU = _crand(rng, 4, V, 3, 3), _crand(rng, 4, V, 3, 3), _crand(rng, 4, V, 3, 3)
+
+    def S(t):
+        fl, ll = oracle.hisq_smear(L, U + t * dU)
+        return redot(Cf, fl) + redot(Cl, ll)
+
+    # the rough first-level links include condition numbers ~100, so the
+    # projection is strongly curved there: a small step keeps the O(h^8) term down
+    fd = fd9(S, 3e-5)
+    assert abs(redot(oracle.hisq_smear_deriv(L, U, Cf, Cl), dU) - fd) < 1e-9 * abs(fd)
+
+
+def _g8(oracle, L, U):
+    fl, ll = oracle.hisq_smear(L, U)
+    return np.ascontiguousarray(np.stack([t for mu in range(4) for t in (fl[mu], ll[mu])]))
+
+
+def test_hisq_fermion_force_against_differences(oracle):
+    """dS/dU of S = phi^+ (m^2 - D_eo D_oe)^-1 phi with the HISQ operator built
+    from smear(U): the outer products 1/2 eps psi psi(x+mu)^+ and
+    1/2 eps psi psi(x+3mu)^+ chained through both smearing levels."""
+    L, m = (4, 4, 4, 6), 0.2
+    V = oracle.volume(L)
+    rng = np.random.default_rng(SEED + 3)
+    U = oracle.phased(L, oracle.gauge_random(L, SEED + 9))
+    phi = oracle.source_gauss(L, 5, 0, 0)
+
+    def S(t):
+        y, _, _ = oracle.solveXX(L, _g8(oracle, L, U + t * dU), 0, phi, m, 1e-30)
+        return float(np.vdot(phi[: V // 2], 4 * y[: V // 2]).real)
+
+    psi, _ = oracle.force_solve(L, _g8(oracle, L, U), phi, m, 1e-30)
+    G8 = oracle.hisq_outer(L, psi)
+    GU = oracle.hisq_smear_deriv(L, U, 0.5 * G8[0::2], 0.5 * G8[1::2])
+    dU = _crand(rng, 4, V, 3, 3)
+    fd = fd9(S, 3e-5)       # a first-level link has condition number ~430 here
+    assert abs(redot(GU, dU) - fd) < 1e-9 * abs(fd)
+
+
+def test_trivial_smearing_reproduces_the_plain_force(oracle):
+    """With fl = projectU(U) = U and no Naik term the chain rule through the
+    projection changes only the Hermitian part, which TAH removes: TAH(2 dS/dU U^+)
+    equals the plain fermion force."""
+    L = (4, 4, 4, 4)
+    U = oracle.phased(L, oracle.gauge_random(L, SEED + 10))
+    psi = oracle.source_gauss(L, 6, 0, 2)
+    one = oracle.fat7_coefs(1.0)
+    G8 = oracle.hisq_outer(L, psi)
+    GU = oracle.hisq_smear_deriv(L, U, 0.5 * G8[0::2], 0.5 * G8[1::2], coefs=(one, one, 0.0))
+    T = 2 * GU @ dag(U)
+    F = 0.5 * (T - dag(T))
+    F -= np.trace(F, axis1=-2, axis2=-1)[..., None, None] / 3 * np.eye(3)
+    assert rel(F, oracle.ferm_force(L, U, psi)) < 1e-12
