@@ -224,6 +224,29 @@ int qexhip_project_u(long n, double *u_dev, const double *x_dev);
    phased, BC-applied gauge field (tests/examples/testStagProp.nim:24-33). */
 int qexhip_hisq_smear(qexhip_lattice *lat, const qexhip_hisq_coefs *c, const double *g_dev, double *fl_dev,
                       double *ll_dev);
+/* ---- HISQ smearing chain rule: derivatives with respect to the complex link
+   entries in QEX's pairing redot(A, B) = Re tr(A^+ B); outputs accumulate ---- */
+/* fat7lDeriv, gauge/fat7lderiv.nim:122-475: deriv += dS/dgauge given mid = dS/dfl
+   for fl = makeImpLinks(gauge, coef); when llderiv and llmid are not NULL and
+   naik != 0, llderiv += dS/dllgauge given llmid = dS/dll (llgauge = gauge when
+   NULL; llderiv may be deriv) */
+int qexhip_fat7l_deriv(qexhip_lattice *lat, double *deriv_dev, const double *gauge_dev, const double *mid_dev,
+                       const qexhip_fat7_coefs *coef, double *llderiv_dev, const double *llgauge_dev,
+                       const double *llmid_dev, double naik);
+/* projectUderiv, maths/matrixFunctions.nim:361-392: r = dS/dx for u = projectU(x),
+   chain = dS/du, on n 3x3 matrices */
+int qexhip_project_u_deriv(long n, double *r_dev, const double *u_dev, const double *x_dev, const double *chain_dev);
+/* smearGetForce's returned proc, gauge/hisqsmear.nim:54-91: dg += dS/dg given
+   dfl = dS/dfl and dll = dS/dll for (fl, ll) = smear(g); c = NULL: defaults */
+int qexhip_hisq_smear_deriv(qexhip_lattice *lat, const qexhip_hisq_coefs *c, const double *g_dev,
+                            const double *dfl_dev, const double *dll_dev, double *dg_dev);
+/* HISQ fermion force, hmc/hmcAction.nim:851-876 with fermForce's smearing chain
+   (gauge/hisqsmear.nim:54-91): F_mu(x) = TAH(2 dS/dg_mu(x) g_mu(x)^+) for
+   S = phi^+ (m^2 - D_eo D_oe)^-1 phi, psi from qexhip_force_solve on the HISQ
+   links l = newStag3(smear(g)), g the phased BC-applied gauge field.  Same
+   convention as qexhip_force: for g -> exp(wX)g, dS/dw = -1/2 Re tr(X F). */
+int qexhip_hisq_force(qexhip_links *l, const qexhip_hisq_coefs *c, const double *g_dev, const qexhip_field *psi,
+                      double *F_dev);
 /* timing: ms[0] per smear with default coefficients; ms[1..3] k_fat3, k_fat5,
    k_fatf of the first level, ms[4..6] of the second */
 int qexhip_time_hisq_smear(qexhip_lattice *lat, const double *g_dev, double *fl_dev, double *ll_dev, int nrep,

@@ -9,4 +9,5 @@ from .stag import (D, Ddag, Field, Layout, SloppyType, SolverBackend, SolverPara
                    stagD, stagD2, stagD2ee, stagD2oo, stagForceSolve, time_hop, autotune_report, set_autotune,
                    set_cg_form, set_cg_graph, stagD2multi, stagDN, time_hop_multi,
                    stream_read_gbs)
-from .hisq import Fat7lCoefs, HisqCoefs, hisqStag, makeImpLinks, newHISQ, projectU  # noqa: F401,E402
+from .hisq import (Fat7lCoefs, HisqCoefs, fat7lDeriv, hisqForce, hisqStag, makeImpLinks, newHISQ,  # noqa: F401,E402
+                   projectU, projectUderiv, smearGetForce)

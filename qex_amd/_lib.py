@@ -30,7 +30,8 @@ EXPORTS = [
     "qexhip_set_cg_form", "qexhip_stagD2_multi", "qexhip_time_hop_multi", "qexhip_stream_read",
     "qexhip_set_cg_graph",
     "qexhip_hisq_coefs_init", "qexhip_make_imp_links", "qexhip_project_u", "qexhip_hisq_smear",
-    "qexhip_time_hisq_smear",
+    "qexhip_time_hisq_smear", "qexhip_fat7l_deriv", "qexhip_project_u_deriv", "qexhip_hisq_smear_deriv",
+    "qexhip_hisq_force",
 ]
 
 # qexhip_solve_batch / qexhip_solve_multishift_batch flags
@@ -114,6 +115,10 @@ def load(path=LIB_PATH):
         "qexhip_project_u": [ctypes.c_long, vp, vp],
         "qexhip_hisq_smear": [vp, ctypes.POINTER(HisqCoefsC), vp, vp, vp],
         "qexhip_time_hisq_smear": [vp, vp, vp, vp, I, dp],
+        "qexhip_fat7l_deriv": [vp, vp, vp, vp, ctypes.POINTER(Fat7CoefsC), vp, vp, vp, D],
+        "qexhip_project_u_deriv": [ctypes.c_long, vp, vp, vp, vp],
+        "qexhip_hisq_smear_deriv": [vp, ctypes.POINTER(HisqCoefsC), vp, vp, vp, vp],
+        "qexhip_hisq_force": [vp, ctypes.POINTER(HisqCoefsC), vp, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -1583,15 +1583,15 @@ int qexhip_force(qexhip_links *l, const qexhip_field *psi, double *F, int F_on_d
   API_BEGIN
   check_init();
   check_same_lat(l, psi);
-  require(l->nl == 4, "qexhip_force: plain (nl=4) links required");
+  require(l->nl == 4 || outer_only, "qexhip_force: HISQ (nl=8) links give outer products only (outer_only=1)");
   require(psi->prec == P64, "force fields must be FP64");
   require(F != nullptr, "null output");
   qexhip_lattice *lat = l->lat;
-  const size_t bytes = (size_t)4 * lat->V * 18 * sizeof(double);
+  const size_t bytes = (size_t)l->nl * lat->V * 18 * sizeof(double);
   double *dF = F;
   if (!F_on_device) dF = dalloc<double>(bytes);
   launch_force(lat->g, links_store(l, P64, 18), lkblk_bytes(l, P64, 18), (const double *)psi->par(0),
-               (const double *)psi->par(1), dF, outer_only, g_stream);
+               (const double *)psi->par(1), dF, outer_only, g_stream, l->nl);
   check_launch();
   if (!F_on_device) {
     HIPCHK(hipMemcpyAsync(F, dF, bytes, hipMemcpyDeviceToHost, g_stream));
@@ -1745,6 +1745,87 @@ int qexhip_hisq_smear(qexhip_lattice *lat, const qexhip_hisq_coefs *c, const dou
   fat7_array(&c->fat7first, c1);
   fat7_array(&c->fat7second, c2);
   launch_hisq_smear(lat->g, lat->smear, c1, c2, c->naik, g, fl, ll, g_remap, g_stream, nullptr);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+int qexhip_fat7l_deriv(qexhip_lattice *lat, double *deriv, const double *gauge, const double *mid,
+                       const qexhip_fat7_coefs *coef, double *llderiv, const double *llgauge, const double *llmid,
+                       double naik) {
+  API_BEGIN
+  check_init();
+  require(lat && deriv && gauge && mid && coef, "null argument");
+  require(deriv != gauge && deriv != mid, "deriv must not alias gauge or mid");
+  if (llderiv && llmid && naik != 0.0)
+    for (int d = 0; d < 4; d++) require(lat->g.L[d] >= 3, "long links need extents >= 3");
+  double c[5];
+  fat7_array(coef, c);
+  launch_fat7l_deriv(lat->g, lat->smear, deriv, gauge, mid, c, llderiv, llgauge, llmid, naik, g_remap, g_stream);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+int qexhip_project_u_deriv(long n, double *r, const double *u, const double *x, const double *chain) {
+  API_BEGIN
+  check_init();
+  require(r && u && x && chain && n >= 0, "bad arguments");
+  if (n > 0) launch_project_u_deriv(n, r, u, x, chain, g_stream);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+static void hisq_arrays(const qexhip_hisq_coefs *c, double c1[5], double c2[5], double *naik) {
+  qexhip_hisq_coefs def;
+  if (!c) {
+    qexhip_hisq_coefs_init(&def);
+    c = &def;
+  }
+  fat7_array(&c->fat7first, c1);
+  fat7_array(&c->fat7second, c2);
+  *naik = c->naik;
+}
+
+int qexhip_hisq_smear_deriv(qexhip_lattice *lat, const qexhip_hisq_coefs *c, const double *g, const double *dfl,
+                            const double *dll, double *dg) {
+  API_BEGIN
+  check_init();
+  require(lat && g && dfl && dll && dg, "null argument");
+  require(dg != g && dg != dfl && dg != dll, "dg must not alias the inputs");
+  for (int d = 0; d < 4; d++) require(lat->g.L[d] >= 3, "HISQ links need extents >= 3");
+  double c1[5], c2[5], naik;
+  hisq_arrays(c, c1, c2, &naik);
+  launch_hisq_smear_deriv(lat->g, lat->smear, c1, c2, naik, g, dfl, dll, lat->V, dg, 1, g_remap, g_stream);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+int qexhip_hisq_force(qexhip_links *l, const qexhip_hisq_coefs *c, const double *g, const qexhip_field *psi,
+                      double *F) {
+  API_BEGIN
+  check_init();
+  check_same_lat(l, psi);
+  require(l->nl == 8, "qexhip_hisq_force: HISQ (nl=8) links required");
+  require(psi->prec == P64, "force fields must be FP64");
+  require(g && F && F != g, "bad arguments");
+  qexhip_lattice *lat = l->lat;
+  double c1[5], c2[5], naik;
+  hisq_arrays(c, c1, c2, &naik);
+  // eps psi psi(x+mu)^+ at [2mu], eps psi psi(x+3mu)^+ at [2mu+1]: twice dS/d(fat, long)
+  double *outer = dalloc<double>((size_t)8 * lat->V * 18 * sizeof(double));
+  struct Free {
+    double *p;
+    ~Free() { (void)hipFree(p); }
+  } guard{outer};
+  launch_force(lat->g, links_store(l, P64, 18), lkblk_bytes(l, P64, 18), (const double *)psi->par(0),
+               (const double *)psi->par(1), outer, 1, g_stream, 8);
+  // F = 2 dS/dg, through both smearing levels; then F <- TAH(F g^+)
+  launch_hisq_smear_deriv(lat->g, lat->smear, c1, c2, naik, g, outer, outer + (size_t)lat->V * 18, 2 * lat->V, F, 0,
+                          g_remap, g_stream);
+  launch_tah_mul(lat->V, F, F, g, g_stream);
   check_launch();
   HIPCHK(hipStreamSynchronize(g_stream));
   API_END

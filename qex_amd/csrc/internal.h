@@ -170,7 +170,7 @@ void launch_cgm_xr(const Geom &g, int nshift, double *r, const double *Ap, doubl
 void launch_cgm_p(const Geom &g, int nshift, const double *r, double **p, ShiftScal *T, hipStream_t st);
 void launch_cgm_alpha(ShiftScal *T, Scal *S, hipStream_t st);
 void launch_force(const Geom &g, const char *links64, long lkblk, const double *psi_e, const double *psi_o,
-                  double *F, int outer_only, hipStream_t st);
+                  double *F, int outer_only, hipStream_t st, int nl = 4);
 
 // batch.hip
 void launch_bcg_iter(const BatchItem *T, int n, long nblocks, int nl, int block, hipStream_t st, int form);
@@ -192,6 +192,7 @@ void launch_link_update(const Geom &g, const double *P, double eps, double *U, h
 struct SmearWs {
   double2 *buf = nullptr;
   long V = 0;
+  int nfields = 0;
   SmearWs() = default;
   SmearWs(const SmearWs &) = delete;
   SmearWs &operator=(const SmearWs &) = delete;
@@ -206,6 +207,17 @@ void launch_make_imp_links(const Geom &g, SmearWs &w, double *fl, const double *
 void launch_project_u(long n, double *u, const double *x, hipStream_t st);
 void launch_hisq_smear(const Geom &g, SmearWs &w, const double first[5], const double second[5], double naik,
                        const double *gin, double *fl, double *ll, int remap, hipStream_t st, SmearTimes *tm);
+// chain rule (derivatives in the pairing Re tr(A^+ B)); deriv/llderiv/dg accumulate
+void launch_fat7l_deriv(const Geom &g, SmearWs &w, double *deriv, const double *gf, const double *mid,
+                        const double coef[5], double *llderiv, const double *gl, const double *llmid, double naik,
+                        int remap, hipStream_t st);
+void launch_project_u_deriv(long n, double *r, const double *u, const double *x, const double *chain, hipStream_t st);
+// dfl/dll: 4 fields each, element (mu, site) at base + (mu*mstride + site)*18 doubles
+void launch_hisq_smear_deriv(const Geom &g, SmearWs &w, const double first[5], const double second[5], double naik,
+                             const double *gin, const double *dfl, const double *dll, long mstride, double *dg,
+                             int acc, int remap, hipStream_t st);
+// F_mu(x) = TAH(D_mu(x) U_mu(x)^+), in place on F = D allowed
+void launch_tah_mul(long V, double *F, const double *D, const double *U, hipStream_t st);
 
 // gen.hip
 void launch_gauge_random(const Geom &g, uint64_t seed, double *canon, hipStream_t st);

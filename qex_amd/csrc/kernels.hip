@@ -335,16 +335,19 @@ void launch_cgm_p(const Geom &g, int nshift, const double *r, double **p, ShiftS
 // F_mu(x) = TAH(eps(x) psi(x) psi(x+mu)^+ U_mu(x)^+)   (hmc/hmcAction.nim:735-790,
 // unsmeared; TAH = maths/matrixFunctions.nim:414-419).  One thread per
 // (canonical site, mu); output in QEX canonical order [mu][site][3][3].
+// nl = 8 (HISQ links, outer products only): output k = 2mu holds the 1-hop
+// outer product, k = 2mu+1 the 3-hop one (both link x to a site of the other parity)
 __global__ void __launch_bounds__(256) k_force(Geom g, const char *links, long lkblk, const double2 *pe,
-                                               const double2 *po, double2 *F, int outer_only) {
+                                               const double2 *po, double2 *F, int outer_only, int nl) {
   const long V = 2 * g.Vh;
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int mu = blockIdx.y;
+  const int k = blockIdx.y;
+  const int mu = nl == 8 ? k >> 1 : k;
   if (idx >= V) return;
   const int par = idx >= g.Vh;
   const long i = idx - par * g.Vh;
   const Site s = decode(g, i, par);
-  const long j = nbr(g, s, mu, 1);
+  const long j = nbr(g, s, mu, (nl == 8 && (k & 1)) ? 3 : 1);
   const double2 *ps = par ? po : pe, *pn = par ? pe : po;
   const double eps = par ? -1.0 : 1.0;
   double2 a[3], b[3];
@@ -360,7 +363,7 @@ __global__ void __launch_bounds__(256) k_force(Geom g, const char *links, long l
 #pragma unroll
     for (int c = 0; c < 3; c++)  // a_r conj(b_c)
       G[3 * r + c] = make_double2(a[r].x * b[c].x + a[r].y * b[c].y, a[r].y * b[c].x - a[r].x * b[c].y);
-  double2 *out = F + ((long)mu * V + idx) * 9;
+  double2 *out = F + ((long)k * V + idx) * 9;
   if (outer_only) {
 #pragma unroll
     for (int e = 0; e < 9; e++) out[e] = G[e];
@@ -399,10 +402,10 @@ __global__ void __launch_bounds__(256) k_force(Geom g, const char *links, long l
 }
 
 void launch_force(const Geom &g, const char *links64, long lkblk, const double *psi_e, const double *psi_o,
-                  double *F, int outer_only, hipStream_t st) {
-  dim3 grid(nblk(2 * g.Vh, 256), 4);
+                  double *F, int outer_only, hipStream_t st, int nl) {
+  dim3 grid(nblk(2 * g.Vh, 256), nl);
   hipLaunchKernelGGL(k_force, grid, dim3(256), 0, st, g, links64, lkblk, (const double2 *)psi_e,
-                     (const double2 *)psi_o, (double2 *)F, outer_only);
+                     (const double2 *)psi_o, (double2 *)F, outer_only, nl);
 }
 
 }  // namespace qh

@@ -119,6 +119,62 @@ def hisqStag(lo, g, coefs=None):
     return newStag3(lo, fl, ll), fl, ll
 
 
+def fat7lDeriv(lo, deriv, gauge, mid, coef, llderiv=None, llgauge=None, llmid=None, naik=0.0):
+    """fat7lDeriv (gauge/fat7lderiv.nim:122-475): deriv += dS/dgauge given
+    mid = dS/dfl; llderiv += dS/dllgauge given llmid = dS/dll.  Derivatives
+    are with respect to the complex link entries in QEX's pairing
+    redot(A, B) = Re tr(A^+ B)."""
+    for t in (deriv, gauge, mid) + tuple(t for t in (llderiv, llgauge, llmid) if t is not None):
+        _check(lo, t)
+    c = coef._c()
+    torch.cuda.synchronize()
+    opt = lambda t: _p(t) if t is not None else None  # noqa: E731
+    call("qexhip_fat7l_deriv", lo.h, _p(deriv), _p(gauge), _p(mid), ctypes.byref(c), opt(llderiv), opt(llgauge),
+         opt(llmid), float(naik))
+    return deriv
+
+
+def projectUderiv(u, x, chain, out=None):
+    """projectUderiv (maths/matrixFunctions.nim:361-392): dS/dx for u = projectU(x)."""
+    for t in (u, x, chain):
+        if not (t.is_cuda and t.dtype == torch.complex128 and t.is_contiguous() and t.shape == x.shape):
+            raise ValueError("projectUderiv takes contiguous complex128 CUDA tensors of one shape")
+    out = torch.empty_like(x) if out is None else out
+    torch.cuda.synchronize()
+    call("qexhip_project_u_deriv", ctypes.c_long(x.numel() // 9), _p(out), _p(u), _p(x), _p(chain))
+    return out
+
+
+def smearGetForce(lo, coefs, u):
+    """HisqCoefs.smearGetForce (gauge/hisqsmear.nim:54-91): smear u and return
+    (fl, ll, f) with f(dsdu, dsdfl, dsdll) adding dS/du to dsdu."""
+    fl, ll = coefs.smear(lo, u)
+    c = coefs._c()
+
+    def smearedForce(dsdu, dsdfl, dsdll):
+        for t in (dsdu, dsdfl, dsdll):
+            _check(lo, t)
+        torch.cuda.synchronize()
+        call("qexhip_hisq_smear_deriv", lo.h, ctypes.byref(c), _p(u), _p(dsdfl), _p(dsdll), _p(dsdu))
+        return dsdu
+
+    return fl, ll, smearedForce
+
+
+def hisqForce(s, coefs, g, psi, F=None):
+    """HISQ fermion force (hmc/hmcAction.nim:851-876): F_mu(x) =
+    TAH(2 dS/dg_mu(x) g_mu(x)^+) for the operator s = newStag3(smear(g)) and
+    psi from stagForceSolve on s; dS/dw = -1/2 Re tr(X F) for g -> exp(wX) g."""
+    lo = s.l
+    _check(lo, g)
+    F = torch.empty_like(g) if F is None else F
+    _check(lo, F)
+    c = (coefs or newHISQ())._c()
+    torch.cuda.synchronize()
+    call("qexhip_hisq_force", s.h, ctypes.byref(c), _p(g), psi.h, _p(F))
+    return F
+
+
 def time_smear(lo, g, nrep=10):
     """Milliseconds per HISQ smear with default coefficients, and per kernel
     (k_fat3, k_fat5, k_fatf of the first, then the second level)."""

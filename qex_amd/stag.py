@@ -618,13 +618,16 @@ def stagForceSolve(s, psi, phi, mass, sp):
 def fermForce(s, psi, out=None, outer_only=False, device=None):
     """F_mu(x) = TAH(eps(x) psi(x) psi(x+mu)^+ U_mu(x)^+) (hmcAction.nim:723-792, unsmeared).
 
-    Returns a canonical (4, V, 3, 3) complex128 array (torch on `device` if given)."""
+    Returns a canonical (4, V, 3, 3) complex128 array (torch on `device` if given).
+    With outer_only the outer products eps psi psi(x+mu)^+ are returned; for
+    HISQ links (8 fields) as [2mu] = 1-hop and [2mu+1] = 3-hop (x+3mu)."""
     V = s.l.nSites
+    nk = s.nl if outer_only else 4
     if out is None:
         if device is not None:
-            out = torch.zeros((4, V, 3, 3), dtype=torch.complex128, device=device)
+            out = torch.zeros((nk, V, 3, 3), dtype=torch.complex128, device=device)
         else:
-            out = np.zeros((4, V, 3, 3), dtype=np.complex128)
+            out = np.zeros((nk, V, 3, 3), dtype=np.complex128)
     p, dev = _ptr(out)
     call("qexhip_force", s.h, psi.h, p, dev, int(outer_only))
     return out
