@@ -276,6 +276,21 @@ def hisq_links_study(cpu=True):
                 "GB_s": round(gbs, 1), "TFLOP_s": round(fl * V / t / 1e12, 2),
                 "frac_hbm_peak": round(gbs / PEAK_GBS, 4),
                 "frac_fp64_peak": round(fl * V / t / 1e12 / PEAK_FP64_TFLOPS, 4)}
+    # chain rule: dS/dU of the two-level smearing for given dS/dfl, dS/dll
+    # (forward smear recomputed inside), best of 3 after a warmup call
+    fl, ll, force = hisq.smearGetForce(lo, hisq.newHISQ(), U)
+    dU = torch.zeros_like(U)
+    force(dU, fl, ll)
+    best = None
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        force(dU, fl, ll)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    res["ms_per_smear_force"] = round(best * 1e3, 3)
+    del fl, ll, dU
     del U
     torch.cuda.empty_cache()
     if cpu:

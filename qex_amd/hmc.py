@@ -1,4 +1,5 @@
-"""Plain-staggered HMC with a Wilson gauge action, on the GPU through the C ABI.
+"""Staggered HMC (plain or HISQ fermions) with a Wilson gauge action, on the GPU
+through the C ABI.
 
 Mirrors QEX's staggered HMC (src/examples/staghmc.nim:81-353, and the action
 pieces of src/hmc/hmcAction.nim:794-885):
@@ -24,6 +25,7 @@ import numpy as np
 import torch
 
 from . import gauge as G
+from . import hisq as H
 from . import stag as Q
 from ._lib import call
 
@@ -69,8 +71,12 @@ def kinetic(P):
 
 
 class StaggeredHMC:
+    """Staggered HMC; hisq=True smears the phased links into HISQ fat and long
+    links (physics/hisqLinks.nim:39-51) for the fermion operator and chains the
+    fermion force through the smearing (gauge/hisqsmear.nim:54-91)."""
+
     def __init__(self, lo, U, beta, mass, r2req_force=1e-20, r2req_action=1e-24, sloppy=Q.SloppyType.SloppyNone,
-                 integrator="leapfrog", lam=OMELYAN_LAMBDA, sloppy_links=None, recon=18):
+                 integrator="leapfrog", lam=OMELYAN_LAMBDA, sloppy_links=None, recon=18, hisq=False):
         if integrator not in ("leapfrog", "2MN"):
             raise ValueError("integrator must be 'leapfrog' or '2MN'")
         self.integrator, self.lam = integrator, float(lam)
@@ -80,7 +86,8 @@ class StaggeredHMC:
         self.mass = float(mass)
         self.Uph = self.U.clone()
         G.phase_bc(lo, self.Uph)
-        self.s = Q.Staggered(lo, self.Uph)
+        self.hisq = H.newHISQ() if hisq else None
+        self.s = Q.Staggered(lo, self._fermion_links())
         self.phi = lo.ColorVector()
         self.psi = lo.ColorVector()
         self.y = lo.ColorVector()
@@ -96,10 +103,16 @@ class StaggeredHMC:
         sp.sloppySolve, sp.sloppyLinks, sp.recon = self.sloppy, self.sloppy_links, self.recon
         return sp
 
+    def _fermion_links(self):
+        if self.hisq is None:
+            return self.Uph
+        fl, ll = self.hisq.smear(self.lo, self.Uph)
+        return torch.stack([t for mu in range(4) for t in (fl[mu], ll[mu])]).contiguous()
+
     def _links_changed(self):
         self.Uph.copy_(self.U)
         G.phase_bc(self.lo, self.Uph)
-        self.s.update(self.Uph)
+        self.s.update(self._fermion_links())
 
     def heatbath(self, seed):
         """phi_e = (D^+(m) eta)_e (hmcAction.nim:794-818)."""
@@ -121,7 +134,11 @@ class StaggeredHMC:
         sp = self._sp(self.r2f)
         Q.stagForceSolve(self.s, self.psi, self.phi, self.mass, sp)
         self.solver_its += sp.iterations
-        Q.fermForce(self.s, self.psi, out=self.Ff)
+        if self.hisq is None:
+            Q.fermForce(self.s, self.psi, out=self.Ff)
+        else:
+            # force with respect to the phased links; the staggered signs drop out of TAH(2 dS/dU U^+)
+            H.hisqForce(self.s, self.hisq, self.Uph, self.psi, F=self.Ff)
         wilson_force(self.lo, self.U, self.beta, out=self.Fg)
         return -0.5 * (self.Fg + self.Ff)
 
@@ -376,6 +393,7 @@ def main(argv=None):
     ap.add_argument("--reversibility", action="store_true")
     ap.add_argument("--integrator", default="leapfrog", help="leapfrog or 2MN[,lambda]")
     ap.add_argument("--warmup", type=int, default=0, help="trajectories without Metropolis first")
+    ap.add_argument("--hisq", action="store_true", help="HISQ fermions: smeared fat + Naik links")
     args = ap.parse_args(argv)
 
     from ._lib import init
@@ -394,7 +412,7 @@ def main(argv=None):
                      r2req_action=args.arsq, sloppy=Q.SloppyType(args.sloppy), integrator=name,
                      lam=float(rest[0]) if rest else OMELYAN_LAMBDA,
                      sloppy_links=None if args.sloppy_links < 0 else Q.SloppyType(args.sloppy_links),
-                     recon=args.recon)
+                     recon=args.recon, hisq=args.hisq)
     rng = np.random.default_rng(args.seed)
     for n in range(args.warmup + args.trajs):
         its0 = h.solver_its

@@ -181,3 +181,65 @@ def test_hisq_force_against_action_differences(oracle, gpu):
     print(f"HISQ force: analytic {analytic:.12e} fd {fd:.12e}", flush=True)
     # measured agreement 2.4e-10; the north-star bar for the force is 1e-8
     assert abs(fd - analytic) < 1e-8 * abs(analytic)
+
+
+# ------------------------------------------------------------- HISQ HMC
+def _hisq_hmc(gpu, integrator="leapfrog", mass=0.2):
+    from qex_amd import gauge, hmc
+    lo = gpu.Layout((8, 8, 8, 8))
+    h = hmc.StaggeredHMC(lo, gauge.unit_gauge(lo, device="cuda"), 5.6, mass, r2req_force=1e-26,
+                         r2req_action=1e-26, integrator=integrator, hisq=True)
+    # off the unit configuration, so that the smearing is not trivial
+    hmc.link_update(lo, hmc.random_momenta(lo, 3), 0.1, h.U)
+    h._links_changed()
+    return h
+
+
+def test_hisq_hmc_dH_scales_as_dt2(gpu):
+    from qex_amd import hmc
+    dH = []
+    for n in (10, 20, 40):
+        h = _hisq_hmc(gpu)
+        P = hmc.random_momenta(h.lo, 5)
+        h.heatbath(6)
+        H0 = h.hamiltonian(P)
+        P = h.integrate(P, 0.5, n)
+        dH.append(h.hamiltonian(P) - H0)
+    print(f"HISQ HMC dH {dH}", flush=True)
+    assert abs(dH[-1]) > 1e-6, dH
+    assert 3.5 < dH[1] / dH[2] < 4.5, dH
+
+
+def test_hisq_hmc_reversible(gpu):
+    h = _hisq_hmc(gpu, integrator="2MN")
+    dU, dP = h.reversibility(11, tau=0.5, nsteps=8)
+    assert dU < 1e-10 and dP < 1e-9, (dU, dP)
+
+
+def test_hisq_md_force_matches_action_difference(gpu):
+    """-2 Pdot of the HISQ HMC (gauge + smeared fermion force) is dS/dw along
+    U_mu(x) -> exp(w X) U_mu(x) for single links."""
+    import scipy.linalg
+    import torch
+    h = _hisq_hmc(gpu, mass=0.3)
+    h.heatbath(4)
+    F = (-2.0 * h.pdot()).cpu().numpy()
+    rng = np.random.default_rng(2)
+    U0 = h.U.clone()
+    step = 1e-4
+    for trial in range(3):
+        mu, site = int(rng.integers(4)), int(rng.integers(h.lo.nSites))
+        A = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        X = 0.5 * (A - A.conj().T)
+        X -= np.trace(X) / 3 * np.eye(3)
+        S = []
+        for w in (step, -step):
+            h.U.copy_(U0)
+            h.U[mu, site] = torch.from_numpy(scipy.linalg.expm(w * X)).cuda() @ U0[mu, site]
+            h._links_changed()
+            S.append(h.gauge_action() + h.fermion_action())
+        fd = (S[0] - S[1]) / (2 * step)
+        an = -0.5 * np.trace(X @ F[mu, site]).real
+        assert abs(fd - an) < 1e-5 * max(1.0, abs(an)), (trial, fd, an)
+    h.U.copy_(U0)
+    h._links_changed()
