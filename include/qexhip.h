@@ -252,6 +252,27 @@ int qexhip_hisq_force(qexhip_links *l, const qexhip_hisq_coefs *c, const double 
 int qexhip_time_hisq_smear(qexhip_lattice *lat, const double *g_dev, double *fl_dev, double *ll_dev, int nrep,
                            double *ms);
 
+/* ---- MILC binary gauge files (src/new/milcIO.nim, src/new/milcFileField.nim):
+   96-byte header (magic 20103, dims, 64-char time stamp, order, sum29/sum31
+   checksums), then per site in lexicographic order the four links as 3x3
+   complex float32.  Links on the device are canonical FP64 [mu][site][3][3]. */
+typedef struct {
+  int dims[4];
+  char timestamp[65];
+  int order;
+  int byteswapped;   /* file written with the other byte order */
+  uint32_t sum29, sum31;
+  int checksum_ok;   /* set by qexhip_milc_read */
+} qexhip_milc_info;
+/* header only (MilcHeader, milcIO.nim:9-25) */
+int qexhip_milc_read_info(const char *path, qexhip_milc_info *info);
+/* openMilcFileGaugeField (milcFileField.nim:46-57) + conversion to the device
+   layout; fails on a dimension or checksum mismatch (info filled either way) */
+int qexhip_milc_read(qexhip_lattice *lat, const char *path, double *U_dev, qexhip_milc_info *info);
+/* newMilcFileGaugeField + close (milcFileField.nim:30-37,58-69): writes single
+   precision with the checksum; timestamp may be NULL */
+int qexhip_milc_write(qexhip_lattice *lat, const char *path, const double *U_dev, const char *timestamp);
+
 int qexhip_gen_gauge(qexhip_lattice *lat, uint64_t seed, double *U_dev);
 int qexhip_gen_source(qexhip_lattice *lat, uint64_t seed, int stream, int par, double *x_dev);
 int qexhip_gen_phase_bc(qexhip_lattice *lat, double *U_dev);

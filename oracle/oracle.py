@@ -69,6 +69,7 @@ def _declare(l):
         "qo_project_u_deriv": (None, [L, _dp, _dp, _dp, _dp]),
         "qo_hisq_smear_deriv": (None, [_ip, _dp, _dp, D, _dp, _dp, _dp, _dp]),
         "qo_hisq_outer": (None, [_ip, _dp, _dp]),
+        "qo_milc_checksum": (None, [ctypes.POINTER(U32), L, L, ctypes.POINTER(U32)]),
         "qo_su3_exp_i": (None, [_dp, D, _dp]),
         "qo_su3_exp": (None, [_dp, _dp]),
         "qo_mat_mul": (None, [_dp, _dp, _dp]),
@@ -366,3 +367,55 @@ def hisq_outer(L, psi):
     G = np.empty((8, volume(L), 3, 3), dtype=np.complex128)
     lib().qo_hisq_outer(_L(L), _p(psi), _p(G))
     return G
+
+
+# ------------------------------------------------------------ MILC gauge files
+MILC_MAGIC = 20103
+
+
+def milc_checksum(words, offset=0):
+    """(sum29, sum31) of uint32 data words (src/new/milcIO.nim:43-51)."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    out = (ctypes.c_uint32 * 2)()
+    lib().qo_milc_checksum(w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), w.size, int(offset), out)
+    return int(out[0]), int(out[1])
+
+
+def milc_write(path, L, U, stamp=b"oracle", byteswap=False):
+    """Write canonical links U (4, V, 3, 3) as a MILC file: header, then sites in
+    lexicographic order with U_0..U_3 as complex float32 (milcFileField.nim)."""
+    import struct
+    V = volume(L)
+    C = all_coords(L)                      # canonical index -> coords
+    lex = C[:, 0] + L[0] * (C[:, 1] + L[1] * (C[:, 2] + L[2] * C[:, 3]))
+    data = np.empty((V, 4, 3, 3), dtype=np.complex64)
+    data[lex] = np.transpose(U, (1, 0, 2, 3))
+    words = np.frombuffer(data.tobytes(), dtype=np.uint32)
+    s29, s31 = milc_checksum(words)
+    fmt = ">" if byteswap else "<"
+    hdr = struct.pack(fmt + "i4i64siII", MILC_MAGIC, *[int(v) for v in L], stamp.ljust(64, b"\0")[:64], 0, s29, s31)
+    body = words.byteswap() if byteswap else words
+    with open(path, "wb") as f:
+        f.write(hdr)
+        f.write(body.tobytes())
+    return s29, s31
+
+
+def milc_read(path):
+    """(L, U canonical complex128, (sum29, sum31) stored, (sum29, sum31) computed)."""
+    import struct
+    with open(path, "rb") as f:
+        raw = f.read()
+    fmt = "<"
+    if struct.unpack("<i", raw[:4])[0] != MILC_MAGIC:
+        fmt = ">"
+    magic, d0, d1, d2, d3, stamp, order, s29, s31 = struct.unpack(fmt + "i4i64siII", raw[:96])
+    assert magic == MILC_MAGIC and order == 0
+    L = (d0, d1, d2, d3)
+    V = volume(L)
+    words = np.frombuffer(raw[96:96 + V * 288], dtype=np.dtype(fmt + "u4")).astype(np.uint32)
+    data = np.frombuffer(words.tobytes(), dtype=np.complex64).reshape(V, 4, 3, 3)
+    C = all_coords(L)
+    lex = C[:, 0] + L[0] * (C[:, 1] + L[1] * (C[:, 2] + L[2] * C[:, 3]))
+    U = np.ascontiguousarray(np.transpose(data[lex], (1, 0, 2, 3)).astype(np.complex128))
+    return L, U, (s29, s31), milc_checksum(words)

@@ -1121,3 +1121,22 @@ void qo_hisq_outer(const int L[4], const double *psi, double *G) {
     }
   }
 }
+
+/* ----------------------------------------------------- MILC gauge-file checksum */
+/* src/new/milcIO.nim:43-51: sum29 ^= rotl(w_k, k mod 29), sum31 ^= rotl(w_k, k mod 31)
+   over the 32-bit data words w_k, k counted from offset.  The reference writes
+   the rotation as (t shl k) + (t shr (32-k)); at k = 0 that shifts by 32, whose
+   result depends on the platform, so this follows MILC's file format, which
+   defines the rotation (a plain t at k = 0). */
+void qo_milc_checksum(const uint32_t *w, long n, long offset, uint32_t out[2]) {
+  uint32_t a = 0, b = 0;
+  for (long i = 0; i < n; i++) {
+    const long k = offset + i;
+    const int k29 = (int)(k % 29), k31 = (int)(k % 31);
+    const uint32_t t = w[i];
+    a ^= k29 ? (t << k29) | (t >> (32 - k29)) : t;
+    b ^= k31 ? (t << k31) | (t >> (32 - k31)) : t;
+  }
+  out[0] = a;
+  out[1] = b;
+}

@@ -131,6 +131,9 @@ void qo_hisq_smear_deriv(const int L[4], const double first[5], const double sec
 /* HISQ force outer products: G[2mu] = eps psi psi(x+mu)^+, G[2mu+1] = eps psi psi(x+3mu)^+ */
 void qo_hisq_outer(const int L[4], const double *psi, double *G);
 
+/* ---- MILC gauge files (src/new/milcIO.nim) ---- */
+void qo_milc_checksum(const uint32_t *w, long n, long offset, uint32_t out[2]);
+
 /* ---- 3x3 helpers used by tests ---- */
 void qo_su3_exp_i(const double *H /*18, Hermitian*/, double s, double *U /*18*/);
 void qo_su3_exp(const double *A /*18, general*/, double *U /*18*/);

@@ -31,7 +31,7 @@ EXPORTS = [
     "qexhip_set_cg_graph",
     "qexhip_hisq_coefs_init", "qexhip_make_imp_links", "qexhip_project_u", "qexhip_hisq_smear",
     "qexhip_time_hisq_smear", "qexhip_fat7l_deriv", "qexhip_project_u_deriv", "qexhip_hisq_smear_deriv",
-    "qexhip_hisq_force",
+    "qexhip_hisq_force", "qexhip_milc_read_info", "qexhip_milc_read", "qexhip_milc_write",
 ]
 
 # qexhip_solve_batch / qexhip_solve_multishift_batch flags
@@ -63,6 +63,12 @@ class Fat7CoefsC(ctypes.Structure):
 
 class HisqCoefsC(ctypes.Structure):
     _fields_ = [("fat7first", Fat7CoefsC), ("fat7second", Fat7CoefsC), ("naik", ctypes.c_double)]
+
+
+class MilcInfoC(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int * 4), ("timestamp", ctypes.c_char * 65), ("order", ctypes.c_int),
+                ("byteswapped", ctypes.c_int), ("sum29", ctypes.c_uint32), ("sum31", ctypes.c_uint32),
+                ("checksum_ok", ctypes.c_int)]
 
 
 class QexHipError(RuntimeError):
@@ -119,6 +125,9 @@ def load(path=LIB_PATH):
         "qexhip_project_u_deriv": [ctypes.c_long, vp, vp, vp, vp],
         "qexhip_hisq_smear_deriv": [vp, ctypes.POINTER(HisqCoefsC), vp, vp, vp, vp],
         "qexhip_hisq_force": [vp, ctypes.POINTER(HisqCoefsC), vp, vp, vp],
+        "qexhip_milc_read_info": [ctypes.c_char_p, ctypes.POINTER(MilcInfoC)],
+        "qexhip_milc_read": [vp, ctypes.c_char_p, vp, ctypes.POINTER(MilcInfoC)],
+        "qexhip_milc_write": [vp, ctypes.c_char_p, vp, ctypes.c_char_p],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -1873,6 +1873,50 @@ int qexhip_time_hisq_smear(qexhip_lattice *lat, const double *g, double *fl, dou
   API_END
 }
 
+static void milc_info_out(const MilcInfo &m, qexhip_milc_info *info) {
+  if (!info) return;
+  for (int d = 0; d < 4; d++) info->dims[d] = m.dims[d];
+  memcpy(info->timestamp, m.stamp, sizeof info->timestamp);
+  info->order = m.order;
+  info->byteswapped = m.byteswapped;
+  info->sum29 = m.sum29;
+  info->sum31 = m.sum31;
+  info->checksum_ok = m.checksum_ok;
+}
+
+int qexhip_milc_read_info(const char *path, qexhip_milc_info *info) {
+  API_BEGIN
+  require(path && info, "null argument");
+  MilcInfo m{};
+  milc_read_header(path, &m);
+  m.checksum_ok = 0;
+  milc_info_out(m, info);
+  API_END
+}
+
+int qexhip_milc_read(qexhip_lattice *lat, const char *path, double *U, qexhip_milc_info *info) {
+  MilcInfo m{};
+  API_BEGIN
+  check_init();
+  require(lat && path && U, "null argument");
+  try {
+    milc_read(lat->g, path, U, &m, g_stream);
+  } catch (...) {
+    milc_info_out(m, info);
+    throw;
+  }
+  milc_info_out(m, info);
+  API_END
+}
+
+int qexhip_milc_write(qexhip_lattice *lat, const char *path, const double *U, const char *timestamp) {
+  API_BEGIN
+  check_init();
+  require(lat && path && U, "null argument");
+  milc_write(lat->g, path, U, timestamp, g_stream);
+  API_END
+}
+
 int qexhip_links_recon_ok(qexhip_links *l, int *ok) {
   API_BEGIN
   check_init();

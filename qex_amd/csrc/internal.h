@@ -219,6 +219,19 @@ void launch_hisq_smear_deriv(const Geom &g, SmearWs &w, const double first[5], c
 // F_mu(x) = TAH(D_mu(x) U_mu(x)^+), in place on F = D allowed
 void launch_tah_mul(long V, double *F, const double *D, const double *U, hipStream_t st);
 
+// milcio.hip: MILC binary gauge files (single precision, site order)
+struct MilcInfo {
+  int dims[4];
+  char stamp[65];
+  int order, byteswapped;
+  uint32_t sum29, sum31;
+  int checksum_ok;
+};
+void milc_checksum(const uint32_t *w, size_t n, uint32_t *s29, uint32_t *s31);
+void milc_read_header(const char *path, MilcInfo *info);
+void milc_read(const Geom &g, const char *path, double *U, MilcInfo *info, hipStream_t st);
+void milc_write(const Geom &g, const char *path, const double *U, const char *stamp, hipStream_t st);
+
 // gen.hip
 void launch_gauge_random(const Geom &g, uint64_t seed, double *canon, hipStream_t st);
 void launch_source_gauss(const Geom &g, uint64_t seed, int stream, int par, double *canon, hipStream_t st);

@@ -1,4 +1,5 @@
-"""Synthetic inputs generated on the GPU by the qexhip library.
+"""Gauge fields on the GPU through the qexhip library: synthetic inputs and MILC
+binary gauge files.
 
 Philox4x32-10 keyed by (seed, lexicographic site, stream, component) so the
 fields do not depend on the layout; U = exp(i*0.5*H) with H traceless
@@ -54,3 +55,39 @@ def source(lo, seed, stream=0, par=0, device=None):
     torch.cuda.synchronize()
     call("qexhip_gen_source", lo.h, ctypes.c_uint64(seed), int(stream), int(par), ctypes.c_void_p(x.data_ptr()))
     return x
+
+
+def milcHeader(path):
+    """Header of a MILC binary gauge file (src/new/milcIO.nim:9-25) as a dict."""
+    from ._lib import MilcInfoC
+    info = MilcInfoC()
+    call("qexhip_milc_read_info", str(path).encode(), ctypes.byref(info))
+    return _info(info)
+
+
+def _info(info):
+    return {"dims": tuple(info.dims), "timestamp": info.timestamp.decode(errors="replace"), "order": info.order,
+            "byteswapped": bool(info.byteswapped), "sum29": info.sum29, "sum31": info.sum31,
+            "checksum_ok": bool(info.checksum_ok)}
+
+
+def readMilc(lo, path, device=None):
+    """Read a MILC gauge file (openMilcFileGaugeField, src/new/milcFileField.nim:46-57)
+    into canonical (4, V, 3, 3) complex128 links on the GPU; the checksum is verified.
+    Returns (U, header dict)."""
+    from ._lib import MilcInfoC
+    U = torch.zeros((4, lo.nSites, 3, 3), dtype=torch.complex128, device=_dev(device))
+    info = MilcInfoC()
+    torch.cuda.synchronize()
+    call("qexhip_milc_read", lo.h, str(path).encode(), ctypes.c_void_p(U.data_ptr()), ctypes.byref(info))
+    return U, _info(info)
+
+
+def writeMilc(lo, path, U, timestamp=None):
+    """Write canonical links U as a MILC gauge file in single precision with its
+    checksum (newMilcFileGaugeField, src/new/milcFileField.nim:30-37,58-69)."""
+    if not (U.is_cuda and U.dtype == torch.complex128 and U.is_contiguous() and U.shape == (4, lo.nSites, 3, 3)):
+        raise ValueError("U must be a contiguous complex128 CUDA tensor of shape (4, nSites, 3, 3)")
+    torch.cuda.synchronize()
+    call("qexhip_milc_write", lo.h, str(path).encode(), ctypes.c_void_p(U.data_ptr()),
+         timestamp.encode() if timestamp else None)

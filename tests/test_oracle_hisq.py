@@ -300,3 +300,4 @@ def test_trivial_smearing_reproduces_the_plain_force(oracle):
     F = 0.5 * (T - dag(T))
     F -= np.trace(F, axis1=-2, axis2=-1)[..., None, None] / 3 * np.eye(3)
     assert rel(F, oracle.ferm_force(L, U, psi)) < 1e-12
+
