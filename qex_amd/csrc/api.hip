@@ -30,6 +30,7 @@ int g_remap = 1;
 bool g_block_forced = false;  // QEXHIP_BLOCK set: no autotuning
 bool g_autotune = true;
 bool g_cg_graph = true;       // QEXHIP_CG_GRAPH=0: launch CG chunks kernel by kernel
+bool g_smear_walk = true;     // QEXHIP_SMEAR_WALK=0: HISQ smearing kernels walk sites in canonical order
 
 #define HIPCHK(x)                                                                                  \
   do {                                                                                             \
@@ -1155,6 +1156,7 @@ int qexhip_init(int device) {
     if (v == 2 || v == 3) g_cg_form = v;
   }
   if (const char *r = getenv("QEXHIP_XCD_REMAP")) g_remap = atoi(r) != 0;
+  if (const char *w = getenv("QEXHIP_SMEAR_WALK")) g_smear_walk = atoi(w) != 0;
   API_END
 }
 
@@ -1714,7 +1716,7 @@ int qexhip_make_imp_links(qexhip_lattice *lat, double *fl, const double *gf, con
     for (int d = 0; d < 4; d++) require(lat->g.L[d] >= 3, "long links need extents >= 3");
   double c[5];
   fat7_array(coef, c);
-  launch_make_imp_links(lat->g, lat->smear, fl, gf, c, ll, gf_long, naik, g_remap, g_stream, nullptr);
+  launch_make_imp_links(lat->g, lat->smear, fl, gf, c, ll, gf_long, naik, g_remap | (g_smear_walk ? 2 : 0), g_stream, nullptr);
   check_launch();
   HIPCHK(hipStreamSynchronize(g_stream));
   API_END
@@ -1744,7 +1746,7 @@ int qexhip_hisq_smear(qexhip_lattice *lat, const qexhip_hisq_coefs *c, const dou
   double c1[5], c2[5];
   fat7_array(&c->fat7first, c1);
   fat7_array(&c->fat7second, c2);
-  launch_hisq_smear(lat->g, lat->smear, c1, c2, c->naik, g, fl, ll, g_remap, g_stream, nullptr);
+  launch_hisq_smear(lat->g, lat->smear, c1, c2, c->naik, g, fl, ll, g_remap | (g_smear_walk ? 2 : 0), g_stream, nullptr);
   check_launch();
   HIPCHK(hipStreamSynchronize(g_stream));
   API_END
@@ -1761,7 +1763,7 @@ int qexhip_fat7l_deriv(qexhip_lattice *lat, double *deriv, const double *gauge, 
     for (int d = 0; d < 4; d++) require(lat->g.L[d] >= 3, "long links need extents >= 3");
   double c[5];
   fat7_array(coef, c);
-  launch_fat7l_deriv(lat->g, lat->smear, deriv, gauge, mid, c, llderiv, llgauge, llmid, naik, g_remap, g_stream);
+  launch_fat7l_deriv(lat->g, lat->smear, deriv, gauge, mid, c, llderiv, llgauge, llmid, naik, g_remap | (g_smear_walk ? 2 : 0), g_stream);
   check_launch();
   HIPCHK(hipStreamSynchronize(g_stream));
   API_END
@@ -1797,7 +1799,7 @@ int qexhip_hisq_smear_deriv(qexhip_lattice *lat, const qexhip_hisq_coefs *c, con
   for (int d = 0; d < 4; d++) require(lat->g.L[d] >= 3, "HISQ links need extents >= 3");
   double c1[5], c2[5], naik;
   hisq_arrays(c, c1, c2, &naik);
-  launch_hisq_smear_deriv(lat->g, lat->smear, c1, c2, naik, g, dfl, dll, lat->V, dg, 1, g_remap, g_stream);
+  launch_hisq_smear_deriv(lat->g, lat->smear, c1, c2, naik, g, dfl, dll, lat->V, dg, 1, g_remap | (g_smear_walk ? 2 : 0), g_stream);
   check_launch();
   HIPCHK(hipStreamSynchronize(g_stream));
   API_END
@@ -1824,7 +1826,7 @@ int qexhip_hisq_force(qexhip_links *l, const qexhip_hisq_coefs *c, const double 
                (const double *)psi->par(1), outer, 1, g_stream, 8);
   // F = 2 dS/dg, through both smearing levels; then F <- TAH(F g^+)
   launch_hisq_smear_deriv(lat->g, lat->smear, c1, c2, naik, g, outer, outer + (size_t)lat->V * 18, 2 * lat->V, F, 0,
-                          g_remap, g_stream);
+                          g_remap | (g_smear_walk ? 2 : 0), g_stream);
   launch_tah_mul(lat->V, F, F, g, g_stream);
   check_launch();
   HIPCHK(hipStreamSynchronize(g_stream));
@@ -1846,10 +1848,10 @@ int qexhip_time_hisq_smear(qexhip_lattice *lat, const double *g, double *fl, dou
   HIPCHK(hipEventCreate(&e1));
   for (auto &t : tm)
     for (auto &e : t.ev) HIPCHK(hipEventCreate(&e));
-  launch_hisq_smear(lat->g, lat->smear, c1, c2, c.naik, g, fl, ll, g_remap, g_stream, nullptr);
+  launch_hisq_smear(lat->g, lat->smear, c1, c2, c.naik, g, fl, ll, g_remap | (g_smear_walk ? 2 : 0), g_stream, nullptr);
   HIPCHK(hipEventRecord(e0, g_stream));
   for (int i = 0; i < nrep; i++)
-    launch_hisq_smear(lat->g, lat->smear, c1, c2, c.naik, g, fl, ll, g_remap, g_stream, nullptr);
+    launch_hisq_smear(lat->g, lat->smear, c1, c2, c.naik, g, fl, ll, g_remap | (g_smear_walk ? 2 : 0), g_stream, nullptr);
   HIPCHK(hipEventRecord(e1, g_stream));
   HIPCHK(hipEventSynchronize(e1));
   float t = 0;
@@ -1857,7 +1859,7 @@ int qexhip_time_hisq_smear(qexhip_lattice *lat, const double *g, double *fl, dou
   ms[0] = (double)t / nrep;
   for (int k = 1; k < 7; k++) ms[k] = 0.0;
   for (int i = 0; i < nrep; i++) {
-    launch_hisq_smear(lat->g, lat->smear, c1, c2, c.naik, g, fl, ll, g_remap, g_stream, tm);
+    launch_hisq_smear(lat->g, lat->smear, c1, c2, c.naik, g, fl, ll, g_remap | (g_smear_walk ? 2 : 0), g_stream, tm);
     HIPCHK(hipStreamSynchronize(g_stream));
     for (int lv = 0; lv < 2; lv++)
       for (int k = 0; k < 3; k++) {

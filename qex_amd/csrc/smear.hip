@@ -135,12 +135,38 @@ struct Fat7 {
 // neighbourhood at the same time, so those loads hit in the CU's L1 and the
 // XCD's L2 instead of returning in a later pass over the lattice.  Workgroups
 // walk the sites in XCD-contiguous ranges.
-__device__ __forceinline__ long site_of(int remap) { return xcd_block(remap) * 64 + (threadIdx.x & 63); }
+//
+// With bit 1 of `remap` set the groups walk t before the rest of the slice:
+// each t-slice of one parity (S = hx*Ly*Lz half-sites) is cut into chunks of
+// C half-sites, and consecutive groups take (parity, then t) within one chunk
+// before moving to the next chunk.  The sites in flight on an XCD then span a
+// few t-slices and both parities of one chunk, so their t-neighbours are read
+// while still in the L2; only neighbours across chunk faces (in z) return from
+// the memory side.  The host sets the bit only when S is a multiple of 64.
+#ifndef QH_SMEAR_CHUNK
+#define QH_SMEAR_CHUNK 1024
+#endif
+__device__ __forceinline__ long site_of(const Geom &g, int remap) {
+  const long G = xcd_block(remap & 1);
+  const int lane = threadIdx.x & 63;
+  if (!(remap & 2)) return G * 64 + lane;
+  if (G * 64 >= 2 * g.Vh) return 2 * g.Vh;  // padding group of the XCD-rounded grid
+  const long S = (long)g.hx * g.L[1] * g.L[2];
+  long C = QH_SMEAR_CHUNK;
+  while (C > 64 && S % C) C >>= 1;
+  const long gpc = C >> 6;
+  const long gi = G % gpc;
+  long r = G / gpc;
+  const long p = r & 1;
+  r >>= 1;
+  const long t = r % g.L[3], k = r / g.L[3];
+  return p * g.Vh + t * S + k * C + gi * 64 + lane;
+}
 __device__ __forceinline__ int wave_of() { return threadIdx.x >> 6; }
 
 __global__ void __launch_bounds__(256) k_fat3(Geom g, SmearFields f, int remap) {
   const long V = 2 * g.Vh;
-  const long x = site_of(remap);
+  const long x = site_of(g, remap);
   const int mu = wave_of();
   if (x >= V) return;
   int c[4];
@@ -155,7 +181,7 @@ __global__ void __launch_bounds__(256) k_fat3(Geom g, SmearFields f, int remap) 
 
 __global__ void __launch_bounds__(192) k_fat5(Geom g, SmearFields f, int remap) {
   const long V = 2 * g.Vh;
-  const long x = site_of(remap);
+  const long x = site_of(g, remap);
   const int mu = blockIdx.y, ks = wave_of();
   if (x >= V) return;
   const int sig = other(mu, ks);
@@ -181,7 +207,7 @@ __global__ void __launch_bounds__(192) k_fat5(Geom g, SmearFields f, int remap) 
 template <int HAVE>
 __global__ void __launch_bounds__(256) k_fatf(Geom g, SmearFields f, Fat7 c, int remap) {
   const long V = 2 * g.Vh;
-  const long x = site_of(remap);
+  const long x = site_of(g, remap);
   const int mu = wave_of();
   if (x >= V) return;
   M3 acc = ldf(f.U[mu], x);
@@ -212,7 +238,7 @@ __global__ void __launch_bounds__(256) k_fatf(Geom g, SmearFields f, Fat7 c, int
 __global__ void __launch_bounds__(256) k_naik(Geom g, MF W0, MF W1, MF W2, MF W3, MF o0, MF o1, MF o2, MF o3,
                                               double naik, int remap) {
   const long V = 2 * g.Vh;
-  const long x = site_of(remap);
+  const long x = site_of(g, remap);
   const int mu = wave_of();
   if (x >= V) return;
   const MF W = mu == 0 ? W0 : (mu == 1 ? W1 : (mu == 2 ? W2 : W3));
@@ -350,7 +376,7 @@ __device__ __forceinline__ void side(M3 &acc, double a, const Geom &g, const MF 
 // CT_{mu,sig} = c5 C_mu + c7 S_sig[C_mu]; one wave per sig, mu = blockIdx.y
 __global__ void __launch_bounds__(192) k_dct(Geom g, DerivFields f, Fat7 c, int remap) {
   const long V = 2 * g.Vh;
-  const long x = site_of(remap);
+  const long x = site_of(g, remap);
   const int mu = blockIdx.y, ks = wave_of();
   if (x >= V) return;
   const int sig = other(mu, ks);
@@ -371,7 +397,7 @@ __global__ void __launch_bounds__(192) k_dct(Geom g, DerivFields f, Fat7 c, int 
 template <int HAVE5>
 __global__ void __launch_bounds__(192) k_dcs(Geom g, DerivFields f, Fat7 c, int remap) {
   const long V = 2 * g.Vh;
-  const long x = site_of(remap);
+  const long x = site_of(g, remap);
   const int mu = blockIdx.y, kn = wave_of();
   if (x >= V) return;
   const int nu = other(mu, kn);
@@ -404,7 +430,7 @@ __global__ void __launch_bounds__(192) k_dcs(Geom g, DerivFields f, Fat7 c, int 
 template <int HAVE5>
 __global__ void __launch_bounds__(192) k_dside(Geom g, DerivFields f, Fat7 c, int remap) {
   const long V = 2 * g.Vh;
-  const long y = site_of(remap);
+  const long y = site_of(g, remap);
   const int lam = blockIdx.y, k = wave_of();
   if (y >= V) return;
   const int mu = other(lam, k);
@@ -436,7 +462,7 @@ __global__ void __launch_bounds__(192) k_dside(Geom g, DerivFields f, Fat7 c, in
 template <int HAVE3>
 __global__ void __launch_bounds__(256) k_dfin(Geom g, DerivFields f, Fat7 c, int acc_out, int remap) {
   const long V = 2 * g.Vh;
-  const long y = site_of(remap);
+  const long y = site_of(g, remap);
   const int lam = wave_of();
   if (y >= V) return;
   M3 d = ldf(f.C[lam], y);
@@ -465,7 +491,7 @@ __global__ void __launch_bounds__(256) k_dfin(Geom g, DerivFields f, Fat7 c, int
 __global__ void __launch_bounds__(256) k_dnaik(Geom g, MF W0, MF W1, MF W2, MF W3, MF C0, MF C1, MF C2, MF C3, MF o0,
                                                MF o1, MF o2, MF o3, double naik, int acc_out, int remap) {
   const long V = 2 * g.Vh;
-  const long y = site_of(remap);
+  const long y = site_of(g, remap);
   const int mu = wave_of();
   if (y >= V) return;
   const MF W = mu == 0 ? W0 : (mu == 1 ? W1 : (mu == 2 ? W2 : W3));
@@ -668,7 +694,14 @@ static unsigned nblk(long V) { return (unsigned)((V + 255) / 256); }
 // workgroups of 64 sites, a multiple of 8 when they are remapped to XCDs
 static unsigned site_groups(long V, int remap) {
   const unsigned n = (unsigned)((V + 63) / 64);
-  return remap ? (n + 7) & ~7u : n;
+  return (remap & 1) ? (n + 7) & ~7u : n;
+}
+
+// remap bit 0: XCD-contiguous groups; bit 1: t-first walk (when the t-slice
+// of one parity is a whole number of groups)
+static int walk_mode(const Geom &g, int remap, int walk) {
+  const long S = (long)g.hx * g.L[1] * g.L[2];
+  return (remap ? 1 : 0) | ((walk && S % 64 == 0) ? 2 : 0);
 }
 
 // one fat7 level: input links U (4 views), output fat links out (4 views)
@@ -721,6 +754,7 @@ static void project(long V, const MF in[4], const MF out[4], hipStream_t st) {
 
 void launch_make_imp_links(const Geom &g, SmearWs &w, double *fl, const double *gf, const double coef[5], double *ll,
                            const double *gl, double naik, int remap, hipStream_t st, SmearTimes *tm) {
+  remap = walk_mode(g, remap & 1, remap & 2);
   const long V = 2 * g.Vh;
   ws_ensure(w, V, WS_SMEAR);
   MF in[4], U[4], out[4];
@@ -749,6 +783,7 @@ void launch_project_u(long n, double *u, const double *x, hipStream_t st) {
 
 void launch_hisq_smear(const Geom &g, SmearWs &w, const double first[5], const double second[5], double naik,
                        const double *gin, double *fl, double *ll, int remap, hipStream_t st, SmearTimes *tm) {
+  remap = walk_mode(g, remap & 1, remap & 2);
   const long V = 2 * g.Vh;
   ws_ensure(w, V, WS_SMEAR);
   MF in[4], U[4], V1[4], W[4], out[4], lo[4];
@@ -833,6 +868,7 @@ static void naik_deriv(const Geom &g, const MF W[4], const MF C[4], const MF o[4
 void launch_fat7l_deriv(const Geom &g, SmearWs &w, double *deriv, const double *gf, const double *mid,
                         const double coef[5], double *llderiv, const double *gl, const double *llmid, double naik,
                         int remap, hipStream_t st) {
+  remap = walk_mode(g, remap & 1, remap & 2);
   const long V = 2 * g.Vh;
   ws_ensure(w, V, WS_DERIV);
   MF in[4], U[4], Cin[4], C[4], out[4];
@@ -868,6 +904,7 @@ void launch_project_u_deriv(long n, double *r, const double *u, const double *x,
 void launch_hisq_smear_deriv(const Geom &g, SmearWs &w, const double first[5], const double second[5], double naik,
                              const double *gin, const double *dfl, const double *dll, long mstride, double *dg,
                              int acc, int remap, hipStream_t st) {
+  remap = walk_mode(g, remap & 1, remap & 2);
   const long V = 2 * g.Vh;
   ws_ensure(w, V, WS_DERIV);
   MF in[4], U[4], V1[4], W[4], Cf[4], Cl[4], Cfc[4], Clc[4], t[4], t2[4], out[4];
