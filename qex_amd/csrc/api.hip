@@ -1817,11 +1817,7 @@ int qexhip_hisq_force(qexhip_links *l, const qexhip_hisq_coefs *c, const double 
   double c1[5], c2[5], naik;
   hisq_arrays(c, c1, c2, &naik);
   // eps psi psi(x+mu)^+ at [2mu], eps psi psi(x+3mu)^+ at [2mu+1]: twice dS/d(fat, long)
-  double *outer = dalloc<double>((size_t)8 * lat->V * 18 * sizeof(double));
-  struct Free {
-    double *p;
-    ~Free() { (void)hipFree(p); }
-  } guard{outer};
+  double *outer = smear_force_outer(lat->smear, lat->V);
   launch_force(lat->g, links_store(l, P64, 18), lkblk_bytes(l, P64, 18), (const double *)psi->par(0),
                (const double *)psi->par(1), outer, 1, g_stream, 8);
   // F = 2 dS/dg, through both smearing levels; then F <- TAH(F g^+)

@@ -216,6 +216,9 @@ void launch_project_u_deriv(long n, double *r, const double *u, const double *x,
 void launch_hisq_smear_deriv(const Geom &g, SmearWs &w, const double first[5], const double second[5], double naik,
                              const double *gin, const double *dfl, const double *dll, long mstride, double *dg,
                              int acc, int remap, hipStream_t st);
+// 8 site-major link fields for the HISQ force outer products, kept in the
+// smearing workspace (past the fields the chain rule uses)
+double *smear_force_outer(SmearWs &w, long V);
 // F_mu(x) = TAH(D_mu(x) U_mu(x)^+), in place on F = D allowed
 void launch_tah_mul(long V, double *F, const double *D, const double *U, hipStream_t st);
 

@@ -669,7 +669,8 @@ SmearWs::~SmearWs() {
 //   32-35 projected links W; the chain rule adds 36-47 CT, 48-59 CS,
 //   60-71 side-link partials, 72-75 fat chain, 76-79 long chain,
 //   80-83 dS/dW, 84-87 dS/dV1.
-enum { WS_SMEAR = 36, WS_DERIV = 88 };
+//   88-95 HISQ force outer products, site-major [k][site] (smear_force_outer).
+enum { WS_SMEAR = 36, WS_DERIV = 88, WS_FORCE = 96 };
 
 static void ws_ensure(SmearWs &w, long V, int nfields) {
   if (w.buf && w.V == V && w.nfields >= nfields) return;
@@ -935,6 +936,11 @@ void launch_hisq_smear_deriv(const Geom &g, SmearWs &w, const double first[5], c
                      V1[3], t[0], t[1], t[2], t[3], t2[0], t2[1], t2[2], t2[3]);
   // first level: dg (+)= dS/dU
   fat7_deriv_level(g, w, U, t2, out, first, acc, remap, st);
+}
+
+double *smear_force_outer(SmearWs &w, long V) {
+  ws_ensure(w, V, WS_FORCE);
+  return (double *)(w.buf + (size_t)WS_DERIV * V * 9);
 }
 
 void launch_tah_mul(long V, double *F, const double *D, const double *U, hipStream_t st) {
