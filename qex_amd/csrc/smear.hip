@@ -255,12 +255,13 @@ __global__ void __launch_bounds__(256) k_naik(Geom g, MF W0, MF W1, MF W2, MF W3
   stf(o, x, t);
 }
 
-// u = x (x^+ x + 1e-20)^{-1/2}.  The eigenvalues s_i^2 of H = x^+ x + 1e-20 come
-// from the trigonometric solution of its characteristic cubic (Smith, CACM 4
-// (1961) 168); H^{-1/2} = f0 + f1 H + f2 H^2 is the quadratic through 1/s_i:
-// with u1 = sum s_i, v = sum_{i<j} s_i s_j, w = prod s_i, d = w (u1 v - w):
-// f0 = (u1 v^2 - v w - u1^2 w)/d, f1 = -(u1 tr H + w)/d, f2 = u1/d.
-__device__ __forceinline__ M3 project_u(const M3 &X) {
+// H^{-1/2} for H = x^+ x + 1e-20 (Hermitian positive).  The eigenvalues s_i^2
+// of H come from the trigonometric solution of its characteristic cubic
+// (Smith, CACM 4 (1961) 168); H^{-1/2} = f0 + f1 H + f2 H^2 is the quadratic
+// through 1/s_i: with u1 = sum s_i, v = sum_{i<j} s_i s_j, w = prod s_i and
+// d = w (u1 v - w): f0 = (u1 v^2 - v w - u1^2 w)/d, f1 = -(u1 tr H + w)/d,
+// f2 = u1/d.
+__device__ __forceinline__ M3 rsqrt_xdx(const M3 &X) {
   M3 H = mmul_dn(X, X);
 #pragma unroll
   for (int k = 0; k < 3; k++) H.e[4 * k].x += 1e-20;
@@ -290,15 +291,18 @@ __device__ __forceinline__ M3 project_u(const M3 &X) {
   const double di = 1.0 / (w * (u1 * v - w));
   const double f0 = (u1 * v * v - v * w - u1 * u1 * w) * di, f1 = -(u1 * 3.0 * q + w) * di, f2 = u1 * di;
   const M3 H2 = mmul(H, H);
-  M3 R;
+  M3 Z;
 #pragma unroll
-  for (int e = 0; e < 9; e++) R.e[e] = make_double2(f1 * H.e[e].x + f2 * H2.e[e].x, f1 * H.e[e].y + f2 * H2.e[e].y);
+  for (int e = 0; e < 9; e++) Z.e[e] = make_double2(f1 * H.e[e].x + f2 * H2.e[e].x, f1 * H.e[e].y + f2 * H2.e[e].y);
 #pragma unroll
-  for (int k = 0; k < 3; k++) R.e[4 * k].x += f0;
-  return mmul(X, R);
+  for (int k = 0; k < 3; k++) Z.e[4 * k].x += f0;
+  return Z;
 }
 
-// n matrices; matrix i of field i / V at site i % V (per-field views)
+// projectU: u = x (x^+ x + 1e-20)^{-1/2}
+__device__ __forceinline__ M3 project_u(const M3 &X) { return mmul(X, rsqrt_xdx(X)); }
+
+// projectU on V sites of four link fields (blockIdx.y selects the field views)
 __global__ void __launch_bounds__(256) k_project(long V, MF i0, MF i1, MF i2, MF i3, MF o0, MF o1, MF o2, MF o3) {
   const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int mu = blockIdx.y;
@@ -570,40 +574,7 @@ __device__ __forceinline__ M3 sylsolve(const M3 &A, const M3 &C) {
 // projectUderiv: for u = x z, z = (x^+ x)^{-1/2}, y = z^{-1}, chain c = dS/du:
 // dS/dx = c z - x (t + t^+) with y t + t y = u^+ c z
 __device__ __forceinline__ M3 project_u_deriv(const M3 &Uu, const M3 &X, const M3 &C) {
-  M3 H = mmul_dn(X, X);
-#pragma unroll
-  for (int k = 0; k < 3; k++) H.e[4 * k].x += 1e-20;
-  // z = H^{-1/2} as in project_u
-  const double h0 = H.e[0].x, h1 = H.e[4].x, h2 = H.e[8].x;
-  const double q = (h0 + h1 + h2) * (1.0 / 3.0);
-  const double b0 = h0 - q, b1 = h1 - q, b2 = h2 - q;
-  const double2 z01 = H.e[1], z02 = H.e[2], z12 = H.e[5];
-  const double o01 = z01.x * z01.x + z01.y * z01.y, o02 = z02.x * z02.x + z02.y * z02.y;
-  const double o12 = z12.x * z12.x + z12.y * z12.y;
-  const double p2 = (b0 * b0 + b1 * b1 + b2 * b2 + 2.0 * (o01 + o02 + o12)) * (1.0 / 6.0);
-  double e0 = q, e1 = q, e2 = q;
-  if (p2 > 0.0) {
-    const double p = sqrt(p2);
-    const double2 a = make_double2(z01.x * z12.x - z01.y * z12.y, z01.x * z12.y + z01.y * z12.x);
-    const double re3 = a.x * z02.x + a.y * z02.y;
-    const double detb = b0 * b1 * b2 + 2.0 * re3 - b0 * o12 - b1 * o02 - b2 * o01;
-    double r = detb / (2.0 * p2 * p);
-    r = fmin(1.0, fmax(-1.0, r));
-    const double phi = acos(r) * (1.0 / 3.0);
-    e0 = q + 2.0 * p * cos(phi);
-    e2 = q + 2.0 * p * cos(phi + 2.0943951023931954923);
-    e1 = 3.0 * q - e0 - e2;
-  }
-  const double s0 = sqrt(fabs(e0)), s1 = sqrt(fabs(e1)), s2 = sqrt(fabs(e2));
-  const double u1 = s0 + s1 + s2, v = s0 * s1 + s1 * s2 + s2 * s0, w = s0 * s1 * s2;
-  const double di = 1.0 / (w * (u1 * v - w));
-  const double f0 = (u1 * v * v - v * w - u1 * u1 * w) * di, f1 = -(u1 * 3.0 * q + w) * di, f2 = u1 * di;
-  const M3 H2 = mmul(H, H);
-  M3 Z;
-#pragma unroll
-  for (int e = 0; e < 9; e++) Z.e[e] = make_double2(f1 * H.e[e].x + f2 * H2.e[e].x, f1 * H.e[e].y + f2 * H2.e[e].y);
-#pragma unroll
-  for (int k = 0; k < 3; k++) Z.e[4 * k].x += f0;
+  const M3 Z = rsqrt_xdx(X);
   const M3 Y = minv(Z);
   M3 R = mmul(C, Z);
   const M3 t2 = sylsolve(Y, mmul_dn(Uu, R));
