@@ -1184,7 +1184,15 @@ void *qexhip_stream(void) { return (void *)g_stream; }
 int qexhip_lattice_create(const int dims[4], qexhip_lattice **out) {
   API_BEGIN
   check_init();
-  for (int d = 0; d < 4; d++) require(dims[d] >= 2 && dims[d] % 2 == 0, "lattice extents must be even and >= 2");
+  require(dims != nullptr && out != nullptr, "null argument");
+  long vol = 1;
+  for (int d = 0; d < 4; d++) {
+    require(dims[d] >= 2 && dims[d] % 2 == 0, "lattice extents must be even and >= 2");
+    require(dims[d] <= (1 << 20), "lattice extent too large");
+    vol *= dims[d];
+    // site indices and the generators' site counters are 32-bit
+    require(vol <= (1L << 31), "lattice too large: more than 2^31 sites");
+  }
   qexhip_lattice *lat = new qexhip_lattice();
   for (int d = 0; d < 4; d++) lat->g.L[d] = dims[d];
   lat->g.hx = dims[0] / 2;
