@@ -121,11 +121,6 @@ void launch_bcg_flush(const BatchItem *T, int n, long nblocks, int block, hipStr
 }
 
 // --------------------------------------------- (2) multi-shift, multi-RHS
-// acc -= conj(a)^T-row product: acc -= conj(a) * b
-__device__ __forceinline__ void cjmsub(double2 &acc, double2 a, double2 b) {
-  acc.x -= a.x * b.x + a.y * b.y;
-  acc.y -= a.x * b.y - a.y * b.x;
-}
 
 // MODE 0: t_j = (2D) p0_j on parity q (x = site of parity q).
 // MODE 1: Ap_j = m4 p0_j - (2D) t_j on parity q with the partial sums of

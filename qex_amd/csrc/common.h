@@ -132,14 +132,27 @@ template <class A> __device__ __forceinline__ A mk(typename cx<A>::R x, typename
   A a; a.x = x; a.y = y; return a;
 }
 // acc += a*b
+// Complex multiply-accumulates as chains of fused multiply-adds on the
+// accumulator (four per product; written as acc += (p - q) they would cost six).
 template <class A> __device__ __forceinline__ void cmac(A &acc, A a, A b) {
-  acc.x += a.x * b.x - a.y * b.y;
-  acc.y += a.x * b.y + a.y * b.x;
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
 }
 // acc += conj(a)*b
 template <class A> __device__ __forceinline__ void cjmac(A &acc, A a, A b) {
-  acc.x += a.x * b.x + a.y * b.y;
-  acc.y += a.x * b.y - a.y * b.x;
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(-a.y, b.x, acc.y);
+}
+// acc -= conj(a)*b
+template <class A> __device__ __forceinline__ void cjmsub(A &acc, A a, A b) {
+  acc.x = fma(-a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(-a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
 }
 
 // ------------------------------------------------------------------ geometry
@@ -379,8 +392,14 @@ __device__ __forceinline__ void ld_link(const char *blk, long stride, long i, A 
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const A a1 = U[(k + 1) % 3], a2 = U[(k + 2) % 3], b1 = U[3 + (k + 1) % 3], b2 = U[3 + (k + 2) % 3];
-      const typename cx<A>::R re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
-      const typename cx<A>::R im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
+      // (a1 b2 - a2 b1), as one fused chain per component
+      typename cx<A>::R re = a1.x * b2.x, im = a1.x * b2.y;
+      re = fma(-a1.y, b2.y, re);
+      re = fma(-a2.x, b1.x, re);
+      re = fma(a2.y, b1.y, re);
+      im = fma(a1.y, b2.x, im);
+      im = fma(-a2.x, b1.y, im);
+      im = fma(-a2.y, b1.x, im);
       U[6 + k] = mk<A>(sgn * re, -sgn * im);
     }
   }

@@ -77,10 +77,7 @@ __device__ __forceinline__ void stencil(const Geom &g, const char *links, long l
 #pragma unroll
         for (int r = 0; r < 3; r++)
 #pragma unroll
-          for (int c = 0; c < 3; c++) {
-            af[r].x -= L0.U[3 * c + r].x * L0.v[c].x + L0.U[3 * c + r].y * L0.v[c].y;
-            af[r].y -= L0.U[3 * c + r].x * L0.v[c].y - L0.U[3 * c + r].y * L0.v[c].x;
-          }
+          for (int c = 0; c < 3; c++) cjmsub(af[r], L0.U[3 * c + r], L0.v[c]);
       }
       L0 = L1;
     }
@@ -98,10 +95,7 @@ __device__ __forceinline__ void stencil(const Geom &g, const char *links, long l
 #pragma unroll
         for (int r = 0; r < 3; r++)
 #pragma unroll
-          for (int c = 0; c < 3; c++) {
-            af[r].x -= L0.U[3 * c + r].x * L0.v[c].x + L0.U[3 * c + r].y * L0.v[c].y;
-            af[r].y -= L0.U[3 * c + r].x * L0.v[c].y - L0.U[3 * c + r].y * L0.v[c].x;
-          }
+          for (int c = 0; c < 3; c++) cjmsub(af[r], L0.U[3 * c + r], L0.v[c]);
       }
     }
   } else if constexpr (D == -1) {  // rolled loop, one hop (two links) per trip
