@@ -273,6 +273,48 @@ int qexhip_milc_read(qexhip_lattice *lat, const char *path, double *U_dev, qexhi
    precision with the checksum; timestamp may be NULL */
 int qexhip_milc_write(qexhip_lattice *lat, const char *path, const double *U_dev, const char *timestamp);
 
+/* ---- SciDAC / ILDG lattice files (LIME containers), QEX's default gauge and
+   field format (src/gauge/gaugeUtils.nim:96-131; src/io/readerQiolite.nim,
+   src/io/writerQiolite.nim over the qiolite package).  Field data on the
+   device are canonical complex128 arrays [nobj][site][nelem]: nelem 9 for
+   3x3 matrices (a gauge field is nobj = 4), 3 for color vectors, 1 for
+   complex numbers; nobj 1..4.  The per-site CRC-32 SciDAC checksum is
+   computed on the device while converting and verified on reading. */
+typedef struct {
+  int dims[4];
+  int nrecords;          /* field records in the file */
+  char datatype[64];     /* of the selected record, e.g. QDP_D3_ColorMatrix */
+  char date[64];
+  char precision;        /* 'F' or 'D' */
+  int colors, spins, typesize, datacount;
+  int ildg;              /* binary record is ildg-binary-data */
+  int has_checksum;
+  uint32_t suma, sumb;            /* stored in the file */
+  uint32_t calc_suma, calc_sumb;  /* computed from the data read or written */
+  int checksum_ok;
+  long long filemd_len, recmd_len;  /* metadata lengths without the NUL */
+} qexhip_scidac_info;
+/* getFileLattice + newReader + recordMetadata (readerQiolite.nim:11-17,38-51,
+   60-61): file and record description of field record `record` (0-based);
+   filemd/recmd (may be NULL) receive up to cap-1 bytes of the file and record
+   metadata, NUL-terminated */
+int qexhip_scidac_read_info(const char *path, int record, qexhip_scidac_info *info, char *filemd,
+                            long long filemd_cap, char *recmd, long long recmd_cap);
+/* Reader.read (readerQiolite.nim:95-161,174-183): fails when the lattice,
+   datacount or element count differ from the record's, or on a checksum
+   mismatch; single-precision records are widened to FP64 */
+int qexhip_scidac_read(qexhip_lattice *lat, const char *path, int record, double *data_dev, int nobj, int nelem,
+                       qexhip_scidac_info *info);
+/* newWriter (writerQiolite.nim:29-42): creates the file and writes its
+   file message with metadata filemd */
+int qexhip_scidac_create(const char *path, const int dims[4], const char *filemd);
+/* Writer.write (writerQiolite.nim:83-160): appends one field record in
+   precision 'F' or 'D' with metadata recmd; ildg != 0 adds the ildg-format
+   record and names the data ildg-binary-data (gauge fields only); date NULL
+   means the current UTC time.  info (may be NULL) receives the checksums. */
+int qexhip_scidac_append(qexhip_lattice *lat, const char *path, const char *recmd, const double *data_dev, int nobj,
+                         int nelem, char precision, int ildg, const char *date, qexhip_scidac_info *info);
+
 int qexhip_gen_gauge(qexhip_lattice *lat, uint64_t seed, double *U_dev);
 int qexhip_gen_source(qexhip_lattice *lat, uint64_t seed, int stream, int par, double *x_dev);
 int qexhip_gen_phase_bc(qexhip_lattice *lat, double *U_dev);

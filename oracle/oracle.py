@@ -70,6 +70,8 @@ def _declare(l):
         "qo_hisq_smear_deriv": (None, [_ip, _dp, _dp, D, _dp, _dp, _dp, _dp]),
         "qo_hisq_outer": (None, [_ip, _dp, _dp]),
         "qo_milc_checksum": (None, [ctypes.POINTER(U32), L, L, ctypes.POINTER(U32)]),
+        "qo_crc32": (U32, [ctypes.c_char_p, L]),
+        "qo_scidac_checksum": (None, [ctypes.c_char_p, L, L, L, ctypes.POINTER(U32)]),
         "qo_su3_exp_i": (None, [_dp, D, _dp]),
         "qo_su3_exp": (None, [_dp, _dp]),
         "qo_mat_mul": (None, [_dp, _dp, _dp]),
@@ -419,3 +421,154 @@ def milc_read(path):
     lex = C[:, 0] + L[0] * (C[:, 1] + L[1] * (C[:, 2] + L[2] * C[:, 3]))
     U = np.ascontiguousarray(np.transpose(data[lex], (1, 0, 2, 3)).astype(np.complex128))
     return L, U, (s29, s31), milc_checksum(words)
+
+
+# ---------------------------------------------------------------------------
+# SciDAC / ILDG files.  QEX reads and writes them through the qiolite package
+# (qex.nimble:35, github.com/usqcd-software/qiolite at #head, not vendored);
+# src/io/readerQiolite.nim:38-161 and writerQiolite.nim:29-160 are its call
+# sites.  The container is LIME: records of a 144-byte big-endian header
+# {uint32 magic 0x456789ab, uint16 version 1, uint16 flags (0x8000 message
+# begin, 0x4000 message end), uint64 data length, char type[128]} and data
+# padded with zeros to a multiple of 8 bytes.  A SciDAC file holds a file
+# message {scidac-private-file-xml, scidac-file-xml} and then one message per
+# field record {scidac-private-record-xml, scidac-record-xml, [ildg-format],
+# scidac-binary-data or ildg-binary-data, scidac-checksum}.  The binary data
+# are big-endian, sites in lexicographic order (x fastest), each site holding
+# datacount objects of typesize bytes (a 3x3 complex matrix is row-major).
+
+LIME_MAGIC = 0x456789AB
+
+
+def crc32(data):
+    """CRC-32 (src/io/crc32.nim:1-40)."""
+    b = bytes(data)
+    return int(lib().qo_crc32(b, len(b)))
+
+
+def scidac_checksum(data, site_bytes, rank0=0):
+    """(suma, sumb) of a SciDAC binary record (per-site CRC-32 rotated by the
+    site's lexicographic rank mod 29 and mod 31)."""
+    b = bytes(data)
+    out = (ctypes.c_uint32 * 2)()
+    lib().qo_scidac_checksum(b, len(b) // site_bytes, int(site_bytes), int(rank0), out)
+    return int(out[0]), int(out[1])
+
+
+def lime_record(rtype, data, mb, me):
+    """One LIME record: header, data, zero padding to 8 bytes."""
+    import struct
+    data = bytes(data)
+    flags = (0x8000 if mb else 0) | (0x4000 if me else 0)
+    t = rtype.encode() if isinstance(rtype, str) else rtype
+    hdr = struct.pack(">IHHQ128s", LIME_MAGIC, 1, flags, len(data), t.ljust(128, b"\0"))
+    return hdr + data + b"\0" * ((-len(data)) % 8)
+
+
+def lime_records(raw):
+    """[(type, mb, me, data bytes)] of a LIME byte string."""
+    import struct
+    out, pos = [], 0
+    while pos + 144 <= len(raw):
+        magic, ver, flags, n, t = struct.unpack(">IHHQ128s", raw[pos:pos + 144])
+        assert magic == LIME_MAGIC and ver == 1, "not a LIME record"
+        out.append((t.split(b"\0")[0].decode(), bool(flags & 0x8000), bool(flags & 0x4000),
+                    raw[pos + 144:pos + 144 + n]))
+        pos += 144 + n + ((-n) % 8)
+    return out
+
+
+_XML = '<?xml version="1.0" encoding="UTF-8"?>'
+_TYPES = {9: "ColorMatrix", 3: "ColorVector", 1: "Complex"}
+
+
+def scidac_private_file(L):
+    return (_XML + "<scidacFile><version>1.1</version><spacetime>4</spacetime><dims>"
+            + "".join(f"{v} " for v in L) + "</dims><volfmt>0</volfmt></scidacFile>")
+
+
+def scidac_private_record(nobj, nelem, prec, date):
+    ts = nelem * 2 * (4 if prec == "F" else 8)
+    dt = f"QDP_{prec}3_{_TYPES[nelem]}" if nelem > 1 else f"QDP_{prec}_Complex"
+    return (_XML + f"<scidacRecord><version>1.1</version><date>{date}</date><recordtype>0</recordtype>"
+            f"<datatype>{dt}</datatype><precision>{prec}</precision><colors>{3 if nelem > 1 else 1}</colors>"
+            f"<spins>1</spins><typesize>{ts}</typesize><datacount>{nobj}</datacount></scidacRecord>")
+
+
+def ildg_format(L, prec):
+    return (_XML + '<ildgFormat xmlns="http://www.lqcd.org/ildg" '
+            'xmlns:xsi="http://www.w3.org/2001/XMLSchema-instance" '
+            'xsi:schemaLocation="http://www.lqcd.org/ildg http://www.lqcd.org/ildg/filefmt.xsd">'
+            f"<version>1.0</version><field>su3gauge</field><precision>{32 if prec == 'F' else 64}</precision>"
+            f"<lx>{L[0]}</lx><ly>{L[1]}</ly><lz>{L[2]}</lz><lt>{L[3]}</lt></ildgFormat>")
+
+
+def scidac_checksum_xml(a, b):
+    return _XML + f"<scidacChecksum><version>1.0</version><suma>{a:x}</suma><sumb>{b:x}</sumb></scidacChecksum>"
+
+
+def _lex_perm(L):
+    C = all_coords(L)
+    return C[:, 0] + L[0] * (C[:, 1] + L[1] * (C[:, 2] + L[2] * C[:, 3]))
+
+
+def scidac_binary(L, fields, prec):
+    """File bytes of canonical fields (nobj, V, nelem) complex128, big-endian,
+    lexicographic site order, objects of one site together."""
+    nobj, V, nelem = fields.shape
+    data = np.empty((V, nobj, nelem), dtype=np.complex128)
+    data[_lex_perm(L)] = np.transpose(fields, (1, 0, 2))
+    dt = ">c8" if prec == "F" else ">c16"
+    return data.astype(dt).tobytes()
+
+
+def scidac_file(L, filemd, records):
+    """Bytes of a SciDAC file.  records: [(fields (nobj, V, nelem), prec, recmd,
+    date, ildg)]."""
+    out = lime_record("scidac-private-file-xml", scidac_private_file(L).encode(), True, False)
+    out += lime_record("scidac-file-xml", filemd.encode(), False, True)
+    for fields, prec, recmd, date, ildg in records:
+        nobj, V, nelem = fields.shape
+        binary = scidac_binary(L, fields, prec)
+        a, b = scidac_checksum(binary, nobj * nelem * (8 if prec == "F" else 16))
+        out += lime_record("scidac-private-record-xml", scidac_private_record(nobj, nelem, prec, date).encode(),
+                           True, False)
+        out += lime_record("scidac-record-xml", recmd.encode(), False, False)
+        if ildg:
+            out += lime_record("ildg-format", ildg_format(L, prec).encode(), False, False)
+        out += lime_record("ildg-binary-data" if ildg else "scidac-binary-data", binary, False, False)
+        out += lime_record("scidac-checksum", scidac_checksum_xml(a, b).encode(), False, True)
+    return out
+
+
+def _tag(doc, t):
+    import re
+    m = re.search(f"<{t}>(.*?)</{t}>", doc, re.S)
+    return m.group(1) if m else None
+
+
+def scidac_read(raw, L, nobj, nelem, record=0):
+    """Fields (nobj, V, nelem) complex128 canonical of field record `record`,
+    with the record's private XML, user XML and (stored, computed) checksums."""
+    recs = lime_records(raw)
+    msgs, cur = [], None
+    for t, mb, me, d in recs:
+        if mb or cur is None:
+            cur = []
+            msgs.append(cur)
+        cur.append((t, d))
+    fields = [m for m in msgs if any(t.endswith("binary-data") for t, _ in m)]
+    m = dict(fields[record])
+    priv = m.get("scidac-private-record-xml", b"").decode()
+    prec = _tag(priv, "precision") or ("F" if _tag(m["ildg-format"].decode(), "precision") == "32" else "D")
+    binary = m.get("scidac-binary-data", m.get("ildg-binary-data"))
+    V = volume(L)
+    dt = ">c8" if prec == "F" else ">c16"
+    data = np.frombuffer(binary, dtype=dt).astype(np.complex128).reshape(V, nobj, nelem)
+    out = np.ascontiguousarray(np.transpose(data[_lex_perm(L)], (1, 0, 2)))
+    stored = None
+    if "scidac-checksum" in m:
+        ck = m["scidac-checksum"].decode()
+        stored = (int(_tag(ck, "suma"), 16), int(_tag(ck, "sumb"), 16))
+    computed = scidac_checksum(binary, nobj * nelem * (8 if prec == "F" else 16))
+    return out, priv, m.get("scidac-record-xml", b"").decode(), stored, computed

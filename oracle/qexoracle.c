@@ -1140,3 +1140,45 @@ void qo_milc_checksum(const uint32_t *w, long n, long offset, uint32_t out[2]) {
   out[0] = a;
   out[1] = b;
 }
+
+/* ------------------------------------------------------ SciDAC record checksum */
+/* CRC-32 as src/io/crc32.nim:1-40 defines it: table of the reflected polynomial
+   0xedb88320, register started at ~0, one table step per byte, complemented at
+   the end.  The reference's own check (crc32.nim:113-116) is the known answer
+   0x414FA339 for "The quick brown fox jumps over the lazy dog". */
+static uint32_t crc_tbl[256];
+static int crc_tbl_ready;
+
+uint32_t qo_crc32(const unsigned char *p, long n) {
+  uint32_t *tbl = crc_tbl;
+  if (!crc_tbl_ready) {
+    for (uint32_t i = 0; i < 256; i++) {
+      uint32_t r = i;
+      for (int j = 0; j < 8; j++) r = (r & 1) ? (r >> 1) ^ 0xedb88320u : r >> 1;
+      tbl[i] = r;
+    }
+    crc_tbl_ready = 1;
+  }
+  uint32_t c = 0xffffffffu;
+  for (long i = 0; i < n; i++) c = (c >> 8) ^ tbl[(c ^ p[i]) & 0xff];
+  return ~c;
+}
+
+/* The SciDAC checksum of a binary record, as the qiolite package QEX reads and
+   writes its files with (qex.nimble:35 pins github.com/usqcd-software/qiolite
+   at #head; it follows USQCD QIO's DML checksum): every site's bytes, as they
+   stand in the file (big-endian), get their own CRC-32 c; with r the site's
+   global lexicographic rank (x fastest), suma ^= rotl(c, r mod 29) and
+   sumb ^= rotl(c, r mod 31). */
+void qo_scidac_checksum(const unsigned char *data, long nsites, long site_bytes, long rank0, uint32_t out[2]) {
+  uint32_t a = 0, b = 0;
+  for (long k = 0; k < nsites; k++) {
+    const uint32_t c = qo_crc32(data + k * site_bytes, site_bytes);
+    const long r = rank0 + k;
+    const int r29 = (int)(r % 29), r31 = (int)(r % 31);
+    a ^= r29 ? (c << r29) | (c >> (32 - r29)) : c;
+    b ^= r31 ? (c << r31) | (c >> (32 - r31)) : c;
+  }
+  out[0] = a;
+  out[1] = b;
+}

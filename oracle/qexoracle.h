@@ -134,6 +134,14 @@ void qo_hisq_outer(const int L[4], const double *psi, double *G);
 /* ---- MILC gauge files (src/new/milcIO.nim) ---- */
 void qo_milc_checksum(const uint32_t *w, long n, long offset, uint32_t out[2]);
 
+/* ---- SciDAC/ILDG gauge and field files (src/io/readerQiolite.nim, writerQiolite.nim) ---- */
+/* CRC-32 of n bytes, reflected polynomial 0xedb88320, initial ~0, final
+   complement (src/io/crc32.nim:1-40) */
+uint32_t qo_crc32(const unsigned char *p, long n);
+/* SciDAC record checksum over nsites sites of site_bytes file bytes each, site
+   k having global lexicographic rank rank0 + k: out[0] = suma, out[1] = sumb */
+void qo_scidac_checksum(const unsigned char *data, long nsites, long site_bytes, long rank0, uint32_t out[2]);
+
 /* ---- 3x3 helpers used by tests ---- */
 void qo_su3_exp_i(const double *H /*18, Hermitian*/, double s, double *U /*18*/);
 void qo_su3_exp(const double *A /*18, general*/, double *U /*18*/);

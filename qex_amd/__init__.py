@@ -11,3 +11,5 @@ from .stag import (D, Ddag, Field, Layout, SloppyType, SolverBackend, SolverPara
                    stream_read_gbs)
 from .hisq import (Fat7lCoefs, HisqCoefs, fat7lDeriv, hisqForce, hisqStag, makeImpLinks, newHISQ,  # noqa: F401,E402
                    projectU, projectUderiv, smearGetForce)
+from .fileio import (Reader, Writer, getFileLattice, loadGauge, newReader, newWriter,  # noqa: F401,E402
+                     saveGauge)

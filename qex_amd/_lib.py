@@ -32,6 +32,7 @@ EXPORTS = [
     "qexhip_hisq_coefs_init", "qexhip_make_imp_links", "qexhip_project_u", "qexhip_hisq_smear",
     "qexhip_time_hisq_smear", "qexhip_fat7l_deriv", "qexhip_project_u_deriv", "qexhip_hisq_smear_deriv",
     "qexhip_hisq_force", "qexhip_milc_read_info", "qexhip_milc_read", "qexhip_milc_write",
+    "qexhip_scidac_read_info", "qexhip_scidac_read", "qexhip_scidac_create", "qexhip_scidac_append",
 ]
 
 # qexhip_solve_batch / qexhip_solve_multishift_batch flags
@@ -69,6 +70,15 @@ class MilcInfoC(ctypes.Structure):
     _fields_ = [("dims", ctypes.c_int * 4), ("timestamp", ctypes.c_char * 65), ("order", ctypes.c_int),
                 ("byteswapped", ctypes.c_int), ("sum29", ctypes.c_uint32), ("sum31", ctypes.c_uint32),
                 ("checksum_ok", ctypes.c_int)]
+
+
+class ScidacInfoC(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int * 4), ("nrecords", ctypes.c_int), ("datatype", ctypes.c_char * 64),
+                ("date", ctypes.c_char * 64), ("precision", ctypes.c_char), ("colors", ctypes.c_int),
+                ("spins", ctypes.c_int), ("typesize", ctypes.c_int), ("datacount", ctypes.c_int),
+                ("ildg", ctypes.c_int), ("has_checksum", ctypes.c_int), ("suma", ctypes.c_uint32),
+                ("sumb", ctypes.c_uint32), ("calc_suma", ctypes.c_uint32), ("calc_sumb", ctypes.c_uint32),
+                ("checksum_ok", ctypes.c_int), ("filemd_len", ctypes.c_longlong), ("recmd_len", ctypes.c_longlong)]
 
 
 class QexHipError(RuntimeError):
@@ -128,6 +138,13 @@ def load(path=LIB_PATH):
         "qexhip_milc_read_info": [ctypes.c_char_p, ctypes.POINTER(MilcInfoC)],
         "qexhip_milc_read": [vp, ctypes.c_char_p, vp, ctypes.POINTER(MilcInfoC)],
         "qexhip_milc_write": [vp, ctypes.c_char_p, vp, ctypes.c_char_p],
+        "qexhip_scidac_read_info": [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ScidacInfoC), ctypes.c_char_p,
+                                    ctypes.c_longlong, ctypes.c_char_p, ctypes.c_longlong],
+        "qexhip_scidac_read": [vp, ctypes.c_char_p, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
+                               ctypes.POINTER(ScidacInfoC)],
+        "qexhip_scidac_create": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.c_char_p],
+        "qexhip_scidac_append": [vp, ctypes.c_char_p, ctypes.c_char_p, vp, ctypes.c_int, ctypes.c_int, ctypes.c_char,
+                                 ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ScidacInfoC)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

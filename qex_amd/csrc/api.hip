@@ -1923,6 +1923,83 @@ int qexhip_milc_write(qexhip_lattice *lat, const char *path, const double *U, co
   API_END
 }
 
+static void copy_str(char *dst, size_t cap, const std::string &s) {
+  if (!dst || cap == 0) return;
+  const size_t n = s.size() < cap - 1 ? s.size() : cap - 1;
+  memcpy(dst, s.data(), n);
+  dst[n] = 0;
+}
+
+static void scidac_info_out(const ScidacInfo &m, qexhip_scidac_info *info) {
+  if (!info) return;
+  memset(info, 0, sizeof *info);
+  for (int d = 0; d < 4; d++) info->dims[d] = m.dims[d];
+  info->nrecords = m.nrecords;
+  copy_str(info->datatype, sizeof info->datatype, m.datatype);
+  copy_str(info->date, sizeof info->date, m.date);
+  info->precision = m.prec;
+  info->colors = m.colors;
+  info->spins = m.spins;
+  info->typesize = m.typesize;
+  info->datacount = m.datacount;
+  info->ildg = m.ildg;
+  info->has_checksum = m.has_checksum;
+  info->suma = m.suma;
+  info->sumb = m.sumb;
+  info->calc_suma = m.calc_suma;
+  info->calc_sumb = m.calc_sumb;
+  info->checksum_ok = m.checksum_ok;
+  info->filemd_len = (long long)m.filemd.size();
+  info->recmd_len = (long long)m.recmd.size();
+}
+
+int qexhip_scidac_read_info(const char *path, int record, qexhip_scidac_info *info, char *filemd,
+                            long long filemd_cap, char *recmd, long long recmd_cap) {
+  API_BEGIN
+  require(path && info, "null argument");
+  require(filemd_cap >= 0 && recmd_cap >= 0, "negative buffer capacity");
+  ScidacInfo m;
+  scidac_info(path, record, &m);
+  scidac_info_out(m, info);
+  copy_str(filemd, (size_t)filemd_cap, m.filemd);
+  copy_str(recmd, (size_t)recmd_cap, m.recmd);
+  API_END
+}
+
+int qexhip_scidac_read(qexhip_lattice *lat, const char *path, int record, double *data, int nobj, int nelem,
+                       qexhip_scidac_info *info) {
+  ScidacInfo m;
+  API_BEGIN
+  check_init();
+  require(lat && path && data, "null argument");
+  try {
+    scidac_read(lat->g, path, record, data, nobj, nelem, &m, g_stream);
+  } catch (...) {
+    scidac_info_out(m, info);
+    throw;
+  }
+  scidac_info_out(m, info);
+  API_END
+}
+
+int qexhip_scidac_create(const char *path, const int dims[4], const char *filemd) {
+  API_BEGIN
+  require(path && dims, "null argument");
+  scidac_create(path, dims, filemd);
+  API_END
+}
+
+int qexhip_scidac_append(qexhip_lattice *lat, const char *path, const char *recmd, const double *data, int nobj,
+                         int nelem, char precision, int ildg, const char *date, qexhip_scidac_info *info) {
+  API_BEGIN
+  check_init();
+  require(lat && path && data, "null argument");
+  ScidacInfo m;
+  scidac_append(lat->g, path, recmd, data, nobj, nelem, precision, ildg, date, &m, g_stream);
+  scidac_info_out(m, info);
+  API_END
+}
+
 int qexhip_links_recon_ok(qexhip_links *l, int *ok) {
   API_BEGIN
   check_init();

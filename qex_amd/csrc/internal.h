@@ -235,6 +235,25 @@ void milc_read_header(const char *path, MilcInfo *info);
 void milc_read(const Geom &g, const char *path, double *U, MilcInfo *info, hipStream_t st);
 void milc_write(const Geom &g, const char *path, const double *U, const char *stamp, hipStream_t st);
 
+// scidac.hip: SciDAC/ILDG (LIME) files; field data are canonical complex128
+// arrays [nobj][site][nelem] on the device, nelem 9 (3x3 matrix), 3 (color
+// vector) or 1 (complex number), nobj 1..4
+struct ScidacInfo {
+  int dims[4] = {0, 0, 0, 0};
+  int nrecords = 0;
+  std::string filemd, recmd, datatype, date;
+  char prec = 0;
+  int colors = 0, spins = 0, typesize = 0, datacount = 0;
+  int ildg = 0, has_checksum = 0, checksum_ok = 0;
+  uint32_t suma = 0, sumb = 0, calc_suma = 0, calc_sumb = 0;
+};
+void scidac_info(const char *path, int record, ScidacInfo *info);
+void scidac_read(const Geom &g, const char *path, int record, double *dst, int nobj, int nelem, ScidacInfo *info,
+                 hipStream_t st);
+void scidac_create(const char *path, const int dims[4], const char *filemd);
+void scidac_append(const Geom &g, const char *path, const char *recmd, const double *src, int nobj, int nelem,
+                   char prec, int ildg, const char *date, ScidacInfo *info, hipStream_t st);
+
 // gen.hip
 void launch_gauge_random(const Geom &g, uint64_t seed, double *canon, hipStream_t st);
 void launch_source_gauss(const Geom &g, uint64_t seed, int stream, int par, double *canon, hipStream_t st);

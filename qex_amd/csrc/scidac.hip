@@ -1,0 +1,711 @@
+// SciDAC and ILDG lattice files, the format of QEX's loadGauge/saveGauge and of
+// its Reader/Writer (src/gauge/gaugeUtils.nim:96-131, src/io/readerQiolite.nim,
+// src/io/writerQiolite.nim; QEX does the container work in the qiolite package,
+// qex.nimble:35, which follows USQCD's QIO and c-lime).
+//
+// Container (LIME): records of a 144-byte big-endian header {uint32 magic
+// 0x456789ab, uint16 version 1, uint16 flags (0x8000 begins a message, 0x4000
+// ends one), uint64 data length, char type[128]}, then the data, zero-padded
+// to a multiple of 8 bytes.  A SciDAC file is a file message
+// {scidac-private-file-xml, scidac-file-xml} followed by one message per field
+// record {scidac-private-record-xml, scidac-record-xml, [ildg-format],
+// scidac-binary-data | ildg-binary-data, scidac-checksum}.  Binary data are
+// big-endian, sites in lexicographic order (x fastest), each site holding
+// `datacount` objects of `typesize` bytes (a 3x3 complex matrix row-major).
+// The checksum: every site's file bytes get their own CRC-32 c, and with r the
+// site's lexicographic rank, suma ^= rotl(c, r mod 29), sumb ^= rotl(c, r mod 31).
+//
+// Division of work: the host scans and writes the LIME headers and XML; the
+// device converts between the library's canonical complex128 arrays
+// [nobj][site][nelem] and file bytes and computes the per-site CRCs and the
+// rank-rotated XOR partials in the same pass (one wave per 64 sites, the
+// site's words staged through LDS so that file bytes move coalesced).  The
+// binary record streams through two pinned host buffers in whole t-slices of
+// at most 64 MiB, so disk transfers of one chunk overlap the device work on
+// the next.
+#include <stdio.h>
+#include <string.h>
+#include <time.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "su3.h"
+
+namespace qh {
+
+namespace {
+
+constexpr uint32_t LIME_MAGIC = 0x456789abu;
+constexpr int LIME_HEADER = 144;
+constexpr int SITES_PER_BLOCK = 64;  // one wave, one site per lane
+constexpr size_t CHUNK_BYTES = size_t(64) << 20;
+
+// ---------------------------------------------------------------------------
+// device side
+
+__device__ __forceinline__ uint32_t rotl32(uint32_t v, int k) { return k ? (v << k) | (v >> (32 - k)) : v; }
+
+// byte table of the reflected CRC-32 polynomial 0xedb88320 (src/io/crc32.nim:8-14)
+__device__ __forceinline__ void crc_table(uint32_t *tbl) {
+  for (int e = threadIdx.x; e < 256; e += blockDim.x) {
+    uint32_t r = (uint32_t)e;
+    for (int j = 0; j < 8; j++) r = (r & 1) ? (r >> 1) ^ 0xedb88320u : r >> 1;
+    tbl[e] = r;
+  }
+}
+
+// four file bytes, w's bytes in memory order, into the CRC register
+__device__ __forceinline__ uint32_t crc_word(const uint32_t *tbl, uint32_t c, uint32_t w) {
+  c ^= w;
+  c = tbl[c & 0xff] ^ (c >> 8);
+  c = tbl[c & 0xff] ^ (c >> 8);
+  c = tbl[c & 0xff] ^ (c >> 8);
+  c = tbl[c & 0xff] ^ (c >> 8);
+  return c;
+}
+
+__device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v ^= (uint32_t)__shfl_xor((int)v, o);
+  return v;
+}
+
+__device__ __forceinline__ long canon_of_lex(const Geom &g, long lex) {
+  int c[4];
+  long r = lex;
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    c[d] = (int)(r % g.L[d]);
+    r /= g.L[d];
+  }
+  return canon_index(g, c);
+}
+
+// canonical [nobj][V][nelem] complex128 -> file words of the sites
+// lex0 .. lex0+n-1 (n counted from the chunk start), plus the block's
+// (suma, sumb) partial in part[2*blockIdx.x ..]
+template <bool SINGLE>
+__global__ void __launch_bounds__(SITES_PER_BLOCK)
+    k_scidac_pack(Geom g, const double2 *src, int nobj, int nelem, long lex0, long n, uint32_t *out, uint32_t *part) {
+  extern __shared__ uint32_t lds[];
+  uint32_t *tbl = lds, *tile = lds + 256;
+  const int W = nobj * nelem * (SINGLE ? 2 : 4);  // 32-bit words per site
+  const int ld = W + 1;                            // odd row stride: no bank aliasing between lanes
+  crc_table(tbl);
+  __syncthreads();
+  const long V = 2 * g.Vh;
+  const long s0 = (long)blockIdx.x * SITES_PER_BLOCK;
+  uint32_t a = 0, b = 0;
+  if (s0 + threadIdx.x < n) {
+    const long lex = lex0 + s0 + threadIdx.x;
+    const long idx = canon_of_lex(g, lex);
+    uint32_t *row = tile + threadIdx.x * ld;
+    uint32_t crc = 0xffffffffu;
+    int k = 0;
+    for (int o = 0; o < nobj; o++) {
+      const double2 *p = src + ((long)o * V + idx) * nelem;
+      for (int e = 0; e < nelem; e++) {
+        const double2 z = p[e];
+        if (SINGLE) {
+          const uint32_t wr = __builtin_bswap32(__float_as_uint((float)z.x));
+          const uint32_t wi = __builtin_bswap32(__float_as_uint((float)z.y));
+          row[k] = wr;
+          row[k + 1] = wi;
+          crc = crc_word(tbl, crc_word(tbl, crc, wr), wi);
+          k += 2;
+        } else {
+          const uint64_t br = __builtin_bswap64((uint64_t)__double_as_longlong(z.x));
+          const uint64_t bi = __builtin_bswap64((uint64_t)__double_as_longlong(z.y));
+          const uint32_t w0 = (uint32_t)br, w1 = (uint32_t)(br >> 32);
+          const uint32_t w2 = (uint32_t)bi, w3 = (uint32_t)(bi >> 32);
+          row[k] = w0;
+          row[k + 1] = w1;
+          row[k + 2] = w2;
+          row[k + 3] = w3;
+          crc = crc_word(tbl, crc_word(tbl, crc_word(tbl, crc_word(tbl, crc, w0), w1), w2), w3);
+          k += 4;
+        }
+      }
+    }
+    crc = ~crc;
+    a = rotl32(crc, (int)(lex % 29));
+    b = rotl32(crc, (int)(lex % 31));
+  }
+  a = wave_xor(a);
+  b = wave_xor(b);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+  __syncthreads();
+  const long nv = n - s0 < SITES_PER_BLOCK ? n - s0 : SITES_PER_BLOCK;
+  const int total = (int)nv * W;
+  uint32_t *dst = out + s0 * W;
+  for (int w = threadIdx.x; w < total; w += SITES_PER_BLOCK) {
+    const int s = w / W;
+    dst[w] = tile[s * ld + (w - s * W)];
+  }
+}
+
+// file words -> canonical complex128, with the same checksum partials
+template <bool SINGLE>
+__global__ void __launch_bounds__(SITES_PER_BLOCK)
+    k_scidac_unpack(Geom g, const uint32_t *in, int nobj, int nelem, long lex0, long n, double2 *dst, uint32_t *part) {
+  extern __shared__ uint32_t lds[];
+  uint32_t *tbl = lds, *tile = lds + 256;
+  const int W = nobj * nelem * (SINGLE ? 2 : 4);
+  const int ld = W + 1;
+  crc_table(tbl);
+  const long s0 = (long)blockIdx.x * SITES_PER_BLOCK;
+  const long nv = n - s0 < SITES_PER_BLOCK ? n - s0 : SITES_PER_BLOCK;
+  const int total = (int)nv * W;
+  const uint32_t *src = in + s0 * W;
+  for (int w = threadIdx.x; w < total; w += SITES_PER_BLOCK) {
+    const int s = w / W;
+    tile[s * ld + (w - s * W)] = src[w];
+  }
+  __syncthreads();
+  const long V = 2 * g.Vh;
+  uint32_t a = 0, b = 0;
+  if (threadIdx.x < nv) {
+    const long lex = lex0 + s0 + threadIdx.x;
+    const long idx = canon_of_lex(g, lex);
+    const uint32_t *row = tile + threadIdx.x * ld;
+    uint32_t crc = 0xffffffffu;
+    int k = 0;
+    for (int o = 0; o < nobj; o++) {
+      double2 *p = dst + ((long)o * V + idx) * nelem;
+      for (int e = 0; e < nelem; e++) {
+        double2 z;
+        if (SINGLE) {
+          const uint32_t wr = row[k], wi = row[k + 1];
+          crc = crc_word(tbl, crc_word(tbl, crc, wr), wi);
+          z.x = (double)__uint_as_float(__builtin_bswap32(wr));
+          z.y = (double)__uint_as_float(__builtin_bswap32(wi));
+          k += 2;
+        } else {
+          const uint32_t w0 = row[k], w1 = row[k + 1], w2 = row[k + 2], w3 = row[k + 3];
+          crc = crc_word(tbl, crc_word(tbl, crc_word(tbl, crc_word(tbl, crc, w0), w1), w2), w3);
+          z.x = __longlong_as_double((long long)__builtin_bswap64(((uint64_t)w1 << 32) | w0));
+          z.y = __longlong_as_double((long long)__builtin_bswap64(((uint64_t)w3 << 32) | w2));
+          k += 4;
+        }
+        p[e] = z;
+      }
+    }
+    crc = ~crc;
+    a = rotl32(crc, (int)(lex % 29));
+    b = rotl32(crc, (int)(lex % 31));
+  }
+  a = wave_xor(a);
+  b = wave_xor(b);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side: files, LIME records, XML
+
+struct File {
+  FILE *f = nullptr;
+  File(const char *path, const char *mode) : f(fopen(path, mode)) {
+    if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+  }
+  ~File() {
+    if (f) fclose(f);
+  }
+  File(const File &) = delete;
+  File &operator=(const File &) = delete;
+};
+
+void put_be(unsigned char *p, uint64_t v, int nbytes) {
+  for (int i = nbytes - 1; i >= 0; i--, v >>= 8) p[i] = (unsigned char)(v & 0xff);
+}
+
+uint64_t get_be(const unsigned char *p, int nbytes) {
+  uint64_t v = 0;
+  for (int i = 0; i < nbytes; i++) v = (v << 8) | p[i];
+  return v;
+}
+
+void fwrite_all(FILE *f, const void *p, size_t n) {
+  if (n && fwrite(p, 1, n, f) != n) throw std::runtime_error("SciDAC write: write failed");
+}
+
+void lime_header(FILE *f, const char *type, uint64_t len, bool mb, bool me) {
+  unsigned char h[LIME_HEADER];
+  memset(h, 0, sizeof h);
+  put_be(h, LIME_MAGIC, 4);
+  put_be(h + 4, 1, 2);
+  put_be(h + 6, (mb ? 0x8000u : 0u) | (me ? 0x4000u : 0u), 2);
+  put_be(h + 8, len, 8);
+  strncpy((char *)h + 16, type, 128);
+  fwrite_all(f, h, sizeof h);
+}
+
+void lime_pad(FILE *f, uint64_t len) {
+  static const unsigned char zero[8] = {0};
+  fwrite_all(f, zero, (8 - len % 8) % 8);
+}
+
+void lime_record(FILE *f, const char *type, const std::string &data, bool mb, bool me) {
+  lime_header(f, type, data.size(), mb, me);
+  fwrite_all(f, data.data(), data.size());
+  lime_pad(f, data.size());
+}
+
+struct LimeRec {
+  std::string type;
+  long long off = 0, len = 0;  // data offset and length in the file
+  bool mb = false, me = false;
+};
+
+std::vector<LimeRec> lime_scan(FILE *f) {
+  if (fseeko(f, 0, SEEK_END) != 0) throw std::runtime_error("SciDAC read: seek failed");
+  const long long size = ftello(f);
+  std::vector<LimeRec> recs;
+  long long pos = 0;
+  while (pos < size) {
+    unsigned char h[LIME_HEADER];
+    if (fseeko(f, pos, SEEK_SET) != 0 || fread(h, 1, sizeof h, f) != sizeof h)
+      throw std::runtime_error("LIME file: truncated record header");
+    if (get_be(h, 4) != LIME_MAGIC)
+      throw std::runtime_error(recs.empty() ? "not a LIME (SciDAC/ILDG) file" : "LIME file: corrupt record header");
+    LimeRec r;
+    const unsigned flags = (unsigned)get_be(h + 6, 2);
+    r.mb = (flags & 0x8000u) != 0;
+    r.me = (flags & 0x4000u) != 0;
+    const uint64_t len = get_be(h + 8, 8);
+    r.type.assign((const char *)h + 16, strnlen((const char *)h + 16, 128));
+    r.off = pos + LIME_HEADER;
+    if (len > (uint64_t)(size - r.off)) throw std::runtime_error("LIME file: truncated record '" + r.type + "'");
+    r.len = (long long)len;
+    recs.push_back(r);
+    pos = r.off + r.len + (8 - r.len % 8) % 8;
+  }
+  return recs;
+}
+
+std::string read_string(FILE *f, const LimeRec &r) {
+  if (r.len > (1ll << 30)) throw std::runtime_error("LIME file: metadata record too large");
+  std::string s((size_t)r.len, '\0');
+  if (fseeko(f, r.off, SEEK_SET) != 0 || (r.len && fread(&s[0], 1, (size_t)r.len, f) != (size_t)r.len))
+    throw std::runtime_error("LIME file: short read");
+  s.resize(strnlen(s.data(), s.size()));  // writers may include the C string's NUL
+  return s;
+}
+
+bool xml_tag(const std::string &doc, const char *tag, std::string *val) {
+  const std::string open = std::string("<") + tag + ">", close = std::string("</") + tag + ">";
+  const size_t a = doc.find(open);
+  if (a == std::string::npos) return false;
+  const size_t b = doc.find(close, a + open.size());
+  if (b == std::string::npos) return false;
+  *val = doc.substr(a + open.size(), b - a - open.size());
+  return true;
+}
+
+long xml_long(const std::string &doc, const char *tag, const char *what) {
+  std::string v;
+  if (!xml_tag(doc, tag, &v)) throw std::runtime_error(std::string(what) + ": no <" + tag + ">");
+  char *end = nullptr;
+  const long x = strtol(v.c_str(), &end, 10);
+  if (end == v.c_str()) throw std::runtime_error(std::string(what) + ": bad <" + tag + ">");
+  return x;
+}
+
+const char *XML_DECL = "<?xml version=\"1.0\" encoding=\"UTF-8\"?>";
+
+struct SciRecord {
+  std::string priv, user, ildg, date, datatype;
+  LimeRec bin;
+  bool has_ck = false;
+  uint32_t suma = 0, sumb = 0;
+  char prec = 'D';
+  int colors = 0, spins = 0, typesize = 0, datacount = 0;
+  bool ildg_bin = false;
+};
+
+struct SciFile {
+  int dims[4] = {0, 0, 0, 0};
+  std::string filemd;
+  std::vector<SciRecord> recs;
+};
+
+void finish_record(SciRecord &r) {
+  if (!r.priv.empty()) {
+    const char *w = "scidac-private-record-xml";
+    xml_tag(r.priv, "datatype", &r.datatype);
+    xml_tag(r.priv, "date", &r.date);
+    std::string p;
+    if (!xml_tag(r.priv, "precision", &p) || (p != "F" && p != "D"))
+      throw std::runtime_error("SciDAC record: precision must be F or D");
+    r.prec = p[0];
+    r.colors = (int)xml_long(r.priv, "colors", w);
+    r.spins = (int)xml_long(r.priv, "spins", w);
+    r.typesize = (int)xml_long(r.priv, "typesize", w);
+    r.datacount = (int)xml_long(r.priv, "datacount", w);
+  } else if (!r.ildg.empty()) {  // ILDG-only record: an SU(3) gauge field, four links per site
+    std::string field;
+    if (!xml_tag(r.ildg, "field", &field) || field != "su3gauge")
+      throw std::runtime_error("ILDG record: only su3gauge fields are supported");
+    const long p = xml_long(r.ildg, "precision", "ildg-format");
+    if (p != 32 && p != 64) throw std::runtime_error("ILDG record: precision must be 32 or 64");
+    r.prec = p == 32 ? 'F' : 'D';
+    r.datatype = p == 32 ? "QDP_F3_ColorMatrix" : "QDP_D3_ColorMatrix";
+    r.colors = 3;
+    r.spins = 1;
+    r.typesize = p == 32 ? 72 : 144;
+    r.datacount = 4;
+  } else {
+    throw std::runtime_error("SciDAC file: binary record without a record description");
+  }
+  if (r.typesize <= 0 || r.datacount <= 0) throw std::runtime_error("SciDAC record: bad typesize or datacount");
+}
+
+SciFile scidac_scan(FILE *f) {
+  const std::vector<LimeRec> lime = lime_scan(f);
+  SciFile sf;
+  bool have_dims = false;
+  std::string ildg_first;
+  size_t i = 0;
+  while (i < lime.size()) {
+    // one LIME message: from a record with the begin flag up to one with the end flag
+    size_t j = i;
+    while (j + 1 < lime.size() && !lime[j].me && !lime[j + 1].mb) j++;
+    SciRecord r;
+    bool has_bin = false;
+    for (size_t k = i; k <= j; k++) {
+      const LimeRec &l = lime[k];
+      if (l.type == "scidac-private-file-xml") {
+        const std::string x = read_string(f, l);
+        if (xml_long(x, "spacetime", "scidac-private-file-xml") != 4)
+          throw std::runtime_error("SciDAC file: only 4-dimensional lattices are supported");
+        std::string d;
+        if (!xml_tag(x, "dims", &d) ||
+            sscanf(d.c_str(), "%d %d %d %d", &sf.dims[0], &sf.dims[1], &sf.dims[2], &sf.dims[3]) != 4)
+          throw std::runtime_error("SciDAC file: bad <dims>");
+        have_dims = true;
+      } else if (l.type == "scidac-file-xml") {
+        sf.filemd = read_string(f, l);
+      } else if (l.type == "scidac-private-record-xml") {
+        r.priv = read_string(f, l);
+      } else if (l.type == "scidac-record-xml") {
+        r.user = read_string(f, l);
+      } else if (l.type == "ildg-format") {
+        r.ildg = read_string(f, l);
+        if (ildg_first.empty()) ildg_first = r.ildg;
+      } else if (l.type == "scidac-binary-data" || l.type == "ildg-binary-data") {
+        if (has_bin) throw std::runtime_error("SciDAC file: two binary records in one message");
+        r.bin = l;
+        r.ildg_bin = l.type == "ildg-binary-data";
+        has_bin = true;
+      } else if (l.type == "scidac-checksum") {
+        const std::string x = read_string(f, l);
+        std::string a, b;
+        if (!xml_tag(x, "suma", &a) || !xml_tag(x, "sumb", &b))
+          throw std::runtime_error("SciDAC file: bad scidac-checksum record");
+        r.suma = (uint32_t)strtoul(a.c_str(), nullptr, 16);
+        r.sumb = (uint32_t)strtoul(b.c_str(), nullptr, 16);
+        r.has_ck = true;
+      }  // other record types (ildg-data-lfn, ...) carry nothing needed here
+    }
+    if (has_bin) {
+      finish_record(r);
+      sf.recs.push_back(r);
+    }
+    i = j + 1;
+  }
+  if (!have_dims) {
+    if (ildg_first.empty()) throw std::runtime_error("SciDAC file: no lattice dimensions");
+    const char *tags[4] = {"lx", "ly", "lz", "lt"};
+    for (int d = 0; d < 4; d++) sf.dims[d] = (int)xml_long(ildg_first, tags[d], "ildg-format");
+  }
+  long long V = 1;
+  for (int d = 0; d < 4; d++) {
+    if (sf.dims[d] <= 0) throw std::runtime_error("SciDAC file: bad lattice dimensions");
+    V *= sf.dims[d];
+  }
+  for (const SciRecord &r : sf.recs)
+    if (r.bin.len != V * r.datacount * (long long)r.typesize)
+      throw std::runtime_error("SciDAC file: binary record length does not match lattice, typesize and datacount");
+  return sf;
+}
+
+void info_of(const SciFile &sf, int record, ScidacInfo *info) {
+  for (int d = 0; d < 4; d++) info->dims[d] = sf.dims[d];
+  info->nrecords = (int)sf.recs.size();
+  info->filemd = sf.filemd;
+  if (record < 0 || record >= (int)sf.recs.size()) return;
+  const SciRecord &r = sf.recs[record];
+  info->recmd = r.user;
+  info->datatype = r.datatype;
+  info->date = r.date;
+  info->prec = r.prec;
+  info->colors = r.colors;
+  info->spins = r.spins;
+  info->typesize = r.typesize;
+  info->datacount = r.datacount;
+  info->ildg = r.ildg_bin ? 1 : 0;
+  info->has_checksum = r.has_ck ? 1 : 0;
+  info->suma = r.suma;
+  info->sumb = r.sumb;
+}
+
+std::string type_name(int nelem, char prec) {
+  switch (nelem) {
+    case 9: return std::string("QDP_") + prec + "3_ColorMatrix";
+    case 3: return std::string("QDP_") + prec + "3_ColorVector";
+    default: return std::string("QDP_") + prec + "_Complex";
+  }
+}
+
+std::string now_utc() {
+  const time_t t = time(nullptr);
+  struct tm tmv;
+  gmtime_r(&t, &tmv);
+  char buf[64];
+  strftime(buf, sizeof buf, "%a %b %e %H:%M:%S %Y UTC", &tmv);
+  return buf;
+}
+
+// device and pinned buffers of the two-stage chunk pipeline
+struct Pipe {
+  hipStream_t st;
+  uint32_t *dev[2] = {nullptr, nullptr};
+  uint32_t *host[2] = {nullptr, nullptr};
+  uint32_t *part = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  Pipe(hipStream_t s, size_t chunk_bytes, long nblocks) : st(s) {
+    for (int b = 0; b < 2; b++) {
+      if (hipMalloc((void **)&dev[b], chunk_bytes) != hipSuccess ||
+          hipHostMalloc((void **)&host[b], chunk_bytes, hipHostMallocDefault) != hipSuccess ||
+          hipEventCreateWithFlags(&ev[b], hipEventDisableTiming) != hipSuccess)
+        throw std::runtime_error("SciDAC I/O: out of memory for the transfer buffers");
+    }
+    if (hipMalloc((void **)&part, (size_t)nblocks * 2 * sizeof(uint32_t)) != hipSuccess)
+      throw std::runtime_error("SciDAC I/O: out of device memory");
+  }
+  ~Pipe() {
+    (void)hipStreamSynchronize(st);  // nothing may still be reading or writing the buffers
+    for (int b = 0; b < 2; b++) {
+      if (dev[b]) (void)hipFree(dev[b]);
+      if (host[b]) (void)hipHostFree(host[b]);
+      if (ev[b]) (void)hipEventDestroy(ev[b]);
+    }
+    if (part) (void)hipFree(part);
+  }
+  Pipe(const Pipe &) = delete;
+  Pipe &operator=(const Pipe &) = delete;
+};
+
+struct Chunking {
+  long slab, per, nchunks, nblocks;  // sites per t-slice and per chunk, chunks, blocks in all
+  size_t site_bytes;
+  Chunking(const Geom &g, size_t sb) : site_bytes(sb) {
+    slab = (long)g.L[0] * g.L[1] * g.L[2];
+    long nt = (long)(CHUNK_BYTES / ((size_t)slab * sb));
+    if (nt < 1) nt = 1;
+    if (nt > g.L[3]) nt = g.L[3];
+    per = nt * slab;
+    nchunks = (g.L[3] + nt - 1) / nt;
+    nblocks = 0;
+    for (long c = 0; c < nchunks; c++) nblocks += (sites(g, c) + SITES_PER_BLOCK - 1) / SITES_PER_BLOCK;
+  }
+  long sites(const Geom &g, long c) const {
+    const long V = 2 * g.Vh, s = V - c * per;
+    return s < per ? s : per;
+  }
+};
+
+size_t lds_bytes(int nobj, int nelem, bool single) {
+  const int W = nobj * nelem * (single ? 2 : 4);
+  return (256 + (size_t)SITES_PER_BLOCK * (W + 1)) * sizeof(uint32_t);
+}
+
+void xor_partials(const Pipe &p, long nblocks, uint32_t *suma, uint32_t *sumb) {
+  std::vector<uint32_t> h((size_t)nblocks * 2);
+  if (hipMemcpy(h.data(), p.part, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+    throw std::runtime_error("SciDAC I/O: checksum copy failed");
+  uint32_t a = 0, b = 0;
+  for (long k = 0; k < nblocks; k++) {
+    a ^= h[2 * k];
+    b ^= h[2 * k + 1];
+  }
+  *suma = a;
+  *sumb = b;
+}
+
+void check_shape(int nobj, int nelem) {
+  if (nelem != 1 && nelem != 3 && nelem != 9) throw std::runtime_error("SciDAC I/O: nelem must be 1, 3 or 9");
+  if (nobj < 1 || nobj > 4) throw std::runtime_error("SciDAC I/O: nobj must be 1 to 4");
+}
+
+}  // namespace
+
+void scidac_info(const char *path, int record, ScidacInfo *info) {
+  File f(path, "rb");
+  const SciFile sf = scidac_scan(f.f);
+  if (record < 0 || record >= (int)sf.recs.size()) {
+    if (!(record == 0 && sf.recs.empty())) throw std::runtime_error("SciDAC file: record index out of range");
+  }
+  info_of(sf, record, info);
+}
+
+void scidac_read(const Geom &g, const char *path, int record, double *dst, int nobj, int nelem, ScidacInfo *info,
+                 hipStream_t st) {
+  check_shape(nobj, nelem);
+  File f(path, "rb");
+  const SciFile sf = scidac_scan(f.f);
+  if (record < 0 || record >= (int)sf.recs.size()) throw std::runtime_error("SciDAC file: record index out of range");
+  info_of(sf, record, info);
+  for (int d = 0; d < 4; d++)
+    if (sf.dims[d] != g.L[d]) throw std::runtime_error("SciDAC file: field lattice differs from the file's lattice");
+  const SciRecord &r = sf.recs[record];
+  if (r.datacount != nobj) throw std::runtime_error("SciDAC file: field count differs from the record's datacount");
+  const bool single = r.prec == 'F';
+  if (r.typesize != nelem * 2 * (single ? 4 : 8))
+    throw std::runtime_error("SciDAC file: field data elements differ from the record's typesize");
+  const Chunking ch(g, (size_t)nobj * r.typesize);
+  Pipe p(st, (size_t)ch.per * ch.site_bytes, ch.nblocks);
+  if (fseeko(f.f, r.bin.off, SEEK_SET) != 0) throw std::runtime_error("SciDAC read: seek failed");
+  const size_t lds = lds_bytes(nobj, nelem, single);
+  long blk = 0;
+  for (long c = 0; c < ch.nchunks; c++) {
+    const int b = (int)(c & 1);
+    const long n = ch.sites(g, c);
+    const size_t bytes = (size_t)n * ch.site_bytes;
+    if (c >= 2 && hipEventSynchronize(p.ev[b]) != hipSuccess) throw std::runtime_error("SciDAC read: device error");
+    if (fread(p.host[b], 1, bytes, f.f) != bytes) throw std::runtime_error("SciDAC read: short read");
+    if (hipMemcpyAsync(p.dev[b], p.host[b], bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+      throw std::runtime_error("SciDAC read: copy to device failed");
+    const unsigned grid = (unsigned)((n + SITES_PER_BLOCK - 1) / SITES_PER_BLOCK);
+    if (single)
+      hipLaunchKernelGGL(k_scidac_unpack<true>, dim3(grid), dim3(SITES_PER_BLOCK), lds, st, g, p.dev[b], nobj, nelem,
+                         c * ch.per, n, (double2 *)dst, p.part + 2 * blk);
+    else
+      hipLaunchKernelGGL(k_scidac_unpack<false>, dim3(grid), dim3(SITES_PER_BLOCK), lds, st, g, p.dev[b], nobj, nelem,
+                         c * ch.per, n, (double2 *)dst, p.part + 2 * blk);
+    if (hipGetLastError() != hipSuccess || hipEventRecord(p.ev[b], st) != hipSuccess)
+      throw std::runtime_error("SciDAC read: kernel launch failed");
+    blk += grid;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("SciDAC read: device error");
+  xor_partials(p, ch.nblocks, &info->calc_suma, &info->calc_sumb);
+  info->checksum_ok = r.has_ck ? (info->calc_suma == r.suma && info->calc_sumb == r.sumb) : 1;
+  if (!info->checksum_ok) throw std::runtime_error("SciDAC file: checksum mismatch");
+}
+
+void scidac_create(const char *path, const int dims[4], const char *filemd) {
+  for (int d = 0; d < 4; d++)
+    if (dims[d] <= 0) throw std::runtime_error("SciDAC write: bad lattice dimensions");
+  char priv[512];
+  snprintf(priv, sizeof priv,
+           "%s<scidacFile><version>1.1</version><spacetime>4</spacetime><dims>%d %d %d %d </dims>"
+           "<volfmt>0</volfmt></scidacFile>",
+           XML_DECL, dims[0], dims[1], dims[2], dims[3]);
+  File f(path, "wb");
+  lime_record(f.f, "scidac-private-file-xml", priv, true, false);
+  lime_record(f.f, "scidac-file-xml", filemd ? filemd : "", false, true);
+  if (fflush(f.f) != 0) throw std::runtime_error("SciDAC write: write failed");
+}
+
+void scidac_append(const Geom &g, const char *path, const char *recmd, const double *src, int nobj, int nelem,
+                   char prec, int ildg, const char *date, ScidacInfo *info, hipStream_t st) {
+  check_shape(nobj, nelem);
+  if (prec != 'F' && prec != 'D') throw std::runtime_error("SciDAC write: precision must be F or D");
+  if (ildg && !(nobj == 4 && nelem == 9)) throw std::runtime_error("SciDAC write: ILDG records hold gauge fields only");
+  File f(path, "r+b");
+  const SciFile sf = scidac_scan(f.f);
+  for (int d = 0; d < 4; d++)
+    if (sf.dims[d] != g.L[d]) throw std::runtime_error("SciDAC write: field lattice differs from the file's lattice");
+  const bool single = prec == 'F';
+  const int typesize = nelem * 2 * (single ? 4 : 8);
+  const std::string when = date ? std::string(date) : now_utc();
+  const std::string dtype = type_name(nelem, prec);
+  char priv[1024];
+  snprintf(priv, sizeof priv,
+           "%s<scidacRecord><version>1.1</version><date>%s</date><recordtype>0</recordtype>"
+           "<datatype>%s</datatype><precision>%c</precision><colors>%d</colors><spins>1</spins>"
+           "<typesize>%d</typesize><datacount>%d</datacount></scidacRecord>",
+           XML_DECL, when.c_str(), dtype.c_str(), prec, nelem > 1 ? 3 : 1, typesize, nobj);
+  if (fseeko(f.f, 0, SEEK_END) != 0) throw std::runtime_error("SciDAC write: seek failed");
+  lime_record(f.f, "scidac-private-record-xml", priv, true, false);
+  lime_record(f.f, "scidac-record-xml", recmd ? recmd : "", false, false);
+  if (ildg) {
+    char fmt[768];
+    snprintf(fmt, sizeof fmt,
+             "%s<ildgFormat xmlns=\"http://www.lqcd.org/ildg\" xmlns:xsi=\"http://www.w3.org/2001/XMLSchema-instance\" "
+             "xsi:schemaLocation=\"http://www.lqcd.org/ildg http://www.lqcd.org/ildg/filefmt.xsd\">"
+             "<version>1.0</version><field>su3gauge</field><precision>%d</precision>"
+             "<lx>%d</lx><ly>%d</ly><lz>%d</lz><lt>%d</lt></ildgFormat>",
+             XML_DECL, single ? 32 : 64, g.L[0], g.L[1], g.L[2], g.L[3]);
+    lime_record(f.f, "ildg-format", fmt, false, false);
+  }
+  const long V = 2 * g.Vh;
+  const Chunking ch(g, (size_t)nobj * typesize);
+  const uint64_t total = (uint64_t)V * ch.site_bytes;
+  lime_header(f.f, ildg ? "ildg-binary-data" : "scidac-binary-data", total, false, false);
+  Pipe p(st, (size_t)ch.per * ch.site_bytes, ch.nblocks);
+  const size_t lds = lds_bytes(nobj, nelem, single);
+  std::vector<size_t> pending(ch.nchunks, 0);
+  long blk = 0;
+  auto drain = [&](long c) {  // write chunk c's bytes once its copy to the host is done
+    const int b = (int)(c & 1);
+    if (hipEventSynchronize(p.ev[b]) != hipSuccess) throw std::runtime_error("SciDAC write: device error");
+    fwrite_all(f.f, p.host[b], pending[c]);
+  };
+  for (long c = 0; c < ch.nchunks; c++) {
+    const int b = (int)(c & 1);
+    if (c >= 2) drain(c - 2);
+    const long n = ch.sites(g, c);
+    pending[c] = (size_t)n * ch.site_bytes;
+    const unsigned grid = (unsigned)((n + SITES_PER_BLOCK - 1) / SITES_PER_BLOCK);
+    if (single)
+      hipLaunchKernelGGL(k_scidac_pack<true>, dim3(grid), dim3(SITES_PER_BLOCK), lds, st, g, (const double2 *)src,
+                         nobj, nelem, c * ch.per, n, p.dev[b], p.part + 2 * blk);
+    else
+      hipLaunchKernelGGL(k_scidac_pack<false>, dim3(grid), dim3(SITES_PER_BLOCK), lds, st, g, (const double2 *)src,
+                         nobj, nelem, c * ch.per, n, p.dev[b], p.part + 2 * blk);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(p.host[b], p.dev[b], pending[c], hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipEventRecord(p.ev[b], st) != hipSuccess)
+      throw std::runtime_error("SciDAC write: conversion failed");
+    blk += grid;
+  }
+  for (long c = ch.nchunks >= 2 ? ch.nchunks - 2 : 0; c < ch.nchunks; c++) drain(c);
+  lime_pad(f.f, total);
+  uint32_t a, b;
+  xor_partials(p, ch.nblocks, &a, &b);
+  char ck[256];
+  snprintf(ck, sizeof ck,
+           "%s<scidacChecksum><version>1.0</version><suma>%x</suma><sumb>%x</sumb></scidacChecksum>", XML_DECL, a,
+           b);
+  lime_record(f.f, "scidac-checksum", ck, false, true);
+  if (fflush(f.f) != 0) throw std::runtime_error("SciDAC write: write failed");
+  if (info) {
+    for (int d = 0; d < 4; d++) info->dims[d] = g.L[d];
+    info->nrecords = (int)sf.recs.size() + 1;
+    info->filemd = sf.filemd;
+    info->recmd = recmd ? recmd : "";
+    info->datatype = dtype;
+    info->date = when;
+    info->prec = prec;
+    info->colors = nelem > 1 ? 3 : 1;
+    info->spins = 1;
+    info->typesize = typesize;
+    info->datacount = nobj;
+    info->ildg = ildg ? 1 : 0;
+    info->has_checksum = 1;
+    info->suma = info->calc_suma = a;
+    info->sumb = info->calc_sumb = b;
+    info->checksum_ok = 1;
+  }
+}
+
+}  // namespace qh
