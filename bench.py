@@ -53,6 +53,7 @@ def parse():
                     help="add the HISQ-form 32^3x64 m=0.005 storage-precision study")
     ap.add_argument("--batched", action="store_true",
                     help="add the batched-solve study (multi-shift x multi-RHS, uneven CG batch)")
+    ap.add_argument("--gauge-io", action="store_true", help="only the SciDAC gauge-file study")
     ap.add_argument("--hisq-links", action="store_true",
                     help="only run the HISQ link-construction study (32^3x64 smearing)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -310,6 +311,56 @@ def hisq_links_study(cpu=True):
     return res
 
 
+def gauge_io_study():
+    """SciDAC gauge files of a random 32^3x64 field: device time per launch of
+    the conversion + checksum kernels (pack on writing, unpack on reading)
+    with their algorithmic GB/s (canonical FP64 links 576 B/site in or out,
+    file bytes 576 B/site FP64 or 288 B/site FP32), and the end-to-end
+    saveGauge / loadGauge rates through a temporary file (these include the
+    host file system and PCIe; they are not kernel figures)."""
+    import shutil
+    import tempfile
+    import torch
+    import qex_amd as Q
+    from qex_amd import fileio, gauge
+    lo = Q.Layout(SMEAR_LAT)
+    V = lo.nSites
+    U = gauge.random_gauge(lo, SEED)
+    res = {"lattice": list(SMEAR_LAT)}
+    tmp = tempfile.mkdtemp()
+    try:
+        for prec in ("D", "F"):
+            fb = 4 * 9 * (16 if prec == "D" else 8)
+            pk, up = fileio.time_io_kernels(lo, U, prec, nrep=5)
+            r = {"file_bytes_per_site": fb}
+            for name, ms in (("k_scidac_pack", pk), ("k_scidac_unpack", up)):
+                gbs = (576 + fb) * V / (ms * 1e-3) / 1e9
+                r[name] = {"ms": round(ms, 4), "GB_s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_GBS, 4)}
+            fn = os.path.join(tmp, "g.lime")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fileio.saveGauge(lo, U, fn, prec)
+            t1 = time.perf_counter()
+            G = torch.empty_like(U)
+            fileio.loadGauge(lo, G, fn)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            size = os.path.getsize(fn)
+            want = U if prec == "D" else U.to(torch.complex64).to(torch.complex128)
+            r["file_GB"] = round(size / 1e9, 3)
+            r["saveGauge_GB_s"] = round(size / (t1 - t0) / 1e9, 2)
+            r["loadGauge_GB_s"] = round(size / (t2 - t1) / 1e9, 2)
+            r["roundtrip_exact"] = bool(torch.equal(G, want))
+            os.unlink(fn)
+            del G, want
+            res["FP64" if prec == "D" else "FP32"] = r
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    del U
+    torch.cuda.empty_cache()
+    return res
+
+
 def batched_study():
     """Batched solves against one-at-a-time solves, FP64, |r|/|b| <= 1e-10.
 
@@ -459,6 +510,9 @@ def main():
     if args.hisq_links:
         print(json.dumps({"hisq_links": hisq_links_study(cpu=not args.no_cpu)}), flush=True)
         return
+    if args.gauge_io:
+        print(json.dumps({"gauge_io": gauge_io_study()}), flush=True)
+        return
 
     lo = Q.Layout(LAT)
     vh = lo.nEven
@@ -596,6 +650,7 @@ def main():
 
     if not args.no_table:
         out["hisq_links"] = hisq_links_study(cpu=not args.no_cpu)
+        out["gauge_io"] = gauge_io_study()
 
     if args.hisq_cg or args.batched:
         del s

@@ -314,6 +314,10 @@ int qexhip_scidac_create(const char *path, const int dims[4], const char *filemd
    means the current UTC time.  info (may be NULL) receives the checksums. */
 int qexhip_scidac_append(qexhip_lattice *lat, const char *path, const char *recmd, const double *data_dev, int nobj,
                          int nelem, char precision, int ildg, const char *date, qexhip_scidac_info *info);
+/* timing: ms[0], ms[1] = device time per whole-lattice launch of the
+   conversion-plus-checksum kernels of Writer.write and Reader.read (no file) */
+int qexhip_time_scidac(qexhip_lattice *lat, const double *data_dev, int nobj, int nelem, char precision, int nrep,
+                       double *ms);
 
 int qexhip_gen_gauge(qexhip_lattice *lat, uint64_t seed, double *U_dev);
 int qexhip_gen_source(qexhip_lattice *lat, uint64_t seed, int stream, int par, double *x_dev);

@@ -2000,6 +2000,15 @@ int qexhip_scidac_append(qexhip_lattice *lat, const char *path, const char *recm
   API_END
 }
 
+int qexhip_time_scidac(qexhip_lattice *lat, const double *data, int nobj, int nelem, char precision, int nrep,
+                       double *ms) {
+  API_BEGIN
+  check_init();
+  require(lat && data && ms, "null argument");
+  scidac_time(lat->g, data, nobj, nelem, precision, nrep, ms, g_stream);
+  API_END
+}
+
 int qexhip_links_recon_ok(qexhip_links *l, int *ok) {
   API_BEGIN
   check_init();

@@ -45,26 +45,53 @@ constexpr size_t CHUNK_BYTES = size_t(64) << 20;
 
 // ---------------------------------------------------------------------------
 // device side
+//
+// A block holds 64 consecutive sites and Q = 4 (or 2) neighbouring lanes per
+// site, so that the lanes of a site load and store neighbouring complex
+// numbers of that site together.  Lane (s, k) converts site s's
+// complex numbers q = k, k+Q, ... into the site's row of the LDS tile, then
+// runs the CRC register over words [k W/Q, (k+1) W/Q) of that row with
+// slicing-by-4 tables (four independent lookups per 32-bit word).  Its
+// register r_k is moved to the end of the site by the GF(2)-linear map Z^(Q-1-k),
+// Z = the register update over W/Q*4 zero bytes (the same combination as
+// src/io/crc32.nim:78-91 zeroPadCrc32): crc = ~(Z^3 r_0 ^ Z^2 r_1 ^ Z r_2 ^ r_3)
+// with r_0 started from ~0 and the others from 0.  Z^1..Z^3 come in as 32
+// columns each, built on the host.
+
+constexpr int MAXQ = 4;
 
 __device__ __forceinline__ uint32_t rotl32(uint32_t v, int k) { return k ? (v << k) | (v >> (32 - k)) : v; }
 
-// byte table of the reflected CRC-32 polynomial 0xedb88320 (src/io/crc32.nim:8-14)
-__device__ __forceinline__ void crc_table(uint32_t *tbl) {
+// T[0] = byte table of the reflected CRC-32 polynomial 0xedb88320
+// (src/io/crc32.nim:8-14); T[j][i] = (T[j-1][i] >> 8) ^ T[0][T[j-1][i] & 0xff]
+__device__ __forceinline__ void crc_tables(uint32_t *T) {
   for (int e = threadIdx.x; e < 256; e += blockDim.x) {
     uint32_t r = (uint32_t)e;
     for (int j = 0; j < 8; j++) r = (r & 1) ? (r >> 1) ^ 0xedb88320u : r >> 1;
-    tbl[e] = r;
+    T[e] = r;
+  }
+  __syncthreads();
+  for (int j = 1; j < 4; j++) {
+    for (int e = threadIdx.x; e < 256; e += blockDim.x) {
+      const uint32_t p = T[(j - 1) * 256 + e];
+      T[j * 256 + e] = (p >> 8) ^ T[p & 0xff];
+    }
+    __syncthreads();
   }
 }
 
-// four file bytes, w's bytes in memory order, into the CRC register
-__device__ __forceinline__ uint32_t crc_word(const uint32_t *tbl, uint32_t c, uint32_t w) {
+// four file bytes (w's bytes in memory order) through the CRC register
+__device__ __forceinline__ uint32_t crc_word(const uint32_t *T, uint32_t c, uint32_t w) {
   c ^= w;
-  c = tbl[c & 0xff] ^ (c >> 8);
-  c = tbl[c & 0xff] ^ (c >> 8);
-  c = tbl[c & 0xff] ^ (c >> 8);
-  c = tbl[c & 0xff] ^ (c >> 8);
-  return c;
+  return T[768 + (c & 0xff)] ^ T[512 + ((c >> 8) & 0xff)] ^ T[256 + ((c >> 16) & 0xff)] ^ T[c >> 24];
+}
+
+// M (32 columns) applied to c over GF(2)
+__device__ __forceinline__ uint32_t gf2_apply(const uint32_t *M, uint32_t c) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int i = 0; i < 32; i++) o ^= M[i] & (0u - ((c >> i) & 1u));
+  return o;
 }
 
 __device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
@@ -84,128 +111,144 @@ __device__ __forceinline__ long canon_of_lex(const Geom &g, long lex) {
   return canon_index(g, c);
 }
 
-// canonical [nobj][V][nelem] complex128 -> file words of the sites
-// lex0 .. lex0+n-1 (n counted from the chunk start), plus the block's
-// (suma, sumb) partial in part[2*blockIdx.x ..]
-template <bool SINGLE>
-__global__ void __launch_bounds__(SITES_PER_BLOCK)
-    k_scidac_pack(Geom g, const double2 *src, int nobj, int nelem, long lex0, long n, uint32_t *out, uint32_t *part) {
-  extern __shared__ uint32_t lds[];
-  uint32_t *tbl = lds, *tile = lds + 256;
-  const int W = nobj * nelem * (SINGLE ? 2 : 4);  // 32-bit words per site
-  const int ld = W + 1;                            // odd row stride: no bank aliasing between lanes
-  crc_table(tbl);
-  __syncthreads();
-  const long V = 2 * g.Vh;
-  const long s0 = (long)blockIdx.x * SITES_PER_BLOCK;
+// LDS: 4 CRC tables, the Z^1..Z^3 columns, the tile (64 rows of W+1 words,
+// an odd stride so lanes of one wave hit distinct banks), Q x 64 registers
+struct Smem {
+  uint32_t *T, *Z, *tile, *comb;
+  int W, ld;
+  __device__ Smem(uint32_t *lds, int w) : W(w), ld(w + 1) {
+    T = lds;
+    Z = lds + 1024;
+    tile = Z + 96;
+    comb = tile + SITES_PER_BLOCK * ld;
+  }
+};
+
+__device__ __forceinline__ void load_shift(uint32_t *Z, const uint32_t *zmat) {
+  for (int i = threadIdx.x; i < 96; i += blockDim.x) Z[i] = zmat[i];
+}
+
+// CRC of lane (s, k)'s part of row `row`, moved to the end of the site
+__device__ __forceinline__ uint32_t part_crc(const Smem &m, const uint32_t *row, int k, int Q) {
+  const int wq = m.W / Q;
+  uint32_t r = k == 0 ? 0xffffffffu : 0u;
+  for (int w = k * wq; w < (k + 1) * wq; w++) r = crc_word(m.T, r, row[w]);
+  const int sh = Q - 1 - k;
+  return sh > 0 ? gf2_apply(m.Z + 32 * (sh - 1), r) : r;
+}
+
+// wave 0: combine the Q registers of every site, rotate by the site's rank,
+// XOR over the block into part[2*blockIdx.x ..]
+__device__ __forceinline__ void block_checksum(const Smem &m, int Q, long lex_first, long nv, uint32_t *part) {
+  if (threadIdx.x >= SITES_PER_BLOCK) return;
+  const int s = threadIdx.x;
   uint32_t a = 0, b = 0;
-  if (s0 + threadIdx.x < n) {
-    const long lex = lex0 + s0 + threadIdx.x;
-    const long idx = canon_of_lex(g, lex);
-    uint32_t *row = tile + threadIdx.x * ld;
-    uint32_t crc = 0xffffffffu;
-    int k = 0;
-    for (int o = 0; o < nobj; o++) {
-      const double2 *p = src + ((long)o * V + idx) * nelem;
-      for (int e = 0; e < nelem; e++) {
-        const double2 z = p[e];
-        if (SINGLE) {
-          const uint32_t wr = __builtin_bswap32(__float_as_uint((float)z.x));
-          const uint32_t wi = __builtin_bswap32(__float_as_uint((float)z.y));
-          row[k] = wr;
-          row[k + 1] = wi;
-          crc = crc_word(tbl, crc_word(tbl, crc, wr), wi);
-          k += 2;
-        } else {
-          const uint64_t br = __builtin_bswap64((uint64_t)__double_as_longlong(z.x));
-          const uint64_t bi = __builtin_bswap64((uint64_t)__double_as_longlong(z.y));
-          const uint32_t w0 = (uint32_t)br, w1 = (uint32_t)(br >> 32);
-          const uint32_t w2 = (uint32_t)bi, w3 = (uint32_t)(bi >> 32);
-          row[k] = w0;
-          row[k + 1] = w1;
-          row[k + 2] = w2;
-          row[k + 3] = w3;
-          crc = crc_word(tbl, crc_word(tbl, crc_word(tbl, crc_word(tbl, crc, w0), w1), w2), w3);
-          k += 4;
-        }
-      }
-    }
-    crc = ~crc;
-    a = rotl32(crc, (int)(lex % 29));
-    b = rotl32(crc, (int)(lex % 31));
+  if (s < nv) {
+    uint32_t c = 0;
+    for (int k = 0; k < Q; k++) c ^= m.comb[k * SITES_PER_BLOCK + s];
+    c = ~c;
+    const long lex = lex_first + s;
+    a = rotl32(c, (int)(lex % 29));
+    b = rotl32(c, (int)(lex % 31));
   }
   a = wave_xor(a);
   b = wave_xor(b);
-  if (threadIdx.x == 0) {
+  if (s == 0) {
     part[2 * blockIdx.x] = a;
     part[2 * blockIdx.x + 1] = b;
   }
-  __syncthreads();
+}
+
+// canonical [nobj][V][nelem] complex128 -> file words of the sites
+// lex0 .. lex0+n-1 (n counted from the chunk start), and the block's
+// (suma, sumb) partial; blockDim = 64 Q
+template <bool SINGLE>
+__global__ void __launch_bounds__(SITES_PER_BLOCK *MAXQ)
+    k_scidac_pack(Geom g, const double2 *src, int nobj, int nelem, long lex0, long n, uint32_t *out, uint32_t *part,
+                  const uint32_t *zmat) {
+  extern __shared__ uint32_t lds[];
+  constexpr int WPC = SINGLE ? 2 : 4;  // words per complex number
+  const int N = nobj * nelem;
+  const Smem m(lds, N * WPC);
+  const int Q = blockDim.x / SITES_PER_BLOCK;
+  const int s = threadIdx.x / Q, k = threadIdx.x % Q;  // a site's Q lanes side by side in one wave
+  load_shift(m.Z, zmat);
+  crc_tables(m.T);
+  const long V = 2 * g.Vh;
+  const long s0 = (long)blockIdx.x * SITES_PER_BLOCK;
   const long nv = n - s0 < SITES_PER_BLOCK ? n - s0 : SITES_PER_BLOCK;
-  const int total = (int)nv * W;
-  uint32_t *dst = out + s0 * W;
-  for (int w = threadIdx.x; w < total; w += SITES_PER_BLOCK) {
-    const int s = w / W;
-    dst[w] = tile[s * ld + (w - s * W)];
+  uint32_t *row = m.tile + s * m.ld;
+  if (s < nv) {
+    const long idx = canon_of_lex(g, lex0 + s0 + s);
+    for (int q = k; q < N; q += Q) {
+      const int o = q / nelem, e = q - o * nelem;
+      const double2 z = src[((long)o * V + idx) * nelem + e];
+      if (SINGLE) {
+        row[2 * q] = __builtin_bswap32(__float_as_uint((float)z.x));
+        row[2 * q + 1] = __builtin_bswap32(__float_as_uint((float)z.y));
+      } else {
+        const uint64_t br = __builtin_bswap64((uint64_t)__double_as_longlong(z.x));
+        const uint64_t bi = __builtin_bswap64((uint64_t)__double_as_longlong(z.y));
+        row[4 * q] = (uint32_t)br;
+        row[4 * q + 1] = (uint32_t)(br >> 32);
+        row[4 * q + 2] = (uint32_t)bi;
+        row[4 * q + 3] = (uint32_t)(bi >> 32);
+      }
+    }
+  }
+  __syncthreads();
+  m.comb[k * SITES_PER_BLOCK + s] = s < nv ? part_crc(m, row, k, Q) : 0u;
+  __syncthreads();
+  block_checksum(m, Q, lex0 + s0, nv, part);
+  const int total = (int)nv * m.W;
+  uint32_t *dst = out + s0 * m.W;
+  for (int w = threadIdx.x; w < total; w += blockDim.x) {
+    const int r = w / m.W;
+    dst[w] = m.tile[r * m.ld + (w - r * m.W)];
   }
 }
 
 // file words -> canonical complex128, with the same checksum partials
 template <bool SINGLE>
-__global__ void __launch_bounds__(SITES_PER_BLOCK)
-    k_scidac_unpack(Geom g, const uint32_t *in, int nobj, int nelem, long lex0, long n, double2 *dst, uint32_t *part) {
+__global__ void __launch_bounds__(SITES_PER_BLOCK *MAXQ)
+    k_scidac_unpack(Geom g, const uint32_t *in, int nobj, int nelem, long lex0, long n, double2 *dst, uint32_t *part,
+                    const uint32_t *zmat) {
   extern __shared__ uint32_t lds[];
-  uint32_t *tbl = lds, *tile = lds + 256;
-  const int W = nobj * nelem * (SINGLE ? 2 : 4);
-  const int ld = W + 1;
-  crc_table(tbl);
+  constexpr int WPC = SINGLE ? 2 : 4;
+  const int N = nobj * nelem;
+  const Smem m(lds, N * WPC);
+  const int Q = blockDim.x / SITES_PER_BLOCK;
+  const int s = threadIdx.x / Q, k = threadIdx.x % Q;  // a site's Q lanes side by side in one wave
+  load_shift(m.Z, zmat);
   const long s0 = (long)blockIdx.x * SITES_PER_BLOCK;
   const long nv = n - s0 < SITES_PER_BLOCK ? n - s0 : SITES_PER_BLOCK;
-  const int total = (int)nv * W;
-  const uint32_t *src = in + s0 * W;
-  for (int w = threadIdx.x; w < total; w += SITES_PER_BLOCK) {
-    const int s = w / W;
-    tile[s * ld + (w - s * W)] = src[w];
+  const int total = (int)nv * m.W;
+  const uint32_t *src = in + s0 * m.W;
+  for (int w = threadIdx.x; w < total; w += blockDim.x) {
+    const int r = w / m.W;
+    m.tile[r * m.ld + (w - r * m.W)] = src[w];
   }
-  __syncthreads();
+  crc_tables(m.T);  // ends with a barrier, after which the tile is complete
   const long V = 2 * g.Vh;
-  uint32_t a = 0, b = 0;
-  if (threadIdx.x < nv) {
-    const long lex = lex0 + s0 + threadIdx.x;
-    const long idx = canon_of_lex(g, lex);
-    const uint32_t *row = tile + threadIdx.x * ld;
-    uint32_t crc = 0xffffffffu;
-    int k = 0;
-    for (int o = 0; o < nobj; o++) {
-      double2 *p = dst + ((long)o * V + idx) * nelem;
-      for (int e = 0; e < nelem; e++) {
-        double2 z;
-        if (SINGLE) {
-          const uint32_t wr = row[k], wi = row[k + 1];
-          crc = crc_word(tbl, crc_word(tbl, crc, wr), wi);
-          z.x = (double)__uint_as_float(__builtin_bswap32(wr));
-          z.y = (double)__uint_as_float(__builtin_bswap32(wi));
-          k += 2;
-        } else {
-          const uint32_t w0 = row[k], w1 = row[k + 1], w2 = row[k + 2], w3 = row[k + 3];
-          crc = crc_word(tbl, crc_word(tbl, crc_word(tbl, crc_word(tbl, crc, w0), w1), w2), w3);
-          z.x = __longlong_as_double((long long)__builtin_bswap64(((uint64_t)w1 << 32) | w0));
-          z.y = __longlong_as_double((long long)__builtin_bswap64(((uint64_t)w3 << 32) | w2));
-          k += 4;
-        }
-        p[e] = z;
+  const uint32_t *row = m.tile + s * m.ld;
+  if (s < nv) {
+    const long idx = canon_of_lex(g, lex0 + s0 + s);
+    for (int q = k; q < N; q += Q) {
+      const int o = q / nelem, e = q - o * nelem;
+      double2 z;
+      if (SINGLE) {
+        z.x = (double)__uint_as_float(__builtin_bswap32(row[2 * q]));
+        z.y = (double)__uint_as_float(__builtin_bswap32(row[2 * q + 1]));
+      } else {
+        z.x = __longlong_as_double((long long)__builtin_bswap64(((uint64_t)row[4 * q + 1] << 32) | row[4 * q]));
+        z.y = __longlong_as_double((long long)__builtin_bswap64(((uint64_t)row[4 * q + 3] << 32) | row[4 * q + 2]));
       }
+      dst[((long)o * V + idx) * nelem + e] = z;
     }
-    crc = ~crc;
-    a = rotl32(crc, (int)(lex % 29));
-    b = rotl32(crc, (int)(lex % 31));
   }
-  a = wave_xor(a);
-  b = wave_xor(b);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = a;
-    part[2 * blockIdx.x + 1] = b;
-  }
+  m.comb[k * SITES_PER_BLOCK + s] = s < nv ? part_crc(m, row, k, Q) : 0u;
+  __syncthreads();
+  block_checksum(m, Q, lex0 + s0, nv, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -479,7 +522,7 @@ struct Pipe {
   hipStream_t st;
   uint32_t *dev[2] = {nullptr, nullptr};
   uint32_t *host[2] = {nullptr, nullptr};
-  uint32_t *part = nullptr;
+  uint32_t *part = nullptr, *zmat = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   Pipe(hipStream_t s, size_t chunk_bytes, long nblocks) : st(s) {
     for (int b = 0; b < 2; b++) {
@@ -488,7 +531,8 @@ struct Pipe {
           hipEventCreateWithFlags(&ev[b], hipEventDisableTiming) != hipSuccess)
         throw std::runtime_error("SciDAC I/O: out of memory for the transfer buffers");
     }
-    if (hipMalloc((void **)&part, (size_t)nblocks * 2 * sizeof(uint32_t)) != hipSuccess)
+    if (hipMalloc((void **)&part, (size_t)nblocks * 2 * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void **)&zmat, 96 * sizeof(uint32_t)) != hipSuccess)
       throw std::runtime_error("SciDAC I/O: out of device memory");
   }
   ~Pipe() {
@@ -499,6 +543,7 @@ struct Pipe {
       if (ev[b]) (void)hipEventDestroy(ev[b]);
     }
     if (part) (void)hipFree(part);
+    if (zmat) (void)hipFree(zmat);
   }
   Pipe(const Pipe &) = delete;
   Pipe &operator=(const Pipe &) = delete;
@@ -523,9 +568,58 @@ struct Chunking {
   }
 };
 
-size_t lds_bytes(int nobj, int nelem, bool single) {
-  const int W = nobj * nelem * (single ? 2 : 4);
-  return (256 + (size_t)SITES_PER_BLOCK * (W + 1)) * sizeof(uint32_t);
+// launch shape of the pack/unpack kernels: Q lanes per site (W/Q words each)
+struct Kern {
+  int W, Q;
+  dim3 block;
+  size_t lds;
+  Kern(int nobj, int nelem, bool single) {
+    W = nobj * nelem * (single ? 2 : 4);
+    Q = W % 4 == 0 ? 4 : 2;
+    block = dim3(SITES_PER_BLOCK * Q);
+    lds = (1024 + 96 + (size_t)SITES_PER_BLOCK * (W + 1) + (size_t)SITES_PER_BLOCK * Q) * sizeof(uint32_t);
+  }
+};
+
+// Z^1..Z^3 as 32 columns each, Z = the CRC register update over W/Q*4 zero
+// bytes (column i = the register 1 << i after those bytes), to dev (96 words)
+void upload_shift(uint32_t *dev, const Kern &k, hipStream_t st) {
+  uint32_t T0[256];
+  for (uint32_t e = 0; e < 256; e++) {
+    uint32_t r = e;
+    for (int j = 0; j < 8; j++) r = (r & 1) ? (r >> 1) ^ 0xedb88320u : r >> 1;
+    T0[e] = r;
+  }
+  const int bytes = k.W / k.Q * 4;
+  uint32_t M[96];
+  for (int j = 1; j <= 3; j++)
+    for (int i = 0; i < 32; i++) {
+      uint32_t c = 1u << i;
+      for (int b = 0; b < j * bytes; b++) c = T0[c & 0xff] ^ (c >> 8);
+      M[(j - 1) * 32 + i] = c;
+    }
+  if (hipMemcpyAsync(dev, M, sizeof M, hipMemcpyHostToDevice, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+    throw std::runtime_error("SciDAC I/O: copy to device failed");
+}
+
+void launch_pack(bool single, const Kern &k, unsigned grid, hipStream_t st, const Geom &g, const double *src,
+                 int nobj, int nelem, long lex0, long n, uint32_t *out, uint32_t *part, const uint32_t *zmat) {
+  if (single)
+    hipLaunchKernelGGL(k_scidac_pack<true>, dim3(grid), k.block, k.lds, st, g, (const double2 *)src, nobj, nelem, lex0,
+                       n, out, part, zmat);
+  else
+    hipLaunchKernelGGL(k_scidac_pack<false>, dim3(grid), k.block, k.lds, st, g, (const double2 *)src, nobj, nelem,
+                       lex0, n, out, part, zmat);
+}
+
+void launch_unpack(bool single, const Kern &k, unsigned grid, hipStream_t st, const Geom &g, const uint32_t *in,
+                   int nobj, int nelem, long lex0, long n, double *dst, uint32_t *part, const uint32_t *zmat) {
+  if (single)
+    hipLaunchKernelGGL(k_scidac_unpack<true>, dim3(grid), k.block, k.lds, st, g, in, nobj, nelem, lex0, n,
+                       (double2 *)dst, part, zmat);
+  else
+    hipLaunchKernelGGL(k_scidac_unpack<false>, dim3(grid), k.block, k.lds, st, g, in, nobj, nelem, lex0, n,
+                       (double2 *)dst, part, zmat);
 }
 
 void xor_partials(const Pipe &p, long nblocks, uint32_t *suma, uint32_t *sumb) {
@@ -574,7 +668,8 @@ void scidac_read(const Geom &g, const char *path, int record, double *dst, int n
   const Chunking ch(g, (size_t)nobj * r.typesize);
   Pipe p(st, (size_t)ch.per * ch.site_bytes, ch.nblocks);
   if (fseeko(f.f, r.bin.off, SEEK_SET) != 0) throw std::runtime_error("SciDAC read: seek failed");
-  const size_t lds = lds_bytes(nobj, nelem, single);
+  const Kern kern(nobj, nelem, single);
+  upload_shift(p.zmat, kern, st);
   long blk = 0;
   for (long c = 0; c < ch.nchunks; c++) {
     const int b = (int)(c & 1);
@@ -585,12 +680,7 @@ void scidac_read(const Geom &g, const char *path, int record, double *dst, int n
     if (hipMemcpyAsync(p.dev[b], p.host[b], bytes, hipMemcpyHostToDevice, st) != hipSuccess)
       throw std::runtime_error("SciDAC read: copy to device failed");
     const unsigned grid = (unsigned)((n + SITES_PER_BLOCK - 1) / SITES_PER_BLOCK);
-    if (single)
-      hipLaunchKernelGGL(k_scidac_unpack<true>, dim3(grid), dim3(SITES_PER_BLOCK), lds, st, g, p.dev[b], nobj, nelem,
-                         c * ch.per, n, (double2 *)dst, p.part + 2 * blk);
-    else
-      hipLaunchKernelGGL(k_scidac_unpack<false>, dim3(grid), dim3(SITES_PER_BLOCK), lds, st, g, p.dev[b], nobj, nelem,
-                         c * ch.per, n, (double2 *)dst, p.part + 2 * blk);
+    launch_unpack(single, kern, grid, st, g, p.dev[b], nobj, nelem, c * ch.per, n, dst, p.part + 2 * blk, p.zmat);
     if (hipGetLastError() != hipSuccess || hipEventRecord(p.ev[b], st) != hipSuccess)
       throw std::runtime_error("SciDAC read: kernel launch failed");
     blk += grid;
@@ -652,7 +742,8 @@ void scidac_append(const Geom &g, const char *path, const char *recmd, const dou
   const uint64_t total = (uint64_t)V * ch.site_bytes;
   lime_header(f.f, ildg ? "ildg-binary-data" : "scidac-binary-data", total, false, false);
   Pipe p(st, (size_t)ch.per * ch.site_bytes, ch.nblocks);
-  const size_t lds = lds_bytes(nobj, nelem, single);
+  const Kern kern(nobj, nelem, single);
+  upload_shift(p.zmat, kern, st);
   std::vector<size_t> pending(ch.nchunks, 0);
   long blk = 0;
   auto drain = [&](long c) {  // write chunk c's bytes once its copy to the host is done
@@ -666,12 +757,7 @@ void scidac_append(const Geom &g, const char *path, const char *recmd, const dou
     const long n = ch.sites(g, c);
     pending[c] = (size_t)n * ch.site_bytes;
     const unsigned grid = (unsigned)((n + SITES_PER_BLOCK - 1) / SITES_PER_BLOCK);
-    if (single)
-      hipLaunchKernelGGL(k_scidac_pack<true>, dim3(grid), dim3(SITES_PER_BLOCK), lds, st, g, (const double2 *)src,
-                         nobj, nelem, c * ch.per, n, p.dev[b], p.part + 2 * blk);
-    else
-      hipLaunchKernelGGL(k_scidac_pack<false>, dim3(grid), dim3(SITES_PER_BLOCK), lds, st, g, (const double2 *)src,
-                         nobj, nelem, c * ch.per, n, p.dev[b], p.part + 2 * blk);
+    launch_pack(single, kern, grid, st, g, src, nobj, nelem, c * ch.per, n, p.dev[b], p.part + 2 * blk, p.zmat);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(p.host[b], p.dev[b], pending[c], hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipEventRecord(p.ev[b], st) != hipSuccess)
@@ -706,6 +792,58 @@ void scidac_append(const Geom &g, const char *path, const char *recmd, const dou
     info->sumb = info->calc_sumb = b;
     info->checksum_ok = 1;
   }
+}
+
+void scidac_time(const Geom &g, const double *src, int nobj, int nelem, char prec, int nrep, double ms[2],
+                 hipStream_t st) {
+  check_shape(nobj, nelem);
+  if (prec != 'F' && prec != 'D') throw std::runtime_error("SciDAC timing: precision must be F or D");
+  if (nrep < 1) throw std::runtime_error("SciDAC timing: nrep must be positive");
+  const bool single = prec == 'F';
+  const long V = 2 * g.Vh;
+  const size_t site_bytes = (size_t)nobj * nelem * 2 * (single ? 4 : 8);
+  const unsigned grid = (unsigned)((V + SITES_PER_BLOCK - 1) / SITES_PER_BLOCK);
+  const Kern kern(nobj, nelem, single);
+  struct Bufs {
+    uint32_t *file = nullptr, *part = nullptr, *zmat = nullptr;
+    double *canon = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipStream_t st;
+    ~Bufs() {
+      (void)hipStreamSynchronize(st);
+      if (file) (void)hipFree(file);
+      if (part) (void)hipFree(part);
+      if (zmat) (void)hipFree(zmat);
+      if (canon) (void)hipFree(canon);
+      for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    }
+  } b;
+  b.st = st;
+  if (hipMalloc((void **)&b.file, (size_t)V * site_bytes) != hipSuccess ||
+      hipMalloc((void **)&b.part, (size_t)grid * 2 * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void **)&b.zmat, 96 * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void **)&b.canon, (size_t)nobj * V * nelem * sizeof(double2)) != hipSuccess)
+    throw std::runtime_error("SciDAC timing: out of device memory");
+  for (auto &e : b.ev)
+    if (hipEventCreate(&e) != hipSuccess) throw std::runtime_error("SciDAC timing: event creation failed");
+  upload_shift(b.zmat, kern, st);
+  auto pack = [&]() { launch_pack(single, kern, grid, st, g, src, nobj, nelem, 0L, V, b.file, b.part, b.zmat); };
+  auto unpack = [&]() { launch_unpack(single, kern, grid, st, g, b.file, nobj, nelem, 0L, V, b.canon, b.part, b.zmat); };
+  pack();
+  unpack();
+  if (hipEventRecord(b.ev[0], st) != hipSuccess) throw std::runtime_error("SciDAC timing: event failed");
+  for (int r = 0; r < nrep; r++) pack();
+  if (hipEventRecord(b.ev[1], st) != hipSuccess) throw std::runtime_error("SciDAC timing: event failed");
+  for (int r = 0; r < nrep; r++) unpack();
+  if (hipEventRecord(b.ev[2], st) != hipSuccess || hipGetLastError() != hipSuccess ||
+      hipEventSynchronize(b.ev[2]) != hipSuccess)
+    throw std::runtime_error("SciDAC timing: kernels failed");
+  float t0 = 0, t1 = 0;
+  (void)hipEventElapsedTime(&t0, b.ev[0], b.ev[1]);
+  (void)hipEventElapsedTime(&t1, b.ev[1], b.ev[2]);
+  ms[0] = t0 / nrep;
+  ms[1] = t1 / nrep;
 }
 
 }  // namespace qh

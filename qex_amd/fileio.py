@@ -224,3 +224,13 @@ def saveGauge(lo, g, fn, prec="", filemd=defFileMd, recordmd=defRecMd, ildg=Fals
     wr.write(g, recordmd, prec, ildg=ildg)
     wr.close()
     return wr.status
+
+
+def time_io_kernels(lo, v, prec="D", nrep=5):
+    """Device milliseconds per whole-lattice launch of the write-side (pack)
+    and read-side (unpack) conversion + checksum kernels for fields v."""
+    c = _Canon(lo, v)
+    ms = (ctypes.c_double * 2)()
+    torch.cuda.synchronize()
+    call("qexhip_time_scidac", lo.h, c.ptr(), int(c.nobj), int(c.nelem), prec.encode(), int(nrep), ms)
+    return ms[0], ms[1]

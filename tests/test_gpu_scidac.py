@@ -190,3 +190,12 @@ def test_roundtrip_sizes(oracle, gpu, tmp_path, L):
         _, _, _, stored, computed = oracle.scidac_read(raw, L, 4, 9, 0)
         assert stored == computed == (winfo["suma"], winfo["sumb"])
     fn.unlink()
+
+
+def test_kernel_timing_entry(gpu):
+    from qex_amd import fileio, gauge
+    lo = gpu.Layout((8, 8, 8, 8))
+    U = gauge.random_gauge(lo, SEED + 6)
+    for prec in ("D", "F"):
+        pk, up = fileio.time_io_kernels(lo, U, prec, nrep=2)
+        assert pk > 0 and up > 0
