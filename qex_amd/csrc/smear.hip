@@ -98,7 +98,69 @@ __device__ __forceinline__ long step(const Geom &g, const int c[4], int mu, int 
   return canon_index(g, y);
 }
 
-// S_nu[K](x), x with coordinates c
+// acc += a L M R^+ (BACK false) or a L^+ M R (BACK true), the two halves of a
+// staple.  One row of L M is formed at a time and folded into acc at once, so
+// the live set is M, R, acc and a row; the chain-rule kernels, which hold
+// several accumulators, use it to keep two waves per SIMD (one before).
+template <bool BACK>
+__device__ __forceinline__ void triple(M3 &acc, double a, const MF &L, long il, const MF &M, long im, const MF &R,
+                                       long ir) {
+  const M3 m = ldf(M, im);
+  const M3 r = ldf(R, ir);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    double2 l[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      l[k] = L.p[(BACK ? 3 * k + i : 3 * i + k) * L.es + il * L.ss];
+      if (BACK) l[k].y = -l[k].y;
+    }
+    double2 t[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double2 s = make_double2(0, 0);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double2 y = m.e[3 * k + j];
+        s.x += l[k].x * y.x - l[k].y * y.y;
+        s.y += l[k].x * y.y + l[k].y * y.x;
+      }
+      t[j] = s;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double2 s = make_double2(0, 0);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (BACK) {  // (L^+ M)_{ik} R_{kj}
+          const double2 y = r.e[3 * k + j];
+          s.x += t[k].x * y.x - t[k].y * y.y;
+          s.y += t[k].x * y.y + t[k].y * y.x;
+        } else {  // (L M)_{ik} conj(R_{jk})
+          const double2 y = r.e[3 * j + k];
+          s.x += t[k].x * y.x + t[k].y * y.y;
+          s.y += t[k].y * y.x - t[k].x * y.y;
+        }
+      }
+      acc.e[3 * i + j].x += a * s.x;
+      acc.e[3 * i + j].y += a * s.y;
+    }
+  }
+}
+
+// acc += a S_nu[K](x), x with coordinates c
+__device__ __forceinline__ void gstaple_acc(M3 &acc, double a, const Geom &g, const MF &Unu, const MF &K,
+                                            const int c[4], long x, int mu, int nu) {
+  int cm[4] = {c[0], c[1], c[2], c[3]};
+  cm[nu] = c[nu] > 0 ? c[nu] - 1 : g.L[nu] - 1;
+  const long xpn = step(g, c, nu, 1), xpm = step(g, c, mu, 1);
+  const long xmn = canon_index(g, cm), xmnpm = step(g, cm, mu, 1);
+  triple<false>(acc, a, Unu, x, K, xpn, Unu, xpm);
+  triple<true>(acc, a, Unu, xmn, K, xmn, Unu, xmnpm);
+}
+
+// S_nu[K](x) as two full matrix products per half: faster in the forward
+// sweeps, whose threads hold fewer other matrices than the chain-rule ones
 __device__ __forceinline__ M3 gstaple(const Geom &g, const MF &Unu, const MF &K, const int c[4], long x, int mu,
                                       int nu) {
   int cm[4] = {c[0], c[1], c[2], c[3]};
@@ -369,12 +431,10 @@ __device__ __forceinline__ void side(M3 &acc, double a, const Geom &g, const MF 
   cm[mu] = c[mu] > 0 ? c[mu] - 1 : g.L[mu] - 1;
   const long ypm = step(g, c, mu, 1), ypl = step(g, c, lam, 1);
   const long ymm = canon_index(g, cm), ymmpl = step(g, cm, lam, 1);
-  const M3 up = ldf(Ul, ypm), um = ldf(Ul, ymm);
-  M3 t = mmul_nd(mmul(ldf(C, y), up), ldf(K, ypl));
-  madd(t, mmul_nd(mmul(ldf(K, y), up), ldf(C, ypl)));
-  madd(t, mmul(mmul_dn(ldf(C, ymm), um), ldf(K, ymmpl)));
-  madd(t, mmul(mmul_dn(ldf(K, ymm), um), ldf(C, ymmpl)));
-  macc(acc, a, t);
+#pragma unroll 1
+  for (int t = 0; t < 2; t++) triple<false>(acc, a, t ? K : C, y, Ul, ypm, t ? C : K, ypl);
+#pragma unroll 1
+  for (int t = 0; t < 2; t++) triple<true>(acc, a, t ? K : C, ymm, Ul, ymm, t ? C : K, ymmpl);
 }
 
 // CT_{mu,sig} = c5 C_mu + c7 S_sig[C_mu]; one wave per sig, mu = blockIdx.y
@@ -393,7 +453,7 @@ __global__ void __launch_bounds__(192) k_dct(Geom g, DerivFields f, Fat7 c, int 
     t.e[e].x *= c.c5;
     t.e[e].y *= c.c5;
   }
-  if (c.c7 != 0.0) macc(t, c.c7, gstaple(g, f.U[sig], f.C[mu], cc, x, mu, sig));
+  if (c.c7 != 0.0) gstaple_acc(t, c.c7, g, f.U[sig], f.C[mu], cc, x, mu, sig);
   stf(f.CT[3 * mu + ks], x, t);
 }
 
@@ -415,14 +475,14 @@ __global__ void __launch_bounds__(192) k_dcs(Geom g, DerivFields f, Fat7 c, int 
     t.e[e].y *= c.c3;
   }
   if (HAVE5) {
-    if (c.cL != 0.0) macc(t, c.cL, gstaple(g, f.U[nu], f.C[mu], cc, x, mu, nu));
+    if (c.cL != 0.0) gstaple_acc(t, c.cL, g, f.U[nu], f.C[mu], cc, x, mu, nu);
     if (c.c5 != 0.0 || c.c7 != 0.0) {
 #pragma unroll 1
       for (int k = 0; k < 3; k++) {
         const int sig = other(mu, k);
         if (sig == nu) continue;
         const int rho = third_dir(mu, nu, sig);
-        madd(t, gstaple(g, f.U[rho], f.CT[3 * mu + k], cc, x, mu, rho));
+        gstaple_acc(t, 1.0, g, f.U[rho], f.CT[3 * mu + k], cc, x, mu, rho);
       }
     }
   }
@@ -445,19 +505,37 @@ __global__ void __launch_bounds__(192) k_dside(Geom g, DerivFields f, Fat7 c, in
   M3 acc;
 #pragma unroll
   for (int e = 0; e < 9; e++) acc.e[e] = make_double2(0, 0);
-  side(acc, 1.0, g, f.U[lam], f.CS[sl], f.U[mu], cc, y, mu, lam);  // S_lam[U_mu]
-  if (HAVE5) {
-    if (c.cL != 0.0) side(acc, c.cL, g, f.U[lam], f.C[mu], f.S[sl], cc, y, mu, lam);  // Lepage
-    if (c.c7 != 0.0) side(acc, c.c7, g, f.U[lam], f.C[mu], f.T[sl], cc, y, mu, lam);  // 7-link
-    if (c.c5 != 0.0 || c.c7 != 0.0) {
+  // terms j: 0 S_lam[U_mu] in S_{mu,lam}; 1 Lepage; 2 7-link; 3, 4 S_lam[S_{mu,rho}]
+  // inside T_{mu,sig} for the two sig != lam.  One rolled loop, so that the
+  // loads of one term are not hoisted into another's.
+  if (!HAVE5) {
+    side(acc, 1.0, g, f.U[lam], f.CS[sl], f.U[mu], cc, y, mu, lam);
+    stf(f.P[3 * lam + k], y, acc);
+    return;
+  }
 #pragma unroll 1
-      for (int ks = 0; ks < 3; ks++) {  // S_lam[S_{mu,other}] inside T_{mu,sig}
-        const int sig = other(mu, ks);
-        if (sig == lam) continue;
-        side(acc, 1.0, g, f.U[lam], f.CT[3 * mu + ks], f.S[3 * mu + slot(mu, third_dir(mu, lam, sig))], cc, y, mu,
-             lam);
-      }
+  for (int j = 0; j < 5; j++) {
+    double a = 1.0;
+    MF Cf = f.CS[sl], Kf = f.U[mu];
+    if (j == 1) {
+      if (c.cL == 0.0) continue;
+      a = c.cL;
+      Cf = f.C[mu];
+      Kf = f.S[sl];
+    } else if (j == 2) {
+      if (c.c7 == 0.0) continue;
+      a = c.c7;
+      Cf = f.C[mu];
+      Kf = f.T[sl];
+    } else if (j >= 3) {
+      if (c.c5 == 0.0 && c.c7 == 0.0) continue;
+      const int skip = slot(mu, lam);  // the k with other(mu, k) == lam
+      const int ks = j == 3 ? (skip == 0 ? 1 : 0) : (skip == 2 ? 1 : 2);
+      const int sig = other(mu, ks);
+      Cf = f.CT[3 * mu + ks];
+      Kf = f.S[3 * mu + slot(mu, third_dir(mu, lam, sig))];
     }
+    side(acc, a, g, f.U[lam], Cf, Kf, cc, y, mu, lam);
   }
   stf(f.P[3 * lam + k], y, acc);
 }
@@ -482,7 +560,7 @@ __global__ void __launch_bounds__(256) k_dfin(Geom g, DerivFields f, Fat7 c, int
 #pragma unroll 1
     for (int k = 0; k < 3; k++) {
       const int nu = other(lam, k);
-      madd(d, gstaple(g, f.U[nu], f.CS[3 * lam + k], cc, y, lam, nu));
+      gstaple_acc(d, 1.0, g, f.U[nu], f.CS[3 * lam + k], cc, y, lam, nu);
       madd(d, ldf(f.P[3 * lam + k], y));
     }
   }
