@@ -27,8 +27,10 @@ defRecMd = '<?xml version="1.0"?>\n<note>gauge configuration</note>\n'  # gaugeU
 
 
 def _info_dict(info):
-    return {"dims": tuple(info.dims), "nrecords": info.nrecords, "datatype": info.datatype.decode(),
-            "date": info.date.decode(), "precision": info.precision.decode(), "colors": info.colors,
+    # file text may hold any bytes: decode without failing
+    return {"dims": tuple(info.dims), "nrecords": info.nrecords, "datatype": info.datatype.decode(errors="replace"),
+            "date": info.date.decode(errors="replace"), "precision": info.precision.decode(errors="replace"),
+            "colors": info.colors,
             "spins": info.spins, "typesize": info.typesize, "datacount": info.datacount, "ildg": bool(info.ildg),
             "has_checksum": bool(info.has_checksum), "suma": info.suma, "sumb": info.sumb,
             "calc_suma": info.calc_suma, "calc_sumb": info.calc_sumb, "checksum_ok": bool(info.checksum_ok)}
