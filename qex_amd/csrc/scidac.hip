@@ -21,14 +21,17 @@
 // rank-rotated XOR partials in the same pass (one wave per 64 sites, the
 // site's words staged through LDS so that file bytes move coalesced).  The
 // binary record streams through two pinned host buffers in whole t-slices of
-// at most 64 MiB, so disk transfers of one chunk overlap the device work on
-// the next.
+// at most 64 MiB, so disk transfers of one chunk (pread/pwrite from up to 8
+// host threads) overlap the device work on the next.
 #include <stdio.h>
 #include <string.h>
+#include <fcntl.h>
 #include <time.h>
+#include <unistd.h>
 
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -278,6 +281,35 @@ uint64_t get_be(const unsigned char *p, int nbytes) {
 
 void fwrite_all(FILE *f, const void *p, size_t n) {
   if (n && fwrite(p, 1, n, f) != n) throw std::runtime_error("SciDAC write: write failed");
+}
+
+// n bytes between buf and the file at offset off, split over up to 8 host
+// threads of at least 8 MiB each (one thread copying into the page cache
+// moves about 6 GB/s; parallel file systems also take parallel streams)
+void pio(bool write, int fd, char *buf, size_t n, off_t off) {
+  constexpr size_t MIN_PART = size_t(8) << 20;
+  const size_t nparts = n / MIN_PART < 1 ? 1 : (n / MIN_PART > 8 ? 8 : n / MIN_PART);
+  const size_t part = (n + nparts - 1) / nparts;
+  std::vector<int> err(nparts, 0);
+  auto run = [&](size_t k) {
+    size_t done = k * part;
+    const size_t end = done + part < n ? done + part : n;
+    while (done < end) {
+      const ssize_t r = write ? pwrite(fd, buf + done, end - done, off + (off_t)done)
+                              : pread(fd, buf + done, end - done, off + (off_t)done);
+      if (r <= 0) {
+        err[k] = 1;
+        return;
+      }
+      done += (size_t)r;
+    }
+  };
+  std::vector<std::thread> th;
+  for (size_t k = 1; k < nparts; k++) th.emplace_back(run, k);
+  run(0);
+  for (auto &t : th) t.join();
+  for (int e : err)
+    if (e) throw std::runtime_error(write ? "SciDAC write: write failed" : "SciDAC read: short read");
 }
 
 void lime_header(FILE *f, const char *type, uint64_t len, bool mb, bool me) {
@@ -667,7 +699,7 @@ void scidac_read(const Geom &g, const char *path, int record, double *dst, int n
     throw std::runtime_error("SciDAC file: field data elements differ from the record's typesize");
   const Chunking ch(g, (size_t)nobj * r.typesize);
   Pipe p(st, (size_t)ch.per * ch.site_bytes, ch.nblocks);
-  if (fseeko(f.f, r.bin.off, SEEK_SET) != 0) throw std::runtime_error("SciDAC read: seek failed");
+  const int fd = fileno(f.f);
   const Kern kern(nobj, nelem, single);
   upload_shift(p.zmat, kern, st);
   long blk = 0;
@@ -676,7 +708,7 @@ void scidac_read(const Geom &g, const char *path, int record, double *dst, int n
     const long n = ch.sites(g, c);
     const size_t bytes = (size_t)n * ch.site_bytes;
     if (c >= 2 && hipEventSynchronize(p.ev[b]) != hipSuccess) throw std::runtime_error("SciDAC read: device error");
-    if (fread(p.host[b], 1, bytes, f.f) != bytes) throw std::runtime_error("SciDAC read: short read");
+    pio(false, fd, (char *)p.host[b], bytes, (off_t)(r.bin.off + (long long)(c * ch.per) * (long long)ch.site_bytes));
     if (hipMemcpyAsync(p.dev[b], p.host[b], bytes, hipMemcpyHostToDevice, st) != hipSuccess)
       throw std::runtime_error("SciDAC read: copy to device failed");
     const unsigned grid = (unsigned)((n + SITES_PER_BLOCK - 1) / SITES_PER_BLOCK);
@@ -741,6 +773,10 @@ void scidac_append(const Geom &g, const char *path, const char *recmd, const dou
   const Chunking ch(g, (size_t)nobj * typesize);
   const uint64_t total = (uint64_t)V * ch.site_bytes;
   lime_header(f.f, ildg ? "ildg-binary-data" : "scidac-binary-data", total, false, false);
+  if (fflush(f.f) != 0) throw std::runtime_error("SciDAC write: write failed");
+  const off_t base = ftello(f.f);
+  const int fd = fileno(f.f);
+  (void)posix_fallocate(fd, base, (off_t)total);  // reserve the record's blocks; a refusal is harmless
   Pipe p(st, (size_t)ch.per * ch.site_bytes, ch.nblocks);
   const Kern kern(nobj, nelem, single);
   upload_shift(p.zmat, kern, st);
@@ -749,7 +785,7 @@ void scidac_append(const Geom &g, const char *path, const char *recmd, const dou
   auto drain = [&](long c) {  // write chunk c's bytes once its copy to the host is done
     const int b = (int)(c & 1);
     if (hipEventSynchronize(p.ev[b]) != hipSuccess) throw std::runtime_error("SciDAC write: device error");
-    fwrite_all(f.f, p.host[b], pending[c]);
+    pio(true, fd, (char *)p.host[b], pending[c], base + (off_t)(c * ch.per) * (off_t)ch.site_bytes);
   };
   for (long c = 0; c < ch.nchunks; c++) {
     const int b = (int)(c & 1);
@@ -765,6 +801,7 @@ void scidac_append(const Geom &g, const char *path, const char *recmd, const dou
     blk += grid;
   }
   for (long c = ch.nchunks >= 2 ? ch.nchunks - 2 : 0; c < ch.nchunks; c++) drain(c);
+  if (fseeko(f.f, base + (off_t)total, SEEK_SET) != 0) throw std::runtime_error("SciDAC write: seek failed");
   lime_pad(f.f, total);
   uint32_t a, b;
   xor_partials(p, ch.nblocks, &a, &b);
