@@ -317,7 +317,8 @@ def gauge_io_study():
     with their algorithmic GB/s (canonical FP64 links 576 B/site in or out,
     file bytes 576 B/site FP64 or 288 B/site FP32), and the end-to-end
     saveGauge / loadGauge rates through a temporary file (these include the
-    host file system and PCIe; they are not kernel figures)."""
+    host file system and PCIe; they are not kernel figures), and the same for
+    MILC files."""
     import shutil
     import tempfile
     import torch
@@ -354,6 +355,21 @@ def gauge_io_study():
             os.unlink(fn)
             del G, want
             res["FP64" if prec == "D" else "FP32"] = r
+        # MILC files (single precision, their own checksum) through the same pipeline
+        fn = os.path.join(tmp, "g.milc")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gauge.writeMilc(lo, fn, U, timestamp="bench")
+        t1 = time.perf_counter()
+        G, info = gauge.readMilc(lo, fn)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        size = os.path.getsize(fn)
+        res["MILC"] = {"file_GB": round(size / 1e9, 3), "writeMilc_GB_s": round(size / (t1 - t0) / 1e9, 2),
+                       "readMilc_GB_s": round(size / (t2 - t1) / 1e9, 2), "checksum_ok": info["checksum_ok"],
+                       "roundtrip_exact": bool(torch.equal(G, U.to(torch.complex64).to(torch.complex128)))}
+        os.unlink(fn)
+        del G
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     del U

@@ -31,10 +31,10 @@
 
 #include <stdexcept>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "internal.h"
+#include "iopipe.h"
 #include "su3.h"
 
 namespace qh {
@@ -43,8 +43,7 @@ namespace {
 
 constexpr uint32_t LIME_MAGIC = 0x456789abu;
 constexpr int LIME_HEADER = 144;
-constexpr int SITES_PER_BLOCK = 64;  // one wave, one site per lane
-constexpr size_t CHUNK_BYTES = size_t(64) << 20;
+using namespace iop;  // SITES_PER_BLOCK (64 sites, one per lane of a wave), Pipe, Chunking, pio
 
 // ---------------------------------------------------------------------------
 // device side
@@ -62,8 +61,6 @@ constexpr size_t CHUNK_BYTES = size_t(64) << 20;
 // columns each, built on the host.
 
 constexpr int MAXQ = 4;
-
-__device__ __forceinline__ uint32_t rotl32(uint32_t v, int k) { return k ? (v << k) | (v >> (32 - k)) : v; }
 
 // T[0] = byte table of the reflected CRC-32 polynomial 0xedb88320
 // (src/io/crc32.nim:8-14); T[j][i] = (T[j-1][i] >> 8) ^ T[0][T[j-1][i] & 0xff]
@@ -95,12 +92,6 @@ __device__ __forceinline__ uint32_t gf2_apply(const uint32_t *M, uint32_t c) {
 #pragma unroll
   for (int i = 0; i < 32; i++) o ^= M[i] & (0u - ((c >> i) & 1u));
   return o;
-}
-
-__device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v ^= (uint32_t)__shfl_xor((int)v, o);
-  return v;
 }
 
 __device__ __forceinline__ long canon_of_lex(const Geom &g, long lex) {
@@ -281,35 +272,6 @@ uint64_t get_be(const unsigned char *p, int nbytes) {
 
 void fwrite_all(FILE *f, const void *p, size_t n) {
   if (n && fwrite(p, 1, n, f) != n) throw std::runtime_error("SciDAC write: write failed");
-}
-
-// n bytes between buf and the file at offset off, split over up to 8 host
-// threads of at least 8 MiB each (one thread copying into the page cache
-// moves about 6 GB/s; parallel file systems also take parallel streams)
-void pio(bool write, int fd, char *buf, size_t n, off_t off) {
-  constexpr size_t MIN_PART = size_t(8) << 20;
-  const size_t nparts = n / MIN_PART < 1 ? 1 : (n / MIN_PART > 8 ? 8 : n / MIN_PART);
-  const size_t part = (n + nparts - 1) / nparts;
-  std::vector<int> err(nparts, 0);
-  auto run = [&](size_t k) {
-    size_t done = k * part;
-    const size_t end = done + part < n ? done + part : n;
-    while (done < end) {
-      const ssize_t r = write ? pwrite(fd, buf + done, end - done, off + (off_t)done)
-                              : pread(fd, buf + done, end - done, off + (off_t)done);
-      if (r <= 0) {
-        err[k] = 1;
-        return;
-      }
-      done += (size_t)r;
-    }
-  };
-  std::vector<std::thread> th;
-  for (size_t k = 1; k < nparts; k++) th.emplace_back(run, k);
-  run(0);
-  for (auto &t : th) t.join();
-  for (int e : err)
-    if (e) throw std::runtime_error(write ? "SciDAC write: write failed" : "SciDAC read: short read");
 }
 
 void lime_header(FILE *f, const char *type, uint64_t len, bool mb, bool me) {
@@ -549,57 +511,6 @@ std::string now_utc() {
   return buf;
 }
 
-// device and pinned buffers of the two-stage chunk pipeline
-struct Pipe {
-  hipStream_t st;
-  uint32_t *dev[2] = {nullptr, nullptr};
-  uint32_t *host[2] = {nullptr, nullptr};
-  uint32_t *part = nullptr, *zmat = nullptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  Pipe(hipStream_t s, size_t chunk_bytes, long nblocks) : st(s) {
-    for (int b = 0; b < 2; b++) {
-      if (hipMalloc((void **)&dev[b], chunk_bytes) != hipSuccess ||
-          hipHostMalloc((void **)&host[b], chunk_bytes, hipHostMallocDefault) != hipSuccess ||
-          hipEventCreateWithFlags(&ev[b], hipEventDisableTiming) != hipSuccess)
-        throw std::runtime_error("SciDAC I/O: out of memory for the transfer buffers");
-    }
-    if (hipMalloc((void **)&part, (size_t)nblocks * 2 * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc((void **)&zmat, 96 * sizeof(uint32_t)) != hipSuccess)
-      throw std::runtime_error("SciDAC I/O: out of device memory");
-  }
-  ~Pipe() {
-    (void)hipStreamSynchronize(st);  // nothing may still be reading or writing the buffers
-    for (int b = 0; b < 2; b++) {
-      if (dev[b]) (void)hipFree(dev[b]);
-      if (host[b]) (void)hipHostFree(host[b]);
-      if (ev[b]) (void)hipEventDestroy(ev[b]);
-    }
-    if (part) (void)hipFree(part);
-    if (zmat) (void)hipFree(zmat);
-  }
-  Pipe(const Pipe &) = delete;
-  Pipe &operator=(const Pipe &) = delete;
-};
-
-struct Chunking {
-  long slab, per, nchunks, nblocks;  // sites per t-slice and per chunk, chunks, blocks in all
-  size_t site_bytes;
-  Chunking(const Geom &g, size_t sb) : site_bytes(sb) {
-    slab = (long)g.L[0] * g.L[1] * g.L[2];
-    long nt = (long)(CHUNK_BYTES / ((size_t)slab * sb));
-    if (nt < 1) nt = 1;
-    if (nt > g.L[3]) nt = g.L[3];
-    per = nt * slab;
-    nchunks = (g.L[3] + nt - 1) / nt;
-    nblocks = 0;
-    for (long c = 0; c < nchunks; c++) nblocks += (sites(g, c) + SITES_PER_BLOCK - 1) / SITES_PER_BLOCK;
-  }
-  long sites(const Geom &g, long c) const {
-    const long V = 2 * g.Vh, s = V - c * per;
-    return s < per ? s : per;
-  }
-};
-
 // launch shape of the pack/unpack kernels: Q lanes per site (W/Q words each)
 struct Kern {
   int W, Q;
@@ -654,19 +565,6 @@ void launch_unpack(bool single, const Kern &k, unsigned grid, hipStream_t st, co
                        (double2 *)dst, part, zmat);
 }
 
-void xor_partials(const Pipe &p, long nblocks, uint32_t *suma, uint32_t *sumb) {
-  std::vector<uint32_t> h((size_t)nblocks * 2);
-  if (hipMemcpy(h.data(), p.part, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
-    throw std::runtime_error("SciDAC I/O: checksum copy failed");
-  uint32_t a = 0, b = 0;
-  for (long k = 0; k < nblocks; k++) {
-    a ^= h[2 * k];
-    b ^= h[2 * k + 1];
-  }
-  *suma = a;
-  *sumb = b;
-}
-
 void check_shape(int nobj, int nelem) {
   if (nelem != 1 && nelem != 3 && nelem != 9) throw std::runtime_error("SciDAC I/O: nelem must be 1, 3 or 9");
   if (nobj < 1 || nobj > 4) throw std::runtime_error("SciDAC I/O: nobj must be 1 to 4");
@@ -698,27 +596,28 @@ void scidac_read(const Geom &g, const char *path, int record, double *dst, int n
   if (r.typesize != nelem * 2 * (single ? 4 : 8))
     throw std::runtime_error("SciDAC file: field data elements differ from the record's typesize");
   const Chunking ch(g, (size_t)nobj * r.typesize);
-  Pipe p(st, (size_t)ch.per * ch.site_bytes, ch.nblocks);
+  Pipe p(st, (size_t)ch.per * ch.site_bytes, ch.nblocks, 96, "SciDAC I/O");
   const int fd = fileno(f.f);
   const Kern kern(nobj, nelem, single);
-  upload_shift(p.zmat, kern, st);
+  upload_shift(p.aux, kern, st);
   long blk = 0;
   for (long c = 0; c < ch.nchunks; c++) {
     const int b = (int)(c & 1);
     const long n = ch.sites(g, c);
     const size_t bytes = (size_t)n * ch.site_bytes;
     if (c >= 2 && hipEventSynchronize(p.ev[b]) != hipSuccess) throw std::runtime_error("SciDAC read: device error");
-    pio(false, fd, (char *)p.host[b], bytes, (off_t)(r.bin.off + (long long)(c * ch.per) * (long long)ch.site_bytes));
+    pio(false, fd, (char *)p.host[b], bytes, (off_t)(r.bin.off + (long long)(c * ch.per) * (long long)ch.site_bytes),
+        "SciDAC read");
     if (hipMemcpyAsync(p.dev[b], p.host[b], bytes, hipMemcpyHostToDevice, st) != hipSuccess)
       throw std::runtime_error("SciDAC read: copy to device failed");
     const unsigned grid = (unsigned)((n + SITES_PER_BLOCK - 1) / SITES_PER_BLOCK);
-    launch_unpack(single, kern, grid, st, g, p.dev[b], nobj, nelem, c * ch.per, n, dst, p.part + 2 * blk, p.zmat);
+    launch_unpack(single, kern, grid, st, g, p.dev[b], nobj, nelem, c * ch.per, n, dst, p.part + 2 * blk, p.aux);
     if (hipGetLastError() != hipSuccess || hipEventRecord(p.ev[b], st) != hipSuccess)
       throw std::runtime_error("SciDAC read: kernel launch failed");
     blk += grid;
   }
   if (hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("SciDAC read: device error");
-  xor_partials(p, ch.nblocks, &info->calc_suma, &info->calc_sumb);
+  p.checksum(ch.nblocks, &info->calc_suma, &info->calc_sumb, "SciDAC read");
   info->checksum_ok = r.has_ck ? (info->calc_suma == r.suma && info->calc_sumb == r.sumb) : 1;
   if (!info->checksum_ok) throw std::runtime_error("SciDAC file: checksum mismatch");
 }
@@ -777,15 +676,15 @@ void scidac_append(const Geom &g, const char *path, const char *recmd, const dou
   const off_t base = ftello(f.f);
   const int fd = fileno(f.f);
   (void)posix_fallocate(fd, base, (off_t)total);  // reserve the record's blocks; a refusal is harmless
-  Pipe p(st, (size_t)ch.per * ch.site_bytes, ch.nblocks);
+  Pipe p(st, (size_t)ch.per * ch.site_bytes, ch.nblocks, 96, "SciDAC I/O");
   const Kern kern(nobj, nelem, single);
-  upload_shift(p.zmat, kern, st);
+  upload_shift(p.aux, kern, st);
   std::vector<size_t> pending(ch.nchunks, 0);
   long blk = 0;
   auto drain = [&](long c) {  // write chunk c's bytes once its copy to the host is done
     const int b = (int)(c & 1);
     if (hipEventSynchronize(p.ev[b]) != hipSuccess) throw std::runtime_error("SciDAC write: device error");
-    pio(true, fd, (char *)p.host[b], pending[c], base + (off_t)(c * ch.per) * (off_t)ch.site_bytes);
+    pio(true, fd, (char *)p.host[b], pending[c], base + (off_t)(c * ch.per) * (off_t)ch.site_bytes, "SciDAC write");
   };
   for (long c = 0; c < ch.nchunks; c++) {
     const int b = (int)(c & 1);
@@ -793,7 +692,7 @@ void scidac_append(const Geom &g, const char *path, const char *recmd, const dou
     const long n = ch.sites(g, c);
     pending[c] = (size_t)n * ch.site_bytes;
     const unsigned grid = (unsigned)((n + SITES_PER_BLOCK - 1) / SITES_PER_BLOCK);
-    launch_pack(single, kern, grid, st, g, src, nobj, nelem, c * ch.per, n, p.dev[b], p.part + 2 * blk, p.zmat);
+    launch_pack(single, kern, grid, st, g, src, nobj, nelem, c * ch.per, n, p.dev[b], p.part + 2 * blk, p.aux);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(p.host[b], p.dev[b], pending[c], hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipEventRecord(p.ev[b], st) != hipSuccess)
@@ -804,7 +703,7 @@ void scidac_append(const Geom &g, const char *path, const char *recmd, const dou
   if (fseeko(f.f, base + (off_t)total, SEEK_SET) != 0) throw std::runtime_error("SciDAC write: seek failed");
   lime_pad(f.f, total);
   uint32_t a, b;
-  xor_partials(p, ch.nblocks, &a, &b);
+  p.checksum(ch.nblocks, &a, &b, "SciDAC write");
   char ck[256];
   snprintf(ck, sizeof ck,
            "%s<scidacChecksum><version>1.0</version><suma>%x</suma><sumb>%x</sumb></scidacChecksum>", XML_DECL, a,
