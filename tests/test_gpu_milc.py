@@ -60,3 +60,24 @@ def test_corrupt_and_mismatched_files(oracle, gpu, tmp_path):
     (tmp_path / "x.milc").write_bytes(b"\0" * 200)
     with pytest.raises(QexHipError, match="not a MILC"):
         gauge.milcHeader(tmp_path / "x.milc")
+
+
+@pytest.mark.parametrize("L", [(2, 2, 2, 2), (32, 32, 32, 32)])
+def test_roundtrip_sizes_and_checksum(oracle, gpu, tmp_path, L):
+    """2^4: a partial 64-site block; 32^4: six t-slice chunks, the last one
+    ragged.  The header's checksum equals the oracle's over the file's words."""
+    import torch
+    from qex_amd import gauge
+    print(f"milc round trip L={L}", flush=True)
+    lo = gpu.Layout(L)
+    U = gauge.random_gauge(lo, SEED + 7)
+    path = tmp_path / "big.milc"
+    gauge.writeMilc(lo, path, U, timestamp="sizes")
+    hdr = gauge.milcHeader(path)
+    words = np.fromfile(path, dtype=np.uint32, offset=96)
+    assert words.size == 72 * lo.nSites
+    assert (hdr["sum29"], hdr["sum31"]) == oracle.milc_checksum(words)
+    del words
+    G, info = gauge.readMilc(lo, path)
+    assert info["checksum_ok"] and torch.equal(G, U.to(torch.complex64).to(torch.complex128))
+    path.unlink()
