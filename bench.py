@@ -506,8 +506,8 @@ def batched_study():
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "this library runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is not built)"}))
         sys.exit(2)
     import torch
     if args.copy_calib_mb:

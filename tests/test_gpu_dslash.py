@@ -1,5 +1,5 @@
 """GPU parity of the staggered hop (stagD2/stagD/stagD2xx/D/Ddag) against the
-CPU oracle, through the C ABI.  FP64: 1e-12 relative;
+CPU oracle, through the C ABI.  FP64: 1e-12 relative (the FP64 parity bar);
 FP32 arithmetic: 1e-5; FP16/BF16 storage: 4e-3 / 3e-2."""
 import numpy as np
 import pytest

@@ -179,7 +179,7 @@ def test_hisq_force_against_action_differences(oracle, gpu):
         vals.append(action(Uk))
     fd = float(np.dot(W9, vals) / h)
     print(f"HISQ force: analytic {analytic:.12e} fd {fd:.12e}", flush=True)
-    # measured agreement 2.4e-10; the north-star bar for the force is 1e-8
+    # measured agreement 2.4e-10; the force must agree with finite differences to 1e-8
     assert abs(fd - analytic) < 1e-8 * abs(analytic)
 
 
