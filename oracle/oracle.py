@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/qexoracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product package qex_amd
+cpu_baseline leg of bench.py, to check results against.  The product package qex_amd
 never imports it.
 
 Fields are numpy complex128 arrays in QEX's canonical single-rank site order

@@ -1,7 +1,7 @@
 /*
  * qexoracle.c -- CPU restatement of QEX's staggered Dslash / CG / CGM / force.
  *
- * TEST INFRASTRUCTURE ONLY (the parity checker and the CPU baseline); see
+ * TEST INFRASTRUCTURE ONLY (the parity reference and the CPU baseline); see
  * qexoracle.h for the conventions.  Every routine cites the QEX source it
  * restates (paths relative to QEX's repository root).
  *
@@ -16,6 +16,21 @@
  *      independent single-shift CG.
  *  qo_ferm_force    : central finite differences along random su(3)
  *      directions, gauge covariance.
+ *  qo_plaquette / qo_wilson_force / qo_random_momenta / qo_link_update :
+ *      unit gauge = 18 V exactly and gauge invariance; central differences
+ *      through an independent numpy staple sum; anti-Hermitian traceless
+ *      momenta with <|P|^2/2> = 4 per link; scipy's expm.
+ *  qo_make_imp_links / qo_hisq_smear : unit-link path counts (fat7l.nim:183-217),
+ *      phased unit links (9/8 U, -1/24 U U U), an explicit sum over the 90
+ *      link paths, gauge covariance (tests/test_oracle_hisq.py).
+ *  qo_project_u     : numpy's SVD polar factor.
+ *  qo_fat7l_deriv / qo_project_u_deriv / qo_hisq_smear_deriv / qo_hisq_outer :
+ *      exact (9-point) and central differences of the smeared action; the
+ *      trivial smearing reproduces qo_ferm_force.
+ *  qo_milc_checksum : answers worked out by hand (tests/test_oracle_milc.py).
+ *  qo_crc32 / qo_scidac_checksum : the reference's CRC-32 known answer
+ *      (crc32.nim:113-116), zlib's crc32, and the rank rotations recomputed
+ *      site by site with zlib (tests/test_oracle_scidac.py).
  */
 #include "qexoracle.h"
 #include <complex.h>

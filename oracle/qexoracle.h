@@ -2,7 +2,7 @@
  * qexoracle.h -- CPU restatement of QEX's even-odd staggered Dslash, CG,
  * multi-shift CG and staggered fermion force, in plain C (C99 + OpenMP).
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity reference for the HIP
  * implementation in qex_amd/.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it; the product path never does.
  *
