@@ -436,6 +436,11 @@ def milc_read(path):
 # scidac-binary-data or ildg-binary-data, scidac-checksum}.  The binary data
 # are big-endian, sites in lexicographic order (x fastest), each site holding
 # datacount objects of typesize bytes (a 3x3 complex matrix is row-major).
+# Parity: crc32 and scidac_checksum are pinned by the reference's CRC-32 known
+# answer and by zlib; lime_record/lime_records by bytes assembled by hand.  The
+# XML record text (scidac_private_file, scidac_private_record, ildg_format,
+# scidac_checksum_xml) is parity unpinned: no QIO-written file exists here to
+# compare with, and QEX's reader only checks the fields that the tests pin.
 
 LIME_MAGIC = 0x456789AB
 
