@@ -314,6 +314,21 @@ int qexhip_scidac_create(const char *path, const int dims[4], const char *filemd
    means the current UTC time.  info (may be NULL) receives the checksums. */
 int qexhip_scidac_append(qexhip_lattice *lat, const char *path, const char *recmd, const double *data_dev, int nobj,
                          int nelem, char precision, int ildg, const char *date, qexhip_scidac_info *info);
+/* ---- QEX's own field storage on one rank <-> canonical order, on the device
+   (the host repack of quda/qudaWrapperImpl.nim:204-246).  A QEX field of
+   Layout[V] (layout/qlayout.nim:10-150, field/fieldET.nim:18-22) with
+   inner_geom = Layout.innerGeom holds, per outer site o and complex component
+   c, the real parts of its V = prod(inner_geom) SIMD lanes, then the
+   imaginary parts: double ((o*ncomp + c)*2 + r)*V + lane.  qex_dev holds
+   nfields such fields back to back (ncomp = 9 for a ColorMatrix, 3 for a
+   ColorVector); canon_dev is canonical complex128 [nfields][site][ncomp].
+   Layouts QEX refuses (innerGeom entries not dividing the lattice or above
+   2, no valid checkerboard fix-up) are refused. */
+int qexhip_from_qex_layout(qexhip_lattice *lat, const int inner_geom[4], const double *qex_dev, int nfields,
+                           int ncomp, double *canon_dev);
+int qexhip_to_qex_layout(qexhip_lattice *lat, const int inner_geom[4], const double *canon_dev, int nfields,
+                         int ncomp, double *qex_dev);
+
 /* timing: ms[0], ms[1] = device time per whole-lattice launch of the
    conversion-plus-checksum kernels of Writer.write and Reader.read (no file) */
 int qexhip_time_scidac(qexhip_lattice *lat, const double *data_dev, int nobj, int nelem, char precision, int nrep,

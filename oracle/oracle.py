@@ -577,3 +577,137 @@ def scidac_read(raw, L, nobj, nelem, record=0):
         stored = (int(_tag(ck, "suma"), 16), int(_tag(ck, "sumb"), 16))
     computed = scidac_checksum(binary, nobj * nelem * (8 if prec == "F" else 16))
     return out, priv, m.get("scidac-record-xml", b"").decode(), stored, computed
+
+
+# ---------------------------------------------------------------------------
+# QEX's own SIMD field layout (one rank).  layoutSetupQ (src/layout/qlayout.nim:
+# 10-67) derives outerGeom = L / innerGeom and the checkerboard fix-up
+# (innerCb, innerCbDir); layoutIndexImpl (qlayout.nim:122-150) maps a site to
+# index = (oi div 2) * nSitesInner + ii: outer site index // nSitesInner, SIMD
+# lane ii.  A field (field/fieldET.nim:18-22) is an array over outer sites of
+# the vectorized site type; for n complex components (a 3x3 matrix row-major,
+# maths/matrixConcept.nim:124-125) each component is {re: lanes, im: lanes}
+# (maths/complexType.nim:11-13), so the double at (outer o, component c, re/im
+# r, lane l) sits at ((o * n + c) * 2 + r) * nSitesInner + l.
+# Pinned by layoutCoordQ (qlayout.nim:154-213) being its inverse on every site,
+# the map being a bijection with the even sites first, and innerGeom = 1
+# reducing it to the canonical order (tests/test_oracle_qexlayout.py).
+
+def qex_layout_setup(L, ig):
+    og = [L[d] // ig[d] for d in range(4)]
+    assert all(og[d] * ig[d] == L[d] for d in range(4)), "innerGeom must divide the lattice"
+    assert all(1 <= v <= 2 for v in ig), "innerGeom entries above 2 are not QEX layouts"  # layoutX.nim:124-131
+    icb, icbd = 0, -1
+    for d in range(4):
+        if ig[d] > 1 and (og[d] & 1) == 1:
+            icb += 1
+        if ig[d] == 1 and (og[d] & 1) == 0:
+            icbd = d
+    if icb == 0:
+        icbd = 0
+    else:
+        assert icbd >= 0, "not enough 2's in localGeom"
+        icb = og[icbd] // 2
+        assert icb & 1, "error in cb choice"
+    nouter = int(np.prod(og))
+    ninner = int(np.prod(ig))
+    return {"og": og, "ig": list(ig), "innerCb": icb, "innerCbDir": icbd, "nSitesOuter": nouter,
+            "nSitesInner": ninner, "nEvenOuter": nouter - nouter // 2}
+
+
+def qex_layout_index(q, x):
+    ii = oi = ib = p = ocb = mcb = 0
+    mi = mo = 1
+    for d in range(4):
+        og = q["og"][d]
+        o = x[d] % og
+        k = (x[d] // og) % q["ig"][d]
+        ii += k * mi
+        mi *= q["ig"][d]
+        if d == q["innerCbDir"]:
+            ocb, mcb = o, mo
+        oi += o * mo
+        mo *= og
+        ib += k * og
+        p += x[d]
+    if ib & 1:
+        oi += ((ocb + q["innerCb"]) % q["og"][q["innerCbDir"]] - ocb) * mcb
+    if p & 1:
+        oi += q["nSitesOuter"]
+    return (oi // 2) * q["nSitesInner"] + ii
+
+
+def qex_layout_coord(q, index):
+    """layoutCoordQ (qlayout.nim:154-204) for one rank."""
+    og, ig = q["og"], q["ig"]
+    c = [0, 0, 0, 0]
+    p = 0
+    ll = index % q["nSitesInner"]
+    ib = 0
+    for d in range(4):
+        k = ll % ig[d]
+        c[d] += k * og[d]
+        p += c[d]
+        ll //= ig[d]
+        ib += k * og[d]
+    ib &= 1
+    ii = index // q["nSitesInner"]
+    if ii >= q["nEvenOuter"]:
+        ii -= q["nEvenOuter"]
+        p += 1
+    ii *= 2
+    for d in range(4):
+        k = ii % og[d]
+        if d == q["innerCbDir"]:
+            k = (k + q["innerCb"] * ib) % og[d]
+        c[d] += k
+        p += k
+        ii //= og[d]
+    if p & 1:
+        for d in range(4):
+            wl = og[d]
+            k = c[d] % wl
+            b = c[d] - k
+            if d == q["innerCbDir"]:
+                k = (k + q["innerCb"] * ib) % wl
+            k += 1
+            if k >= wl:
+                k = 0
+                if d == q["innerCbDir"]:
+                    k = (k + q["innerCb"] * ib) % wl
+                c[d] = b + k
+            else:
+                if d == q["innerCbDir"]:
+                    k = (k + q["innerCb"] * ib) % wl
+                c[d] = b + k
+                break
+    return tuple(c)
+
+
+def to_qex_layout(L, ig, fields):
+    """Canonical (nf, V, n) complex128 -> QEX storage (nf, nSitesOuter * n * 2 * nSitesInner) doubles."""
+    q = qex_layout_setup(L, ig)
+    nf, V, n = fields.shape
+    ni = q["nSitesInner"]
+    out = np.zeros((nf, q["nSitesOuter"], n, 2, ni))
+    C = all_coords(L)
+    for s in range(V):
+        j = qex_layout_index(q, C[s])
+        o, l = divmod(j, ni)
+        out[:, o, :, 0, l] = fields[:, s, :].real
+        out[:, o, :, 1, l] = fields[:, s, :].imag
+    return out.reshape(nf, -1)
+
+
+def from_qex_layout(L, ig, raw, n):
+    q = qex_layout_setup(L, ig)
+    nf = raw.shape[0]
+    ni = q["nSitesInner"]
+    r = raw.reshape(nf, q["nSitesOuter"], n, 2, ni)
+    V = volume(L)
+    out = np.zeros((nf, V, n), dtype=np.complex128)
+    C = all_coords(L)
+    for s in range(V):
+        o, l = divmod(qex_layout_index(q, C[s]), ni)
+        out[:, s, :] = r[:, o, :, 0, l] + 1j * r[:, o, :, 1, l]
+    return out

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["dslash.hip", "kernels.hip", "reduce.hip", "gen.hip", "gauge.hip", "batch.hip", "smear.hip", "milcio.hip", "scidac.hip", "api.hip"]
+SOURCES = ["dslash.hip", "kernels.hip", "reduce.hip", "gen.hip", "gauge.hip", "batch.hip", "smear.hip", "milcio.hip", "scidac.hip", "qexlayout.hip", "api.hip"]
 OUT = os.path.join(HERE, "libqexhip.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

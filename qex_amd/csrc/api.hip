@@ -2009,6 +2009,26 @@ int qexhip_time_scidac(qexhip_lattice *lat, const double *data, int nobj, int ne
   API_END
 }
 
+int qexhip_from_qex_layout(qexhip_lattice *lat, const int inner_geom[4], const double *qex_dev, int nfields,
+                           int ncomp, double *canon_dev) {
+  API_BEGIN
+  check_init();
+  require(lat && inner_geom && qex_dev && canon_dev, "null argument");
+  qex_layout_convert(lat->g, inner_geom, 1, const_cast<double *>(qex_dev), nfields, ncomp, canon_dev, g_stream);
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+int qexhip_to_qex_layout(qexhip_lattice *lat, const int inner_geom[4], const double *canon_dev, int nfields,
+                         int ncomp, double *qex_dev) {
+  API_BEGIN
+  check_init();
+  require(lat && inner_geom && qex_dev && canon_dev, "null argument");
+  qex_layout_convert(lat->g, inner_geom, 0, qex_dev, nfields, ncomp, const_cast<double *>(canon_dev), g_stream);
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
 int qexhip_links_recon_ok(qexhip_links *l, int *ok) {
   API_BEGIN
   check_init();

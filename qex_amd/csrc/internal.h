@@ -253,6 +253,10 @@ void scidac_read(const Geom &g, const char *path, int record, double *dst, int n
 void scidac_create(const char *path, const int dims[4], const char *filemd);
 void scidac_append(const Geom &g, const char *path, const char *recmd, const double *src, int nobj, int nelem,
                    char prec, int ildg, const char *date, ScidacInfo *info, hipStream_t st);
+// qexlayout.hip: QEX's one-rank SIMD field storage (innerGeom lanes) <->
+// canonical complex128 [nfields][site][ncomp]
+void qex_layout_convert(const Geom &g, const int inner_geom[4], int from_qex, double *qex, int nfields, int ncomp,
+                        double *canon, hipStream_t st);
 // ms[0], ms[1]: device time per launch of the whole-lattice pack and unpack kernels
 void scidac_time(const Geom &g, const double *src, int nobj, int nelem, char prec, int nrep, double ms[2],
                  hipStream_t st);

@@ -91,3 +91,40 @@ def writeMilc(lo, path, U, timestamp=None):
     torch.cuda.synchronize()
     call("qexhip_milc_write", lo.h, str(path).encode(), ctypes.c_void_p(U.data_ptr()),
          timestamp.encode() if timestamp else None)
+
+
+def _qex_shape(lo, innerGeom, nfields, ncomp):
+    ig = [int(v) for v in innerGeom]
+    if len(ig) != 4:
+        raise ValueError("innerGeom needs 4 entries")
+    return (ctypes.c_int * 4)(*ig), (nfields, lo.nSites * ncomp * 2)
+
+
+def fromQexLayout(lo, innerGeom, raw, ncomp):
+    """Canonical (nfields, V, ncomp) complex128 fields from QEX's own storage
+    of Layout(innerGeom) fields: raw is a float64 CUDA tensor (nfields,
+    V*ncomp*2) holding each field's site data as QEX keeps it (the host repack
+    of quda/qudaWrapperImpl.nim:204-246, done on the device)."""
+    ig, shape = _qex_shape(lo, innerGeom, raw.shape[0], ncomp)
+    if not (raw.is_cuda and raw.dtype == torch.float64 and raw.is_contiguous() and tuple(raw.shape) == shape):
+        raise ValueError(f"raw must be a contiguous float64 CUDA tensor of shape {shape}")
+    out = torch.empty((shape[0], lo.nSites, ncomp), dtype=torch.complex128, device=raw.device)
+    torch.cuda.synchronize()
+    call("qexhip_from_qex_layout", lo.h, ig, ctypes.c_void_p(raw.data_ptr()), shape[0], ncomp,
+         ctypes.c_void_p(out.data_ptr()))
+    return out
+
+
+def toQexLayout(lo, innerGeom, fields):
+    """QEX's own storage (nfields, V*ncomp*2) float64 of canonical fields
+    (nfields, V, ncomp) complex128."""
+    if not (fields.is_cuda and fields.dtype == torch.complex128 and fields.is_contiguous() and fields.dim() == 3
+            and fields.shape[1] == lo.nSites):
+        raise ValueError("fields must be a contiguous complex128 CUDA tensor (nfields, V, ncomp)")
+    nf, _, ncomp = fields.shape
+    ig, shape = _qex_shape(lo, innerGeom, nf, ncomp)
+    out = torch.empty(shape, dtype=torch.float64, device=fields.device)
+    torch.cuda.synchronize()
+    call("qexhip_to_qex_layout", lo.h, ig, ctypes.c_void_p(fields.data_ptr()), nf, ncomp,
+         ctypes.c_void_p(out.data_ptr()))
+    return out
