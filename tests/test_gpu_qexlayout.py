@@ -76,3 +76,33 @@ def test_refused(gpu):
         gauge.toQexLayout(lo, (1, 1, 4, 1), f)
     with pytest.raises(QexHipError, match="cb choice"):
         gauge.toQexLayout(lo, (1, 1, 1, 2), f)
+
+
+def test_solve_from_qex_storage(oracle, gpu):
+    """The path of a QEX shim: HISQ-form links and a source in QEX's own
+    4-lane storage are uploaded as they are, converted on the device, solved,
+    and the solution converted back; it equals the oracle's solution laid out
+    by the oracle's restatement of QEX's storage."""
+    import torch
+    from qex_amd import gauge
+    L, ig = (8, 8, 4, 8), (1, 1, 2, 2)
+    lo = gpu.Layout(L)
+    V = lo.nSites
+    U = oracle.phased(L, oracle.gauge_random(L, SEED + 2))
+    g8 = oracle.hisq_links(L, SEED + 3, U)                         # (8, V, 3, 3) fat/long interleaved
+    b = oracle.source_gauss(L, 9, 0, 0)                             # even sites
+    raw_g = oracle.to_qex_layout(L, ig, g8.reshape(8, V, 9))
+    raw_b = oracle.to_qex_layout(L, ig, b.reshape(1, V, 3))
+    g_dev = gauge.fromQexLayout(lo, ig, _cuda(raw_g), 9).reshape(8, V, 3, 3)
+    b_dev = gauge.fromQexLayout(lo, ig, _cuda(raw_b), 3).reshape(V, 3)
+    s = gpu.Staggered(lo, g_dev)
+    x = lo.ColorVector()
+    sp = gpu.SolverParams()
+    sp.r2req, sp.maxits, sp.verbosity = 1e-20, 2000, 0
+    gpu.solveEE(s, x, lo.ColorVector().set(b_dev), 0.1, sp)
+    raw_x = gauge.toQexLayout(lo, ig, x.get(device=torch.device("cuda")).reshape(1, V, 3).contiguous())
+    want, its, _ = oracle.solveXX(L, g8, 0, b, 0.1, 1e-20)
+    want_raw = oracle.to_qex_layout(L, ig, want.reshape(1, V, 3))
+    got = raw_x.cpu().numpy()
+    assert np.linalg.norm(got - want_raw) / np.linalg.norm(want_raw) < 1e-9
+    assert abs(sp.iterations - its) <= 1
