@@ -179,6 +179,12 @@ int qexhip_set_cg_graph(int on);
    psi_o = (2D)_oe psi_e */
 int qexhip_force_solve(qexhip_links *l, qexhip_field *psi, const qexhip_field *phi, double m,
                        qexhip_solver_params *sp);
+/* the outer-product accumulation of stagD2deriv / stagDeriv (stagD.nim:640-715,
+   matrixConcept.nim:718-725): for every site y and link k, F_k(y) += c a(y)
+   b(y + d_k)^+, c = ce on even and co on odd sites, d_k one site in
+   direction mu (three for k = 2mu+1 when nl = 8); F is canonical
+   [nl][site][3][3] FP64 on the device and is accumulated into */
+int qexhip_outer_acc(qexhip_links *l, const qexhip_field *a, const qexhip_field *b, double ce, double co, double *F);
 /* fermForce, unsmeared (hmcAction.nim:723-792): F_mu(x) = TAH(eps(x) psi(x) psi(x+mu)^+ U_mu(x)^+);
    outer_only=1 returns eps(x) psi(x) psi(x+mu)^+ instead.  F: canonical 4 x V x 18 doubles. */
 int qexhip_force(qexhip_links *l, const qexhip_field *psi, double *F, int F_on_device, int outer_only);

@@ -33,7 +33,7 @@ EXPORTS = [
     "qexhip_time_hisq_smear", "qexhip_fat7l_deriv", "qexhip_project_u_deriv", "qexhip_hisq_smear_deriv",
     "qexhip_hisq_force", "qexhip_milc_read_info", "qexhip_milc_read", "qexhip_milc_write",
     "qexhip_scidac_read_info", "qexhip_scidac_read", "qexhip_scidac_create", "qexhip_scidac_append",
-    "qexhip_time_scidac", "qexhip_from_qex_layout", "qexhip_to_qex_layout",
+    "qexhip_time_scidac", "qexhip_from_qex_layout", "qexhip_to_qex_layout", "qexhip_outer_acc",
 ]
 
 # qexhip_solve_batch / qexhip_solve_multishift_batch flags
@@ -148,6 +148,7 @@ def load(path=LIB_PATH):
                                  ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ScidacInfoC)],
         "qexhip_time_scidac": [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_char, ctypes.c_int, dp],
         "qexhip_from_qex_layout": [vp, ip, vp, I, I, vp], "qexhip_to_qex_layout": [vp, ip, vp, I, I, vp],
+        "qexhip_outer_acc": [vp, vp, vp, D, D, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

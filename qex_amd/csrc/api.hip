@@ -1589,6 +1589,20 @@ int qexhip_force_solve(qexhip_links *l, qexhip_field *psi, const qexhip_field *p
   API_END
 }
 
+int qexhip_outer_acc(qexhip_links *l, const qexhip_field *a, const qexhip_field *b, double ce, double co, double *F) {
+  API_BEGIN
+  check_init();
+  check_same_lat(l, a);
+  check_same_lat(l, b);
+  require(a->prec == P64 && b->prec == P64, "outer products need FP64 fields");
+  require(F != nullptr, "null output");
+  launch_outer_acc(l->lat->g, (const double *)a->par(0), (const double *)a->par(1), (const double *)b->par(0),
+                   (const double *)b->par(1), ce, co, F, l->nl, g_stream);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
 int qexhip_force(qexhip_links *l, const qexhip_field *psi, double *F, int F_on_device, int outer_only) {
   API_BEGIN
   check_init();

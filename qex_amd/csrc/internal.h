@@ -169,6 +169,8 @@ void launch_cgm_xr(const Geom &g, int nshift, double *r, const double *Ap, doubl
                    ShiftScal *T, Scal *S, RedWs red, hipStream_t st);
 void launch_cgm_p(const Geom &g, int nshift, const double *r, double **p, ShiftScal *T, hipStream_t st);
 void launch_cgm_alpha(ShiftScal *T, Scal *S, hipStream_t st);
+void launch_outer_acc(const Geom &g, const double *a_e, const double *a_o, const double *b_e, const double *b_o,
+                      double ce, double co, double *F, int nl, hipStream_t st);
 void launch_force(const Geom &g, const char *links64, long lkblk, const double *psi_e, const double *psi_o,
                   double *F, int outer_only, hipStream_t st, int nl = 4);
 

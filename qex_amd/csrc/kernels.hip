@@ -401,6 +401,53 @@ __global__ void __launch_bounds__(256) k_force(Geom g, const char *links, long l
   for (int e = 0; e < 9; e++) out[e] = Rm[e];
 }
 
+// F_k(y) += c(y) a(y) b(y + d_k)^+ on every site y, c = ce on even and co on
+// odd sites, d_k the hop of link k (one site in direction mu; three for
+// k = 2mu+1 when nl = 8): the outer-product accumulations of stagD2deriv and
+// stagDeriv (physics/stagD.nim:640-715, peqOuter/meqOuter of
+// maths/matrixConcept.nim:718-725)
+__global__ void __launch_bounds__(256) k_outer_acc(Geom g, const double2 *ae, const double2 *ao, const double2 *be,
+                                                   const double2 *bo, double ce, double co, double2 *F, int nl) {
+  const long V = 2 * g.Vh;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  const int mu = nl == 8 ? k >> 1 : k;
+  if (idx >= V) return;
+  const int par = idx >= g.Vh;
+  const long i = idx - par * g.Vh;
+  const Site s = decode(g, i, par);
+  const int len = (nl == 8 && (k & 1)) ? 3 : 1;
+  const long j = nbr(g, s, mu, len);
+  // a 3-hop neighbour has the other parity too (odd displacement)
+  const double2 *as = par ? ao : ae, *bn = par ? be : bo;
+  const double c = par ? co : ce;
+  double2 a[3], b[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    a[q] = as[q * g.stride + i];
+    b[q] = bn[q * g.stride + j];
+    a[q].x *= c;
+    a[q].y *= c;
+  }
+  double2 *out = F + ((long)k * V + idx) * 9;
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int q = 0; q < 3; q++) {  // a_r conj(b_q)
+      double2 o = out[3 * r + q];
+      o.x += a[r].x * b[q].x + a[r].y * b[q].y;
+      o.y += a[r].y * b[q].x - a[r].x * b[q].y;
+      out[3 * r + q] = o;
+    }
+}
+
+void launch_outer_acc(const Geom &g, const double *a_e, const double *a_o, const double *b_e, const double *b_o,
+                      double ce, double co, double *F, int nl, hipStream_t st) {
+  dim3 grid(nblk(2 * g.Vh, 256), nl);
+  hipLaunchKernelGGL(k_outer_acc, grid, dim3(256), 0, st, g, (const double2 *)a_e, (const double2 *)a_o,
+                     (const double2 *)b_e, (const double2 *)b_o, ce, co, (double2 *)F, nl);
+}
+
 void launch_force(const Geom &g, const char *links64, long lkblk, const double *psi_e, const double *psi_o,
                   double *F, int outer_only, hipStream_t st, int nl) {
   dim3 grid(nblk(2 * g.Vh, 256), nl);

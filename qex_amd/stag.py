@@ -216,6 +216,22 @@ def stagD(sd, r, g, x, m, sc=1.0, a=0.0):
     call("qexhip_stagD", s.h, par, r.h, x.h, float(m), float(sc), float(a))
 
 
+def stagD1(sd, r, g, x):
+    """r = (2D)x on sd's subset (stagD.nim:467-469, stagDP with r initialised to 0)."""
+    stagD2(sd, r, g, x, 0.0, 0.0)
+
+
+def stagD1x(sd, r, g, x):
+    """r = -(2D)x on sd's subset (stagD.nim:471-473, stagDM)."""
+    stagD(sd, r, g, x, 0.0, -2.0)
+
+
+def stagDb(sd, r, g, x, m, sc=1.0):
+    """r = (m/(0.5 sc)) x + (2D)x: stagD without the final 0.5 sc scaling
+    (stagD.nim:476-478)."""
+    stagD2(sd, r, g, x, 0.0, m / (0.5 * sc))
+
+
 def stagD2multi(sd, rs, g, xs, a, b):
     """r_j = a*r_j + b*x_j + (2D)x_j for every field pair on sd's subset, each link
     loaded once for up to 8 fields (the multi-RHS form of stagD2)."""
@@ -253,6 +269,12 @@ def Ddag(s, r, x, m):
     """D^+(m) = m - D on all sites (stagD.nim:620-622)."""
     stagD(s.se, r, s.g, x, m, -1.0)
     stagD(s.so, r, s.g, x, m, -1.0)
+
+
+def peqDdag(s, r, x, m):
+    """r += D^+(m) x on all sites (stagD.nim:623-625)."""
+    stagD(s.se, r, s.g, x, m, -1.0, 1.0)
+    stagD(s.so, r, s.g, x, m, -1.0, 1.0)
 
 
 def eoReduce(s, r, b, m):
@@ -631,6 +653,41 @@ def fermForce(s, psi, out=None, outer_only=False, device=None):
     p, dev = _ptr(out)
     call("qexhip_force", s.h, psi.h, p, dev, int(outer_only))
     return out
+
+
+def _outer_acc(s, a, b, ce, co, F):
+    if not (torch is not None and isinstance(F, torch.Tensor) and F.is_cuda and F.dtype == torch.complex128
+            and F.is_contiguous() and tuple(F.shape) == (s.nl, s.l.nSites, 3, 3)):
+        raise QexHipError(f"derivative fields must be a contiguous complex128 CUDA tensor ({s.nl}, V, 3, 3)")
+    torch.cuda.synchronize()
+    call("qexhip_outer_acc", s.h, a.h, b.h, float(ce), float(co), ctypes.c_void_p(F.data_ptr()))
+
+
+def stagD2deriv(s, g, c, x):
+    """g_k += d/dU_k Re<c, (2D) x>: g_k(y) += c(y) x(y+d_k)^+ - x(y) c(y+d_k)^+ on
+    every site (stagD.nim:640-684); g is (nl, V, 3, 3) complex128 on the device."""
+    _outer_acc(s, c, x, 1.0, 1.0, g)
+    _outer_acc(s, x, c, -1.0, -1.0, g)
+
+
+def stagDeriv(s, f, x):
+    """f_mu(y) += eps(y) x(y) x(y+mu)^+ (eps = +1 even, -1 odd), then f is rephased
+    with the antiperiodic-t boundary and the staggered phases (stagD.nim:686-715,
+    rephase :110-113); plain links, f (4, V, 3, 3) complex128 on the device."""
+    if s.nl != 4:
+        raise QexHipError("stagDeriv: plain (4-link) operators only")
+    _outer_acc(s, x, x, 1.0, -1.0, f)
+    rephase(s, f)
+
+
+def rephase(s, g):
+    """setBC then stagPhase on the plain link array g, in place (stagD.nim:110-113).
+    The operator holds its own copy of its links: after rephasing them, pass
+    them on with s.update(g)."""
+    from . import gauge
+    if tuple(g.shape) != (4, s.l.nSites, 3, 3):
+        raise QexHipError("rephase: g must be (4, V, 3, 3)")
+    gauge.phase_bc(s.l, g)
 
 
 def time_hop(s, prec=FP64, nrep=100, recon=18):
