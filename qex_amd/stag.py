@@ -529,9 +529,23 @@ def _solve_recon_l(s, x, b, m, sp, b2e, b2o):
     eoReconstruct(s, x, b, m)
 
 
-def solve(s, x, b, m, sp):
+def solve(s, x, b, m, sp, cpuonly=False, sloppySolve=None):
     """Full-lattice D(m) x = b (stagSolve.nim:469-557): even-odd preconditioned
-    inner solves (solveInner, :313-326) inside an FP64 true-residual loop."""
+    inner solves (solveInner, :313-326) inside an FP64 true-residual loop.
+    With a number in place of sp it is the residual r2req of QEX's short form
+    (stagSolve.nim:683-693); cpuonly asks for QEX's own CPU backend, which
+    this library does not provide, and is refused."""
+    if isinstance(sp, (int, float)):
+        res = float(sp)
+        sp = SolverParams()
+        sp.r2req = res
+        sp.verbosity = 1
+        sp.subsetName = "even"
+        sp.sloppySolve = SloppyType.SloppyNone if sloppySolve is None else sloppySolve
+        if cpuonly:
+            sp.backend = SolverBackend.sbQex
+    if sp.backend != SolverBackend.sbHip:
+        raise QexHipError(f"backend {SolverBackend(sp.backend).name} is not provided by this library")
     lo = s.l
     b2 = b.norm2()
     r2stop = sp.r2req * b2
