@@ -14,4 +14,4 @@ from .hisq import (Fat7lCoefs, HisqCoefs, fat7lDeriv, hisqForce, hisqStag, makeI
                    projectU, projectUderiv, smearGetForce)
 from .fileio import (Reader, Writer, getFileLattice, loadGauge, newReader, newWriter,  # noqa: F401,E402
                      saveGauge)
-from .cg import CgPrecon, CgState, cgSolve, newCgState  # noqa: F401,E402
+from .cg import CgmState, CgPrecon, CgState, cgSolve, newCgmState, newCgState  # noqa: F401,E402

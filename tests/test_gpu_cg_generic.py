@@ -102,3 +102,28 @@ def test_solve_short_form(oracle, gpu):
     assert rel(r.get(), b) < 1e-9 and sp.r2 <= 1e-19
     with pytest.raises(Q.QexHipError, match="sbQex"):
         Q.solve(s, x, lo.ColorVector().set(b), m, 1e-20, cpuonly=True)
+
+
+def test_cgm_state_shifts(oracle, gpu):
+    """Generic multi-shift CG with A = 4 m^2 - D_eo D_oe: every shifted solution
+    equals an independent single-shift solve at mass sqrt(m^2 + sigma/4), and
+    the unshifted one the fused solveMulti's."""
+    import qex_amd as Q
+    L, m = (8, 8, 4, 8), 0.1
+    lo, U, s = _setup(oracle, gpu, L)
+    op = EvenOp(Q, s, m)
+    b = oracle.source_gauss(L, 8, 0, 0)
+    sig = [0.0, 0.01, 0.1, 1.0]
+    xs = [lo.ColorVector() for _ in sig]
+    sp = Q.SolverParams()
+    sp.r2req, sp.maxits, sp.verbosity, sp.subsetName = 1e-20, 2000, 0, "even"
+    st = Q.newCgmState(xs, lo.ColorVector().set(b), sig)
+    st.solve(lambda a, p, shift=0.0: op.apply(a, p), sp)
+    for k, sgm in enumerate(sig):
+        want, _, _ = oracle.solveXX(L, U, 0, b, float(np.sqrt(m * m + sgm / 4.0)), 1e-22)
+        assert rel(xs[k].get(), want) < 1e-8, sgm
+    xm = [lo.ColorVector() for _ in sig]
+    spm = Q.SolverParams()
+    spm.r2req, spm.verbosity = 1e-20, 0
+    Q.solveMulti(s, xm, lo.ColorVector().set(b), m, sig, spm)
+    assert abs(spm.iterations - sp.iterations) <= 2
