@@ -3,7 +3,7 @@ behind QEX's solver-backend seam.  The compute path is the HIP library
 qex_amd/libqexhip.so (C ABI in include/qexhip.h); this package is its host-side
 mirror of QEX's Nim API."""
 from ._lib import ALL, BF16, EVEN, FP16, FP32, FP64, ODD, QexHipError, load  # noqa: F401
-from .stag import (peqDdag, rephase, stagD1, stagD1x, stagD2deriv, stagDb, stagDeriv,  # noqa: F401
+from .stag import (peqDdag, rephase, stagD1, stagD1x, stagD2deriv, stagDb, stagDeriv, toSingle,  # noqa: F401
                    D, Ddag, Field, Layout, SloppyType, SolverBackend, SolverParams,
                    Staggered, eoReconstruct, eoReduce, fermForce, newLayout, newSolverParams, newStag,
                    newStag3, solve, solveBatch, solveEE, solveMulti, solveMultiBatch, solveMultiMass, solveOO,

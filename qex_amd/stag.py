@@ -669,6 +669,20 @@ def fermForce(s, psi, out=None, outer_only=False, device=None):
     return out
 
 
+def toSingle(x):
+    """toSingle (stagD.nim:56-86, fieldET.nim:346-352).  For an operator: the
+    same operator, since this library keeps per-precision copies of the links
+    and picks them by the precision of the fields it is applied to.  For a
+    Field: a single-precision copy."""
+    if isinstance(x, Staggered):
+        return x
+    if isinstance(x, Field):
+        if torch is None or not torch.cuda.is_available():
+            raise QexHipError("toSingle needs the GPU")
+        return x.l.ColorVector(prec=FP32).set(x.get(device=torch.device("cuda")))
+    raise QexHipError("toSingle: expected a Staggered operator or a Field")
+
+
 def _outer_acc(s, a, b, ce, co, F):
     if not (torch is not None and isinstance(F, torch.Tensor) and F.is_cuda and F.dtype == torch.complex128
             and F.is_contiguous() and tuple(F.shape) == (s.nl, s.l.nSites, 3, 3)):

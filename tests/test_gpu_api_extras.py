@@ -108,3 +108,23 @@ def test_stagDeriv_and_rephase(oracle, gpu):
     with pytest.raises(Q.QexHipError, match="plain"):
         Q.stagDeriv(gpu.Staggered(lo, oracle.hisq_links(L, 1, U)), torch.zeros((8, V, 3, 3), dtype=torch.complex128,
                                                                                  device="cuda"), lo.ColorVector())
+
+
+def test_toSingle(oracle, gpu):
+    """toSingle: the operator applied to single-precision fields runs the FP32
+    links and arithmetic, within single-precision rounding of FP64."""
+    import qex_amd as Q
+    L = (8, 8, 4, 8)
+    lo = gpu.Layout(L)
+    U = oracle.phased(L, oracle.gauge_random(L, SEED + 9))
+    s = gpu.Staggered(lo, U)
+    assert Q.toSingle(s) is s
+    x = oracle.source_gauss(L, 10, 0, 2)
+    fx = lo.ColorVector().set(x)
+    fx1 = Q.toSingle(fx)
+    assert fx1.prec == Q.FP32 and rel(fx1.get(), x) < 1e-7
+    r1 = lo.ColorVector(prec=Q.FP32)
+    Q.stagD2(s.se, r1, s.g, fx1, 0.0, 0.3)
+    want = np.zeros_like(x)
+    oracle.stagD2(L, U, 0, want, x, 0.0, 0.3)
+    assert rel(r1.get(sub="even"), np.where(np.arange(lo.nSites)[:, None] < lo.nSites // 2, want, 0)) < 1e-6
