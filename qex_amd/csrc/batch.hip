@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
       const int len = (NL == 8 && (k & 1)) ? 3 : 1;
       const long n = nbr(g, s, mu, fwd ? len : -len);
       double2 U[9];
-      ld_link<P64, 18, true, double2>(links + (long)(2 * k + (fwd ? q : 1 - q)) * lkblk, g.stride, fwd ? i : n, U,
+      ld_link<P64, 18, true, double2, link_blocked<P64, NL>()>(links + (long)(2 * k + (fwd ? q : 1 - q)) * lkblk, g.stride, fwd ? i : n, U,
                                       1.0);
 #pragma unroll
       for (int r = 0; r < NR; r++) {
@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(256) k_hop_multi(Geom g, const char *links, lo
       const int len = (NL == 8 && (k & 1)) ? 3 : 1;
       const long n = nbr(g, s, mu, fwd ? len : -len);
       A U[9];
-      ld_link<PL, 18, true, A>(links + (long)(2 * k + (fwd ? par : 1 - par)) * lkblk, g.stride, fwd ? i : n, U,
+      ld_link<PL, 18, true, A, link_blocked<PL, NL>()>(links + (long)(2 * k + (fwd ? par : 1 - par)) * lkblk, g.stride, fwd ? i : n, U,
                                (R)1.0);
 #pragma unroll
       for (int r = 0; r < NR; r++) {

@@ -268,7 +268,11 @@ __global__ void k_pack_links(Geom g, int nl, const double2 *canon, char *out, lo
       w[c] = f_to_bf16((float)d.x) | (f_to_bf16((float)d.y) << 16);
     }
   }
-  st_link_words<PL, RC>(out + (long)(2 * k + par) * lkblk, g.stride, i, w);
+  // the layout each reader expects: blocked for plain links where that pays
+  if (nl == 4 && link_blocked<PL, 4>())
+    st_link_words<PL, RC, true>(out + (long)(2 * k + par) * lkblk, g.stride, i, w);
+  else
+    st_link_words<PL, RC, false>(out + (long)(2 * k + par) * lkblk, g.stride, i, w);
 }
 
 template <int PL>

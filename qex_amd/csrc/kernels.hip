@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(256) k_force(Geom g, const char *links, long l
     return;
   }
   double2 U[9];
-  ld_link<P64, 18, false, double2>(links + (long)(2 * mu + par) * lkblk, g.stride, i, U, 1.0);
+  ld_link<P64, 18, false, double2, link_blocked<P64, 4>()>(links + (long)(2 * mu + par) * lkblk, g.stride, i, U, 1.0);
   double2 T[9];
 #pragma unroll
   for (int r = 0; r < 3; r++)

@@ -23,7 +23,7 @@ __device__ __forceinline__ void leg_load(const Geom &g, const char *links, long 
   const int mu = (NL == 4) ? k : (k >> 1);
   const int len = (NL == 8 && (k & 1)) ? 3 : 1;
   const long n = nbr(g, s, mu, fwd ? len : -len);
-  ld_link<PL, RC, true, A>(links + (long)(2 * k + (fwd ? par : 1 - par)) * lkblk, g.stride, fwd ? i : n, L.U,
+  ld_link<PL, RC, true, A, link_blocked<PL, NL>()>(links + (long)(2 * k + (fwd ? par : 1 - par)) * lkblk, g.stride, fwd ? i : n, L.U,
                            (RT<A>)(RC == 12 ? link_sign(g, s, mu, fwd) : 1.0f), PL == PQ16 ? qs[k] : 0.0f);
   ld(n, L.v);
 }
