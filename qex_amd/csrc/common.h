@@ -323,14 +323,19 @@ typedef unsigned int u2v __attribute__((ext_vector_type(2)));
 // 8-link HISQ FP64 hop (401 -> 421 us at 32^3 x 64) and some 16-bit plain
 // hops, which keep the plane layout.
 template <int PL, int NL> constexpr bool link_blocked() { return NL == 4 && (PL == P64 || PL == P32); }
+#ifndef QH_LBLK_LOG2
+#define QH_LBLK_LOG2 6  // sites per block: 64
+#endif
+constexpr int LBLK = 1 << QH_LBLK_LOG2;
 template <int PL, int RC, bool BLK> __host__ __device__ __forceinline__ long lplane_off(long stride, long i, int p) {
   using F = LFmt<PL, RC>;
-  if (BLK) return (i >> 6) * (64L * F::bytes) + (long)p * 1024 + (i & 63) * 16;
+  if (BLK) return (i >> QH_LBLK_LOG2) * ((long)LBLK * F::bytes) + (long)p * LBLK * 16 + (i & (LBLK - 1)) * 16;
   return ((long)p * stride + i) * 16;
 }
 template <int PL, int RC, bool BLK> __host__ __device__ __forceinline__ long lrem_off(long stride, long i) {
   using F = LFmt<PL, RC>;
-  if (BLK) return (i >> 6) * (64L * F::bytes) + (long)F::nfull * 1024 + (i & 63) * F::rem;
+  if (BLK)
+    return (i >> QH_LBLK_LOG2) * ((long)LBLK * F::bytes) + (long)F::nfull * LBLK * 16 + (i & (LBLK - 1)) * F::rem;
   return (long)F::nfull * 16 * stride + i * F::rem;
 }
 
@@ -366,9 +371,10 @@ __device__ __forceinline__ void ld_link(const char *blk, long stride, long i, A 
   using F = LFmt<PL, RC>;
   // plane p and remainder offsets of site i (uint32: a link block is < 4 GiB)
   constexpr bool B = BLK;
-  const uint32_t base = B ? (uint32_t)(i >> 6) * (uint32_t)(64 * F::bytes) : 0u;
-  const uint32_t pstep = B ? 1024u : (uint32_t)stride * 16u, l16 = (uint32_t)(B ? (i & 63) : i) * 16u;
-  const uint32_t rbase = base + (uint32_t)F::nfull * pstep + (uint32_t)(B ? (i & 63) : i) * (uint32_t)F::rem;
+  const uint32_t base = B ? (uint32_t)(i >> QH_LBLK_LOG2) * (uint32_t)(LBLK * F::bytes) : 0u;
+  const uint32_t lane = (uint32_t)(B ? (i & (LBLK - 1)) : i);
+  const uint32_t pstep = B ? (uint32_t)LBLK * 16u : (uint32_t)stride * 16u, l16 = lane * 16u;
+  const uint32_t rbase = base + (uint32_t)F::nfull * pstep + lane * (uint32_t)F::rem;
   auto poff = [&](int p) { return base + (uint32_t)p * pstep + l16; };
   if constexpr (PL == P64) {
 #pragma unroll
