@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
         const double2 *src = (const double2 *)(MODE == 0 ? P.p0[r] : P.t[r]);
         double2 v[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[c] = src[c * g.stride + n];
+        for (int c = 0; c < 3; c++) v[c] = src[vix(g, c, n)];
         if (fwd) {
 #pragma unroll
           for (int a = 0; a < 3; a++)
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
         if (!act[r]) continue;
         double2 *dst = (double2 *)P.t[r];
 #pragma unroll
-        for (int c = 0; c < 3; c++) dst[c * g.stride + i] = acc[r][c];
+        for (int c = 0; c < 3; c++) dst[vix(g, c, i)] = acc[r][c];
       }
   } else {
     double sums[3 * NR];
@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
         double2 *dst = (double2 *)P.Ap[r];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          const long o = c * g.stride + i;
+          const long o = vix(g, c, i);
           const double2 pv = pp[o], rv = rr[o];
           const double2 av = make_double2(m4 * pv.x - acc[r][c].x, m4 * pv.y - acc[r][c].y);
           dst[o] = av;
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(256) k_hop_multi(Geom g, const char *links, lo
       for (int r = 0; r < NR; r++) {
         A v[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[c] = ldc<PL, A>(P.in[r], c * g.stride + n);
+        for (int c = 0; c < 3; c++) v[c] = ldc<PL, A>(P.in[r], vix(g, c, n));
         if (fwd) {
 #pragma unroll
           for (int a = 0; a < 3; a++)
@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(256) k_hop_multi(Geom g, const char *links, lo
     for (int r = 0; r < NR; r++)
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const long o = c * g.stride + i;
+        const long o = vix(g, c, i);
         A v = acc[r][c];
         if (cb != 0.0) {
           const A x = ldc<PL, A>(P.own[r], o);
@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(256) k_msb_update(Geom g, int nshift, MsbPtrs 
       const double2 *av = (const double2 *)P.Ap[r];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const long o = c * g.stride + i;
+        const long o = vix(g, c, i);
         const double2 a = av[o], x = rv[o];
         rn[c] = make_double2(x.x - alpha * a.x, x.y - alpha * a.y);
         rv[o] = rn[c];
@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(256) k_msb_update(Geom g, int nshift, MsbPtrs 
         double2 *xk = (double2 *)P.x[r * nshift + k], *pk = (double2 *)P.p[r * nshift + k];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          const long o = c * g.stride + i;
+          const long o = vix(g, c, i);
           double2 xv = xk[o];
           const double2 pv = pk[o];
           xv.x += xc * pv.x;

@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(256, QH_HOP_WAVES) k_hop(HopArgs P) {
     A res[3];
     auto ld = [&](long j, A(&v)[3]) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) v[c] = ldc<PI, A>(P.in, c * g.stride + j);
+      for (int c = 0; c < 3; c++) v[c] = ldc<PI, A>(P.in, vix(g, c, j));
     };
     stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, P.par, i, s, ld, res);
     const R chop = (R)P.chop;
@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(256, QH_HOP_WAVES) k_hop(HopArgs P) {
       if (P.ca_m2) ca *= 4.0 * P.S->m2;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        av[c] = ldc<PA, A>(P.a, c * g.stride + i);
+        av[c] = ldc<PA, A>(P.a, vix(g, c, i));
         res[c].x += (R)ca * av[c].x;
         res[c].y += (R)ca * av[c].y;
       }
@@ -57,13 +57,13 @@ __global__ void __launch_bounds__(256, QH_HOP_WAVES) k_hop(HopArgs P) {
       const R cb = (R)P.cb;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const A bv = ldc<PA, A>(P.b, c * g.stride + i);
+        const A bv = ldc<PA, A>(P.b, vix(g, c, i));
         res[c].x += cb * bv.x;
         res[c].y += cb * bv.y;
       }
     }
 #pragma unroll
-    for (int c = 0; c < 3; c++) stc_out<PO, A>(P.out, c * g.stride + i, res[c]);
+    for (int c = 0; c < 3; c++) stc_out<PO, A>(P.out, vix(g, c, i), res[c]);
     if (RED == 1) {
 #pragma unroll
       for (int c = 0; c < 3; c++) red += (double)av[c].x * (double)res[c].x + (double)av[c].y * (double)res[c].y;

@@ -19,7 +19,7 @@ __global__ void k_pack_vec(Geom g, const double2 *canon, void *out) {
     double2 d = canon[3 * i + c];
     A v;
     v.x = d.x; v.y = d.y;
-    stc<PO, A>(out, c * g.stride + i, v);
+    stc<PO, A>(out, vix(g, c, i), v);
   }
 }
 
@@ -30,9 +30,9 @@ __global__ void k_unpack_vec(Geom g, const void *in, double2 *canon) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     if (PI == P64) {
-      canon[3 * i + c] = ldc<P64, double2>(in, c * g.stride + i);
+      canon[3 * i + c] = ldc<P64, double2>(in, vix(g, c, i));
     } else {
-      float2 f = ldc<PI, float2>(in, c * g.stride + i);
+      float2 f = ldc<PI, float2>(in, vix(g, c, i));
       canon[3 * i + c] = make_double2(f.x, f.y);
     }
   }
@@ -109,8 +109,8 @@ __global__ void k_axpby64(Geom g, double a, const double2 *x, double b, double2 
   if (i >= g.Vh) return;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    double2 xv = x[c * g.stride + i], yv = y[c * g.stride + i];
-    y[c * g.stride + i] = make_double2(a * xv.x + b * yv.x, a * xv.y + b * yv.y);
+    double2 xv = x[vix(g, c, i)], yv = y[vix(g, c, i)];
+    y[vix(g, c, i)] = make_double2(a * xv.x + b * yv.x, a * xv.y + b * yv.y);
   }
 }
 
@@ -129,11 +129,11 @@ __global__ void __launch_bounds__(256) k_reduce(Geom g, const void *x, const voi
   if (i < g.Vh) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      A a = ldc<P, A>(x, c * g.stride + i);
+      A a = ldc<P, A>(x, vix(g, c, i));
       if (MODE == 0) {
         s += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
       } else {
-        A b = ldc<P, A>(y, c * g.stride + i);
+        A b = ldc<P, A>(y, vix(g, c, i));
         s += (double)a.x * (double)b.x + (double)a.y * (double)b.y;
       }
     }
@@ -179,7 +179,7 @@ __global__ void k_xfold(Geom g, double2 *x, void *xs, Scal *S) {
   const double is = 1.0 / S->sigma;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const long o = c * g.stride + i;
+    const long o = vix(g, c, i);
     const float2 s = ldc<PX, float2>(xs, o);
     double2 v = x[o];
     v.x += is * (double)s.x;
@@ -198,7 +198,7 @@ __global__ void k_rs_from_true(Geom g, void *rs, const double2 *rt, Scal *S) {
   const double sg = S->sigma;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const long o = c * g.stride + i;
+    const long o = vix(g, c, i);
     double2 v = rt[o];
     stc<PR, float2>(rs, o, make_float2((float)(sg * v.x), (float)(sg * v.y)));
   }
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(256) k_cgm_xr(Geom g, int nshift, double2 *r, 
     const double alpha = T->alpha;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const long o = c * g.stride + i;
+      const long o = vix(g, c, i);
       double2 rv = r[o], av = Ap[o];
       rv.x -= alpha * av.x; rv.y -= alpha * av.y;
       r[o] = rv;
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(256) k_cgm_xr(Geom g, int nshift, double2 *r, 
       const double2 *pk = p[k];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const long o = c * g.stride + i;
+        const long o = vix(g, c, i);
         double2 xv = xk[o], pv = pk[o];
         xv.x += xc * pv.x; xv.y += xc * pv.y;
         xk[o] = xv;
@@ -305,13 +305,13 @@ __global__ void k_cgm_p(Geom g, int nshift, const double2 *r, double2 *const *p,
   if (i >= g.Vh) return;
   double2 rv[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++) rv[c] = r[c * g.stride + i];
+  for (int c = 0; c < 3; c++) rv[c] = r[vix(g, c, i)];
   for (int k = 0; k < nshift; k++) {
     const double a = T->pc1[k], b = T->pc2[k];
     double2 *pk = p[k];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const long o = c * g.stride + i;
+      const long o = vix(g, c, i);
       double2 pv = pk[o];
       pk[o] = make_double2(a * rv[c].x + b * pv.x, a * rv[c].y + b * pv.y);
     }
@@ -353,8 +353,8 @@ __global__ void __launch_bounds__(256) k_force(Geom g, const char *links, long l
   double2 a[3], b[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    a[c] = ps[c * g.stride + i];
-    b[c] = pn[c * g.stride + j];
+    a[c] = ps[vix(g, c, i)];
+    b[c] = pn[vix(g, c, j)];
     a[c].x *= eps; a[c].y *= eps;
   }
   double2 G[9];
@@ -424,8 +424,8 @@ __global__ void __launch_bounds__(256) k_outer_acc(Geom g, const double2 *ae, co
   double2 a[3], b[3];
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    a[q] = as[q * g.stride + i];
-    b[q] = bn[q * g.stride + j];
+    a[q] = as[vix(g, q, i)];
+    b[q] = bn[vix(g, q, j)];
     a[q].x *= c;
     a[q].y *= c;
   }

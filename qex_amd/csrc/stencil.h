@@ -181,13 +181,13 @@ __device__ __forceinline__ void cg1_body(const CgArgs &P, long blk) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       A rn, pv, av;
-      p_next<PS, PR, A>(P, c * g.stride + j, alpha, beta, first, rn, v[c], pv, av);
+      p_next<PS, PR, A>(P, vix(g, c, j), alpha, beta, first, rn, v[c], pv, av);
     }
   };
   A res[3];
   stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, q, i, s, ld, res);
 #pragma unroll
-  for (int c = 0; c < 3; c++) stc<PS, A>(P.t, c * g.stride + i, res[c]);
+  for (int c = 0; c < 3; c++) stc<PS, A>(P.t, vix(g, c, i), res[c]);
 }
 
 // lds: 16*4 doubles; the block's four partial sums go to slot pidx
@@ -206,7 +206,7 @@ __device__ __forceinline__ void cg2_body(const CgArgs &P, long blk, double *lds,
     A rv[3], av[3], pv[3], xv[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const long off = c * g.stride + i;
+      const long off = vix(g, c, i);
       rv[c] = ldc<PR, A>(P.r, off);
       av[c] = ldc<PS, A>(P.Ap, off);
       pv[c] = first ? mk<A>(0, 0) : ldc<PS, A>(P.p, off);
@@ -215,13 +215,13 @@ __device__ __forceinline__ void cg2_body(const CgArgs &P, long blk, double *lds,
     const Site s = decode(g, i, P.par);
     auto ld = [&](long j, A(&v)[3]) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) v[c] = ldc<PS, A>(P.t, c * g.stride + j);
+      for (int c = 0; c < 3; c++) v[c] = ldc<PS, A>(P.t, vix(g, c, j));
     };
     A H[3];
     stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, P.par, i, s, ld, H);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const long off = c * g.stride + i;
+      const long off = vix(g, c, i);
       const A rn = rt_store<PR, A>(mk<A>(rv[c].x - alpha * av[c].x, rv[c].y - alpha * av[c].y));
       const A pn = rt_store<PS, A>(first ? rn : mk<A>(rn.x + beta * pv[c].x, rn.y + beta * pv[c].y));
       const A xn = mk<A>(xv[c].x + alpha * pv[c].x, xv[c].y + alpha * pv[c].y);
@@ -257,7 +257,7 @@ __device__ __forceinline__ void cg0_body(const CgArgs &P, long blk) {
   const R alpha = (R)S->alpha, beta = (R)(S->beta * S->pscale);
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const long off = c * g.stride + i;
+    const long off = vix(g, c, i);
     const A rv = ldc<PR, A>(P.r, off), av = ldc<PS, A>(P.Ap, off);
     const A pv = first ? mk<A>(0, 0) : ldc<PS, A>(P.p, off);
     const A xv = ldc<PX, A>(P.x, off);
@@ -280,12 +280,12 @@ __device__ __forceinline__ void cg1b_body(const CgArgs &P, long blk) {
   const Site s = decode(g, i, q);
   auto ld = [&](long j, A(&v)[3]) {
 #pragma unroll
-    for (int c = 0; c < 3; c++) v[c] = ldc<PS, A>(P.p, c * g.stride + j);
+    for (int c = 0; c < 3; c++) v[c] = ldc<PS, A>(P.p, vix(g, c, j));
   };
   A res[3];
   stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, q, i, s, ld, res);
 #pragma unroll
-  for (int c = 0; c < 3; c++) stc_out<PS, A>(P.t, c * g.stride + i, res[c]);
+  for (int c = 0; c < 3; c++) stc_out<PS, A>(P.t, vix(g, c, i), res[c]);
 }
 
 template <int PL, int RC, int PS, int PR, class A, int NL>
@@ -301,21 +301,21 @@ __device__ __forceinline__ void cg2b_body(const CgArgs &P, long blk, double *lds
     A rv[3], pv[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const long off = c * g.stride + i;
+      const long off = vix(g, c, i);
       rv[c] = ldc<PR, A>(P.r, off);
       pv[c] = ldc<PS, A>(P.p, off);
     }
     const Site s = decode(g, i, P.par);
     auto ld = [&](long j, A(&v)[3]) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) v[c] = ldc<PS, A>(P.t, c * g.stride + j);
+      for (int c = 0; c < 3; c++) v[c] = ldc<PS, A>(P.t, vix(g, c, j));
     };
     A H[3];
     stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, P.par, i, s, ld, H);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const A apn = rt_store<PS, A>(mk<A>(m4 * pv[c].x - H[c].x, m4 * pv[c].y - H[c].y));
-      stc_out<PS, A>(P.Ap, c * g.stride + i, apn);
+      stc_out<PS, A>(P.Ap, vix(g, c, i), apn);
       red[0] += (double)pv[c].x * (double)apn.x + (double)pv[c].y * (double)apn.y;
       red[1] += (double)rv[c].x * (double)apn.x + (double)rv[c].y * (double)apn.y;
       red[2] += (double)apn.x * (double)apn.x + (double)apn.y * (double)apn.y;
@@ -337,7 +337,7 @@ __device__ __forceinline__ void cg_flush_body(const CgArgs &P, long blk, double 
     const R alpha = (R)S->alpha;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const long off = c * g.stride + i;
+      const long off = vix(g, c, i);
       A rv = ldc<PR, A>(P.r, off), av = ldc<PS, A>(P.Ap, off), pv = ldc<PS, A>(P.p, off), xv = ldc<PX, A>(P.x, off);
       rv.x -= alpha * av.x; rv.y -= alpha * av.y;
       xv.x += alpha * pv.x; xv.y += alpha * pv.y;
