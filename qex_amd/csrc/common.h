@@ -337,7 +337,12 @@ typedef unsigned int u2v __attribute__((ext_vector_type(2)));
 // 103.5 us, 12-real 79 -> 69 us, FP32 58.6 -> 54 us) and slower for the
 // 8-link HISQ FP64 hop (401 -> 421 us at 32^3 x 64) and some 16-bit plain
 // hops, which keep the plane layout.
-template <int PL, int NL> constexpr bool link_blocked() { return NL == 4 && (PL == P64 || PL == P32); }
+#ifndef QH_LBLK_HISQ
+#define QH_LBLK_HISQ 0
+#endif
+template <int PL, int NL> constexpr bool link_blocked() {
+  return (NL == 4 || QH_LBLK_HISQ) && (PL == P64 || PL == P32);
+}
 #ifndef QH_LBLK_LOG2
 #define QH_LBLK_LOG2 6  // sites per block: 64
 #endif

@@ -269,7 +269,7 @@ __global__ void k_pack_links(Geom g, int nl, const double2 *canon, char *out, lo
     }
   }
   // the layout each reader expects: blocked for plain links where that pays
-  if (nl == 4 && link_blocked<PL, 4>())
+  if (nl == 4 ? link_blocked<PL, 4>() : link_blocked<PL, 8>())
     st_link_words<PL, RC, true>(out + (long)(2 * k + par) * lkblk, g.stride, i, w);
   else
     st_link_words<PL, RC, false>(out + (long)(2 * k + par) * lkblk, g.stride, i, w);
