@@ -168,8 +168,8 @@ struct Geom {
 // (192 elements per block), one contiguous region per wave; measured at 32^4
 // against component planes a stride apart: FP64 hop 103.1 -> 101.3 us, FP16
 // and Q16 12-real hops -4 to -5%, FP64 CG -1.8%, FP32-inner CG -3%; the 8-link
-// HISQ FP64 hop +3.7% (397 -> 412 us at 32^3 x 64).  QH_VEC_BLOCKED=0 gives
-// the plane layout.
+// HISQ FP64 hop at 32^3 x 64 397 -> 412 us with its links in planes, 386 us
+// with them blocked as well.  QH_VEC_BLOCKED=0 gives the plane layout.
 #ifndef QH_VEC_BLOCKED
 #define QH_VEC_BLOCKED 1
 #endif
@@ -332,13 +332,14 @@ typedef unsigned int u2v __attribute__((ext_vector_type(2)));
 // together, plane p of site i at (i/64)*64*bytes + p*1024 + (i%64)*16 and the
 // remainder plane at (i/64)*64*bytes + nfull*1024 + (i%64)*rem, so that one
 // wave's link loads come from one contiguous region instead of nfull+1
-// regions a plane apart (stride is a multiple of 64).  Measured faster for
-// plain (4-link) FP64 and FP32 fields (one-parity hop at 32^4: FP64 110 ->
-// 103.5 us, 12-real 79 -> 69 us, FP32 58.6 -> 54 us) and slower for the
-// 8-link HISQ FP64 hop (401 -> 421 us at 32^3 x 64) and some 16-bit plain
-// hops, which keep the plane layout.
+// regions a plane apart (stride is a multiple of 64).  Used for FP64 and FP32
+// fields, plain and HISQ (one-parity hop at 32^4: FP64 110 -> 103.5 us,
+// 12-real 79 -> 69 us, FP32 58.6 -> 54 us; the 8-link HISQ FP64 hop at
+// 32^3 x 64: 410 -> 386 us with blocked color vectors, though 401 -> 421 us
+// with planar ones).  The 16-bit formats moved within +-2% and keep the plane
+// layout.  QH_LBLK_HISQ=0 keeps HISQ link sets in planes.
 #ifndef QH_LBLK_HISQ
-#define QH_LBLK_HISQ 0
+#define QH_LBLK_HISQ 1
 #endif
 template <int PL, int NL> constexpr bool link_blocked() {
   return (NL == 4 || QH_LBLK_HISQ) && (PL == P64 || PL == P32);

@@ -268,7 +268,7 @@ __global__ void k_pack_links(Geom g, int nl, const double2 *canon, char *out, lo
       w[c] = f_to_bf16((float)d.x) | (f_to_bf16((float)d.y) << 16);
     }
   }
-  // the layout each reader expects: blocked for plain links where that pays
+  // the layout each reader expects: blocked for FP64 and FP32 links
   if (nl == 4 ? link_blocked<PL, 4>() : link_blocked<PL, 8>())
     st_link_words<PL, RC, true>(out + (long)(2 * k + par) * lkblk, g.stride, i, w);
   else
