@@ -236,7 +236,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* 16*NV 
   __syncthreads();
   // the first wave adds the (at most 16) wave sums: lanes 0-15 each take one,
   // then a 4-step butterfly leaves the total in every one of them (one thread
-  // adding them in turn made the one-block finish of the CG dot products
+  // adding them in turn made the 1024-thread finish of the CG dot products
   // 6.2 us instead of 5.4 us)
   if (w == 0) {
 #pragma unroll

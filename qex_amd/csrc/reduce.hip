@@ -7,13 +7,13 @@
 namespace qh {
 
 template <int NV, int EPI>
-__global__ void __launch_bounds__(1024) k_finish(FinArgs F) {
+__global__ void __launch_bounds__(QH_FIN_THREADS) k_finish(FinArgs F) {
   __shared__ double lds[16 * NV];
   finish_body<NV, EPI>(F, lds);
 }
 
 void launch_finish(int epi, const FinArgs &f, hipStream_t st) {
-  const dim3 g(1), b(1024);
+  const dim3 g(1), b(QH_FIN_THREADS);
   switch (epi) {
     case EPI_TMP: hipLaunchKernelGGL((k_finish<1, EPI_TMP>), g, b, 0, st, f); break;
     case EPI_PAP: hipLaunchKernelGGL((k_finish<1, EPI_PAP>), g, b, 0, st, f); break;
