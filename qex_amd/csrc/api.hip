@@ -91,7 +91,7 @@ struct qexhip_lattice {
 struct qexhip_field {
   qexhip_lattice *lat;
   int prec;
-  char *data;  // [parity][3][stride] words
+  char *data;  // per parity: stride/64 blocks of [3 colors][64 sites] words
   size_t par_bytes;
   void *par(int p) const { return data + p * par_bytes; }
 };
@@ -286,8 +286,10 @@ Tuned tune(const std::string &key, const std::function<void(const Tuned &)> &f,
   const int fixed_form = g_cg_form ? g_cg_form : 2;
   if (g_block_forced || !g_autotune) return Tuned{g_block, g_remap, 0.0, fixed_form, 0};
   Tuned best{256, 1, 1e300, fixed_form, 0};
+  bool pinned = false;  // a pinned CG form restricts only the lists that hold it
+  for (int form : forms) pinned |= (g_cg_form != 0 && form == g_cg_form);
   for (int form : forms) {
-    if (g_cg_form && form != g_cg_form) continue;
+    if (pinned && form != g_cg_form) continue;
     for (int cap : caps)
       for (int block : {64, 128, 256})
         for (int remap : {1, 0}) {
