@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--gauge-io", action="store_true", help="only the SciDAC gauge-file study")
     ap.add_argument("--hisq-links", action="store_true",
                     help="only run the HISQ link-construction study (32^3x64 smearing)")
+    ap.add_argument("--hisq-hop", action="store_true",
+                    help="only time the HISQ-form one-parity hop at 32^3x64 (--steps launches per precision)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -174,6 +176,26 @@ def _solve_stats(Q, torch, s, x, b, mass, r2req, maxits, sloppy, sloppy_links=No
     dt = time.perf_counter() - t0
     return {"seconds": round(dt, 5), "iterations": sp.iterations, "reliable_updates": sp.reliableUpdates,
             "true_rel_resid": float(sp.r2) ** 0.5, "qex_gflops": round(1e-9 * sp.flops / dt, 1)}
+
+
+def hisq_hop_study(steps, warmup):
+    """HISQ-form one-parity hop at 32^3x64 (8 fat + 8 long links, 2400 B per
+    output site in FP64): HIP-event time per launch for each storage precision."""
+    import qex_amd as Q
+    from qex_amd import gauge
+    L = (32, 32, 32, 64)
+    lo = Q.Layout(L)
+    U = gauge.random_gauge(lo, SEED)
+    gauge.phase_bc(lo, U)
+    s = Q.Staggered(lo, gauge.hisq_links(lo, SEED + 1, U))
+    out = {"lattice": list(L), "steps": steps, "hop": {}}
+    for prec in (0, 1, 2, 4):
+        Q.time_hop(s, prec, warmup)
+        ms = Q.time_hop(s, prec, steps)
+        out["hop"][PREC_NAME[prec]] = {"ms": round(ms, 5), "algorithmic_bytes_per_launch":
+                                       HISQ_REALS * BYTES_PER_REAL[prec] * lo.nEven,
+                                       "GB_s": round(HISQ_REALS * BYTES_PER_REAL[prec] * lo.nEven / (ms * 1e-3) / 1e9, 1)}
+    return out
 
 
 def hisq_study():
@@ -533,6 +555,9 @@ def main():
     _lib.init(0)
     if args.hisq_links:
         print(json.dumps({"hisq_links": hisq_links_study(cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.hisq_hop:
+        print(json.dumps({"hisq_hop": hisq_hop_study(args.steps, args.warmup)}), flush=True)
         return
     if args.gauge_io:
         print(json.dumps({"gauge_io": gauge_io_study()}), flush=True)
