@@ -232,6 +232,7 @@ struct Tuned {
   double us;
   int form = 2;  // CG iteration: 2 or 3 kernels (dslash.hip)
   int cap = 0;   // hop: persistent form with at most cap workgroups (0: one pass)
+  int block1 = 0;  // CG form 3: block size of the other-parity stencil kernel (0: block)
 };
 std::map<std::string, Tuned> g_tune;
 
@@ -392,6 +393,25 @@ Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, int rx, const CgArgs &re
         launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, c.block, c.form, rx);
       },
       {2, 3});
+  // form 3: the other-parity stencil kernel reduces nothing, so its block size
+  // is free; time the iteration with each while the rest keep the chosen shape
+  if (t.form == 3) {
+    Tuned best = t;
+    best.us = 1e300;
+    a.remap = t.remap;
+    for (int b1 : {64, 128, 256}) {
+      const double us = time_us([&] { launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, t.block, 3, rx, b1); }, 2, 8);
+      if (us < best.us) {
+        best.us = us;
+        best.block1 = b1;
+      }
+    }
+    g_tune[key] = best;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    for (int k = 0; k < 5; k++) (void)hipFree(v[k]);
+    (void)hipFree(S);
+    return best;
+  }
   HIPCHK(hipStreamSynchronize(g_stream));
   for (int k = 0; k < 5; k++) (void)hipFree(v[k]);
   (void)hipFree(S);
@@ -490,7 +510,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   auto chunk = [&] {
-    for (int k = 0; k < kChunk; k++) launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, tu.block, tu.form, rx);
+    for (int k = 0; k < kChunk; k++) launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, tu.block, tu.form, rx, tu.block1);
   };
   if (g_cg_graph && !Sh->done) {
     capture(chunk, &graph, &exec);
@@ -2114,8 +2134,8 @@ int qexhip_autotune_report(char *buf, int len) {
   std::string out;
   for (auto &kv : g_tune) {
     char line[256];
-    snprintf(line, sizeof line, "%s: block=%d remap=%d form=%d cap=%d us=%.2f\n", kv.first.c_str(),
-             kv.second.block, kv.second.remap, kv.second.form, kv.second.cap, kv.second.us);
+    snprintf(line, sizeof line, "%s: block=%d remap=%d form=%d cap=%d block1=%d us=%.2f\n", kv.first.c_str(),
+             kv.second.block, kv.second.remap, kv.second.form, kv.second.cap, kv.second.block1, kv.second.us);
     out += line;
   }
   snprintf(buf, (size_t)len, "%s", out.c_str());

@@ -166,12 +166,15 @@ __global__ void __launch_bounds__(256) k_cg_flush(CgArgs P) {
 }
 
 template <int PL, int RC, int PS, int PRX, int NL>
-static void cg_iter_t(const CgArgs &a, hipStream_t st, int block, int form) {
+static void cg_iter_t(const CgArgs &a, hipStream_t st, int block, int form, int block1) {
   dim3 grid((unsigned)((a.g.Vh + block - 1) / block)), blk(block);
+  // the other-parity stencil kernel of form 3 may run its own block size
+  const int b1 = block1 > 0 ? block1 : block;
+  dim3 grid1((unsigned)((a.g.Vh + b1 - 1) / b1)), blk1(b1);
   if constexpr (PL == P64) {
     if (form == 3) {
       hipLaunchKernelGGL((k_cg0<P64, P64, P64, double2>), grid, blk, 0, st, a);
-      hipLaunchKernelGGL((k_cg1b<PL, RC, P64, double2, NL>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg1b<PL, RC, P64, double2, NL>), grid1, blk1, 0, st, a);
       hipLaunchKernelGGL((k_cg2b<PL, RC, P64, P64, double2, NL>), grid, blk, 0, st, a);
     } else {
       hipLaunchKernelGGL((k_cg1<PL, RC, P64, P64, P64, double2, NL>), grid, blk, 0, st, a);
@@ -180,7 +183,7 @@ static void cg_iter_t(const CgArgs &a, hipStream_t st, int block, int form) {
   } else {
     if (form == 3) {
       hipLaunchKernelGGL((k_cg0<PS, PRX, PRX, float2>), grid, blk, 0, st, a);
-      hipLaunchKernelGGL((k_cg1b<PL, RC, PS, float2, NL>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg1b<PL, RC, PS, float2, NL>), grid1, blk1, 0, st, a);
       hipLaunchKernelGGL((k_cg2b<PL, RC, PS, PRX, float2, NL>), grid, blk, 0, st, a);
     } else {
       hipLaunchKernelGGL((k_cg1<PL, RC, PS, PRX, PRX, float2, NL>), grid, blk, 0, st, a);
@@ -192,29 +195,30 @@ static void cg_iter_t(const CgArgs &a, hipStream_t st, int block, int form) {
 }
 
 template <int PL, int PS, int PRX = P32>
-static bool cg_iter_rc(int rc, int nl, const CgArgs &a, hipStream_t st, int block, int form) {
-  if (rc == 18 && nl == 4) { cg_iter_t<PL, 18, PS, PRX, 4>(a, st, block, form); return true; }
-  if (rc == 18 && nl == 8) { cg_iter_t<PL, 18, PS, PRX, 8>(a, st, block, form); return true; }
-  if (rc == 12 && nl == 4) { cg_iter_t<PL, 12, PS, PRX, 4>(a, st, block, form); return true; }
+static bool cg_iter_rc(int rc, int nl, const CgArgs &a, hipStream_t st, int block, int form, int b1) {
+  if (rc == 18 && nl == 4) { cg_iter_t<PL, 18, PS, PRX, 4>(a, st, block, form, b1); return true; }
+  if (rc == 18 && nl == 8) { cg_iter_t<PL, 18, PS, PRX, 8>(a, st, block, form, b1); return true; }
+  if (rc == 12 && nl == 4) { cg_iter_t<PL, 12, PS, PRX, 4>(a, st, block, form, b1); return true; }
   return false;
 }
 
 // form 2: two stencil kernels (p formed on the fly); form 3: a streaming update
 // kernel plus two stencil kernels that gather one vector each.  rx: storage of
 // the sloppy residual and solution increment (P32, or P16 with FP16 vectors).
-void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form, int rx) {
+void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form, int rx,
+                    int block1) {
   bool ok = false;
   if (rx == P16 && ps == P16) {
-    if (pl == P16) ok = cg_iter_rc<P16, P16, P16>(rc, nl, a, st, block, form);
-    else if (pl == PQ16) ok = cg_iter_rc<PQ16, P16, P16>(rc, nl, a, st, block, form);
-  } else if (pl == P64 && ps == P64) ok = cg_iter_rc<P64, P64>(rc, nl, a, st, block, form);
-  else if (pl == P32 && ps == P32) ok = cg_iter_rc<P32, P32>(rc, nl, a, st, block, form);
-  else if (pl == P16 && ps == P16) ok = cg_iter_rc<P16, P16>(rc, nl, a, st, block, form);
-  else if (pl == P16 && ps == P32) ok = cg_iter_rc<P16, P32>(rc, nl, a, st, block, form);
-  else if (pl == PBF && ps == PBF) ok = cg_iter_rc<PBF, PBF>(rc, nl, a, st, block, form);
-  else if (pl == PBF && ps == P32) ok = cg_iter_rc<PBF, P32>(rc, nl, a, st, block, form);
-  else if (pl == PQ16 && ps == P16) ok = cg_iter_rc<PQ16, P16>(rc, nl, a, st, block, form);
-  else if (pl == PQ16 && ps == P32) ok = cg_iter_rc<PQ16, P32>(rc, nl, a, st, block, form);
+    if (pl == P16) ok = cg_iter_rc<P16, P16, P16>(rc, nl, a, st, block, form, block1);
+    else if (pl == PQ16) ok = cg_iter_rc<PQ16, P16, P16>(rc, nl, a, st, block, form, block1);
+  } else if (pl == P64 && ps == P64) ok = cg_iter_rc<P64, P64>(rc, nl, a, st, block, form, block1);
+  else if (pl == P32 && ps == P32) ok = cg_iter_rc<P32, P32>(rc, nl, a, st, block, form, block1);
+  else if (pl == P16 && ps == P16) ok = cg_iter_rc<P16, P16>(rc, nl, a, st, block, form, block1);
+  else if (pl == P16 && ps == P32) ok = cg_iter_rc<P16, P32>(rc, nl, a, st, block, form, block1);
+  else if (pl == PBF && ps == PBF) ok = cg_iter_rc<PBF, PBF>(rc, nl, a, st, block, form, block1);
+  else if (pl == PBF && ps == P32) ok = cg_iter_rc<PBF, P32>(rc, nl, a, st, block, form, block1);
+  else if (pl == PQ16 && ps == P16) ok = cg_iter_rc<PQ16, P16>(rc, nl, a, st, block, form, block1);
+  else if (pl == PQ16 && ps == P32) ok = cg_iter_rc<PQ16, P32>(rc, nl, a, st, block, form, block1);
   if (!ok)
     fprintf(stderr, "qexhip: no CG instantiation for pl=%d rc=%d ps=%d rx=%d nl=%d\n", pl, rc, ps, rx, nl);
 }

@@ -145,7 +145,7 @@ void launch_finish(int epi, const FinArgs &f, hipStream_t st);
 // dslash.hip
 void launch_hop(const HopKey &k, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block, int cap);
 void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form,
-                    int rx = P32);
+                    int rx = P32, int block1 = 0);
 void launch_cg_flush(int ps, int rx, const CgArgs &a, hipStream_t st);
 void launch_pack_links(int prec, int rc, const Geom &g, int nl, const double *canon, char *out, const float *qs,
                        hipStream_t st);

@@ -468,9 +468,10 @@ def set_cg_form(form=0):
 
 
 def autotune_report():
-    """{kernel family and shape: {"block", "remap", "form", "cap", "us"}} chosen so
-    far (form: 2- or 3-kernel CG iteration; cap: workgroup cap of the persistent
-    hop, 0 = one pass; us: time of one timed unit)."""
+    """{kernel family and shape: {"block", "remap", "form", "cap", "block1", "us"}}
+    chosen so far (form: 2- or 3-kernel CG iteration; cap: workgroup cap of the
+    persistent hop, 0 = one pass; block1: block size of the 3-kernel iteration's
+    other-parity stencil kernel, 0 = block; us: time of one timed unit)."""
     buf = ctypes.create_string_buffer(1 << 16)
     call("qexhip_autotune_report", buf, len(buf))
     out = {}
@@ -478,7 +479,7 @@ def autotune_report():
         key, _, rest = line.rpartition(": ")
         f = dict(kv.split("=") for kv in rest.split())
         out[key] = {"block": int(f["block"]), "remap": int(f["remap"]), "form": int(f["form"]),
-                    "cap": int(f["cap"]), "us": float(f["us"])}
+                    "cap": int(f["cap"]), "block1": int(f.get("block1", 0)), "us": float(f["us"])}
     return out
 
 
