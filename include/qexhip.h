@@ -68,9 +68,14 @@ typedef struct {
   /* outputs */
   int iterations;         /* QEX iterations */
   int reliable_updates;   /* FP64 reliable updates performed (mixed mode) */
-  double r2;              /* final FP64 true |b-Ax|^2/|b|^2 */
+  double r2;              /* final FP64 true |b-Ax|^2/|b|^2 (multi-shift: of the base system
+                             sigma_0 = 0, recomputed from the returned solution) */
   double seconds;         /* wall time of the call */
   double flops;           /* QEX flop count (nl*4*72+60)*nEven*its (stagSolve.nim:121) */
+  /* multi-shift solves only: NULL, or an array of nshift doubles that receives
+     each shifted system's FP64 true |b-(A+sigma_k)x_k|^2/|b|^2 (the residuals
+     cgm.nim:208-239 prints at verbosity 3); the caller owns it */
+  double *shift_r2;
 } qexhip_solver_params;
 
 /* ---- library lifetime (QEX qexGlobalInitializers/Finalizers pattern,

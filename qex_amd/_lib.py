@@ -56,6 +56,7 @@ class SolverParamsC(ctypes.Structure):
         ("r2", ctypes.c_double),
         ("seconds", ctypes.c_double),
         ("flops", ctypes.c_double),
+        ("shift_r2", ctypes.POINTER(ctypes.c_double)),
     ]
 
 

@@ -599,6 +599,34 @@ void solve_xx(qexhip_links *l, int par, double *r, const double *x, double m, qe
   sp->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// |b - (A + sigma) x|^2 in FP64 on parity par, A = 4(m^2 - D_pq D_qp): the
+// true residual of one shifted system, which cgm.nim:208-239 computes for
+// every shift at verbosity 3.  Uses the links' FP64 work vectors r64, t64.
+double shifted_resid2(qexhip_links *l, int par, const double *x, const double *b, double m, double sigma) {
+  Work *w = work(l);
+  Scal *S = w->S, *Sh = w->Sh;
+  memset(Sh, 0, sizeof(Scal));
+  Sh->m2 = m * m + 0.25 * sigma;
+  push(S, Sh);
+  resid64(l, par, x, b, w->r64, S, 3, w->t64);
+  check_launch();
+  pull(Sh, S);
+  return Sh->tmp[0];
+}
+
+// Fill the true-residual outputs of a multi-shift solve: sp->r2 is the base
+// system's |b - A x_0|^2/|b|^2 and sp->shift_r2[k] (when given) each shifted
+// system's, all recomputed in FP64 from the returned solutions.
+void multishift_true_r2(qexhip_links *l, int par, const double *const *xs, const double *b, double b2, double m,
+                        const double *sg, int nshift, qexhip_solver_params *sp) {
+  for (int k = 0; k < nshift; k++) {
+    if (k > 0 && !sp->shift_r2) break;
+    const double r2 = (b2 > 0) ? shifted_resid2(l, par, xs[k], b, m, k == 0 ? 0.0 : sg[k]) / b2 : 0.0;
+    if (k == 0) sp->r2 = r2;
+    if (sp->shift_r2) sp->shift_r2[k] = r2;
+  }
+}
+
 // Multi-shift CG (src/solvers/cgm.nim:68-246), FP64, parity par.
 void solve_ms(qexhip_links *l, int par, double *const *xs, const double *b, double m, const double *sigmas,
               int nshift, qexhip_solver_params *sp) {
@@ -666,8 +694,8 @@ void solve_ms(qexhip_links *l, int par, double *const *xs, const double *b, doub
     if (!Th->continuing) break;
   }
   sp->iterations = Th->its;
-  sp->r2 = (b2 > 0) ? Th->r2i / b2 : 0.0;
   sp->flops = flops_per_its(l) * Th->its + (double)(4 * g.Vh + 15) * Th->its * nshift;
+  multishift_true_r2(l, par, xs, b, b2, m, sigmas, nshift, sp);
   sp->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -1127,11 +1155,15 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
       HIPCHK(hipMemcpyAsync(xs[j * nshift + k]->par(par), w->x[j * nshift + k], vb, hipMemcpyDeviceToDevice,
                             g_stream));
   HIPCHK(hipStreamSynchronize(g_stream));
+  for (int j = 0; j < nrhs; j++) {
+    std::vector<const double *> xj(nshift);
+    for (int k = 0; k < nshift; k++) xj[k] = (const double *)xs[j * nshift + k]->par(par);
+    multishift_true_r2(l, par, xj.data(), (const double *)bf[j]->par(par), b2[j], m, sigmas, nshift, &sp[j]);
+  }
   const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (int j = 0; j < nrhs; j++) {
     sp[j].iterations = w->Th[j].its;
     sp[j].reliable_updates = 0;
-    sp[j].r2 = (b2[j] > 0) ? w->Th[j].r2i / b2[j] : 0.0;
     sp[j].flops = flops_per_its(l) * w->Th[j].its + (double)(4 * lat->g.Vh + 15) * w->Th[j].its * nshift;
     sp[j].seconds = sec;
   }
@@ -1205,7 +1237,8 @@ void *qexhip_stream(void) { return (void *)g_stream; }
 
 int qexhip_lattice_create(const int dims[4], qexhip_lattice **out) {
   API_BEGIN
-  check_init();
+  // the extents are checked before the device, so that a caller learns that a
+  // lattice is refused without a GPU
   require(dims != nullptr && out != nullptr, "null argument");
   long vol = 1;
   for (int d = 0; d < 4; d++) {
@@ -1215,6 +1248,19 @@ int qexhip_lattice_create(const int dims[4], qexhip_lattice **out) {
     // site indices and the generators' site counters are 32-bit
     require(vol <= (1L << 31), "lattice too large: more than 2^31 sites");
   }
+  {
+    // Kernels address one parity of a color-vector field (3 x stride complex
+    // words), and a link block in the plane layout, with 32-bit byte offsets
+    // from the field's base: the FP64 vector block, 48 B x stride, must stay
+    // below 4 GiB (at most 89 478 400 sites per parity, 178 956 800 sites).
+    // Blocked link storage (FP64/FP32) takes a 64-bit base per 64-site block.
+    const long stride = (vol / 2 + 255) / 256 * 256;
+    const long plane_link_bytes = link_blocked<P64, 8>() ? 36 : 144;  // FP16 18-real, or FP64 HISQ planes
+    require(stride * 48 < (1L << 32) && stride * plane_link_bytes < (1L << 32),
+            "lattice too large: one parity of an FP64 color vector would pass 4 GiB "
+            "(at most 178956800 sites)");
+  }
+  check_init();
   qexhip_lattice *lat = new qexhip_lattice();
   for (int d = 0; d < 4; d++) lat->g.L[d] = dims[d];
   lat->g.hx = dims[0] / 2;

@@ -393,17 +393,33 @@ template <class T, bool NT> __device__ __forceinline__ T ldv(const char *p) {
 
 // Load one link (LFmt layout, see above) straight into 3x3 complex form,
 // rebuilding row 2 for 12-real storage: row2_k = sgn*conj(row0 x row1)_k.
+// Where ld_link reads plane p (p == nfull: the remainder plane) of the link of
+// site i: a base pointer and a 32-bit offset from it.  Blocked storage moves
+// the base to the site's 64-site block in 64-bit arithmetic (an FP64 18-real
+// link block passes 4 GiB from 29.8M sites per parity on), so the offset
+// stays below one block (9 KiB); the plane layout keeps the block pointer and
+// a 32-bit offset, which qexhip_lattice_create bounds (36 B x stride for the
+// 16-bit formats).  The host test of the address arithmetic calls this too.
+template <int PL, int RC, bool BLK>
+__host__ __device__ __forceinline__ const char *link_base(const char *blk, long i) {
+  using F = LFmt<PL, RC>;
+  return BLK ? blk + (size_t)(i >> QH_LBLK_LOG2) * (size_t)(LBLK * F::bytes) : blk;
+}
+template <int PL, int RC, bool BLK>
+__host__ __device__ __forceinline__ uint32_t link_off(long stride, long i, int p) {
+  using F = LFmt<PL, RC>;
+  const uint32_t lane = (uint32_t)(BLK ? (i & (LBLK - 1)) : i);
+  const uint32_t pstep = BLK ? (uint32_t)LBLK * 16u : (uint32_t)stride * 16u;
+  return p < F::nfull ? (uint32_t)p * pstep + lane * 16u : (uint32_t)F::nfull * pstep + lane * (uint32_t)F::rem;
+}
+
 template <int PL, int RC, bool NT, class A, bool BLK>
-__device__ __forceinline__ void ld_link(const char *blk, long stride, long i, A (&U)[9], typename cx<A>::R sgn,
+__device__ __forceinline__ void ld_link(const char *blk0, long stride, long i, A (&U)[9], typename cx<A>::R sgn,
                                         float qscale = 0.0f) {
   using F = LFmt<PL, RC>;
-  // plane p and remainder offsets of site i (uint32: a link block is < 4 GiB)
-  constexpr bool B = BLK;
-  const uint32_t base = B ? (uint32_t)(i >> QH_LBLK_LOG2) * (uint32_t)(LBLK * F::bytes) : 0u;
-  const uint32_t lane = (uint32_t)(B ? (i & (LBLK - 1)) : i);
-  const uint32_t pstep = B ? (uint32_t)LBLK * 16u : (uint32_t)stride * 16u, l16 = lane * 16u;
-  const uint32_t rbase = base + (uint32_t)F::nfull * pstep + lane * (uint32_t)F::rem;
-  auto poff = [&](int p) { return base + (uint32_t)p * pstep + l16; };
+  const char *blk = link_base<PL, RC, BLK>(blk0, i);
+  const uint32_t rbase = link_off<PL, RC, BLK>(stride, i, F::nfull);
+  auto poff = [&](int p) { return link_off<PL, RC, BLK>(stride, i, p); };
   if constexpr (PL == P64) {
 #pragma unroll
     for (int c = 0; c < RC / 2; c++) {

@@ -338,6 +338,8 @@ class SolverParams:
         self.flops = 0.0
         self.r2 = 0.0
         self.reliableUpdates = 0
+        # multi-shift solves: FP64 true |b-(A+sigma_k)x_k|^2/|b|^2 per shift
+        self.shiftR2 = []
 
     @property
     def finalIterations(self):
@@ -405,8 +407,11 @@ def solveMulti(s, xs, b, m, sigmas, sp, subset="even"):
     c = sp._c()
     arr = (ctypes.c_void_p * len(xs))(*[f.h.value for f in xs])
     sig = (ctypes.c_double * len(sigmas))(*[float(v) for v in sigmas])
+    r2s = (ctypes.c_double * len(sigmas))()
+    c.shift_r2 = r2s
     call("qexhip_solve_multishift", s.h, _par(subset), arr, b.h, float(m), sig, len(sigmas), ctypes.byref(c))
     sp._add(c)
+    sp.shiftR2 = list(r2s)
     return c
 
 
@@ -442,12 +447,16 @@ def solveMultiBatch(s, xs, bs, m, sigmas, sps, subset="even", graph=True, tune=T
     if len(xs) != nr or any(len(row) != ns for row in xs) or len(sps) != nr:
         raise QexHipError("solveMultiBatch: xs must be nrhs lists of nshift fields, one SolverParams per rhs")
     cs = (_lib.SolverParamsC * nr)(*[sp._c() for sp in sps])
+    r2s = [(ctypes.c_double * ns)() for _ in range(nr)]
+    for c, a in zip(cs, r2s):
+        c.shift_r2 = a
     flags = (_lib.BATCH_GRAPH if graph else 0) | (_lib.BATCH_TUNE if tune else 0) | \
         (0 if freeze else _lib.BATCH_NOFREEZE)
     call("qexhip_solve_multishift_batch", s.h, _par(subset), nr, _vec([f for row in xs for f in row]), _vec(bs),
          float(m), (ctypes.c_double * ns)(*[float(v) for v in sigmas]), ns, cs, flags)
-    for sp, c in zip(sps, cs):
+    for sp, c, a in zip(sps, cs, r2s):
         sp._add(c)
+        sp.shiftR2 = list(a)
     return sps
 
 

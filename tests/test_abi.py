@@ -43,8 +43,9 @@ def test_library_is_gfx950():
 def test_solver_params_struct_layout():
     from qex_amd._lib import SolverParamsC
     # doubles and ints in declaration order, natural alignment (include/qexhip.h)
-    assert ctypes.sizeof(SolverParamsC) == 80
+    assert ctypes.sizeof(SolverParamsC) == 88
     assert SolverParamsC.reliable_delta.offset == 24 and SolverParamsC.r2.offset == 56
+    assert SolverParamsC.shift_r2.offset == 80
 
 
 def test_solver_params_defaults_like_qex():

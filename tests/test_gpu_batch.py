@@ -13,6 +13,19 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def iteration_bar(its, ne):
+    """Allowed difference of FP64 CG iteration counts from the oracle's.
+
+    +-1 for a system that converges well below its dimension (3*ne complex
+    unknowns): the GPU's summation order (fixed by the launch shape) changes
+    the residuals by rounding only.  The (4,6,8,4) system of the uneven batch
+    (m = 0.058) needs 509 iterations for 1152 unknowns (0.44 of its
+    dimension): there CG's vectors have lost orthogonality and the residual
+    decreases in near-flat steps, so rounding moves the stopping point by up
+    to two steps (seen: 511 vs 509).  That system alone gets +-2."""
+    return 1 if its <= 0.25 * 3 * ne else 2
+
+
 def _system(O, gpu, L, seed, hisq=False):
     U = O.phased(L, O.gauge_random(L, seed))
     g = O.hisq_links(L, seed + 1, U) if hisq else U
@@ -39,9 +52,7 @@ def test_uneven_batch_matches_oracle_and_single(oracle, gpu, graph, tune):
     for k, (L, (lo, g, s)) in enumerate(zip(lats, systems)):
         assert sps[k].r2 <= 1e-20, (k, sps[k].r2)
         want, its, _ = O.cg(L, g, 0, bs[k], ms[k], 1e-20)
-        # the launch shape the tuner picks fixes the FP64 summation order; over
-        # ~500 iterations that moves the stopping point by a step or two
-        assert abs(sps[k].iterations - its) <= max(2, its // 100), (k, sps[k].iterations, its)
+        assert abs(sps[k].iterations - its) <= iteration_bar(its, lo.nEven), (k, sps[k].iterations, its)
         got = fx[k].get(sub=0)[: lo.nEven]
         assert rel(got, want[: lo.nEven]) < 1e-8, k
         r = bs[k] - O.stagD2xx(L, g, 0, fx[k].get(), ms[k] ** 2)
@@ -151,5 +162,5 @@ def test_uneven_batch_both_iteration_forms(oracle, gpu, form):
         gpu.set_cg_form(0)
     for k, (L, (lo, g, s)) in enumerate(zip(lats, systems)):
         want, its, _ = O.cg(L, g, 0, bs[k], [0.1, 0.2, 0.15][k], 1e-20)
-        assert sps[k].r2 <= 1e-20 and abs(sps[k].iterations - its) <= 2
+        assert sps[k].r2 <= 1e-20 and abs(sps[k].iterations - its) <= iteration_bar(its, lo.nEven)
         assert rel(fx[k].get(sub=0)[: lo.nEven], want[: lo.nEven]) < 1e-8

@@ -95,9 +95,13 @@ def test_multishift_matches_oracle(oracle, gpu):
     for k, f in enumerate(fxs):
         got = f.get(sub=0)[:ne]
         assert rel(got, xs_o[k][:ne]) < 1e-9
-        # each shifted system's own residual (oracle operator)
+        # each shifted system's own residual (oracle operator), equal to the
+        # FP64 true residual the solver reports for that shift
         r = b - O.stagD2xx(L, g, 0, f.get(), m * m) - sig[k] * f.get()
-        assert np.sqrt(O.norm2(r[:ne]) / O.norm2(b)) < 1e-9
+        r2 = O.norm2(r[:ne]) / O.norm2(b)
+        assert np.sqrt(r2) < 1e-9
+        assert abs(sp.shiftR2[k] - r2) <= 1e-2 * r2 + 1e-30, (k, sp.shiftR2[k], r2)
+    assert sp.r2 == sp.shiftR2[0] and sp.r2 <= 1.0001e-20
 
 
 def test_force_matches_oracle(oracle, gpu):
