@@ -125,6 +125,19 @@ int qexhip_stagD(qexhip_links *l, int par, qexhip_field *r, const qexhip_field *
                  double a);
 /* stagD2ee / stagD2oo (stagD.nim:485-520): r_p = 4*m2*x_p - (2D)_pq (2D)_qp x_p */
 int qexhip_stagD2xx(qexhip_links *l, int par, qexhip_field *r, const qexhip_field *x, double m2);
+/* No counterpart in QEX: qexhip_stagD2xx for callers that cannot allocate
+   device memory (code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc).  Same operator, arguments and result as
+   qexhip_stagD2xx; it calls no hipMalloc/hipFree and no allocating routine:
+   the odd-parity temporary lives in a static 128 MiB device buffer that each
+   GPU holds for the life of the process (lattices up to 2 796 032 sites per
+   parity in FP64, 5 592 064 in FP32; larger ones are refused with an error),
+   the launch shape is the tuner's earlier choice or the default (no tuning
+   run), and the links must already hold a copy in the fields' precision
+   (FP64 always; FP32 after any FP32 operator call or solve on them, else an
+   error).  FP64 or FP32 fields.  All calls on one GPU share the buffer:
+   calls that overlap on one GPU (other host threads, other streams) race. */
+int qexhip_stagD2xx_noalloc(qexhip_links *l, int par, qexhip_field *r, const qexhip_field *x, double m2);
 /* multi-RHS stagD2 (stagDPN/stagDN, stagD.nim:132-199,451-456): r_j = a*r_j + b*x_j + (2D)x_j
    for j < n on parity par, each link loaded once for up to 8 fields; FP64 or FP32 fields.
    QEX's stagDN(sd, r, g, x, m) is a = 0, b = m. */

@@ -34,6 +34,7 @@ EXPORTS = [
     "qexhip_hisq_force", "qexhip_milc_read_info", "qexhip_milc_read", "qexhip_milc_write",
     "qexhip_scidac_read_info", "qexhip_scidac_read", "qexhip_scidac_create", "qexhip_scidac_append",
     "qexhip_time_scidac", "qexhip_from_qex_layout", "qexhip_to_qex_layout", "qexhip_outer_acc",
+    "qexhip_stagD2xx_noalloc",
 ]
 
 # qexhip_solve_batch / qexhip_solve_multishift_batch flags
@@ -113,7 +114,7 @@ def load(path=LIB_PATH):
         "qexhip_field_norm2": [vp, I, dp], "qexhip_field_redot": [vp, vp, I, dp],
         "qexhip_field_axpby": [vp, vp, I, D, D],
         "qexhip_stagD2": [vp, I, vp, vp, D, D], "qexhip_stagD": [vp, I, vp, vp, D, D, D],
-        "qexhip_stagD2xx": [vp, I, vp, vp, D],
+        "qexhip_stagD2xx": [vp, I, vp, vp, D], "qexhip_stagD2xx_noalloc": [vp, I, vp, vp, D],
         "qexhip_solve_xx": [vp, I, vp, vp, D, spp],
         "qexhip_solve_ee_host": [vp, vp, vp, D, spp], "qexhip_solve_oo_host": [vp, vp, vp, D, spp],
         "qexhip_solve_multishift": [vp, I, vpp, vp, D, dp, I, spp],

@@ -20,7 +20,20 @@
 
 using namespace qh;
 
+// Device scratch of qexhip_stagD2xx_noalloc: the odd-parity temporary of the
+// two-hop operator, 128 MiB (an FP64 color vector of up to 2 796 032 sites per
+// parity: 48^4 fits).  Each GPU the library runs on holds its own copy until
+// the process ends.  Host code never passes the array's name to a kernel: the
+// name is a host-side stand-in, and qexhip_init looks up this device's address
+// of the array once (g_scratch below).
+#define QEXHIP_SCRATCH_BYTES (128ul << 20)
+static __device__ __align__(256) char scratch[QEXHIP_SCRATCH_BYTES];
+
 namespace {
+
+// device address of `scratch` per device, set by qexhip_init with that device current
+constexpr int kMaxDevices = 64;
+void *g_scratch[kMaxDevices] = {};
 
 thread_local std::string g_err;
 hipStream_t g_stream = nullptr;
@@ -305,11 +318,13 @@ Tuned tune(const std::string &key, const std::function<void(const Tuned &)> &f,
 
 Tuned tuned_hop(qexhip_links *l, int prec, int rc);
 
-void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const void *in, void *out, const void *a,
-         const void *b, double ca, double cb, double chop, int ca_m2, int pred, Scal *S) {
+// One hop with links of storage prec packed in lk and launch shape tu.
+void hop_shaped(qexhip_links *l, const char *lk, const Tuned &tu, int prec, int rc, int par, int mode, int redm,
+                const void *in, void *out, const void *a, const void *b, double ca, double cb, double chop,
+                int ca_m2, int pred, Scal *S) {
   HopArgs h;
   h.g = l->lat->g;
-  h.links = links_store(l, prec, rc);
+  h.links = lk;
   h.lkblk = lkblk_bytes(l, prec, rc);
   for (int k = 0; k < 8; k++) h.qs[k] = l->qs[k];
   h.in = in;
@@ -322,7 +337,6 @@ void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const v
   h.ca_m2 = ca_m2;
   h.par = par;
   h.pred = pred;
-  const Tuned tu = tuned_hop(l, prec, rc);
   h.remap = tu.remap;
   h.S = S;
   h.red = l->lat->red;
@@ -331,10 +345,28 @@ void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const v
   launch_hop(k, l->nl, mode, redm, h, g_stream, tu.block, tu.cap);
 }
 
-Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
+void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const void *in, void *out, const void *a,
+         const void *b, double ca, double cb, double chop, int ca_m2, int pred, Scal *S) {
+  const char *lk = links_store(l, prec, rc);
+  hop_shaped(l, lk, tuned_hop(l, prec, rc), prec, rc, par, mode, redm, in, out, a, b, ca, cb, chop, ca_m2, pred, S);
+}
+
+std::string hop_key(const qexhip_links *l, int prec, int rc) {
   char kb[64];
   snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d ", l->nl, prec, rc);
-  const std::string key = kb + geom_key(l->lat->g);
+  return kb + geom_key(l->lat->g);
+}
+
+// The hop's launch shape without tuning (tuning times candidates on buffers it
+// allocates): the tuner's earlier choice for this shape, else the default.
+Tuned cached_hop_shape(const qexhip_links *l, int prec, int rc) {
+  auto it = g_tune.find(hop_key(l, prec, rc));
+  if (it != g_tune.end()) return it->second;
+  return Tuned{g_block, g_remap, 0.0, 2, 0};
+}
+
+Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
+  const std::string key = hop_key(l, prec, rc);
   if (g_tune.count(key) || g_block_forced || !g_autotune) return tune(key, nullptr);
   const size_t vb = vec64_bytes(l->lat);
   char *in = dalloc<char>(vb), *out = dalloc<char>(vb);
@@ -1193,7 +1225,14 @@ int qexhip_init(int device) {
   int n = 0;
   HIPCHK(hipGetDeviceCount(&n));
   require(device >= 0 && device < n, "qexhip_init: no such device");
+  require(device < kMaxDevices, "qexhip_init: device index beyond the library's table");
   HIPCHK(hipSetDevice(device));
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, "qexhip_init: no device address for the scratch buffer");
+    g_scratch[device] = p;
+  }
   HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   g_device = device;
   if (const char *b = getenv("QEXHIP_BLOCK")) {
@@ -1579,6 +1618,43 @@ int qexhip_stagD2xx(qexhip_links *l, int par, qexhip_field *r, const qexhip_fiel
   // t = (2D)_qp x_p  (stagDP, rir := 0) ; r = 4 m2 x - (2D)_pq t  (stagDM)
   hop(l, pr, 18, 1 - par, 0, 0, x->par(par), t, nullptr, nullptr, 0, 0, 1.0, 0, 0, nullptr);
   hop(l, pr, 18, par, 1, 0, t, r->par(par), x->par(par), nullptr, 4.0 * m2, 0, -1.0, 0, 0, nullptr);
+  check_launch();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  API_END
+}
+
+int qexhip_stagD2xx_noalloc(qexhip_links *l, int par, qexhip_field *r, const qexhip_field *x, double m2) {
+  API_BEGIN
+  check_init();
+  check_same_lat(l, r);
+  check_same_lat(l, x);
+  require(r->prec == x->prec, "r and x must have the same precision");
+  require(r->prec == P64 || r->prec == P32, "stagD2xx_noalloc: fields must be FP64 or FP32");
+  require(par == 0 || par == 1, "stagD2xx parity must be 0 (ee) or 1 (oo)");
+  require(r != x, "stagD2xx_noalloc: r and x must be different fields");
+  const int pr = r->prec;
+  // no packing here: the FP64 copy exists from qexhip_links_create, the FP32
+  // one from the first FP32 operator call or solve on these links
+  const char *lk = l->st[pr][0];
+  require(lk != nullptr,
+          "stagD2xx_noalloc: the FP32 copy of these links is not packed yet (any FP32 operator call "
+          "or solve on them packs it)");
+  const size_t need = (size_t)3 * l->lat->g.stride * word_bytes(pr);
+  if (need > QEXHIP_SCRATCH_BYTES) {
+    char msg[200];
+    snprintf(msg, sizeof msg,
+             "stagD2xx_noalloc: the lattice needs %zu bytes of scratch, more than the %lu-byte static buffer "
+             "(at most %lu sites per parity in FP64)",
+             need, (unsigned long)QEXHIP_SCRATCH_BYTES, (unsigned long)(QEXHIP_SCRATCH_BYTES / 48 / 256 * 256));
+    throw std::runtime_error(msg);
+  }
+  void *t = g_scratch[g_device];
+  require(t != nullptr, "stagD2xx_noalloc: scratch buffer not set up");
+  const Tuned tu = cached_hop_shape(l, pr, 18);
+  // t = (2D)_qp x_p into the scratch ; r = 4 m2 x - (2D)_pq t.  The first hop
+  // writes every site the second reads, so the scratch needs no clearing.
+  hop_shaped(l, lk, tu, pr, 18, 1 - par, 0, 0, x->par(par), t, nullptr, nullptr, 0, 0, 1.0, 0, 0, nullptr);
+  hop_shaped(l, lk, tu, pr, 18, par, 1, 0, t, r->par(par), x->par(par), nullptr, 4.0 * m2, 0, -1.0, 0, 0, nullptr);
   check_launch();
   HIPCHK(hipStreamSynchronize(g_stream));
   API_END
