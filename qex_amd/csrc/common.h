@@ -391,35 +391,41 @@ template <class T, bool NT> __device__ __forceinline__ T ldv(const char *p) {
   return NT ? __builtin_nontemporal_load((const T *)p) : *(const T *)p;
 }
 
-// Load one link (LFmt layout, see above) straight into 3x3 complex form,
-// rebuilding row 2 for 12-real storage: row2_k = sgn*conj(row0 x row1)_k.
 // Where ld_link reads plane p (p == nfull: the remainder plane) of the link of
-// site i: a base pointer and a 32-bit offset from it.  Blocked storage moves
-// the base to the site's 64-site block in 64-bit arithmetic (an FP64 18-real
-// link block passes 4 GiB from 29.8M sites per parity on), so the offset
-// stays below one block (9 KiB); the plane layout keeps the block pointer and
-// a 32-bit offset, which qexhip_lattice_create bounds (36 B x stride for the
-// 16-bit formats).  The host test of the address arithmetic calls this too.
+// site i: a base pointer and a 32-bit offset from it.  While one link field's
+// block (stride x bytes per link) stays below 4 GiB the base is the block
+// pointer and the offset holds everything (scalar base + 32-bit vector offset
+// addressing).  A larger blocked field (FP64 18-real links from 29.8M sites
+// per parity on) is "wide": the base moves to the site's 64-site block in
+// 64-bit arithmetic and the offset stays below one block (9 KiB).  The plane
+// layout (16-bit formats) is never wide: qexhip_lattice_create bounds it
+// (36 B x stride).  The host test of the address arithmetic calls these too.
 template <int PL, int RC, bool BLK>
+__host__ __device__ __forceinline__ bool link_wide(long stride) {
+  return BLK && (unsigned long)stride * (unsigned long)LFmt<PL, RC>::bytes > 0xffffffffUL;
+}
+template <int PL, int RC, bool BLK, bool WIDE>
 __host__ __device__ __forceinline__ const char *link_base(const char *blk, long i) {
   using F = LFmt<PL, RC>;
-  return BLK ? blk + (size_t)(i >> QH_LBLK_LOG2) * (size_t)(LBLK * F::bytes) : blk;
+  return (BLK && WIDE) ? blk + (size_t)(i >> QH_LBLK_LOG2) * (size_t)(LBLK * F::bytes) : blk;
 }
-template <int PL, int RC, bool BLK>
+template <int PL, int RC, bool BLK, bool WIDE>
 __host__ __device__ __forceinline__ uint32_t link_off(long stride, long i, int p) {
   using F = LFmt<PL, RC>;
+  const uint32_t base = (BLK && !WIDE) ? (uint32_t)(i >> QH_LBLK_LOG2) * (uint32_t)(LBLK * F::bytes) : 0u;
   const uint32_t lane = (uint32_t)(BLK ? (i & (LBLK - 1)) : i);
   const uint32_t pstep = BLK ? (uint32_t)LBLK * 16u : (uint32_t)stride * 16u;
-  return p < F::nfull ? (uint32_t)p * pstep + lane * 16u : (uint32_t)F::nfull * pstep + lane * (uint32_t)F::rem;
+  return base + (p < F::nfull ? (uint32_t)p * pstep + lane * 16u : (uint32_t)F::nfull * pstep + lane * (uint32_t)F::rem);
 }
 
-template <int PL, int RC, bool NT, class A, bool BLK>
-__device__ __forceinline__ void ld_link(const char *blk0, long stride, long i, A (&U)[9], typename cx<A>::R sgn,
-                                        float qscale = 0.0f) {
+// The stored reals of one link (LFmt layout) into U: rows 0-1 for 12-real
+// storage, all of it for 18.
+template <int PL, int RC, bool NT, class A, bool BLK, bool WIDE>
+__device__ __forceinline__ void ld_link_planes(const char *blk0, long stride, long i, A (&U)[9], float qscale) {
   using F = LFmt<PL, RC>;
-  const char *blk = link_base<PL, RC, BLK>(blk0, i);
-  const uint32_t rbase = link_off<PL, RC, BLK>(stride, i, F::nfull);
-  auto poff = [&](int p) { return link_off<PL, RC, BLK>(stride, i, p); };
+  const char *blk = link_base<PL, RC, BLK, WIDE>(blk0, i);
+  const uint32_t rbase = link_off<PL, RC, BLK, WIDE>(stride, i, F::nfull);
+  auto poff = [&](int p) { return link_off<PL, RC, BLK, WIDE>(stride, i, p); };
   if constexpr (PL == P64) {
 #pragma unroll
     for (int c = 0; c < RC / 2; c++) {
@@ -464,6 +470,19 @@ __device__ __forceinline__ void ld_link(const char *blk0, long stride, long i, A
       }
     }
   }
+}
+
+// Load one link (LFmt layout, see above) straight into 3x3 complex form,
+// rebuilding row 2 for 12-real storage: row2_k = sgn*conj(row0 x row1).  The
+// wide/narrow choice depends only on stride, a kernel argument, so the branch
+// is uniform over the wave.
+template <int PL, int RC, bool NT, class A, bool BLK>
+__device__ __forceinline__ void ld_link(const char *blk0, long stride, long i, A (&U)[9], typename cx<A>::R sgn,
+                                        float qscale = 0.0f) {
+  if (link_wide<PL, RC, BLK>(stride))
+    ld_link_planes<PL, RC, NT, A, BLK, true>(blk0, stride, i, U, qscale);
+  else
+    ld_link_planes<PL, RC, NT, A, BLK, false>(blk0, stride, i, U, qscale);
   if constexpr (RC == 12) {
 #pragma unroll
     for (int k = 0; k < 3; k++) {
