@@ -188,6 +188,12 @@ int qexhip_autotune_report(char *buf, int len);
 /* CG iteration form: 2 = two stencil kernels forming p on the fly, 3 = a
    streaming update kernel plus two stencil kernels; 0 = chosen by the tuner */
 int qexhip_set_cg_form(int form);
+/* multi-RHS hop (qexhip_stagD2_multi) launch shape, no QEX counterpart: the
+   batches in which a thread loads its right-hand sides' vectors per link (1
+   or 2; 0 = chosen by the tuner) and the site walk (0: canonical order; a
+   multiple of 256: t-slices cut into chunks of that many sites, walked t
+   first; -1 = chosen by the tuner) */
+int qexhip_set_hop_multi_shape(int batches, long chunk);
 /* replay each chunk of CG iterations from a HIP graph (1, default; QEXHIP_CG_GRAPH=0
    at init turns it off) or launch kernel by kernel (0) */
 int qexhip_set_cg_graph(int on);

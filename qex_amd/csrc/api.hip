@@ -44,6 +44,8 @@ bool g_block_forced = false;  // QEXHIP_BLOCK set: no autotuning
 bool g_autotune = true;
 bool g_cg_graph = true;       // QEXHIP_CG_GRAPH=0: launch CG chunks kernel by kernel
 bool g_smear_walk = true;     // QEXHIP_SMEAR_WALK=0: HISQ smearing kernels walk sites in canonical order
+int g_multi_batches = 0;      // multi-RHS hop: vector-load batches per link pinned (1, 2) or 0 (tuner)
+long g_multi_chunk = -1;      // multi-RHS hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 
 #define HIPCHK(x)                                                                                  \
   do {                                                                                             \
@@ -246,6 +248,8 @@ struct Tuned {
   int form = 2;  // CG iteration: 2 or 3 kernels (dslash.hip)
   int cap = 0;   // hop: persistent form with at most cap workgroups (0: one pass)
   int block1 = 0;  // CG form 3: block size of the other-parity stencil kernel (0: block)
+  int batches = 1;  // multi-RHS hop: vector-load batches per link
+  long chunk = 0;   // site walk (make_walk): t-march chunk, 0 = canonical order
 };
 std::map<std::string, Tuned> g_tune;
 
@@ -1509,6 +1513,57 @@ int qexhip_stagD2(qexhip_links *l, int par, qexhip_field *r, const qexhip_field 
   API_END
 }
 
+// Launch shape of the multi-RHS hop for nr fields (block size, vector-load
+// batches, site-walk chunk), timed (tuner on) with the group's inputs and
+// scratch outputs of its own, one per field (a = b = 0), so the caller's
+// fields are untouched and the candidates write as many bytes as the real call.
+static Tuned tuned_hop_multi(qexhip_links *l, int prec, int nr, const MultiPtrs &P, const char *links, long lkblk,
+                             int par) {
+  std::vector<int> nbs;
+  for (int nb : {1, 2})
+    if ((nb == 1 || nr >= 2) && (!g_multi_batches || nb == g_multi_batches)) nbs.push_back(nb);
+  if (nbs.empty()) nbs.push_back(1);  // a pin the group cannot take
+  std::vector<long> chunks;
+  if (g_multi_chunk >= 0) chunks.push_back(g_multi_chunk);
+  else chunks = {0, 1024, 2048, 4096};
+  char kb[96];
+  snprintf(kb, sizeof kb, "hop_multi nl=%d prec=%d nr=%d pin=%d/%ld ", l->nl, prec, nr, g_multi_batches, g_multi_chunk);
+  const std::string key = kb + geom_key(l->lat->g);
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) return it->second;
+  Tuned best{256, 1, 1e300};
+  best.batches = nbs[0];
+  best.chunk = chunks[0];
+  if (g_block_forced || !g_autotune) {
+    best.block = g_block;
+    best.us = 0.0;
+    return best;
+  }
+  const size_t ob = (size_t)3 * l->lat->g.stride * word_bytes(prec);
+  std::vector<char *> outs(nr);
+  for (auto &o : outs) o = dalloc<char>(ob);
+  MultiPtrs Q = P;
+  for (int j = 0; j < nr; j++) Q.out[j] = outs[j];  // candidates only write these
+  for (int nb : nbs)
+    for (long ch : chunks)
+      for (int block : {64, 128, 256}) {
+        const double us = time_us(
+            [&] {
+              launch_hop_multi(prec, l->nl, nr, nb, l->lat->g, links, lkblk, par, 0.0, 0.0, Q, block, 0, ch, g_stream);
+            },
+            2, 8);
+        if (us < best.us) {
+          best = Tuned{block, 1, us};
+          best.batches = nb;
+          best.chunk = ch;
+        }
+      }
+  HIPCHK(hipStreamSynchronize(g_stream));
+  for (auto *o : outs) (void)hipFree(o);
+  g_tune[key] = best;
+  return best;
+}
+
 // r_j = a r_j + b x_j + (2D) x_j, j < n, links loaded once per group of up to 8
 static void dslash_multi(qexhip_links *l, int par, int n, qexhip_field *const *r, const qexhip_field *const *x,
                          double ca, double cb) {
@@ -1536,7 +1591,10 @@ static void dslash_multi(qexhip_links *l, int par, int n, qexhip_field *const *r
         P.own[j] = x[j0 + j]->par(p);
         P.out[j] = r[j0 + j]->par(p);
       }
-      launch_hop_multi(pr, l->nl, gsz, l->lat->g, links, lkblk, p, ca, cb, P, 256, 0, g_stream);
+      const Tuned tu = tuned_hop_multi(l, pr, gsz, P, links, lkblk, p);
+      require(launch_hop_multi(pr, l->nl, gsz, tu.batches, l->lat->g, links, lkblk, p, ca, cb, P, tu.block, 0,
+                               tu.chunk, g_stream),
+              "stagD2_multi: no kernel for this shape");
       j0 += gsz;
     }
     check_launch();
@@ -2238,6 +2296,15 @@ int qexhip_set_autotune(int on) {
   return 0;
 }
 
+int qexhip_set_hop_multi_shape(int batches, long chunk) {
+  API_BEGIN
+  require(batches >= 0 && batches <= 2, "vector batches must be 0 (tuner), 1 or 2");
+  require(chunk == -1 || (chunk >= 0 && chunk % 256 == 0), "site-walk chunk must be -1 (tuner), 0 or a multiple of 256");
+  g_multi_batches = batches;
+  g_multi_chunk = chunk;
+  API_END
+}
+
 int qexhip_set_cg_graph(int on) {
   g_cg_graph = on != 0;
   return 0;
@@ -2256,8 +2323,9 @@ int qexhip_autotune_report(char *buf, int len) {
   std::string out;
   for (auto &kv : g_tune) {
     char line[256];
-    snprintf(line, sizeof line, "%s: block=%d remap=%d form=%d cap=%d block1=%d us=%.2f\n", kv.first.c_str(),
-             kv.second.block, kv.second.remap, kv.second.form, kv.second.cap, kv.second.block1, kv.second.us);
+    snprintf(line, sizeof line, "%s: block=%d remap=%d form=%d cap=%d block1=%d batches=%d chunk=%ld us=%.2f\n",
+             kv.first.c_str(), kv.second.block, kv.second.remap, kv.second.form, kv.second.cap, kv.second.block1,
+             kv.second.batches, kv.second.chunk, kv.second.us);
     out += line;
   }
   snprintf(buf, (size_t)len, "%s", out.c_str());

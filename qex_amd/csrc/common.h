@@ -303,6 +303,45 @@ __device__ __forceinline__ BlockIter block_iter(long nblk, int remap) {
   return BlockIter{x * Q + lane, end, per};
 }
 
+// Site walk of a one-parity stencil launch.  chunk = 0: logical block lb
+// covers sites [lb*spb, (lb+1)*spb) in canonical order.  chunk = C > 0 (a
+// multiple of spb): the S = Vh/Lt sites of one t-slice are cut into chunks of
+// C consecutive sites, and the logical blocks walk t fastest, then the chunk:
+// block lb = (k*Lt + t)*B + w covers sites t*S + k*C + w*spb + [0, spb), B =
+// C/spb.  The t -/+ 1 neighbours of a chunk are then read again while the
+// previous and next t-slices of the same chunk are in flight on the same XCD
+// (with the XCD-contiguous block ranges of block_iter), instead of 2 S sites
+// of traffic later.  The last chunk of a slice may be partial.
+struct Walk {
+  long nblk;  // logical blocks
+  long S;     // sites per t-slice of one parity
+  long C;     // chunk (0: canonical order)
+  long B;     // logical blocks per chunk and t
+  int Lt;
+};
+__host__ __device__ __forceinline__ Walk make_walk(const Geom &g, long spb, long chunk) {
+  Walk w;
+  w.Lt = g.L[3];
+  w.S = g.Vh / g.L[3];
+  w.C = (chunk > 0 && chunk % spb == 0 && chunk < w.S) ? chunk : 0;
+  if (w.C) {
+    w.B = w.C / spb;
+    w.nblk = (w.S + w.C - 1) / w.C * w.Lt * w.B;
+  } else {
+    w.B = 0;
+    w.nblk = (g.Vh + spb - 1) / spb;
+  }
+  return w;
+}
+// site of local index l (0 <= l < spb) in logical block lb, or -1 past the lattice
+__host__ __device__ __forceinline__ long walk_site(const Walk &w, long lb, long spb, long l) {
+  if (!w.C) return lb * spb + l;  // the caller bounds it by Vh
+  const long wb = lb % w.B, r = lb / w.B;
+  const long t = r % w.Lt, k = r / w.Lt;
+  const long off = k * w.C + wb * spb + l;
+  return off < w.S ? t * w.S + off : -1;
+}
+
 // grid for nblk logical blocks: capped at cap workgroups when cap > 0,
 // rounded up to a multiple of 8 for the XCD-aware order
 inline unsigned grid_blocks(long nblk, int remap, int cap) {

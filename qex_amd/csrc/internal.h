@@ -177,8 +177,9 @@ void launch_force(const Geom &g, const char *links64, long lkblk, const double *
 // batch.hip
 void launch_bcg_iter(const BatchItem *T, int n, long nblocks, int nl, int block, hipStream_t st, int form);
 void launch_bcg_flush(const BatchItem *T, int n, long nblocks, int block, hipStream_t st);
-void launch_hop_multi(int prec, int nl, int nr, const Geom &g, const char *links, long lkblk, int par, double ca,
-                      double cb, const MultiPtrs &P, int block, int cap, hipStream_t st);
+// false when no instantiation serves (prec, nl, nr, batches); chunk: site walk (make_walk)
+bool launch_hop_multi(int prec, int nl, int nr, int batches, const Geom &g, const char *links, long lkblk, int par,
+                      double ca, double cb, const MultiPtrs &P, int block, int cap, long chunk, hipStream_t st);
 void launch_msb_iter(const Geom &g, const char *links64, long lkblk, int nl, int par, int nr, int nshift,
                      double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, hipStream_t st);
 

@@ -471,16 +471,25 @@ def set_cg_graph(on=True):
     call("qexhip_set_cg_graph", int(bool(on)))
 
 
+def set_hop_multi_shape(batches=0, chunk=-1):
+    """Pin the multi-RHS hop's vector-load batches per link (1 or 2; 0: tuner)
+    and its site walk (0: canonical order, a multiple of 256: t-first walk over
+    chunks of that many sites of a t-slice; -1: tuner)."""
+    call("qexhip_set_hop_multi_shape", int(batches), int(chunk))
+
+
 def set_cg_form(form=0):
     """Pin the CG iteration form (2 or 3 kernels per iteration); 0 lets the tuner choose."""
     call("qexhip_set_cg_form", int(form))
 
 
 def autotune_report():
-    """{kernel family and shape: {"block", "remap", "form", "cap", "block1", "us"}}
+    """{kernel family and shape: {"block", "remap", "form", "cap", "block1", "batches", "chunk", "us"}}
     chosen so far (form: 2- or 3-kernel CG iteration; cap: workgroup cap of the
     persistent hop, 0 = one pass; block1: block size of the 3-kernel iteration's
-    other-parity stencil kernel, 0 = block; us: time of one timed unit)."""
+    other-parity stencil kernel, 0 = block; batches: vector-load batches per link
+    of the multi-RHS hop; chunk: its t-first site-walk chunk, 0 = canonical
+    order; us: time of one timed unit)."""
     buf = ctypes.create_string_buffer(1 << 16)
     call("qexhip_autotune_report", buf, len(buf))
     out = {}
@@ -488,7 +497,8 @@ def autotune_report():
         key, _, rest = line.rpartition(": ")
         f = dict(kv.split("=") for kv in rest.split())
         out[key] = {"block": int(f["block"]), "remap": int(f["remap"]), "form": int(f["form"]),
-                    "cap": int(f["cap"]), "block1": int(f.get("block1", 0)), "us": float(f["us"])}
+                    "cap": int(f["cap"]), "block1": int(f.get("block1", 0)),
+                    "batches": int(f.get("batches", 1)), "chunk": int(f.get("chunk", 0)), "us": float(f["us"])}
     return out
 
 

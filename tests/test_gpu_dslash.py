@@ -148,14 +148,38 @@ def test_large_volume_properties(gpu, L):
     assert float(torch.linalg.norm(lhs - rhs) / torch.linalg.norm(rhs)) < 1e-14
 
 
+@pytest.mark.parametrize("shape", [(0, -1), (1, 0), (2, 0)])
 @pytest.mark.parametrize("n", [1, 3, 8, 11])
 @pytest.mark.parametrize("hisq", [False, True])
 @pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-6)])
-def test_stagD2_multi_rhs(oracle, gpu, n, hisq, prec, tol):
+def test_stagD2_multi_rhs(oracle, gpu, n, hisq, prec, tol, shape):
     """Multi-RHS stagD2 (each link loaded once per group of up to 8 fields)
-    equals the oracle's stagD2 applied field by field."""
+    equals the oracle's stagD2 applied field by field, with the vectors loaded
+    in 1 or 2 batches per link and with the tuner's choice."""
+    gpu.set_hop_multi_shape(*shape)
+    try:
+        _multi_rhs_case(oracle, gpu, n, hisq, prec, tol)
+    finally:
+        gpu.set_hop_multi_shape(0, -1)
+
+
+@pytest.mark.parametrize("L,chunk", [((12, 12, 12, 8), 256), ((12, 12, 12, 8), 512), ((16, 8, 8, 6), 256)])
+@pytest.mark.parametrize("hisq", [False, True])
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 2e-6)])
+def test_stagD2_multi_rhs_t_walk(oracle, gpu, L, chunk, hisq, prec, tol):
+    """The multi-RHS hop over the t-first site walk: t-slices of 864 sites per
+    parity (12^3) cut into chunks of 256 or 512 sites leave a partial last
+    chunk; 16x8x8 slices (512 sites) take two exact chunks of 256.  Every site
+    of both parities still equals the oracle's stagD2."""
+    gpu.set_hop_multi_shape(1, chunk)
+    try:
+        _multi_rhs_case(oracle, gpu, 8, hisq, prec, tol, L=L)
+    finally:
+        gpu.set_hop_multi_shape(0, -1)
+
+
+def _multi_rhs_case(oracle, gpu, n, hisq, prec, tol, L=(8, 4, 6, 8)):
     O = oracle
-    L = (8, 4, 6, 8)
     g = make_links(O, L, hisq)
     lo = gpu.Layout(L)
     s = gpu.Staggered(lo, g)
