@@ -125,15 +125,18 @@ void launch_bcg_flush(const BatchItem *T, int n, long nblocks, int block, hipStr
 // MODE 0: t_j = (2D) p0_j on parity q (x = site of parity q).
 // MODE 1: Ap_j = m4 p0_j - (2D) t_j on parity q with the partial sums of
 //         <p0_j, Ap_j>, <r_j, Ap_j>, |Ap_j|^2 (slots 3j, 3j+1, 3j+2).
-// Each link is loaded once and applied to the NR right-hand sides.
+// Each link is loaded once and applied to the NR right-hand sides.  The
+// vector loads of every right-hand side are issued before the link's
+// arithmetic (as in k_hop_multi): loads and arithmetic run for all NR, and a
+// right-hand side that has converged is left out only at the stores and sums
+// (its vectors stay allocated in the workspace, so the loads stay in bounds).
+// Skipping it inside the loop made every right-hand side's loads wait for the
+// previous one's.
 template <int NL, int NR, int MODE>
 __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, long lkblk, int q, double m4,
                                                  MsbPtrs P, const MsbScal *T, RedWs red) {
   __shared__ double lds[16 * 3 * NR];
   const long i = xcd_block(1) * blockDim.x + threadIdx.x;
-  bool act[NR];
-#pragma unroll
-  for (int r = 0; r < NR; r++) act[r] = T[r].continuing != 0;
   double2 acc[NR][3];
 #pragma unroll
   for (int r = 0; r < NR; r++)
@@ -149,26 +152,27 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
       const int len = (NL == 8 && (k & 1)) ? 3 : 1;
       const long n = nbr(g, s, mu, fwd ? len : -len);
       double2 U[9];
-      ld_link<P64, 18, true, double2, link_blocked<P64, NL>()>(links + (long)(2 * k + (fwd ? q : 1 - q)) * lkblk, g.stride, fwd ? i : n, U,
-                                      1.0);
+      ld_link<P64, 18, true, double2, link_blocked<P64, NL>()>(links + (long)(2 * k + (fwd ? q : 1 - q)) * lkblk,
+                                                              g.stride, fwd ? i : n, U, 1.0);
+      double2 v[NR][3];
 #pragma unroll
-      for (int r = 0; r < NR; r++) {
-        if (!act[r]) continue;
-        const double2 *src = (const double2 *)(MODE == 0 ? P.p0[r] : P.t[r]);
-        double2 v[3];
+      for (int r = 0; r < NR; r++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[c] = src[vix(g, c, n)];
-        if (fwd) {
+        for (int c = 0; c < 3; c++) v[r][c] = ldc<P64, double2>(MODE == 0 ? (const void *)P.p0[r] : P.t[r], vix(g, c, n));
+      if (fwd) {
 #pragma unroll
-          for (int a = 0; a < 3; a++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) cmac(acc[r][a], U[3 * a + c], v[c]);
-        } else {
+        for (int r = 0; r < NR; r++)
 #pragma unroll
           for (int a = 0; a < 3; a++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) cjmsub(acc[r][a], U[3 * c + a], v[c]);
-        }
+            for (int c = 0; c < 3; c++) cmac(acc[r][a], U[3 * a + c], v[r][c]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+#pragma unroll
+          for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) cjmsub(acc[r][a], U[3 * c + a], v[r][c]);
       }
     }
   }
@@ -176,7 +180,7 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
     if (in)
 #pragma unroll
       for (int r = 0; r < NR; r++) {
-        if (!act[r]) continue;
+        if (!T[r].continuing) continue;
         double2 *dst = (double2 *)P.t[r];
 #pragma unroll
         for (int c = 0; c < 3; c++) dst[vix(g, c, i)] = acc[r][c];
@@ -188,7 +192,7 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
     if (in)
 #pragma unroll
       for (int r = 0; r < NR; r++) {
-        if (!act[r]) continue;
+        if (!T[r].continuing) continue;
         const double2 *pp = (const double2 *)P.p0[r], *rr = (const double2 *)P.r[r];
         double2 *dst = (double2 *)P.Ap[r];
 #pragma unroll
