@@ -186,7 +186,9 @@ int qexhip_solve_multishift_batch(qexhip_links *l, int par, int nrhs, qexhip_fie
 int qexhip_set_autotune(int on);
 int qexhip_autotune_report(char *buf, int len);
 /* CG iteration form: 2 = two stencil kernels forming p on the fly, 3 = a
-   streaming update kernel plus two stencil kernels; 0 = chosen by the tuner */
+   streaming update kernel plus two stencil kernels, 4 = form 3 with the dot-
+   product finish folded into the next iteration's streaming kernel (single
+   systems; batches take 3); 0 = chosen by the tuner */
 int qexhip_set_cg_form(int form);
 /* multi-RHS hop (qexhip_stagD2_multi) launch shape, no QEX counterpart: the
    batches in which a thread loads its right-hand sides' vectors per link (1

@@ -117,6 +117,7 @@ struct Work {
   // sloppy parity vectors (sized for FP32 words, which also fit FP16/BF16)
   void *xs = nullptr, *rs = nullptr, *Aps = nullptr, *ps = nullptr, *ts = nullptr;
   Scal *S = nullptr, *Sh = nullptr;
+  Scal *S1 = nullptr;  // second scalar buffer of the form-4 CG iteration
   // multi-shift
   int nms = 0;
   std::vector<double *> msp;
@@ -207,6 +208,7 @@ Work *work(qexhip_links *l) {
       launch_fill(*p, b32, g_stream);
     }
     w->S = dalloc<Scal>(sizeof(Scal));
+    w->S1 = dalloc<Scal>(sizeof(Scal));
     HIPCHK(hipHostMalloc((void **)&w->Sh, sizeof(Scal), hipHostMallocDefault));
     memset(w->Sh, 0, sizeof(Scal));
     HIPCHK(hipMemcpyAsync(w->S, w->Sh, sizeof(Scal), hipMemcpyHostToDevice, g_stream));
@@ -218,7 +220,7 @@ Work *work(qexhip_links *l) {
 void free_work(Work *w) {
   if (!w) return;
   for (void *p : {(void *)w->y, (void *)w->rho, (void *)w->t64, (void *)w->r64, (void *)w->p64, (void *)w->Ap64,
-                  w->xs, w->rs, w->Aps, w->ps, w->ts, (void *)w->S, (void *)w->d_x, (void *)w->d_p,
+                  w->xs, w->rs, w->Aps, w->ps, w->ts, (void *)w->S, (void *)w->S1, (void *)w->d_x, (void *)w->d_p,
                   (void *)w->T})
     if (p) (void)hipFree(p);
   for (double *p : w->msp) (void)hipFree(p);
@@ -276,7 +278,7 @@ double time_us(const std::function<void()> &f, int warm, int reps) {
   return 1e3 * (double)t / reps;
 }
 
-int g_cg_form = 0;  // QEXHIP_CG_FORM=2/3 pins the CG iteration form; 0: tuned
+int g_cg_form = 0;  // QEXHIP_CG_FORM=2/3/4 pins the CG iteration form; 0: tuned
 
 // Capture the launches of f on the library stream into an instantiated graph.
 // A failure inside f ends the capture before rethrowing, so the stream is
@@ -399,7 +401,13 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
   return t;
 }
 
-// fused CG iteration: scratch vectors and device scalars that never converge
+// Fused CG iteration: the launch shape and iteration form, timed on chunks of
+// a few iterations with scratch vectors and device scalars that never
+// converge.  Forms 2 and 3 over block size x XCD order; form 4 also over the
+// workgroups of its persistent streaming kernel (cap); then, for forms 3 and
+// 4, the block size of the other-parity stencil kernel (block1).  The us
+// recorded is per iteration.
+const int kTuneIters = 4;
 Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, int rx, const CgArgs &real) {
   char kb[80];
   snprintf(kb, sizeof kb, "cg nl=%d links=%d rc=%d vec=%d rx=%d ", l->nl, pl, rc, ps, rx);
@@ -413,7 +421,7 @@ Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, int rx, const CgArgs &re
     launch_fill(v[k], vb, g_stream);
   }
   a.x = v[0]; a.r = v[1]; a.p = v[2]; a.Ap = v[3]; a.t = v[4];
-  Scal *S = dalloc<Scal>(sizeof(Scal));
+  Scal *S = dalloc<Scal>(2 * sizeof(Scal));
   Scal h;
   memset(&h, 0, sizeof h);
   h.r2stop = -1.0;
@@ -421,37 +429,46 @@ Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, int rx, const CgArgs &re
   h.maxits = INT_MAX;
   h.sigma = h.pscale = 1.0;
   HIPCHK(hipMemcpyAsync(S, &h, sizeof h, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(S + 1, &h, sizeof h, hipMemcpyHostToDevice, g_stream));
   a.S = S;
-  const Tuned t = tune(
-      key,
-      [&](const Tuned &c) {
-        a.remap = c.remap;
-        launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, c.block, c.form, rx);
-      },
-      {2, 3});
-  // form 3: the other-parity stencil kernel reduces nothing, so its block size
-  // is free; time the iteration with each while the rest keep the chosen shape
-  if (t.form == 3) {
-    Tuned best = t;
-    best.us = 1e300;
-    a.remap = t.remap;
+  auto run = [&](const Tuned &c) {
+    a.remap = c.remap;
+    launch_cg_chunk(pl, rc, ps, l->nl, a, g_stream, c.block, c.form, rx, c.block1, kTuneIters, S + 1, c.cap);
+  };
+  Tuned best{256, 1, 1e300};
+  for (int form : {2, 3, 4}) {
+    if (g_cg_form && form != g_cg_form) continue;
+    for (int cap : (form == 4 ? std::vector<int>{256, 512, 1024} : std::vector<int>{0}))
+      for (int block : {64, 128, 256})
+        for (int remap : {1, 0}) {
+          Tuned c{block, remap, 0.0, form, cap};
+          const double us = time_us([&] { run(c); }, 2, 8);
+          if (us < best.us) {
+            best = c;
+            best.us = us;
+          }
+        }
+  }
+  if (best.form >= 3) {
+    // the other-parity stencil kernel reduces nothing, so its block size is free
+    Tuned b = best;
     for (int b1 : {64, 128, 256}) {
-      const double us = time_us([&] { launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, t.block, 3, rx, b1); }, 2, 8);
-      if (us < best.us) {
-        best.us = us;
-        best.block1 = b1;
+      Tuned c = best;
+      c.block1 = b1;
+      const double us = time_us([&] { run(c); }, 2, 8);
+      if (us < b.us) {
+        b = c;
+        b.us = us;
       }
     }
-    g_tune[key] = best;
-    HIPCHK(hipStreamSynchronize(g_stream));
-    for (int k = 0; k < 5; k++) (void)hipFree(v[k]);
-    (void)hipFree(S);
-    return best;
+    best = b;
   }
+  best.us /= kTuneIters;
+  g_tune[key] = best;
   HIPCHK(hipStreamSynchronize(g_stream));
   for (int k = 0; k < 5; k++) (void)hipFree(v[k]);
   (void)hipFree(S);
-  return t;
+  return best;
 }
 
 // A = 4(m2 - D_pq D_qp) applied to x (FP64, own parity par): out = cx*x + cb*b - sgn...
@@ -546,7 +563,8 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   auto chunk = [&] {
-    for (int k = 0; k < kChunk; k++) launch_cg_iter(pl, rc, ps, l->nl, a, g_stream, tu.block, tu.form, rx, tu.block1);
+    require(launch_cg_chunk(pl, rc, ps, l->nl, a, g_stream, tu.block, tu.form, rx, tu.block1, kChunk, w->S1, tu.cap),
+            "no CG kernels for this storage combination");
   };
   if (g_cg_graph && !Sh->done) {
     capture(chunk, &graph, &exec);
@@ -890,7 +908,9 @@ void batch_chunk(BatchWs *w, const BatchItem *T) {
 }
 
 void batch_shape(BatchWs *w, bool autotune) {
-  int block = 256, form = g_cg_form ? g_cg_form : 2;
+  // the batch has forms 2 and 3; a pinned form 4 (single systems only) means 3
+  const int pinned = (g_cg_form == 4) ? 3 : g_cg_form;
+  int block = 256, form = pinned ? pinned : 2;
   if (autotune && !g_block_forced) {
     std::string key = "batch-cg nl=" + std::to_string(w->nl);
     for (auto *l : w->links) key += " " + geom_key(l->lat->g);
@@ -906,7 +926,7 @@ void batch_shape(BatchWs *w, bool autotune) {
       HIPCHK(hipMemcpyAsync(w->Sx, h.data(), sizeof(Scal) * w->n, hipMemcpyHostToDevice, g_stream));
       Tuned best{256, 0, 1e300};
       for (int f : {2, 3}) {
-        if (g_cg_form && f != g_cg_form) continue;
+        if (pinned && f != pinned) continue;
         for (int b : {64, 128, 256}) {
           const long nb = batch_table(w, b, w->Sx, w->Tx);
           const double us = time_us([&] { launch_bcg_iter(w->Tx, w->n, nb, w->nl, b, g_stream, f); }, 2, 8);
@@ -1250,7 +1270,7 @@ int qexhip_init(int device) {
   if (const char *c = getenv("QEXHIP_CG_GRAPH")) g_cg_graph = atoi(c) != 0;
   if (const char *f = getenv("QEXHIP_CG_FORM")) {
     const int v = atoi(f);
-    if (v == 2 || v == 3) g_cg_form = v;
+    if (v >= 2 && v <= 4) g_cg_form = v;
   }
   if (const char *r = getenv("QEXHIP_XCD_REMAP")) g_remap = atoi(r) != 0;
   if (const char *w = getenv("QEXHIP_SMEAR_WALK")) g_smear_walk = atoi(w) != 0;
@@ -2312,7 +2332,7 @@ int qexhip_set_cg_graph(int on) {
 
 int qexhip_set_cg_form(int form) {
   API_BEGIN
-  require(form == 0 || form == 2 || form == 3, "cg form must be 0 (tuned), 2 or 3");
+  require(form == 0 || (form >= 2 && form <= 4), "cg form must be 0 (tuned), 2, 3 or 4");
   g_cg_form = form;
   API_END
 }

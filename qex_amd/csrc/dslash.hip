@@ -148,6 +148,19 @@ __global__ void __launch_bounds__(256) k_cg0(CgArgs P) {
   cg0_body<PS, PR, PX, A>(P, blockIdx.x);
 }
 
+template <int PS, int PR, int PX, class A>
+__global__ void __launch_bounds__(QH_FIN_THREADS) k_cg0f(CgArgs P, const Scal *Sr, Scal *Sw, int nb) {
+  __shared__ double lds[16 * 4];
+  __shared__ Scal sc;
+  cg0f_body<PS, PR, PX, A>(P, Sr, Sw, nb, lds, &sc);
+}
+
+template <int PL, int RC, int PS, int PR, class A, int NL>
+__global__ void __launch_bounds__(256) k_cg2bp(CgArgs P) {
+  __shared__ double lds[16 * 4];
+  cg2b_persist_body<PL, RC, PS, PR, A, NL>(P, lds);
+}
+
 template <int PL, int RC, int PS, class A, int NL>
 __global__ void __launch_bounds__(256) k_cg1b(CgArgs P) {
   cg1b_body<PL, RC, PS, A, NL>(P, xcd_block(P.remap));
@@ -194,6 +207,47 @@ static void cg_iter_t(const CgArgs &a, hipStream_t st, int block, int form, int 
   launch_finish(EPI_CG2, f, st);
 }
 
+// Form 4 chunk of niter iterations: the first iteration as form 3 without
+// its finish; in each following one the streaming kernel k_cg0f finishes the
+// previous iteration's dot products itself, reading the scalars from one
+// buffer and writing the other (a.S, S1 alternately); a one-block finish
+// leaves the chunk's last result in a.S for the host.  The own-parity stencil
+// kernel runs persistent on cap2 workgroups (k_cg2bp), so it leaves cap2
+// partials for k_cg0f to re-add instead of one per block.
+template <int PL, int RC, int PS, int PRX, int NL>
+static void cg_chunk4_t(const CgArgs &a, Scal *S1, hipStream_t st, int block, int block1, int niter, int cap2) {
+  constexpr int PX = (PL == P64) ? P64 : PRX;
+  constexpr int PR = PX;
+  using A = typename std::conditional<PL == P64, double2, float2>::type;
+  const long nblk = (a.g.Vh + block - 1) / block;
+  dim3 grid((unsigned)nblk), blk(block);
+  // cap2 = 0 (no tuned shape): 512 workgroups, two per CU
+  dim3 grid2(grid_blocks(nblk, a.remap, cap2 > 0 ? cap2 : 512)), blk2(block);
+  const int b1 = block1 > 0 ? block1 : block;
+  dim3 grid1((unsigned)((a.g.Vh + b1 - 1) / b1)), blk1(b1);
+  dim3 grid0((unsigned)((a.g.Vh + QH_FIN_THREADS - 1) / QH_FIN_THREADS)), blk0(QH_FIN_THREADS);
+  Scal *S[2] = {a.S, S1};
+  const int nb = (int)grid2.x;  // partials of k_cg2bp
+  for (int k = 0; k < niter; k++) {
+    CgArgs ak = a;
+    ak.S = S[k & 1];
+    if (k == 0) hipLaunchKernelGGL((k_cg0<PS, PR, PX, A>), grid, blk, 0, st, ak);
+    else hipLaunchKernelGGL((k_cg0f<PS, PR, PX, A>), grid0, blk0, 0, st, a, S[(k - 1) & 1], S[k & 1], nb);
+    hipLaunchKernelGGL((k_cg1b<PL, RC, PS, A, NL>), grid1, blk1, 0, st, ak);
+    hipLaunchKernelGGL((k_cg2bp<PL, RC, PS, PR, A, NL>), grid2, blk2, 0, st, ak);
+  }
+  launch_finish_cg2_to(a.red, nb, S[(niter - 1) & 1], a.S, st);
+}
+
+template <int PL, int PS, int PRX = P32>
+static bool cg_chunk4_rc(int rc, int nl, const CgArgs &a, Scal *S1, hipStream_t st, int block, int b1, int niter,
+                         int g0) {
+  if (rc == 18 && nl == 4) { cg_chunk4_t<PL, 18, PS, PRX, 4>(a, S1, st, block, b1, niter, g0); return true; }
+  if (rc == 18 && nl == 8) { cg_chunk4_t<PL, 18, PS, PRX, 8>(a, S1, st, block, b1, niter, g0); return true; }
+  if (rc == 12 && nl == 4) { cg_chunk4_t<PL, 12, PS, PRX, 4>(a, S1, st, block, b1, niter, g0); return true; }
+  return false;
+}
+
 template <int PL, int PS, int PRX = P32>
 static bool cg_iter_rc(int rc, int nl, const CgArgs &a, hipStream_t st, int block, int form, int b1) {
   if (rc == 18 && nl == 4) { cg_iter_t<PL, 18, PS, PRX, 4>(a, st, block, form, b1); return true; }
@@ -221,6 +275,30 @@ void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t
   else if (pl == PQ16 && ps == P32) ok = cg_iter_rc<PQ16, P32>(rc, nl, a, st, block, form, block1);
   if (!ok)
     fprintf(stderr, "qexhip: no CG instantiation for pl=%d rc=%d ps=%d rx=%d nl=%d\n", pl, rc, ps, rx, nl);
+}
+
+// niter CG iterations of form 2, 3 (launch_cg_iter each) or 4 (cg_chunk4_t,
+// whose second scalar buffer is S1; g0: its streaming kernel's workgroups)
+bool launch_cg_chunk(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form, int rx,
+                     int block1, int niter, Scal *S1, int g0) {
+  if (form != 4) {
+    for (int k = 0; k < niter; k++) launch_cg_iter(pl, rc, ps, nl, a, st, block, form, rx, block1);
+    return true;
+  }
+  if (rx == P16 && ps == P16) {
+    if (pl == P16) return cg_chunk4_rc<P16, P16, P16>(rc, nl, a, S1, st, block, block1, niter, g0);
+    if (pl == PQ16) return cg_chunk4_rc<PQ16, P16, P16>(rc, nl, a, S1, st, block, block1, niter, g0);
+    return false;
+  }
+  if (pl == P64 && ps == P64) return cg_chunk4_rc<P64, P64>(rc, nl, a, S1, st, block, block1, niter, g0);
+  if (pl == P32 && ps == P32) return cg_chunk4_rc<P32, P32>(rc, nl, a, S1, st, block, block1, niter, g0);
+  if (pl == P16 && ps == P16) return cg_chunk4_rc<P16, P16>(rc, nl, a, S1, st, block, block1, niter, g0);
+  if (pl == P16 && ps == P32) return cg_chunk4_rc<P16, P32>(rc, nl, a, S1, st, block, block1, niter, g0);
+  if (pl == PBF && ps == PBF) return cg_chunk4_rc<PBF, PBF>(rc, nl, a, S1, st, block, block1, niter, g0);
+  if (pl == PBF && ps == P32) return cg_chunk4_rc<PBF, P32>(rc, nl, a, S1, st, block, block1, niter, g0);
+  if (pl == PQ16 && ps == P16) return cg_chunk4_rc<PQ16, P16>(rc, nl, a, S1, st, block, block1, niter, g0);
+  if (pl == PQ16 && ps == P32) return cg_chunk4_rc<PQ16, P32>(rc, nl, a, S1, st, block, block1, niter, g0);
+  return false;
 }
 
 void launch_cg_flush(int ps, int rx, const CgArgs &a, hipStream_t st) {

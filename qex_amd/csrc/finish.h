@@ -122,24 +122,23 @@ __device__ inline void epi_cgm(ShiftScal *T, Scal *S, double r2ip1) {
 #define QH_FIN_THREADS 256
 #endif
 
-// Sum the nb block partials of F.red and apply epilogue EPI (one block).
+// Block total of the nb block partials of red, in a fixed order that depends
+// only on nb and blockDim.x (every block running this with the same inputs
+// gets the same bits); the totals land in thread 0 (and the first wave).
 // lds: 16*NV doubles.
-template <int NV, int EPI>
-__device__ __forceinline__ void finish_body(const FinArgs &F, double *lds) {
-  if (F.pred == 1 && (F.S->done | F.S->need_rel)) return;
-  if (F.pred == 3 && !F.T->continuing) return;
-  double s[NV];
+template <int NV>
+__device__ __forceinline__ void sum_partials(const RedWs &red, int nb, double (&s)[NV], double *lds) {
 #pragma unroll
   for (int k = 0; k < NV; k++) s[k] = 0.0;
   // QH_FIN_UNROLL strides per trip: their loads are all issued before the adds
   const int bs = blockDim.x;
-  for (int j0 = threadIdx.x; j0 < F.nb; j0 += QH_FIN_UNROLL * bs) {
+  for (int j0 = threadIdx.x; j0 < nb; j0 += QH_FIN_UNROLL * bs) {
     double t[QH_FIN_UNROLL][NV];
 #pragma unroll
     for (int u = 0; u < QH_FIN_UNROLL; u++) {
       const int j = j0 + u * bs;
 #pragma unroll
-      for (int k = 0; k < NV; k++) t[u][k] = j < F.nb ? F.red.partials[(long)k * F.red.maxblocks + j] : 0.0;
+      for (int k = 0; k < NV; k++) t[u][k] = j < nb ? red.partials[(long)k * red.maxblocks + j] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < QH_FIN_UNROLL; u++)
@@ -147,6 +146,16 @@ __device__ __forceinline__ void finish_body(const FinArgs &F, double *lds) {
       for (int k = 0; k < NV; k++) s[k] += t[u][k];
   }
   block_sum<NV>(s, lds);
+}
+
+// Sum the nb block partials of F.red and apply epilogue EPI (one block).
+// lds: 16*NV doubles.
+template <int NV, int EPI>
+__device__ __forceinline__ void finish_body(const FinArgs &F, double *lds) {
+  if (F.pred == 1 && (F.S->done | F.S->need_rel)) return;
+  if (F.pred == 3 && !F.T->continuing) return;
+  double s[NV];
+  sum_partials<NV>(F.red, F.nb, s, lds);
   if (threadIdx.x != 0) return;
   if (EPI == EPI_TMP) F.S->tmp[F.slot] = s[0];
   else if (EPI == EPI_PAP) F.S->pAp = s[0];
@@ -154,6 +163,19 @@ __device__ __forceinline__ void finish_body(const FinArgs &F, double *lds) {
   else if (EPI == EPI_CG2) epi_cg2(F.S, s[0], s[1 % NV], s[2 % NV], s[3 % NV]);
   else if (EPI == EPI_FLUSH) epi_flush(F.S, s[0]);
   else if (EPI == EPI_CGM) epi_cgm(F.T, F.S, s[0]);
+}
+
+// The CG dot-product finish (EPI_CG2) on a copy of the scalars: Sout gets
+// Sin with the epilogue applied, or Sin unchanged once the solve is done or
+// waits for a reliable update (the double-buffered scalars of the fused CG
+// form, api.hip).  One block of QH_FIN_THREADS threads.
+__device__ __forceinline__ void finish_cg2_to(const RedWs &red, int nb, const Scal *Sin, Scal *Sout, double *lds) {
+  double s[4];
+  sum_partials<4>(red, nb, s, lds);
+  if (threadIdx.x != 0) return;
+  Scal sc = *Sin;
+  if (!(sc.done | sc.need_rel)) epi_cg2(&sc, s[0], s[1], s[2], s[3]);
+  *Sout = sc;
 }
 
 }  // namespace qh

@@ -141,11 +141,17 @@ struct HopKey {       // precision/format of one hop instantiation
 
 // reduce.hip
 void launch_finish(int epi, const FinArgs &f, hipStream_t st);
+// the CG finish into a second scalar buffer: Sout = Sin with EPI_CG2 applied
+void launch_finish_cg2_to(const RedWs &red, int nb, const Scal *Sin, Scal *Sout, hipStream_t st);
 
 // dslash.hip
 void launch_hop(const HopKey &k, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block, int cap);
 void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form,
                     int rx = P32, int block1 = 0);
+// niter iterations of CG form 2, 3 or 4 (4: scalars alternate between a.S and
+// S1, the streaming kernel runs g0 workgroups); false if no instantiation
+bool launch_cg_chunk(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form, int rx,
+                     int block1, int niter, Scal *S1, int g0);
 void launch_cg_flush(int ps, int rx, const CgArgs &a, hipStream_t st);
 void launch_pack_links(int prec, int rc, const Geom &g, int nl, const double *canon, char *out, const float *qs,
                        hipStream_t st);

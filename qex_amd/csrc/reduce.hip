@@ -12,6 +12,15 @@ __global__ void __launch_bounds__(QH_FIN_THREADS) k_finish(FinArgs F) {
   finish_body<NV, EPI>(F, lds);
 }
 
+__global__ void __launch_bounds__(QH_FIN_THREADS) k_finish_cg2_to(RedWs red, int nb, const Scal *Sin, Scal *Sout) {
+  __shared__ double lds[16 * 4];
+  finish_cg2_to(red, nb, Sin, Sout, lds);
+}
+
+void launch_finish_cg2_to(const RedWs &red, int nb, const Scal *Sin, Scal *Sout, hipStream_t st) {
+  hipLaunchKernelGGL(k_finish_cg2_to, dim3(1), dim3(QH_FIN_THREADS), 0, st, red, nb, Sin, Sout);
+}
+
 void launch_finish(int epi, const FinArgs &f, hipStream_t st) {
   const dim3 g(1), b(QH_FIN_THREADS);
   switch (epi) {

@@ -4,6 +4,7 @@
 // bodies of the fused two-kernel CG iteration.
 #pragma once
 #include "internal.h"
+#include "finish.h"
 
 namespace qh {
 
@@ -269,6 +270,58 @@ __device__ __forceinline__ void cg0_body(const CgArgs &P, long blk) {
   }
 }
 
+// Form 4: the streaming kernel of form 3 with the previous iteration's
+// dot-product finish folded in.  Every workgroup adds the nb block partials
+// of the last cg2b launch (sum_partials: the same order, hence the same bits,
+// as the one-block finish of form 3 when blockDim.x = QH_FIN_THREADS),
+// applies the CG epilogue to its own copy of the scalars read from Sr, and
+// uses that copy; workgroup 0 stores it to Sw, which the iteration's other
+// kernels read.  Sr and Sw alternate between two buffers from one iteration
+// to the next, so no workgroup reads scalars that another one is writing.
+// The workgroup's vector loads are issued before the partial sums, whose
+// latency they overlap; form 4 runs cg2b on a persistent grid so that nb
+// (its workgroups) stays small.  lds: 16*4 doubles.
+template <int PS, int PR, int PX, class A>
+__device__ __forceinline__ void cg0f_body(const CgArgs &P, const Scal *Sr, Scal *Sw, int nb, double *lds,
+                                          Scal *sc) {
+  using R = RT<A>;
+  const Geom &g = P.g;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < g.Vh;
+  A rv[3], av[3], pv[3], xv[3];
+  if (in)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const long off = vix(g, c, i);
+      rv[c] = ldc<PR, A>(P.r, off);
+      av[c] = ldc<PS, A>(P.Ap, off);
+      pv[c] = ldc<PS, A>(P.p, off);
+      xv[c] = ldc<PX, A>(P.x, off);
+    }
+  double tot[4];
+  sum_partials<4>(P.red, nb, tot, lds);
+  if (threadIdx.x == 0) {
+    Scal v = *Sr;
+    if (!(v.done | v.need_rel)) epi_cg2(&v, tot[0], tot[1], tot[2], tot[3]);
+    *sc = v;
+    if (blockIdx.x == 0) *Sw = v;
+  }
+  __syncthreads();
+  if ((sc->done | sc->need_rel) || !in) return;
+  const bool first = sc->first != 0;
+  const R alpha = (R)sc->alpha, beta = (R)(sc->beta * sc->pscale);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const long off = vix(g, c, i);
+    const A p0 = first ? mk<A>(0, 0) : pv[c];
+    const A rn = mk<A>(rv[c].x - alpha * av[c].x, rv[c].y - alpha * av[c].y);
+    const A pn = rt_store<PS, A>(first ? rn : mk<A>(rn.x + beta * p0.x, rn.y + beta * p0.y));
+    stc<PX, A>(P.x, off, mk<A>(xv[c].x + alpha * p0.x, xv[c].y + alpha * p0.y));
+    stc<PR, A>(P.r, off, rn);
+    stc<PS, A>(P.p, off, pn);
+  }
+}
+
 template <int PL, int RC, int PS, class A, int NL>
 __device__ __forceinline__ void cg1b_body(const CgArgs &P, long blk) {
   const Scal *S = P.S;
@@ -288,14 +341,13 @@ __device__ __forceinline__ void cg1b_body(const CgArgs &P, long blk) {
   for (int c = 0; c < 3; c++) stc_out<PS, A>(P.t, vix(g, c, i), res[c]);
 }
 
+// cg2b's work on logical block blk, its four dot products added to red
 template <int PL, int RC, int PS, int PR, class A, int NL>
-__device__ __forceinline__ void cg2b_body(const CgArgs &P, long blk, double *lds, long pidx) {
+__device__ __forceinline__ void cg2b_accum(const CgArgs &P, long blk, double (&red)[4]) {
   using R = RT<A>;
   Scal *S = P.S;
-  if (S->done | S->need_rel) return;
   const Geom &g = P.g;
   const long i = blk * blockDim.x + threadIdx.x;
-  double red[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < g.Vh) {
     const R m4 = (R)(4.0 * S->m2);
     A rv[3], pv[3];
@@ -322,7 +374,27 @@ __device__ __forceinline__ void cg2b_body(const CgArgs &P, long blk, double *lds
       red[3] += (double)rv[c].x * (double)rv[c].x + (double)rv[c].y * (double)rv[c].y;
     }
   }
+}
+
+template <int PL, int RC, int PS, int PR, class A, int NL>
+__device__ __forceinline__ void cg2b_body(const CgArgs &P, long blk, double *lds, long pidx) {
+  if (P.S->done | P.S->need_rel) return;
+  double red[4] = {0.0, 0.0, 0.0, 0.0};
+  cg2b_accum<PL, RC, PS, PR, A, NL>(P, blk, red);
   block_partials_at<4>(red, lds, P.red, pidx);
+}
+
+// cg2b on a persistent grid (form 4): each workgroup walks its logical blocks
+// (block_iter) and writes one set of partials, gridDim.x in all
+template <int PL, int RC, int PS, int PR, class A, int NL>
+__device__ __forceinline__ void cg2b_persist_body(const CgArgs &P, double *lds) {
+  if (P.S->done | P.S->need_rel) return;
+  double red[4] = {0.0, 0.0, 0.0, 0.0};
+  const long nblk = (P.g.Vh + blockDim.x - 1) / blockDim.x;
+  const BlockIter it = block_iter(nblk, P.remap);
+#pragma unroll 1
+  for (long lb = it.cur; lb < it.end; lb += it.step) cg2b_accum<PL, RC, PS, PR, A, NL>(P, lb, red);
+  block_partials_at<4>(red, lds, P.red, blockIdx.x);
 }
 
 // x += alpha p ; r -= alpha Ap ; |r|^2 (the pending update of the last iteration)

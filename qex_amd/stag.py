@@ -479,7 +479,9 @@ def set_hop_multi_shape(batches=0, chunk=-1):
 
 
 def set_cg_form(form=0):
-    """Pin the CG iteration form (2 or 3 kernels per iteration); 0 lets the tuner choose."""
+    """Pin the CG iteration form (2 or 3 kernels per iteration, or 4: form 3
+    with the dot-product finish folded into the next iteration's streaming
+    kernel); 0 lets the tuner choose."""
     call("qexhip_set_cg_form", int(form))
 
 

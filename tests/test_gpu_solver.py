@@ -343,3 +343,50 @@ def test_cg_graph_replay_matches_eager(oracle, gpu, sloppy, slinks, recon):
         out.append((fx.get(), sp.iterations, sp.r2))
     assert out[0][1] == out[1][1] and out[0][2] == out[1][2]
     assert np.array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("L", [(8, 8, 8, 8), (16, 16, 8, 8)])
+@pytest.mark.parametrize("sloppy,slinks,recon", [(0, None, 18), (2, 4, 12), (1, None, 18)])
+def test_cg_form4(oracle, gpu, L, sloppy, slinks, recon):
+    """Form 4 (the dot-product finish folded into the next iteration's
+    streaming kernel, every workgroup re-adding the persistent stencil
+    kernel's partials in one fixed order, scalars double-buffered): FP64
+    iteration counts equal the oracle's (+-1) and form 3's, every variant's
+    true residual meets the request by the oracle's operator, and the iterates
+    are bit-identical between graph replay and kernel-by-kernel launches and
+    from one run to the next."""
+    O = oracle
+    g = links(O, L, False)
+    b = O.source_gauss(L, 31, 0, 0)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    m = 0.03
+
+    def solve(form, graph=True):
+        fx = lo.ColorVector()
+        sp = gpu.SolverParams()
+        sp.r2req, sp.verbosity, sp.sloppySolve, sp.sloppyLinks, sp.recon = 1e-20, 0, sloppy, slinks, recon
+        gpu.set_cg_form(form)
+        gpu.set_cg_graph(graph)
+        try:
+            gpu.solveEE(s, fx, fb, m, sp)
+        finally:
+            gpu.set_cg_form(0)
+            gpu.set_cg_graph(True)
+        return fx.get(), sp.iterations, sp.r2, sp.reliableUpdates
+
+    f3 = solve(3)
+    runs = [solve(4, True), solve(4, False), solve(4, True)]
+    for r in runs[1:]:
+        assert r[1:] == runs[0][1:], (r[1:], runs[0][1:])
+        assert np.array_equal(r[0], runs[0][0])
+    x4, its4, r2_4, _ = runs[0]
+    assert r2_4 <= 1e-20
+    r = b - O.stagD2xx(L, g, 0, x4, m * m)
+    assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20
+    if sloppy == 0:
+        _, its_o, _ = O.solveXX(L, g, 0, b, m, 1e-20)
+        assert abs(its4 - its_o) <= 1 and abs(f3[1] - its_o) <= 1, (its4, f3[1], its_o)
+    else:
+        assert abs(its4 - f3[1]) <= max(2, f3[1] // 50), (its4, f3[1])
