@@ -62,6 +62,9 @@ def parse():
                     help="only run the HISQ link-construction study (32^3x64 smearing)")
     ap.add_argument("--hisq-hop", action="store_true",
                     help="only time the HISQ-form one-parity hop at 32^3x64 (--steps launches per precision)")
+    ap.add_argument("--hop-lattice", default="",
+                    help="only time the plain FP64 one-parity hop on this lattice (e.g. 48,48,48,48), "
+                         "--steps launches after --warmup")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -598,6 +601,20 @@ def main():
         return
     if args.gauge_io:
         print(json.dumps({"gauge_io": gauge_io_study()}), flush=True)
+        return
+    if args.hop_lattice:
+        L = tuple(int(v) for v in args.hop_lattice.split(","))
+        lo = Q.Layout(L)
+        U = gauge.random_gauge(lo, SEED)
+        gauge.phase_bc(lo, U)
+        s = Q.Staggered(lo, U)
+        del U
+        Q.time_hop(s, 0, max(1, args.warmup))
+        ms = Q.time_hop(s, 0, args.steps)
+        gbs = hop_gbs(0, ms, lo.nEven)
+        print(json.dumps({"hop_lattice": list(L), "steps": args.steps, "ms": round(ms, 5),
+                          "algorithmic_bytes_per_launch": HOP_REALS * 8 * lo.nEven, "GB_s": round(gbs, 1),
+                          "frac": round(gbs / PEAK_GBS, 4), "autotune": Q.autotune_report()}), flush=True)
         return
 
     lo = Q.Layout(LAT)

@@ -190,6 +190,11 @@ int qexhip_autotune_report(char *buf, int len);
    product finish folded into the next iteration's streaming kernel (single
    systems; batches take 3); 0 = chosen by the tuner */
 int qexhip_set_cg_form(int form);
+/* site walk of the one-parity hop of every operator and solver call, no QEX
+   counterpart: 0 = canonical order, a multiple of 256 = t-slices cut into
+   chunks of that many sites and walked t first (their t +- 1 neighbours are
+   read again while still in L2), -1 = chosen by the tuner */
+int qexhip_set_hop_walk(long chunk);
 /* multi-RHS hop (qexhip_stagD2_multi) launch shape, no QEX counterpart: the
    batches in which a thread loads its right-hand sides' vectors per link (1
    or 2; 0 = chosen by the tuner) and the site walk (0: canonical order; a

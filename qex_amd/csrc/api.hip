@@ -46,6 +46,7 @@ bool g_cg_graph = true;       // QEXHIP_CG_GRAPH=0: launch CG chunks kernel by k
 bool g_smear_walk = true;     // QEXHIP_SMEAR_WALK=0: HISQ smearing kernels walk sites in canonical order
 int g_multi_batches = 0;      // multi-RHS hop: vector-load batches per link pinned (1, 2) or 0 (tuner)
 long g_multi_chunk = -1;      // multi-RHS hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
+long g_hop_chunk = -1;        // one-parity hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 
 #define HIPCHK(x)                                                                                  \
   do {                                                                                             \
@@ -346,6 +347,7 @@ void hop_shaped(qexhip_links *l, const char *lk, const Tuned &tu, int prec, int 
   h.remap = tu.remap;
   h.S = S;
   h.red = l->lat->red;
+  h.chunk = tu.chunk;
   const int pv = (prec == PQ16) ? P16 : prec;  // 16-bit fixed-point links go with FP16 vectors
   HopKey k{prec, rc, pv, pv, pv};
   launch_hop(k, l->nl, mode, redm, h, g_stream, tu.block, tu.cap);
@@ -358,8 +360,11 @@ void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const v
 }
 
 std::string hop_key(const qexhip_links *l, int prec, int rc) {
-  char kb[64];
-  snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d ", l->nl, prec, rc);
+  char kb[96];
+  if (g_hop_chunk >= 0)
+    snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d walk=%ld ", l->nl, prec, rc, g_hop_chunk);
+  else
+    snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d ", l->nl, prec, rc);
   return kb + geom_key(l->lat->g);
 }
 
@@ -368,12 +373,15 @@ std::string hop_key(const qexhip_links *l, int prec, int rc) {
 Tuned cached_hop_shape(const qexhip_links *l, int prec, int rc) {
   auto it = g_tune.find(hop_key(l, prec, rc));
   if (it != g_tune.end()) return it->second;
-  return Tuned{g_block, g_remap, 0.0, 2, 0};
+  Tuned t{g_block, g_remap, 0.0, 2, 0};
+  t.chunk = g_hop_chunk > 0 ? g_hop_chunk : 0;
+  return t;
 }
 
 Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
   const std::string key = hop_key(l, prec, rc);
-  if (g_tune.count(key) || g_block_forced || !g_autotune) return tune(key, nullptr);
+  if (g_tune.count(key)) return g_tune[key];
+  if (g_block_forced || !g_autotune) return cached_hop_shape(l, prec, rc);
   const size_t vb = vec64_bytes(l->lat);
   char *in = dalloc<char>(vb), *out = dalloc<char>(vb);
   launch_fill(in, vb, g_stream);
@@ -388,17 +396,32 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
   h.red = l->lat->red;
   const int pv = (prec == PQ16) ? P16 : prec;
   const HopKey hk{prec, rc, pv, pv, pv};
-  const Tuned t = tune(
-      key,
-      [&](const Tuned &c) {
-        h.remap = c.remap;
-        launch_hop(hk, l->nl, 0, 0, h, g_stream, c.block, c.cap);
-      },
-      {2}, {0, 1024, 2048});
+  // block size x XCD order x workgroup cap (one pass or persistent) x site walk
+  Tuned best{256, 1, 1e300};
+  // (the walk trims the gathered vector's re-fetch but breaks the sequential
+  // stream of the links, which carry most of the bytes: at 48^4 FP64 it read
+  // 3.26 instead of 3.58 GB per hop in 582 instead of 545 us; a candidate still)
+  std::vector<long> chunks = {0L, 2048L, 8192L};
+  if (g_hop_chunk >= 0) chunks = {g_hop_chunk};
+  for (long chunk : chunks) {
+    if (chunk && g_hop_chunk < 0 && make_walk(h.g, 256, chunk).C == 0) continue;  // t-slices too small
+    for (int cap : {0, 1024, 2048})
+      for (int block : {64, 128, 256})
+        for (int remap : {1, 0}) {
+          h.remap = remap;
+          h.chunk = chunk;
+          const double us = time_us([&] { launch_hop(hk, l->nl, 0, 0, h, g_stream, block, cap); }, 2, 8);
+          if (us < best.us) {
+            best = Tuned{block, remap, us, 2, cap};
+            best.chunk = chunk;
+          }
+        }
+  }
+  g_tune[key] = best;
   HIPCHK(hipStreamSynchronize(g_stream));
   (void)hipFree(in);
   (void)hipFree(out);
-  return t;
+  return best;
 }
 
 // Fused CG iteration: the launch shape and iteration form, timed on chunks of
@@ -2314,6 +2337,13 @@ int qexhip_solve_multishift_batch(qexhip_links *l, int par, int nrhs, qexhip_fie
 int qexhip_set_autotune(int on) {
   g_autotune = on != 0;
   return 0;
+}
+
+int qexhip_set_hop_walk(long chunk) {
+  API_BEGIN
+  require(chunk == -1 || (chunk >= 0 && chunk % 256 == 0), "site-walk chunk must be -1 (tuner), 0 or a multiple of 256");
+  g_hop_chunk = chunk;
+  API_END
 }
 
 int qexhip_set_hop_multi_shape(int batches, long chunk) {

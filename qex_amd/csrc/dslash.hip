@@ -8,7 +8,9 @@
 //   H(x) = sum_k [ U_k(x) in(x + l_k mu_k) - U_k(x - l_k mu_k)^+ in(x - l_k mu_k) ]
 // One thread per output site; links are streamed with non-temporal 16-byte
 // loads (each link is read once per hop), the other-parity color vectors are
-// gathered through L2 with an XCD-aware block order.
+// gathered through L2 with an XCD-aware block order and, where the tuner
+// picks it, the t-first site walk (make_walk), which keeps a chunk's t +- 1
+// neighbours in L2 between their uses.
 #include <stdio.h>
 #include <type_traits>
 #include "stencil.h"
@@ -27,11 +29,12 @@ __global__ void __launch_bounds__(256, QH_HOP_WAVES) k_hop(HopArgs P) {
   if (P.pred && (P.S->done | P.S->need_rel)) return;
   const Geom &g = P.g;
   double red = 0.0;
-  const BlockIter it = block_iter((g.Vh + blockDim.x - 1) / blockDim.x, P.remap);
+  const Walk wk = make_walk(g, blockDim.x, P.chunk);
+  const BlockIter it = block_iter(wk.nblk, P.remap);
 #pragma unroll 1
   for (long lb = it.cur; lb < it.end; lb += it.step) {
-    const long i = lb * blockDim.x + threadIdx.x;
-    if (i >= g.Vh) continue;
+    const long i = walk_site(wk, lb, blockDim.x, threadIdx.x);
+    if (i < 0 || i >= g.Vh) continue;
     const Site s = decode(g, i, P.par);
     A res[3];
     auto ld = [&](long j, A(&v)[3]) {
@@ -80,7 +83,7 @@ __global__ void __launch_bounds__(256, QH_HOP_WAVES) k_hop(HopArgs P) {
 
 template <int PL, int RC, int PI, int PO, int PA, class A, int NL>
 static bool hop_modes(int mode, int redm, const HopArgs &a, hipStream_t st, int block, int cap) {
-  dim3 grid(grid_blocks((a.g.Vh + block - 1) / block, a.remap, cap));
+  dim3 grid(grid_blocks(make_walk(a.g, block, a.chunk).nblk, a.remap, cap));
 #define QH_HOP(M, R)                                                                                 \
   if (mode == M && redm == R) {                                                                      \
     hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R>), grid, dim3(block), 0, st, a);       \

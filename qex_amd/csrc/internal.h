@@ -69,6 +69,7 @@ struct HopArgs {
   int remap;          // XCD-aware block order
   Scal *S;
   RedWs red;
+  long chunk;         // site walk (make_walk): t-march chunk, 0 = canonical order
 };
 
 // Fused CG iteration (two kernels): see dslash.hip.

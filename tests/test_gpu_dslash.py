@@ -227,3 +227,40 @@ def test_time_hop_multi_runs(gpu, nrhs):
 def test_stream_read_probe(gpu):
     gbs = gpu.stream_read_gbs(256, 5)
     assert 500.0 < gbs < 8000.0
+
+
+@pytest.mark.parametrize("L,chunk", [((12, 12, 12, 8), 512), ((16, 16, 16, 8), 1024), ((8, 8, 8, 8), 0)])
+@pytest.mark.parametrize("hisq", [False, True])
+def test_hop_t_walk(oracle, gpu, L, chunk, hisq):
+    """The one-parity hop over the t-first site walk (12^3 t-slices: 864 sites
+    per parity, chunks of 512 leave a partial one; 16^3: two chunks of 1024;
+    chunk 0: canonical order) equals the oracle: stagD2 on each parity and
+    both, FP64 and FP32, and an FP64 and a mixed CG solve (whose reliable
+    updates and true residuals run reducing hops) to the oracle's residual."""
+    O = oracle
+    g = make_links(O, L, hisq)
+    gpu.set_hop_walk(chunk)
+    try:
+        lo = gpu.Layout(L)
+        s = gpu.Staggered(lo, g)
+        x = O.source_gauss(L, 41, 0, 2)
+        r0 = O.source_gauss(L, 42, 1, 2)
+        for prec, tol in ((gpu.FP64, 1e-12), (gpu.FP32, 1e-5)):
+            for par in (0, 1, 2):
+                fx = lo.ColorVector(prec).set(x)
+                fr = lo.ColorVector(prec).set(r0)
+                gpu.stagD2(sd_of(s, par), fr, s.g, fx, 0.5, -0.25)
+                want = r0.copy()
+                O.stagD2(L, g, par, want, x, 0.5, -0.25)
+                assert rel(fr.get(), want) < tol, (prec, par)
+        b = O.source_gauss(L, 43, 0, 0)
+        fb = lo.ColorVector().set(b)
+        for sloppy in (0, 1):
+            fs = lo.ColorVector()
+            sp = gpu.SolverParams()
+            sp.r2req, sp.verbosity, sp.sloppySolve = 1e-20, 0, sloppy
+            gpu.solveEE(s, fs, fb, 0.1, sp)
+            r = b - O.stagD2xx(L, g, 0, fs.get(), 0.01)
+            assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20, sloppy
+    finally:
+        gpu.set_hop_walk(-1)
