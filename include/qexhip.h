@@ -201,6 +201,9 @@ int qexhip_set_hop_walk(long chunk);
    multiple of 256: t-slices cut into chunks of that many sites, walked t
    first; -1 = chosen by the tuner) */
 int qexhip_set_hop_multi_shape(int batches, long chunk);
+/* block size of the other-parity stencil kernel of CG forms 3 and 4 (64,
+   128 or 256; 0 = chosen by the tuner), independent of the other kernels' */
+int qexhip_set_cg_block1(int block1);
 /* replay each chunk of CG iterations from a HIP graph (1, default; QEXHIP_CG_GRAPH=0
    at init turns it off) or launch kernel by kernel (0) */
 int qexhip_set_cg_graph(int on);

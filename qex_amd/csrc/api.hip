@@ -47,6 +47,7 @@ bool g_smear_walk = true;     // QEXHIP_SMEAR_WALK=0: HISQ smearing kernels walk
 int g_multi_batches = 0;      // multi-RHS hop: vector-load batches per link pinned (1, 2) or 0 (tuner)
 long g_multi_chunk = -1;      // multi-RHS hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 long g_hop_chunk = -1;        // one-parity hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
+int g_cg_block1 = 0;          // CG forms 3/4: other-parity stencil kernel's block size pinned, 0 (tuner)
 
 #define HIPCHK(x)                                                                                  \
   do {                                                                                             \
@@ -557,6 +558,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   a.red = lat->red;
   Tuned tu = tuned_cg(l, pl, rc, ps, rx, a);
   if (g_cg_form) tu.form = g_cg_form;
+  if (g_cg_block1) tu.block1 = g_cg_block1;
   a.remap = tu.remap;
   if (!mixed) {
     a.x = x; a.r = w->r64; a.p = w->p64; a.Ap = w->Ap64; a.t = w->t64;
@@ -2337,6 +2339,13 @@ int qexhip_solve_multishift_batch(qexhip_links *l, int par, int nrhs, qexhip_fie
 int qexhip_set_autotune(int on) {
   g_autotune = on != 0;
   return 0;
+}
+
+int qexhip_set_cg_block1(int block1) {
+  API_BEGIN
+  require(block1 == 0 || block1 == 64 || block1 == 128 || block1 == 256, "block1 must be 0 (tuner), 64, 128 or 256");
+  g_cg_block1 = block1;
+  API_END
 }
 
 int qexhip_set_hop_walk(long chunk) {

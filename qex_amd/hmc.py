@@ -393,6 +393,14 @@ def main(argv=None):
     ap.add_argument("--reversibility", action="store_true")
     ap.add_argument("--integrator", default="leapfrog", help="leapfrog or 2MN[,lambda]")
     ap.add_argument("--warmup", type=int, default=0, help="trajectories without Metropolis first")
+    ap.add_argument("--warmup-integrator", default="",
+                    help="integrator of the warmup trajectories (default: --integrator)")
+    ap.add_argument("--warmup-steps", type=int, default=0, help="MD steps per warmup trajectory (default: --steps)")
+    ap.add_argument("--warmup-sloppy", type=int, default=-1,
+                    help="SloppyType of the warmup solves (default: --sloppy); links per --sloppy-links/--recon")
+    ap.add_argument("--save", default="", help="write the configuration after the warmup (SciDAC, FP64)")
+    ap.add_argument("--load", default="", help="start from this gauge file (saveGauge / SciDAC / ILDG) instead "
+                                               "of the cold start")
     ap.add_argument("--hisq", action="store_true", help="HISQ fermions: smeared fat + Naik links")
     args = ap.parse_args(argv)
 
@@ -407,22 +415,66 @@ def main(argv=None):
                                      beta=args.beta, U=U)
         print(json.dumps({"check": "force_fd", "lat": args.lat, "mass": args.mass, "links": args.fd_links,
                           "fermion_worst_rel": wf, "gauge_worst_rel": wg, "seconds": round(sec, 2)}), flush=True)
-    name, *rest = args.integrator.split(",")
-    h = StaggeredHMC(lo, G.unit_gauge(lo), args.beta, args.mass, r2req_force=args.frsq,
+    def parse_integrator(spec):
+        name, *rest = spec.split(",")
+        return name, float(rest[0]) if rest else OMELYAN_LAMBDA
+
+    from . import fileio
+    U0 = G.unit_gauge(lo)
+    if args.load:
+        fileio.loadGauge(lo, U0, args.load)
+    name, lam = parse_integrator(args.integrator)
+    h = StaggeredHMC(lo, U0, args.beta, args.mass, r2req_force=args.frsq,
                      r2req_action=args.arsq, sloppy=Q.SloppyType(args.sloppy), integrator=name,
-                     lam=float(rest[0]) if rest else OMELYAN_LAMBDA,
-                     sloppy_links=None if args.sloppy_links < 0 else Q.SloppyType(args.sloppy_links),
+                     lam=lam, sloppy_links=None if args.sloppy_links < 0 else Q.SloppyType(args.sloppy_links),
                      recon=args.recon, hisq=args.hisq)
     rng = np.random.default_rng(args.seed)
-    for n in range(args.warmup + args.trajs):
+    # warmup: its own integrator, step count and solver storage if given
+    prod = (h.integrator, h.lam, h.sloppy)
+    if args.warmup_integrator:
+        h.integrator, h.lam = parse_integrator(args.warmup_integrator)
+    if args.warmup_sloppy >= 0:
+        h.sloppy = Q.SloppyType(args.warmup_sloppy)
+    wsteps = args.warmup_steps or args.steps
+    for n in range(args.warmup):
         its0 = h.solver_its
-        warm = n < args.warmup
-        r = h.trajectory(1000 * (n + 1) + args.seed, args.tau, args.steps, rng=rng, metropolis=not warm)
-        r.update({"traj": n + 1 - args.warmup, "warmup": warm, "solver_its": h.solver_its - its0})
+        r = h.trajectory(1000 * (n + 1) + args.seed, args.tau, wsteps, rng=rng, metropolis=False)
+        r.update({"traj": n + 1 - args.warmup, "warmup": True, "solver_its": h.solver_its - its0})
+        print(json.dumps(r), flush=True)
+    h.integrator, h.lam, h.sloppy = prod
+    if args.save:
+        fileio.saveGauge(lo, h.U, args.save, "D")
+        back = torch.empty_like(h.U)
+        fileio.loadGauge(lo, back, args.save)
+        print(json.dumps({"saved": args.save, "roundtrip_exact": bool(torch.equal(back, h.U)),
+                          "plaq": plaquette(lo, h.U) / (18.0 * lo.nSites)}), flush=True)
+        del back
+    rows = []
+    for n in range(args.warmup, args.warmup + args.trajs):
+        its0 = h.solver_its
+        r = h.trajectory(1000 * (n + 1) + args.seed, args.tau, args.steps, rng=rng, metropolis=True)
+        r.update({"traj": n + 1 - args.warmup, "warmup": False, "solver_its": h.solver_its - its0})
         if args.reversibility:
             dU, dP = h.reversibility(777 + n, args.tau, args.steps)
             r.update({"rev_dU": dU, "rev_dP": dP})
+        rows.append(r)
         print(json.dumps(r), flush=True)
+    if rows:
+        dh = np.array([r["dH"] for r in rows])
+        ex = np.exp(-dh)
+        summary = {"summary": True, "lat": args.lat, "beta": args.beta, "mass": args.mass, "tau": args.tau,
+                   "steps": args.steps, "integrator": args.integrator, "trajectories": len(rows),
+                   "acceptance": float(np.mean([r["accept"] for r in rows])),
+                   "dH_mean": float(dh.mean()), "dH_abs_max": float(np.abs(dh).max()),
+                   "exp_minus_dH_mean": float(ex.mean()),
+                   "exp_minus_dH_stderr": float(ex.std(ddof=1) / np.sqrt(len(ex))) if len(ex) > 1 else None,
+                   "seconds_per_trajectory": float(np.mean([r["seconds"] for r in rows])),
+                   "solver_its_per_trajectory": float(np.mean([r["solver_its"] for r in rows])),
+                   "plaq_last": rows[-1]["plaq"]}
+        if args.reversibility:
+            summary["rev_dU_max"] = max(r["rev_dU"] for r in rows)
+            summary["rev_dP_max"] = max(r["rev_dP"] for r in rows)
+        print(json.dumps(summary), flush=True)
 
 
 if __name__ == "__main__":

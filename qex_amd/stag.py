@@ -471,6 +471,12 @@ def set_cg_graph(on=True):
     call("qexhip_set_cg_graph", int(bool(on)))
 
 
+def set_cg_block1(block1=0):
+    """Pin the block size (64, 128, 256) of the CG iteration's other-parity
+    stencil kernel (forms 3 and 4); 0 lets the tuner choose."""
+    call("qexhip_set_cg_block1", int(block1))
+
+
 def set_hop_walk(chunk=-1):
     """Pin the one-parity hop's site walk: 0 canonical order, a multiple of 256
     the t-first walk over chunks of that many sites of a t-slice; -1 lets the

@@ -390,3 +390,36 @@ def test_cg_form4(oracle, gpu, L, sloppy, slinks, recon):
         assert abs(its4 - its_o) <= 1 and abs(f3[1] - its_o) <= 1, (its4, f3[1], its_o)
     else:
         assert abs(its4 - f3[1]) <= max(2, f3[1] // 50), (its4, f3[1])
+
+
+@pytest.mark.parametrize("form", [3, 4])
+@pytest.mark.parametrize("block1", [64, 256])
+def test_cg_block1_independent_of_block(oracle, gpu, form, block1):
+    """The other-parity stencil kernel of forms 3 and 4 pinned to a block size
+    of its own, 64 and 256 (so that in one of the two cases at least it
+    differs from the tuned block size of the other kernels): FP64 iteration
+    count equal to the oracle's (+-1), true residual by the oracle's operator,
+    the solution to 1e-9."""
+    O = oracle
+    L = (8, 8, 8, 16)
+    g = links(O, L, False)
+    b = O.source_gauss(L, 37, 0, 0)
+    want, its_o, _ = O.solveXX(L, g, 0, b, 0.05, 1e-20)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    fx = lo.ColorVector()
+    sp = gpu.SolverParams()
+    sp.r2req, sp.verbosity = 1e-20, 0
+    gpu.set_cg_form(form)
+    gpu.set_cg_block1(block1)
+    try:
+        gpu.solveEE(s, fx, fb, 0.05, sp)
+    finally:
+        gpu.set_cg_form(0)
+        gpu.set_cg_block1(0)
+    assert abs(sp.iterations - its_o) <= 1, (sp.iterations, its_o)
+    x = fx.get()
+    r = b - O.stagD2xx(L, g, 0, x, 0.05 ** 2)
+    assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20
+    assert rel(x[: lo.nEven], want[: lo.nEven]) < 1e-9
