@@ -1,16 +1,19 @@
 """Staggered HMC (plain or HISQ fermions) with a Wilson gauge action, on the GPU
 through the C ABI.
 
-Mirrors QEX's staggered HMC (src/examples/staghmc.nim:81-353, and the action
-pieces of src/hmc/hmcAction.nim:794-885):
+Mirrors QEX's staggered HMC (src/examples/staghmc.nim:81-353,
+src/examples/generichmc.nim:71-111, and the fermion actions of
+src/hmc/hmcAction.nim:794-1213):
 
-- one pseudofermion field on even sites, phi = (D^+(m) eta)_e with eta complex
-  Gaussian (heatbath, hmcAction.nim:794-818);
-- S_f = phi^+ (m^2 - D_eo D_oe)^{-1} phi;
+- pseudofermion fields on even sites, one per fermion action: the plain
+  StaggeredFermionAction (phi = (D^+(m) eta)_e, S_f = phi^+ (m^2 - D_eo D_oe)^{-1}
+  phi), the Pauli-Villars action and the Hasenbusch ratio action, eta complex
+  Gaussian;
 - S_g = -(beta/3) sum_P Re tr U_P (gauge/gaugeAction.nim:397-427);
-- momenta P traceless anti-Hermitian, H = 1/2 sum |P|^2 + S_g + S_f;
-- leapfrog (QEX's mdevolve leapfrog, src/algorithms/integrator.nim:32-34, with
-  the update order of staghmc.nim mdt/mdv/mdvf).
+- momenta P traceless anti-Hermitian, H = 1/2 sum |P|^2 + S_g + sum S_f;
+- leapfrog or Omelyan 2MN (QEX's mdevolve integrators,
+  src/algorithms/integrator.nim:32-34, with the update order of staghmc.nim
+  mdt/mdv/mdvf), on one level or with the gauge force on a nested inner level.
 
 Force convention (shared by qexhip_force and qexhip_wilson_force): for
 U -> exp(w X) U, dS/dw = -1/2 Re tr(X F), so Pdot = -F/2 conserves H.
@@ -70,16 +73,152 @@ def kinetic(P):
     return 0.5 * float(torch.sum(P.real ** 2 + P.imag ** 2))
 
 
+def _gaussian(lo, seed, k):
+    """Complex Gaussian eta on all sites, E|eta|^2 = 1 per component (density
+    exp(-eta^+ eta)); action k of a trajectory draws from its own stream."""
+    return lo.ColorVector().set(G.source(lo, seed, stream=7 if k == 0 else 100 + k, par=2))
+
+
+class StaggeredFermionAction:
+    """QEX's StaggeredFermionAction (hmcAction.nim:794-885): one pseudofermion on
+    even sites,
+        heatbath  phi_e = (D^+(m) eta)_e,
+        action    S = phi^+ M(m)^-1 phi,  M(m) = m^2 - D_eo D_oe,
+        force     fermForce of stagForceSolve(phi, m), weight +1.
+    QEX writes the heatbath as D(-m) eta and the action as 1/2 |D(-m)^-1 phi|^2
+    for unit-variance real Gaussians; with E|eta|^2 = 1 per complex component
+    the same distribution reads as above."""
+
+    name = "StaggeredFermionAction"
+
+    def __init__(self, mass):
+        self.mass = float(mass)
+
+    def describe(self):
+        return f"{self.name}(mass: {self.mass})"
+
+    def bind(self, hmc):
+        self.h = hmc
+        self.phi = hmc.lo.ColorVector()
+
+    def heatbath(self, eta):
+        Q.Ddag(self.h.s, self.phi, eta, self.mass)
+        self.phi.zero("odd")
+
+    def action(self):
+        h = self.h
+        sp = h._sp(h.r2a)
+        Q.solveEE(h.s, h.y, self.phi, self.mass, sp)
+        h.solver_its += sp.iterations
+        return 4.0 * self.phi.redot(h.y, "even")
+
+    def force(self, out):
+        """Writes F of psi = stagForceSolve(phi, m) into out; returns its weight."""
+        h = self.h
+        sp = h._sp(h.r2f)
+        Q.stagForceSolve(h.s, h.psi, self.phi, self.mass, sp)
+        h.solver_its += sp.iterations
+        h._force_of(h.psi, out)
+        return 1.0
+
+
+class StaggeredPauliVillarsAction(StaggeredFermionAction):
+    """QEX's StaggeredPauliVillarsAction (hmcAction.nim:947-1033), a bosonic
+    field that cancels one quartet's determinant at the heavy mass m:
+        heatbath  phi_e = (D(m)^-1 eta)_e  (a full-lattice solve, :962-964),
+        action    S = |D(m) phi_e|^2 = phi^+ M(m) phi  (:983-1002),
+        force     fermForce of psi = [phi_e ; (2D)_oe phi_e], weight -1
+                  (QEX's sc = -0.25 dtau, :1004-1033)."""
+
+    name = "StaggeredPauliVillarsAction"
+
+    def heatbath(self, eta):
+        h = self.h
+        sp = h._sp(h.r2a)
+        Q.solve(h.s, self.phi, eta, self.mass, sp)
+        h.solver_its += sp.iterations
+        self.phi.zero("odd")
+
+    def action(self):
+        h = self.h
+        Q.D(h.s, h.y, self.phi, self.mass)
+        return h.y.norm2()
+
+    def force(self, out):
+        h = self.h
+        Q.stagD2(h.s.so, h.psi, h.s.g, self.phi, 0.0, 0.0)
+        h.psi.copy_from(self.phi, "even")
+        h._force_of(h.psi, out)
+        return -1.0
+
+
+class StaggeredRatioAction:
+    """QEX's StaggeredRatioAction (hmcAction.nim:1095-1152), the Hasenbusch
+    ratio det M(massNum) / det M(massDen):
+        heatbath  phi_e = (D(-massDen)^-1 D(-massNum) eta)_e  (:1104-1108),
+        action    S = |D(-massNum)^-1 D(-massDen) phi_e|^2  (:1113-1131)
+                    = |phi|^2 + (massDen^2 - massNum^2) phi^+ M(massNum)^-1 phi,
+        force     fermForce of stagForceSolve(phi, massNum),
+                  weight massDen^2 - massNum^2  (:1133-1152).
+    Both solves are full-lattice solves at negated masses, as in QEX."""
+
+    name = "StaggeredRatioAction"
+
+    def __init__(self, massNum, massDen):
+        self.massNum, self.massDen = float(massNum), float(massDen)
+
+    def describe(self):
+        return f"{self.name}(massNum: {self.massNum}, massDen: {self.massDen})"
+
+    def bind(self, hmc):
+        self.h = hmc
+        self.phi = hmc.lo.ColorVector()
+
+    def heatbath(self, eta):
+        h = self.h
+        Q.D(h.s, h.y, eta, -self.massNum)
+        sp = h._sp(h.r2a)
+        Q.solve(h.s, self.phi, h.y, -self.massDen, sp)
+        h.solver_its += sp.iterations
+        self.phi.zero("odd")
+
+    def action(self):
+        h = self.h
+        Q.D(h.s, h.y, self.phi, -self.massDen)
+        sp = h._sp(h.r2a)
+        Q.solve(h.s, h.psi, h.y, -self.massNum, sp)
+        h.solver_its += sp.iterations
+        return h.psi.norm2()
+
+    def force(self, out):
+        h = self.h
+        sp = h._sp(h.r2f)
+        Q.stagForceSolve(h.s, h.psi, self.phi, self.massNum, sp)
+        h.solver_its += sp.iterations
+        h._force_of(h.psi, out)
+        return self.massDen * self.massDen - self.massNum * self.massNum
+
+
 class StaggeredHMC:
     """Staggered HMC; hisq=True smears the phased links into HISQ fat and long
     links (physics/hisqLinks.nim:39-51) for the fermion operator and chains the
-    fermion force through the smearing (gauge/hisqsmear.nim:54-91)."""
+    fermion force through the smearing (gauge/hisqsmear.nim:54-91).
+
+    actions lists the fermion actions (default: one StaggeredFermionAction at
+    mass), all on the one operator of these links.  gauge_steps > 0 puts the
+    gauge force on an inner level of its own, gauge_steps integrator steps per
+    outer position step, as QEX's nested action levels do
+    (hmc/hmcAction.nim ActionLevel, examples/generichmc.nim:93-111)."""
 
     def __init__(self, lo, U, beta, mass, r2req_force=1e-20, r2req_action=1e-24, sloppy=Q.SloppyType.SloppyNone,
-                 integrator="leapfrog", lam=OMELYAN_LAMBDA, sloppy_links=None, recon=18, hisq=False):
-        if integrator not in ("leapfrog", "2MN"):
-            raise ValueError("integrator must be 'leapfrog' or '2MN'")
+                 integrator="leapfrog", lam=OMELYAN_LAMBDA, sloppy_links=None, recon=18, hisq=False,
+                 actions=None, gauge_steps=0, gauge_integrator=None, gauge_lam=OMELYAN_LAMBDA):
+        for name in (integrator, gauge_integrator or integrator):
+            if name not in ("leapfrog", "2MN"):
+                raise ValueError("integrator must be 'leapfrog' or '2MN'")
         self.integrator, self.lam = integrator, float(lam)
+        self.gauge_steps = int(gauge_steps)
+        self.gauge_integrator, self.gauge_lam = gauge_integrator or integrator, float(gauge_lam)
         self.lo = lo
         self.U = U.contiguous()
         self.beta = float(beta)
@@ -88,7 +227,7 @@ class StaggeredHMC:
         G.phase_bc(lo, self.Uph)
         self.hisq = H.newHISQ() if hisq else None
         self.s = Q.Staggered(lo, self._fermion_links())
-        self.phi = lo.ColorVector()
+        self._stale = False
         self.psi = lo.ColorVector()
         self.y = lo.ColorVector()
         self.r2f, self.r2a = r2req_force, r2req_action
@@ -96,6 +235,15 @@ class StaggeredHMC:
         self.solver_its = 0
         self.Ff = torch.empty_like(self.U)
         self.Fg = torch.empty_like(self.U)
+        self._Ft = None
+        self.actions = list(actions) if actions is not None else [StaggeredFermionAction(mass)]
+        if not self.actions:
+            raise ValueError("at least one fermion action is needed")
+        for a in self.actions:
+            a.bind(self)
+
+    def describe(self):
+        return [a.describe() for a in self.actions]
 
     def _sp(self, r2req):
         sp = Q.SolverParams()
@@ -110,72 +258,109 @@ class StaggeredHMC:
         return torch.stack([t for mu in range(4) for t in (fl[mu], ll[mu])]).contiguous()
 
     def _links_changed(self):
-        self.Uph.copy_(self.U)
-        G.phase_bc(self.lo, self.Uph)
-        self.s.update(self._fermion_links())
+        """U has changed: the fermion operator (and its smearing) is rebuilt
+        before the next fermion action, force or heatbath needs it."""
+        self._stale = True
+
+    def _fermion_ready(self):
+        if self._stale:
+            self.Uph.copy_(self.U)
+            G.phase_bc(self.lo, self.Uph)
+            self.s.update(self._fermion_links())
+            self._stale = False
+
+    def _force_of(self, psi, out):
+        if self.hisq is None:
+            Q.fermForce(self.s, psi, out=out)
+        else:
+            # force with respect to the phased links; the staggered signs drop out of TAH(2 dS/dU U^+)
+            H.hisqForce(self.s, self.hisq, self.Uph, psi, F=out)
 
     def heatbath(self, seed):
-        """phi_e = (D^+(m) eta)_e (hmcAction.nim:794-818)."""
-        eta = self.lo.ColorVector().set(G.source(self.lo, seed, stream=7, par=2))
-        Q.Ddag(self.s, self.phi, eta, self.mass)
-        self.phi.zero("odd")
+        """A fresh pseudofermion for every action (hmcAction.nim:794-818, :947-981, :1095-1111)."""
+        self._fermion_ready()
+        for k, a in enumerate(self.actions):
+            a.heatbath(_gaussian(self.lo, seed, k))
 
     def fermion_action(self):
-        sp = self._sp(self.r2a)
-        Q.solveEE(self.s, self.y, self.phi, self.mass, sp)
-        self.solver_its += sp.iterations
-        return 4.0 * self.phi.redot(self.y, "even")
+        self._fermion_ready()
+        return sum(a.action() for a in self.actions)
 
     def gauge_action(self):
         return -(self.beta / 3.0) * plaquette(self.lo, self.U)
 
+    def fermion_force(self):
+        """F of all fermion actions, each scaled by its weight (QEX's addForce
+        with sc = weight * dtau / 4 per action)."""
+        self._fermion_ready()
+        for k, a in enumerate(self.actions):
+            if k == 0:
+                w = a.force(self.Ff)
+                if w != 1.0:
+                    self.Ff.mul_(w)
+            else:
+                if self._Ft is None:
+                    self._Ft = torch.empty_like(self.U)
+                w = a.force(self._Ft)
+                self.Ff.add_(self._Ft, alpha=w)
+        return self.Ff
+
+    def gauge_force(self):
+        return wilson_force(self.lo, self.U, self.beta, out=self.Fg)
+
     def pdot(self):
         """-1/2 (F_gauge + F_fermion)."""
-        sp = self._sp(self.r2f)
-        Q.stagForceSolve(self.s, self.psi, self.phi, self.mass, sp)
-        self.solver_its += sp.iterations
-        if self.hisq is None:
-            Q.fermForce(self.s, self.psi, out=self.Ff)
-        else:
-            # force with respect to the phased links; the staggered signs drop out of TAH(2 dS/dU U^+)
-            H.hisqForce(self.s, self.hisq, self.Uph, self.psi, F=self.Ff)
-        wilson_force(self.lo, self.U, self.beta, out=self.Fg)
-        return -0.5 * (self.Fg + self.Ff)
+        Ff = self.fermion_force()
+        return -0.5 * (self.gauge_force() + Ff)
 
     def hamiltonian(self, P):
         return kinetic(P) + self.gauge_action() + self.fermion_action()
 
-    def leapfrog(self, P, tau, nsteps):
-        dt = tau / nsteps
-        P = P + (0.5 * dt) * self.pdot()
-        for i in range(nsteps):
-            link_update(self.lo, P, dt, self.U)
+    def _levels(self, nsteps):
+        if self.gauge_steps <= 0:
+            return [(self.pdot, nsteps, self.integrator, self.lam)]
+        return [(lambda: -0.5 * self.fermion_force(), nsteps, self.integrator, self.lam),
+                (lambda: -0.5 * self.gauge_force(), self.gauge_steps, self.gauge_integrator, self.gauge_lam)]
+
+    def _evolve(self, P, tau, levels):
+        """Integrate the first level's force over tau; its position steps are
+        the evolution of the remaining levels, the innermost one the link
+        update U <- exp(dt P) U."""
+        if not levels:
+            link_update(self.lo, P, tau, self.U)
             self._links_changed()
-            P = P + (dt if i < nsteps - 1 else 0.5 * dt) * self.pdot()
+            return P
+        pdot, nsteps, name, lam = levels[0]
+        inner = levels[1:]
+        dt = tau / nsteps
+        if name == "leapfrog":
+            P = P + (0.5 * dt) * pdot()
+            for i in range(nsteps):
+                P = self._evolve(P, dt, inner)
+                P = P + (dt if i < nsteps - 1 else 0.5 * dt) * pdot()
+            return P
+        # Omelyan's second-order minimum-norm integrator, steps of
+        # V(lam dt) T(dt/2) V((1-2 lam) dt) T(dt/2) V(lam dt) with adjacent V
+        # updates merged (mdevolve's 2MN, selected by the name "2MN[,lam]" in
+        # algorithms/integrator.nim:32-34); lam = 0 is the position leapfrog
+        if lam != 0.0:
+            P = P + (lam * dt) * pdot()
+        for i in range(nsteps):
+            P = self._evolve(P, 0.5 * dt, inner)
+            P = P + ((1.0 - 2.0 * lam) * dt) * pdot()
+            P = self._evolve(P, 0.5 * dt, inner)
+            if lam != 0.0:
+                P = P + ((2.0 * lam if i < nsteps - 1 else lam) * dt) * pdot()
         return P
+
+    def leapfrog(self, P, tau, nsteps):
+        return self._evolve(P, tau, [(self.pdot, nsteps, "leapfrog", 0.0)])
 
     def omelyan2mn(self, P, tau, nsteps, lam=OMELYAN_LAMBDA):
-        """Omelyan's second-order minimum-norm integrator, steps of
-        V(lam dt) T(dt/2) V((1-2 lam) dt) T(dt/2) V(lam dt) with adjacent V
-        updates merged (mdevolve's 2MN, selected by the name "2MN[,lam]" in
-        algorithms/integrator.nim:32-34); lam = 0 is the position leapfrog."""
-        dt = tau / nsteps
-        if lam != 0.0:
-            P = P + (lam * dt) * self.pdot()
-        for i in range(nsteps):
-            link_update(self.lo, P, 0.5 * dt, self.U)
-            self._links_changed()
-            P = P + ((1.0 - 2.0 * lam) * dt) * self.pdot()
-            link_update(self.lo, P, 0.5 * dt, self.U)
-            self._links_changed()
-            if lam != 0.0:
-                P = P + ((2.0 * lam if i < nsteps - 1 else lam) * dt) * self.pdot()
-        return P
+        return self._evolve(P, tau, [(self.pdot, nsteps, "2MN", lam)])
 
     def integrate(self, P, tau, nsteps):
-        if self.integrator == "leapfrog":
-            return self.leapfrog(P, tau, nsteps)
-        return self.omelyan2mn(P, tau, nsteps, self.lam)
+        return self._evolve(P, tau, self._levels(nsteps))
 
     def trajectory(self, seed, tau=1.0, nsteps=20, rng=None, metropolis=True):
         """One HMC trajectory, with a Metropolis test unless metropolis is False
@@ -369,8 +554,9 @@ def force_fd_check(lo, Uph, phi, mass, nlinks=64, seed=1, h=1e-5, r2req=1e-28, b
 
 
 def main(argv=None):
-    """Plain-staggered HMC run in the manner of staghmc.nim (cold start, Metropolis),
-    one JSON line per trajectory; --fd-links N first runs force_fd_check."""
+    """Staggered HMC run in the manner of staghmc.nim and generichmc.nim (cold
+    start or a loaded configuration, Metropolis), one JSON line per trajectory;
+    --fd-links N first runs force_fd_check."""
     import argparse
     import json
 
@@ -402,6 +588,14 @@ def main(argv=None):
     ap.add_argument("--load", default="", help="start from this gauge file (saveGauge / SciDAC / ILDG) instead "
                                                "of the cold start")
     ap.add_argument("--hisq", action="store_true", help="HISQ fermions: smeared fat + Naik links")
+    ap.add_argument("--hasenbusch-mass", type=float, default=0.0,
+                    help="split the quartet as StaggeredFermionAction(this mass) x "
+                         "StaggeredRatioAction(--mass / this mass), as generichmc.nim does")
+    ap.add_argument("--pv-mass", type=float, default=0.0, help="add --num-pv StaggeredPauliVillarsAction(this mass)")
+    ap.add_argument("--num-pv", type=int, default=1)
+    ap.add_argument("--gauge-steps", type=int, default=0,
+                    help="gauge force on an inner level with this many steps per outer step (0: one level)")
+    ap.add_argument("--gauge-integrator", default="", help="integrator of the inner gauge level (default: --integrator)")
     args = ap.parse_args(argv)
 
     from ._lib import init
@@ -424,10 +618,20 @@ def main(argv=None):
     if args.load:
         fileio.loadGauge(lo, U0, args.load)
     name, lam = parse_integrator(args.integrator)
+    if args.hasenbusch_mass > 0:
+        actions = [StaggeredFermionAction(args.hasenbusch_mass),
+                   StaggeredRatioAction(args.mass, args.hasenbusch_mass)]
+    else:
+        actions = [StaggeredFermionAction(args.mass)]
+    if args.pv_mass > 0:
+        actions += [StaggeredPauliVillarsAction(args.pv_mass) for _ in range(args.num_pv)]
+    gname, glam = parse_integrator(args.gauge_integrator or args.integrator)
     h = StaggeredHMC(lo, U0, args.beta, args.mass, r2req_force=args.frsq,
                      r2req_action=args.arsq, sloppy=Q.SloppyType(args.sloppy), integrator=name,
                      lam=lam, sloppy_links=None if args.sloppy_links < 0 else Q.SloppyType(args.sloppy_links),
-                     recon=args.recon, hisq=args.hisq)
+                     recon=args.recon, hisq=args.hisq, actions=actions, gauge_steps=args.gauge_steps,
+                     gauge_integrator=gname, gauge_lam=glam)
+    print(json.dumps({"actions": h.describe(), "gauge_steps": args.gauge_steps}), flush=True)
     rng = np.random.default_rng(args.seed)
     # warmup: its own integrator, step count and solver storage if given
     prod = (h.integrator, h.lam, h.sloppy)
@@ -463,7 +667,8 @@ def main(argv=None):
         dh = np.array([r["dH"] for r in rows])
         ex = np.exp(-dh)
         summary = {"summary": True, "lat": args.lat, "beta": args.beta, "mass": args.mass, "tau": args.tau,
-                   "steps": args.steps, "integrator": args.integrator, "trajectories": len(rows),
+                   "steps": args.steps, "integrator": args.integrator, "actions": h.describe(),
+                   "gauge_steps": args.gauge_steps, "trajectories": len(rows),
                    "acceptance": float(np.mean([r["accept"] for r in rows])),
                    "dH_mean": float(dh.mean()), "dH_abs_max": float(np.abs(dh).max()),
                    "exp_minus_dH_mean": float(ex.mean()),
