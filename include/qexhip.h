@@ -195,6 +195,11 @@ int qexhip_set_cg_form(int form);
    chunks of that many sites and walked t first (their t +- 1 neighbours are
    read again while still in L2), -1 = chosen by the tuner */
 int qexhip_set_hop_walk(long chunk);
+/* site walk of the two multi-RHS hops of the batched multi-shift solve
+   (qexhip_solve_multishift_batch), no QEX counterpart: same values as
+   qexhip_set_hop_walk; -1 = chosen by the tuner (with QEXHIP_BATCH_TUNE, else
+   the canonical order) */
+int qexhip_set_msb_walk(long chunk);
 /* multi-RHS hop (qexhip_stagD2_multi) launch shape, no QEX counterpart: the
    batches in which a thread loads its right-hand sides' vectors per link (1
    or 2; 0 = chosen by the tuner) and the site walk (0: canonical order; a

@@ -47,6 +47,7 @@ bool g_smear_walk = true;     // QEXHIP_SMEAR_WALK=0: HISQ smearing kernels walk
 int g_multi_batches = 0;      // multi-RHS hop: vector-load batches per link pinned (1, 2) or 0 (tuner)
 long g_multi_chunk = -1;      // multi-RHS hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 long g_hop_chunk = -1;        // one-parity hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
+long g_msb_walk = -1;         // batched multi-shift hops: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 int g_cg_block1 = 0;          // CG forms 3/4: other-parity stencil kernel's block size pinned, 0 (tuner)
 
 #define HIPCHK(x)                                                                                  \
@@ -1052,6 +1053,7 @@ struct MsbWs {
   RedWs red{};
   std::vector<std::pair<int, int>> groups;  // (first rhs, size)
   int block = 0;
+  long chunk = -1;   // site walk of the hops baked into the captured graph
   double m2 = -1.0;  // mass^2 baked into the captured graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1130,13 +1132,13 @@ MsbPtrs msb_ptrs(MsbWs *w, int j0, int nr) {
   return P;
 }
 
-void msb_iter(MsbWs *w, MsbScal *T, double m2, int block) {
+void msb_iter(MsbWs *w, MsbScal *T, double m2, int block, long chunk) {
   qexhip_links *l = w->l;
   const char *links = links_store(l, P64, 18);
   const long lkblk = lkblk_bytes(l, P64, 18);
   for (auto &gr : w->groups)
     launch_msb_iter(l->lat->g, links, lkblk, l->nl, w->par, gr.second, w->nshift, m2,
-                    msb_ptrs(w, gr.first, gr.second), T + gr.first, w->red, block, g_stream);
+                    msb_ptrs(w, gr.first, gr.second), T + gr.first, w->red, block, chunk, g_stream);
 }
 
 void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexhip_field *const *bf, double m,
@@ -1147,14 +1149,17 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
   qexhip_lattice *lat = l->lat;
   const size_t vb = vec64_bytes(lat);
   const double m2 = m * m;
-  // launch shape
+  // launch shape: block and site walk of the hops (a pinned walk fixes the latter)
   int block = g_block_forced ? g_block : 256;
+  long chunk = g_msb_walk > 0 ? g_msb_walk : 0;
   if ((flags & QEXHIP_BATCH_TUNE) && !g_block_forced) {
     const std::string key = "msb nl=" + std::to_string(l->nl) + " nrhs=" + std::to_string(nrhs) +
-                            " nshift=" + std::to_string(nshift) + " " + geom_key(lat->g);
+                            " nshift=" + std::to_string(nshift) + " walk=" + std::to_string(g_msb_walk) + " " +
+                            geom_key(lat->g);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
       block = it->second.block;
+      chunk = it->second.chunk;
     } else {
       std::vector<MsbScal> h(nrhs);
       memset(h.data(), 0, sizeof(MsbScal) * nrhs);
@@ -1171,17 +1176,26 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
         launch_fill(w->p[k], vb, g_stream);
       }
       Tuned best{256, 1, 1e300};
-      for (int b : {64, 128, 256}) {
-        const double us = time_us([&] { msb_iter(w, w->Tx, m2, b); }, 1, 4);
-        if (us < best.us) best = Tuned{b, 1, us};
-      }
+      const long S = lat->g.Vh / lat->g.L[3];
+      for (int b : {64, 128, 256})
+        for (long c : {0L, 512L, 1024L, 2048L}) {
+          if (g_msb_walk >= 0 && c != (g_msb_walk > 0 ? g_msb_walk : 0)) continue;
+          if (c && (c % b || c >= S)) continue;
+          const double us = time_us([&] { msb_iter(w, w->Tx, m2, b, c); }, 1, 4);
+          if (us < best.us) {
+            best = Tuned{b, 1, us};
+            best.chunk = c;
+          }
+        }
       g_tune[key] = best;
       block = best.block;
+      chunk = best.chunk;
     }
   }
   const bool graph = (flags & QEXHIP_BATCH_GRAPH) != 0;
-  if (block != w->block || m2 != w->m2) {
+  if (block != w->block || m2 != w->m2 || chunk != w->chunk) {
     w->block = block;
+    w->chunk = chunk;
     w->m2 = m2;
     if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
     if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
@@ -1216,7 +1230,7 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
   }
   HIPCHK(hipMemcpyAsync(w->T, w->Th, sizeof(MsbScal) * nrhs, hipMemcpyHostToDevice, g_stream));
   if (graph && !w->exec) {
-    capture([&] { for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block); }, &w->graph, &w->exec);
+    capture([&] { for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block, chunk); }, &w->graph, &w->exec);
   }
   for (;;) {
     bool any = false;
@@ -1225,7 +1239,7 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
     if (graph) {
       HIPCHK(hipGraphLaunch(w->exec, g_stream));
     } else {
-      for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block);
+      for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block, chunk);
     }
     check_launch();
     HIPCHK(hipMemcpyAsync(w->Th, w->T, sizeof(MsbScal) * nrhs, hipMemcpyDeviceToHost, g_stream));
@@ -2352,6 +2366,13 @@ int qexhip_set_hop_walk(long chunk) {
   API_BEGIN
   require(chunk == -1 || (chunk >= 0 && chunk % 256 == 0), "site-walk chunk must be -1 (tuner), 0 or a multiple of 256");
   g_hop_chunk = chunk;
+  API_END
+}
+
+int qexhip_set_msb_walk(long chunk) {
+  API_BEGIN
+  require(chunk == -1 || (chunk >= 0 && chunk % 256 == 0), "site-walk chunk must be -1 (tuner), 0 or a multiple of 256");
+  g_msb_walk = chunk;
   API_END
 }
 

@@ -131,19 +131,30 @@ void launch_bcg_flush(const BatchItem *T, int n, long nblocks, int block, hipStr
 // right-hand side that has converged is left out only at the stores and sums
 // (its vectors stay allocated in the workspace, so the loads stay in bounds).
 // Skipping it inside the loop made every right-hand side's loads wait for the
-// previous one's.
+// previous one's.  Sites are taken in the walk of make_walk (chunk 0: the
+// canonical order; else each XCD runs a chunk of a t-slice through all t, so
+// the t +-1, +-3 neighbours of the 8 gathered vectors are read while the
+// same chunk's nearby t-slices are in flight); a workgroup may serve several
+// logical blocks and writes one set of partial sums.
 template <int NL, int NR, int MODE>
 __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, long lkblk, int q, double m4,
-                                                 MsbPtrs P, const MsbScal *T, RedWs red) {
+                                                 MsbPtrs P, const MsbScal *T, RedWs red, long chunk) {
   __shared__ double lds[16 * 3 * NR];
-  const long i = xcd_block(1) * blockDim.x + threadIdx.x;
-  double2 acc[NR][3];
+  const long spb = blockDim.x;
+  const Walk wk = make_walk(g, spb, chunk);
+  const BlockIter it = block_iter(wk.nblk, 1);
+  double sums[3 * NR];
 #pragma unroll
-  for (int r = 0; r < NR; r++)
+  for (int k = 0; k < 3 * NR; k++) sums[k] = 0.0;
+#pragma unroll 1
+  for (long lb = it.cur; lb < it.end; lb += it.step) {
+    const long i = walk_site(wk, lb, spb, threadIdx.x);
+    if (i < 0 || i >= g.Vh) continue;
+    double2 acc[NR][3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) acc[r][c] = make_double2(0.0, 0.0);
-  const bool in = i < g.Vh;
-  if (in) {
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) acc[r][c] = make_double2(0.0, 0.0);
     const Site s = decode(g, i, q);
 #pragma unroll 1
     for (int j = 0; j < 2 * NL; j++) {
@@ -175,9 +186,7 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
             for (int c = 0; c < 3; c++) cjmsub(acc[r][a], U[3 * c + a], v[r][c]);
       }
     }
-  }
-  if constexpr (MODE == 0) {
-    if (in)
+    if constexpr (MODE == 0) {
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         if (!T[r].continuing) continue;
@@ -185,11 +194,7 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
 #pragma unroll
         for (int c = 0; c < 3; c++) dst[vix(g, c, i)] = acc[r][c];
       }
-  } else {
-    double sums[3 * NR];
-#pragma unroll
-    for (int k = 0; k < 3 * NR; k++) sums[k] = 0.0;
-    if (in)
+    } else {
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         if (!T[r].continuing) continue;
@@ -206,8 +211,9 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
           sums[3 * r + 2] += av.x * av.x + av.y * av.y;
         }
       }
-    block_partials<3 * NR>(sums, lds, red);
+    }
   }
+  if constexpr (MODE == 1) block_partials<3 * NR>(sums, lds, red);
 }
 
 // Multi-RHS stagD2 (QEX's stagDPN/stagDN, stagD.nim:132-199,451-456): on parity
@@ -476,13 +482,21 @@ __global__ void __launch_bounds__(1024) k_msb_fin2(RedWs red, int nb, MsbScal *T
   }
 }
 
+// Workgroups of the multi-RHS hops for chunk (see k_mrhs_hop): one per
+// logical block, capped by the partial-sum slots of red (the hop then runs
+// persistent).
+unsigned mrhs_grid(const Geom &g, int block, long chunk, const RedWs &red) {
+  const Walk wk = make_walk(g, block, chunk);
+  return grid_blocks(wk.nblk, 1, red.maxblocks & ~7);
+}
+
 template <int NL, int NR>
 static void msb_iter_t(const Geom &g, const char *links, long lkblk, int par, int nshift, double m2,
-                       const MsbPtrs &P, MsbScal *T, RedWs red, int block, hipStream_t st) {
-  const unsigned nb = (unsigned)((g.Vh + block - 1) / block);
+                       const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, hipStream_t st) {
+  const unsigned nb = mrhs_grid(g, block, chunk, red);
   const dim3 grid(nb), blk(block);
-  hipLaunchKernelGGL((k_mrhs_hop<NL, NR, 0>), grid, blk, 0, st, g, links, lkblk, 1 - par, 0.0, P, T, red);
-  hipLaunchKernelGGL((k_mrhs_hop<NL, NR, 1>), grid, blk, 0, st, g, links, lkblk, par, 4.0 * m2, P, T, red);
+  hipLaunchKernelGGL((k_mrhs_hop<NL, NR, 0>), grid, blk, 0, st, g, links, lkblk, 1 - par, 0.0, P, T, red, chunk);
+  hipLaunchKernelGGL((k_mrhs_hop<NL, NR, 1>), grid, blk, 0, st, g, links, lkblk, par, 4.0 * m2, P, T, red, chunk);
   hipLaunchKernelGGL((k_msb_fin1<NR>), dim3(1), dim3(1024), 0, st, red, (int)nb, T);
   hipLaunchKernelGGL((k_msb_update<NR>), dim3((unsigned)((g.Vh + 255) / 256)), dim3(256), 0, st, g, nshift, P, T,
                      red);
@@ -491,20 +505,20 @@ static void msb_iter_t(const Geom &g, const char *links, long lkblk, int par, in
 
 template <int NL>
 static bool msb_nr(int nr, const Geom &g, const char *links, long lkblk, int par, int nshift, double m2,
-                   const MsbPtrs &P, MsbScal *T, RedWs red, int block, hipStream_t st) {
+                   const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, hipStream_t st) {
   switch (nr) {
-    case 1: msb_iter_t<NL, 1>(g, links, lkblk, par, nshift, m2, P, T, red, block, st); return true;
-    case 2: msb_iter_t<NL, 2>(g, links, lkblk, par, nshift, m2, P, T, red, block, st); return true;
-    case 4: msb_iter_t<NL, 4>(g, links, lkblk, par, nshift, m2, P, T, red, block, st); return true;
-    case 8: msb_iter_t<NL, 8>(g, links, lkblk, par, nshift, m2, P, T, red, block, st); return true;
+    case 1: msb_iter_t<NL, 1>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, st); return true;
+    case 2: msb_iter_t<NL, 2>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, st); return true;
+    case 4: msb_iter_t<NL, 4>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, st); return true;
+    case 8: msb_iter_t<NL, 8>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, st); return true;
   }
   return false;
 }
 
 void launch_msb_iter(const Geom &g, const char *links64, long lkblk, int nl, int par, int nr, int nshift,
-                     double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, hipStream_t st) {
-  const bool ok = (nl == 8) ? msb_nr<8>(nr, g, links64, lkblk, par, nshift, m2, P, T, red, block, st)
-                            : msb_nr<4>(nr, g, links64, lkblk, par, nshift, m2, P, T, red, block, st);
+                     double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, hipStream_t st) {
+  const bool ok = (nl == 8) ? msb_nr<8>(nr, g, links64, lkblk, par, nshift, m2, P, T, red, block, chunk, st)
+                            : msb_nr<4>(nr, g, links64, lkblk, par, nshift, m2, P, T, red, block, chunk, st);
   if (!ok) fprintf(stderr, "qexhip: no multi-RHS instantiation for nr=%d nl=%d\n", nr, nl);
 }
 

@@ -187,8 +187,9 @@ void launch_bcg_flush(const BatchItem *T, int n, long nblocks, int block, hipStr
 // false when no instantiation serves (prec, nl, nr, batches); chunk: site walk (make_walk)
 bool launch_hop_multi(int prec, int nl, int nr, int batches, const Geom &g, const char *links, long lkblk, int par,
                       double ca, double cb, const MultiPtrs &P, int block, int cap, long chunk, hipStream_t st);
+// chunk: site walk of the two multi-RHS hops (make_walk; 0 = canonical order)
 void launch_msb_iter(const Geom &g, const char *links64, long lkblk, int nl, int par, int nr, int nshift,
-                     double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, hipStream_t st);
+                     double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, hipStream_t st);
 
 // gauge.hip
 void launch_plaquette(const Geom &g, const double *U, Scal *S, int slot, RedWs red, hipStream_t st);

@@ -484,6 +484,13 @@ def set_hop_walk(chunk=-1):
     call("qexhip_set_hop_walk", int(chunk))
 
 
+def set_msb_walk(chunk=-1):
+    """Pin the site walk of the batched multi-shift solve's two multi-RHS hops:
+    0 canonical order, a multiple of 256 the t-first walk over chunks of that
+    many sites of a t-slice; -1 lets the tuner choose (when the solve tunes)."""
+    call("qexhip_set_msb_walk", int(chunk))
+
+
 def set_hop_multi_shape(batches=0, chunk=-1):
     """Pin the multi-RHS hop's vector-load batches per link (1 or 2; 0: tuner)
     and its site walk (0: canonical order, a multiple of 256: t-first walk over

@@ -110,6 +110,40 @@ def test_multishift_batch_matches_oracle(oracle, gpu, hisq, freeze):
             assert O.norm2(r[: lo.nEven]) / O.norm2(bs[j]) < 1e-18, (j, k)
 
 
+@pytest.mark.parametrize("walk,tune", [(0, False), (512, True), (1024, False)])
+def test_multishift_batch_site_walk_matches_oracle(oracle, gpu, walk, tune):
+    """The multi-RHS hops of the batched multi-shift solve walked t first over
+    chunks of a t-slice (2048 sites per slice and parity here) give the
+    oracle's solutions, iteration counts and true residuals."""
+    O = oracle
+    L = (16, 16, 16, 8)
+    lo, g, s = _system(O, gpu, L, 83, True)
+    m = 0.05
+    sigmas = [0.0, 0.003, 0.05, 0.6]
+    nr = 2
+    bs = [O.source_gauss(L, 700 + j, 0, 0) for j in range(nr)]
+    fb = [lo.ColorVector().set(b) for b in bs]
+    xs = [[lo.ColorVector() for _ in sigmas] for _ in range(nr)]
+    sps = [gpu.SolverParams() for _ in range(nr)]
+    for sp in sps:
+        sp.r2req, sp.verbosity = 1e-20, 0
+    gpu.set_msb_walk(walk)
+    try:
+        gpu.solveMultiBatch(s, xs, fb, m, sigmas, sps, graph=tune, tune=tune)
+    finally:
+        gpu.set_msb_walk(-1)
+    print(f"walk {walk}: iterations {[sp.iterations for sp in sps]}", flush=True)
+    for j in range(nr):
+        want, its, _ = O.cgm(L, g, 0, bs[j], m, sigmas, 1e-20)
+        assert abs(sps[j].iterations - its) <= 1, (j, sps[j].iterations, its)
+        for k, sg in enumerate(sigmas):
+            got = xs[j][k].get(sub=0)
+            assert rel(got[: lo.nEven], want[k][: lo.nEven]) < 1e-8, (j, k)
+            r = bs[j] - O.stagD2xx(L, g, 0, got, m * m) - sg * got
+            assert O.norm2(r[: lo.nEven]) / O.norm2(bs[j]) < 1e-18, (j, k)
+            assert sps[j].shiftR2[k] < 1e-18, (j, k, sps[j].shiftR2)
+
+
 def test_multishift_batch_eight_rhs_matches_single(oracle, gpu):
     O = oracle
     L = (8, 8, 8, 4)
