@@ -441,7 +441,9 @@ def batched_study(compact=False):
         (QEX normalization 4(m^2 - D_eo D_oe) + s_k with 4m^2 = sigma_0,
         s_k = sigma_k - sigma_0): one batched multi-shift solve (each link load
         serves the 8 sources; hipGraph replay; autotuned block) vs 8 calls of
-        the single-source multi-shift solver.
+        the single-source multi-shift solver; then source 0 through the
+        mixed-precision multi-shift solve (FP32 recursion, every shift refined
+        by the mixed CG with FP32, or Q16-link/FP16, storage).
     (b) 16 plain solves, 4 each on 8^4, 12^4, 16^4 and 16^3x32, distinct
         seeds, masses uniform in [0.01, 0.1]: one batched solve vs 16 calls of
         solveEE (same FP64 CG).
@@ -523,6 +525,21 @@ def batched_study(compact=False):
     res["max_rel_diff_vs_single"] = max(errs)
     for name, _ in msb_forms:
         res[name]["speedup_vs_one_at_a_time"] = round(t1 / res[name]["seconds"], 3)
+    # one source through the mixed-precision multi-shift solve (FP32 recursion,
+    # then each shift refined by the mixed CG on the given storage), against
+    # the FP64 single-source solve above (t1 / nr per source)
+    for name, sl, lk in (("one_source_mixed_fp32", Q.SloppyType.SloppySingle, None),
+                         ("one_source_mixed_q16links_fp16", Q.SloppyType.SloppyHalf, Q.SloppyType.SloppyHalfFixed)):
+        def spm(maxits=100000):
+            sp = sp_new()
+            sp.maxits, sp.sloppySolve, sp.sloppyLinks = maxits, sl, lk
+            return sp
+        Q.solveMulti(s, ys, bs[0], m, shifts, spm(64))  # tunes the FP32 hop and the mixed CG
+        spx = spm()
+        tm = timed(lambda: Q.solveMulti(s, ys, bs[0], m, shifts, spx))
+        res[name] = {"seconds": round(tm, 4), "iterations": spx.iterations, "reliable_updates": spx.reliableUpdates,
+                     "max_true_r2_any_shift": max(spx.shiftR2),
+                     "speedup_vs_fp64_single_source": round(t1 / nr / tm, 3)}
     out["multishift_multirhs"] = res
     del s, bs, xs, ys, e
     torch.cuda.empty_cache()

@@ -156,7 +156,13 @@ int qexhip_solve_ee_host(qexhip_links *l, double *r, const double *t, double m, 
 int qexhip_solve_oo_host(qexhip_links *l, double *r, const double *t, double m, qexhip_solver_params *sp);
 /* multi-shift CG on a parity (solveXXInternal multi-shift, stagSolve.nim:164-229,
    and CgmState.solve, src/solvers/cgm.nim:68-246): solves
-   (A + sigma_k) x_k = b, k = 0..nshift-1, with sigma_0 treated as 0. */
+   (A + sigma_k) x_k = b, k = 0..nshift-1, with sigma_0 treated as 0.
+   sp->sloppy == QEXHIP_FP64: QEX's FP64 recursion.  Otherwise (no QEX
+   counterpart) the recursion runs on FP32 storage until |r|^2 <=
+   max(r2req, 1e-12) |b|^2 (QEXHIP_MS_SLOPPY_R2 sets the 1e-12), and each shift is then
+   refined to r2req by defect correction around the mixed CG of
+   qexhip_solve_xx (sp's sloppy storage); iterations count both phases.
+   sp->r2 and sp->shift_r2 are FP64 true residuals either way. */
 int qexhip_solve_multishift(qexhip_links *l, int par, qexhip_field *const *xs, const qexhip_field *b, double m,
                             const double *sigmas, int nshift, qexhip_solver_params *sp);
 

@@ -48,6 +48,7 @@ int g_multi_batches = 0;      // multi-RHS hop: vector-load batches per link pin
 long g_multi_chunk = -1;      // multi-RHS hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 long g_hop_chunk = -1;        // one-parity hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 long g_msb_walk = -1;         // batched multi-shift hops: site-walk chunk pinned (0: canonical order) or -1 (tuner)
+double g_ms_sloppy_r2 = 1e-12; // mixed multi-shift: FP32 phase ends at |r|^2 <= this |b|^2 (QEXHIP_MS_SLOPPY_R2)
 int g_cg_block1 = 0;          // CG forms 3/4: other-parity stencil kernel's block size pinned, 0 (tuner)
 
 #define HIPCHK(x)                                                                                  \
@@ -126,6 +127,10 @@ struct Work {
   std::vector<double *> msp;
   double **d_x = nullptr, **d_p = nullptr;
   ShiftScal *T = nullptr, *Th = nullptr;
+  // mixed-precision multi-shift: FP32 shifted solutions and search vectors
+  int nms32 = 0;
+  std::vector<void *> msx32, msp32;
+  void **d_x32 = nullptr, **d_p32 = nullptr;
 };
 
 struct qexhip_links {
@@ -227,6 +232,10 @@ void free_work(Work *w) {
                   (void *)w->T})
     if (p) (void)hipFree(p);
   for (double *p : w->msp) (void)hipFree(p);
+  for (void *p : w->msx32) (void)hipFree(p);
+  for (void *p : w->msp32) (void)hipFree(p);
+  if (w->d_x32) (void)hipFree(w->d_x32);
+  if (w->d_p32) (void)hipFree(w->d_p32);
   if (w->Sh) (void)hipHostFree(w->Sh);
   if (w->Th) (void)hipHostFree(w->Th);
   delete w;
@@ -679,6 +688,46 @@ void solve_xx(qexhip_links *l, int par, double *r, const double *x, double m, qe
   sp->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Defect correction of an FP64 solution x of 4(m2 - D_pq D_qp) x = b on parity
+// par (the loop of solve_xx, started from x instead of 0): while the true
+// residual exceeds r2stop and falls, solve for the correction with cg_solve
+// (sp's sloppy storage, reliable updates) and add it.  Adds the iterations
+// and reliable updates to *its, *nrel; returns the final |b - A x|^2.
+double refine_xx(qexhip_links *l, int par, double *x, const double *b, double m2, double r2stop,
+                 const qexhip_solver_params *sp, int *its, int *nrel) {
+  Work *w = work(l);
+  qexhip_lattice *lat = l->lat;
+  const Geom &g = lat->g;
+  Scal *S = w->S, *Sh = w->Sh;
+  const size_t vb = vec64_bytes(lat);
+  auto residual = [&] {
+    memset(Sh, 0, sizeof(Scal));
+    Sh->m2 = m2;
+    push(S, Sh);
+    resid64(l, par, x, b, w->rho, S, 3, w->t64);  // rho = b - A x
+    check_launch();
+    pull(Sh, S);
+    return Sh->tmp[0];
+  };
+  double r2 = residual(), r2prev = INFINITY;
+  int used = 0;
+  while (r2 > r2stop && r2 < r2prev) {
+    r2prev = r2;
+    const int mx = sp->maxits - used;
+    if (mx <= 0) break;
+    launch_fill(w->y, vb, g_stream);
+    int it = 0, nr = 0;
+    cg_solve(l, par, w->y, w->rho, std::sqrt(m2), r2stop / r2, mx, sp->sloppy, sp->sloppy_links, sp->recon,
+             sp->reliable_delta, sp->reliable_max_its, &it, &nr);
+    used += it;
+    *nrel += nr;
+    launch_axpby64(g, 1.0, w->y, 1.0, x, g_stream);  // x += y
+    r2 = residual();
+  }
+  *its += used;
+  return r2;
+}
+
 // |b - (A + sigma) x|^2 in FP64 on parity par, A = 4(m^2 - D_pq D_qp): the
 // true residual of one shifted system, which cgm.nim:208-239 computes for
 // every shift at verbosity 3.  Uses the links' FP64 work vectors r64, t64.
@@ -707,9 +756,17 @@ void multishift_true_r2(qexhip_links *l, int par, const double *const *xs, const
   }
 }
 
-// Multi-shift CG (src/solvers/cgm.nim:68-246), FP64, parity par.
+void solve_ms_mixed(qexhip_links *l, int par, double *const *xs, const double *b, double m, const double *sigmas,
+                    int nshift, qexhip_solver_params *sp);
+
+// Multi-shift CG (src/solvers/cgm.nim:68-246), FP64, parity par.  With
+// sloppy storage requested (sp->sloppy != FP64) the mixed form below runs.
 void solve_ms(qexhip_links *l, int par, double *const *xs, const double *b, double m, const double *sigmas,
               int nshift, qexhip_solver_params *sp) {
+  if (sp->sloppy != P64) {
+    solve_ms_mixed(l, par, xs, b, m, sigmas, nshift, sp);
+    return;
+  }
   auto t0 = std::chrono::steady_clock::now();
   Work *w = work(l);
   qexhip_lattice *lat = l->lat;
@@ -753,6 +810,7 @@ void solve_ms(qexhip_links *l, int par, double *const *xs, const double *b, doub
   for (int k = 0; k < nshift; k++) {
     Th->sg[k] = (k == 0) ? 0.0 : sigmas[k];
     Th->zi[k] = Th->zim1[k] = 1.0;
+    Th->active[k] = 1;
   }
   Th->continuing = (b2 > Th->r2stop) && sp->maxits > 0;
   HIPCHK(hipMemcpyAsync(T, Th, sizeof(ShiftScal), hipMemcpyHostToDevice, g_stream));
@@ -765,8 +823,9 @@ void solve_ms(qexhip_links *l, int par, double *const *xs, const double *b, doub
       hop(l, P64, 18, 1 - par, 0, 0, w->msp[0], w->t64, nullptr, nullptr, 0, 0, 1.0, 0, 1, S);
       hop(l, P64, 18, par, 1, 1, w->t64, w->Ap64, w->msp[0], nullptr, 1.0, 0, -1.0, 1, 1, S);
       launch_cgm_alpha(T, S, g_stream);
-      launch_cgm_xr(g, nshift, w->r64, w->Ap64, w->d_x, w->d_p, T, S, lat->red, g_stream);
-      launch_cgm_p(g, nshift, w->r64, w->d_p, T, g_stream);
+      launch_cgm_xr(g, nshift, P64, w->r64, w->Ap64, (void *const *)w->d_x, (void *const *)w->d_p, T, S, lat->red,
+                    g_stream);
+      launch_cgm_p(g, nshift, P64, w->r64, (void *const *)w->d_p, T, g_stream);
     }
     check_launch();
     HIPCHK(hipMemcpyAsync(Th, T, sizeof(ShiftScal), hipMemcpyDeviceToHost, g_stream));
@@ -775,6 +834,109 @@ void solve_ms(qexhip_links *l, int par, double *const *xs, const double *b, doub
   }
   sp->iterations = Th->its;
   sp->flops = flops_per_its(l) * Th->its + (double)(4 * g.Vh + 15) * Th->its * nshift;
+  multishift_true_r2(l, par, xs, b, b2, m, sigmas, nshift, sp);
+  sp->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Mixed-precision multi-shift CG, no QEX counterpart (cgm.nim runs in FP64
+// only).  Two phases:
+//  1. the multi-shift recursion of solve_ms on FP32 storage with FP32 links
+//     (FP32 stencil arithmetic, FP64 updates and reductions) until the base
+//     residual |r|^2 <= max(r2req, g_ms_sloppy_r2) |b|^2; a shift whose
+//     residual zeta_k^2 |r|^2 met that bound stops being updated;
+//  2. every shift refined from its phase-1 solution to r2req by the defect
+//     correction of refine_xx on its own system 4(m^2 + sigma_k/4 - D D),
+//     whose inner solves are the mixed CG of cg_solve on sp's sloppy storage.
+// sp->iterations counts both phases; the true residuals are recomputed in FP64.
+void solve_ms_mixed(qexhip_links *l, int par, double *const *xs, const double *b, double m, const double *sigmas,
+                    int nshift, qexhip_solver_params *sp) {
+  auto t0 = std::chrono::steady_clock::now();
+  Work *w = work(l);
+  qexhip_lattice *lat = l->lat;
+  const Geom &g = lat->g;
+  const size_t vb32 = (size_t)3 * g.stride * 8;
+  require(nshift >= 1 && nshift <= QH_MAXSHIFT, "multishift: nshift out of range");
+  for (int k = 1; k < nshift; k++)
+    require(m * m + 0.25 * sigmas[k] > 0.0, "multishift: a shifted system is not positive definite");
+  if (w->nms32 < nshift) {
+    for (int k = w->nms32; k < nshift; k++) {
+      w->msx32.push_back(dalloc<void>(vb32));
+      w->msp32.push_back(dalloc<void>(vb32));
+    }
+    if (w->d_x32) (void)hipFree(w->d_x32);
+    if (w->d_p32) (void)hipFree(w->d_p32);
+    w->d_x32 = dalloc<void *>(sizeof(void *) * nshift);
+    w->d_p32 = dalloc<void *>(sizeof(void *) * nshift);
+    w->nms32 = nshift;
+  }
+  if (w->nms < nshift) {  // the FP64 output pointer array
+    if (w->d_x) (void)hipFree(w->d_x);
+    if (w->d_p) (void)hipFree(w->d_p);
+    for (int k = w->nms; k < nshift; k++) w->msp.push_back(dalloc<double>(vec64_bytes(lat)));
+    w->d_x = dalloc<double *>(sizeof(double *) * nshift);
+    w->d_p = dalloc<double *>(sizeof(double *) * nshift);
+    w->nms = nshift;
+  }
+  if (!w->T) {
+    w->T = dalloc<ShiftScal>(sizeof(ShiftScal));
+    HIPCHK(hipHostMalloc((void **)&w->Th, sizeof(ShiftScal), hipHostMallocDefault));
+  }
+  std::vector<double *> xv(xs, xs + nshift);
+  HIPCHK(hipMemcpyAsync(w->d_x, xv.data(), sizeof(double *) * nshift, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(w->d_x32, w->msx32.data(), sizeof(void *) * nshift, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(w->d_p32, w->msp32.data(), sizeof(void *) * nshift, hipMemcpyHostToDevice, g_stream));
+  Scal *S = w->S, *Sh = w->Sh;
+  ShiftScal *T = w->T, *Th = w->Th;
+  memset(Sh, 0, sizeof(Scal));
+  Sh->m2 = m * m;
+  push(S, Sh);
+  launch_norm2(P64, g, b, S, 1, lat->red, g_stream);
+  check_launch();
+  pull(Sh, S);
+  const double b2 = Sh->tmp[1];
+  // phase 1
+  launch_cgm_start32(g, nshift, b, w->rs, w->d_x32, w->d_p32, g_stream);
+  memset(Th, 0, sizeof(ShiftScal));
+  Th->nshift = nshift;
+  Th->maxits = sp->maxits;
+  Th->r2stop = std::max(sp->r2req, g_ms_sloppy_r2) * b2;
+  Th->r2i = b2;
+  Th->alphaim1 = -1.0;
+  Th->betaim1 = 0.0;
+  Th->freeze = 1;
+  for (int k = 0; k < nshift; k++) {
+    Th->sg[k] = (k == 0) ? 0.0 : sigmas[k];
+    Th->zi[k] = Th->zim1[k] = 1.0;
+    Th->active[k] = 1;
+  }
+  Th->continuing = (b2 > Th->r2stop) && sp->maxits > 0;
+  HIPCHK(hipMemcpyAsync(T, Th, sizeof(ShiftScal), hipMemcpyHostToDevice, g_stream));
+  Sh->done = !Th->continuing;
+  push(S, Sh);
+  void *p0 = w->msp32[0];
+  while (Th->continuing) {
+    for (int c = 0; c < kChunk; c++) {
+      hop(l, P32, 18, 1 - par, 0, 0, p0, w->ts, nullptr, nullptr, 0, 0, 1.0, 0, 1, S);
+      hop(l, P32, 18, par, 1, 1, w->ts, w->Aps, p0, nullptr, 1.0, 0, -1.0, 1, 1, S);
+      launch_cgm_alpha(T, S, g_stream);
+      launch_cgm_xr(g, nshift, P32, w->rs, w->Aps, w->d_x32, w->d_p32, T, S, lat->red, g_stream);
+      launch_cgm_p(g, nshift, P32, w->rs, w->d_p32, T, g_stream);
+    }
+    check_launch();
+    HIPCHK(hipMemcpyAsync(Th, T, sizeof(ShiftScal), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+  }
+  const int its1 = Th->its;
+  launch_cgm_export32(g, nshift, w->d_x32, w->d_x, g_stream);
+  check_launch();
+  // phase 2
+  int its = its1, nrel = 0;
+  const double r2stop = sp->r2req * b2;
+  for (int k = 0; k < nshift; k++)
+    refine_xx(l, par, xs[k], b, m * m + (k == 0 ? 0.0 : 0.25 * sigmas[k]), r2stop, sp, &its, &nrel);
+  sp->iterations = its;
+  sp->reliable_updates = nrel;
+  sp->flops = flops_per_its(l) * its + (double)(4 * g.Vh + 15) * its1 * nshift;
   multishift_true_r2(l, par, xs, b, b2, m, sigmas, nshift, sp);
   sp->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -1313,6 +1475,7 @@ int qexhip_init(int device) {
   }
   if (const char *r = getenv("QEXHIP_XCD_REMAP")) g_remap = atoi(r) != 0;
   if (const char *w = getenv("QEXHIP_SMEAR_WALK")) g_smear_walk = atoi(w) != 0;
+  if (const char *w = getenv("QEXHIP_MS_SLOPPY_R2")) g_ms_sloppy_r2 = atof(w);
   API_END
 }
 

@@ -95,6 +95,7 @@ static bool hop_modes(int mode, int redm, const HopArgs &a, hipStream_t st, int 
   }
   QH_HOP(0, 0) QH_HOP(1, 0) QH_HOP(2, 0)
   if constexpr (PL == P64) { QH_HOP(1, 1) QH_HOP(2, 2) QH_HOP(2, 3) }
+  if constexpr (PL == P32) { QH_HOP(1, 1) }
 #undef QH_HOP
   return false;
 }

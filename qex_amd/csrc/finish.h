@@ -105,7 +105,10 @@ __device__ inline void epi_cgm(ShiftScal *T, Scal *S, double r2ip1) {
   S->done = !T->continuing;
   T->pc1[0] = 1.0;
   T->pc2[0] = beta;
-  for (int k = 1; k < T->nshift; k++) T->pc2[k] *= beta;
+  for (int k = 1; k < T->nshift; k++) {
+    T->pc2[k] *= beta;
+    if (T->freeze && T->zi[k] * T->zi[k] * r2ip1 <= T->r2stop) T->active[k] = 0;
+  }
   T->alphaim1 = T->alpha;
   T->betaim1 = beta;
   T->r2i = r2ip1;

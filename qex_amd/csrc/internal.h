@@ -50,6 +50,8 @@ struct ShiftScal {
   double alpha, beta, alphaim1, betaim1, r2i, r2ip1;
   int nshift, continuing, its, maxits;
   double r2stop;
+  int freeze;                  // stop updating shift k once zeta_k^2 |r|^2 <= r2stop
+  int active[QH_MAXSHIFT];     // shifts still updated (all, unless freeze)
 };
 
 // Generic one-parity hop: out = ca*A + cb*B + chop*(2D)in on parity par.
@@ -172,9 +174,14 @@ void launch_redot(int prec, const Geom &g, const void *x, const void *y, Scal *S
 void launch_rs_from_true(const Geom &g, void *rs, const double *rtrue, Scal *S, int pr, hipStream_t st);
 void launch_xfold(const Geom &g, double *x, void *xs, Scal *S, int px, hipStream_t st);
 void launch_scal_init(Scal *S, hipStream_t st, int mode);
-void launch_cgm_xr(const Geom &g, int nshift, double *r, const double *Ap, double **x, double **p,
+// multi-shift updates on FP64 (prec P64) or FP32 (P32) storage; x, p: device
+// arrays of nshift vector pointers
+void launch_cgm_xr(const Geom &g, int nshift, int prec, void *r, const void *Ap, void *const *x, void *const *p,
                    ShiftScal *T, Scal *S, RedWs red, hipStream_t st);
-void launch_cgm_p(const Geom &g, int nshift, const double *r, double **p, ShiftScal *T, hipStream_t st);
+void launch_cgm_p(const Geom &g, int nshift, int prec, const void *r, void *const *p, ShiftScal *T, hipStream_t st);
+void launch_cgm_start32(const Geom &g, int nshift, const double *b, void *r, void *const *x, void *const *p,
+                        hipStream_t st);
+void launch_cgm_export32(const Geom &g, int nshift, void *const *x32, double *const *x64, hipStream_t st);
 void launch_cgm_alpha(ShiftScal *T, Scal *S, hipStream_t st);
 void launch_outer_acc(const Geom &g, const double *a_e, const double *a_o, const double *b_e, const double *b_o,
                       double ce, double co, double *F, int nl, hipStream_t st);

@@ -263,10 +263,20 @@ void launch_cgm_alpha(ShiftScal *T, Scal *S, hipStream_t st) {
   hipLaunchKernelGGL(k_cgm_alpha, dim3(1), dim3(64), 0, st, T, S);
 }
 
-// r -= alpha Ap ; x_k += xc[k] p_k (all k) ; r2 = |r|^2
-__global__ void __launch_bounds__(256) k_cgm_xr(Geom g, int nshift, double2 *r, const double2 *Ap,
-                                                double2 *const *x, double2 *const *p, ShiftScal *T,
-                                                Scal *S, RedWs red) {
+// Loads and stores of one complex entry of an FP64 (double2) or FP32
+// (float2) parity vector; the multi-shift updates compute in FP64 either way.
+__device__ __forceinline__ double2 ld2d(const double2 *p, long o) { return p[o]; }
+__device__ __forceinline__ double2 ld2d(const float2 *p, long o) {
+  const float2 v = p[o];
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st2d(double2 *p, long o, double2 v) { p[o] = v; }
+__device__ __forceinline__ void st2d(float2 *p, long o, double2 v) { p[o] = make_float2((float)v.x, (float)v.y); }
+
+// r -= alpha Ap ; x_k += xc[k] p_k (every active k) ; r2 = |r|^2
+template <class V>
+__global__ void __launch_bounds__(256) k_cgm_xr(Geom g, int nshift, V *r, const V *Ap, V *const *x, V *const *p,
+                                                ShiftScal *T, Scal *S, RedWs red) {
   __shared__ double lds[16];
   if (!T->continuing) return;
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -276,21 +286,24 @@ __global__ void __launch_bounds__(256) k_cgm_xr(Geom g, int nshift, double2 *r, 
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const long o = vix(g, c, i);
-      double2 rv = r[o], av = Ap[o];
+      double2 rv = ld2d(r, o);
+      const double2 av = ld2d(Ap, o);
       rv.x -= alpha * av.x; rv.y -= alpha * av.y;
-      r[o] = rv;
+      st2d(r, o, rv);
       s += rv.x * rv.x + rv.y * rv.y;
     }
     for (int k = 0; k < nshift; k++) {
+      if (!T->active[k]) continue;
       const double xc = T->xc[k];
-      double2 *xk = x[k];
-      const double2 *pk = p[k];
+      V *xk = x[k];
+      const V *pk = p[k];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const long o = vix(g, c, i);
-        double2 xv = xk[o], pv = pk[o];
+        double2 xv = ld2d(xk, o);
+        const double2 pv = ld2d(pk, o);
         xv.x += xc * pv.x; xv.y += xc * pv.y;
-        xk[o] = xv;
+        st2d(xk, o, xv);
       }
     }
   }
@@ -298,37 +311,85 @@ __global__ void __launch_bounds__(256) k_cgm_xr(Geom g, int nshift, double2 *r, 
   block_partials<1>(v, lds, red);
 }
 
-// p_k = pc1[k] r + pc2[k] p_k (only while continuing, cgm.nim:186,198-199)
-__global__ void k_cgm_p(Geom g, int nshift, const double2 *r, double2 *const *p, ShiftScal *T) {
+// p_k = pc1[k] r + pc2[k] p_k (every active k, only while continuing, cgm.nim:186,198-199)
+template <class V>
+__global__ void k_cgm_p(Geom g, int nshift, const V *r, V *const *p, ShiftScal *T) {
   if (!T->continuing) return;
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.Vh) return;
   double2 rv[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++) rv[c] = r[vix(g, c, i)];
+  for (int c = 0; c < 3; c++) rv[c] = ld2d(r, vix(g, c, i));
   for (int k = 0; k < nshift; k++) {
+    if (!T->active[k]) continue;
     const double a = T->pc1[k], b = T->pc2[k];
-    double2 *pk = p[k];
+    V *pk = p[k];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const long o = vix(g, c, i);
-      double2 pv = pk[o];
-      pk[o] = make_double2(a * rv[c].x + b * pv.x, a * rv[c].y + b * pv.y);
+      const double2 pv = ld2d(pk, o);
+      st2d(pk, o, make_double2(a * rv[c].x + b * pv.x, a * rv[c].y + b * pv.y));
     }
   }
 }
 
-void launch_cgm_xr(const Geom &g, int nshift, double *r, const double *Ap, double **x, double **p, ShiftScal *T,
-                   Scal *S, RedWs red, hipStream_t st) {
-  hipLaunchKernelGGL(k_cgm_xr, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (double2 *)r,
-                     (const double2 *)Ap, (double2 *const *)x, (double2 *const *)p, T, S, red);
+void launch_cgm_xr(const Geom &g, int nshift, int prec, void *r, const void *Ap, void *const *x, void *const *p,
+                   ShiftScal *T, Scal *S, RedWs red, hipStream_t st) {
+  if (prec == P32)
+    hipLaunchKernelGGL(k_cgm_xr<float2>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (float2 *)r,
+                       (const float2 *)Ap, (float2 *const *)x, (float2 *const *)p, T, S, red);
+  else
+    hipLaunchKernelGGL(k_cgm_xr<double2>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (double2 *)r,
+                       (const double2 *)Ap, (double2 *const *)x, (double2 *const *)p, T, S, red);
   FinArgs f{red, (int)nblk(g.Vh, 256), S, T, 0, 0.0, 3};
   launch_finish(EPI_CGM, f, st);
 }
 
-void launch_cgm_p(const Geom &g, int nshift, const double *r, double **p, ShiftScal *T, hipStream_t st) {
-  hipLaunchKernelGGL(k_cgm_p, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (const double2 *)r,
-                     (double2 *const *)p, T);
+void launch_cgm_p(const Geom &g, int nshift, int prec, const void *r, void *const *p, ShiftScal *T, hipStream_t st) {
+  if (prec == P32)
+    hipLaunchKernelGGL(k_cgm_p<float2>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (const float2 *)r,
+                       (float2 *const *)p, T);
+  else
+    hipLaunchKernelGGL(k_cgm_p<double2>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (const double2 *)r,
+                       (double2 *const *)p, T);
+}
+
+// Start of the FP32 multi-shift recursion: r = b, p_k = b, x_k = 0 (cgm.nim:122-158)
+__global__ void k_cgm_start32(Geom g, int nshift, const double2 *b, float2 *r, float2 *const *x, float2 *const *p) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.Vh) return;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const long o = vix(g, c, i);
+    const double2 bv = b[o];
+    st2d(r, o, bv);
+    for (int k = 0; k < nshift; k++) {
+      st2d(p[k], o, bv);
+      x[k][o] = make_float2(0.f, 0.f);
+    }
+  }
+}
+
+// FP64 copies of the FP32 shifted solutions
+__global__ void k_cgm_export32(Geom g, int nshift, float2 *const *x32, double2 *const *x64) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.Vh) return;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const long o = vix(g, c, i);
+    for (int k = 0; k < nshift; k++) x64[k][o] = ld2d(x32[k], o);
+  }
+}
+
+void launch_cgm_start32(const Geom &g, int nshift, const double *b, void *r, void *const *x, void *const *p,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_cgm_start32, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (const double2 *)b,
+                     (float2 *)r, (float2 *const *)x, (float2 *const *)p);
+}
+
+void launch_cgm_export32(const Geom &g, int nshift, void *const *x32, double *const *x64, hipStream_t st) {
+  hipLaunchKernelGGL(k_cgm_export32, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (float2 *const *)x32,
+                     (double2 *const *)x64);
 }
 
 // ------------------------------------------------------------------- force

@@ -104,6 +104,54 @@ def test_multishift_matches_oracle(oracle, gpu):
     assert sp.r2 == sp.shiftR2[0] and sp.r2 <= 1.0001e-20
 
 
+@pytest.mark.parametrize("hisq,sloppy,slinks", [(False, 1, -1), (False, 2, 4), (True, 1, -1), (True, 2, 4)])
+def test_mixed_multishift_matches_oracle(oracle, gpu, hisq, sloppy, slinks):
+    """Mixed-precision multi-shift (FP32 recursion, then every shift refined by
+    the mixed CG on FP32 or Q16-link/FP16 storage): the oracle's solutions and
+    each shifted system's FP64 true residual at r2req."""
+    O = oracle
+    L = (8, 8, 8, 8)
+    m = 0.05
+    g = links(O, L, hisq)
+    b = O.source_gauss(L, 19, 0, 0)
+    sig = [0.0, 0.002, 0.03, 0.4, 2.0]
+    xs_o, its_o, _ = O.cgm(L, g, 0, b, m, sig, 1e-24)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    fxs = [lo.ColorVector() for _ in sig]
+    sp = gpu.SolverParams()
+    sp.r2req, sp.verbosity = 1e-20, 0
+    sp.sloppySolve = gpu.SloppyType(sloppy)
+    sp.sloppyLinks = None if slinks < 0 else gpu.SloppyType(slinks)
+    gpu.solveMulti(s, fxs, fb, m, sig, sp)
+    print(f"hisq={hisq} sloppy={sloppy} links={slinks}: {sp.iterations} iterations "
+          f"(FP64 CGM {its_o}), reliable updates {sp.reliableUpdates}", flush=True)
+    ne = lo.nEven
+    for k, f in enumerate(fxs):
+        got = f.get(sub=0)[:ne]
+        assert rel(got, xs_o[k][:ne]) < 1e-8, k
+        r = b - O.stagD2xx(L, g, 0, f.get(), m * m) - sig[k] * f.get()
+        r2 = O.norm2(r[:ne]) / O.norm2(b)
+        assert r2 <= 1.0001e-20, (k, r2)
+        assert abs(sp.shiftR2[k] - r2) <= 1e-2 * r2 + 1e-30, (k, sp.shiftR2[k], r2)
+    assert sp.r2 == sp.shiftR2[0]
+
+
+def test_mixed_multishift_rejects_indefinite_shift(gpu):
+    lo = gpu.Layout((4, 4, 4, 4))
+    import torch
+    from qex_amd import gauge
+    s = gpu.Staggered(lo, gauge.unit_gauge(lo, device="cuda"))
+    fb = lo.ColorVector().set(gauge.source(lo, 3, 0, 0))
+    fxs = [lo.ColorVector() for _ in range(2)]
+    sp = gpu.SolverParams()
+    sp.sloppySolve = gpu.SloppyType.SloppySingle
+    with pytest.raises(gpu.QexHipError, match="positive definite"):
+        gpu.solveMulti(s, fxs, fb, 0.1, [0.0, -0.05], sp)
+    torch.cuda.synchronize()
+
+
 def test_force_matches_oracle(oracle, gpu):
     O = oracle
     L = (8, 8, 8, 8)

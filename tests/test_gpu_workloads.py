@@ -269,6 +269,49 @@ def test_multishift_batch_24x24x24x48_eight_sources(oracle, gpu, fixtures):
     assert max(fx["true_r2_per_shift"]) <= 1e-19
 
 
+@pytest.mark.parametrize("sloppy,slinks", [(1, -1), (2, 4)])
+def test_mixed_multishift_24x24x24x48(oracle, gpu, fixtures, sloppy, slinks):
+    """Source 0 of the same workload through the mixed-precision multi-shift
+    solve (FP32 recursion, every shift refined by the mixed CG): each shifted
+    system's oracle-recomputed true residual at r2req and the fixture's
+    solution sample."""
+    import time
+    import torch
+    O = oracle
+    fx = need(fixtures, "cgm_24x24x24x48")
+    L = tuple(fx["lattice"])
+    U = O.phased(L, O.gauge_random(L, fx["gauge_seed"]))
+    g = O.hisq_links(L, fx["hisq_seed"], U)
+    del U
+    m, shifts = fx["mass"], fx["shifts"]
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    b = O.source_gauss(L, fx["source_seed"], 0, 0)
+    fb = lo.ColorVector().set(b)
+    xs = [lo.ColorVector() for _ in shifts]
+    sp = gpu.SolverParams()
+    sp.r2req, sp.verbosity, sp.maxits = R2REQ, 0, 100000
+    sp.sloppySolve = gpu.SloppyType(sloppy)
+    sp.sloppyLinks = None if slinks < 0 else gpu.SloppyType(slinks)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gpu.solveMulti(s, xs, fb, m, shifts, sp)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    print(f"mixed multi-shift 24^3x48 sloppy={sloppy} links={slinks}: {sp.iterations} iterations, "
+          f"{sp.reliableUpdates} reliable updates, {sec:.3f} s (FP64 CGM: {fx['iterations']} iterations)", flush=True)
+    worst = 0.0
+    for k, sg in enumerate(shifts):
+        x = xs[k].get()
+        r2 = true_r2(O, L, g, b, x, m, sg)
+        worst = max(worst, r2)
+        assert abs(sp.shiftR2[k] - r2) <= 1e-2 * r2 + 1e-30, (k, sp.shiftR2[k], r2)
+        assert r2 <= 1.0001 * R2REQ, (k, r2)
+        e = rel(sample(x, fx["sample_sites"]), fixture_sample(fx["x_sample_per_shift"][k]))
+        assert e < 1e-6, (k, e)
+    print(f"  worst oracle-recomputed shifted r2 {worst:.3e}", flush=True)
+
+
 # ---------------------------------------------------------------- config 4(b)
 def test_uneven_batch_sixteen_lattices(oracle, gpu, fixtures):
     O = oracle
