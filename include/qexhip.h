@@ -157,7 +157,10 @@ int qexhip_solve_oo_host(qexhip_links *l, double *r, const double *t, double m, 
 /* multi-shift CG on a parity (solveXXInternal multi-shift, stagSolve.nim:164-229,
    and CgmState.solve, src/solvers/cgm.nim:68-246): solves
    (A + sigma_k) x_k = b, k = 0..nshift-1, with sigma_0 treated as 0.
-   sp->sloppy == QEXHIP_FP64: QEX's FP64 recursion.  Otherwise (no QEX
+   sp->sloppy == QEXHIP_FP64: QEX's FP64 recursion, every shift updated until
+   the base system converges; the shifted x_k, p_k are advanced once per 16
+   iterations from that window's residuals (the same sums, reassociated; the
+   batched solve's machinery with one right-hand side).  Otherwise (no QEX
    counterpart) the recursion runs on FP32 storage until |r|^2 <=
    max(r2req, 1e-12) |b|^2 (QEXHIP_MS_SLOPPY_R2 sets the 1e-12), and each shift is then
    refined to r2req by defect correction around the mixed CG of

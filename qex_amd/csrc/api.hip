@@ -124,8 +124,7 @@ struct Work {
   Scal *S1 = nullptr;  // second scalar buffer of the form-4 CG iteration
   // multi-shift
   int nms = 0;
-  std::vector<double *> msp;
-  double **d_x = nullptr, **d_p = nullptr;
+  double **d_x = nullptr;  // mixed multi-shift: FP64 solution pointers
   ShiftScal *T = nullptr, *Th = nullptr;
   // mixed-precision multi-shift: FP32 shifted solutions and search vectors
   int nms32 = 0;
@@ -228,10 +227,9 @@ Work *work(qexhip_links *l) {
 void free_work(Work *w) {
   if (!w) return;
   for (void *p : {(void *)w->y, (void *)w->rho, (void *)w->t64, (void *)w->r64, (void *)w->p64, (void *)w->Ap64,
-                  w->xs, w->rs, w->Aps, w->ps, w->ts, (void *)w->S, (void *)w->S1, (void *)w->d_x, (void *)w->d_p,
+                  w->xs, w->rs, w->Aps, w->ps, w->ts, (void *)w->S, (void *)w->S1, (void *)w->d_x,
                   (void *)w->T})
     if (p) (void)hipFree(p);
-  for (double *p : w->msp) (void)hipFree(p);
   for (void *p : w->msx32) (void)hipFree(p);
   for (void *p : w->msp32) (void)hipFree(p);
   if (w->d_x32) (void)hipFree(w->d_x32);
@@ -756,92 +754,11 @@ void multishift_true_r2(qexhip_links *l, int par, const double *const *xs, const
   }
 }
 
-void solve_ms_mixed(qexhip_links *l, int par, double *const *xs, const double *b, double m, const double *sigmas,
-                    int nshift, qexhip_solver_params *sp);
-
-// Multi-shift CG (src/solvers/cgm.nim:68-246), FP64, parity par.  With
-// sloppy storage requested (sp->sloppy != FP64) the mixed form below runs.
-void solve_ms(qexhip_links *l, int par, double *const *xs, const double *b, double m, const double *sigmas,
-              int nshift, qexhip_solver_params *sp) {
-  if (sp->sloppy != P64) {
-    solve_ms_mixed(l, par, xs, b, m, sigmas, nshift, sp);
-    return;
-  }
-  auto t0 = std::chrono::steady_clock::now();
-  Work *w = work(l);
-  qexhip_lattice *lat = l->lat;
-  const Geom &g = lat->g;
-  const size_t vb = vec64_bytes(lat);
-  require(nshift >= 1 && nshift <= QH_MAXSHIFT, "multishift: nshift out of range");
-  if (w->nms < nshift) {
-    for (int k = w->nms; k < nshift; k++) w->msp.push_back(dalloc<double>(vb));
-    if (w->d_x) (void)hipFree(w->d_x);
-    if (w->d_p) (void)hipFree(w->d_p);
-    w->d_x = dalloc<double *>(sizeof(double *) * nshift);
-    w->d_p = dalloc<double *>(sizeof(double *) * nshift);
-    w->nms = nshift;
-  }
-  if (!w->T) {
-    w->T = dalloc<ShiftScal>(sizeof(ShiftScal));
-    HIPCHK(hipHostMalloc((void **)&w->Th, sizeof(ShiftScal), hipHostMallocDefault));
-  }
-  std::vector<double *> xv(xs, xs + nshift);
-  HIPCHK(hipMemcpyAsync(w->d_x, xv.data(), sizeof(double *) * nshift, hipMemcpyHostToDevice, g_stream));
-  HIPCHK(hipMemcpyAsync(w->d_p, w->msp.data(), sizeof(double *) * nshift, hipMemcpyHostToDevice, g_stream));
-  Scal *S = w->S, *Sh = w->Sh;
-  ShiftScal *T = w->T, *Th = w->Th;
-  // r := b ; xs := 0 ; b2 = |b|^2 (cgm.nim:122-127)
-  HIPCHK(hipMemcpyAsync(w->r64, b, vb, hipMemcpyDeviceToDevice, g_stream));
-  for (int k = 0; k < nshift; k++) launch_fill(xs[k], vb, g_stream);
-  memset(Sh, 0, sizeof(Scal));
-  Sh->m2 = m * m;
-  push(S, Sh);
-  launch_norm2(P64, g, b, S, 1, lat->red, g_stream);
-  check_launch();
-  pull(Sh, S);
-  const double b2 = Sh->tmp[1];
-  memset(Th, 0, sizeof(ShiftScal));
-  Th->nshift = nshift;
-  Th->maxits = sp->maxits;
-  Th->r2stop = sp->r2req * b2;
-  Th->r2i = b2;
-  Th->alphaim1 = -1.0;
-  Th->betaim1 = 0.0;
-  for (int k = 0; k < nshift; k++) {
-    Th->sg[k] = (k == 0) ? 0.0 : sigmas[k];
-    Th->zi[k] = Th->zim1[k] = 1.0;
-    Th->active[k] = 1;
-  }
-  Th->continuing = (b2 > Th->r2stop) && sp->maxits > 0;
-  HIPCHK(hipMemcpyAsync(T, Th, sizeof(ShiftScal), hipMemcpyHostToDevice, g_stream));
-  Sh->done = !Th->continuing;
-  push(S, Sh);
-  for (int k = 0; k < nshift; k++)
-    HIPCHK(hipMemcpyAsync(w->msp[k], w->r64, vb, hipMemcpyDeviceToDevice, g_stream));  // ps[m] := r
-  while (Th->continuing) {
-    for (int c = 0; c < kChunk; c++) {
-      hop(l, P64, 18, 1 - par, 0, 0, w->msp[0], w->t64, nullptr, nullptr, 0, 0, 1.0, 0, 1, S);
-      hop(l, P64, 18, par, 1, 1, w->t64, w->Ap64, w->msp[0], nullptr, 1.0, 0, -1.0, 1, 1, S);
-      launch_cgm_alpha(T, S, g_stream);
-      launch_cgm_xr(g, nshift, P64, w->r64, w->Ap64, (void *const *)w->d_x, (void *const *)w->d_p, T, S, lat->red,
-                    g_stream);
-      launch_cgm_p(g, nshift, P64, w->r64, (void *const *)w->d_p, T, g_stream);
-    }
-    check_launch();
-    HIPCHK(hipMemcpyAsync(Th, T, sizeof(ShiftScal), hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipStreamSynchronize(g_stream));
-    if (!Th->continuing) break;
-  }
-  sp->iterations = Th->its;
-  sp->flops = flops_per_its(l) * Th->its + (double)(4 * g.Vh + 15) * Th->its * nshift;
-  multishift_true_r2(l, par, xs, b, b2, m, sigmas, nshift, sp);
-  sp->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-}
-
 // Mixed-precision multi-shift CG, no QEX counterpart (cgm.nim runs in FP64
 // only).  Two phases:
-//  1. the multi-shift recursion of solve_ms on FP32 storage with FP32 links
-//     (FP32 stencil arithmetic, FP64 updates and reductions) until the base
+//  1. the multi-shift recursion (cgm.nim:68-246: k_cgm_alpha, epi_cgm) on FP32
+//     storage with FP32 links (FP32 stencil arithmetic, FP64 updates and
+//     reductions), x_k and p_k updated every iteration, until the base
 //     residual |r|^2 <= max(r2req, g_ms_sloppy_r2) |b|^2; a shift whose
 //     residual zeta_k^2 |r|^2 met that bound stops being updated;
 //  2. every shift refined from its phase-1 solution to r2req by the defect
@@ -871,10 +788,7 @@ void solve_ms_mixed(qexhip_links *l, int par, double *const *xs, const double *b
   }
   if (w->nms < nshift) {  // the FP64 output pointer array
     if (w->d_x) (void)hipFree(w->d_x);
-    if (w->d_p) (void)hipFree(w->d_p);
-    for (int k = w->nms; k < nshift; k++) w->msp.push_back(dalloc<double>(vec64_bytes(lat)));
     w->d_x = dalloc<double *>(sizeof(double *) * nshift);
-    w->d_p = dalloc<double *>(sizeof(double *) * nshift);
     w->nms = nshift;
   }
   if (!w->T) {
@@ -919,8 +833,8 @@ void solve_ms_mixed(qexhip_links *l, int par, double *const *xs, const double *b
       hop(l, P32, 18, 1 - par, 0, 0, p0, w->ts, nullptr, nullptr, 0, 0, 1.0, 0, 1, S);
       hop(l, P32, 18, par, 1, 1, w->ts, w->Aps, p0, nullptr, 1.0, 0, -1.0, 1, 1, S);
       launch_cgm_alpha(T, S, g_stream);
-      launch_cgm_xr(g, nshift, P32, w->rs, w->Aps, w->d_x32, w->d_p32, T, S, lat->red, g_stream);
-      launch_cgm_p(g, nshift, P32, w->rs, w->d_p32, T, g_stream);
+      launch_cgm_xr32(g, nshift, w->rs, w->Aps, w->d_x32, w->d_p32, T, S, lat->red, g_stream);
+      launch_cgm_p32(g, nshift, w->rs, w->d_p32, T, g_stream);
     }
     check_launch();
     HIPCHK(hipMemcpyAsync(Th, T, sizeof(ShiftScal), hipMemcpyDeviceToHost, g_stream));
@@ -1209,21 +1123,27 @@ void solve_batch(int n, qexhip_links *const *ls, int par, qexhip_field *const *x
 struct MsbWs {
   qexhip_links *l = nullptr;
   int par = 0, nrhs = 0, nshift = 0;
-  std::vector<double *> x, p, r, Ap, t;
+  std::vector<double *> x, p, r, Ap, t, p0s;  // r: QH_MSB_RING vectors per right-hand side
   double **dx = nullptr, **dp = nullptr;
+  double *hist = nullptr, *coef = nullptr;
+  int *cinfo = nullptr;
   MsbScal *T = nullptr, *Th = nullptr, *Tx = nullptr;
   RedWs red{};
   std::vector<std::pair<int, int>> groups;  // (first rhs, size)
   int block = 0;
   long chunk = -1;   // site walk of the hops baked into the captured graph
+  int batches = 0;   // vector-load batches of the hops baked into the captured graph
   double m2 = -1.0;  // mass^2 baked into the captured graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~MsbWs() {
-    for (auto *v : {&x, &p, &r, &Ap, &t})
+    for (auto *v : {&x, &p, &r, &Ap, &t, &p0s})
       for (double *q : *v) (void)hipFree(q);
     (void)hipFree(dx);
     (void)hipFree(dp);
+    (void)hipFree(hist);
+    (void)hipFree(coef);
+    (void)hipFree(cinfo);
     (void)hipFree(T);
     (void)hipHostFree(Th);
     (void)hipFree(Tx);
@@ -1256,11 +1176,19 @@ MsbWs *get_msb(qexhip_links *l, int par, int nrhs, int nshift) {
     w->x.push_back(dalloc<double>(vb));
     w->p.push_back(dalloc<double>(vb));
   }
-  for (int j = 0; j < nrhs; j++)
-    for (auto *v : {&w->r, &w->Ap, &w->t}) {
+  for (int j = 0; j < nrhs; j++) {
+    for (auto *v : {&w->Ap, &w->t, &w->p0s}) {
       v->push_back(dalloc<double>(vb));
       launch_fill(v->back(), vb, g_stream);
     }
+    w->r.push_back(dalloc<double>(vb * QH_MSB_RING));
+    launch_fill(w->r.back(), vb * QH_MSB_RING, g_stream);
+  }
+  w->hist = dalloc<double>(sizeof(double) * nrhs * QH_MSB_WIN * nshift * 3);
+  w->coef = dalloc<double>(sizeof(double) * nrhs * nshift * QH_MSB_NCOEF);
+  w->cinfo = dalloc<int>(sizeof(int) * nrhs * (2 + QH_MAXSHIFT));
+  launch_fill(w->hist, sizeof(double) * nrhs * QH_MSB_WIN * nshift * 3, g_stream);
+  launch_fill(w->cinfo, sizeof(int) * nrhs * (2 + QH_MAXSHIFT), g_stream);
   w->dx = dalloc<double *>(sizeof(double *) * nrhs * nshift);
   w->dp = dalloc<double *>(sizeof(double *) * nrhs * nshift);
   HIPCHK(hipMemcpyAsync(w->dx, w->x.data(), sizeof(double *) * nrhs * nshift, hipMemcpyHostToDevice, g_stream));
@@ -1268,7 +1196,9 @@ MsbWs *get_msb(qexhip_links *l, int par, int nrhs, int nshift) {
   w->T = dalloc<MsbScal>(sizeof(MsbScal) * nrhs);
   w->Tx = dalloc<MsbScal>(sizeof(MsbScal) * nrhs);
   HIPCHK(hipHostMalloc((void **)&w->Th, sizeof(MsbScal) * nrhs, hipHostMallocDefault));
-  w->red.maxblocks = (int)((l->lat->g.Vh + 63) / 64) + 64;
+  // one partial-sum slot per workgroup of the multi-RHS hops at any block size
+  // and site walk (mrhs_grid: below 2 Vh/64 + 8)
+  w->red.maxblocks = 2 * (int)((l->lat->g.Vh + 63) / 64) + 64;
   w->red.partials = dalloc<double>(sizeof(double) * 3 * QH_MAXRHS * w->red.maxblocks);
   for (int j = 0; j < nrhs;) {
     int g = 8;
@@ -1288,19 +1218,31 @@ MsbPtrs msb_ptrs(MsbWs *w, int j0, int nr) {
     P.t[j] = w->t[j0 + j];
     P.Ap[j] = w->Ap[j0 + j];
     P.r[j] = w->r[j0 + j];
+    P.p0s[j] = w->p0s[j0 + j];
   }
+  P.rstride = (long)(vec64_bytes(w->l->lat) / sizeof(double));
   P.x = w->dx + j0 * w->nshift;
   P.p = w->dp + j0 * w->nshift;
+  P.hist = w->hist + (size_t)j0 * QH_MSB_WIN * w->nshift * 3;
+  P.coef = w->coef + (size_t)j0 * w->nshift * QH_MSB_NCOEF;
+  P.cinfo = w->cinfo + j0 * (2 + QH_MAXSHIFT);
   return P;
 }
 
-void msb_iter(MsbWs *w, MsbScal *T, double m2, int block, long chunk) {
+// end of a window of iterations: every group's shifted x_k, p_k brought up to date
+void msb_flush(MsbWs *w, MsbScal *T) {
+  for (auto &gr : w->groups)
+    launch_msb_flush(w->l->lat->g, gr.second, w->nshift, msb_ptrs(w, gr.first, gr.second), T + gr.first, g_stream);
+}
+
+void msb_iter(MsbWs *w, MsbScal *T, double m2, int block, long chunk, int batches) {
   qexhip_links *l = w->l;
+  require(mrhs_grid(l->lat->g, block, chunk) <= (unsigned)w->red.maxblocks, "multi-shift batch: hop grid too large");
   const char *links = links_store(l, P64, 18);
   const long lkblk = lkblk_bytes(l, P64, 18);
   for (auto &gr : w->groups)
     launch_msb_iter(l->lat->g, links, lkblk, l->nl, w->par, gr.second, w->nshift, m2,
-                    msb_ptrs(w, gr.first, gr.second), T + gr.first, w->red, block, chunk, g_stream);
+                    msb_ptrs(w, gr.first, gr.second), T + gr.first, w->red, block, chunk, batches, g_stream);
 }
 
 void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexhip_field *const *bf, double m,
@@ -1314,6 +1256,7 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
   // launch shape: block and site walk of the hops (a pinned walk fixes the latter)
   int block = g_block_forced ? g_block : 256;
   long chunk = g_msb_walk > 0 ? g_msb_walk : 0;
+  int batches = 1;
   if ((flags & QEXHIP_BATCH_TUNE) && !g_block_forced) {
     const std::string key = "msb nl=" + std::to_string(l->nl) + " nrhs=" + std::to_string(nrhs) +
                             " nshift=" + std::to_string(nshift) + " walk=" + std::to_string(g_msb_walk) + " " +
@@ -1322,6 +1265,7 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
     if (it != g_tune.end()) {
       block = it->second.block;
       chunk = it->second.chunk;
+      batches = it->second.batches;
     } else {
       std::vector<MsbScal> h(nrhs);
       memset(h.data(), 0, sizeof(MsbScal) * nrhs);
@@ -1340,24 +1284,29 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
       Tuned best{256, 1, 1e300};
       const long S = lat->g.Vh / lat->g.L[3];
       for (int b : {64, 128, 256})
-        for (long c : {0L, 512L, 1024L, 2048L}) {
-          if (g_msb_walk >= 0 && c != (g_msb_walk > 0 ? g_msb_walk : 0)) continue;
-          if (c && (c % b || c >= S)) continue;
-          const double us = time_us([&] { msb_iter(w, w->Tx, m2, b, c); }, 1, 4);
-          if (us < best.us) {
-            best = Tuned{b, 1, us};
-            best.chunk = c;
+        for (long c : {0L, 1024L})
+          for (int nb : {1, 2, 4}) {
+            if (g_msb_walk >= 0 && c != 0) continue;  // a pinned walk: one candidate per block and batches
+            const long cw = g_msb_walk >= 0 ? g_msb_walk : c;
+            if (cw && (cw % b || cw >= S)) continue;
+            const double us = time_us([&] { msb_iter(w, w->Tx, m2, b, cw, nb); }, 1, 4);
+            if (us < best.us) {
+              best = Tuned{b, 1, us};
+              best.chunk = cw;
+              best.batches = nb;
+            }
           }
-        }
       g_tune[key] = best;
       block = best.block;
       chunk = best.chunk;
+      batches = best.batches;
     }
   }
   const bool graph = (flags & QEXHIP_BATCH_GRAPH) != 0;
-  if (block != w->block || m2 != w->m2 || chunk != w->chunk) {
+  if (block != w->block || m2 != w->m2 || chunk != w->chunk || batches != w->batches) {
     w->block = block;
     w->chunk = chunk;
+    w->batches = batches;
     w->m2 = m2;
     if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
     if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
@@ -1391,9 +1340,13 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
     }
   }
   HIPCHK(hipMemcpyAsync(w->T, w->Th, sizeof(MsbScal) * nrhs, hipMemcpyHostToDevice, g_stream));
-  if (graph && !w->exec) {
-    capture([&] { for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block, chunk); }, &w->graph, &w->exec);
-  }
+  // one chunk = one window of the deferred shift updates: kChunk iterations, then the flush
+  static_assert(kChunk <= QH_MSB_WIN, "a chunk must fit one window of the multi-shift updates");
+  auto msb_chunk = [&] {
+    for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block, chunk, batches);
+    msb_flush(w, w->T);
+  };
+  if (graph && !w->exec) capture(msb_chunk, &w->graph, &w->exec);
   for (;;) {
     bool any = false;
     for (int j = 0; j < nrhs; j++) any = any || w->Th[j].continuing;
@@ -1401,12 +1354,13 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
     if (graph) {
       HIPCHK(hipGraphLaunch(w->exec, g_stream));
     } else {
-      for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block, chunk);
+      msb_chunk();
     }
     check_launch();
     HIPCHK(hipMemcpyAsync(w->Th, w->T, sizeof(MsbScal) * nrhs, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
   }
+  for (int j = 0; j < nrhs; j++) require(!w->Th[j].winflow, "multi-shift batch: update window overflow");
   for (int j = 0; j < nrhs; j++)
     for (int k = 0; k < nshift; k++)
       HIPCHK(hipMemcpyAsync(xs[j * nshift + k]->par(par), w->x[j * nshift + k], vb, hipMemcpyDeviceToDevice,
@@ -1990,7 +1944,16 @@ int qexhip_solve_multishift(qexhip_links *l, int par, qexhip_field *const *xs, c
     require(xs[k]->prec == P64, "multishift fields must be FP64");
     xp[k] = (double *)xs[k]->par(par);
   }
-  solve_ms(l, par, xp.data(), (const double *)b->par(par), m, sigmas, nshift, sp);
+  require(nshift >= 1 && nshift <= QH_MAXSHIFT, "multishift: nshift out of range");
+  if (sp->sloppy == P64) {
+    // cgm.nim's FP64 recursion as the batched solve with one right-hand side:
+    // the shift updates are deferred to one pass per window of iterations, and
+    // every shift is updated until the base system converges, as cgm.nim does
+    qexhip_field *bp = const_cast<qexhip_field *>(b);
+    solve_msb(l, par, 1, xs, &bp, m, sigmas, nshift, sp, QEXHIP_BATCH_GRAPH | QEXHIP_BATCH_TUNE | QEXHIP_BATCH_NOFREEZE);
+  } else {
+    solve_ms_mixed(l, par, xp.data(), (const double *)b->par(par), m, sigmas, nshift, sp);
+  }
   API_END
 }
 

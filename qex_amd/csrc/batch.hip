@@ -136,21 +136,18 @@ void launch_bcg_flush(const BatchItem *T, int n, long nblocks, int block, hipStr
 // the t +-1, +-3 neighbours of the 8 gathered vectors are read while the
 // same chunk's nearby t-slices are in flight); a workgroup may serve several
 // logical blocks and writes one set of partial sums.
-template <int NL, int NR, int MODE>
+template <int NL, int NR, int MODE, int NB>
 __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, long lkblk, int q, double m4,
                                                  MsbPtrs P, const MsbScal *T, RedWs red, long chunk) {
+  static_assert(NR % NB == 0, "NB must divide NR");
   __shared__ double lds[16 * 3 * NR];
   const long spb = blockDim.x;
   const Walk wk = make_walk(g, spb, chunk);
+  // one logical block per workgroup (the launch has a partial-sum slot for each)
   const BlockIter it = block_iter(wk.nblk, 1);
-  double sums[3 * NR];
-#pragma unroll
-  for (int k = 0; k < 3 * NR; k++) sums[k] = 0.0;
-#pragma unroll 1
-  for (long lb = it.cur; lb < it.end; lb += it.step) {
-    const long i = walk_site(wk, lb, spb, threadIdx.x);
-    if (i < 0 || i >= g.Vh) continue;
-    double2 acc[NR][3];
+  const long i = it.cur < it.end ? walk_site(wk, it.cur, spb, threadIdx.x) : -1;
+  double2 acc[NR][3];
+  if (i >= 0 && i < g.Vh) {
 #pragma unroll
     for (int r = 0; r < NR; r++)
 #pragma unroll
@@ -165,28 +162,36 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
       double2 U[9];
       ld_link<P64, 18, true, double2, link_blocked<P64, NL>()>(links + (long)(2 * k + (fwd ? q : 1 - q)) * lkblk,
                                                               g.stride, fwd ? i : n, U, 1.0);
-      double2 v[NR][3];
+      // the right-hand sides' vectors in batches of NB, each batch's loads
+      // issued before its arithmetic
 #pragma unroll
-      for (int r = 0; r < NR; r++)
+      for (int r0 = 0; r0 < NR; r0 += NB) {
+        double2 v[NB][3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[r][c] = ldc<P64, double2>(MODE == 0 ? (const void *)P.p0[r] : P.t[r], vix(g, c, n));
-      if (fwd) {
+        for (int b = 0; b < NB; b++)
 #pragma unroll
-        for (int r = 0; r < NR; r++)
+          for (int c = 0; c < 3; c++)
+            v[b][c] = ldc<P64, double2>(MODE == 0 ? (const void *)P.p0[r0 + b] : P.t[r0 + b], vix(g, c, n));
+        if (fwd) {
 #pragma unroll
-          for (int a = 0; a < 3; a++)
+          for (int b = 0; b < NB; b++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) cmac(acc[r][a], U[3 * a + c], v[r][c]);
-      } else {
+            for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int r = 0; r < NR; r++)
+              for (int c = 0; c < 3; c++) cmac(acc[r0 + b][a], U[3 * a + c], v[b][c]);
+        } else {
 #pragma unroll
-          for (int a = 0; a < 3; a++)
+          for (int b = 0; b < NB; b++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) cjmsub(acc[r][a], U[3 * c + a], v[r][c]);
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+              for (int c = 0; c < 3; c++) cjmsub(acc[r0 + b][a], U[3 * c + a], v[b][c]);
+        }
       }
     }
-    if constexpr (MODE == 0) {
+  }
+  if constexpr (MODE == 0) {
+    if (i >= 0 && i < g.Vh) {
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         if (!T[r].continuing) continue;
@@ -194,11 +199,17 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
 #pragma unroll
         for (int c = 0; c < 3; c++) dst[vix(g, c, i)] = acc[r][c];
       }
-    } else {
+    }
+  } else {
+    double sums[3 * NR];
+#pragma unroll
+    for (int k = 0; k < 3 * NR; k++) sums[k] = 0.0;
+    if (i >= 0 && i < g.Vh) {
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         if (!T[r].continuing) continue;
-        const double2 *pp = (const double2 *)P.p0[r], *rr = (const double2 *)P.r[r];
+        const double2 *pp = (const double2 *)P.p0[r];
+        const double2 *rr = (const double2 *)(P.r[r] + (long)(T[r].its % QH_MSB_RING) * P.rstride);
         double2 *dst = (double2 *)P.Ap[r];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -212,8 +223,8 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
         }
       }
     }
+    block_partials<3 * NR>(sums, lds, red);
   }
-  if constexpr (MODE == 1) block_partials<3 * NR>(sums, lds, red);
 }
 
 // Multi-RHS stagD2 (QEX's stagDPN/stagDN, stagD.nim:132-199,451-456): on parity
@@ -346,7 +357,7 @@ bool launch_hop_multi(int prec, int nl, int nr, int batches, const Geom &g, cons
 // (right-hand side, shift): every thread reads the old per-source scalars,
 // then (after a barrier) shift 0's thread writes the new ones.
 template <int NR>
-__global__ void __launch_bounds__(1024) k_msb_fin1(RedWs red, int nb, MsbScal *T) {
+__global__ void __launch_bounds__(1024) k_msb_fin1(RedWs red, int nb, MsbScal *T, double *hist) {
   __shared__ double lds[16 * 3 * NR];
   __shared__ double tot[3 * NR];
   double s[3 * NR];
@@ -365,7 +376,9 @@ __global__ void __launch_bounds__(1024) k_msb_fin1(RedWs red, int nb, MsbScal *T
   MsbScal *t = T + (mine ? r : 0);
   const bool cont = mine && t->continuing && k < t->nshift;
   double alpha = 0.0, beta = 0.0, aim1 = 0.0, bim1 = 0.0;
+  int jw = 0;  // this iteration's index in the window
   if (cont) {
+    jw = t->its - t->its0;
     const double pAp = tot[3 * r], rAp = tot[3 * r + 1], ApAp = tot[3 * r + 2];
     alpha = (pAp != 0.0) ? t->r2i / pAp : 0.0;
     double r2n = t->r2i - 2.0 * alpha * rAp + alpha * alpha * ApAp;
@@ -376,6 +389,9 @@ __global__ void __launch_bounds__(1024) k_msb_fin1(RedWs red, int nb, MsbScal *T
   }
   __syncthreads();
   if (!cont) return;
+  // (xc, pc1, pc2) of this iteration: x_k += xc p_k ; p_k = pc1 r_new + pc2 p_k ; a
+  // frozen shift records the identity (0, 0, 1)
+  double xc = 0.0, c1 = 0.0, c2 = 1.0;
   if (k == 0) {
     t->alpha = alpha;
     t->beta = beta;
@@ -385,6 +401,9 @@ __global__ void __launch_bounds__(1024) k_msb_fin1(RedWs red, int nb, MsbScal *T
     t->alphaim1 = alpha;
     t->betaim1 = beta;
     t->its += 1;
+    xc = alpha;
+    c1 = 1.0;
+    c2 = beta;
   } else if (t->active[k]) {
     double zip1d = alpha * bim1 * (t->zim1[k] - t->zi[k]);
     zip1d += t->zim1[k] * aim1 * (1.0 + t->sg[k] * alpha);
@@ -395,13 +414,26 @@ __global__ void __launch_bounds__(1024) k_msb_fin1(RedWs red, int nb, MsbScal *T
     t->pc2[k] = beta * zr * zr;
     t->zim1[k] = t->zi[k];
     t->zi[k] = zip1;
+    xc = t->xc[k];
+    c1 = zip1;
+    c2 = t->pc2[k];
+  }
+  if (jw < QH_MSB_WIN) {
+    double *h = hist + (((long)r * QH_MSB_WIN + jw) * t->nshift + k) * 3;
+    h[0] = xc;
+    h[1] = c1;
+    h[2] = c2;
+  } else if (k == 0) {
+    t->winflow = 1;
   }
 }
 
-// r_j -= alpha_j Ap_j ; for every active shift x_jk += xc p_jk, p_jk = pc1 r_j + pc2 p_jk ;
-// partial sums of the exact |r_j|^2.
+// One iteration's vector update of every right-hand side:
+//   r_new = r - alpha Ap into the next ring slot, p0 = r_new + beta p0 (p0 saved
+//   at the first iteration of a window), partial sums of the exact |r_new|^2.
+// The shifted x_k, p_k (and x_0) wait for the end of the window (k_msb_flush).
 template <int NR>
-__global__ void __launch_bounds__(256) k_msb_update(Geom g, int nshift, MsbPtrs P, const MsbScal *T, RedWs red) {
+__global__ void __launch_bounds__(256) k_msb_step(Geom g, MsbPtrs P, const MsbScal *T, RedWs red) {
   __shared__ double lds[16 * NR];
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double rr[NR];
@@ -412,37 +444,136 @@ __global__ void __launch_bounds__(256) k_msb_update(Geom g, int nshift, MsbPtrs 
     for (int r = 0; r < NR; r++) {
       const MsbScal *t = T + r;
       if (!t->continuing) continue;
-      const double alpha = t->alpha;
-      double2 rn[3];
-      double2 *rv = (double2 *)P.r[r];
+      const int it = t->its - 1;  // this iteration (fin1 counted it already)
+      const double alpha = t->alpha, beta = t->beta;
+      const bool first = (it == t->its0);
+      const double2 *rv = (const double2 *)(P.r[r] + (long)(it % QH_MSB_RING) * P.rstride);
+      double2 *rw = (double2 *)(P.r[r] + (long)((it + 1) % QH_MSB_RING) * P.rstride);
       const double2 *av = (const double2 *)P.Ap[r];
+      double2 *p0 = (double2 *)P.p0[r], *p0s = (double2 *)P.p0s[r];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const long o = vix(g, c, i);
-        const double2 a = av[o], x = rv[o];
-        rn[c] = make_double2(x.x - alpha * a.x, x.y - alpha * a.y);
-        rv[o] = rn[c];
-        rr[r] += rn[c].x * rn[c].x + rn[c].y * rn[c].y;
-      }
-#pragma unroll 1
-      for (int k = 0; k < nshift; k++) {
-        if (!t->active[k]) continue;
-        const double xc = t->xc[k], c1 = t->pc1[k], c2 = t->pc2[k];
-        double2 *xk = (double2 *)P.x[r * nshift + k], *pk = (double2 *)P.p[r * nshift + k];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          const long o = vix(g, c, i);
-          double2 xv = xk[o];
-          const double2 pv = pk[o];
-          xv.x += xc * pv.x;
-          xv.y += xc * pv.y;
-          xk[o] = xv;
-          pk[o] = make_double2(c1 * rn[c].x + c2 * pv.x, c1 * rn[c].y + c2 * pv.y);
-        }
+        const double2 a = av[o], x = rv[o], pv = p0[o];
+        const double2 rn = make_double2(x.x - alpha * a.x, x.y - alpha * a.y);
+        rw[o] = rn;
+        rr[r] += rn.x * rn.x + rn.y * rn.y;
+        if (first) p0s[o] = pv;
+        p0[o] = make_double2(rn.x + beta * pv.x, rn.y + beta * pv.y);
       }
     }
   }
   block_partials<NR>(rr, lds, red);
+}
+
+// Window coefficients, one thread per (right-hand side, shift).  Over the w
+// iterations j = 0..w-1 of a window starting at x, p with residuals
+// r_1..r_w after each iteration:
+//   p_w = C p + sum_{i=1..w} D_i r_i ,  x_w = x + E p + sum_{i=1..w-1} F_i r_i
+// with H_j = prod_{l=j..w-1} pc2_l, G_j = xc_j + pc2_j G_{j+1} (G_w = 0):
+// C = H_0, E = G_0, D_i = pc1_{i-1} H_i, F_i = pc1_{i-1} G_i.  The window then
+// restarts at the current iteration.
+template <int NR>
+__global__ void __launch_bounds__(NR * QH_MAXSHIFT) k_msb_coef(MsbPtrs P, MsbScal *T) {
+  const int r = threadIdx.x / QH_MAXSHIFT, k = threadIdx.x % QH_MAXSHIFT;
+  MsbScal *t = T + r;
+  const int w = t->its - t->its0, its0 = t->its0, ns = t->nshift;
+  __syncthreads();
+  int *ci = P.cinfo + r * (2 + QH_MAXSHIFT);
+  if (k == 0) {
+    ci[0] = w < QH_MSB_WIN ? w : QH_MSB_WIN;
+    ci[1] = its0 % QH_MSB_RING;
+    t->its0 = t->its;
+  }
+  if (k >= ns) return;
+  const int wl = w < QH_MSB_WIN ? w : QH_MSB_WIN;
+  const double *h = P.hist + (long)r * QH_MSB_WIN * ns * 3;
+  double *cf = P.coef + ((long)r * ns + k) * QH_MSB_NCOEF;
+  double G = 0.0, H = 1.0;
+  bool any = false;
+  for (int j = wl; j >= 0; j--) {
+    if (j < wl) {
+      const double xc = h[((long)j * ns + k) * 3], c2 = h[((long)j * ns + k) * 3 + 2];
+      any = any || xc != 0.0 || h[((long)j * ns + k) * 3 + 1] != 0.0 || c2 != 1.0;
+      G = xc + c2 * G;
+      H = c2 * H;
+    }
+    if (j >= 1) {
+      const double c1 = h[((long)(j - 1) * ns + k) * 3 + 1];
+      cf[2 + j - 1] = c1 * G;                // F_j (F_w unused)
+      cf[2 + QH_MSB_WIN + j - 1] = c1 * H;   // D_j
+    } else {
+      cf[0] = G;  // E
+      cf[1] = H;  // C
+    }
+  }
+  ci[2 + k] = any ? 1 : 0;
+}
+
+// x_k, p_k of every shift advanced over the window (x_0 only: p_0 was kept up
+// to date every iteration; its window-start copy p0s stands for p).  One color
+// component at a time, so that the accumulators of up to QH_MSB_FLUSH_K shifts
+// (4 doubles each) stay in registers and each residual of the window is read
+// once per component for all of them.
+#define QH_MSB_FLUSH_K 16
+template <int NR>
+__global__ void __launch_bounds__(256) k_msb_flush(Geom g, int nshift, MsbPtrs P) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.Vh) return;
+#pragma unroll 1
+  for (int r = 0; r < NR; r++) {
+    const int *ci = P.cinfo + r * (2 + QH_MAXSHIFT);
+    const int w = ci[0], base = ci[1];
+    if (w == 0) continue;
+    const double *ring = P.r[r];
+#pragma unroll 1
+    for (int k0 = 0; k0 < nshift; k0 += QH_MSB_FLUSH_K) {
+      bool act[QH_MSB_FLUSH_K];
+      bool anyg = false;
+#pragma unroll
+      for (int q = 0; q < QH_MSB_FLUSH_K; q++) {
+        act[q] = (k0 + q < nshift) && ci[2 + k0 + q];
+        anyg = anyg || act[q];
+      }
+      if (!anyg) continue;
+#pragma unroll 1
+      for (int c = 0; c < 3; c++) {
+        const long o = vix(g, c, i);
+        double2 ax[QH_MSB_FLUSH_K], ap[QH_MSB_FLUSH_K];
+#pragma unroll
+        for (int q = 0; q < QH_MSB_FLUSH_K; q++) {
+          if (!act[q]) continue;
+          const int k = k0 + q;
+          const double *cf = P.coef + ((long)r * nshift + k) * QH_MSB_NCOEF;
+          const double2 xv = ((const double2 *)P.x[r * nshift + k])[o];
+          const double2 pv = ((const double2 *)(k == 0 ? P.p0s[r] : P.p[r * nshift + k]))[o];
+          ax[q] = make_double2(xv.x + cf[0] * pv.x, xv.y + cf[0] * pv.y);
+          ap[q] = make_double2(cf[1] * pv.x, cf[1] * pv.y);
+        }
+#pragma unroll 1
+        for (int s = 1; s <= w; s++) {
+          const double2 rv = ((const double2 *)(ring + (long)((base + s) % QH_MSB_RING) * P.rstride))[o];
+#pragma unroll
+          for (int q = 0; q < QH_MSB_FLUSH_K; q++) {
+            if (!act[q]) continue;
+            const double *cf = P.coef + ((long)r * nshift + k0 + q) * QH_MSB_NCOEF;
+            const double F = (s < w) ? cf[2 + s - 1] : 0.0, D = cf[2 + QH_MSB_WIN + s - 1];
+            ax[q].x += F * rv.x;
+            ax[q].y += F * rv.y;
+            ap[q].x += D * rv.x;
+            ap[q].y += D * rv.y;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < QH_MSB_FLUSH_K; q++) {
+          if (!act[q]) continue;
+          const int k = k0 + q;
+          ((double2 *)P.x[r * nshift + k])[o] = ax[q];
+          if (k > 0) ((double2 *)P.p[r * nshift + k])[o] = ap[q];
+        }
+      }
+    }
+  }
 }
 
 // exact |r_j|^2: next alpha, stopping test, shift freezing (cgm.nim:176-186);
@@ -482,43 +613,74 @@ __global__ void __launch_bounds__(1024) k_msb_fin2(RedWs red, int nb, MsbScal *T
   }
 }
 
-// Workgroups of the multi-RHS hops for chunk (see k_mrhs_hop): one per
-// logical block, capped by the partial-sum slots of red (the hop then runs
-// persistent).
-unsigned mrhs_grid(const Geom &g, int block, long chunk, const RedWs &red) {
+// Workgroups of the multi-RHS hops for chunk (see k_mrhs_hop): one per logical
+// block, rounded up to 8 for the XCD order.  Fewer than 2 Vh/block + 8: a
+// walk pads each t-slice to whole chunks, less than one t-slice's blocks per t.
+unsigned mrhs_grid(const Geom &g, int block, long chunk) {
   const Walk wk = make_walk(g, block, chunk);
-  return grid_blocks(wk.nblk, 1, red.maxblocks & ~7);
+  return grid_blocks(wk.nblk, 1, 0);
+}
+
+template <int NL, int NR, int NB>
+static void msb_hops_t(const Geom &g, const char *links, long lkblk, int par, double m2, const MsbPtrs &P,
+                       const MsbScal *T, RedWs red, dim3 grid, dim3 blk, long chunk, hipStream_t st) {
+  hipLaunchKernelGGL((k_mrhs_hop<NL, NR, 0, NB>), grid, blk, 0, st, g, links, lkblk, 1 - par, 0.0, P, T, red, chunk);
+  hipLaunchKernelGGL((k_mrhs_hop<NL, NR, 1, NB>), grid, blk, 0, st, g, links, lkblk, par, 4.0 * m2, P, T, red, chunk);
 }
 
 template <int NL, int NR>
 static void msb_iter_t(const Geom &g, const char *links, long lkblk, int par, int nshift, double m2,
-                       const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, hipStream_t st) {
-  const unsigned nb = mrhs_grid(g, block, chunk, red);
+                       const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, int batches, hipStream_t st) {
+  const unsigned nb = mrhs_grid(g, block, chunk);
+  if (nb > (unsigned)red.maxblocks) {
+    fprintf(stderr, "qexhip: multi-RHS hop grid %u above the %d partial-sum slots\n", nb, red.maxblocks);
+    return;
+  }
   const dim3 grid(nb), blk(block);
-  hipLaunchKernelGGL((k_mrhs_hop<NL, NR, 0>), grid, blk, 0, st, g, links, lkblk, 1 - par, 0.0, P, T, red, chunk);
-  hipLaunchKernelGGL((k_mrhs_hop<NL, NR, 1>), grid, blk, 0, st, g, links, lkblk, par, 4.0 * m2, P, T, red, chunk);
-  hipLaunchKernelGGL((k_msb_fin1<NR>), dim3(1), dim3(1024), 0, st, red, (int)nb, T);
-  hipLaunchKernelGGL((k_msb_update<NR>), dim3((unsigned)((g.Vh + 255) / 256)), dim3(256), 0, st, g, nshift, P, T,
-                     red);
+  if (batches >= 4 && NR >= 4)
+    msb_hops_t<NL, NR, (NR >= 4 ? NR / 4 : 1)>(g, links, lkblk, par, m2, P, T, red, grid, blk, chunk, st);
+  else if (batches >= 2 && NR >= 2)
+    msb_hops_t<NL, NR, (NR >= 2 ? NR / 2 : 1)>(g, links, lkblk, par, m2, P, T, red, grid, blk, chunk, st);
+  else
+    msb_hops_t<NL, NR, NR>(g, links, lkblk, par, m2, P, T, red, grid, blk, chunk, st);
+  hipLaunchKernelGGL((k_msb_fin1<NR>), dim3(1), dim3(1024), 0, st, red, (int)nb, T, P.hist);
+  hipLaunchKernelGGL((k_msb_step<NR>), dim3((unsigned)((g.Vh + 255) / 256)), dim3(256), 0, st, g, P, T, red);
   hipLaunchKernelGGL((k_msb_fin2<NR>), dim3(1), dim3(1024), 0, st, red, (int)((g.Vh + 255) / 256), T);
 }
 
 template <int NL>
 static bool msb_nr(int nr, const Geom &g, const char *links, long lkblk, int par, int nshift, double m2,
-                   const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, hipStream_t st) {
+                   const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, int batches, hipStream_t st) {
   switch (nr) {
-    case 1: msb_iter_t<NL, 1>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, st); return true;
-    case 2: msb_iter_t<NL, 2>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, st); return true;
-    case 4: msb_iter_t<NL, 4>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, st); return true;
-    case 8: msb_iter_t<NL, 8>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, st); return true;
+    case 1: msb_iter_t<NL, 1>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st); return true;
+    case 2: msb_iter_t<NL, 2>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st); return true;
+    case 4: msb_iter_t<NL, 4>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st); return true;
+    case 8: msb_iter_t<NL, 8>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st); return true;
   }
   return false;
 }
 
+template <int NR>
+static void msb_flush_t(const Geom &g, int nshift, const MsbPtrs &P, MsbScal *T, hipStream_t st) {
+  hipLaunchKernelGGL((k_msb_coef<NR>), dim3(1), dim3(NR * QH_MAXSHIFT), 0, st, P, T);
+  hipLaunchKernelGGL((k_msb_flush<NR>), dim3((unsigned)((g.Vh + 255) / 256)), dim3(256), 0, st, g, nshift, P);
+}
+
+void launch_msb_flush(const Geom &g, int nr, int nshift, const MsbPtrs &P, MsbScal *T, hipStream_t st) {
+  switch (nr) {
+    case 1: msb_flush_t<1>(g, nshift, P, T, st); break;
+    case 2: msb_flush_t<2>(g, nshift, P, T, st); break;
+    case 4: msb_flush_t<4>(g, nshift, P, T, st); break;
+    case 8: msb_flush_t<8>(g, nshift, P, T, st); break;
+    default: fprintf(stderr, "qexhip: no multi-shift flush for nr=%d\n", nr);
+  }
+}
+
 void launch_msb_iter(const Geom &g, const char *links64, long lkblk, int nl, int par, int nr, int nshift,
-                     double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, hipStream_t st) {
-  const bool ok = (nl == 8) ? msb_nr<8>(nr, g, links64, lkblk, par, nshift, m2, P, T, red, block, chunk, st)
-                            : msb_nr<4>(nr, g, links64, lkblk, par, nshift, m2, P, T, red, block, chunk, st);
+                     double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, int batches,
+                     hipStream_t st) {
+  const bool ok = (nl == 8) ? msb_nr<8>(nr, g, links64, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st)
+                            : msb_nr<4>(nr, g, links64, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st);
   if (!ok) fprintf(stderr, "qexhip: no multi-RHS instantiation for nr=%d nl=%d\n", nr, nl);
 }
 

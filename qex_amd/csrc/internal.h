@@ -112,10 +112,18 @@ struct BatchItem {
 
 // One right-hand side of the batched multi-shift CG (batch.hip); shift 0 is
 // the base system A = 4(m^2 - D_eo D_oe), shift k > 0 adds sg[k].
+// The batched multi-shift solve advances every shifted x_k, p_k once per
+// window of QH_MSB_WIN iterations (one graph chunk), from the residuals of the
+// window kept in a ring of QH_MSB_RING vectors per right-hand side.
+#define QH_MSB_WIN 16
+#define QH_MSB_RING (QH_MSB_WIN + 1)
+#define QH_MSB_NCOEF (2 + 2 * QH_MSB_WIN)
 struct MsbScal {
   double r2i, r2stop, alpha, beta, alphaim1, betaim1;
   int its, maxits, continuing, nshift;
   int freeze;                  // stop updating shifts whose residual met r2stop
+  int its0;                    // first iteration of the current window
+  int winflow;                 // set if a window outgrew QH_MSB_WIN (never, with flushes every chunk)
   int pad;
   double sg[QH_MAXSHIFT], zi[QH_MAXSHIFT], zim1[QH_MAXSHIFT];
   double xc[QH_MAXSHIFT];      // x_k += xc[k] p_k (old p_k)
@@ -125,11 +133,18 @@ struct MsbScal {
 };
 #define QH_MAXRHS 8
 struct MsbPtrs {              // one group of nr right-hand sides
-  const double *p0[QH_MAXRHS];  // base search vectors (own parity)
+  double *p0[QH_MAXRHS];        // base search vectors (own parity), updated every iteration
   double *t[QH_MAXRHS];         // (2D) p0 on the other parity
   double *Ap[QH_MAXRHS];
-  double *r[QH_MAXRHS];
+  double *r[QH_MAXRHS];         // residual ring: QH_MSB_RING vectors of rstride doubles; r after
+                                // iteration i in slot i % QH_MSB_RING
+  double *p0s[QH_MAXRHS];       // p0 at the start of the window
+  long rstride;
   double **x, **p;              // device arrays [nr*nshift]: shifted solutions and search vectors
+  double *hist;                 // [nr][QH_MSB_WIN][nshift][3]: per iteration of the window and
+                                // shift, (xc, pc1, pc2) of x_k += xc p_k, p_k = pc1 r + pc2 p_k
+  double *coef;                 // [nr][nshift][QH_MSB_NCOEF]: window coefficients (k_msb_coef)
+  int *cinfo;                   // [nr][2 + QH_MAXSHIFT]: window length, first slot, shift activity
 };
 
 struct MultiPtrs {     // one group of up to QH_MAXRHS fields of the multi-RHS stagD2
@@ -174,11 +189,11 @@ void launch_redot(int prec, const Geom &g, const void *x, const void *y, Scal *S
 void launch_rs_from_true(const Geom &g, void *rs, const double *rtrue, Scal *S, int pr, hipStream_t st);
 void launch_xfold(const Geom &g, double *x, void *xs, Scal *S, int px, hipStream_t st);
 void launch_scal_init(Scal *S, hipStream_t st, int mode);
-// multi-shift updates on FP64 (prec P64) or FP32 (P32) storage; x, p: device
-// arrays of nshift vector pointers
-void launch_cgm_xr(const Geom &g, int nshift, int prec, void *r, const void *Ap, void *const *x, void *const *p,
-                   ShiftScal *T, Scal *S, RedWs red, hipStream_t st);
-void launch_cgm_p(const Geom &g, int nshift, int prec, const void *r, void *const *p, ShiftScal *T, hipStream_t st);
+// the multi-shift updates of the mixed solve's FP32 recursion (FP64 arithmetic);
+// x, p: device arrays of nshift vector pointers
+void launch_cgm_xr32(const Geom &g, int nshift, void *r, const void *Ap, void *const *x, void *const *p,
+                     ShiftScal *T, Scal *S, RedWs red, hipStream_t st);
+void launch_cgm_p32(const Geom &g, int nshift, const void *r, void *const *p, ShiftScal *T, hipStream_t st);
 void launch_cgm_start32(const Geom &g, int nshift, const double *b, void *r, void *const *x, void *const *p,
                         hipStream_t st);
 void launch_cgm_export32(const Geom &g, int nshift, void *const *x32, double *const *x64, hipStream_t st);
@@ -194,9 +209,15 @@ void launch_bcg_flush(const BatchItem *T, int n, long nblocks, int block, hipStr
 // false when no instantiation serves (prec, nl, nr, batches); chunk: site walk (make_walk)
 bool launch_hop_multi(int prec, int nl, int nr, int batches, const Geom &g, const char *links, long lkblk, int par,
                       double ca, double cb, const MultiPtrs &P, int block, int cap, long chunk, hipStream_t st);
-// chunk: site walk of the two multi-RHS hops (make_walk; 0 = canonical order)
+// workgroups of the multi-RHS multi-shift hops at this block size and site walk
+unsigned mrhs_grid(const Geom &g, int block, long chunk);
+// chunk: site walk of the two multi-RHS hops (make_walk; 0 = canonical order);
+// batches: the hops load the right-hand sides' vectors of a link in 1, 2 or 4 batches
 void launch_msb_iter(const Geom &g, const char *links64, long lkblk, int nl, int par, int nr, int nshift,
-                     double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, hipStream_t st);
+                     double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, int batches,
+                     hipStream_t st);
+// end of a window: every x_k, p_k of the group advanced over the window's iterations
+void launch_msb_flush(const Geom &g, int nr, int nshift, const MsbPtrs &P, MsbScal *T, hipStream_t st);
 
 // gauge.hip
 void launch_plaquette(const Geom &g, const double *U, Scal *S, int slot, RedWs red, hipStream_t st);

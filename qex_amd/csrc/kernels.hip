@@ -264,7 +264,7 @@ void launch_cgm_alpha(ShiftScal *T, Scal *S, hipStream_t st) {
 }
 
 // Loads and stores of one complex entry of an FP64 (double2) or FP32
-// (float2) parity vector; the multi-shift updates compute in FP64 either way.
+// (float2) parity vector, computing in FP64.
 __device__ __forceinline__ double2 ld2d(const double2 *p, long o) { return p[o]; }
 __device__ __forceinline__ double2 ld2d(const float2 *p, long o) {
   const float2 v = p[o];
@@ -333,25 +333,17 @@ __global__ void k_cgm_p(Geom g, int nshift, const V *r, V *const *p, ShiftScal *
   }
 }
 
-void launch_cgm_xr(const Geom &g, int nshift, int prec, void *r, const void *Ap, void *const *x, void *const *p,
-                   ShiftScal *T, Scal *S, RedWs red, hipStream_t st) {
-  if (prec == P32)
-    hipLaunchKernelGGL(k_cgm_xr<float2>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (float2 *)r,
-                       (const float2 *)Ap, (float2 *const *)x, (float2 *const *)p, T, S, red);
-  else
-    hipLaunchKernelGGL(k_cgm_xr<double2>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (double2 *)r,
-                       (const double2 *)Ap, (double2 *const *)x, (double2 *const *)p, T, S, red);
+void launch_cgm_xr32(const Geom &g, int nshift, void *r, const void *Ap, void *const *x, void *const *p,
+                     ShiftScal *T, Scal *S, RedWs red, hipStream_t st) {
+  hipLaunchKernelGGL(k_cgm_xr<float2>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (float2 *)r,
+                     (const float2 *)Ap, (float2 *const *)x, (float2 *const *)p, T, S, red);
   FinArgs f{red, (int)nblk(g.Vh, 256), S, T, 0, 0.0, 3};
   launch_finish(EPI_CGM, f, st);
 }
 
-void launch_cgm_p(const Geom &g, int nshift, int prec, const void *r, void *const *p, ShiftScal *T, hipStream_t st) {
-  if (prec == P32)
-    hipLaunchKernelGGL(k_cgm_p<float2>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (const float2 *)r,
-                       (float2 *const *)p, T);
-  else
-    hipLaunchKernelGGL(k_cgm_p<double2>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (const double2 *)r,
-                       (double2 *const *)p, T);
+void launch_cgm_p32(const Geom &g, int nshift, const void *r, void *const *p, ShiftScal *T, hipStream_t st) {
+  hipLaunchKernelGGL(k_cgm_p<float2>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (const float2 *)r,
+                     (float2 *const *)p, T);
 }
 
 // Start of the FP32 multi-shift recursion: r = b, p_k = b, x_k = 0 (cgm.nim:122-158)

@@ -96,11 +96,14 @@ def test_multishift_matches_oracle(oracle, gpu):
         got = f.get(sub=0)[:ne]
         assert rel(got, xs_o[k][:ne]) < 1e-9
         # each shifted system's own residual (oracle operator), equal to the
-        # FP64 true residual the solver reports for that shift
+        # FP64 true residual the solver reports for that shift: to 1 %, or to
+        # the rounding of b - Ax in FP64 (2 |r| |b| 1e-16 relative to |b|^2,
+        # ~3e-30 at the 2e-28 that the largest shifts reach while every shift
+        # is updated until the base converges)
         r = b - O.stagD2xx(L, g, 0, f.get(), m * m) - sig[k] * f.get()
         r2 = O.norm2(r[:ne]) / O.norm2(b)
         assert np.sqrt(r2) < 1e-9
-        assert abs(sp.shiftR2[k] - r2) <= 1e-2 * r2 + 1e-30, (k, sp.shiftR2[k], r2)
+        assert abs(sp.shiftR2[k] - r2) <= 1e-2 * r2 + 1e-15 * np.sqrt(r2), (k, sp.shiftR2[k], r2)
     assert sp.r2 == sp.shiftR2[0] and sp.r2 <= 1.0001e-20
 
 
@@ -134,7 +137,7 @@ def test_mixed_multishift_matches_oracle(oracle, gpu, hisq, sloppy, slinks):
         r = b - O.stagD2xx(L, g, 0, f.get(), m * m) - sig[k] * f.get()
         r2 = O.norm2(r[:ne]) / O.norm2(b)
         assert r2 <= 1.0001e-20, (k, r2)
-        assert abs(sp.shiftR2[k] - r2) <= 1e-2 * r2 + 1e-30, (k, sp.shiftR2[k], r2)
+        assert abs(sp.shiftR2[k] - r2) <= 1e-2 * r2 + 1e-15 * np.sqrt(r2), (k, sp.shiftR2[k], r2)
     assert sp.r2 == sp.shiftR2[0]
 
 

@@ -258,7 +258,7 @@ def test_multishift_batch_24x24x24x48_eight_sources(oracle, gpu, fixtures):
             r2 = true_r2(O, L, g, bs[j], x, m, sg)
             worst = max(worst, r2)
             # the solver's own FP64 recomputation agrees with the oracle's
-            assert abs(sps[j].shiftR2[k] - r2) <= 1e-2 * r2 + 1e-30, (j, k, sps[j].shiftR2[k], r2)
+            assert abs(sps[j].shiftR2[k] - r2) <= 1e-2 * r2 + 1e-15 * np.sqrt(r2), (j, k, sps[j].shiftR2[k], r2)
             assert r2 <= 1e-19, (j, k, r2)
             if j == 0:
                 want = fixture_sample(fx["x_sample_per_shift"][k])
@@ -305,7 +305,7 @@ def test_mixed_multishift_24x24x24x48(oracle, gpu, fixtures, sloppy, slinks):
         x = xs[k].get()
         r2 = true_r2(O, L, g, b, x, m, sg)
         worst = max(worst, r2)
-        assert abs(sp.shiftR2[k] - r2) <= 1e-2 * r2 + 1e-30, (k, sp.shiftR2[k], r2)
+        assert abs(sp.shiftR2[k] - r2) <= 1e-2 * r2 + 1e-15 * np.sqrt(r2), (k, sp.shiftR2[k], r2)
         assert r2 <= 1.0001 * R2REQ, (k, r2)
         e = rel(sample(x, fx["sample_sites"]), fixture_sample(fx["x_sample_per_shift"][k]))
         assert e < 1e-6, (k, e)
