@@ -183,7 +183,7 @@ struct Site {
   long row;      // y + Ly*(z + Lz*t)
 };
 
-__device__ __forceinline__ Site decode(const Geom &g, long i, int par) {
+__host__ __device__ __forceinline__ Site decode(const Geom &g, long i, int par) {
   Site s;
   int xh = (int)(i % g.hx);
   long r = i / g.hx;
@@ -196,13 +196,13 @@ __device__ __forceinline__ Site decode(const Geom &g, long i, int par) {
   return s;
 }
 
-__device__ __forceinline__ int wrap(int v, int L) {
+__host__ __device__ __forceinline__ int wrap(int v, int L) {
   v = (v >= L) ? v - L : v;
   return (v < 0) ? v + L : v;
 }
 
 // checkerboard index (in the opposite parity) of the site s + d*mu_hat
-__device__ __forceinline__ long nbr(const Geom &g, const Site &s, int mu, int d) {
+__host__ __device__ __forceinline__ long nbr(const Geom &g, const Site &s, int mu, int d) {
   if (mu == 0) {
     int x = wrap(s.x + d, g.L[0]);
     return (x >> 1) + (long)g.hx * s.row;

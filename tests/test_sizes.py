@@ -1,5 +1,7 @@
 """Lattice-size limits of the device layout, on the CPU (no GPU call).
 
+- The site and neighbour index arithmetic of the stencils matches plain
+  coordinates on lattices with unequal extents (tests/c/site_index.cpp).
 - The link-load address arithmetic equals the packer's 64-bit offsets at
   lattices whose FP64 link blocks pass 4 GiB (96^4, 64^3 x 256) and at the
   largest accepted lattice (tests/c/link_offsets.cpp, host build).
@@ -29,6 +31,21 @@ def test_link_offsets_host(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout
     assert "link offsets ok" in out.stdout
+
+
+def test_site_indices_host(tmp_path):
+    """decode and nbr against plain coordinates on every site of
+    lattices with unequal extents (tests/c/site_index.cpp, host build)."""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("no hipcc")
+    exe = str(tmp_path / "site_index")
+    subprocess.run([hipcc, "-x", "hip", "--offload-host-only", "-std=c++17", "-O1",
+                    "-I" + os.path.join(ROOT, "qex_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "c", "site_index.cpp"), "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout
+    assert "site indices ok" in out.stdout
 
 
 def _create(dims):
