@@ -218,8 +218,11 @@ int qexhip_set_hop_multi_shape(int batches, long chunk);
 /* block size of the other-parity stencil kernel of CG forms 3 and 4 (64,
    128 or 256; 0 = chosen by the tuner), independent of the other kernels' */
 int qexhip_set_cg_block1(int block1);
-/* replay each chunk of CG iterations from a HIP graph (1, default; QEXHIP_CG_GRAPH=0
-   at init turns it off) or launch kernel by kernel (0) */
+/* CG launches: 1 (default) replays HIP graphs, in the mixed-precision CG one per
+   reliable-update interval (its iterations, the FP64 reliable update and the
+   best-iterate copy); 2 replays graphs of 16 iterations with the reliable
+   updates launched from the host; 0 launches kernel by kernel
+   (QEXHIP_CG_GRAPH=0 and QEXHIP_CG_INTERVALS=0 at init select 0 and 2) */
 int qexhip_set_cg_graph(int on);
 
 /* ---- fermion force (src/hmc/hmcAction.nim:669-792) ---- */

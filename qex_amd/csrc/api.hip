@@ -43,6 +43,7 @@ int g_remap = 1;
 bool g_block_forced = false;  // QEXHIP_BLOCK set: no autotuning
 bool g_autotune = true;
 bool g_cg_graph = true;       // QEXHIP_CG_GRAPH=0: launch CG chunks kernel by kernel
+bool g_cg_intervals = true;   // QEXHIP_CG_INTERVALS=0: mixed CG replays kChunk chunks, not whole intervals
 bool g_smear_walk = true;     // QEXHIP_SMEAR_WALK=0: HISQ smearing kernels walk sites in canonical order
 int g_multi_batches = 0;      // multi-RHS hop: vector-load batches per link pinned (1, 2) or 0 (tuner)
 long g_multi_chunk = -1;      // multi-RHS hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
@@ -516,6 +517,8 @@ double flops_per_its(const qexhip_links *l) {
 }
 
 const int kChunk = 16;
+// longest reliable-update interval captured as one graph (4 kernels per iteration)
+const int kMaxInterval = 256;
 
 // CG on A = 4(m^2 - D_pq D_qp), parity par (solvers/cg.nim:55-272, cpNone),
 // two fused kernels per iteration (dslash.hip).  FP64 mode follows QEX's
@@ -593,48 +596,88 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   // The chunk of iterations is captured once per solve and replayed: one graph
   // launch instead of ~4*kChunk kernel launches, so the host never limits the
   // short sloppy iterations (the kernels read every scalar from S).
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  auto chunk = [&] {
-    require(launch_cg_chunk(pl, rc, ps, l->nl, a, g_stream, tu.block, tu.form, rx, tu.block1, kChunk, w->S1, tu.cap),
-            "no CG kernels for this storage combination");
-  };
-  if (g_cg_graph && !Sh->done) {
-    capture(chunk, &graph, &exec);
-  }
+  //
+  // Mixed mode replays whole intervals instead.  A reliable update comes after
+  // relmax iterations at the latest and leaves since = 0, so an interval of
+  // relmax iterations always ends with need_rel set (relmax reached, or an
+  // earlier residual drop, breakdown or maxits, after which the remaining
+  // iterations do nothing), and the reliable update that follows it needs no
+  // predicate.  One graph holds the interval, the update and the device-side
+  // copy of the best iterate: one host round trip per interval instead of one
+  // per kChunk iterations plus three per update.  An interval cut short by an
+  // earlier trigger switches the rest of the solve to chunks, whose idle tail
+  // after a trigger is at most kChunk - 1 iterations.
+  hipGraph_t graph = nullptr, igraph = nullptr;
+  hipGraphExec_t exec = nullptr, iexec = nullptr;
   struct GraphGuard {
-    hipGraph_t &g;
-    hipGraphExec_t &e;
+    hipGraph_t &g, &ig;
+    hipGraphExec_t &e, &ie;
     ~GraphGuard() {
       if (e) (void)hipGraphExecDestroy(e);
       if (g) (void)hipGraphDestroy(g);
+      if (ie) (void)hipGraphExecDestroy(ie);
+      if (ig) (void)hipGraphDestroy(ig);
     }
-  } guard{graph, exec};
+  } guard{graph, igraph, exec, iexec};
+  auto chunk = [&](int n) {
+    require(launch_cg_chunk(pl, rc, ps, l->nl, a, g_stream, tu.block, tu.form, rx, tu.block1, n, w->S1, tu.cap),
+            "no CG kernels for this storage combination");
+  };
+  // the reliable update (FP64 mode: the flush of the pending update alone)
+  auto update = [&] {
+    launch_cg_flush(ps, rx, a, g_stream);  // apply the pending x/r update; exact |r|^2
+    if (mixed) {
+      launch_xfold(g, x, w->xs, S, rx, g_stream);
+      resid64(l, par, x, b, w->r64, S, 2, w->t64);
+      launch_rs_from_true(g, w->rs, w->r64, S, rx, g_stream);
+    }
+  };
+  // the sloppy recursion diverged: back to the best iterate, whose true
+  // residual epi_reliable accepts again, restarting p = r
+  auto rollback = [&] {
+    HIPCHK(hipMemcpyAsync(x, w->p64, vec64_bytes(lat), hipMemcpyDeviceToDevice, g_stream));
+    resid64(l, par, x, b, w->r64, S, 2, w->t64);
+    launch_rs_from_true(g, w->rs, w->r64, S, rx, g_stream);
+    check_launch();
+    pull(Sh, S);
+  };
+  const int nint = Sh->relmax;
+  bool intervals = mixed && g_cg_graph && g_cg_intervals && nint <= kMaxInterval && !Sh->done;
+  if (intervals) {
+    capture(
+        [&] {
+          chunk(nint);
+          update();
+          launch_keep_best(g, w->p64, x, S, g_stream);
+        },
+        &igraph, &iexec);
+  } else if (g_cg_graph && !Sh->done) {
+    capture([&] { chunk(kChunk); }, &graph, &exec);
+  }
   while (!Sh->done) {
+    if (intervals) {
+      const int its0 = Sh->its;
+      HIPCHK(hipGraphLaunch(iexec, g_stream));
+      check_launch();
+      pull(Sh, S);
+      if (Sh->rollback) rollback();
+      if (!Sh->done && Sh->its - its0 < nint) {
+        intervals = false;
+        capture([&] { chunk(kChunk); }, &graph, &exec);
+      }
+      continue;
+    }
     if (exec) HIPCHK(hipGraphLaunch(exec, g_stream));
-    else chunk();
+    else chunk(kChunk);
     check_launch();
     pull(Sh, S);
     if (Sh->done || Sh->need_rel) {
-      launch_cg_flush(ps, rx, a, g_stream);  // apply the pending x/r update; exact |r|^2
-      if (mixed) {
-        launch_xfold(g, x, w->xs, S, rx, g_stream);
-        resid64(l, par, x, b, w->r64, S, 2, w->t64);
-        launch_rs_from_true(g, w->rs, w->r64, S, rx, g_stream);
-      }
+      update();
       check_launch();
       pull(Sh, S);
-      if (mixed && Sh->rollback) {
-        // the sloppy recursion diverged: back to the best iterate, whose true
-        // residual epi_reliable accepts again, restarting p = r
-        HIPCHK(hipMemcpyAsync(x, w->p64, vec64_bytes(lat), hipMemcpyDeviceToDevice, g_stream));
-        resid64(l, par, x, b, w->r64, S, 2, w->t64);
-        launch_rs_from_true(g, w->rs, w->r64, S, rx, g_stream);
-        check_launch();
-        pull(Sh, S);
-      } else if (mixed && Sh->improved) {
+      if (mixed && Sh->rollback) rollback();
+      else if (mixed && Sh->improved)
         HIPCHK(hipMemcpyAsync(w->p64, x, vec64_bytes(lat), hipMemcpyDeviceToDevice, g_stream));
-      }
     }
   }
   *its_out = Sh->its;
@@ -1423,6 +1466,7 @@ int qexhip_init(int device) {
   }
   if (const char *t = getenv("QEXHIP_TUNE")) g_autotune = atoi(t) != 0;
   if (const char *c = getenv("QEXHIP_CG_GRAPH")) g_cg_graph = atoi(c) != 0;
+  if (const char *c = getenv("QEXHIP_CG_INTERVALS")) g_cg_intervals = atoi(c) != 0;
   if (const char *f = getenv("QEXHIP_CG_FORM")) {
     const int v = atoi(f);
     if (v >= 2 && v <= 4) g_cg_form = v;
@@ -2512,8 +2556,11 @@ int qexhip_set_hop_multi_shape(int batches, long chunk) {
 }
 
 int qexhip_set_cg_graph(int on) {
+  API_BEGIN
+  require(on >= 0 && on <= 2, "cg graph mode must be 0, 1 or 2");
   g_cg_graph = on != 0;
-  return 0;
+  g_cg_intervals = on == 1;
+  API_END
 }
 
 int qexhip_set_cg_form(int form) {

@@ -188,6 +188,8 @@ void launch_redot(int prec, const Geom &g, const void *x, const void *y, Scal *S
                   hipStream_t st);
 void launch_rs_from_true(const Geom &g, void *rs, const double *rtrue, Scal *S, int pr, hipStream_t st);
 void launch_xfold(const Geom &g, double *x, void *xs, Scal *S, int px, hipStream_t st);
+// best = x when the last reliable update improved on the best residual and did not roll back
+void launch_keep_best(const Geom &g, double *best, const double *x, const Scal *S, hipStream_t st);
 void launch_scal_init(Scal *S, hipStream_t st, int mode);
 // the multi-shift updates of the mixed solve's FP32 recursion (FP64 arithmetic);
 // x, p: device arrays of nshift vector pointers

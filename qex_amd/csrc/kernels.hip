@@ -220,6 +220,26 @@ void launch_xfold(const Geom &g, double *x, void *xs, Scal *S, int px, hipStream
     hipLaunchKernelGGL(k_xfold<P32>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, (double2 *)x, xs, S);
 }
 
+// reliable update, after the true residual: best = x when epi_reliable found
+// the residual no larger than the best one so far (and did not roll back);
+// the device-side form of the mixed CG's best-iterate copy, for a reliable
+// update that runs inside a captured graph
+__global__ void k_keep_best(Geom g, double2 *best, const double2 *x, const Scal *S) {
+  if (S->rollback || !S->improved) return;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.Vh) return;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const long o = vix(g, c, i);
+    best[o] = x[o];
+  }
+}
+
+void launch_keep_best(const Geom &g, double *best, const double *x, const Scal *S, hipStream_t st) {
+  hipLaunchKernelGGL(k_keep_best, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, (double2 *)best, (const double2 *)x,
+                     S);
+}
+
 __global__ void k_scal_init(Scal *S, int mode) {
   if (threadIdx.x != 0) return;
   if (mode == 0) {  // start of a CG solve: fresh state (cg.nim:21-27)

@@ -466,9 +466,13 @@ def set_autotune(on=True):
 
 
 def set_cg_graph(on=True):
-    """Replay each chunk of CG iterations from a HIP graph (default) or launch
-    kernel by kernel."""
-    call("qexhip_set_cg_graph", int(bool(on)))
+    """CG launches: True/1 (default) replays HIP graphs, in the mixed-precision
+    CG one graph per reliable-update interval (its iterations, the FP64
+    reliable update and the best-iterate copy); 2 replays graphs of 16
+    iterations with the reliable updates launched from the host; False/0
+    launches kernel by kernel."""
+    mode = on if isinstance(on, int) and not isinstance(on, bool) else int(bool(on))
+    call("qexhip_set_cg_graph", mode)
 
 
 def set_cg_block1(block1=0):

@@ -369,11 +369,17 @@ def test_fp16_all_storage_reaches_fp64_residual(oracle, gpu, slinks, recon, hisq
     assert rel(fx.get(sub=0)[: lo.nEven], want[: lo.nEven]) < 1e-6
 
 
-@pytest.mark.parametrize("sloppy,slinks,recon", [(0, None, 18), (2, 4, 12)])
-def test_cg_graph_replay_matches_eager(oracle, gpu, sloppy, slinks, recon):
-    """The CG chunks replayed from a HIP graph (with reliable updates and
-    synchronisations between replays) give bit-identical iterates to the
-    kernel-by-kernel launches."""
+@pytest.mark.parametrize("sloppy,slinks,recon,m,relmax", [(0, None, 18, 0.03, 100), (2, 4, 12, 0.03, 100),
+                                                          (2, 4, 12, 0.01, 100), (1, None, 18, 0.01, 37)])
+def test_cg_graph_replay_matches_eager(oracle, gpu, sloppy, slinks, recon, m, relmax):
+    """The CG replayed from HIP graphs gives bit-identical iterates, iteration
+    and reliable-update counts to the kernel-by-kernel launches: mode 1 (the
+    mixed CG's graph per reliable-update interval, with the update and the
+    best-iterate copy inside the graph), mode 2 (graphs of 16 iterations,
+    updates from the host) and mode 0.  Masses and intervals cover intervals
+    that run to relmax (including an odd one) and intervals cut short by a
+    residual drop, after which the solve continues in 16-iteration graphs;
+    the true residual meets the request by the oracle's operator."""
     O = oracle
     L = (8, 8, 8, 8)
     g = links(O, L, False)
@@ -382,18 +388,22 @@ def test_cg_graph_replay_matches_eager(oracle, gpu, sloppy, slinks, recon):
     s = gpu.Staggered(lo, g)
     fb = lo.ColorVector().set(b)
     out = []
-    for graph in (True, False):
+    for graph in (1, 2, 0):
         fx = lo.ColorVector()
         sp = gpu.SolverParams()
         sp.r2req, sp.verbosity, sp.sloppySolve, sp.sloppyLinks, sp.recon = 1e-20, 0, sloppy, slinks, recon
+        sp.reliableMaxIts = relmax
         gpu.set_cg_graph(graph)
         try:
-            gpu.solveEE(s, fx, fb, 0.03, sp)
+            gpu.solveEE(s, fx, fb, m, sp)
         finally:
             gpu.set_cg_graph(True)
-        out.append((fx.get(), sp.iterations, sp.r2))
-    assert out[0][1] == out[1][1] and out[0][2] == out[1][2]
-    assert np.array_equal(out[0][0], out[1][0])
+        out.append((fx.get(), sp.iterations, sp.r2, sp.reliableUpdates))
+    for o in out[1:]:
+        assert o[1:] == out[0][1:], (o[1:], out[0][1:])
+        assert np.array_equal(o[0], out[0][0])
+    r = b - O.stagD2xx(L, g, 0, out[0][0], m * m)
+    assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20
 
 
 @pytest.mark.parametrize("L", [(8, 8, 8, 8), (16, 16, 8, 8)])
