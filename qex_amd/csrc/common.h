@@ -20,6 +20,15 @@
 #ifndef QH_PIPE
 #define QH_PIPE -3
 #endif
+// the same for the stencils of 16-bit link formats.  Measured at 32^4 (Q16
+// 12-real hop / mixed CG iteration): -3 22.8 / 70.4 us, -1 24.5 / 73.5 us,
+// -4 25.3 / 78.3 us; the directions fully unrolled (40% fewer vector
+// instructions, all loads in flight, 4 waves per SIMD instead of 8) were 3%
+// slower in the CG.  Waves per SIMD, not instruction issue or loads in flight
+// per wave, set these kernels' speed.
+#ifndef QH_PIPE16
+#define QH_PIPE16 -3
+#endif
 
 namespace qh {
 

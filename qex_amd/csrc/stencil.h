@@ -54,11 +54,15 @@ __device__ __forceinline__ void mem_fence_compiler() { asm volatile("" ::: "memo
 // compiler barrier after each leg).  QH_PIPE = -1 / -2 / -3: a rolled loop over
 // hops / links / links with a single accumulator, which bounds the registers
 // a wave holds.
+template <int PL> constexpr int stencil_form() {
+  return (PL == P16 || PL == PBF || PL == PQ16) ? QH_PIPE16 : QH_PIPE;
+}
+
 template <int PL, int RC, int NL, class A, class LD>
 __device__ __forceinline__ void stencil(const Geom &g, const char *links, long lkblk, const float *qs, int par,
                                         long i, const Site &s, LD &&ld, A (&res)[3]) {
   constexpr int NJ = 2 * NL;
-  constexpr int D = QH_PIPE;
+  constexpr int D = stencil_form<PL>();
   A af[3], ab[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) { af[c] = mk<A>(0, 0); ab[c] = mk<A>(0, 0); }
