@@ -402,7 +402,8 @@ struct DerivFields {
   MF S[12], T[12];      // forward intermediates of this level
   MF C[4];              // chain dS/dF
   MF CT[12], CS[12];    // chains of T and S
-  MF P[12];             // side-link partials, [3*lam + k] from mu = other(lam, k)
+  MF P[12];             // side-link partials at the site, [3*lam + k] from mu = other(lam, k)
+  MF Q[12];             // side-link partials one site back along mu (added at y from y - mu)
   MF out[4];            // dS/dU (accumulated when acc)
 };
 
@@ -422,19 +423,70 @@ __device__ __forceinline__ int third_dir(int a, int b, int c) {
 }
 
 // Side-link derivative of the staple S_lam[K] in the (mu,lam) plane with chain
-// C, at site y (coordinates c), added to acc with weight a:
+// C, weight a, added at y:
 //   C(y) U(y+mu) K(y+lam)^+ + K(y) U(y+mu) C(y+lam)^+
-//   + C(y-mu)^+ U(y-mu) K(y-mu+lam) + K(y-mu)^+ U(y-mu) C(y-mu+lam),  U = U_lam
-__device__ __forceinline__ void side(M3 &acc, double a, const Geom &g, const MF &Ul, const MF &C, const MF &K,
-                                     const int c[4], long y, int mu, int lam) {
-  int cm[4] = {c[0], c[1], c[2], c[3]};
-  cm[mu] = c[mu] > 0 ? c[mu] - 1 : g.L[mu] - 1;
-  const long ypm = step(g, c, mu, 1), ypl = step(g, c, lam, 1);
-  const long ymm = canon_index(g, cm), ymmpl = step(g, cm, lam, 1);
-#pragma unroll 1
-  for (int t = 0; t < 2; t++) triple<false>(acc, a, t ? K : C, y, Ul, ypm, t ? C : K, ypl);
-#pragma unroll 1
-  for (int t = 0; t < 2; t++) triple<true>(acc, a, t ? K : C, ymm, Ul, ymm, t ? C : K, ymmpl);
+//   + C(y-mu)^+ U(y-mu) K(y-mu+lam) + K(y-mu)^+ U(y-mu) C(y-mu+lam),  U = U_lam.
+// It is evaluated as A(y) + B(y - mu) with, at one site z,
+//   A(z) = C(z) U(z+mu) K(z+lam)^+ + K(z) U(z+mu) C(z+lam)^+
+//   B(z) = C(z)^+ U(z) K(z+lam) + K(z)^+ U(z) C(z+lam),
+// whose four products share C(z), K(z), C(z+lam), K(z+lam).  side_half adds
+// one product of A (BACK false: acc += a L(z) U(z+mu) R(z+lam)^+) or of B
+// (BACK true: acc += a L(z)^+ U(z) R(z+lam)), holding R, U R^+ (or U R) and
+// the accumulator, with L and U read a row at a time.
+template <bool BACK>
+__device__ __forceinline__ void side_half(M3 &acc, double a, const MF &Ul, long zu, const MF &L, long z, const MF &R,
+                                          long zpl) {
+  const M3 r = ldf(R, zpl);
+  M3 x;  // U(zu) R^+ (A) or U(zu) R (B)
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    double2 u[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) u[k] = Ul.p[(3 * i + k) * Ul.es + zu * Ul.ss];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double2 t = make_double2(0, 0);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (BACK) {  // u_k r_kj
+          t.x += u[k].x * r.e[3 * k + j].x - u[k].y * r.e[3 * k + j].y;
+          t.y += u[k].x * r.e[3 * k + j].y + u[k].y * r.e[3 * k + j].x;
+        } else {  // u_k conj(r_jk)
+          t.x += u[k].x * r.e[3 * j + k].x + u[k].y * r.e[3 * j + k].y;
+          t.y += u[k].y * r.e[3 * j + k].x - u[k].x * r.e[3 * j + k].y;
+        }
+      }
+      x.e[3 * i + j] = t;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {  // row k of L
+    double2 l[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) l[i] = L.p[(3 * k + i) * L.es + z * L.ss];
+    if (BACK) {  // acc_ij += a conj(L_ki) x_kj
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          const double2 y = x.e[3 * k + j];
+          acc.e[3 * i + j].x += a * (l[i].x * y.x + l[i].y * y.y);
+          acc.e[3 * i + j].y += a * (l[i].x * y.y - l[i].y * y.x);
+        }
+    } else {  // acc_kj += a L_ki x_ij
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        double2 t = make_double2(0, 0);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          t.x += l[i].x * x.e[3 * i + j].x - l[i].y * x.e[3 * i + j].y;
+          t.y += l[i].x * x.e[3 * i + j].y + l[i].y * x.e[3 * i + j].x;
+        }
+        acc.e[3 * k + j].x += a * t.x;
+        acc.e[3 * k + j].y += a * t.y;
+      }
+    }
+  }
 }
 
 // CT_{mu,sig} = c5 C_mu + c7 S_sig[C_mu]; one wave per sig, mu = blockIdx.y
@@ -489,32 +541,37 @@ __global__ void __launch_bounds__(192) k_dcs(Geom g, DerivFields f, Fat7 c, int 
   stf(f.CS[3 * mu + kn], x, t);
 }
 
-// side-link partials of dS/dU_lam from the staples of the (mu,lam) plane,
-// mu = other(lam, k): one wave per k, lam = blockIdx.y
+// side-link partials of dS/dU_lam from the staples of the (mu,lam) plane: A
+// into P at the site, B into Q (k_dfin adds it one site along mu).  One
+// workgroup per 64 sites and direction mu = blockIdx.y, with a pair of waves
+// per lam = other(mu, k), one for A and one for B: the six waves read the same
+// mu-fields (U_mu, C_mu, S_{mu,*}, T_{mu,*}, CT_{mu,*}) at the same sites, and
+// each pair the same C, K at z and z + lam.
+#ifndef QH_DSIDE_WAVES
+#define QH_DSIDE_WAVES 1  // minimum waves per SIMD asked of the register allocator
+#endif
 template <int HAVE5>
-__global__ void __launch_bounds__(192) k_dside(Geom g, DerivFields f, Fat7 c, int remap) {
+__global__ void __launch_bounds__(384, QH_DSIDE_WAVES) k_dside(Geom g, DerivFields f, Fat7 c, int remap) {
   const long V = 2 * g.Vh;
-  const long y = site_of(g, remap);
-  const int lam = blockIdx.y, k = wave_of();
-  if (y >= V) return;
-  const int mu = other(lam, k);
+  const long z = site_of(g, remap);
+  const int mu = blockIdx.y, w = wave_of();
+  if (z >= V) return;
+  const int lam = other(mu, w >> 1);
+  const bool back = w & 1;
   const int sl = 3 * mu + slot(mu, lam);
   int cc[4];
   uint32_t lex;
-  coords_of(g, y, cc, lex);
+  coords_of(g, z, cc, lex);
+  const long zpl = step(g, cc, lam, 1);
+  const long zu = back ? z : step(g, cc, mu, 1);
   M3 acc;
 #pragma unroll
   for (int e = 0; e < 9; e++) acc.e[e] = make_double2(0, 0);
   // terms j: 0 S_lam[U_mu] in S_{mu,lam}; 1 Lepage; 2 7-link; 3, 4 S_lam[S_{mu,rho}]
-  // inside T_{mu,sig} for the two sig != lam.  One rolled loop, so that the
-  // loads of one term are not hoisted into another's.
-  if (!HAVE5) {
-    side(acc, 1.0, g, f.U[lam], f.CS[sl], f.U[mu], cc, y, mu, lam);
-    stf(f.P[3 * lam + k], y, acc);
-    return;
-  }
+  // inside T_{mu,sig} for the two sig != lam.  Rolled loops, so that the
+  // loads of one product are not hoisted into another's.
 #pragma unroll 1
-  for (int j = 0; j < 5; j++) {
+  for (int j = 0; j < (HAVE5 ? 5 : 1); j++) {
     double a = 1.0;
     MF Cf = f.CS[sl], Kf = f.U[mu];
     if (j == 1) {
@@ -535,9 +592,14 @@ __global__ void __launch_bounds__(192) k_dside(Geom g, DerivFields f, Fat7 c, in
       Cf = f.CT[3 * mu + ks];
       Kf = f.S[3 * mu + slot(mu, third_dir(mu, lam, sig))];
     }
-    side(acc, a, g, f.U[lam], Cf, Kf, cc, y, mu, lam);
+#pragma unroll 1
+    for (int h = 0; h < 2; h++) {
+      if (back) side_half<true>(acc, a, f.U[lam], zu, h ? Kf : Cf, z, h ? Cf : Kf, zpl);
+      else side_half<false>(acc, a, f.U[lam], zu, h ? Kf : Cf, z, h ? Cf : Kf, zpl);
+    }
   }
-  stf(f.P[3 * lam + k], y, acc);
+  const int o = 3 * lam + slot(lam, mu);
+  stf(back ? f.Q[o] : f.P[o], z, acc);
 }
 
 // dS/dU_lam (+)= (c1 - 6 cL) C_lam + sum_nu S_nu[CS_{lam,nu}] + side partials
@@ -562,6 +624,7 @@ __global__ void __launch_bounds__(256) k_dfin(Geom g, DerivFields f, Fat7 c, int
       const int nu = other(lam, k);
       gstaple_acc(d, 1.0, g, f.U[nu], f.CS[3 * lam + k], cc, y, lam, nu);
       madd(d, ldf(f.P[3 * lam + k], y));
+      madd(d, ldf(f.Q[3 * lam + k], step(g, cc, nu, -1)));
     }
   }
   if (acc_out) madd(d, ldf(f.out[lam], y));
@@ -716,10 +779,10 @@ SmearWs::~SmearWs() {
 // Workspace fields (component-major, V x 9 complex each):
 //   0-3 level input U, 4-15 S, 16-27 T, 28-31 first-level fat links V1,
 //   32-35 projected links W; the chain rule adds 36-47 CT, 48-59 CS,
-//   60-71 side-link partials, 72-75 fat chain, 76-79 long chain,
-//   80-83 dS/dW, 84-87 dS/dV1.
-//   88-95 HISQ force outer products, site-major [k][site] (smear_force_outer).
-enum { WS_SMEAR = 36, WS_DERIV = 88, WS_FORCE = 96 };
+//   60-71 side-link partials at the site, 72-75 fat chain, 76-79 long chain,
+//   80-83 dS/dW, 84-87 dS/dV1, 88-99 side-link partials one site back.
+//   100-107 HISQ force outer products, site-major [k][site] (smear_force_outer).
+enum { WS_SMEAR = 36, WS_DERIV = 100, WS_FORCE = 108 };
 
 static void ws_ensure(SmearWs &w, long V, int nfields) {
   if (w.buf && w.V == V && w.nfields >= nfields) return;
@@ -865,6 +928,7 @@ static DerivFields deriv_fields(SmearWs &w, long V, const MF U[4], const MF C[4]
     f.CT[k] = comp_major(w.buf, V, 36 + k);
     f.CS[k] = comp_major(w.buf, V, 48 + k);
     f.P[k] = comp_major(w.buf, V, 60 + k);
+    f.Q[k] = comp_major(w.buf, V, 88 + k);
   }
   return f;
 }
@@ -897,10 +961,10 @@ static void fat7_deriv_level(const Geom &g, SmearWs &w, const MF U[4], const MF 
     }
     if (have5) {
       hipLaunchKernelGGL((k_dcs<1>), dim3(nb, 4), dim3(192), 0, st, g, f, c, remap);
-      hipLaunchKernelGGL((k_dside<1>), dim3(nb, 4), dim3(192), 0, st, g, f, c, remap);
+      hipLaunchKernelGGL((k_dside<1>), dim3(nb, 4), dim3(384), 0, st, g, f, c, remap);
     } else {
       hipLaunchKernelGGL((k_dcs<0>), dim3(nb, 4), dim3(192), 0, st, g, f, c, remap);
-      hipLaunchKernelGGL((k_dside<0>), dim3(nb, 4), dim3(192), 0, st, g, f, c, remap);
+      hipLaunchKernelGGL((k_dside<0>), dim3(nb, 4), dim3(384), 0, st, g, f, c, remap);
     }
     hipLaunchKernelGGL((k_dfin<1>), dim3(nb), dim3(256), 0, st, g, f, c, acc, remap);
   } else {
