@@ -204,6 +204,12 @@ int qexhip_set_cg_form(int form);
    chunks of that many sites and walked t first (their t +- 1 neighbours are
    read again while still in L2), -1 = chosen by the tuner */
 int qexhip_set_hop_walk(long chunk);
+/* t-march form of the one-parity hop of plain links (operator calls and the
+   solvers' FP64 residuals), no QEX counterpart: each thread walks this many
+   t-slices of one half-site index and keeps the forward t-neighbour it loaded
+   for the next slice but one, where it is the backward one; 0 = off (the site
+   walk of qexhip_set_hop_walk applies), -1 = chosen by the tuner */
+int qexhip_set_hop_tmarch(int slices);
 /* site walk of the two multi-RHS hops of the batched multi-shift solve
    (qexhip_solve_multishift_batch), no QEX counterpart: same values as
    qexhip_set_hop_walk; -1 = chosen by the tuner (with QEXHIP_BATCH_TUNE, else

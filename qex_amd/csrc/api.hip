@@ -48,6 +48,7 @@ bool g_smear_walk = true;     // QEXHIP_SMEAR_WALK=0: HISQ smearing kernels walk
 int g_multi_batches = 0;      // multi-RHS hop: vector-load batches per link pinned (1, 2) or 0 (tuner)
 long g_multi_chunk = -1;      // multi-RHS hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 long g_hop_chunk = -1;        // one-parity hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
+int g_hop_tmarch = -1;        // one-parity hop of plain links: t-march slices per thread pinned (0: off) or -1 (tuner)
 long g_msb_walk = -1;         // batched multi-shift hops: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 double g_ms_sloppy_r2 = 1e-12; // mixed multi-shift: FP32 phase ends at |r|^2 <= this |b|^2 (QEXHIP_MS_SLOPPY_R2)
 int g_cg_block1 = 0;          // CG forms 3/4: other-parity stencil kernel's block size pinned, 0 (tuner)
@@ -263,6 +264,7 @@ struct Tuned {
   int block1 = 0;  // CG form 3: block size of the other-parity stencil kernel (0: block)
   int batches = 1;  // multi-RHS hop: vector-load batches per link
   long chunk = 0;   // site walk (make_walk): t-march chunk, 0 = canonical order
+  int tmarch = 0;   // hop of plain links: t-slices per thread of the t-march form (0: off)
 };
 std::map<std::string, Tuned> g_tune;
 
@@ -358,6 +360,7 @@ void hop_shaped(qexhip_links *l, const char *lk, const Tuned &tu, int prec, int 
   h.S = S;
   h.red = l->lat->red;
   h.chunk = tu.chunk;
+  h.tmarch = l->nl == 4 ? tu.tmarch : 0;
   const int pv = (prec == PQ16) ? P16 : prec;  // 16-bit fixed-point links go with FP16 vectors
   HopKey k{prec, rc, pv, pv, pv};
   launch_hop(k, l->nl, mode, redm, h, g_stream, tu.block, tu.cap);
@@ -371,8 +374,9 @@ void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const v
 
 std::string hop_key(const qexhip_links *l, int prec, int rc) {
   char kb[96];
-  if (g_hop_chunk >= 0)
-    snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d walk=%ld ", l->nl, prec, rc, g_hop_chunk);
+  if (g_hop_chunk >= 0 || g_hop_tmarch >= 0)
+    snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d walk=%ld tmarch=%d ", l->nl, prec, rc, g_hop_chunk,
+             g_hop_tmarch);
   else
     snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d ", l->nl, prec, rc);
   return kb + geom_key(l->lat->g);
@@ -385,6 +389,7 @@ Tuned cached_hop_shape(const qexhip_links *l, int prec, int rc) {
   if (it != g_tune.end()) return it->second;
   Tuned t{g_block, g_remap, 0.0, 2, 0};
   t.chunk = g_hop_chunk > 0 ? g_hop_chunk : 0;
+  t.tmarch = g_hop_tmarch > 0 ? g_hop_tmarch : 0;
   return t;
 }
 
@@ -413,20 +418,40 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
   // 3.26 instead of 3.58 GB per hop in 582 instead of 545 us; a candidate still)
   std::vector<long> chunks = {0L, 2048L, 8192L};
   if (g_hop_chunk >= 0) chunks = {g_hop_chunk};
-  for (long chunk : chunks) {
-    if (chunk && g_hop_chunk < 0 && make_walk(h.g, 256, chunk).C == 0) continue;  // t-slices too small
-    for (int cap : {0, 1024, 2048})
+  if (g_hop_tmarch <= 0)
+    for (long chunk : chunks) {
+      if (chunk && g_hop_chunk < 0 && make_walk(h.g, 256, chunk).C == 0) continue;  // t-slices too small
+      for (int cap : {0, 1024, 2048})
+        for (int block : {64, 128, 256})
+          for (int remap : {1, 0}) {
+            h.remap = remap;
+            h.chunk = chunk;
+            h.tmarch = 0;
+            const double us = time_us([&] { launch_hop(hk, l->nl, 0, 0, h, g_stream, block, cap); }, 2, 8);
+            if (us < best.us) {
+              best = Tuned{block, remap, us, 2, cap};
+              best.chunk = chunk;
+            }
+          }
+    }
+  // the t-march form (plain links): t-slices per thread x block size x XCD order
+  std::vector<int> tms = {4, 8, 16};
+  if (g_hop_tmarch >= 0) tms = {g_hop_tmarch};
+  if (l->nl == 4 && g_hop_chunk <= 0)
+    for (int tm : tms) {
+      if (tm <= 0) continue;
       for (int block : {64, 128, 256})
         for (int remap : {1, 0}) {
           h.remap = remap;
-          h.chunk = chunk;
-          const double us = time_us([&] { launch_hop(hk, l->nl, 0, 0, h, g_stream, block, cap); }, 2, 8);
+          h.chunk = 0;
+          h.tmarch = tm;
+          const double us = time_us([&] { launch_hop(hk, l->nl, 0, 0, h, g_stream, block, 0); }, 2, 8);
           if (us < best.us) {
-            best = Tuned{block, remap, us, 2, cap};
-            best.chunk = chunk;
+            best = Tuned{block, remap, us, 2, 0};
+            best.tmarch = tm;
           }
         }
-  }
+    }
   g_tune[key] = best;
   HIPCHK(hipStreamSynchronize(g_stream));
   (void)hipFree(in);
@@ -2536,6 +2561,13 @@ int qexhip_set_hop_walk(long chunk) {
   API_BEGIN
   require(chunk == -1 || (chunk >= 0 && chunk % 256 == 0), "site-walk chunk must be -1 (tuner), 0 or a multiple of 256");
   g_hop_chunk = chunk;
+  API_END
+}
+
+int qexhip_set_hop_tmarch(int slices) {
+  API_BEGIN
+  require(slices >= -1 && slices <= 1024, "t-march slices per thread must be -1 (tuner), 0 (off) or 1..1024");
+  g_hop_tmarch = slices;
   API_END
 }
 

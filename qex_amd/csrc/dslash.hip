@@ -81,12 +81,147 @@ __global__ void __launch_bounds__(256, QH_HOP_WAVES) k_hop(HopArgs P) {
   }
 }
 
+// ------------------------------------------------------- t-march hop
+// The hop of plain links (nl = 4) with each thread walking a column of
+// t-slices: thread l of logical workgroup (seg, blk) serves the half-site
+// index is = blk*blockDim.x + l of the output-parity slices t = seg*T ..
+// seg*T + T - 1.  A t-hop keeps x/2, y and z, so the t-neighbours of output
+// site (t, is) are the input sites (t -/+ 1, is), and the input vector loaded
+// as the forward neighbour of slice t is the backward neighbour of slice t+2:
+// it stays in registers (two slots, one per slice parity), and a column's
+// input vectors are each loaded once per segment, plus two at its start.
+// Workgroups are segment-major, so an XCD's contiguous range holds
+// neighbouring blocks of the same segment, which march through t together and
+// share their x, y, z neighbours through its L2; the t-neighbours, which the
+// canonical order reads again a slice later (from memory once two slices
+// outgrow the L2, as at 48^4), no longer come through the caches at all.
+template <int PL, int RC, int PI, int PO, int PA, class A, int MODE, int RED>
+__global__ void __launch_bounds__(256, QH_HOP_WAVES) k_hop_tm(HopArgs P) {
+  using R = RT<A>;
+  __shared__ double lds[16];
+  if (P.pred && (P.S->done | P.S->need_rel)) return;
+  const Geom &g = P.g;
+  const int Lt = g.L[3], T = P.tmarch;
+  const long S = g.Vh / Lt;
+  const long nblk = (S + blockDim.x - 1) / blockDim.x;
+  const long G = xcd_block(P.remap);
+  const long seg = G / nblk, is = (G % nblk) * blockDim.x + threadIdx.x;
+  const int t0 = (int)seg * T;
+  double red = 0.0;
+  if (is < S && t0 < Lt) {
+    const int t1 = t0 + T < Lt ? t0 + T : Lt;
+    auto vin = [&](int t, A(&v)[3]) {
+      const long j = (long)wrap(t, Lt) * S + is;
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[c] = ldc<PI, A>(P.in, vix(g, c, j));
+    };
+    // W[0]: backward t-neighbour of the even steps t0 + 2k, W[1] of the odd ones
+    A W[2][3];
+    vin(t0 - 1, W[0]);
+    vin(t0, W[1]);
+    // one output slice: the legs of the rolled stencil loop, with the backward
+    // t-neighbour from w and the forward one, loaded, left in w for slice t + 2
+    auto step = [&](int t, A(&w)[3]) {
+      const long i = (long)t * S + is;
+      const Site s = decode(g, i, P.par);
+      A af[3], nf[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) af[c] = mk<A>(0, 0);
+#pragma unroll 1
+      for (int j = 0; j < 8; j++) {
+        const int mu = j >> 1, fwd = !(j & 1);
+        const long n = nbr(g, s, mu, fwd ? 1 : -1);
+        A U[9], v[3];
+        ld_link<PL, RC, true, A, link_blocked<PL, 4>()>(P.links + (long)(2 * mu + (fwd ? P.par : 1 - P.par)) * P.lkblk,
+                                                        g.stride, fwd ? i : n, U,
+                                                        (R)(RC == 12 ? link_sign(g, s, mu, fwd) : 1.0f),
+                                                        PL == PQ16 ? P.qs[mu] : 0.0f);
+        if (j == 7) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) v[c] = w[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 3; c++) v[c] = ldc<PI, A>(P.in, vix(g, c, n));
+          if (j == 6)
+#pragma unroll
+            for (int c = 0; c < 3; c++) nf[c] = v[c];
+        }
+        if (fwd) {
+#pragma unroll
+          for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) cmac(af[r], U[3 * r + c], v[c]);
+        } else {
+#pragma unroll
+          for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) cjmsub(af[r], U[3 * c + r], v[c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) w[c] = nf[c];
+      const R chop = (R)P.chop;
+      A res[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) res[c] = mk<A>(chop * af[c].x, chop * af[c].y);
+      A av[3];
+      if (MODE >= 1) {
+        double ca = P.ca;
+        if (P.ca_m2) ca *= 4.0 * P.S->m2;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          av[c] = ldc<PA, A>(P.a, vix(g, c, i));
+          res[c].x += (R)ca * av[c].x;
+          res[c].y += (R)ca * av[c].y;
+        }
+      }
+      if (MODE == 2) {
+        const R cb = (R)P.cb;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const A bv = ldc<PA, A>(P.b, vix(g, c, i));
+          res[c].x += cb * bv.x;
+          res[c].y += cb * bv.y;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) stc_out<PO, A>(P.out, vix(g, c, i), res[c]);
+      if (RED == 1) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) red += (double)av[c].x * (double)res[c].x + (double)av[c].y * (double)res[c].y;
+      } else if (RED >= 2) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) red += (double)res[c].x * (double)res[c].x + (double)res[c].y * (double)res[c].y;
+      }
+    };
+#pragma unroll 1
+    for (int t = t0; t < t1; t += 2) {
+      step(t, W[0]);
+      if (t + 1 < t1) step(t + 1, W[1]);
+    }
+  }
+  if (RED) {
+    double v[1] = {red};
+    block_partials<1>(v, lds, P.red);
+  }
+}
+
+// workgroups of the t-march hop: (slices / tmarch, rounded up) x blocks per
+// slice, rounded up to a multiple of 8 for the XCD-aware order
+inline unsigned tmarch_grid(const Geom &g, int block, int tmarch, int remap) {
+  const long S = g.Vh / g.L[3];
+  const long nseg = (g.L[3] + tmarch - 1) / tmarch;
+  return grid_blocks(nseg * ((S + block - 1) / block), remap, 0);
+}
+
 template <int PL, int RC, int PI, int PO, int PA, class A, int NL>
 static bool hop_modes(int mode, int redm, const HopArgs &a, hipStream_t st, int block, int cap) {
-  dim3 grid(grid_blocks(make_walk(a.g, block, a.chunk).nblk, a.remap, cap));
+  const bool tm = NL == 4 && a.tmarch > 0;
+  dim3 grid(tm ? tmarch_grid(a.g, block, a.tmarch, a.remap) : grid_blocks(make_walk(a.g, block, a.chunk).nblk, a.remap, cap));
 #define QH_HOP(M, R)                                                                                 \
   if (mode == M && redm == R) {                                                                      \
-    hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R>), grid, dim3(block), 0, st, a);       \
+    if (tm) hipLaunchKernelGGL((k_hop_tm<PL, RC, PI, PO, PA, A, M, R>), grid, dim3(block), 0, st, a);   \
+    else hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R>), grid, dim3(block), 0, st, a);  \
     if (R) {                                                                                         \
       FinArgs f{a.red, (int)grid.x, a.S, nullptr, 0, 6.0 * (double)a.g.Vh, a.pred};                  \
       launch_finish(R == 1 ? EPI_PAP : R == 2 ? EPI_RELIABLE : EPI_TMP, f, st);                      \

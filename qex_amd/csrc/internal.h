@@ -72,6 +72,7 @@ struct HopArgs {
   Scal *S;
   RedWs red;
   long chunk;         // site walk (make_walk): t-march chunk, 0 = canonical order
+  int tmarch;         // > 0 (plain links): t-march form, t-slices per thread (k_hop_tm)
 };
 
 // Fused CG iteration (two kernels): see dslash.hip.

@@ -488,6 +488,13 @@ def set_hop_walk(chunk=-1):
     call("qexhip_set_hop_walk", int(chunk))
 
 
+def set_hop_tmarch(slices=-1):
+    """Pin the t-march form of the plain-link one-parity hop: each thread walks
+    this many t-slices and keeps its forward t-neighbour in registers for the
+    next slice but one; 0 off, -1 lets the tuner choose."""
+    call("qexhip_set_hop_tmarch", int(slices))
+
+
 def set_msb_walk(chunk=-1):
     """Pin the site walk of the batched multi-shift solve's two multi-RHS hops:
     0 canonical order, a multiple of 256 the t-first walk over chunks of that
