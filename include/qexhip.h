@@ -224,6 +224,11 @@ int qexhip_set_hop_multi_shape(int batches, long chunk);
 /* block size of the other-parity stencil kernel of CG forms 3 and 4 (64,
    128 or 256; 0 = chosen by the tuner), independent of the other kernels' */
 int qexhip_set_cg_block1(int block1);
+/* workgroups of the persistent kernels of CG forms 3 and 4, no QEX
+   counterpart: form 3 runs both stencil kernels on at most this many
+   workgroups (0 = one workgroup per block), form 4 its streaming kernel (0 =
+   512); -1 = chosen by the tuner */
+int qexhip_set_cg_cap(int cap);
 /* CG launches: 1 (default) replays HIP graphs, in the mixed-precision CG one per
    reliable-update interval (its iterations, the FP64 reliable update and the
    best-iterate copy); 2 replays graphs of 16 iterations with the reliable

@@ -52,6 +52,7 @@ int g_hop_tmarch = -1;        // one-parity hop of plain links: t-march slices p
 long g_msb_walk = -1;         // batched multi-shift hops: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 double g_ms_sloppy_r2 = 1e-12; // mixed multi-shift: FP32 phase ends at |r|^2 <= this |b|^2 (QEXHIP_MS_SLOPPY_R2)
 int g_cg_block1 = 0;          // CG forms 3/4: other-parity stencil kernel's block size pinned, 0 (tuner)
+int g_cg_cap = -1;            // CG forms 3/4: persistent workgroups pinned (0: form 3 one pass), -1 (tuner)
 
 #define HIPCHK(x)                                                                                  \
   do {                                                                                             \
@@ -461,9 +462,10 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
 
 // Fused CG iteration: the launch shape and iteration form, timed on chunks of
 // a few iterations with scratch vectors and device scalars that never
-// converge.  Forms 2 and 3 over block size x XCD order; form 4 also over the
-// workgroups of its persistent streaming kernel (cap); then, for forms 3 and
-// 4, the block size of the other-parity stencil kernel (block1).  The us
+// converge.  Forms 2 and 3 over block size x XCD order; form 3 also with its
+// stencil kernels persistent (cap workgroups), form 4 over the workgroups of
+// its persistent streaming kernel (cap); then, for forms 3 and 4, the block
+// size of the other-parity stencil kernel (block1).  The us
 // recorded is per iteration.
 const int kTuneIters = 4;
 Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, int rx, const CgArgs &real) {
@@ -496,7 +498,10 @@ Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, int rx, const CgArgs &re
   Tuned best{256, 1, 1e300};
   for (int form : {2, 3, 4}) {
     if (g_cg_form && form != g_cg_form) continue;
-    for (int cap : (form == 4 ? std::vector<int>{256, 512, 1024} : std::vector<int>{0}))
+    // form 4: its persistent streaming kernel's workgroups; form 3: one pass or
+    // both stencil kernels persistent
+    for (int cap : (form == 4 ? std::vector<int>{256, 512, 1024}
+                              : form == 3 ? std::vector<int>{0, 1024, 2048} : std::vector<int>{0}))
       for (int block : {64, 128, 256})
         for (int remap : {1, 0}) {
           Tuned c{block, remap, 0.0, form, cap};
@@ -595,6 +600,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   Tuned tu = tuned_cg(l, pl, rc, ps, rx, a);
   if (g_cg_form) tu.form = g_cg_form;
   if (g_cg_block1) tu.block1 = g_cg_block1;
+  if (g_cg_cap >= 0) tu.cap = g_cg_cap;
   a.remap = tu.remap;
   if (!mixed) {
     a.x = x; a.r = w->r64; a.p = w->p64; a.Ap = w->Ap64; a.t = w->t64;
@@ -2554,6 +2560,13 @@ int qexhip_set_cg_block1(int block1) {
   API_BEGIN
   require(block1 == 0 || block1 == 64 || block1 == 128 || block1 == 256, "block1 must be 0 (tuner), 64, 128 or 256");
   g_cg_block1 = block1;
+  API_END
+}
+
+int qexhip_set_cg_cap(int cap) {
+  API_BEGIN
+  require(cap >= -1 && cap <= (1 << 20), "cg cap must be -1 (tuner), 0 or a workgroup count up to 2^20");
+  g_cg_cap = cap;
   API_END
 }
 

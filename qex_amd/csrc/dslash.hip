@@ -305,6 +305,16 @@ __global__ void __launch_bounds__(256) k_cg1b(CgArgs P) {
   cg1b_body<PL, RC, PS, A, NL>(P, xcd_block(P.remap));
 }
 
+// the same on a persistent grid: each workgroup serves the logical blocks
+// block_iter gives it (an XCD's workgroups walk its contiguous range together)
+template <int PL, int RC, int PS, class A, int NL>
+__global__ void __launch_bounds__(256) k_cg1bp(CgArgs P) {
+  const long nblk = (P.g.Vh + blockDim.x - 1) / blockDim.x;
+  const BlockIter it = block_iter(nblk, P.remap);
+#pragma unroll 1
+  for (long lb = it.cur; lb < it.end; lb += it.step) cg1b_body<PL, RC, PS, A, NL>(P, lb);
+}
+
 template <int PL, int RC, int PS, int PR, class A, int NL>
 __global__ void __launch_bounds__(256) k_cg2b(CgArgs P) {
   __shared__ double lds[16 * 4];
@@ -317,32 +327,33 @@ __global__ void __launch_bounds__(256) k_cg_flush(CgArgs P) {
   cg_flush_body<PS, PR, PX, A>(P, blockIdx.x, lds, blockIdx.x);
 }
 
+// cap > 0 (form 3): both stencil kernels persistent on at most cap workgroups
+// (k_cg1bp, k_cg2bp; the finish then adds cap partial sums)
 template <int PL, int RC, int PS, int PRX, int NL>
-static void cg_iter_t(const CgArgs &a, hipStream_t st, int block, int form, int block1) {
+static void cg_iter_t(const CgArgs &a, hipStream_t st, int block, int form, int block1, int cap) {
   dim3 grid((unsigned)((a.g.Vh + block - 1) / block)), blk(block);
   // the other-parity stencil kernel of form 3 may run its own block size
   const int b1 = block1 > 0 ? block1 : block;
   dim3 grid1((unsigned)((a.g.Vh + b1 - 1) / b1)), blk1(b1);
-  if constexpr (PL == P64) {
-    if (form == 3) {
-      hipLaunchKernelGGL((k_cg0<P64, P64, P64, double2>), grid, blk, 0, st, a);
-      hipLaunchKernelGGL((k_cg1b<PL, RC, P64, double2, NL>), grid1, blk1, 0, st, a);
-      hipLaunchKernelGGL((k_cg2b<PL, RC, P64, P64, double2, NL>), grid, blk, 0, st, a);
+  const bool pers = form == 3 && cap > 0;
+  const dim3 grid2p(grid_blocks(grid.x, a.remap, cap)), grid1p(grid_blocks(grid1.x, a.remap, cap));
+  using A = typename std::conditional<PL == P64, double2, float2>::type;
+  constexpr int PSA = (PL == P64) ? P64 : PS;
+  constexpr int PRA = (PL == P64) ? P64 : PRX;
+  if (form == 3) {
+    hipLaunchKernelGGL((k_cg0<PSA, PRA, PRA, A>), grid, blk, 0, st, a);
+    if (pers) {
+      hipLaunchKernelGGL((k_cg1bp<PL, RC, PSA, A, NL>), grid1p, blk1, 0, st, a);
+      hipLaunchKernelGGL((k_cg2bp<PL, RC, PSA, PRA, A, NL>), grid2p, blk, 0, st, a);
     } else {
-      hipLaunchKernelGGL((k_cg1<PL, RC, P64, P64, P64, double2, NL>), grid, blk, 0, st, a);
-      hipLaunchKernelGGL((k_cg2<PL, RC, P64, P64, P64, double2, NL>), grid, blk, 0, st, a);
+      hipLaunchKernelGGL((k_cg1b<PL, RC, PSA, A, NL>), grid1, blk1, 0, st, a);
+      hipLaunchKernelGGL((k_cg2b<PL, RC, PSA, PRA, A, NL>), grid, blk, 0, st, a);
     }
   } else {
-    if (form == 3) {
-      hipLaunchKernelGGL((k_cg0<PS, PRX, PRX, float2>), grid, blk, 0, st, a);
-      hipLaunchKernelGGL((k_cg1b<PL, RC, PS, float2, NL>), grid1, blk1, 0, st, a);
-      hipLaunchKernelGGL((k_cg2b<PL, RC, PS, PRX, float2, NL>), grid, blk, 0, st, a);
-    } else {
-      hipLaunchKernelGGL((k_cg1<PL, RC, PS, PRX, PRX, float2, NL>), grid, blk, 0, st, a);
-      hipLaunchKernelGGL((k_cg2<PL, RC, PS, PRX, PRX, float2, NL>), grid, blk, 0, st, a);
-    }
+    hipLaunchKernelGGL((k_cg1<PL, RC, PSA, PRA, PRA, A, NL>), grid, blk, 0, st, a);
+    hipLaunchKernelGGL((k_cg2<PL, RC, PSA, PRA, PRA, A, NL>), grid, blk, 0, st, a);
   }
-  FinArgs f{a.red, (int)grid.x, a.S, nullptr, 0, 0.0, 1};
+  FinArgs f{a.red, (int)(pers ? grid2p.x : grid.x), a.S, nullptr, 0, 0.0, 1};
   launch_finish(EPI_CG2, f, st);
 }
 
@@ -388,10 +399,10 @@ static bool cg_chunk4_rc(int rc, int nl, const CgArgs &a, Scal *S1, hipStream_t 
 }
 
 template <int PL, int PS, int PRX = P32>
-static bool cg_iter_rc(int rc, int nl, const CgArgs &a, hipStream_t st, int block, int form, int b1) {
-  if (rc == 18 && nl == 4) { cg_iter_t<PL, 18, PS, PRX, 4>(a, st, block, form, b1); return true; }
-  if (rc == 18 && nl == 8) { cg_iter_t<PL, 18, PS, PRX, 8>(a, st, block, form, b1); return true; }
-  if (rc == 12 && nl == 4) { cg_iter_t<PL, 12, PS, PRX, 4>(a, st, block, form, b1); return true; }
+static bool cg_iter_rc(int rc, int nl, const CgArgs &a, hipStream_t st, int block, int form, int b1, int cap) {
+  if (rc == 18 && nl == 4) { cg_iter_t<PL, 18, PS, PRX, 4>(a, st, block, form, b1, cap); return true; }
+  if (rc == 18 && nl == 8) { cg_iter_t<PL, 18, PS, PRX, 8>(a, st, block, form, b1, cap); return true; }
+  if (rc == 12 && nl == 4) { cg_iter_t<PL, 12, PS, PRX, 4>(a, st, block, form, b1, cap); return true; }
   return false;
 }
 
@@ -399,19 +410,19 @@ static bool cg_iter_rc(int rc, int nl, const CgArgs &a, hipStream_t st, int bloc
 // kernel plus two stencil kernels that gather one vector each.  rx: storage of
 // the sloppy residual and solution increment (P32, or P16 with FP16 vectors).
 void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form, int rx,
-                    int block1) {
+                    int block1, int cap) {
   bool ok = false;
   if (rx == P16 && ps == P16) {
-    if (pl == P16) ok = cg_iter_rc<P16, P16, P16>(rc, nl, a, st, block, form, block1);
-    else if (pl == PQ16) ok = cg_iter_rc<PQ16, P16, P16>(rc, nl, a, st, block, form, block1);
-  } else if (pl == P64 && ps == P64) ok = cg_iter_rc<P64, P64>(rc, nl, a, st, block, form, block1);
-  else if (pl == P32 && ps == P32) ok = cg_iter_rc<P32, P32>(rc, nl, a, st, block, form, block1);
-  else if (pl == P16 && ps == P16) ok = cg_iter_rc<P16, P16>(rc, nl, a, st, block, form, block1);
-  else if (pl == P16 && ps == P32) ok = cg_iter_rc<P16, P32>(rc, nl, a, st, block, form, block1);
-  else if (pl == PBF && ps == PBF) ok = cg_iter_rc<PBF, PBF>(rc, nl, a, st, block, form, block1);
-  else if (pl == PBF && ps == P32) ok = cg_iter_rc<PBF, P32>(rc, nl, a, st, block, form, block1);
-  else if (pl == PQ16 && ps == P16) ok = cg_iter_rc<PQ16, P16>(rc, nl, a, st, block, form, block1);
-  else if (pl == PQ16 && ps == P32) ok = cg_iter_rc<PQ16, P32>(rc, nl, a, st, block, form, block1);
+    if (pl == P16) ok = cg_iter_rc<P16, P16, P16>(rc, nl, a, st, block, form, block1, cap);
+    else if (pl == PQ16) ok = cg_iter_rc<PQ16, P16, P16>(rc, nl, a, st, block, form, block1, cap);
+  } else if (pl == P64 && ps == P64) ok = cg_iter_rc<P64, P64>(rc, nl, a, st, block, form, block1, cap);
+  else if (pl == P32 && ps == P32) ok = cg_iter_rc<P32, P32>(rc, nl, a, st, block, form, block1, cap);
+  else if (pl == P16 && ps == P16) ok = cg_iter_rc<P16, P16>(rc, nl, a, st, block, form, block1, cap);
+  else if (pl == P16 && ps == P32) ok = cg_iter_rc<P16, P32>(rc, nl, a, st, block, form, block1, cap);
+  else if (pl == PBF && ps == PBF) ok = cg_iter_rc<PBF, PBF>(rc, nl, a, st, block, form, block1, cap);
+  else if (pl == PBF && ps == P32) ok = cg_iter_rc<PBF, P32>(rc, nl, a, st, block, form, block1, cap);
+  else if (pl == PQ16 && ps == P16) ok = cg_iter_rc<PQ16, P16>(rc, nl, a, st, block, form, block1, cap);
+  else if (pl == PQ16 && ps == P32) ok = cg_iter_rc<PQ16, P32>(rc, nl, a, st, block, form, block1, cap);
   if (!ok)
     fprintf(stderr, "qexhip: no CG instantiation for pl=%d rc=%d ps=%d rx=%d nl=%d\n", pl, rc, ps, rx, nl);
 }
@@ -421,7 +432,7 @@ void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t
 bool launch_cg_chunk(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form, int rx,
                      int block1, int niter, Scal *S1, int g0) {
   if (form != 4) {
-    for (int k = 0; k < niter; k++) launch_cg_iter(pl, rc, ps, nl, a, st, block, form, rx, block1);
+    for (int k = 0; k < niter; k++) launch_cg_iter(pl, rc, ps, nl, a, st, block, form, rx, block1, g0);
     return true;
   }
   if (rx == P16 && ps == P16) {

@@ -165,8 +165,9 @@ void launch_finish_cg2_to(const RedWs &red, int nb, const Scal *Sin, Scal *Sout,
 
 // dslash.hip
 void launch_hop(const HopKey &k, int nl, int mode, int redm, const HopArgs &a, hipStream_t st, int block, int cap);
+// cap > 0: form 3's stencil kernels persistent on at most cap workgroups
 void launch_cg_iter(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form,
-                    int rx = P32, int block1 = 0);
+                    int rx = P32, int block1 = 0, int cap = 0);
 // niter iterations of CG form 2, 3 or 4 (4: scalars alternate between a.S and
 // S1, the streaming kernel runs g0 workgroups); false if no instantiation
 bool launch_cg_chunk(int pl, int rc, int ps, int nl, const CgArgs &a, hipStream_t st, int block, int form, int rx,

@@ -475,6 +475,13 @@ def set_cg_graph(on=True):
     call("qexhip_set_cg_graph", mode)
 
 
+def set_cg_cap(cap=-1):
+    """Pin the workgroups of CG forms 3 and 4's persistent kernels: form 3 runs
+    both stencil kernels on at most cap workgroups (0: one per block), form 4
+    its streaming kernel (0: 512); -1 lets the tuner choose."""
+    call("qexhip_set_cg_cap", int(cap))
+
+
 def set_cg_block1(block1=0):
     """Pin the block size (64, 128, 256) of the CG iteration's other-parity
     stencil kernel (forms 3 and 4); 0 lets the tuner choose."""

@@ -453,6 +453,44 @@ def test_cg_form4(oracle, gpu, L, sloppy, slinks, recon):
         assert abs(its4 - f3[1]) <= max(2, f3[1] // 50), (its4, f3[1])
 
 
+@pytest.mark.parametrize("cap", [0, 64, 1024])
+def test_cg_form3_persistent(oracle, gpu, cap):
+    """CG form 3 with its two stencil kernels persistent on cap workgroups (64:
+    several blocks per workgroup at 16^4; 1024: about one; 0: one pass): FP64
+    iteration count equal to the oracle's (+-1), true residual by the oracle's
+    operator and the solution to 1e-9; a Q16-link/FP16 mixed solve on the same
+    shape reaches the FP64 true residual."""
+    O = oracle
+    L = (16, 16, 16, 16)
+    g = links(O, L, False)
+    b = O.source_gauss(L, 39, 0, 0)
+    want, its_o, _ = O.solveXX(L, g, 0, b, 0.05, 1e-20)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    gpu.set_cg_form(3)
+    gpu.set_cg_cap(cap)
+    try:
+        fx = lo.ColorVector()
+        sp = gpu.SolverParams()
+        sp.r2req, sp.verbosity = 1e-20, 0
+        gpu.solveEE(s, fx, fb, 0.05, sp)
+        fm = lo.ColorVector()
+        spm = gpu.SolverParams()
+        spm.r2req, spm.verbosity, spm.sloppySolve, spm.sloppyLinks, spm.recon = 1e-20, 0, 2, 4, 12
+        gpu.solveEE(s, fm, fb, 0.05, spm)
+    finally:
+        gpu.set_cg_form(0)
+        gpu.set_cg_cap(-1)
+    assert abs(sp.iterations - its_o) <= 1, (sp.iterations, its_o)
+    x = fx.get()
+    r = b - O.stagD2xx(L, g, 0, x, 0.05 ** 2)
+    assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20
+    assert rel(x[: lo.nEven], want[: lo.nEven]) < 1e-9
+    r = b - O.stagD2xx(L, g, 0, fm.get(), 0.05 ** 2)
+    assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20
+
+
 @pytest.mark.parametrize("form", [3, 4])
 @pytest.mark.parametrize("block1", [64, 256])
 def test_cg_block1_independent_of_block(oracle, gpu, form, block1):
