@@ -20,6 +20,7 @@
 // one; the Lepage term is the in-plane double staple whose retracing paths the
 // -6 cL one-link shift cancels) with the two 5-link staples that feed the same
 // 7-link direction summed before the outer staple, since S_sig is linear.
+#include <algorithm>
 #include <stdexcept>
 
 #include "internal.h"
@@ -32,16 +33,26 @@ struct MF {
   long es, ss;
 };
 
+// Element e of site s of a view: the base of component e (the same for the
+// whole wave) plus a 32-bit byte offset of the site, so that every load takes
+// a scalar base and one offset register per view.  The views stay below
+// 4 GiB: the smearing entry points refuse lattices above kMaxSmearSites
+// (ws_ensure) and the projections of arbitrary link arrays run in chunks.
+__device__ __forceinline__ double2 &mel(const MF &f, int e, long s) {
+  char *b = (char *)(f.p + (long)e * f.es);
+  return *(double2 *)(b + (uint32_t)s * (uint32_t)(f.ss * 16));
+}
+
 __device__ __forceinline__ M3 ldf(const MF &f, long s) {
   M3 m;
 #pragma unroll
-  for (int e = 0; e < 9; e++) m.e[e] = f.p[e * f.es + s * f.ss];
+  for (int e = 0; e < 9; e++) m.e[e] = mel(f, e, s);
   return m;
 }
 
 __device__ __forceinline__ void stf(const MF &f, long s, const M3 &m) {
 #pragma unroll
-  for (int e = 0; e < 9; e++) f.p[e * f.es + s * f.ss] = m.e[e];
+  for (int e = 0; e < 9; e++) mel(f, e, s) = m.e[e];
 }
 
 // a * b^+
@@ -112,7 +123,7 @@ __device__ __forceinline__ void triple(M3 &acc, double a, const MF &L, long il, 
     double2 l[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      l[k] = L.p[(BACK ? 3 * k + i : 3 * i + k) * L.es + il * L.ss];
+      l[k] = mel(L, BACK ? 3 * k + i : 3 * i + k, il);
       if (BACK) l[k].y = -l[k].y;
     }
     double2 t[3];
@@ -382,7 +393,7 @@ __global__ void __launch_bounds__(256) k_relayout(long V, MF i0, MF i1, MF i2, M
   const MF in = mu == 0 ? i0 : (mu == 1 ? i1 : (mu == 2 ? i2 : i3));
   const MF o = mu == 0 ? o0 : (mu == 1 ? o1 : (mu == 2 ? o2 : o3));
 #pragma unroll
-  for (int e = 0; e < 9; e++) o.p[e * o.es + x * o.ss] = in.p[e * in.es + x * in.ss];
+  for (int e = 0; e < 9; e++) mel(o, e, x) = mel(in, e, x);
 }
 
 // ------------------------------------------------------------- chain rule
@@ -430,61 +441,54 @@ __device__ __forceinline__ int third_dir(int a, int b, int c) {
 //   A(z) = C(z) U(z+mu) K(z+lam)^+ + K(z) U(z+mu) C(z+lam)^+
 //   B(z) = C(z)^+ U(z) K(z+lam) + K(z)^+ U(z) C(z+lam),
 // whose four products share C(z), K(z), C(z+lam), K(z+lam).  side_half adds
-// one product of A (BACK false: acc += a L(z) U(z+mu) R(z+lam)^+) or of B
-// (BACK true: acc += a L(z)^+ U(z) R(z+lam)), holding R, U R^+ (or U R) and
-// the accumulator, with L and U read a row at a time.
+// one product of A (BACK false: acc += a L(z) u R(z+lam)^+, u = U(z+mu)) or of
+// B (BACK true: acc += a L(z)^+ u R(z+lam), u = U(z)), with u held by the
+// caller for all terms, R loaded whole and L a row or column at a time.
 template <bool BACK>
-__device__ __forceinline__ void side_half(M3 &acc, double a, const MF &Ul, long zu, const MF &L, long z, const MF &R,
+__device__ __forceinline__ void side_half(M3 &acc, double a, const M3 &u, const MF &L, long z, const MF &R,
                                           long zpl) {
   const M3 r = ldf(R, zpl);
-  M3 x;  // U(zu) R^+ (A) or U(zu) R (B)
+  // one row i of x = U R^+ (A) or U R (B) at a time, folded into acc at once
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    double2 u[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) u[k] = Ul.p[(3 * i + k) * Ul.es + zu * Ul.ss];
+    double2 x[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       double2 t = make_double2(0, 0);
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        if (BACK) {  // u_k r_kj
-          t.x += u[k].x * r.e[3 * k + j].x - u[k].y * r.e[3 * k + j].y;
-          t.y += u[k].x * r.e[3 * k + j].y + u[k].y * r.e[3 * k + j].x;
-        } else {  // u_k conj(r_jk)
-          t.x += u[k].x * r.e[3 * j + k].x + u[k].y * r.e[3 * j + k].y;
-          t.y += u[k].y * r.e[3 * j + k].x - u[k].x * r.e[3 * j + k].y;
+        const double2 uk = u.e[3 * i + k];
+        if (BACK) {  // u_ik r_kj
+          t.x += uk.x * r.e[3 * k + j].x - uk.y * r.e[3 * k + j].y;
+          t.y += uk.x * r.e[3 * k + j].y + uk.y * r.e[3 * k + j].x;
+        } else {  // u_ik conj(r_jk)
+          t.x += uk.x * r.e[3 * j + k].x + uk.y * r.e[3 * j + k].y;
+          t.y += uk.y * r.e[3 * j + k].x - uk.x * r.e[3 * j + k].y;
         }
       }
-      x.e[3 * i + j] = t;
+      x[j] = t;
     }
-  }
-#pragma unroll
-  for (int k = 0; k < 3; k++) {  // row k of L
     double2 l[3];
+    if (BACK) {  // row i of L: acc_kj += a conj(L_ik) x_j
 #pragma unroll
-    for (int i = 0; i < 3; i++) l[i] = L.p[(3 * k + i) * L.es + z * L.ss];
-    if (BACK) {  // acc_ij += a conj(L_ki) x_kj
+      for (int k = 0; k < 3; k++) l[k] = mel(L, 3 * i + k, z);
 #pragma unroll
-      for (int i = 0; i < 3; i++)
+      for (int k = 0; k < 3; k++)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-          const double2 y = x.e[3 * k + j];
-          acc.e[3 * i + j].x += a * (l[i].x * y.x + l[i].y * y.y);
-          acc.e[3 * i + j].y += a * (l[i].x * y.y - l[i].y * y.x);
+          acc.e[3 * k + j].x += a * (l[k].x * x[j].x + l[k].y * x[j].y);
+          acc.e[3 * k + j].y += a * (l[k].x * x[j].y - l[k].y * x[j].x);
         }
-    } else {  // acc_kj += a L_ki x_ij
+    } else {  // column i of L: acc_kj += a L_ki x_j
 #pragma unroll
-      for (int j = 0; j < 3; j++) {
-        double2 t = make_double2(0, 0);
+      for (int k = 0; k < 3; k++) l[k] = mel(L, 3 * k + i, z);
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-          t.x += l[i].x * x.e[3 * i + j].x - l[i].y * x.e[3 * i + j].y;
-          t.y += l[i].x * x.e[3 * i + j].y + l[i].y * x.e[3 * i + j].x;
+      for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          acc.e[3 * k + j].x += a * (l[k].x * x[j].x - l[k].y * x[j].y);
+          acc.e[3 * k + j].y += a * (l[k].x * x[j].y + l[k].y * x[j].x);
         }
-        acc.e[3 * k + j].x += a * t.x;
-        acc.e[3 * k + j].y += a * t.y;
-      }
     }
   }
 }
@@ -567,6 +571,7 @@ __global__ void __launch_bounds__(384, QH_DSIDE_WAVES) k_dside(Geom g, DerivFiel
   M3 acc;
 #pragma unroll
   for (int e = 0; e < 9; e++) acc.e[e] = make_double2(0, 0);
+  const M3 u = ldf(f.U[lam], zu);  // the side link, common to every term
   // terms j: 0 S_lam[U_mu] in S_{mu,lam}; 1 Lepage; 2 7-link; 3, 4 S_lam[S_{mu,rho}]
   // inside T_{mu,sig} for the two sig != lam.  Rolled loops, so that the
   // loads of one product are not hoisted into another's.
@@ -594,8 +599,8 @@ __global__ void __launch_bounds__(384, QH_DSIDE_WAVES) k_dside(Geom g, DerivFiel
     }
 #pragma unroll 1
     for (int h = 0; h < 2; h++) {
-      if (back) side_half<true>(acc, a, f.U[lam], zu, h ? Kf : Cf, z, h ? Cf : Kf, zpl);
-      else side_half<false>(acc, a, f.U[lam], zu, h ? Kf : Cf, z, h ? Cf : Kf, zpl);
+      if (back) side_half<true>(acc, a, u, h ? Kf : Cf, z, h ? Cf : Kf, zpl);
+      else side_half<false>(acc, a, u, h ? Kf : Cf, z, h ? Cf : Kf, zpl);
     }
   }
   const int o = 3 * lam + slot(lam, mu);
@@ -784,7 +789,12 @@ SmearWs::~SmearWs() {
 //   100-107 HISQ force outer products, site-major [k][site] (smear_force_outer).
 enum { WS_SMEAR = 36, WS_DERIV = 100, WS_FORCE = 108 };
 
+// largest lattice of the smearing kernels' 32-bit view offsets (a site-major
+// 3x3 field below 4 GiB); its workspace could not fit in device memory anyway
+const long kMaxSmearSites = 0xffffffffL / 144;
+
 static void ws_ensure(SmearWs &w, long V, int nfields) {
+  if (V > kMaxSmearSites) throw std::runtime_error("HISQ smearing: lattice above 29826161 sites");
   if (w.buf && w.V == V && w.nfields >= nfields) return;
   if (w.buf) (void)hipFree(w.buf);
   w.buf = nullptr;
@@ -888,10 +898,17 @@ void launch_make_imp_links(const Geom &g, SmearWs &w, double *fl, const double *
   }
 }
 
+// the projections of arbitrary arrays of n matrices, in views of at most
+// 2^24 matrices (2.4 GB, within the views' 32-bit offsets)
+const long kProjChunk = 1L << 24;
+
 void launch_project_u(long n, double *u, const double *x, hipStream_t st) {
-  // n matrices as one view of n sites
-  const MF in{(double2 *)const_cast<double *>(x), 1, 9}, out{(double2 *)u, 1, 9};
-  hipLaunchKernelGGL(k_project, dim3(nblk(n), 1), dim3(256), 0, st, n, in, in, in, in, out, out, out, out);
+  for (long o = 0; o < n; o += kProjChunk) {
+    const long c = std::min(kProjChunk, n - o);
+    // c matrices as one view of c sites
+    const MF in{(double2 *)const_cast<double *>(x) + o * 9, 1, 9}, out{(double2 *)u + o * 9, 1, 9};
+    hipLaunchKernelGGL(k_project, dim3(nblk(c), 1), dim3(256), 0, st, c, in, in, in, in, out, out, out, out);
+  }
 }
 
 void launch_hisq_smear(const Geom &g, SmearWs &w, const double first[5], const double second[5], double naik,
@@ -1009,10 +1026,13 @@ void launch_fat7l_deriv(const Geom &g, SmearWs &w, double *deriv, const double *
 
 void launch_project_u_deriv(long n, double *r, const double *u, const double *x, const double *chain,
                             hipStream_t st) {
-  const MF U{(double2 *)const_cast<double *>(u), 1, 9}, X{(double2 *)const_cast<double *>(x), 1, 9};
-  const MF C{(double2 *)const_cast<double *>(chain), 1, 9}, R{(double2 *)r, 1, 9};
-  hipLaunchKernelGGL(k_projderiv, dim3(nblk(n), 1), dim3(256), 0, st, n, U, U, U, U, X, X, X, X, C, C, C, C, R, R, R,
-                     R);
+  for (long o = 0; o < n; o += kProjChunk) {
+    const long c = std::min(kProjChunk, n - o);
+    const MF U{(double2 *)const_cast<double *>(u) + o * 9, 1, 9}, X{(double2 *)const_cast<double *>(x) + o * 9, 1, 9};
+    const MF C{(double2 *)const_cast<double *>(chain) + o * 9, 1, 9}, R{(double2 *)r + o * 9, 1, 9};
+    hipLaunchKernelGGL(k_projderiv, dim3(nblk(c), 1), dim3(256), 0, st, c, U, U, U, U, X, X, X, X, C, C, C, C, R, R,
+                       R, R);
+  }
 }
 
 void launch_hisq_smear_deriv(const Geom &g, SmearWs &w, const double first[5], const double second[5], double naik,
