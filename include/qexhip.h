@@ -210,6 +210,10 @@ int qexhip_set_hop_walk(long chunk);
    for the next slice but one, where it is the backward one; 0 = off (the site
    walk of qexhip_set_hop_walk applies), -1 = chosen by the tuner */
 int qexhip_set_hop_tmarch(int slices);
+/* stencil loop of the one-parity hop, no QEX counterpart: 1 = both links of a
+   hop per trip (more loads in flight per wave, for lattices too small to fill
+   the device), 0 = one link per trip, -1 = chosen by the tuner */
+int qexhip_set_hop_two(int two);
 /* site walk of the two multi-RHS hops of the batched multi-shift solve
    (qexhip_solve_multishift_batch), no QEX counterpart: same values as
    qexhip_set_hop_walk; -1 = chosen by the tuner (with QEXHIP_BATCH_TUNE, else

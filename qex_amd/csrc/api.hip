@@ -49,6 +49,7 @@ int g_multi_batches = 0;      // multi-RHS hop: vector-load batches per link pin
 long g_multi_chunk = -1;      // multi-RHS hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 long g_hop_chunk = -1;        // one-parity hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 int g_hop_tmarch = -1;        // one-parity hop of plain links: t-march slices per thread pinned (0: off) or -1 (tuner)
+int g_hop_two = -1;           // one-parity hop: both links of a hop per stencil trip pinned (0/1) or -1 (tuner)
 long g_msb_walk = -1;         // batched multi-shift hops: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 double g_ms_sloppy_r2 = 1e-12; // mixed multi-shift: FP32 phase ends at |r|^2 <= this |b|^2 (QEXHIP_MS_SLOPPY_R2)
 int g_cg_block1 = 0;          // CG forms 3/4: other-parity stencil kernel's block size pinned, 0 (tuner)
@@ -266,6 +267,7 @@ struct Tuned {
   int batches = 1;  // multi-RHS hop: vector-load batches per link
   long chunk = 0;   // site walk (make_walk): t-march chunk, 0 = canonical order
   int tmarch = 0;   // hop of plain links: t-slices per thread of the t-march form (0: off)
+  int two = 0;      // hop: both links of a hop per stencil trip
 };
 std::map<std::string, Tuned> g_tune;
 
@@ -362,6 +364,7 @@ void hop_shaped(qexhip_links *l, const char *lk, const Tuned &tu, int prec, int 
   h.red = l->lat->red;
   h.chunk = tu.chunk;
   h.tmarch = l->nl == 4 ? tu.tmarch : 0;
+  h.two = tu.two;
   const int pv = (prec == PQ16) ? P16 : prec;  // 16-bit fixed-point links go with FP16 vectors
   HopKey k{prec, rc, pv, pv, pv};
   launch_hop(k, l->nl, mode, redm, h, g_stream, tu.block, tu.cap);
@@ -375,9 +378,9 @@ void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const v
 
 std::string hop_key(const qexhip_links *l, int prec, int rc) {
   char kb[96];
-  if (g_hop_chunk >= 0 || g_hop_tmarch >= 0)
-    snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d walk=%ld tmarch=%d ", l->nl, prec, rc, g_hop_chunk,
-             g_hop_tmarch);
+  if (g_hop_chunk >= 0 || g_hop_tmarch >= 0 || g_hop_two >= 0)
+    snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d walk=%ld tmarch=%d two=%d ", l->nl, prec, rc, g_hop_chunk,
+             g_hop_tmarch, g_hop_two);
   else
     snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d ", l->nl, prec, rc);
   return kb + geom_key(l->lat->g);
@@ -391,6 +394,7 @@ Tuned cached_hop_shape(const qexhip_links *l, int prec, int rc) {
   Tuned t{g_block, g_remap, 0.0, 2, 0};
   t.chunk = g_hop_chunk > 0 ? g_hop_chunk : 0;
   t.tmarch = g_hop_tmarch > 0 ? g_hop_tmarch : 0;
+  t.two = g_hop_two > 0 ? 1 : 0;
   return t;
 }
 
@@ -424,16 +428,20 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
       if (chunk && g_hop_chunk < 0 && make_walk(h.g, 256, chunk).C == 0) continue;  // t-slices too small
       for (int cap : {0, 1024, 2048})
         for (int block : {64, 128, 256})
-          for (int remap : {1, 0}) {
-            h.remap = remap;
-            h.chunk = chunk;
-            h.tmarch = 0;
-            const double us = time_us([&] { launch_hop(hk, l->nl, 0, 0, h, g_stream, block, cap); }, 2, 8);
-            if (us < best.us) {
-              best = Tuned{block, remap, us, 2, cap};
-              best.chunk = chunk;
+          for (int remap : {1, 0})
+            for (int two : {0, 1}) {
+              if (g_hop_two >= 0 && two != g_hop_two) continue;
+              h.remap = remap;
+              h.chunk = chunk;
+              h.tmarch = 0;
+              h.two = two;
+              const double us = time_us([&] { launch_hop(hk, l->nl, 0, 0, h, g_stream, block, cap); }, 2, 8);
+              if (us < best.us) {
+                best = Tuned{block, remap, us, 2, cap};
+                best.chunk = chunk;
+                best.two = two;
+              }
             }
-          }
     }
   // the t-march form (plain links): t-slices per thread x block size x XCD order
   std::vector<int> tms = {4, 8, 16};
@@ -446,6 +454,7 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
           h.remap = remap;
           h.chunk = 0;
           h.tmarch = tm;
+          h.two = 0;
           const double us = time_us([&] { launch_hop(hk, l->nl, 0, 0, h, g_stream, block, 0); }, 2, 8);
           if (us < best.us) {
             best = Tuned{block, remap, us, 2, 0};
@@ -2577,6 +2586,13 @@ int qexhip_set_hop_walk(long chunk) {
   API_END
 }
 
+int qexhip_set_hop_two(int two) {
+  API_BEGIN
+  require(two >= -1 && two <= 1, "hop stencil trip must be -1 (tuner), 0 (one link) or 1 (both links of a hop)");
+  g_hop_two = two;
+  API_END
+}
+
 int qexhip_set_hop_tmarch(int slices) {
   API_BEGIN
   require(slices >= -1 && slices <= 1024, "t-march slices per thread must be -1 (tuner), 0 (off) or 1..1024");
@@ -2621,9 +2637,10 @@ int qexhip_autotune_report(char *buf, int len) {
   std::string out;
   for (auto &kv : g_tune) {
     char line[256];
-    snprintf(line, sizeof line, "%s: block=%d remap=%d form=%d cap=%d block1=%d batches=%d chunk=%ld us=%.2f\n",
+    snprintf(line, sizeof line,
+             "%s: block=%d remap=%d form=%d cap=%d block1=%d batches=%d chunk=%ld tmarch=%d two=%d us=%.2f\n",
              kv.first.c_str(), kv.second.block, kv.second.remap, kv.second.form, kv.second.cap, kv.second.block1,
-             kv.second.batches, kv.second.chunk, kv.second.us);
+             kv.second.batches, kv.second.chunk, kv.second.tmarch, kv.second.two, kv.second.us);
     out += line;
   }
   snprintf(buf, (size_t)len, "%s", out.c_str());

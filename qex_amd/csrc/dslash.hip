@@ -22,7 +22,11 @@ namespace qh {
 #ifndef QH_HOP_WAVES
 #define QH_HOP_WAVES 1  // minimum waves per SIMD asked of the register allocator
 #endif
-template <int PL, int RC, int PI, int PO, int PA, class A, int NL, int MODE, int RED>
+// TWO: the stencil loop takes the two links of a hop per trip (form -1: twice
+// the loads in flight per wave, 10-20 more registers): 17-27% faster at 16^4,
+// where too few waves fill the device to hide latency otherwise, and 1% at 48^4,
+// but 1% slower at 32^4; a tuned launch dimension.
+template <int PL, int RC, int PI, int PO, int PA, class A, int NL, int MODE, int RED, int TWO = 0>
 __global__ void __launch_bounds__(256, QH_HOP_WAVES) k_hop(HopArgs P) {
   using R = RT<A>;
   __shared__ double lds[16];
@@ -41,7 +45,7 @@ __global__ void __launch_bounds__(256, QH_HOP_WAVES) k_hop(HopArgs P) {
 #pragma unroll
       for (int c = 0; c < 3; c++) v[c] = ldc<PI, A>(P.in, vix(g, c, j));
     };
-    stencil<PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, P.par, i, s, ld, res);
+    stencil_in<TWO ? -1 : stencil_form<PL>(), PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, P.par, i, s, ld, res);
     const R chop = (R)P.chop;
 #pragma unroll
     for (int c = 0; c < 3; c++) res[c] = mk<A>(chop * res[c].x, chop * res[c].y);
@@ -221,6 +225,7 @@ static bool hop_modes(int mode, int redm, const HopArgs &a, hipStream_t st, int 
 #define QH_HOP(M, R)                                                                                 \
   if (mode == M && redm == R) {                                                                      \
     if (tm) hipLaunchKernelGGL((k_hop_tm<PL, RC, PI, PO, PA, A, M, R>), grid, dim3(block), 0, st, a);   \
+    else if (a.two) hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R, 1>), grid, dim3(block), 0, st, a); \
     else hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R>), grid, dim3(block), 0, st, a);  \
     if (R) {                                                                                         \
       FinArgs f{a.red, (int)grid.x, a.S, nullptr, 0, 6.0 * (double)a.g.Vh, a.pred};                  \

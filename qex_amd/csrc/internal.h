@@ -73,6 +73,7 @@ struct HopArgs {
   RedWs red;
   long chunk;         // site walk (make_walk): t-march chunk, 0 = canonical order
   int tmarch;         // > 0 (plain links): t-march form, t-slices per thread (k_hop_tm)
+  int two;            // 1: the stencil loop takes both links of a hop per trip (k_hop's TWO)
 };
 
 // Fused CG iteration (two kernels): see dslash.hip.

@@ -58,11 +58,11 @@ template <int PL> constexpr int stencil_form() {
   return (PL == P16 || PL == PBF || PL == PQ16) ? QH_PIPE16 : QH_PIPE;
 }
 
-template <int PL, int RC, int NL, class A, class LD>
-__device__ __forceinline__ void stencil(const Geom &g, const char *links, long lkblk, const float *qs, int par,
-                                        long i, const Site &s, LD &&ld, A (&res)[3]) {
+// stencil in form D (the forms above); stencil() takes the build's form for PL
+template <int D, int PL, int RC, int NL, class A, class LD>
+__device__ __forceinline__ void stencil_in(const Geom &g, const char *links, long lkblk, const float *qs, int par,
+                                           long i, const Site &s, LD &&ld, A (&res)[3]) {
   constexpr int NJ = 2 * NL;
-  constexpr int D = stencil_form<PL>();
   A af[3], ab[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) { af[c] = mk<A>(0, 0); ab[c] = mk<A>(0, 0); }
@@ -133,6 +133,12 @@ __device__ __forceinline__ void stencil(const Geom &g, const char *links, long l
   }
 #pragma unroll
   for (int c = 0; c < 3; c++) res[c] = mk<A>(af[c].x - ab[c].x, af[c].y - ab[c].y);
+}
+
+template <int PL, int RC, int NL, class A, class LD>
+__device__ __forceinline__ void stencil(const Geom &g, const char *links, long lkblk, const float *qs, int par,
+                                        long i, const Site &s, LD &&ld, A (&res)[3]) {
+  stencil_in<stencil_form<PL>(), PL, RC, NL, A>(g, links, lkblk, qs, par, i, s, ld, res);
 }
 
 // value after a round trip through storage precision P (exact for P32/P64)

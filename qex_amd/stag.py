@@ -495,6 +495,12 @@ def set_hop_walk(chunk=-1):
     call("qexhip_set_hop_walk", int(chunk))
 
 
+def set_hop_two(two=-1):
+    """Pin the one-parity hop's stencil loop: 1 both links of a hop per trip,
+    0 one link per trip; -1 lets the tuner choose."""
+    call("qexhip_set_hop_two", int(two))
+
+
 def set_hop_tmarch(slices=-1):
     """Pin the t-march form of the plain-link one-parity hop: each thread walks
     this many t-slices and keeps its forward t-neighbour in registers for the
@@ -524,12 +530,14 @@ def set_cg_form(form=0):
 
 
 def autotune_report():
-    """{kernel family and shape: {"block", "remap", "form", "cap", "block1", "batches", "chunk", "us"}}
-    chosen so far (form: 2- or 3-kernel CG iteration; cap: workgroup cap of the
-    persistent hop, 0 = one pass; block1: block size of the 3-kernel iteration's
-    other-parity stencil kernel, 0 = block; batches: vector-load batches per link
-    of the multi-RHS hop; chunk: its t-first site-walk chunk, 0 = canonical
-    order; us: time of one timed unit)."""
+    """{kernel family and shape: {"block", "remap", "form", "cap", "block1", "batches", "chunk",
+    "tmarch", "two", "us"}} chosen so far (form: 2-, 3- or 4-kernel CG iteration;
+    cap: workgroups of a persistent form, 0 = one pass; block1: block size of
+    the 3-kernel iteration's other-parity stencil kernel, 0 = block; batches:
+    vector-load batches per link of the multi-RHS hop; chunk: t-first site-walk
+    chunk, 0 = canonical order; tmarch: t-slices per thread of the hop's
+    t-march form, 0 = off; two: the hop's stencil loop takes both links of a
+    hop per trip; us: time of one timed unit)."""
     buf = ctypes.create_string_buffer(1 << 16)
     call("qexhip_autotune_report", buf, len(buf))
     out = {}
@@ -538,7 +546,8 @@ def autotune_report():
         f = dict(kv.split("=") for kv in rest.split())
         out[key] = {"block": int(f["block"]), "remap": int(f["remap"]), "form": int(f["form"]),
                     "cap": int(f["cap"]), "block1": int(f.get("block1", 0)),
-                    "batches": int(f.get("batches", 1)), "chunk": int(f.get("chunk", 0)), "us": float(f["us"])}
+                    "batches": int(f.get("batches", 1)), "chunk": int(f.get("chunk", 0)),
+                    "tmarch": int(f.get("tmarch", 0)), "two": int(f.get("two", 0)), "us": float(f["us"])}
     return out
 
 

@@ -229,6 +229,42 @@ def test_stream_read_probe(gpu):
     assert 500.0 < gbs < 8000.0
 
 
+@pytest.mark.parametrize("L", [(8, 4, 6, 4), (8, 8, 4, 8), (16, 16, 16, 16)])
+@pytest.mark.parametrize("hisq", [False, True])
+def test_hop_two_links_per_trip(oracle, gpu, L, hisq):
+    """The hop's stencil loop taking both links of a hop per trip (the form the
+    tuner picks for small lattices) equals the oracle: stagD2 on each parity
+    and both in FP64 and FP32, and FP64 and mixed CG solves (whose true
+    residuals run the reducing hops) to the oracle's residual."""
+    O = oracle
+    g = make_links(O, L, hisq)
+    gpu.set_hop_two(1)
+    try:
+        lo = gpu.Layout(L)
+        s = gpu.Staggered(lo, g)
+        x = O.source_gauss(L, 61, 0, 2)
+        r0 = O.source_gauss(L, 62, 1, 2)
+        for prec, tol in ((gpu.FP64, 1e-12), (gpu.FP32, 1e-5)):
+            for par in (0, 1, 2):
+                fx = lo.ColorVector(prec).set(x)
+                fr = lo.ColorVector(prec).set(r0)
+                gpu.stagD2(sd_of(s, par), fr, s.g, fx, 0.5, -0.25)
+                want = r0.copy()
+                O.stagD2(L, g, par, want, x, 0.5, -0.25)
+                assert rel(fr.get(), want) < tol, (prec, par)
+        b = O.source_gauss(L, 63, 0, 0)
+        fb = lo.ColorVector().set(b)
+        for sloppy in (0, 1):
+            fs = lo.ColorVector()
+            sp = gpu.SolverParams()
+            sp.r2req, sp.verbosity, sp.sloppySolve = 1e-20, 0, sloppy
+            gpu.solveEE(s, fs, fb, 0.1, sp)
+            r = b - O.stagD2xx(L, g, 0, fs.get(), 0.01)
+            assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20, sloppy
+    finally:
+        gpu.set_hop_two(-1)
+
+
 @pytest.mark.parametrize("L", [(8, 4, 6, 4), (6, 4, 8, 10), (8, 8, 4, 8), (4, 4, 4, 2)])
 @pytest.mark.parametrize("tmarch", [1, 3, 4, 16])
 def test_hop_tmarch(oracle, gpu, L, tmarch):
