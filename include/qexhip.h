@@ -199,6 +199,15 @@ int qexhip_autotune_report(char *buf, int len);
    product finish folded into the next iteration's streaming kernel (single
    systems; batches take 3); 0 = chosen by the tuner */
 int qexhip_set_cg_form(int form);
+/* link bytes that the stencils of lattice lat read with cache-allocating
+   loads, no QEX counterpart: the links of the first directions whose two
+   parity blocks fit in this many bytes are read so that they can stay in the
+   last-level cache from one sweep to the next; the rest, with the
+   non-temporal hint.  0 = all non-temporal; -1 = chosen by the tuner per
+   kernel family and link format (the default).  With
+   lat = NULL, sets the default for lattices created afterwards (-1 restores
+   the built-in one) */
+int qexhip_set_link_cache(qexhip_lattice *lat, long bytes);
 /* site walk of the one-parity hop of every operator and solver call, no QEX
    counterpart: 0 = canonical order, a multiple of 256 = t-slices cut into
    chunks of that many sites and walked t first (their t +- 1 neighbours are

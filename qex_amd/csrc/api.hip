@@ -54,6 +54,12 @@ long g_msb_walk = -1;         // batched multi-shift hops: site-walk chunk pinne
 double g_ms_sloppy_r2 = 1e-12; // mixed multi-shift: FP32 phase ends at |r|^2 <= this |b|^2 (QEXHIP_MS_SLOPPY_R2)
 int g_cg_block1 = 0;          // CG forms 3/4: other-parity stencil kernel's block size pinned, 0 (tuner)
 int g_cg_cap = -1;            // CG forms 3/4: persistent workgroups pinned (0: form 3 one pass), -1 (tuner)
+// Link bytes the stencils of a new lattice read with cache-allocating loads
+// (Geom::lkcache), or -1: chosen per kernel family and link format by the
+// tuner (link_cache_budgets), 0 where nothing is tuned.  QEXHIP_LINK_CACHE_MB
+// at init pins a value for every lattice.
+constexpr long kLinkCacheDefault = -1L;  // chosen by the tuner
+long g_link_cache = kLinkCacheDefault;
 
 #define HIPCHK(x)                                                                                  \
   do {                                                                                             \
@@ -268,8 +274,23 @@ struct Tuned {
   long chunk = 0;   // site walk (make_walk): t-march chunk, 0 = canonical order
   int tmarch = 0;   // hop of plain links: t-slices per thread of the t-march form (0: off)
   int two = 0;      // hop: both links of a hop per stencil trip
+  long lkcache = 0; // link bytes read with cache-allocating loads (Geom::lkcache), for lattices that leave it to the tuner
 };
 std::map<std::string, Tuned> g_tune;
+
+// Candidate link-cache budgets (Geom::lkcache) of the stencil tuners.
+// Measured at 32^4 (CG, us per iteration, budget 0 / best): Q16 12-real links
+// 70.7 / 63.2 (128 MB: the whole 100 MB field cached), FP16 18-real 82.0 /
+// 75.1, FP32 139.1 / 125.7 (192 MB: two of four directions), FP64 252.4 /
+// 233.7 (192 MB: one direction, 151 MB); at 48^4, where the vectors alone
+// nearly fill the 256 MB last-level cache, 0 was best (Q16 12-real 297.9 us,
+// 2% slower at 128 MB; FP16 18-real 377.7 us, 6% slower at 192 MB).
+const long kLinkCacheMB[] = {0, 64, 128, 192, 256};
+
+// the link-cache budget a launch with tuned shape tu uses on lattice lat
+long link_cache_of(const qexhip_lattice *lat, const Tuned &tu) {
+  return lat->g.lkcache >= 0 ? lat->g.lkcache : tu.lkcache;
+}
 
 std::string geom_key(const Geom &g) {
   char b[96];
@@ -365,6 +386,7 @@ void hop_shaped(qexhip_links *l, const char *lk, const Tuned &tu, int prec, int 
   h.chunk = tu.chunk;
   h.tmarch = l->nl == 4 ? tu.tmarch : 0;
   h.two = tu.two;
+  h.g.lkcache = link_cache_of(l->lat, tu);
   const int pv = (prec == PQ16) ? P16 : prec;  // 16-bit fixed-point links go with FP16 vectors
   HopKey k{prec, rc, pv, pv, pv};
   launch_hop(k, l->nl, mode, redm, h, g_stream, tu.block, tu.cap);
@@ -462,6 +484,23 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
           }
         }
     }
+  // the link-cache budget, on the chosen shape (the t-march form and the
+  // two-links trip read every link non-temporally)
+  if (l->lat->g.lkcache < 0 && !best.tmarch && !best.two) {
+    h.remap = best.remap;
+    h.chunk = best.chunk;
+    h.tmarch = 0;
+    h.two = 0;
+    double bus = 1e300;
+    for (long mb : kLinkCacheMB) {
+      h.g.lkcache = mb << 20;
+      const double us = time_us([&] { launch_hop(hk, l->nl, 0, 0, h, g_stream, best.block, best.cap); }, 2, 8);
+      if (us < bus) {
+        bus = us;
+        best.lkcache = mb << 20;
+      }
+    }
+  }
   g_tune[key] = best;
   HIPCHK(hipStreamSynchronize(g_stream));
   (void)hipFree(in);
@@ -534,6 +573,19 @@ Tuned tuned_cg(qexhip_links *l, int pl, int rc, int ps, int rx, const CgArgs &re
       }
     }
     best = b;
+  }
+  // the link-cache budget, on the chosen shape
+  if (l->lat->g.lkcache < 0) {
+    double bus = 1e300;
+    for (long mb : kLinkCacheMB) {
+      a.g.lkcache = mb << 20;
+      const double us = time_us([&] { run(best); }, 2, 8);
+      if (us < bus) {
+        bus = us;
+        best.lkcache = mb << 20;
+      }
+    }
+    best.us = bus;
   }
   best.us /= kTuneIters;
   g_tune[key] = best;
@@ -611,6 +663,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   if (g_cg_block1) tu.block1 = g_cg_block1;
   if (g_cg_cap >= 0) tu.cap = g_cg_cap;
   a.remap = tu.remap;
+  a.g.lkcache = link_cache_of(lat, tu);
   if (!mixed) {
     a.x = x; a.r = w->r64; a.p = w->p64; a.Ap = w->Ap64; a.t = w->t64;
     launch_fill(w->Ap64, vec64_bytes(lat), g_stream);
@@ -1514,6 +1567,7 @@ int qexhip_init(int device) {
   if (const char *r = getenv("QEXHIP_XCD_REMAP")) g_remap = atoi(r) != 0;
   if (const char *w = getenv("QEXHIP_SMEAR_WALK")) g_smear_walk = atoi(w) != 0;
   if (const char *w = getenv("QEXHIP_MS_SLOPPY_R2")) g_ms_sloppy_r2 = atof(w);
+  if (const char *c = getenv("QEXHIP_LINK_CACHE_MB")) g_link_cache = atol(c) << 20;
   API_END
 }
 
@@ -1570,6 +1624,7 @@ int qexhip_lattice_create(const int dims[4], qexhip_lattice **out) {
   lat->V = (long)dims[0] * dims[1] * dims[2] * dims[3];
   lat->g.Vh = lat->V / 2;
   lat->g.stride = (lat->g.Vh + 255) / 256 * 256;
+  lat->g.lkcache = g_link_cache;
   lat->red.maxblocks = (int)((lat->g.stride + 63) / 64) + 64;
   lat->red.partials = dalloc<double>(sizeof(double) * 4 * lat->red.maxblocks);
   lat->S = dalloc<Scal>(sizeof(Scal));
@@ -2579,6 +2634,17 @@ int qexhip_set_cg_cap(int cap) {
   API_END
 }
 
+int qexhip_set_link_cache(qexhip_lattice *lat, long bytes) {
+  API_BEGIN
+  require(bytes >= -1, "link cache bytes must be -1 (default) or >= 0");
+  if (lat) {
+    lat->g.lkcache = bytes < 0 ? g_link_cache : bytes;
+  } else {
+    g_link_cache = bytes < 0 ? kLinkCacheDefault : bytes;
+  }
+  API_END
+}
+
 int qexhip_set_hop_walk(long chunk) {
   API_BEGIN
   require(chunk == -1 || (chunk >= 0 && chunk % 256 == 0), "site-walk chunk must be -1 (tuner), 0 or a multiple of 256");
@@ -2636,13 +2702,17 @@ int qexhip_autotune_report(char *buf, int len) {
   require(buf && len > 0, "bad buffer");
   std::string out;
   for (auto &kv : g_tune) {
+    // the key can be long (a batch lists every lattice): it is appended whole
     char line[256];
     snprintf(line, sizeof line,
-             "%s: block=%d remap=%d form=%d cap=%d block1=%d batches=%d chunk=%ld tmarch=%d two=%d us=%.2f\n",
-             kv.first.c_str(), kv.second.block, kv.second.remap, kv.second.form, kv.second.cap, kv.second.block1,
-             kv.second.batches, kv.second.chunk, kv.second.tmarch, kv.second.two, kv.second.us);
+             ": block=%d remap=%d form=%d cap=%d block1=%d batches=%d chunk=%ld tmarch=%d two=%d lkcache=%ld us=%.2f\n",
+             kv.second.block, kv.second.remap, kv.second.form, kv.second.cap, kv.second.block1, kv.second.batches,
+             kv.second.chunk, kv.second.tmarch, kv.second.two, kv.second.lkcache >> 20, kv.second.us);
+    out += kv.first;
     out += line;
   }
+  if (out.size() >= (size_t)len)
+    throw std::runtime_error("autotune report buffer too small: " + std::to_string(out.size() + 1) + " bytes needed");
   snprintf(buf, (size_t)len, "%s", out.c_str());
   API_END
 }

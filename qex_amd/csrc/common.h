@@ -27,7 +27,7 @@
 // slower in the CG.  Waves per SIMD, not instruction issue or loads in flight
 // per wave, set these kernels' speed.
 #ifndef QH_PIPE16
-#define QH_PIPE16 -3
+#define QH_PIPE16 -5
 #endif
 
 namespace qh {
@@ -170,6 +170,7 @@ struct Geom {
   int hx;        // Lx/2
   long Vh;       // sites per parity
   long stride;   // padded sites per parity in the SoA layout
+  long lkcache;  // link bytes the stencils read with cache-allocating loads (link_cached_dirs)
 };
 
 // Element index of color component c of site i in a vector field.  Blocked
@@ -467,8 +468,9 @@ __host__ __device__ __forceinline__ uint32_t link_off(long stride, long i, int p
 }
 
 // The stored reals of one link (LFmt layout) into U: rows 0-1 for 12-real
-// storage, all of it for 18.
-template <int PL, int RC, bool NT, class A, bool BLK, bool WIDE>
+// storage, all of it for 18.  QS = false leaves PQ16 values unscaled (the
+// integers as floats; the caller applies the field's scale).
+template <int PL, int RC, bool NT, class A, bool BLK, bool WIDE, bool QS = true>
 __device__ __forceinline__ void ld_link_planes(const char *blk0, long stride, long i, A (&U)[9], float qscale) {
   using F = LFmt<PL, RC>;
   const char *blk = link_base<PL, RC, BLK, WIDE>(blk0, i);
@@ -512,7 +514,10 @@ __device__ __forceinline__ void ld_link_planes(const char *blk0, long stride, lo
         const float2 f = __half22float2(h);
         U[c] = mk<A>(f.x, f.y);
       } else if constexpr (PL == PQ16) {
-        U[c] = mk<A>((float)(int16_t)(w[c] & 0xffffu) * qscale, (float)(int16_t)(w[c] >> 16) * qscale);
+        if constexpr (QS)
+          U[c] = mk<A>((float)(int16_t)(w[c] & 0xffffu) * qscale, (float)(int16_t)(w[c] >> 16) * qscale);
+        else
+          U[c] = mk<A>((float)(int16_t)(w[c] & 0xffffu), (float)(int16_t)(w[c] >> 16));
       } else {
         U[c] = mk<A>(__uint_as_float(w[c] << 16), __uint_as_float(w[c] & 0xffff0000u));
       }

@@ -17,14 +17,29 @@ template <class A> struct Leg {
   A v[3];
 };
 
-template <int PL, int RC, int NL, class A, class LD>
+// Cache policy of the stencils' link loads.  A sweep reads every link once,
+// so by default links are loaded with the non-temporal hint, which keeps the
+// gathered color vectors resident in L2 and the last-level cache.  But a
+// solver sweeps the same links again and again, and a link field that fits in
+// the 256 MB last-level cache (16-bit 12-real links: 100 MB at 32^4) can stay
+// there from one sweep to the next if it is read with plain loads.  Geom's
+// lkcache gives the link bytes to read that way: the links of directions
+// k < link_cached_dirs (both parity blocks of a direction, 2 lkblk bytes),
+// the rest non-temporal.  Each range has a loop of its own, so the two kinds
+// of load never meet in one instruction.
+template <int NL> __device__ __forceinline__ int link_cached_dirs(const Geom &g, long lkblk) {
+  int m = 0;
+  while (m < NL && (long)(m + 1) * 2 * lkblk <= g.lkcache) m++;
+  return m;
+}
+template <int PL, int RC, int NL, class A, class LD, bool NT = true>
 __device__ __forceinline__ void leg_load(const Geom &g, const char *links, long lkblk, const float *qs, int par,
                                          long i, const Site &s, LD &ld, int j, Leg<A> &L) {
   const int k = j >> 1, fwd = !(j & 1);
   const int mu = (NL == 4) ? k : (k >> 1);
   const int len = (NL == 8 && (k & 1)) ? 3 : 1;
   const long n = nbr(g, s, mu, fwd ? len : -len);
-  ld_link<PL, RC, true, A, link_blocked<PL, NL>()>(links + (long)(2 * k + (fwd ? par : 1 - par)) * lkblk, g.stride, fwd ? i : n, L.U,
+  ld_link<PL, RC, NT, A, link_blocked<PL, NL>()>(links + (long)(2 * k + (fwd ? par : 1 - par)) * lkblk, g.stride, fwd ? i : n, L.U,
                            (RT<A>)(RC == 12 ? link_sign(g, s, mu, fwd) : 1.0f), PL == PQ16 ? qs[k] : 0.0f);
   ld(n, L.v);
 }
@@ -58,10 +73,152 @@ template <int PL> constexpr int stencil_form() {
   return (PL == P16 || PL == PBF || PL == PQ16) ? QH_PIPE16 : QH_PIPE;
 }
 
+// ---- form -5: lean loop for compact links.  One direction per trip, its
+// forward and then its backward link, each link's loads and arithmetic in one
+// basic block (no branch on the link's orientation), so that 16-bit values
+// enter the fused multiply-adds unconverted (FP16 operands of an FP32 FMA).
+// Neighbour indices come from 32-bit per-site bits and wave-uniform steps
+// instead of a decode per link, the staggered signs likewise, and the PQ16
+// scale of a direction multiplies that direction's sum once instead of every
+// link real (the integers enter the products as they are; the rebuilt row 2,
+// quadratic in them, takes one factor of the scale with its sign).
+
+// Per-site bits of the lean form.  wrap: bit 2k / 2k+1 set when the forward /
+// backward neighbour of direction k (NL = 4: k = mu; NL = 8: k = 2 mu for the
+// one-link hop, 2 mu + 1 for the three-link hop) lies across the lattice
+// edge.  sgn: bit 2 mu / 2 mu + 1 set when link_sign(g, s, mu, 1 / 0) is -1.
+template <int NL>
+__device__ __forceinline__ void lean_bits(const Geom &g, const Site &s, uint32_t &wrap, uint32_t &sgn) {
+  wrap = 0u;
+#pragma unroll
+  for (int k = 0; k < NL; k++) {
+    const int mu = (NL == 4) ? k : (k >> 1);
+    const int len = (NL == 8 && (k & 1)) ? 3 : 1;
+    const int c = mu == 0 ? s.x : mu == 1 ? s.y : mu == 2 ? s.z : s.t;
+    wrap |= (uint32_t)(c + len >= g.L[mu]) << (2 * k);
+    wrap |= (uint32_t)(c < len) << (2 * k + 1);
+  }
+  const uint32_t e0 = (uint32_t)s.t, e1 = (uint32_t)(s.x + s.t), e2 = (uint32_t)(s.x + s.y + s.t);
+  sgn = (e0 & 1u) * 3u | ((e1 & 1u) * 3u) << 2 | ((e2 & 1u) * 3u) << 4 |
+        (uint32_t)(s.t == g.L[3] - 1) << 6 | (uint32_t)(s.t == 0) << 7;
+}
+
+// sign bit (bit 31) of link_sign for orientation bit b = 2 mu + (backward)
+__device__ __forceinline__ uint32_t lean_signbit(uint32_t sgn, int b) { return (sgn << (31 - b)) & 0x80000000u; }
+
+// One link of the lean form: row 2 of a 12-real link rebuilt as
+// f * conj(row0 x row1), f = the link's sign (times the PQ16 scale: rows 0-1
+// are then unscaled, and so is the rebuilt row relative to the true one).
+// The products start from fma(a, b, 0) so that 16-bit operands stay
+// unconverted in every one of them.
+template <int PL, int RC, class A, bool BLK, bool NT>
+__device__ __forceinline__ void lean_link(const char *blk, long stride, long i, RT<A> f, A (&U)[9]) {
+  if (link_wide<PL, RC, BLK>(stride))
+    ld_link_planes<PL, RC, NT, A, BLK, true, false>(blk, stride, i, U, 0.0f);
+  else
+    ld_link_planes<PL, RC, NT, A, BLK, false, false>(blk, stride, i, U, 0.0f);
+  if constexpr (RC == 12) {
+    using R = RT<A>;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const A a1 = U[(k + 1) % 3], a2 = U[(k + 2) % 3], b1 = U[3 + (k + 1) % 3], b2 = U[3 + (k + 2) % 3];
+      R re = fma(a1.x, b2.x, (R)0), im = fma(a1.x, b2.y, (R)0);
+      re = fma(-a1.y, b2.y, re);
+      re = fma(-a2.x, b1.x, re);
+      re = fma(a2.y, b1.y, re);
+      im = fma(a1.y, b2.x, im);
+      im = fma(-a2.x, b1.y, im);
+      im = fma(-a2.y, b1.x, im);
+      U[6 + k] = mk<A>(f * re, -f * im);
+    }
+  }
+}
+
+// A scheduling barrier between the two links of a trip and after the trip:
+// without it the compiler issues the backward link's loads with the forward
+// one's, and the Q16 12-real cg2b kernel takes 71 VGPRs instead of 58 (7
+// waves per SIMD instead of 8).  QH_LEAN_FENCE=0 drops it.
+#ifndef QH_LEAN_FENCE
+#define QH_LEAN_FENCE 1
+#endif
+__device__ __forceinline__ void lean_fence() {
+  if (QH_LEAN_FENCE) {
+    mem_fence_compiler();
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int PL, int RC, int NL, class A, class LD>
+__device__ __forceinline__ void stencil_lean(const Geom &g, const char *links, long lkblk, const float *qs, int par,
+                                             long i, const Site &s, LD &&ld, A (&acc)[3]) {
+  using R = RT<A>;
+  constexpr bool BLK = link_blocked<PL, NL>();
+  constexpr bool Q = (PL == PQ16);
+  uint32_t wrap, sgn;
+  lean_bits<NL>(g, s, wrap, sgn);
+  const uint32_t i32 = (uint32_t)i, ix = i32 + (uint32_t)(s.x & 1);
+  const uint32_t hx = (uint32_t)g.hx, L1 = (uint32_t)g.L[1], L12 = (uint32_t)g.L[1] * (uint32_t)g.L[2];
+#pragma unroll
+  for (int c = 0; c < 3; c++) acc[c] = mk<A>(0, 0);
+  // one trip: direction k, its links loaded non-temporally when ntc is true
+  auto trip = [&](int k, auto ntc) {
+    constexpr bool NT = decltype(ntc)::value;
+    const int mu = (NL == 4) ? k : (k >> 1);
+    const uint32_t len = (NL == 8 && (k & 1)) ? 3u : 1u;
+    // wave-uniform: index step of one hop in mu, and its edge correction
+    const uint32_t Lmu = (uint32_t)(mu == 1 ? g.L[1] : mu == 2 ? g.L[2] : g.L[3]);
+    const uint32_t step = mu == 1 ? hx : mu == 2 ? hx * L1 : hx * L12;
+    const uint32_t dF = mu == 0 ? (len >> 1) : len * step;
+    const uint32_t dB = mu == 0 ? 0u - ((len + 1u) >> 1) : 0u - dF;
+    const uint32_t wF = mu == 0 ? 0u - hx : 0u - step * Lmu;
+    const uint32_t base = mu == 0 ? ix : i32;
+    const uint32_t nF = base + dF + (((wrap >> (2 * k)) & 1u) ? wF : 0u);
+    const uint32_t nB = base + dB + (((wrap >> (2 * k + 1)) & 1u) ? 0u - wF : 0u);
+    const R fs = Q ? (R)qs[k] : (R)1;
+    A T[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) T[c] = Q ? mk<A>(0, 0) : acc[c];
+    {
+      A U[9], v[3];
+      const R f = (R)__uint_as_float(__float_as_uint((float)fs) ^ lean_signbit(sgn, 2 * mu));
+      lean_link<PL, RC, A, BLK, NT>(links + (long)(2 * k + par) * lkblk, g.stride, (long)i32, f, U);
+      ld((long)nF, v);
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) cmac(T[r], U[3 * r + c], v[c]);
+    }
+    lean_fence();
+    {
+      A U[9], v[3];
+      const R f = (R)__uint_as_float(__float_as_uint((float)fs) ^ lean_signbit(sgn, 2 * mu + 1));
+      lean_link<PL, RC, A, BLK, NT>(links + (long)(2 * k + 1 - par) * lkblk, g.stride, (long)nB, f, U);
+      ld((long)nB, v);
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) cjmsub(T[r], U[3 * c + r], v[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      acc[c] = Q ? mk<A>(fma(fs, T[c].x, acc[c].x), fma(fs, T[c].y, acc[c].y)) : T[c];
+    lean_fence();
+  };
+  const int m = link_cached_dirs<NL>(g, lkblk);
+#pragma unroll 1
+  for (int k = 0; k < m; k++) trip(k, std::false_type());
+#pragma unroll 1
+  for (int k = m; k < NL; k++) trip(k, std::true_type());
+}
+
 // stencil in form D (the forms above); stencil() takes the build's form for PL
 template <int D, int PL, int RC, int NL, class A, class LD>
 __device__ __forceinline__ void stencil_in(const Geom &g, const char *links, long lkblk, const float *qs, int par,
                                            long i, const Site &s, LD &&ld, A (&res)[3]) {
+  if constexpr (D == -5) {
+    stencil_lean<PL, RC, NL, A>(g, links, lkblk, qs, par, i, s, ld, res);
+    return;
+  }
   constexpr int NJ = 2 * NL;
   A af[3], ab[3];
 #pragma unroll
@@ -87,10 +244,10 @@ __device__ __forceinline__ void stencil_in(const Geom &g, const char *links, lon
       L0 = L1;
     }
   } else if constexpr (D == -3) {  // rolled loop, one link per trip, one accumulator
-#pragma unroll 1
-    for (int j = 0; j < NJ; j++) {
+    // links j < jc (directions below link_cached_dirs) with cache-allocating loads
+    auto trip = [&](int j, auto ntc) {
       Leg<A> L0;
-      leg_load<PL, RC, NL, A>(g, links, lkblk, qs, par, i, s, ld, j, L0);
+      leg_load<PL, RC, NL, A, LD, decltype(ntc)::value>(g, links, lkblk, qs, par, i, s, ld, j, L0);
       if (!(j & 1)) {
 #pragma unroll
         for (int r = 0; r < 3; r++)
@@ -102,7 +259,12 @@ __device__ __forceinline__ void stencil_in(const Geom &g, const char *links, lon
 #pragma unroll
           for (int c = 0; c < 3; c++) cjmsub(af[r], L0.U[3 * c + r], L0.v[c]);
       }
-    }
+    };
+    const int jc = 2 * link_cached_dirs<NL>(g, lkblk);
+#pragma unroll 1
+    for (int j = 0; j < jc; j++) trip(j, std::false_type());
+#pragma unroll 1
+    for (int j = jc; j < NJ; j++) trip(j, std::true_type());
   } else if constexpr (D == -1) {  // rolled loop, one hop (two links) per trip
 #pragma unroll 1
     for (int j = 0; j < NJ; j += 2) {

@@ -482,6 +482,15 @@ def set_cg_cap(cap=-1):
     call("qexhip_set_cg_cap", int(cap))
 
 
+def set_link_cache(lo=None, nbytes=-1):
+    """Link bytes the stencils read with cache-allocating loads, so that a
+    link field (or its first directions) can stay in the last-level cache
+    between sweeps; the rest is read with the non-temporal hint.  lo: a
+    Layout (None sets the default of Layouts created afterwards); nbytes: 0
+    all non-temporal, -1 (the default) chosen by the tuner."""
+    call("qexhip_set_link_cache", lo.h if lo is not None else None, int(nbytes))
+
+
 def set_cg_block1(block1=0):
     """Pin the block size (64, 128, 256) of the CG iteration's other-parity
     stencil kernel (forms 3 and 4); 0 lets the tuner choose."""
@@ -531,14 +540,16 @@ def set_cg_form(form=0):
 
 def autotune_report():
     """{kernel family and shape: {"block", "remap", "form", "cap", "block1", "batches", "chunk",
-    "tmarch", "two", "us"}} chosen so far (form: 2-, 3- or 4-kernel CG iteration;
+    "tmarch", "two", "lkcache_mb", "us"}} chosen so far (form: 2-, 3- or 4-kernel CG iteration;
     cap: workgroups of a persistent form, 0 = one pass; block1: block size of
     the 3-kernel iteration's other-parity stencil kernel, 0 = block; batches:
     vector-load batches per link of the multi-RHS hop; chunk: t-first site-walk
     chunk, 0 = canonical order; tmarch: t-slices per thread of the hop's
     t-march form, 0 = off; two: the hop's stencil loop takes both links of a
-    hop per trip; us: time of one timed unit)."""
-    buf = ctypes.create_string_buffer(1 << 16)
+    hop per trip; lkcache_mb: link megabytes read with cache-allocating loads,
+    for lattices that leave that budget to the tuner; us: time of one timed
+    unit)."""
+    buf = ctypes.create_string_buffer(1 << 20)
     call("qexhip_autotune_report", buf, len(buf))
     out = {}
     for line in buf.value.decode().splitlines():
@@ -547,7 +558,8 @@ def autotune_report():
         out[key] = {"block": int(f["block"]), "remap": int(f["remap"]), "form": int(f["form"]),
                     "cap": int(f["cap"]), "block1": int(f.get("block1", 0)),
                     "batches": int(f.get("batches", 1)), "chunk": int(f.get("chunk", 0)),
-                    "tmarch": int(f.get("tmarch", 0)), "two": int(f.get("two", 0)), "us": float(f["us"])}
+                    "tmarch": int(f.get("tmarch", 0)), "two": int(f.get("two", 0)),
+                    "lkcache_mb": int(f.get("lkcache", 0)), "us": float(f["us"])}
     return out
 
 

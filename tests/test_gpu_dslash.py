@@ -265,6 +265,68 @@ def test_hop_two_links_per_trip(oracle, gpu, L, hisq):
         gpu.set_hop_two(-1)
 
 
+@pytest.mark.parametrize("L,hisq", [((8, 8, 4, 8), False), ((8, 4, 6, 4), True)])
+def test_link_cache_budgets_bitwise(oracle, gpu, L, hisq):
+    """The link-cache budget (the links of the first directions read with
+    cache-allocating loads, the rest non-temporally) changes no arithmetic.
+    Budgets that cover 0, 1, 2, 3 or all directions of the 16-bit 12-real,
+    16-bit 18-real, FP32 and FP64 link fields give stagD2 (FP64, FP32, each
+    parity) and the FP64, FP32-, FP16- and Q16-link mixed CG solves bitwise
+    the results of budget 0, which equal the oracle's."""
+    O = oracle
+    g = make_links(O, L, hisq)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    stride = -(-(int(np.prod(L)) // 2) // 256) * 256
+    d12, d64 = 2 * 24 * stride, 2 * 144 * stride  # one direction's two parity blocks
+    budgets = [0, d12, 2 * d12 + 1, 3 * d12 + 1, d64, 2 * d64, 3 * d64 + 1, 1 << 30]
+    x = O.source_gauss(L, 71, 0, 2)
+    r0 = O.source_gauss(L, 72, 1, 2)
+    b = O.source_gauss(L, 73, 0, 0)
+    S = gpu.SloppyType
+    solves = [(S.SloppyNone, None, 18), (S.SloppySingle, None, 18), (S.SloppyHalf, None, 18)]
+    if not hisq:
+        solves += [(S.SloppyHalf, S.SloppyHalfFixed, 12), (S.SloppyHalf, None, 12)]
+    ref = None
+    try:
+        for bud in budgets:
+            gpu.set_link_cache(lo, bud)
+            got = []
+            for prec in (gpu.FP64, gpu.FP32):
+                for par in (0, 1):
+                    fx = lo.ColorVector(prec).set(x)
+                    fr = lo.ColorVector(prec).set(r0)
+                    gpu.stagD2(sd_of(s, par), fr, s.g, fx, 0.5, -0.25)
+                    got.append(fr.get())
+            for sl, sll, rc in solves:
+                fs = lo.ColorVector()
+                sp = gpu.SolverParams()
+                sp.r2req, sp.verbosity, sp.sloppySolve, sp.recon = 1e-16, 0, sl, rc
+                if sll is not None:
+                    sp.sloppyLinks = sll
+                gpu.solveEE(s, fs, lo.ColorVector().set(b), 0.1, sp)
+                got.append(fs.get())
+                got.append(np.array([sp.iterations]))
+            if ref is None:
+                ref = got
+                k = 0
+                for prec, tol in ((gpu.FP64, 1e-12), (gpu.FP32, 1e-5)):
+                    for par in (0, 1):
+                        want = r0.copy()
+                        O.stagD2(L, g, par, want, x, 0.5, -0.25)
+                        assert rel(ref[k], want) < tol, (prec, par)
+                        k += 1
+                for _ in solves:
+                    r = b - O.stagD2xx(L, g, 0, ref[k], 0.01)
+                    assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-16
+                    k += 2
+            else:
+                for k, (a, c) in enumerate(zip(got, ref)):
+                    assert np.array_equal(a, c), (bud, k)
+    finally:
+        gpu.set_link_cache(lo, -1)
+
+
 @pytest.mark.parametrize("L", [(8, 4, 6, 4), (6, 4, 8, 10), (8, 8, 4, 8), (4, 4, 4, 2)])
 @pytest.mark.parametrize("tmarch", [1, 3, 4, 16])
 def test_hop_tmarch(oracle, gpu, L, tmarch):
