@@ -271,7 +271,8 @@ def test_link_cache_budgets_bitwise(oracle, gpu, L, hisq):
     cache-allocating loads, the rest non-temporally) changes no arithmetic.
     Budgets that cover 0, 1, 2, 3 or all directions of the 16-bit 12-real,
     16-bit 18-real, FP32 and FP64 link fields give stagD2 (FP64, FP32, each
-    parity) and the FP64, FP32-, FP16- and Q16-link mixed CG solves bitwise
+    parity) and the FP64, FP32-, FP16- and Q16-link (18- and, for plain
+    links, 12-real) mixed CG solves bitwise
     the results of budget 0, which equal the oracle's."""
     O = oracle
     g = make_links(O, L, hisq)
@@ -284,7 +285,8 @@ def test_link_cache_budgets_bitwise(oracle, gpu, L, hisq):
     r0 = O.source_gauss(L, 72, 1, 2)
     b = O.source_gauss(L, 73, 0, 0)
     S = gpu.SloppyType
-    solves = [(S.SloppyNone, None, 18), (S.SloppySingle, None, 18), (S.SloppyHalf, None, 18)]
+    solves = [(S.SloppyNone, None, 18), (S.SloppySingle, None, 18), (S.SloppyHalf, None, 18),
+              (S.SloppyHalf, S.SloppyHalfFixed, 18)]
     if not hisq:
         solves += [(S.SloppyHalf, S.SloppyHalfFixed, 12), (S.SloppyHalf, None, 12)]
     ref = None
