@@ -200,8 +200,8 @@ int qexhip_autotune_report(char *buf, int len);
    systems; batches take 3); 0 = chosen by the tuner */
 int qexhip_set_cg_form(int form);
 /* link bytes that the stencils of lattice lat read with cache-allocating
-   loads, no QEX counterpart: the links of the first directions whose two
-   parity blocks fit in this many bytes are read so that they can stay in the
+   loads, no QEX counterpart: the first link blocks (one direction and parity
+   each) that fit in this many bytes are read so that they can stay in the
    last-level cache from one sweep to the next; the rest, with the
    non-temporal hint.  0 = all non-temporal; -1 = chosen by the tuner per
    kernel family and link format (the default).  With

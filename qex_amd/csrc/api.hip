@@ -285,7 +285,7 @@ std::map<std::string, Tuned> g_tune;
 // 233.7 (192 MB: one direction, 151 MB); at 48^4, where the vectors alone
 // nearly fill the 256 MB last-level cache, 0 was best (Q16 12-real 297.9 us,
 // 2% slower at 128 MB; FP16 18-real 377.7 us, 6% slower at 192 MB).
-const long kLinkCacheMB[] = {0, 64, 128, 192, 256};
+const long kLinkCacheMB[] = {0, 64, 128, 160, 192, 224, 256};
 
 // the link-cache budget a launch with tuned shape tu uses on lattice lat
 long link_cache_of(const qexhip_lattice *lat, const Tuned &tu) {

@@ -170,7 +170,7 @@ struct Geom {
   int hx;        // Lx/2
   long Vh;       // sites per parity
   long stride;   // padded sites per parity in the SoA layout
-  long lkcache;  // link bytes the stencils read with cache-allocating loads (link_cached_dirs)
+  long lkcache;  // link bytes the stencils read with cache-allocating loads (link_cached_blocks)
 };
 
 // Element index of color component c of site i in a vector field.  Blocked

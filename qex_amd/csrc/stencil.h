@@ -23,15 +23,21 @@ template <class A> struct Leg {
 // solver sweeps the same links again and again, and a link field that fits in
 // the 256 MB last-level cache (16-bit 12-real links: 100 MB at 32^4) can stay
 // there from one sweep to the next if it is read with plain loads.  Geom's
-// lkcache gives the link bytes to read that way: the links of directions
-// k < link_cached_dirs (both parity blocks of a direction, 2 lkblk bytes),
-// the rest non-temporal.  Each range has a loop of its own, so the two kinds
-// of load never meet in one instruction.
-template <int NL> __device__ __forceinline__ int link_cached_dirs(const Geom &g, long lkblk) {
+// lkcache gives the link bytes to read that way: the first link_cached_blocks
+// link blocks (block b = 2k + parity, lkblk bytes each), the rest
+// non-temporally.  A hop of either parity reads every block once, so a cached
+// block serves both.  Direction k's forward link comes from block 2k + par and
+// its backward one from block 2k + 1 - par: the directions below mb/2 have
+// both blocks cached, and for odd mb direction mb/2 has block mb - 1, which
+// is its forward link's block when par is even and its backward one's when
+// par is odd.  Each range has a loop of its own, so the two kinds of load
+// never meet in one instruction.
+template <int NL> __device__ __forceinline__ int link_cached_blocks(const Geom &g, long lkblk) {
   int m = 0;
-  while (m < NL && (long)(m + 1) * 2 * lkblk <= g.lkcache) m++;
+  while (m < 2 * NL && (long)(m + 1) * lkblk <= g.lkcache) m++;
   return m;
 }
+
 template <int PL, int RC, int NL, class A, class LD, bool NT = true>
 __device__ __forceinline__ void leg_load(const Geom &g, const char *links, long lkblk, const float *qs, int par,
                                          long i, const Site &s, LD &ld, int j, Leg<A> &L) {
@@ -160,9 +166,10 @@ __device__ __forceinline__ void stencil_lean(const Geom &g, const char *links, l
   const uint32_t hx = (uint32_t)g.hx, L1 = (uint32_t)g.L[1], L12 = (uint32_t)g.L[1] * (uint32_t)g.L[2];
 #pragma unroll
   for (int c = 0; c < 3; c++) acc[c] = mk<A>(0, 0);
-  // one trip: direction k, its links loaded non-temporally when ntc is true
-  auto trip = [&](int k, auto ntc) {
-    constexpr bool NT = decltype(ntc)::value;
+  // one trip: direction k, its forward / backward link loaded non-temporally
+  // when ntf / ntb is true
+  auto trip = [&](int k, auto ntf, auto ntb) {
+    constexpr bool NTF = decltype(ntf)::value, NTB = decltype(ntb)::value;
     const int mu = (NL == 4) ? k : (k >> 1);
     const uint32_t len = (NL == 8 && (k & 1)) ? 3u : 1u;
     // wave-uniform: index step of one hop in mu, and its edge correction
@@ -181,7 +188,7 @@ __device__ __forceinline__ void stencil_lean(const Geom &g, const char *links, l
     {
       A U[9], v[3];
       const R f = (R)__uint_as_float(__float_as_uint((float)fs) ^ lean_signbit(sgn, 2 * mu));
-      lean_link<PL, RC, A, BLK, NT>(links + (long)(2 * k + par) * lkblk, g.stride, (long)i32, f, U);
+      lean_link<PL, RC, A, BLK, NTF>(links + (long)(2 * k + par) * lkblk, g.stride, (long)i32, f, U);
       ld((long)nF, v);
 #pragma unroll
       for (int r = 0; r < 3; r++)
@@ -192,7 +199,7 @@ __device__ __forceinline__ void stencil_lean(const Geom &g, const char *links, l
     {
       A U[9], v[3];
       const R f = (R)__uint_as_float(__float_as_uint((float)fs) ^ lean_signbit(sgn, 2 * mu + 1));
-      lean_link<PL, RC, A, BLK, NT>(links + (long)(2 * k + 1 - par) * lkblk, g.stride, (long)nB, f, U);
+      lean_link<PL, RC, A, BLK, NTB>(links + (long)(2 * k + 1 - par) * lkblk, g.stride, (long)nB, f, U);
       ld((long)nB, v);
 #pragma unroll
       for (int r = 0; r < 3; r++)
@@ -204,11 +211,20 @@ __device__ __forceinline__ void stencil_lean(const Geom &g, const char *links, l
       acc[c] = Q ? mk<A>(fma(fs, T[c].x, acc[c].x), fma(fs, T[c].y, acc[c].y)) : T[c];
     lean_fence();
   };
-  const int m = link_cached_dirs<NL>(g, lkblk);
+  // the first link_cached_blocks blocks with cache-allocating loads
+  const int mb = link_cached_blocks<NL>(g, lkblk), kf = mb >> 1;
 #pragma unroll 1
-  for (int k = 0; k < m; k++) trip(k, std::false_type());
+  for (int k = 0; k < kf; k++) trip(k, std::false_type(), std::false_type());
+  int k1 = kf;
+  if (mb & 1) {  // direction kf: only block mb - 1 = 2 kf cached
+    if (par & 1)
+      trip(kf, std::true_type(), std::false_type());
+    else
+      trip(kf, std::false_type(), std::true_type());
+    k1++;
+  }
 #pragma unroll 1
-  for (int k = m; k < NL; k++) trip(k, std::true_type());
+  for (int k = k1; k < NL; k++) trip(k, std::true_type(), std::true_type());
 }
 
 // stencil in form D (the forms above); stencil() takes the build's form for PL
@@ -244,7 +260,7 @@ __device__ __forceinline__ void stencil_in(const Geom &g, const char *links, lon
       L0 = L1;
     }
   } else if constexpr (D == -3) {  // rolled loop, one link per trip, one accumulator
-    // links j < jc (directions below link_cached_dirs) with cache-allocating loads
+    // the links of the first link_cached_blocks blocks with cache-allocating loads
     auto trip = [&](int j, auto ntc) {
       Leg<A> L0;
       leg_load<PL, RC, NL, A, LD, decltype(ntc)::value>(g, links, lkblk, qs, par, i, s, ld, j, L0);
@@ -260,11 +276,22 @@ __device__ __forceinline__ void stencil_in(const Geom &g, const char *links, lon
           for (int c = 0; c < 3; c++) cjmsub(af[r], L0.U[3 * c + r], L0.v[c]);
       }
     };
-    const int jc = 2 * link_cached_dirs<NL>(g, lkblk);
+    const int mb = link_cached_blocks<NL>(g, lkblk), kf = mb >> 1;
 #pragma unroll 1
-    for (int j = 0; j < jc; j++) trip(j, std::false_type());
+    for (int j = 0; j < 2 * kf; j++) trip(j, std::false_type());
+    int j1 = 2 * kf;
+    if (mb & 1) {  // direction kf: only block mb - 1 = 2 kf cached
+      if (par & 1) {
+        trip(j1, std::true_type());
+        trip(j1 + 1, std::false_type());
+      } else {
+        trip(j1, std::false_type());
+        trip(j1 + 1, std::true_type());
+      }
+      j1 += 2;
+    }
 #pragma unroll 1
-    for (int j = jc; j < NJ; j++) trip(j, std::true_type());
+    for (int j = j1; j < NJ; j++) trip(j, std::true_type());
   } else if constexpr (D == -1) {  // rolled loop, one hop (two links) per trip
 #pragma unroll 1
     for (int j = 0; j < NJ; j += 2) {

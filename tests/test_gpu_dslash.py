@@ -267,20 +267,23 @@ def test_hop_two_links_per_trip(oracle, gpu, L, hisq):
 
 @pytest.mark.parametrize("L,hisq", [((8, 8, 4, 8), False), ((8, 4, 6, 4), True)])
 def test_link_cache_budgets_bitwise(oracle, gpu, L, hisq):
-    """The link-cache budget (the links of the first directions read with
+    """The link-cache budget (the first link blocks read with
     cache-allocating loads, the rest non-temporally) changes no arithmetic.
-    Budgets that cover 0, 1, 2, 3 or all directions of the 16-bit 12-real,
-    16-bit 18-real, FP32 and FP64 link fields give stagD2 (FP64, FP32, each
-    parity) and the FP64, FP32-, FP16- and Q16-link (18- and, for plain
-    links, 12-real) mixed CG solves bitwise
-    the results of budget 0, which equal the oracle's."""
+    Budgets that cover none, some (odd and even counts) or all of the blocks
+    of the 16-bit 12-real, 16-bit 18-real, FP32 and FP64 link fields give
+    stagD2 (FP64, FP32, each parity) and the FP64, FP32-, FP16- and Q16-link
+    (18- and, for plain links, 12-real) mixed CG solves bitwise the results
+    of budget 0, which equal the oracle's."""
     O = oracle
     g = make_links(O, L, hisq)
     lo = gpu.Layout(L)
     s = gpu.Staggered(lo, g)
     stride = -(-(int(np.prod(L)) // 2) // 256) * 256
-    d12, d64 = 2 * 24 * stride, 2 * 144 * stride  # one direction's two parity blocks
-    budgets = [0, d12, 2 * d12 + 1, 3 * d12 + 1, d64, 2 * d64, 3 * d64 + 1, 1 << 30]
+    # one link block (one direction, one parity) is 24, 36, 72 or 144 B x stride
+    # for 16-bit 12-real, 16-bit 18-real, FP32 and FP64 links: odd and even
+    # block counts in every format, so that a direction with one cached block
+    # occurs for both output parities
+    budgets = [0] + [n * 24 * stride for n in (1, 3, 5, 7)] + [n * 144 * stride for n in (1, 3, 5, 7)] + [1 << 30]
     x = O.source_gauss(L, 71, 0, 2)
     r0 = O.source_gauss(L, 72, 1, 2)
     b = O.source_gauss(L, 73, 0, 0)
