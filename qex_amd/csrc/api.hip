@@ -1055,6 +1055,7 @@ struct BatchWs {
   long maxblocks = 0, nblocks = 0;
   BatchItem *T = nullptr, *Tx = nullptr;  // device tables: solve, tuning scratch
   int block = 0, form = 2;
+  long lkc = 0;  // link-cache bytes of the whole batch (batch_table)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~BatchWs() {
@@ -1081,9 +1082,13 @@ void drop_batch() {
 }
 
 // block table for block size `block`, scalars S (device array); returns the
-// number of blocks of the batched grid
-long batch_table(BatchWs *w, int block, Scal *S, BatchItem *dst) {
+// number of blocks of the batched grid.  lkc: link-cache bytes of the whole
+// batch, shared out over the systems in proportion to their link bytes (a
+// lattice with a pinned budget keeps its own).
+long batch_table(BatchWs *w, int block, Scal *S, BatchItem *dst, long lkc = 0) {
   std::vector<BatchItem> T(w->n);
+  double tot = 0.0;
+  for (int k = 0; k < w->n; k++) tot += 8.0 * (double)lkblk_bytes(w->links[k], P64, 18);
   long b0 = 0;
   for (int k = 0; k < w->n; k++) {
     qexhip_links *l = w->links[k];
@@ -1091,6 +1096,8 @@ long batch_table(BatchWs *w, int block, Scal *S, BatchItem *dst) {
     BatchItem &it = T[k];
     memset(&it, 0, sizeof it);
     it.a.g = g;
+    it.a.g.lkcache = g.lkcache >= 0 ? g.lkcache
+                                    : (long)((double)lkc * 8.0 * (double)lkblk_bytes(l, P64, 18) / tot);
     it.a.links = links_store(l, P64, 18);
     it.a.lkblk = lkblk_bytes(l, P64, 18);
     it.a.par = w->par;
@@ -1149,6 +1156,7 @@ void batch_shape(BatchWs *w, bool autotune) {
   // the batch has forms 2 and 3; a pinned form 4 (single systems only) means 3
   const int pinned = (g_cg_form == 4) ? 3 : g_cg_form;
   int block = 256, form = pinned ? pinned : 2;
+  long lkc = 0;
   if (autotune && !g_block_forced) {
     std::string key = "batch-cg nl=" + std::to_string(w->nl);
     for (auto *l : w->links) key += " " + geom_key(l->lat->g);
@@ -1156,6 +1164,7 @@ void batch_shape(BatchWs *w, bool autotune) {
     if (it != g_tune.end()) {
       block = it->second.block;
       form = it->second.form;
+      lkc = it->second.lkcache;
     } else {
       // scratch scalars that never converge
       std::vector<Scal> h(w->n);
@@ -1171,9 +1180,22 @@ void batch_shape(BatchWs *w, bool autotune) {
           if (us < best.us) best = Tuned{b, 0, us, f};
         }
       }
+      // the batch's link-cache budget, on the chosen shape
+      double bus = 1e300;
+      for (long mbytes : kLinkCacheMB) {
+        const long nb = batch_table(w, best.block, w->Sx, w->Tx, mbytes << 20);
+        const double us =
+            time_us([&] { launch_bcg_iter(w->Tx, w->n, nb, w->nl, best.block, g_stream, best.form); }, 2, 8);
+        if (us < bus) {
+          bus = us;
+          best.lkcache = mbytes << 20;
+        }
+      }
+      best.us = bus;
       g_tune[key] = best;
       block = best.block;
       form = best.form;
+      lkc = best.lkcache;
     }
   } else if (g_block_forced) {
     block = g_block;
@@ -1181,10 +1203,14 @@ void batch_shape(BatchWs *w, bool autotune) {
   if (block != w->block || form != w->form || !w->exec) {
     w->block = block;
     w->form = form;
-    w->nblocks = batch_table(w, block, w->S, w->T);
     if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
     if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
   }
+  // the table every call: the link-cache budgets of pinned lattices may have
+  // changed (the graph reads the table from device memory; its grid depends
+  // only on the block size and the lattices)
+  w->lkc = lkc;
+  w->nblocks = batch_table(w, block, w->S, w->T, lkc);
 }
 
 void solve_batch(int n, qexhip_links *const *ls, int par, qexhip_field *const *xf, qexhip_field *const *bf,

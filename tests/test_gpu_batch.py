@@ -198,3 +198,40 @@ def test_uneven_batch_both_iteration_forms(oracle, gpu, form):
         want, its, _ = O.cg(L, g, 0, bs[k], [0.1, 0.2, 0.15][k], 1e-20)
         assert sps[k].r2 <= 1e-20 and abs(sps[k].iterations - its) <= iteration_bar(its, lo.nEven)
         assert rel(fx[k].get(sub=0)[: lo.nEven], want[: lo.nEven]) < 1e-8
+
+
+def test_uneven_batch_link_cache_bitwise(oracle, gpu):
+    """The batched CG with each lattice's link-cache budget pinned to none,
+    an odd or an even number of its FP64 link blocks, or all of them: the
+    solutions and iteration counts are bitwise those of budget 0, which meet
+    the oracle's residual."""
+    O = oracle
+    lats = [(8, 8, 8, 8), (4, 4, 4, 8), (6, 6, 6, 6)]
+    ms = [0.1, 0.2, 0.15]
+    systems = [_system(O, gpu, L, 900 + k) for k, L in enumerate(lats)]
+    bs = [O.source_gauss(L, 950 + k, 0, 0) for k, L in enumerate(lats)]
+    fb = [lo.ColorVector().set(b) for (lo, _, _), b in zip(systems, bs)]
+    blk = [144 * (-(-(int(np.prod(L)) // 2) // 256) * 256) for L in lats]  # one FP64 link block
+    pins = [[0, 0, 0], [blk[0], 3 * blk[1], 2 * blk[2]], [5 * blk[0], 0, 7 * blk[2]], [1 << 30] * 3]
+    ref = None
+    try:
+        for pin in pins:
+            for (lo, _, _), b in zip(systems, pin):
+                gpu.set_link_cache(lo, b)
+            fx = [lo.ColorVector() for lo, _, _ in systems]
+            sps = [gpu.SolverParams() for _ in lats]
+            for sp in sps:
+                sp.r2req, sp.verbosity = 1e-20, 0
+            gpu.solveBatch([s for _, _, s in systems], fx, fb, ms, sps, graph=True, tune=False)
+            got = [f.get(sub=0) for f in fx] + [np.array([sp.iterations for sp in sps])]
+            if ref is None:
+                ref = got
+                for k, (L, (lo, g, s)) in enumerate(zip(lats, systems)):
+                    r = bs[k] - O.stagD2xx(L, g, 0, fx[k].get(), ms[k] ** 2)
+                    assert O.norm2(r[: lo.nEven]) / O.norm2(bs[k]) <= 1.0001e-20, k
+            else:
+                for k, (a, c) in enumerate(zip(got, ref)):
+                    assert np.array_equal(a, c), (pin, k)
+    finally:
+        for lo, _, _ in systems:
+            gpu.set_link_cache(lo, -1)
