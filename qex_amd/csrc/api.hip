@@ -1295,6 +1295,7 @@ struct MsbWs {
   int block = 0;
   long chunk = -1;   // site walk of the hops baked into the captured graph
   int batches = 0;   // vector-load batches of the hops baked into the captured graph
+  long lkc = -2;     // link-cache bytes of the hops baked into the captured graph
   double m2 = -1.0;  // mass^2 baked into the captured graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1397,13 +1398,15 @@ void msb_flush(MsbWs *w, MsbScal *T) {
     launch_msb_flush(w->l->lat->g, gr.second, w->nshift, msb_ptrs(w, gr.first, gr.second), T + gr.first, g_stream);
 }
 
-void msb_iter(MsbWs *w, MsbScal *T, double m2, int block, long chunk, int batches) {
+void msb_iter(MsbWs *w, MsbScal *T, double m2, int block, long chunk, int batches, long lkc) {
   qexhip_links *l = w->l;
   require(mrhs_grid(l->lat->g, block, chunk) <= (unsigned)w->red.maxblocks, "multi-shift batch: hop grid too large");
   const char *links = links_store(l, P64, 18);
   const long lkblk = lkblk_bytes(l, P64, 18);
+  Geom g = l->lat->g;
+  g.lkcache = lkc;
   for (auto &gr : w->groups)
-    launch_msb_iter(l->lat->g, links, lkblk, l->nl, w->par, gr.second, w->nshift, m2,
+    launch_msb_iter(g, links, lkblk, l->nl, w->par, gr.second, w->nshift, m2,
                     msb_ptrs(w, gr.first, gr.second), T + gr.first, w->red, block, chunk, batches, g_stream);
 }
 
@@ -1419,6 +1422,7 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
   int block = g_block_forced ? g_block : 256;
   long chunk = g_msb_walk > 0 ? g_msb_walk : 0;
   int batches = 1;
+  long lkc = 0;
   if ((flags & QEXHIP_BATCH_TUNE) && !g_block_forced) {
     const std::string key = "msb nl=" + std::to_string(l->nl) + " nrhs=" + std::to_string(nrhs) +
                             " nshift=" + std::to_string(nshift) + " walk=" + std::to_string(g_msb_walk) + " " +
@@ -1428,6 +1432,7 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
       block = it->second.block;
       chunk = it->second.chunk;
       batches = it->second.batches;
+      lkc = it->second.lkcache;
     } else {
       std::vector<MsbScal> h(nrhs);
       memset(h.data(), 0, sizeof(MsbScal) * nrhs);
@@ -1451,24 +1456,38 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
             if (g_msb_walk >= 0 && c != 0) continue;  // a pinned walk: one candidate per block and batches
             const long cw = g_msb_walk >= 0 ? g_msb_walk : c;
             if (cw && (cw % b || cw >= S)) continue;
-            const double us = time_us([&] { msb_iter(w, w->Tx, m2, b, cw, nb); }, 1, 4);
+            const double us = time_us([&] { msb_iter(w, w->Tx, m2, b, cw, nb, 0); }, 1, 4);
             if (us < best.us) {
               best = Tuned{b, 1, us};
               best.chunk = cw;
               best.batches = nb;
             }
           }
+      // the link-cache budget of the hops, on the chosen shape
+      double bus = 1e300;
+      for (long mbytes : kLinkCacheMB) {
+        const double us =
+            time_us([&] { msb_iter(w, w->Tx, m2, best.block, best.chunk, best.batches, mbytes << 20); }, 1, 4);
+        if (us < bus) {
+          bus = us;
+          best.lkcache = mbytes << 20;
+        }
+      }
+      best.us = bus;
       g_tune[key] = best;
       block = best.block;
       chunk = best.chunk;
       batches = best.batches;
+      lkc = best.lkcache;
     }
   }
+  if (lat->g.lkcache >= 0) lkc = lat->g.lkcache;  // a pinned lattice budget
   const bool graph = (flags & QEXHIP_BATCH_GRAPH) != 0;
-  if (block != w->block || m2 != w->m2 || chunk != w->chunk || batches != w->batches) {
+  if (block != w->block || m2 != w->m2 || chunk != w->chunk || batches != w->batches || lkc != w->lkc) {
     w->block = block;
     w->chunk = chunk;
     w->batches = batches;
+    w->lkc = lkc;
     w->m2 = m2;
     if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
     if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
@@ -1505,7 +1524,7 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
   // one chunk = one window of the deferred shift updates: kChunk iterations, then the flush
   static_assert(kChunk <= QH_MSB_WIN, "a chunk must fit one window of the multi-shift updates");
   auto msb_chunk = [&] {
-    for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block, chunk, batches);
+    for (int c = 0; c < kChunk; c++) msb_iter(w, w->T, m2, block, chunk, batches, lkc);
     msb_flush(w, w->T);
   };
   if (graph && !w->exec) capture(msb_chunk, &w->graph, &w->exec);

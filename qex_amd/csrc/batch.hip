@@ -153,15 +153,15 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
 #pragma unroll
       for (int c = 0; c < 3; c++) acc[r][c] = make_double2(0.0, 0.0);
     const Site s = decode(g, i, q);
-#pragma unroll 1
-    for (int j = 0; j < 2 * NL; j++) {
+    // one link j, loaded non-temporally when ntc is true (link_cached_blocks)
+    auto trip = [&](int j, auto ntc) {
       const int k = j >> 1, fwd = !(j & 1);
       const int mu = (NL == 4) ? k : (k >> 1);
       const int len = (NL == 8 && (k & 1)) ? 3 : 1;
       const long n = nbr(g, s, mu, fwd ? len : -len);
       double2 U[9];
-      ld_link<P64, 18, true, double2, link_blocked<P64, NL>()>(links + (long)(2 * k + (fwd ? q : 1 - q)) * lkblk,
-                                                              g.stride, fwd ? i : n, U, 1.0);
+      ld_link<P64, 18, decltype(ntc)::value, double2, link_blocked<P64, NL>()>(
+          links + (long)(2 * k + (fwd ? q : 1 - q)) * lkblk, g.stride, fwd ? i : n, U, 1.0);
       // the right-hand sides' vectors in batches of NB, each batch's loads
       // issued before its arithmetic
 #pragma unroll
@@ -188,7 +188,25 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
               for (int c = 0; c < 3; c++) cjmsub(acc[r0 + b][a], U[3 * c + a], v[b][c]);
         }
       }
+    };
+    // the first link_cached_blocks blocks with cache-allocating loads (block
+    // 2k + q holds direction k's forward link, 2k + 1 - q its backward one)
+    const int mb = link_cached_blocks<NL>(g, lkblk), kf = mb >> 1;
+#pragma unroll 1
+    for (int j = 0; j < 2 * kf; j++) trip(j, std::false_type());
+    int j1 = 2 * kf;
+    if (mb & 1) {
+      if (q & 1) {
+        trip(j1, std::true_type());
+        trip(j1 + 1, std::false_type());
+      } else {
+        trip(j1, std::false_type());
+        trip(j1 + 1, std::true_type());
+      }
+      j1 += 2;
     }
+#pragma unroll 1
+    for (int j = j1; j < 2 * NL; j++) trip(j, std::true_type());
   }
   if constexpr (MODE == 0) {
     if (i >= 0 && i < g.Vh) {

@@ -235,3 +235,41 @@ def test_uneven_batch_link_cache_bitwise(oracle, gpu):
     finally:
         for lo, _, _ in systems:
             gpu.set_link_cache(lo, -1)
+
+
+@pytest.mark.parametrize("hisq", [False, True])
+def test_multishift_batch_link_cache_bitwise(oracle, gpu, hisq):
+    """The batched multi-shift solve with the lattice's link-cache budget
+    pinned to none, an odd or an even number of FP64 link blocks, or all of
+    them (the multi-RHS hops read those blocks with cache-allocating loads)
+    gives bitwise the solutions and iteration counts of budget 0, which
+    match the oracle."""
+    O = oracle
+    L = (8, 8, 4, 8)
+    lo, g, s = _system(O, gpu, L, 91, hisq)
+    m = 0.05
+    sigmas = [0.0, 0.02, 0.4]
+    nr = 2
+    bs = [O.source_gauss(L, 910 + j, 0, 0) for j in range(nr)]
+    fb = [lo.ColorVector().set(b) for b in bs]
+    blk = 144 * (-(-(int(np.prod(L)) // 2) // 256) * 256)  # one FP64 link block
+    ref = None
+    try:
+        for bud in (0, blk, 3 * blk, 6 * blk, 1 << 30):
+            gpu.set_link_cache(lo, bud)
+            xs = [[lo.ColorVector() for _ in sigmas] for _ in range(nr)]
+            sps = [gpu.SolverParams() for _ in range(nr)]
+            for sp in sps:
+                sp.r2req, sp.verbosity = 1e-20, 0
+            gpu.solveMultiBatch(s, xs, fb, m, sigmas, sps)
+            got = [x.get(sub=0) for row in xs for x in row] + [np.array([sp.iterations for sp in sps])]
+            if ref is None:
+                ref = got
+                for j in range(nr):
+                    want, its, _ = O.cgm(L, g, 0, bs[j], m, sigmas, 1e-20)
+                    assert abs(sps[j].iterations - its) <= 1, (j, sps[j].iterations, its)
+            else:
+                for k, (a, c) in enumerate(zip(got, ref)):
+                    assert np.array_equal(a, c), (bud, k)
+    finally:
+        gpu.set_link_cache(lo, -1)
