@@ -1904,13 +1904,13 @@ static Tuned tuned_hop_multi(qexhip_links *l, int prec, int nr, const MultiPtrs 
   for (auto &o : outs) o = dalloc<char>(ob);
   MultiPtrs Q = P;
   for (int j = 0; j < nr; j++) Q.out[j] = outs[j];  // candidates only write these
+  Geom g = l->lat->g;
+  g.lkcache = 0;
   for (int nb : nbs)
     for (long ch : chunks)
       for (int block : {64, 128, 256}) {
         const double us = time_us(
-            [&] {
-              launch_hop_multi(prec, l->nl, nr, nb, l->lat->g, links, lkblk, par, 0.0, 0.0, Q, block, 0, ch, g_stream);
-            },
+            [&] { launch_hop_multi(prec, l->nl, nr, nb, g, links, lkblk, par, 0.0, 0.0, Q, block, 0, ch, g_stream); },
             2, 8);
         if (us < best.us) {
           best = Tuned{block, 1, us};
@@ -1918,6 +1918,24 @@ static Tuned tuned_hop_multi(qexhip_links *l, int prec, int nr, const MultiPtrs 
           best.chunk = ch;
         }
       }
+  // the link-cache budget, on the chosen shape
+  if (l->lat->g.lkcache < 0) {
+    double bus = 1e300;
+    for (long mbytes : kLinkCacheMB) {
+      g.lkcache = mbytes << 20;
+      const double us = time_us(
+          [&] {
+            launch_hop_multi(prec, l->nl, nr, best.batches, g, links, lkblk, par, 0.0, 0.0, Q, best.block, 0,
+                             best.chunk, g_stream);
+          },
+          2, 8);
+      if (us < bus) {
+        bus = us;
+        best.lkcache = mbytes << 20;
+      }
+    }
+    best.us = bus;
+  }
   HIPCHK(hipStreamSynchronize(g_stream));
   for (auto *o : outs) (void)hipFree(o);
   g_tune[key] = best;
@@ -1944,6 +1962,16 @@ static void dslash_multi(qexhip_links *l, int par, int n, qexhip_field *const *r
     for (int j0 = 0; j0 < n;) {
       int gsz = QH_MAXRHS;
       while (gsz > n - j0) gsz >>= 1;
+      if (gsz == 1) {  // one field: the one-parity hop of stagD2 (its own tuning)
+        const qexhip_field *xj = x[j0];
+        qexhip_field *rj = r[j0];
+        if (ca == 0.0)
+          hop(l, pr, 18, p, 1, 0, xj->par(1 - p), rj->par(p), xj->par(p), nullptr, cb, 0, 1.0, 0, 0, nullptr);
+        else
+          hop(l, pr, 18, p, 2, 0, xj->par(1 - p), rj->par(p), rj->par(p), xj->par(p), ca, cb, 1.0, 0, 0, nullptr);
+        j0 += 1;
+        continue;
+      }
       MultiPtrs P;
       memset(&P, 0, sizeof P);
       for (int j = 0; j < gsz; j++) {
@@ -1952,8 +1980,10 @@ static void dslash_multi(qexhip_links *l, int par, int n, qexhip_field *const *r
         P.out[j] = r[j0 + j]->par(p);
       }
       const Tuned tu = tuned_hop_multi(l, pr, gsz, P, links, lkblk, p);
-      require(launch_hop_multi(pr, l->nl, gsz, tu.batches, l->lat->g, links, lkblk, p, ca, cb, P, tu.block, 0,
-                               tu.chunk, g_stream),
+      Geom g = l->lat->g;
+      g.lkcache = link_cache_of(l->lat, tu);
+      require(launch_hop_multi(pr, l->nl, gsz, tu.batches, g, links, lkblk, p, ca, cb, P, tu.block, 0, tu.chunk,
+                               g_stream),
               "stagD2_multi: no kernel for this shape");
       j0 += gsz;
     }

@@ -273,16 +273,16 @@ __global__ void __launch_bounds__(256) k_hop_multi(Geom g, const char *links, lo
     for (int q = 0; q < NR; q++)
 #pragma unroll
       for (int c = 0; c < 3; c++) acc[q][c] = mk<A>(0, 0);
-#pragma unroll 1
-    for (int j = 0; j < 2 * NL; j++) {
+    // one link j, loaded non-temporally when ntc is true (link_cached_blocks)
+    auto trip = [&](int j, auto ntc) {
       const int k = j >> 1;
       const bool fwd = !(j & 1);
       const int mu = (NL == 4) ? k : (k >> 1);
       const int len = (NL == 8 && (k & 1)) ? 3 : 1;
       const long n = nbr(g, s, mu, fwd ? len : -len);
       A U[9];
-      ld_link<PL, 18, true, A, link_blocked<PL, NL>()>(links + (long)(2 * k + (fwd ? par : 1 - par)) * lkblk,
-                                                      g.stride, fwd ? i : n, U, (R)1.0);
+      ld_link<PL, 18, decltype(ntc)::value, A, link_blocked<PL, NL>()>(
+          links + (long)(2 * k + (fwd ? par : 1 - par)) * lkblk, g.stride, fwd ? i : n, U, (R)1.0);
 #pragma unroll
       for (int q0 = 0; q0 < NR; q0 += NB) {
         A v[NB][3];
@@ -306,7 +306,24 @@ __global__ void __launch_bounds__(256) k_hop_multi(Geom g, const char *links, lo
               for (int c = 0; c < 3; c++) cjmsub(acc[q0 + q][a], U[3 * c + a], v[q][c]);
         }
       }
+    };
+    // the first link_cached_blocks blocks with cache-allocating loads
+    const int mb = link_cached_blocks<NL>(g, lkblk), kf = mb >> 1;
+#pragma unroll 1
+    for (int j = 0; j < 2 * kf; j++) trip(j, std::false_type());
+    int j1 = 2 * kf;
+    if (mb & 1) {
+      if (par & 1) {
+        trip(j1, std::true_type());
+        trip(j1 + 1, std::false_type());
+      } else {
+        trip(j1, std::false_type());
+        trip(j1 + 1, std::true_type());
+      }
+      j1 += 2;
     }
+#pragma unroll 1
+    for (int j = j1; j < 2 * NL; j++) trip(j, std::true_type());
 #pragma unroll
     for (int q = 0; q < NR; q++)
 #pragma unroll

@@ -271,7 +271,8 @@ def test_link_cache_budgets_bitwise(oracle, gpu, L, hisq):
     cache-allocating loads, the rest non-temporally) changes no arithmetic.
     Budgets that cover none, some (odd and even counts) or all of the blocks
     of the 16-bit 12-real, 16-bit 18-real, FP32 and FP64 link fields give
-    stagD2 (FP64, FP32, each parity) and the FP64, FP32-, FP16- and Q16-link
+    stagD2 (FP64, FP32, each parity), the 8-field multi-RHS hop, and the
+    FP64, FP32-, FP16- and Q16-link
     (18- and, for plain links, 12-real) mixed CG solves bitwise the results
     of budget 0, which equal the oracle's."""
     O = oracle
@@ -303,6 +304,11 @@ def test_link_cache_budgets_bitwise(oracle, gpu, L, hisq):
                     fr = lo.ColorVector(prec).set(r0)
                     gpu.stagD2(sd_of(s, par), fr, s.g, fx, 0.5, -0.25)
                     got.append(fr.get())
+                # the multi-RHS hop: 8 fields per link load
+                fxs = [lo.ColorVector(prec).set(np.roll(x, j)) for j in range(8)]
+                frs = [lo.ColorVector(prec).set(r0) for _ in range(8)]
+                gpu.stagD2multi(s.sa, frs, s.g, fxs, 0.5, -0.25)
+                got += [f.get() for f in frs]
             for sl, sll, rc in solves:
                 fs = lo.ColorVector()
                 sp = gpu.SolverParams()
@@ -320,6 +326,12 @@ def test_link_cache_budgets_bitwise(oracle, gpu, L, hisq):
                         want = r0.copy()
                         O.stagD2(L, g, par, want, x, 0.5, -0.25)
                         assert rel(ref[k], want) < tol, (prec, par)
+                        k += 1
+                    for j in range(8):
+                        want = r0.copy()
+                        for par in (0, 1):
+                            O.stagD2(L, g, par, want, np.roll(x, j), 0.5, -0.25)
+                        assert rel(ref[k], want) < tol, (prec, "multi", j)
                         k += 1
                 for _ in solves:
                     r = b - O.stagD2xx(L, g, 0, ref[k], 0.01)
