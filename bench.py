@@ -7,10 +7,11 @@ algorithmic bandwidth: 1248 B per output site (8 FP64 links + input and
 output color vectors) x 524288 sites / average hop time, measured with HIP
 events on the library stream.  Inputs are synthetic (U = exp(i*0.5*H),
 complex-Gaussian sources, seeded Philox) and resident in HBM before timing.
-The links of the directions that fit in the tuner's link-cache budget are
-read with cache-allocating loads and stay in the last-level cache from one
-step to the next, as they do from one sweep of a solver to the next;
-roofline.links_streamed times the same hop with every link read from HBM.
+The link blocks (one direction and parity each) that fit in the tuner's
+link-cache budget are read with cache-allocating loads and stay in the
+last-level cache from one step to the next, as they do from one sweep of a
+solver to the next; roofline.links_streamed times the same hop with every link
+read from HBM.
 
 The same JSON line carries the CG comparison (FP64 CG vs FP32-inner CG with
 FP64 reliable updates, and the FP16/BF16-storage variants) at mass 0.01 to
@@ -681,8 +682,8 @@ def main():
                        "kernel": "k_hop<P64,P64,P64,P64,double2,4,0,0>",
                        "algorithmic_bytes_per_launch": HOP_REALS * 8 * vh,
                        "avg_launch_ms": round(ms, 5)}
-    # Link-cache budget of this hop (chosen by the tuner): the links of the
-    # first directions that fit in it are read with cache-allocating loads and,
+    # Link-cache budget of this hop (chosen by the tuner): the first link
+    # blocks that fit in it are read with cache-allocating loads and,
     # from one step to the next, come from the 256 MB last-level cache rather
     # than HBM, as in a solver, whose every sweep reads the same links.  The
     # line reports those bytes, an HBM-only estimate, and the same hop with
@@ -690,11 +691,12 @@ def main():
     tun = [v for k, v in Q.autotune_report().items() if k.startswith("hop nl=4 prec=0 rc=18 ")]
     lkc_mb = tun[0]["lkcache_mb"] if tun else 0
     stride = -(-vh // 256) * 256
-    dir_bytes = 2 * 144 * stride  # both parity blocks of one direction's FP64 links
-    ndir = min(4, (lkc_mb << 20) // dir_bytes)
-    cached = ndir * 2 * 144 * vh  # a launch reads both blocks of a direction over its vh sites
+    blk_bytes = 144 * stride  # one link block: one direction's FP64 links on one parity
+    nblk = min(8, (lkc_mb << 20) // blk_bytes)
+    cached = nblk * 144 * vh  # a launch reads every block once, over its vh sites
     alg = HOP_REALS * 8 * vh
     out["roofline"]["link_cache_MB"] = lkc_mb
+    out["roofline"]["link_blocks_cached"] = int(nblk)
     out["roofline"]["links_from_cache_bytes_per_launch"] = int(cached)
     out["roofline"]["hbm_bytes_per_launch_est"] = int(alg - cached)
     out["roofline"]["hbm_GBs_est"] = round((alg - cached) / (ms * 1e-3) / 1e9, 2)
