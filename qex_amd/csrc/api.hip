@@ -132,14 +132,6 @@ struct Work {
   void *xs = nullptr, *rs = nullptr, *Aps = nullptr, *ps = nullptr, *ts = nullptr;
   Scal *S = nullptr, *Sh = nullptr;
   Scal *S1 = nullptr;  // second scalar buffer of the form-4 CG iteration
-  // multi-shift
-  int nms = 0;
-  double **d_x = nullptr;  // mixed multi-shift: FP64 solution pointers
-  ShiftScal *T = nullptr, *Th = nullptr;
-  // mixed-precision multi-shift: FP32 shifted solutions and search vectors
-  int nms32 = 0;
-  std::vector<void *> msx32, msp32;
-  void **d_x32 = nullptr, **d_p32 = nullptr;
 };
 
 struct qexhip_links {
@@ -237,15 +229,9 @@ Work *work(qexhip_links *l) {
 void free_work(Work *w) {
   if (!w) return;
   for (void *p : {(void *)w->y, (void *)w->rho, (void *)w->t64, (void *)w->r64, (void *)w->p64, (void *)w->Ap64,
-                  w->xs, w->rs, w->Aps, w->ps, w->ts, (void *)w->S, (void *)w->S1, (void *)w->d_x,
-                  (void *)w->T})
+                  w->xs, w->rs, w->Aps, w->ps, w->ts, (void *)w->S, (void *)w->S1})
     if (p) (void)hipFree(p);
-  for (void *p : w->msx32) (void)hipFree(p);
-  for (void *p : w->msp32) (void)hipFree(p);
-  if (w->d_x32) (void)hipFree(w->d_x32);
-  if (w->d_p32) (void)hipFree(w->d_p32);
   if (w->Sh) (void)hipHostFree(w->Sh);
-  if (w->Th) (void)hipHostFree(w->Th);
   delete w;
 }
 
@@ -890,11 +876,16 @@ void multishift_true_r2(qexhip_links *l, int par, const double *const *xs, const
   }
 }
 
+void msb_core(qexhip_links *l, int par, int nrhs, const double *const *b, double *const *x, double m,
+              const double *sigmas, int nshift, const double *r2rel, const int *maxits, int flags, int pv, int *its,
+              double *b2);
+
 // Mixed-precision multi-shift CG, no QEX counterpart (cgm.nim runs in FP64
 // only).  Two phases:
-//  1. the multi-shift recursion (cgm.nim:68-246: k_cgm_alpha, epi_cgm) on FP32
-//     storage with FP32 links (FP32 stencil arithmetic, FP64 updates and
-//     reductions), x_k and p_k updated every iteration, until the base
+//  1. the multi-shift recursion (cgm.nim:68-246) on FP32 storage with FP32
+//     links (FP32 stencil arithmetic; FP64 recurrences, vector updates and
+//     reductions), the shifted x_k, p_k advanced once per window of
+//     iterations from the window's residuals (msb_core), until the base
 //     residual |r|^2 <= max(r2req, g_ms_sloppy_r2) |b|^2; a shift whose
 //     residual zeta_k^2 |r|^2 met that bound stops being updated;
 //  2. every shift refined from its phase-1 solution to r2req by the defect
@@ -904,81 +895,16 @@ void multishift_true_r2(qexhip_links *l, int par, const double *const *xs, const
 void solve_ms_mixed(qexhip_links *l, int par, double *const *xs, const double *b, double m, const double *sigmas,
                     int nshift, qexhip_solver_params *sp) {
   auto t0 = std::chrono::steady_clock::now();
-  Work *w = work(l);
-  qexhip_lattice *lat = l->lat;
-  const Geom &g = lat->g;
-  const size_t vb32 = (size_t)3 * g.stride * 8;
+  const Geom &g = l->lat->g;
   require(nshift >= 1 && nshift <= QH_MAXSHIFT, "multishift: nshift out of range");
   for (int k = 1; k < nshift; k++)
     require(m * m + 0.25 * sigmas[k] > 0.0, "multishift: a shifted system is not positive definite");
-  if (w->nms32 < nshift) {
-    for (int k = w->nms32; k < nshift; k++) {
-      w->msx32.push_back(dalloc<void>(vb32));
-      w->msp32.push_back(dalloc<void>(vb32));
-    }
-    if (w->d_x32) (void)hipFree(w->d_x32);
-    if (w->d_p32) (void)hipFree(w->d_p32);
-    w->d_x32 = dalloc<void *>(sizeof(void *) * nshift);
-    w->d_p32 = dalloc<void *>(sizeof(void *) * nshift);
-    w->nms32 = nshift;
-  }
-  if (w->nms < nshift) {  // the FP64 output pointer array
-    if (w->d_x) (void)hipFree(w->d_x);
-    w->d_x = dalloc<double *>(sizeof(double *) * nshift);
-    w->nms = nshift;
-  }
-  if (!w->T) {
-    w->T = dalloc<ShiftScal>(sizeof(ShiftScal));
-    HIPCHK(hipHostMalloc((void **)&w->Th, sizeof(ShiftScal), hipHostMallocDefault));
-  }
-  std::vector<double *> xv(xs, xs + nshift);
-  HIPCHK(hipMemcpyAsync(w->d_x, xv.data(), sizeof(double *) * nshift, hipMemcpyHostToDevice, g_stream));
-  HIPCHK(hipMemcpyAsync(w->d_x32, w->msx32.data(), sizeof(void *) * nshift, hipMemcpyHostToDevice, g_stream));
-  HIPCHK(hipMemcpyAsync(w->d_p32, w->msp32.data(), sizeof(void *) * nshift, hipMemcpyHostToDevice, g_stream));
-  Scal *S = w->S, *Sh = w->Sh;
-  ShiftScal *T = w->T, *Th = w->Th;
-  memset(Sh, 0, sizeof(Scal));
-  Sh->m2 = m * m;
-  push(S, Sh);
-  launch_norm2(P64, g, b, S, 1, lat->red, g_stream);
-  check_launch();
-  pull(Sh, S);
-  const double b2 = Sh->tmp[1];
   // phase 1
-  launch_cgm_start32(g, nshift, b, w->rs, w->d_x32, w->d_p32, g_stream);
-  memset(Th, 0, sizeof(ShiftScal));
-  Th->nshift = nshift;
-  Th->maxits = sp->maxits;
-  Th->r2stop = std::max(sp->r2req, g_ms_sloppy_r2) * b2;
-  Th->r2i = b2;
-  Th->alphaim1 = -1.0;
-  Th->betaim1 = 0.0;
-  Th->freeze = 1;
-  for (int k = 0; k < nshift; k++) {
-    Th->sg[k] = (k == 0) ? 0.0 : sigmas[k];
-    Th->zi[k] = Th->zim1[k] = 1.0;
-    Th->active[k] = 1;
-  }
-  Th->continuing = (b2 > Th->r2stop) && sp->maxits > 0;
-  HIPCHK(hipMemcpyAsync(T, Th, sizeof(ShiftScal), hipMemcpyHostToDevice, g_stream));
-  Sh->done = !Th->continuing;
-  push(S, Sh);
-  void *p0 = w->msp32[0];
-  while (Th->continuing) {
-    for (int c = 0; c < kChunk; c++) {
-      hop(l, P32, 18, 1 - par, 0, 0, p0, w->ts, nullptr, nullptr, 0, 0, 1.0, 0, 1, S);
-      hop(l, P32, 18, par, 1, 1, w->ts, w->Aps, p0, nullptr, 1.0, 0, -1.0, 1, 1, S);
-      launch_cgm_alpha(T, S, g_stream);
-      launch_cgm_xr32(g, nshift, w->rs, w->Aps, w->d_x32, w->d_p32, T, S, lat->red, g_stream);
-      launch_cgm_p32(g, nshift, w->rs, w->d_p32, T, g_stream);
-    }
-    check_launch();
-    HIPCHK(hipMemcpyAsync(Th, T, sizeof(ShiftScal), hipMemcpyDeviceToHost, g_stream));
-    HIPCHK(hipStreamSynchronize(g_stream));
-  }
-  const int its1 = Th->its;
-  launch_cgm_export32(g, nshift, w->d_x32, w->d_x, g_stream);
-  check_launch();
+  const double r2rel1 = std::max(sp->r2req, g_ms_sloppy_r2);
+  const int flags = QEXHIP_BATCH_GRAPH | (g_autotune ? QEXHIP_BATCH_TUNE : 0);
+  int its1 = 0;
+  double b2 = 0.0;
+  msb_core(l, par, 1, &b, xs, m, sigmas, nshift, &r2rel1, &sp->maxits, flags, P32, &its1, &b2);
   // phase 2
   int its = its1, nrel = 0;
   const double r2stop = sp->r2req * b2;
@@ -1108,8 +1034,8 @@ long batch_table(BatchWs *w, int block, Scal *S, BatchItem *dst, long lkc = 0) {
     it.blk0 = b0;
     it.nblk = (g.Vh + block - 1) / block;
     const RedWs slice{w->partials + b0, (int)w->maxblocks};
-    it.fin = FinArgs{slice, (int)it.nblk, S + k, nullptr, 0, 0.0, 1};
-    it.finf = FinArgs{slice, (int)it.nblk, S + k, nullptr, 0, 0.0, 0};
+    it.fin = FinArgs{slice, (int)it.nblk, S + k, 0, 0.0, 1};
+    it.finf = FinArgs{slice, (int)it.nblk, S + k, 0, 0.0, 0};
     b0 += it.nblk;
   }
   HIPCHK(hipMemcpyAsync(dst, T.data(), sizeof(BatchItem) * w->n, hipMemcpyHostToDevice, g_stream));
@@ -1296,6 +1222,8 @@ struct MsbWs {
   long chunk = -1;   // site walk of the hops baked into the captured graph
   int batches = 0;   // vector-load batches of the hops baked into the captured graph
   long lkc = -2;     // link-cache bytes of the hops baked into the captured graph
+  int pv = P64;      // vector storage and link precision of the captured graph (P64; P32: one source)
+  double **dxo = nullptr;  // FP32 storage: device array of the FP64 output pointers
   double m2 = -1.0;  // mass^2 baked into the captured graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1304,6 +1232,7 @@ struct MsbWs {
       for (double *q : *v) (void)hipFree(q);
     (void)hipFree(dx);
     (void)hipFree(dp);
+    (void)hipFree(dxo);
     (void)hipFree(hist);
     (void)hipFree(coef);
     (void)hipFree(cinfo);
@@ -1395,26 +1324,41 @@ MsbPtrs msb_ptrs(MsbWs *w, int j0, int nr) {
 // end of a window of iterations: every group's shifted x_k, p_k brought up to date
 void msb_flush(MsbWs *w, MsbScal *T) {
   for (auto &gr : w->groups)
-    launch_msb_flush(w->l->lat->g, gr.second, w->nshift, msb_ptrs(w, gr.first, gr.second), T + gr.first, g_stream);
+    launch_msb_flush(w->l->lat->g, w->pv, gr.second, w->nshift, msb_ptrs(w, gr.first, gr.second), T + gr.first,
+                     g_stream);
 }
 
 void msb_iter(MsbWs *w, MsbScal *T, double m2, int block, long chunk, int batches, long lkc) {
   qexhip_links *l = w->l;
   require(mrhs_grid(l->lat->g, block, chunk) <= (unsigned)w->red.maxblocks, "multi-shift batch: hop grid too large");
-  const char *links = links_store(l, P64, 18);
-  const long lkblk = lkblk_bytes(l, P64, 18);
+  const char *links = links_store(l, w->pv, 18);
+  const long lkblk = lkblk_bytes(l, w->pv, 18);
   Geom g = l->lat->g;
   g.lkcache = lkc;
   for (auto &gr : w->groups)
-    launch_msb_iter(g, links, lkblk, l->nl, w->par, gr.second, w->nshift, m2,
+    launch_msb_iter(g, w->pv, links, lkblk, l->nl, w->par, gr.second, w->nshift, m2,
                     msb_ptrs(w, gr.first, gr.second), T + gr.first, w->red, block, chunk, batches, g_stream);
 }
 
-void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexhip_field *const *bf, double m,
-               const double *sigmas, int nshift, qexhip_solver_params *sp, int flags) {
-  auto t0 = std::chrono::steady_clock::now();
+// The batched multi-shift CG (cgm.nim:68-246) with deferred shift updates,
+// on storage pv: P64 (FP64 vectors, links and arithmetic), or P32 for one
+// right-hand side (FP32 vectors and links, FP32 stencil arithmetic, FP64
+// updates, recurrences and reductions: the first phase of the mixed solve).
+// From b[j] (FP64, own parity, device) into x[j*nshift + k] (FP64, device),
+// each right-hand side until |r|^2 <= r2rel[j] |b_j|^2 or maxits[j]
+// iterations; its[j] and b2[j] = |b_j|^2 are returned.
+void msb_core(qexhip_links *l, int par, int nrhs, const double *const *b, double *const *x, double m,
+              const double *sigmas, int nshift, const double *r2rel, const int *maxits, int flags, int pv, int *its,
+              double *b2) {
+  require(pv == P64 || (pv == P32 && nrhs == 1), "multi-shift batch: FP64, or FP32 storage for one source");
   MsbWs *w = get_msb(l, par, nrhs, nshift);
-  links_store(l, P64, 18);  // packed before any capture (it may allocate)
+  if (w->pv != pv) {  // the captured graph holds the storage precision's kernels
+    w->pv = pv;
+    if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
+    if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
+    w->block = 0;
+  }
+  links_store(l, pv, 18);  // packed before any capture (it may allocate)
   qexhip_lattice *lat = l->lat;
   const size_t vb = vec64_bytes(lat);
   const double m2 = m * m;
@@ -1425,8 +1369,8 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
   long lkc = 0;
   if ((flags & QEXHIP_BATCH_TUNE) && !g_block_forced) {
     const std::string key = "msb nl=" + std::to_string(l->nl) + " nrhs=" + std::to_string(nrhs) +
-                            " nshift=" + std::to_string(nshift) + " walk=" + std::to_string(g_msb_walk) + " " +
-                            geom_key(lat->g);
+                            " nshift=" + std::to_string(nshift) + " walk=" + std::to_string(g_msb_walk) +
+                            (pv == P32 ? " fp32 " : " ") + geom_key(lat->g);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
       block = it->second.block;
@@ -1492,28 +1436,32 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
     if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
     if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
   }
-  // r := b ; p_k := b ; x_k := 0 (cgm.nim:122-158)
-  std::vector<double> b2(nrhs);
+  // r := b ; p_k := b ; x_k := 0 (cgm.nim:122-158), in the storage precision
   for (int j = 0; j < nrhs; j++) {
-    HIPCHK(hipMemcpyAsync(w->r[j], bf[j]->par(par), vb, hipMemcpyDeviceToDevice, g_stream));
-    for (int k = 0; k < nshift; k++) {
-      HIPCHK(hipMemcpyAsync(w->p[j * nshift + k], w->r[j], vb, hipMemcpyDeviceToDevice, g_stream));
-      launch_fill(w->x[j * nshift + k], vb, g_stream);
+    if (pv == P64) {
+      HIPCHK(hipMemcpyAsync(w->r[j], b[j], vb, hipMemcpyDeviceToDevice, g_stream));
+      for (int k = 0; k < nshift; k++) {
+        HIPCHK(hipMemcpyAsync(w->p[j * nshift + k], w->r[j], vb, hipMemcpyDeviceToDevice, g_stream));
+        launch_fill(w->x[j * nshift + k], vb, g_stream);
+      }
+    } else {
+      launch_cgm_start32(lat->g, nshift, b[j], w->r[j], (void *const *)(w->dx + j * nshift),
+                         (void *const *)(w->dp + j * nshift), g_stream);
     }
-    launch_norm2(P64, lat->g, w->r[j], lat->S, 1, lat->red, g_stream);
+    launch_norm2(P64, lat->g, b[j], lat->S, 1, lat->red, g_stream);
     check_launch();
     pull(lat->Sh, lat->S);
     b2[j] = lat->Sh->tmp[1];
     MsbScal &h = w->Th[j];
     memset(&h, 0, sizeof h);
     h.r2i = b2[j];
-    h.r2stop = sp[j].r2req * b2[j];
+    h.r2stop = r2rel[j] * b2[j];
     h.alphaim1 = -1.0;
     h.betaim1 = 0.0;
-    h.maxits = sp[j].maxits;
+    h.maxits = maxits[j];
     h.nshift = nshift;
     h.freeze = (flags & QEXHIP_BATCH_NOFREEZE) ? 0 : 1;
-    h.continuing = (b2[j] > h.r2stop) && sp[j].maxits > 0;
+    h.continuing = (b2[j] > h.r2stop) && maxits[j] > 0;
     for (int k = 0; k < nshift; k++) {
       h.sg[k] = (k == 0) ? 0.0 : sigmas[k];
       h.zi[k] = h.zim1[k] = 1.0;
@@ -1542,21 +1490,45 @@ void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexh
     HIPCHK(hipStreamSynchronize(g_stream));
   }
   for (int j = 0; j < nrhs; j++) require(!w->Th[j].winflow, "multi-shift batch: update window overflow");
-  for (int j = 0; j < nrhs; j++)
-    for (int k = 0; k < nshift; k++)
-      HIPCHK(hipMemcpyAsync(xs[j * nshift + k]->par(par), w->x[j * nshift + k], vb, hipMemcpyDeviceToDevice,
-                            g_stream));
+  if (pv == P64) {
+    for (int j = 0; j < nrhs; j++)
+      for (int k = 0; k < nshift; k++)
+        HIPCHK(hipMemcpyAsync(x[j * nshift + k], w->x[j * nshift + k], vb, hipMemcpyDeviceToDevice, g_stream));
+  } else {  // FP64 copies of the FP32 solutions
+    if (!w->dxo) w->dxo = dalloc<double *>(sizeof(double *) * nrhs * nshift);
+    HIPCHK(hipMemcpyAsync(w->dxo, x, sizeof(double *) * nrhs * nshift, hipMemcpyHostToDevice, g_stream));
+    launch_cgm_export32(lat->g, nshift, (void *const *)w->dx, w->dxo, g_stream);
+    check_launch();
+  }
   HIPCHK(hipStreamSynchronize(g_stream));
+  for (int j = 0; j < nrhs; j++) its[j] = w->Th[j].its;
+}
+
+void solve_msb(qexhip_links *l, int par, int nrhs, qexhip_field *const *xs, qexhip_field *const *bf, double m,
+               const double *sigmas, int nshift, qexhip_solver_params *sp, int flags) {
+  auto t0 = std::chrono::steady_clock::now();
+  qexhip_lattice *lat = l->lat;
+  std::vector<const double *> b(nrhs);
+  std::vector<double *> x((size_t)nrhs * nshift);
+  std::vector<double> r2rel(nrhs), b2(nrhs);
+  std::vector<int> maxits(nrhs), its(nrhs);
   for (int j = 0; j < nrhs; j++) {
-    std::vector<const double *> xj(nshift);
-    for (int k = 0; k < nshift; k++) xj[k] = (const double *)xs[j * nshift + k]->par(par);
-    multishift_true_r2(l, par, xj.data(), (const double *)bf[j]->par(par), b2[j], m, sigmas, nshift, &sp[j]);
+    b[j] = (const double *)bf[j]->par(par);
+    r2rel[j] = sp[j].r2req;
+    maxits[j] = sp[j].maxits;
+    for (int k = 0; k < nshift; k++) x[j * nshift + k] = (double *)xs[j * nshift + k]->par(par);
+  }
+  msb_core(l, par, nrhs, b.data(), x.data(), m, sigmas, nshift, r2rel.data(), maxits.data(), flags, P64, its.data(),
+           b2.data());
+  for (int j = 0; j < nrhs; j++) {
+    std::vector<const double *> xj(x.begin() + j * nshift, x.begin() + (j + 1) * nshift);
+    multishift_true_r2(l, par, xj.data(), b[j], b2[j], m, sigmas, nshift, &sp[j]);
   }
   const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (int j = 0; j < nrhs; j++) {
-    sp[j].iterations = w->Th[j].its;
+    sp[j].iterations = its[j];
     sp[j].reliable_updates = 0;
-    sp[j].flops = flops_per_its(l) * w->Th[j].its + (double)(4 * lat->g.Vh + 15) * w->Th[j].its * nshift;
+    sp[j].flops = flops_per_its(l) * its[j] + (double)(4 * lat->g.Vh + 15) * its[j] * nshift;
     sp[j].seconds = sec;
   }
 }

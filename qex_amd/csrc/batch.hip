@@ -136,22 +136,23 @@ void launch_bcg_flush(const BatchItem *T, int n, long nblocks, int block, hipStr
 // the t +-1, +-3 neighbours of the 8 gathered vectors are read while the
 // same chunk's nearby t-slices are in flight); a workgroup may serve several
 // logical blocks and writes one set of partial sums.
-template <int NL, int NR, int MODE, int NB>
+template <int PV, class A, int NL, int NR, int MODE, int NB>
 __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, long lkblk, int q, double m4,
                                                  MsbPtrs P, const MsbScal *T, RedWs red, long chunk) {
   static_assert(NR % NB == 0, "NB must divide NR");
+  using R = RT<A>;
   __shared__ double lds[16 * 3 * NR];
   const long spb = blockDim.x;
   const Walk wk = make_walk(g, spb, chunk);
   // one logical block per workgroup (the launch has a partial-sum slot for each)
   const BlockIter it = block_iter(wk.nblk, 1);
   const long i = it.cur < it.end ? walk_site(wk, it.cur, spb, threadIdx.x) : -1;
-  double2 acc[NR][3];
+  A acc[NR][3];
   if (i >= 0 && i < g.Vh) {
 #pragma unroll
     for (int r = 0; r < NR; r++)
 #pragma unroll
-      for (int c = 0; c < 3; c++) acc[r][c] = make_double2(0.0, 0.0);
+      for (int c = 0; c < 3; c++) acc[r][c] = mk<A>(0, 0);
     const Site s = decode(g, i, q);
     // one link j, loaded non-temporally when ntc is true (link_cached_blocks)
     auto trip = [&](int j, auto ntc) {
@@ -159,19 +160,19 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
       const int mu = (NL == 4) ? k : (k >> 1);
       const int len = (NL == 8 && (k & 1)) ? 3 : 1;
       const long n = nbr(g, s, mu, fwd ? len : -len);
-      double2 U[9];
-      ld_link<P64, 18, decltype(ntc)::value, double2, link_blocked<P64, NL>()>(
-          links + (long)(2 * k + (fwd ? q : 1 - q)) * lkblk, g.stride, fwd ? i : n, U, 1.0);
+      A U[9];
+      ld_link<PV, 18, decltype(ntc)::value, A, link_blocked<PV, NL>()>(
+          links + (long)(2 * k + (fwd ? q : 1 - q)) * lkblk, g.stride, fwd ? i : n, U, (R)1.0);
       // the right-hand sides' vectors in batches of NB, each batch's loads
       // issued before its arithmetic
 #pragma unroll
       for (int r0 = 0; r0 < NR; r0 += NB) {
-        double2 v[NB][3];
+        A v[NB][3];
 #pragma unroll
         for (int b = 0; b < NB; b++)
 #pragma unroll
           for (int c = 0; c < 3; c++)
-            v[b][c] = ldc<P64, double2>(MODE == 0 ? (const void *)P.p0[r0 + b] : P.t[r0 + b], vix(g, c, n));
+            v[b][c] = ldc<PV, A>(MODE == 0 ? (const void *)P.p0[r0 + b] : P.t[r0 + b], vix(g, c, n));
         if (fwd) {
 #pragma unroll
           for (int b = 0; b < NB; b++)
@@ -213,9 +214,8 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         if (!T[r].continuing) continue;
-        double2 *dst = (double2 *)P.t[r];
 #pragma unroll
-        for (int c = 0; c < 3; c++) dst[vix(g, c, i)] = acc[r][c];
+        for (int c = 0; c < 3; c++) stc<PV, A>(P.t[r], vix(g, c, i), acc[r][c]);
       }
     }
   } else {
@@ -226,18 +226,16 @@ __global__ void __launch_bounds__(256) k_mrhs_hop(Geom g, const char *links, lon
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         if (!T[r].continuing) continue;
-        const double2 *pp = (const double2 *)P.p0[r];
-        const double2 *rr = (const double2 *)(P.r[r] + (long)(T[r].its % QH_MSB_RING) * P.rstride);
-        double2 *dst = (double2 *)P.Ap[r];
+        const double *rr = P.r[r] + (long)(T[r].its % QH_MSB_RING) * P.rstride;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           const long o = vix(g, c, i);
-          const double2 pv = pp[o], rv = rr[o];
-          const double2 av = make_double2(m4 * pv.x - acc[r][c].x, m4 * pv.y - acc[r][c].y);
-          dst[o] = av;
-          sums[3 * r] += pv.x * av.x + pv.y * av.y;
-          sums[3 * r + 1] += rv.x * av.x + rv.y * av.y;
-          sums[3 * r + 2] += av.x * av.x + av.y * av.y;
+          const A pv = ldc<PV, A>(P.p0[r], o), rv = ldc<PV, A>(rr, o);
+          const A av = rt_store<PV, A>(mk<A>((R)m4 * pv.x - acc[r][c].x, (R)m4 * pv.y - acc[r][c].y));
+          stc<PV, A>(P.Ap[r], o, av);
+          sums[3 * r] += (double)pv.x * (double)av.x + (double)pv.y * (double)av.y;
+          sums[3 * r + 1] += (double)rv.x * (double)av.x + (double)rv.y * (double)av.y;
+          sums[3 * r + 2] += (double)av.x * (double)av.x + (double)av.y * (double)av.y;
         }
       }
     }
@@ -463,11 +461,17 @@ __global__ void __launch_bounds__(1024) k_msb_fin1(RedWs red, int nb, MsbScal *T
   }
 }
 
+// v as storage precision PV holds it (FP32 storage rounds; FP64 is exact)
+template <int PV> __device__ __forceinline__ double2 as_stored(double2 v) {
+  if constexpr (PV == P32) return make_double2((double)(float)v.x, (double)(float)v.y);
+  return v;
+}
+
 // One iteration's vector update of every right-hand side:
 //   r_new = r - alpha Ap into the next ring slot, p0 = r_new + beta p0 (p0 saved
 //   at the first iteration of a window), partial sums of the exact |r_new|^2.
 // The shifted x_k, p_k (and x_0) wait for the end of the window (k_msb_flush).
-template <int NR>
+template <int PV, int NR>
 __global__ void __launch_bounds__(256) k_msb_step(Geom g, MsbPtrs P, const MsbScal *T, RedWs red) {
   __shared__ double lds[16 * NR];
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -482,19 +486,18 @@ __global__ void __launch_bounds__(256) k_msb_step(Geom g, MsbPtrs P, const MsbSc
       const int it = t->its - 1;  // this iteration (fin1 counted it already)
       const double alpha = t->alpha, beta = t->beta;
       const bool first = (it == t->its0);
-      const double2 *rv = (const double2 *)(P.r[r] + (long)(it % QH_MSB_RING) * P.rstride);
-      double2 *rw = (double2 *)(P.r[r] + (long)((it + 1) % QH_MSB_RING) * P.rstride);
-      const double2 *av = (const double2 *)P.Ap[r];
-      double2 *p0 = (double2 *)P.p0[r], *p0s = (double2 *)P.p0s[r];
+      const double *rv = P.r[r] + (long)(it % QH_MSB_RING) * P.rstride;
+      double *rw = P.r[r] + (long)((it + 1) % QH_MSB_RING) * P.rstride;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const long o = vix(g, c, i);
-        const double2 a = av[o], x = rv[o], pv = p0[o];
-        const double2 rn = make_double2(x.x - alpha * a.x, x.y - alpha * a.y);
-        rw[o] = rn;
+        const double2 a = ldc<PV, double2>(P.Ap[r], o), x = ldc<PV, double2>(rv, o), pv = ldc<PV, double2>(P.p0[r], o);
+        // r_new as stored (its norm is that of the stored vector)
+        const double2 rn = as_stored<PV>(make_double2(x.x - alpha * a.x, x.y - alpha * a.y));
+        stc<PV, double2>(rw, o, rn);
         rr[r] += rn.x * rn.x + rn.y * rn.y;
-        if (first) p0s[o] = pv;
-        p0[o] = make_double2(rn.x + beta * pv.x, rn.y + beta * pv.y);
+        if (first) stc<PV, double2>(P.p0s[r], o, pv);
+        stc<PV, double2>(P.p0[r], o, make_double2(rn.x + beta * pv.x, rn.y + beta * pv.y));
       }
     }
   }
@@ -551,7 +554,7 @@ __global__ void __launch_bounds__(NR * QH_MAXSHIFT) k_msb_coef(MsbPtrs P, MsbSca
 // (4 doubles each) stay in registers and each residual of the window is read
 // once per component for all of them.
 #define QH_MSB_FLUSH_K 16
-template <int NR>
+template <int PV, int NR>
 __global__ void __launch_bounds__(256) k_msb_flush(Geom g, int nshift, MsbPtrs P) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.Vh) return;
@@ -580,14 +583,14 @@ __global__ void __launch_bounds__(256) k_msb_flush(Geom g, int nshift, MsbPtrs P
           if (!act[q]) continue;
           const int k = k0 + q;
           const double *cf = P.coef + ((long)r * nshift + k) * QH_MSB_NCOEF;
-          const double2 xv = ((const double2 *)P.x[r * nshift + k])[o];
-          const double2 pv = ((const double2 *)(k == 0 ? P.p0s[r] : P.p[r * nshift + k]))[o];
+          const double2 xv = ldc<PV, double2>(P.x[r * nshift + k], o);
+          const double2 pv = ldc<PV, double2>(k == 0 ? P.p0s[r] : P.p[r * nshift + k], o);
           ax[q] = make_double2(xv.x + cf[0] * pv.x, xv.y + cf[0] * pv.y);
           ap[q] = make_double2(cf[1] * pv.x, cf[1] * pv.y);
         }
 #pragma unroll 1
         for (int s = 1; s <= w; s++) {
-          const double2 rv = ((const double2 *)(ring + (long)((base + s) % QH_MSB_RING) * P.rstride))[o];
+          const double2 rv = ldc<PV, double2>(ring + (long)((base + s) % QH_MSB_RING) * P.rstride, o);
 #pragma unroll
           for (int q = 0; q < QH_MSB_FLUSH_K; q++) {
             if (!act[q]) continue;
@@ -603,8 +606,8 @@ __global__ void __launch_bounds__(256) k_msb_flush(Geom g, int nshift, MsbPtrs P
         for (int q = 0; q < QH_MSB_FLUSH_K; q++) {
           if (!act[q]) continue;
           const int k = k0 + q;
-          ((double2 *)P.x[r * nshift + k])[o] = ax[q];
-          if (k > 0) ((double2 *)P.p[r * nshift + k])[o] = ap[q];
+          stc<PV, double2>(P.x[r * nshift + k], o, ax[q]);
+          if (k > 0) stc<PV, double2>(P.p[r * nshift + k], o, ap[q]);
         }
       }
     }
@@ -656,14 +659,18 @@ unsigned mrhs_grid(const Geom &g, int block, long chunk) {
   return grid_blocks(wk.nblk, 1, 0);
 }
 
-template <int NL, int NR, int NB>
+// PV: vector storage and link precision of the hops (P64 with FP64
+// arithmetic, or P32 with FP32 stencil arithmetic); A its arithmetic type
+template <int PV, class A, int NL, int NR, int NB>
 static void msb_hops_t(const Geom &g, const char *links, long lkblk, int par, double m2, const MsbPtrs &P,
                        const MsbScal *T, RedWs red, dim3 grid, dim3 blk, long chunk, hipStream_t st) {
-  hipLaunchKernelGGL((k_mrhs_hop<NL, NR, 0, NB>), grid, blk, 0, st, g, links, lkblk, 1 - par, 0.0, P, T, red, chunk);
-  hipLaunchKernelGGL((k_mrhs_hop<NL, NR, 1, NB>), grid, blk, 0, st, g, links, lkblk, par, 4.0 * m2, P, T, red, chunk);
+  hipLaunchKernelGGL((k_mrhs_hop<PV, A, NL, NR, 0, NB>), grid, blk, 0, st, g, links, lkblk, 1 - par, 0.0, P, T, red,
+                     chunk);
+  hipLaunchKernelGGL((k_mrhs_hop<PV, A, NL, NR, 1, NB>), grid, blk, 0, st, g, links, lkblk, par, 4.0 * m2, P, T, red,
+                     chunk);
 }
 
-template <int NL, int NR>
+template <int PV, class A, int NL, int NR>
 static void msb_iter_t(const Geom &g, const char *links, long lkblk, int par, int nshift, double m2,
                        const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, int batches, hipStream_t st) {
   const unsigned nb = mrhs_grid(g, block, chunk);
@@ -673,50 +680,67 @@ static void msb_iter_t(const Geom &g, const char *links, long lkblk, int par, in
   }
   const dim3 grid(nb), blk(block);
   if (batches >= 4 && NR >= 4)
-    msb_hops_t<NL, NR, (NR >= 4 ? NR / 4 : 1)>(g, links, lkblk, par, m2, P, T, red, grid, blk, chunk, st);
+    msb_hops_t<PV, A, NL, NR, (NR >= 4 ? NR / 4 : 1)>(g, links, lkblk, par, m2, P, T, red, grid, blk, chunk, st);
   else if (batches >= 2 && NR >= 2)
-    msb_hops_t<NL, NR, (NR >= 2 ? NR / 2 : 1)>(g, links, lkblk, par, m2, P, T, red, grid, blk, chunk, st);
+    msb_hops_t<PV, A, NL, NR, (NR >= 2 ? NR / 2 : 1)>(g, links, lkblk, par, m2, P, T, red, grid, blk, chunk, st);
   else
-    msb_hops_t<NL, NR, NR>(g, links, lkblk, par, m2, P, T, red, grid, blk, chunk, st);
+    msb_hops_t<PV, A, NL, NR, NR>(g, links, lkblk, par, m2, P, T, red, grid, blk, chunk, st);
   hipLaunchKernelGGL((k_msb_fin1<NR>), dim3(1), dim3(1024), 0, st, red, (int)nb, T, P.hist);
-  hipLaunchKernelGGL((k_msb_step<NR>), dim3((unsigned)((g.Vh + 255) / 256)), dim3(256), 0, st, g, P, T, red);
+  hipLaunchKernelGGL((k_msb_step<PV, NR>), dim3((unsigned)((g.Vh + 255) / 256)), dim3(256), 0, st, g, P, T, red);
   hipLaunchKernelGGL((k_msb_fin2<NR>), dim3(1), dim3(1024), 0, st, red, (int)((g.Vh + 255) / 256), T);
 }
 
+// FP64: 1, 2, 4 or 8 right-hand sides; FP32 storage (the mixed solve's first
+// phase): one
 template <int NL>
-static bool msb_nr(int nr, const Geom &g, const char *links, long lkblk, int par, int nshift, double m2,
+static bool msb_nr(int pv, int nr, const Geom &g, const char *links, long lkblk, int par, int nshift, double m2,
                    const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, int batches, hipStream_t st) {
+  if (pv == P32) {
+    if (nr != 1) return false;
+    msb_iter_t<P32, float2, NL, 1>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st);
+    return true;
+  }
+  if (pv != P64) return false;
   switch (nr) {
-    case 1: msb_iter_t<NL, 1>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st); return true;
-    case 2: msb_iter_t<NL, 2>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st); return true;
-    case 4: msb_iter_t<NL, 4>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st); return true;
-    case 8: msb_iter_t<NL, 8>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st); return true;
+    case 1: msb_iter_t<P64, double2, NL, 1>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st); return true;
+    case 2: msb_iter_t<P64, double2, NL, 2>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st); return true;
+    case 4: msb_iter_t<P64, double2, NL, 4>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st); return true;
+    case 8: msb_iter_t<P64, double2, NL, 8>(g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st); return true;
   }
   return false;
 }
 
-template <int NR>
+template <int PV, int NR>
 static void msb_flush_t(const Geom &g, int nshift, const MsbPtrs &P, MsbScal *T, hipStream_t st) {
   hipLaunchKernelGGL((k_msb_coef<NR>), dim3(1), dim3(NR * QH_MAXSHIFT), 0, st, P, T);
-  hipLaunchKernelGGL((k_msb_flush<NR>), dim3((unsigned)((g.Vh + 255) / 256)), dim3(256), 0, st, g, nshift, P);
+  hipLaunchKernelGGL((k_msb_flush<PV, NR>), dim3((unsigned)((g.Vh + 255) / 256)), dim3(256), 0, st, g, nshift, P);
 }
 
-void launch_msb_flush(const Geom &g, int nr, int nshift, const MsbPtrs &P, MsbScal *T, hipStream_t st) {
+void launch_msb_flush(const Geom &g, int pv, int nr, int nshift, const MsbPtrs &P, MsbScal *T, hipStream_t st) {
+  if (pv == P32 && nr == 1) {
+    msb_flush_t<P32, 1>(g, nshift, P, T, st);
+    return;
+  }
+  if (pv != P64) {
+    fprintf(stderr, "qexhip: no multi-shift flush for storage %d, nr=%d\n", pv, nr);
+    return;
+  }
   switch (nr) {
-    case 1: msb_flush_t<1>(g, nshift, P, T, st); break;
-    case 2: msb_flush_t<2>(g, nshift, P, T, st); break;
-    case 4: msb_flush_t<4>(g, nshift, P, T, st); break;
-    case 8: msb_flush_t<8>(g, nshift, P, T, st); break;
+    case 1: msb_flush_t<P64, 1>(g, nshift, P, T, st); break;
+    case 2: msb_flush_t<P64, 2>(g, nshift, P, T, st); break;
+    case 4: msb_flush_t<P64, 4>(g, nshift, P, T, st); break;
+    case 8: msb_flush_t<P64, 8>(g, nshift, P, T, st); break;
     default: fprintf(stderr, "qexhip: no multi-shift flush for nr=%d\n", nr);
   }
 }
 
-void launch_msb_iter(const Geom &g, const char *links64, long lkblk, int nl, int par, int nr, int nshift,
+void launch_msb_iter(const Geom &g, int pv, const char *links, long lkblk, int nl, int par, int nr, int nshift,
                      double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, int batches,
                      hipStream_t st) {
-  const bool ok = (nl == 8) ? msb_nr<8>(nr, g, links64, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st)
-                            : msb_nr<4>(nr, g, links64, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st);
-  if (!ok) fprintf(stderr, "qexhip: no multi-RHS instantiation for nr=%d nl=%d\n", nr, nl);
+  const bool ok =
+      (nl == 8) ? msb_nr<8>(pv, nr, g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st)
+                : msb_nr<4>(pv, nr, g, links, lkblk, par, nshift, m2, P, T, red, block, chunk, batches, st);
+  if (!ok) fprintf(stderr, "qexhip: no multi-RHS instantiation for storage %d, nr=%d, nl=%d\n", pv, nr, nl);
 }
 
 }  // namespace qh

@@ -228,7 +228,7 @@ static bool hop_modes(int mode, int redm, const HopArgs &a, hipStream_t st, int 
     else if (a.two) hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R, 1>), grid, dim3(block), 0, st, a); \
     else hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R>), grid, dim3(block), 0, st, a);  \
     if (R) {                                                                                         \
-      FinArgs f{a.red, (int)grid.x, a.S, nullptr, 0, 6.0 * (double)a.g.Vh, a.pred};                  \
+      FinArgs f{a.red, (int)grid.x, a.S, 0, 6.0 * (double)a.g.Vh, a.pred};                  \
       launch_finish(R == 1 ? EPI_PAP : R == 2 ? EPI_RELIABLE : EPI_TMP, f, st);                      \
     }                                                                                                \
     return true;                                                                                     \
@@ -358,7 +358,7 @@ static void cg_iter_t(const CgArgs &a, hipStream_t st, int block, int form, int 
     hipLaunchKernelGGL((k_cg1<PL, RC, PSA, PRA, PRA, A, NL>), grid, blk, 0, st, a);
     hipLaunchKernelGGL((k_cg2<PL, RC, PSA, PRA, PRA, A, NL>), grid, blk, 0, st, a);
   }
-  FinArgs f{a.red, (int)(pers ? grid2p.x : grid.x), a.S, nullptr, 0, 0.0, 1};
+  FinArgs f{a.red, (int)(pers ? grid2p.x : grid.x), a.S, 0, 0.0, 1};
   launch_finish(EPI_CG2, f, st);
 }
 
@@ -468,7 +468,7 @@ void launch_cg_flush(int ps, int rx, const CgArgs &a, hipStream_t st) {
       case PBF: hipLaunchKernelGGL((k_cg_flush<PBF, P32, P32, float2>), grid, blk, 0, st, a); break;
     }
   }
-  FinArgs f{a.red, (int)grid.x, a.S, nullptr, 0, 0.0, 0};
+  FinArgs f{a.red, (int)grid.x, a.S, 0, 0.0, 0};
   launch_finish(EPI_FLUSH, f, st);
 }
 

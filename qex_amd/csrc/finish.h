@@ -95,25 +95,6 @@ __device__ inline void epi_flush(Scal *S, double rr) {
   if (!S->mixed) S->done = (S->its >= S->maxits) || (rr <= S->r2stop);
 }
 
-// multi-shift CG after r -= alpha Ap and |r|^2 (solvers/cgm.nim:176-186,200-202)
-__device__ inline void epi_cgm(ShiftScal *T, Scal *S, double r2ip1) {
-  T->r2ip1 = r2ip1;
-  T->its += 1;
-  const double beta = (T->r2i != 0.0) ? r2ip1 / T->r2i : 0.0;
-  T->beta = beta;
-  T->continuing = (T->its < T->maxits) && (r2ip1 > T->r2stop);
-  S->done = !T->continuing;
-  T->pc1[0] = 1.0;
-  T->pc2[0] = beta;
-  for (int k = 1; k < T->nshift; k++) {
-    T->pc2[k] *= beta;
-    if (T->freeze && T->zi[k] * T->zi[k] * r2ip1 <= T->r2stop) T->active[k] = 0;
-  }
-  T->alphaim1 = T->alpha;
-  T->betaim1 = beta;
-  T->r2i = r2ip1;
-}
-
 // One-block finish shape: 256 threads, 8 partials per thread loaded before any
 // add (2048 block partials in one trip).  Measured on the CG dot-product
 // finish at 32^4: 4.7-4.8 us, against 5.5-5.6 us with 1024 threads adding one
@@ -156,7 +137,6 @@ __device__ __forceinline__ void sum_partials(const RedWs &red, int nb, double (&
 template <int NV, int EPI>
 __device__ __forceinline__ void finish_body(const FinArgs &F, double *lds) {
   if (F.pred == 1 && (F.S->done | F.S->need_rel)) return;
-  if (F.pred == 3 && !F.T->continuing) return;
   double s[NV];
   sum_partials<NV>(F.red, F.nb, s, lds);
   if (threadIdx.x != 0) return;
@@ -165,7 +145,6 @@ __device__ __forceinline__ void finish_body(const FinArgs &F, double *lds) {
   else if (EPI == EPI_RELIABLE) epi_reliable(F.S, s[0], F.nreal);
   else if (EPI == EPI_CG2) epi_cg2(F.S, s[0], s[1 % NV], s[2 % NV], s[3 % NV]);
   else if (EPI == EPI_FLUSH) epi_flush(F.S, s[0]);
-  else if (EPI == EPI_CGM) epi_cgm(F.T, F.S, s[0]);
 }
 
 // The CG dot-product finish (EPI_CG2) on a copy of the scalars: Sout gets

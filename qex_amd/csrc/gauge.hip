@@ -141,7 +141,7 @@ static inline unsigned nb(long n) { return (unsigned)((n + 255) / 256); }
 void launch_plaquette(const Geom &g, const double *U, Scal *S, int slot, RedWs red, hipStream_t st) {
   const unsigned n = nb(2 * g.Vh);
   hipLaunchKernelGGL(k_plaquette, dim3(n), dim3(256), 0, st, g, (const double2 *)U, red);
-  FinArgs f{red, (int)n, S, nullptr, slot, 0.0, 0};
+  FinArgs f{red, (int)n, S, slot, 0.0, 0};
   launch_finish(EPI_TMP, f, st);
 }
 

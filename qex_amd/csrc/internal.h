@@ -41,18 +41,6 @@ struct Scal {
 #define QH_RES_GROWTH 10.0
 
 #define QH_MAXSHIFT 64
-struct ShiftScal {
-  double sg[QH_MAXSHIFT];
-  double zi[QH_MAXSHIFT], zim1[QH_MAXSHIFT];
-  double xc[QH_MAXSHIFT];   // x_k += xc[k] * p_k      (alpha*zr)
-  double pc1[QH_MAXSHIFT];  // p_k = pc1[k]*r + pc2[k]*p_k
-  double pc2[QH_MAXSHIFT];
-  double alpha, beta, alphaim1, betaim1, r2i, r2ip1;
-  int nshift, continuing, its, maxits;
-  double r2stop;
-  int freeze;                  // stop updating shift k once zeta_k^2 |r|^2 <= r2stop
-  int active[QH_MAXSHIFT];     // shifts still updated (all, unless freeze)
-};
 
 // Generic one-parity hop: out = ca*A + cb*B + chop*(2D)in on parity par.
 struct HopArgs {
@@ -91,15 +79,14 @@ struct CgArgs {
 
 // Finish step of a reduction (reduce.hip): sum the block partials of the
 // kernel just launched and apply one epilogue to the solver scalars.
-enum Epi { EPI_TMP = 0, EPI_PAP = 1, EPI_RELIABLE = 2, EPI_CG2 = 3, EPI_FLUSH = 4, EPI_CGM = 5 };
+enum Epi { EPI_TMP = 0, EPI_PAP = 1, EPI_RELIABLE = 2, EPI_CG2 = 3, EPI_FLUSH = 4 };
 struct FinArgs {
   RedWs red;
   int nb;             // blocks of the producing kernel
   Scal *S;
-  ShiftScal *T;
   int slot;           // EPI_TMP: S->tmp[slot]
   double nreal;       // EPI_RELIABLE: real components of one parity vector
-  int pred;           // 0 none; 1 skip when done|need_rel; 3 skip unless T->continuing
+  int pred;           // 0 none; 1 skip when done|need_rel
 };
 
 // One system of a batched FP64 CG (batch.hip): blocks [blk0, blk0+nblk) of
@@ -196,13 +183,9 @@ void launch_keep_best(const Geom &g, double *best, const double *x, const Scal *
 void launch_scal_init(Scal *S, hipStream_t st, int mode);
 // the multi-shift updates of the mixed solve's FP32 recursion (FP64 arithmetic);
 // x, p: device arrays of nshift vector pointers
-void launch_cgm_xr32(const Geom &g, int nshift, void *r, const void *Ap, void *const *x, void *const *p,
-                     ShiftScal *T, Scal *S, RedWs red, hipStream_t st);
-void launch_cgm_p32(const Geom &g, int nshift, const void *r, void *const *p, ShiftScal *T, hipStream_t st);
 void launch_cgm_start32(const Geom &g, int nshift, const double *b, void *r, void *const *x, void *const *p,
                         hipStream_t st);
 void launch_cgm_export32(const Geom &g, int nshift, void *const *x32, double *const *x64, hipStream_t st);
-void launch_cgm_alpha(ShiftScal *T, Scal *S, hipStream_t st);
 void launch_outer_acc(const Geom &g, const double *a_e, const double *a_o, const double *b_e, const double *b_o,
                       double ce, double co, double *F, int nl, hipStream_t st);
 void launch_force(const Geom &g, const char *links64, long lkblk, const double *psi_e, const double *psi_o,
@@ -217,12 +200,13 @@ bool launch_hop_multi(int prec, int nl, int nr, int batches, const Geom &g, cons
 // workgroups of the multi-RHS multi-shift hops at this block size and site walk
 unsigned mrhs_grid(const Geom &g, int block, long chunk);
 // chunk: site walk of the two multi-RHS hops (make_walk; 0 = canonical order);
-// batches: the hops load the right-hand sides' vectors of a link in 1, 2 or 4 batches
-void launch_msb_iter(const Geom &g, const char *links64, long lkblk, int nl, int par, int nr, int nshift,
+// batches: the hops load the right-hand sides' vectors of a link in 1, 2 or 4 batches;
+// pv: vector storage and link precision (P64; P32 for a group of one right-hand side)
+void launch_msb_iter(const Geom &g, int pv, const char *links, long lkblk, int nl, int par, int nr, int nshift,
                      double m2, const MsbPtrs &P, MsbScal *T, RedWs red, int block, long chunk, int batches,
                      hipStream_t st);
 // end of a window: every x_k, p_k of the group advanced over the window's iterations
-void launch_msb_flush(const Geom &g, int nr, int nshift, const MsbPtrs &P, MsbScal *T, hipStream_t st);
+void launch_msb_flush(const Geom &g, int pv, int nr, int nshift, const MsbPtrs &P, MsbScal *T, hipStream_t st);
 
 // gauge.hip
 void launch_plaquette(const Geom &g, const double *U, Scal *S, int slot, RedWs red, hipStream_t st);

@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(256) k_reduce(Geom g, const void *x, const voi
 }
 
 static void finish_tmp(unsigned nb, Scal *S, int slot, RedWs red, hipStream_t st) {
-  FinArgs f{red, (int)nb, S, nullptr, slot, 0.0, 0};
+  FinArgs f{red, (int)nb, S, slot, 0.0, 0};
   launch_finish(EPI_TMP, f, st);
 }
 
@@ -256,33 +256,6 @@ void launch_scal_init(Scal *S, hipStream_t st, int mode) {
 }
 
 // ------------------------------------------------------------- multi-shift
-// Multi-shift CG (solvers/cgm.nim:163-200).  After Ap and p.Ap:
-//   alpha = r2i/pAp ; for each shift k>0 the zeta recurrence gives
-//   xc[k] = alpha*zr (x_k update) and the p_k update coefficients.
-__global__ void k_cgm_alpha(ShiftScal *T, Scal *S) {
-  if (threadIdx.x != 0) return;
-  if (!T->continuing) return;
-  const double pAp = S->pAp;
-  const double alpha = (pAp != 0.0) ? T->r2i / pAp : 0.0;
-  T->alpha = alpha;
-  T->xc[0] = alpha;
-  for (int k = 1; k < T->nshift; k++) {
-    double zip1d = alpha * T->betaim1 * (T->zim1[k] - T->zi[k]);
-    zip1d += T->zim1[k] * T->alphaim1 * (1.0 + T->sg[k] * alpha);
-    const double zip1 = (zip1d != 0.0) ? T->zi[k] * T->zim1[k] * T->alphaim1 / zip1d : 0.0;
-    const double zr = (T->zi[k] != 0.0) ? zip1 / T->zi[k] : 0.0;
-    T->xc[k] = alpha * zr;
-    T->pc1[k] = zip1;
-    T->pc2[k] = zr * zr;  // times beta, known after |r|^2
-    T->zim1[k] = T->zi[k];
-    T->zi[k] = zip1;
-  }
-}
-
-void launch_cgm_alpha(ShiftScal *T, Scal *S, hipStream_t st) {
-  hipLaunchKernelGGL(k_cgm_alpha, dim3(1), dim3(64), 0, st, T, S);
-}
-
 // Loads and stores of one complex entry of an FP64 (double2) or FP32
 // (float2) parity vector, computing in FP64.
 __device__ __forceinline__ double2 ld2d(const double2 *p, long o) { return p[o]; }
@@ -293,80 +266,8 @@ __device__ __forceinline__ double2 ld2d(const float2 *p, long o) {
 __device__ __forceinline__ void st2d(double2 *p, long o, double2 v) { p[o] = v; }
 __device__ __forceinline__ void st2d(float2 *p, long o, double2 v) { p[o] = make_float2((float)v.x, (float)v.y); }
 
-// r -= alpha Ap ; x_k += xc[k] p_k (every active k) ; r2 = |r|^2
-template <class V>
-__global__ void __launch_bounds__(256) k_cgm_xr(Geom g, int nshift, V *r, const V *Ap, V *const *x, V *const *p,
-                                                ShiftScal *T, Scal *S, RedWs red) {
-  __shared__ double lds[16];
-  if (!T->continuing) return;
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  double s = 0.0;
-  if (i < g.Vh) {
-    const double alpha = T->alpha;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const long o = vix(g, c, i);
-      double2 rv = ld2d(r, o);
-      const double2 av = ld2d(Ap, o);
-      rv.x -= alpha * av.x; rv.y -= alpha * av.y;
-      st2d(r, o, rv);
-      s += rv.x * rv.x + rv.y * rv.y;
-    }
-    for (int k = 0; k < nshift; k++) {
-      if (!T->active[k]) continue;
-      const double xc = T->xc[k];
-      V *xk = x[k];
-      const V *pk = p[k];
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const long o = vix(g, c, i);
-        double2 xv = ld2d(xk, o);
-        const double2 pv = ld2d(pk, o);
-        xv.x += xc * pv.x; xv.y += xc * pv.y;
-        st2d(xk, o, xv);
-      }
-    }
-  }
-  double v[1] = {s};
-  block_partials<1>(v, lds, red);
-}
-
-// p_k = pc1[k] r + pc2[k] p_k (every active k, only while continuing, cgm.nim:186,198-199)
-template <class V>
-__global__ void k_cgm_p(Geom g, int nshift, const V *r, V *const *p, ShiftScal *T) {
-  if (!T->continuing) return;
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.Vh) return;
-  double2 rv[3];
-#pragma unroll
-  for (int c = 0; c < 3; c++) rv[c] = ld2d(r, vix(g, c, i));
-  for (int k = 0; k < nshift; k++) {
-    if (!T->active[k]) continue;
-    const double a = T->pc1[k], b = T->pc2[k];
-    V *pk = p[k];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const long o = vix(g, c, i);
-      const double2 pv = ld2d(pk, o);
-      st2d(pk, o, make_double2(a * rv[c].x + b * pv.x, a * rv[c].y + b * pv.y));
-    }
-  }
-}
-
-void launch_cgm_xr32(const Geom &g, int nshift, void *r, const void *Ap, void *const *x, void *const *p,
-                     ShiftScal *T, Scal *S, RedWs red, hipStream_t st) {
-  hipLaunchKernelGGL(k_cgm_xr<float2>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (float2 *)r,
-                     (const float2 *)Ap, (float2 *const *)x, (float2 *const *)p, T, S, red);
-  FinArgs f{red, (int)nblk(g.Vh, 256), S, T, 0, 0.0, 3};
-  launch_finish(EPI_CGM, f, st);
-}
-
-void launch_cgm_p32(const Geom &g, int nshift, const void *r, void *const *p, ShiftScal *T, hipStream_t st) {
-  hipLaunchKernelGGL(k_cgm_p<float2>, dim3(nblk(g.Vh, 256)), dim3(256), 0, st, g, nshift, (const float2 *)r,
-                     (float2 *const *)p, T);
-}
-
-// Start of the FP32 multi-shift recursion: r = b, p_k = b, x_k = 0 (cgm.nim:122-158)
+// Start of the multi-shift recursion on FP32 storage: r = b, p_k = b, x_k = 0
+// (cgm.nim:122-158)
 __global__ void k_cgm_start32(Geom g, int nshift, const double2 *b, float2 *r, float2 *const *x, float2 *const *p) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.Vh) return;

@@ -29,7 +29,6 @@ void launch_finish(int epi, const FinArgs &f, hipStream_t st) {
     case EPI_RELIABLE: hipLaunchKernelGGL((k_finish<1, EPI_RELIABLE>), g, b, 0, st, f); break;
     case EPI_CG2: hipLaunchKernelGGL((k_finish<4, EPI_CG2>), g, b, 0, st, f); break;
     case EPI_FLUSH: hipLaunchKernelGGL((k_finish<1, EPI_FLUSH>), g, b, 0, st, f); break;
-    case EPI_CGM: hipLaunchKernelGGL((k_finish<1, EPI_CGM>), g, b, 0, st, f); break;
   }
 }
 
