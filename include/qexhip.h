@@ -223,6 +223,11 @@ int qexhip_set_hop_tmarch(int slices);
    hop per trip (more loads in flight per wave, for lattices too small to fill
    the device), 0 = one link per trip, -1 = chosen by the tuner */
 int qexhip_set_hop_two(int two);
+/* threads per output site of the one-parity hop, no QEX counterpart: 2 or 4
+   split each site's links between that many threads, whose sums are added by
+   lane shuffles (more waves for lattices too small to fill the device); 0 or
+   1 = one thread per site; -1 = chosen by the tuner */
+int qexhip_set_hop_split(int split);
 /* site walk of the two multi-RHS hops of the batched multi-shift solve
    (qexhip_solve_multishift_batch), no QEX counterpart: same values as
    qexhip_set_hop_walk; -1 = chosen by the tuner (with QEXHIP_BATCH_TUNE, else

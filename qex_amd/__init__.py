@@ -8,7 +8,7 @@ from .stag import (peqDdag, rephase, stagD1, stagD1x, stagD2deriv, stagDb, stagD
                    Staggered, eoReconstruct, eoReduce, fermForce, newLayout, newSolverParams, newStag,
                    newStag3, solve, solveBatch, solveEE, solveMulti, solveMultiBatch, solveMultiMass, solveOO,
                    stagD, stagD2, stagD2ee, stagD2oo, stagForceSolve, time_hop, autotune_report, set_autotune,
-                   set_cg_form, set_cg_graph, set_hop_multi_shape, set_hop_walk, set_hop_tmarch, set_hop_two, set_msb_walk, set_cg_block1, set_cg_cap, set_link_cache, stagD2multi, stagDN, time_hop_multi,
+                   set_cg_form, set_cg_graph, set_hop_multi_shape, set_hop_walk, set_hop_tmarch, set_hop_two, set_hop_split, set_msb_walk, set_cg_block1, set_cg_cap, set_link_cache, stagD2multi, stagDN, time_hop_multi,
                    stream_read_gbs)
 from .hisq import (Fat7lCoefs, HisqCoefs, fat7lDeriv, hisqForce, hisqStag, makeImpLinks, newHISQ,  # noqa: F401,E402
                    projectU, projectUderiv, smearGetForce)

@@ -34,7 +34,7 @@ EXPORTS = [
     "qexhip_hisq_force", "qexhip_milc_read_info", "qexhip_milc_read", "qexhip_milc_write",
     "qexhip_scidac_read_info", "qexhip_scidac_read", "qexhip_scidac_create", "qexhip_scidac_append",
     "qexhip_time_scidac", "qexhip_from_qex_layout", "qexhip_to_qex_layout", "qexhip_outer_acc",
-    "qexhip_stagD2xx_noalloc", "qexhip_set_hop_multi_shape", "qexhip_set_hop_walk", "qexhip_set_hop_tmarch", "qexhip_set_hop_two", "qexhip_set_msb_walk", "qexhip_set_cg_block1", "qexhip_set_cg_cap", "qexhip_set_link_cache",
+    "qexhip_stagD2xx_noalloc", "qexhip_set_hop_multi_shape", "qexhip_set_hop_walk", "qexhip_set_hop_tmarch", "qexhip_set_hop_two", "qexhip_set_hop_split", "qexhip_set_msb_walk", "qexhip_set_cg_block1", "qexhip_set_cg_cap", "qexhip_set_link_cache",
 ]
 
 # qexhip_solve_batch / qexhip_solve_multishift_batch flags
@@ -127,7 +127,7 @@ def load(path=LIB_PATH):
         "qexhip_solve_batch": [I, vpp, I, vpp, vpp, dp, spp, I],
         "qexhip_solve_multishift_batch": [vp, I, I, vpp, vpp, D, dp, I, spp, I],
         "qexhip_set_autotune": [I], "qexhip_autotune_report": [ctypes.c_char_p, I],
-        "qexhip_set_cg_form": [I], "qexhip_set_hop_multi_shape": [I, ctypes.c_long], "qexhip_set_hop_walk": [ctypes.c_long], "qexhip_set_hop_tmarch": [I], "qexhip_set_hop_two": [I], "qexhip_set_msb_walk": [ctypes.c_long], "qexhip_set_cg_block1": [I], "qexhip_set_cg_cap": [I], "qexhip_set_link_cache": [ctypes.c_void_p, ctypes.c_long], "qexhip_set_cg_graph": [I],
+        "qexhip_set_cg_form": [I], "qexhip_set_hop_multi_shape": [I, ctypes.c_long], "qexhip_set_hop_walk": [ctypes.c_long], "qexhip_set_hop_tmarch": [I], "qexhip_set_hop_two": [I], "qexhip_set_hop_split": [I], "qexhip_set_msb_walk": [ctypes.c_long], "qexhip_set_cg_block1": [I], "qexhip_set_cg_cap": [I], "qexhip_set_link_cache": [ctypes.c_void_p, ctypes.c_long], "qexhip_set_cg_graph": [I],
         "qexhip_stagD2_multi": [vp, I, I, vpp, vpp, D, D], "qexhip_time_hop_multi": [vp, I, I, I, dp],
         "qexhip_stream_read": [ctypes.c_long, I, dp],
         "qexhip_make_imp_links": [vp, vp, vp, ctypes.POINTER(Fat7CoefsC), vp, vp, D],

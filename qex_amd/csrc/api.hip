@@ -50,6 +50,7 @@ long g_multi_chunk = -1;      // multi-RHS hop: site-walk chunk pinned (0: canon
 long g_hop_chunk = -1;        // one-parity hop: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 int g_hop_tmarch = -1;        // one-parity hop of plain links: t-march slices per thread pinned (0: off) or -1 (tuner)
 int g_hop_two = -1;           // one-parity hop: both links of a hop per stencil trip pinned (0/1) or -1 (tuner)
+int g_hop_split = -1;         // one-parity hop: threads per output site pinned (1, 2, 4) or -1 (tuner)
 long g_msb_walk = -1;         // batched multi-shift hops: site-walk chunk pinned (0: canonical order) or -1 (tuner)
 double g_ms_sloppy_r2 = 1e-12; // mixed multi-shift: FP32 phase ends at |r|^2 <= this |b|^2 (QEXHIP_MS_SLOPPY_R2)
 int g_cg_block1 = 0;          // CG forms 3/4: other-parity stencil kernel's block size pinned, 0 (tuner)
@@ -260,6 +261,7 @@ struct Tuned {
   long chunk = 0;   // site walk (make_walk): t-march chunk, 0 = canonical order
   int tmarch = 0;   // hop of plain links: t-slices per thread of the t-march form (0: off)
   int two = 0;      // hop: both links of a hop per stencil trip
+  int split = 1;    // hop: threads per output site (1, 2 or 4)
   long lkcache = 0; // link bytes read with cache-allocating loads (Geom::lkcache), for lattices that leave it to the tuner
 };
 std::map<std::string, Tuned> g_tune;
@@ -372,6 +374,7 @@ void hop_shaped(qexhip_links *l, const char *lk, const Tuned &tu, int prec, int 
   h.chunk = tu.chunk;
   h.tmarch = l->nl == 4 ? tu.tmarch : 0;
   h.two = tu.two;
+  h.split = tu.split;
   h.g.lkcache = link_cache_of(l->lat, tu);
   const int pv = (prec == PQ16) ? P16 : prec;  // 16-bit fixed-point links go with FP16 vectors
   HopKey k{prec, rc, pv, pv, pv};
@@ -386,9 +389,9 @@ void hop(qexhip_links *l, int prec, int rc, int par, int mode, int redm, const v
 
 std::string hop_key(const qexhip_links *l, int prec, int rc) {
   char kb[96];
-  if (g_hop_chunk >= 0 || g_hop_tmarch >= 0 || g_hop_two >= 0)
-    snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d walk=%ld tmarch=%d two=%d ", l->nl, prec, rc, g_hop_chunk,
-             g_hop_tmarch, g_hop_two);
+  if (g_hop_chunk >= 0 || g_hop_tmarch >= 0 || g_hop_two >= 0 || g_hop_split >= 0)
+    snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d walk=%ld tmarch=%d two=%d split=%d ", l->nl, prec, rc,
+             g_hop_chunk, g_hop_tmarch, g_hop_two, g_hop_split);
   else
     snprintf(kb, sizeof kb, "hop nl=%d prec=%d rc=%d ", l->nl, prec, rc);
   return kb + geom_key(l->lat->g);
@@ -403,6 +406,7 @@ Tuned cached_hop_shape(const qexhip_links *l, int prec, int rc) {
   t.chunk = g_hop_chunk > 0 ? g_hop_chunk : 0;
   t.tmarch = g_hop_tmarch > 0 ? g_hop_tmarch : 0;
   t.two = g_hop_two > 0 ? 1 : 0;
+  t.split = (g_hop_split == 2 || g_hop_split == 4) ? g_hop_split : 1;
   return t;
 }
 
@@ -431,7 +435,8 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
   // 3.26 instead of 3.58 GB per hop in 582 instead of 545 us; a candidate still)
   std::vector<long> chunks = {0L, 2048L, 8192L};
   if (g_hop_chunk >= 0) chunks = {g_hop_chunk};
-  if (g_hop_tmarch <= 0)
+  const bool split_pinned = g_hop_split == 2 || g_hop_split == 4;
+  if (g_hop_tmarch <= 0 && !split_pinned)
     for (long chunk : chunks) {
       if (chunk && g_hop_chunk < 0 && make_walk(h.g, 256, chunk).C == 0) continue;  // t-slices too small
       for (int cap : {0, 1024, 2048})
@@ -454,7 +459,7 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
   // the t-march form (plain links): t-slices per thread x block size x XCD order
   std::vector<int> tms = {4, 8, 16};
   if (g_hop_tmarch >= 0) tms = {g_hop_tmarch};
-  if (l->nl == 4 && g_hop_chunk <= 0)
+  if (l->nl == 4 && g_hop_chunk <= 0 && !split_pinned)
     for (int tm : tms) {
       if (tm <= 0) continue;
       for (int block : {64, 128, 256})
@@ -470,13 +475,37 @@ Tuned tuned_hop(qexhip_links *l, int prec, int rc) {
           }
         }
     }
+  // the split form: 2 or 4 threads per output site, each taking a share of
+  // its links (for lattices too small to fill the device with one per site)
+  std::vector<int> splits = {2, 4};
+  if (g_hop_split >= 0) splits = {g_hop_split};
+  if (g_hop_tmarch <= 0 && g_hop_chunk <= 0 && g_hop_two <= 0)
+    for (int sp : splits) {
+      if (sp != 2 && sp != 4) continue;
+      for (int block : {64, 128, 256})
+        for (int remap : {1, 0}) {
+          h.remap = remap;
+          h.chunk = 0;
+          h.tmarch = 0;
+          h.two = 0;
+          h.split = sp;
+          const double us = time_us([&] { launch_hop(hk, l->nl, 0, 0, h, g_stream, block, 0); }, 2, 8);
+          if (us < best.us) {
+            best = Tuned{block, remap, us, 2, 0};
+            best.split = sp;
+          }
+        }
+    }
+  h.split = 1;
   // the link-cache budget, on the chosen shape (the t-march form and the
-  // two-links trip read every link non-temporally)
+  // two-links trip read every link non-temporally; the split form caches the
+  // links only when the whole field fits the budget)
   if (l->lat->g.lkcache < 0 && !best.tmarch && !best.two) {
     h.remap = best.remap;
     h.chunk = best.chunk;
     h.tmarch = 0;
     h.two = 0;
+    h.split = best.split;
     double bus = 1e300;
     for (long mb : kLinkCacheMB) {
       h.g.lkcache = mb << 20;
@@ -2699,6 +2728,14 @@ int qexhip_set_hop_walk(long chunk) {
   API_END
 }
 
+int qexhip_set_hop_split(int split) {
+  API_BEGIN
+  require(split == -1 || split == 0 || split == 1 || split == 2 || split == 4,
+          "hop threads per site must be -1 (tuner), 0 or 1 (one), 2 or 4");
+  g_hop_split = split;
+  API_END
+}
+
 int qexhip_set_hop_two(int two) {
   API_BEGIN
   require(two >= -1 && two <= 1, "hop stencil trip must be -1 (tuner), 0 (one link) or 1 (both links of a hop)");
@@ -2752,9 +2789,11 @@ int qexhip_autotune_report(char *buf, int len) {
     // the key can be long (a batch lists every lattice): it is appended whole
     char line[256];
     snprintf(line, sizeof line,
-             ": block=%d remap=%d form=%d cap=%d block1=%d batches=%d chunk=%ld tmarch=%d two=%d lkcache=%ld us=%.2f\n",
+             ": block=%d remap=%d form=%d cap=%d block1=%d batches=%d chunk=%ld tmarch=%d two=%d split=%d "
+             "lkcache=%ld us=%.2f\n",
              kv.second.block, kv.second.remap, kv.second.form, kv.second.cap, kv.second.block1, kv.second.batches,
-             kv.second.chunk, kv.second.tmarch, kv.second.two, kv.second.lkcache >> 20, kv.second.us);
+             kv.second.chunk, kv.second.tmarch, kv.second.two, kv.second.split, kv.second.lkcache >> 20,
+             kv.second.us);
     out += kv.first;
     out += line;
   }

@@ -26,18 +26,25 @@ namespace qh {
 // the loads in flight per wave, 10-20 more registers): 17-27% faster at 16^4,
 // where too few waves fill the device to hide latency otherwise, and 1% at 48^4,
 // but 1% slower at 32^4; a tuned launch dimension.
-template <int PL, int RC, int PI, int PO, int PA, class A, int NL, int MODE, int RED, int TWO = 0>
+// NS > 1 (split form): NS adjacent threads per output site, each taking
+// 2 NL / NS of its links, their sums added across the group by lane shuffles;
+// the group's first thread finishes the site.  More waves and loads in flight
+// for lattices too small to fill the device with one thread per site.
+template <int PL, int RC, int PI, int PO, int PA, class A, int NL, int MODE, int RED, int TWO = 0, int NS = 1>
 __global__ void __launch_bounds__(256, QH_HOP_WAVES) k_hop(HopArgs P) {
   using R = RT<A>;
   __shared__ double lds[16];
   if (P.pred && (P.S->done | P.S->need_rel)) return;
   const Geom &g = P.g;
   double red = 0.0;
-  const Walk wk = make_walk(g, blockDim.x, P.chunk);
+  const long spb = blockDim.x / NS;  // sites per workgroup
+  const int sub = (int)(threadIdx.x % NS);
+  const Walk wk = make_walk(g, spb, P.chunk);
   const BlockIter it = block_iter(wk.nblk, P.remap);
 #pragma unroll 1
   for (long lb = it.cur; lb < it.end; lb += it.step) {
-    const long i = walk_site(wk, lb, blockDim.x, threadIdx.x);
+    // the NS threads of a site share i, so they skip together
+    const long i = walk_site(wk, lb, spb, threadIdx.x / NS);
     if (i < 0 || i >= g.Vh) continue;
     const Site s = decode(g, i, P.par);
     A res[3];
@@ -45,7 +52,12 @@ __global__ void __launch_bounds__(256, QH_HOP_WAVES) k_hop(HopArgs P) {
 #pragma unroll
       for (int c = 0; c < 3; c++) v[c] = ldc<PI, A>(P.in, vix(g, c, j));
     };
-    stencil_in<TWO ? -1 : stencil_form<PL>(), PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, P.par, i, s, ld, res);
+    if constexpr (NS == 1) {
+      stencil_in<TWO ? -1 : stencil_form<PL>(), PL, RC, NL, A>(g, P.links, P.lkblk, P.qs, P.par, i, s, ld, res);
+    } else {
+      stencil_share<PL, RC, NL, A, NS>(g, P.links, P.lkblk, P.qs, P.par, i, s, ld, sub, res);
+      if (sub != 0) continue;
+    }
     const R chop = (R)P.chop;
 #pragma unroll
     for (int c = 0; c < 3; c++) res[c] = mk<A>(chop * res[c].x, chop * res[c].y);
@@ -221,10 +233,13 @@ inline unsigned tmarch_grid(const Geom &g, int block, int tmarch, int remap) {
 template <int PL, int RC, int PI, int PO, int PA, class A, int NL>
 static bool hop_modes(int mode, int redm, const HopArgs &a, hipStream_t st, int block, int cap) {
   const bool tm = NL == 4 && a.tmarch > 0;
-  dim3 grid(tm ? tmarch_grid(a.g, block, a.tmarch, a.remap) : grid_blocks(make_walk(a.g, block, a.chunk).nblk, a.remap, cap));
+  const int spb = (a.split == 2 || a.split == 4) ? block / a.split : block;  // sites per workgroup
+  dim3 grid(tm ? tmarch_grid(a.g, block, a.tmarch, a.remap) : grid_blocks(make_walk(a.g, spb, a.chunk).nblk, a.remap, cap));
 #define QH_HOP(M, R)                                                                                 \
   if (mode == M && redm == R) {                                                                      \
     if (tm) hipLaunchKernelGGL((k_hop_tm<PL, RC, PI, PO, PA, A, M, R>), grid, dim3(block), 0, st, a);   \
+    else if (a.split == 2) hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R, 0, 2>), grid, dim3(block), 0, st, a); \
+    else if (a.split == 4) hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R, 0, 4>), grid, dim3(block), 0, st, a); \
     else if (a.two) hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R, 1>), grid, dim3(block), 0, st, a); \
     else hipLaunchKernelGGL((k_hop<PL, RC, PI, PO, PA, A, NL, M, R>), grid, dim3(block), 0, st, a);  \
     if (R) {                                                                                         \

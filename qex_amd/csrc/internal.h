@@ -62,6 +62,7 @@ struct HopArgs {
   long chunk;         // site walk (make_walk): t-march chunk, 0 = canonical order
   int tmarch;         // > 0 (plain links): t-march form, t-slices per thread (k_hop_tm)
   int two;            // 1: the stencil loop takes both links of a hop per trip (k_hop's TWO)
+  int split;          // 2 or 4: that many threads per output site (k_hop's NS); else one
 };
 
 // Fused CG iteration (two kernels): see dslash.hip.

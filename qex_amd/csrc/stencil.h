@@ -227,6 +227,55 @@ __device__ __forceinline__ void stencil_lean(const Geom &g, const char *links, l
   for (int k = k1; k < NL; k++) trip(k, std::true_type(), std::true_type());
 }
 
+// The links j in [sub NJ/NS, (sub+1) NJ/NS) of the stencil sum for one of the
+// NS adjacent threads of a site (split form of k_hop), then their sum over the
+// NS threads (lane shuffles within the group): every thread of the group ends
+// with the whole sum.  The links are read with cache-allocating loads when the
+// whole link field fits the link-cache budget (a per-launch choice, the same
+// for every thread), else non-temporally.
+template <int PL, int RC, int NL, class A, int NS, class LD>
+__device__ __forceinline__ void stencil_share(const Geom &g, const char *links, long lkblk, const float *qs, int par,
+                                              long i, const Site &s, LD &&ld, int sub, A (&res)[3]) {
+  constexpr int NJ = 2 * NL, PER = NJ / NS;
+  static_assert(NJ % NS == 0, "NS must divide the links of a site");
+  A af[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) af[c] = mk<A>(0, 0);
+  auto trip = [&](int j, auto ntc) {
+    Leg<A> L0;
+    leg_load<PL, RC, NL, A, LD, decltype(ntc)::value>(g, links, lkblk, qs, par, i, s, ld, j, L0);
+    if (!(j & 1)) {
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) cmac(af[r], L0.U[3 * r + c], L0.v[c]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) cjmsub(af[r], L0.U[3 * c + r], L0.v[c]);
+    }
+  };
+  const int j0 = sub * PER;
+  if (link_cached_blocks<NL>(g, lkblk) == NJ) {
+#pragma unroll 1
+    for (int j = j0; j < j0 + PER; j++) trip(j, std::false_type());
+  } else {
+#pragma unroll 1
+    for (int j = j0; j < j0 + PER; j++) trip(j, std::true_type());
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    auto x = af[c].x, y = af[c].y;
+#pragma unroll
+    for (int o = 1; o < NS; o <<= 1) {
+      x += __shfl_xor(x, o, 64);
+      y += __shfl_xor(y, o, 64);
+    }
+    res[c] = mk<A>(x, y);
+  }
+}
+
 // stencil in form D (the forms above); stencil() takes the build's form for PL
 template <int D, int PL, int RC, int NL, class A, class LD>
 __device__ __forceinline__ void stencil_in(const Geom &g, const char *links, long lkblk, const float *qs, int par,

@@ -510,6 +510,13 @@ def set_hop_two(two=-1):
     call("qexhip_set_hop_two", int(two))
 
 
+def set_hop_split(split=-1):
+    """Pin the one-parity hop's threads per output site: 2 or 4 split each
+    site's links between that many threads; 0 or 1 one thread per site; -1
+    lets the tuner choose."""
+    call("qexhip_set_hop_split", int(split))
+
+
 def set_hop_tmarch(slices=-1):
     """Pin the t-march form of the plain-link one-parity hop: each thread walks
     this many t-slices and keeps its forward t-neighbour in registers for the
@@ -540,13 +547,14 @@ def set_cg_form(form=0):
 
 def autotune_report():
     """{kernel family and shape: {"block", "remap", "form", "cap", "block1", "batches", "chunk",
-    "tmarch", "two", "lkcache_mb", "us"}} chosen so far (form: 2-, 3- or 4-kernel CG iteration;
+    "tmarch", "two", "split", "lkcache_mb", "us"}} chosen so far (form: 2-, 3- or 4-kernel CG iteration;
     cap: workgroups of a persistent form, 0 = one pass; block1: block size of
     the 3-kernel iteration's other-parity stencil kernel, 0 = block; batches:
     vector-load batches per link of the multi-RHS hop; chunk: t-first site-walk
     chunk, 0 = canonical order; tmarch: t-slices per thread of the hop's
     t-march form, 0 = off; two: the hop's stencil loop takes both links of a
-    hop per trip; lkcache_mb: link megabytes read with cache-allocating loads,
+    hop per trip; split: threads per output site of the hop; lkcache_mb: link
+    megabytes read with cache-allocating loads,
     for lattices that leave that budget to the tuner; us: time of one timed
     unit)."""
     buf = ctypes.create_string_buffer(1 << 20)
@@ -558,7 +566,7 @@ def autotune_report():
         out[key] = {"block": int(f["block"]), "remap": int(f["remap"]), "form": int(f["form"]),
                     "cap": int(f["cap"]), "block1": int(f.get("block1", 0)),
                     "batches": int(f.get("batches", 1)), "chunk": int(f.get("chunk", 0)),
-                    "tmarch": int(f.get("tmarch", 0)), "two": int(f.get("two", 0)),
+                    "tmarch": int(f.get("tmarch", 0)), "two": int(f.get("two", 0)), "split": int(f.get("split", 1)),
                     "lkcache_mb": int(f.get("lkcache", 0)), "us": float(f["us"])}
     return out
 

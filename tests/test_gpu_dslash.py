@@ -265,6 +265,44 @@ def test_hop_two_links_per_trip(oracle, gpu, L, hisq):
         gpu.set_hop_two(-1)
 
 
+@pytest.mark.parametrize("L", [(8, 4, 6, 4), (8, 8, 4, 8), (16, 16, 16, 16)])
+@pytest.mark.parametrize("hisq", [False, True])
+@pytest.mark.parametrize("split", [2, 4])
+def test_hop_split_threads_per_site(oracle, gpu, L, hisq, split):
+    """The hop's split form (2 or 4 threads per output site, each taking a
+    share of its links, the sums added by lane shuffles; the tuner's pick for
+    small lattices) equals the oracle: stagD2 on each parity in FP64, FP32 and
+    FP16 storage, and FP64 and mixed CG solves, whose true residuals run the
+    reducing hops."""
+    O = oracle
+    g = make_links(O, L, hisq)
+    gpu.set_hop_split(split)
+    try:
+        lo = gpu.Layout(L)
+        s = gpu.Staggered(lo, g)
+        x = O.source_gauss(L, 81, 0, 2)
+        r0 = O.source_gauss(L, 82, 1, 2)
+        for prec, tol in ((gpu.FP64, 1e-12), (gpu.FP32, 1e-5), (gpu.FP16, 4e-3)):
+            for par in (0, 1, 2):
+                fx = lo.ColorVector(prec).set(x)
+                fr = lo.ColorVector(prec).set(r0)
+                gpu.stagD2(sd_of(s, par), fr, s.g, fx, 0.5, -0.25)
+                want = r0.copy()
+                O.stagD2(L, g, par, want, x, 0.5, -0.25)
+                assert rel(fr.get(), want) < tol, (prec, par)
+        b = O.source_gauss(L, 83, 0, 0)
+        fb = lo.ColorVector().set(b)
+        for sloppy in (0, 1):
+            fs = lo.ColorVector()
+            sp = gpu.SolverParams()
+            sp.r2req, sp.verbosity, sp.sloppySolve = 1e-20, 0, sloppy
+            gpu.solveEE(s, fs, fb, 0.1, sp)
+            r = b - O.stagD2xx(L, g, 0, fs.get(), 0.01)
+            assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20, sloppy
+    finally:
+        gpu.set_hop_split(-1)
+
+
 @pytest.mark.parametrize("L,hisq", [((8, 8, 4, 8), False), ((8, 4, 6, 4), True)])
 def test_link_cache_budgets_bitwise(oracle, gpu, L, hisq):
     """The link-cache budget (the first link blocks read with
