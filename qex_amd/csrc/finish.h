@@ -98,7 +98,10 @@ __device__ inline void epi_flush(Scal *S, double rr) {
 // One-block finish shape: 256 threads, 8 partials per thread loaded before any
 // add (2048 block partials in one trip).  Measured on the CG dot-product
 // finish at 32^4: 4.7-4.8 us, against 5.5-5.6 us with 1024 threads adding one
-// partial per step, 5.0 us with 128 threads and 6.5 us with one wave.
+// partial per step, 5.0 us with 128 threads and 6.5 us with one wave.  Loading
+// the predicate's flags together with the partials (tested after the sum)
+// gave 4.76 us, within noise; loading the whole scalar struct into thread 0's
+// registers there as well, and storing it back whole, 5.68 us.
 #ifndef QH_FIN_UNROLL
 #define QH_FIN_UNROLL 8
 #endif
