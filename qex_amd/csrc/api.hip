@@ -126,7 +126,17 @@ struct qexhip_field {
   void *par(int p) const { return data + p * par_bytes; }
 };
 
+// An instantiated graph of cg_solve's launches, kept for the next solve on
+// the same links whose launches would be the same: key holds every argument
+// value the captured kernels bake in.
+struct CachedGraph {
+  std::string key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
 struct Work {
+  CachedGraph cg[2];  // cg_solve: [0] one reliable-update interval, [1] a chunk of iterations
   // FP64 parity vectors
   double *y = nullptr, *rho = nullptr, *t64 = nullptr, *r64 = nullptr, *p64 = nullptr, *Ap64 = nullptr;
   // sloppy parity vectors (sized for FP32 words, which also fit FP16/BF16)
@@ -229,6 +239,11 @@ Work *work(qexhip_links *l) {
 
 void free_work(Work *w) {
   if (!w) return;
+  HIPCHK(hipStreamSynchronize(g_stream));
+  for (CachedGraph &c : w->cg) {
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+  }
   for (void *p : {(void *)w->y, (void *)w->rho, (void *)w->t64, (void *)w->r64, (void *)w->p64, (void *)w->Ap64,
                   w->xs, w->rs, w->Aps, w->ps, w->ts, (void *)w->S, (void *)w->S1})
     if (p) (void)hipFree(p);
@@ -320,6 +335,24 @@ void capture(const std::function<void()> &f, hipGraph_t *graph, hipGraphExec_t *
   }
   HIPCHK(hipStreamEndCapture(g_stream, graph));
   HIPCHK(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+}
+
+// Bumped by every launch-shape setter of the C ABI: a graph cached before
+// the call may hold launches of an earlier shape.
+long g_cfg_gen = 0;
+
+// c's graph when its key is key, else f captured into c (replacing the
+// graph there).  The caller has synchronized the stream since c's last launch.
+hipGraphExec_t cached_graph(CachedGraph &c, const std::string &key, const std::function<void()> &f) {
+  if (c.exec && c.key == key) return c.exec;
+  if (c.exec) (void)hipGraphExecDestroy(c.exec);
+  if (c.graph) (void)hipGraphDestroy(c.graph);
+  c.exec = nullptr;
+  c.graph = nullptr;
+  c.key.clear();
+  capture(f, &c.graph, &c.exec);
+  c.key = key;
+  return c.exec;
 }
 
 // f(t) runs the kernel sequence once with shape t; the candidates are every
@@ -666,6 +699,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   Sh->r2stop = r2req * b2;
   push(S, Sh);
   CgArgs a;
+  memset(&a, 0, sizeof a);  // padding included: the bytes of a are part of a graph key
   a.g = g;
   a.links = links_store(l, pl, rc);
   a.lkblk = lkblk_bytes(l, pl, rc);
@@ -715,18 +749,20 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   // per kChunk iterations plus three per update.  An interval cut short by an
   // earlier trigger switches the rest of the solve to chunks, whose idle tail
   // after a trigger is at most kChunk - 1 iterations.
-  hipGraph_t graph = nullptr, igraph = nullptr;
+  //
+  // The graphs stay with the links' workspace: a later solve whose launches
+  // bake in the same values (kernel arguments, shapes, x and b, interval)
+  // replays them instead of capturing again (0.5-1.6 ms per capture).
   hipGraphExec_t exec = nullptr, iexec = nullptr;
-  struct GraphGuard {
-    hipGraph_t &g, &ig;
-    hipGraphExec_t &e, &ie;
-    ~GraphGuard() {
-      if (e) (void)hipGraphExecDestroy(e);
-      if (g) (void)hipGraphDestroy(g);
-      if (ie) (void)hipGraphExecDestroy(ie);
-      if (ig) (void)hipGraphDestroy(ig);
-    }
-  } guard{graph, igraph, exec, iexec};
+  auto graph_key = [&](int kind, int n) {
+    const Tuned hop = tuned_hop(l, P64, 18);  // the FP64 hops of resid64
+    const long v[] = {kind, n, tu.block, tu.remap, tu.form, tu.cap, tu.block1, tu.lkcache, pl, rc, ps, rx,
+                      l->nl, mixed, (long)(uintptr_t)x, (long)(uintptr_t)b, hop.block, hop.remap, hop.cap,
+                      hop.chunk, hop.tmarch, hop.two, hop.split, hop.lkcache, g_cfg_gen};
+    std::string k((const char *)&a, sizeof a);
+    k.append((const char *)v, sizeof v);
+    return k;
+  };
   auto chunk = [&](int n) {
     require(launch_cg_chunk(pl, rc, ps, l->nl, a, g_stream, tu.block, tu.form, rx, tu.block1, n, w->S1, tu.cap),
             "no CG kernels for this storage combination");
@@ -752,15 +788,13 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   const int nint = Sh->relmax;
   bool intervals = mixed && g_cg_graph && g_cg_intervals && nint <= kMaxInterval && !Sh->done;
   if (intervals) {
-    capture(
-        [&] {
-          chunk(nint);
-          update();
-          launch_keep_best(g, w->p64, x, S, g_stream);
-        },
-        &igraph, &iexec);
+    iexec = cached_graph(w->cg[0], graph_key(0, nint), [&] {
+      chunk(nint);
+      update();
+      launch_keep_best(g, w->p64, x, S, g_stream);
+    });
   } else if (g_cg_graph && !Sh->done) {
-    capture([&] { chunk(kChunk); }, &graph, &exec);
+    exec = cached_graph(w->cg[1], graph_key(1, kChunk), [&] { chunk(kChunk); });
   }
   while (!Sh->done) {
     if (intervals) {
@@ -771,7 +805,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
       if (Sh->rollback) rollback();
       if (!Sh->done && Sh->its - its0 < nint) {
         intervals = false;
-        capture([&] { chunk(kChunk); }, &graph, &exec);
+        exec = cached_graph(w->cg[1], graph_key(1, kChunk), [&] { chunk(kChunk); });
       }
       continue;
     }
@@ -2692,12 +2726,14 @@ int qexhip_solve_multishift_batch(qexhip_links *l, int par, int nrhs, qexhip_fie
 }
 
 int qexhip_set_autotune(int on) {
+  g_cfg_gen++;
   g_autotune = on != 0;
   return 0;
 }
 
 int qexhip_set_cg_block1(int block1) {
   API_BEGIN
+  g_cfg_gen++;
   require(block1 == 0 || block1 == 64 || block1 == 128 || block1 == 256, "block1 must be 0 (tuner), 64, 128 or 256");
   g_cg_block1 = block1;
   API_END
@@ -2705,6 +2741,7 @@ int qexhip_set_cg_block1(int block1) {
 
 int qexhip_set_cg_cap(int cap) {
   API_BEGIN
+  g_cfg_gen++;
   require(cap >= -1 && cap <= (1 << 20), "cg cap must be -1 (tuner), 0 or a workgroup count up to 2^20");
   g_cg_cap = cap;
   API_END
@@ -2712,6 +2749,7 @@ int qexhip_set_cg_cap(int cap) {
 
 int qexhip_set_link_cache(qexhip_lattice *lat, long bytes) {
   API_BEGIN
+  g_cfg_gen++;
   require(bytes >= -1, "link cache bytes must be -1 (default) or >= 0");
   if (lat) {
     lat->g.lkcache = bytes < 0 ? g_link_cache : bytes;
@@ -2723,6 +2761,7 @@ int qexhip_set_link_cache(qexhip_lattice *lat, long bytes) {
 
 int qexhip_set_hop_walk(long chunk) {
   API_BEGIN
+  g_cfg_gen++;
   require(chunk == -1 || (chunk >= 0 && chunk % 256 == 0), "site-walk chunk must be -1 (tuner), 0 or a multiple of 256");
   g_hop_chunk = chunk;
   API_END
@@ -2730,6 +2769,7 @@ int qexhip_set_hop_walk(long chunk) {
 
 int qexhip_set_hop_split(int split) {
   API_BEGIN
+  g_cfg_gen++;
   require(split == -1 || split == 0 || split == 1 || split == 2 || split == 4,
           "hop threads per site must be -1 (tuner), 0 or 1 (one), 2 or 4");
   g_hop_split = split;
@@ -2738,6 +2778,7 @@ int qexhip_set_hop_split(int split) {
 
 int qexhip_set_hop_two(int two) {
   API_BEGIN
+  g_cfg_gen++;
   require(two >= -1 && two <= 1, "hop stencil trip must be -1 (tuner), 0 (one link) or 1 (both links of a hop)");
   g_hop_two = two;
   API_END
@@ -2745,6 +2786,7 @@ int qexhip_set_hop_two(int two) {
 
 int qexhip_set_hop_tmarch(int slices) {
   API_BEGIN
+  g_cfg_gen++;
   require(slices >= -1 && slices <= 1024, "t-march slices per thread must be -1 (tuner), 0 (off) or 1..1024");
   g_hop_tmarch = slices;
   API_END
@@ -2752,6 +2794,7 @@ int qexhip_set_hop_tmarch(int slices) {
 
 int qexhip_set_msb_walk(long chunk) {
   API_BEGIN
+  g_cfg_gen++;
   require(chunk == -1 || (chunk >= 0 && chunk % 256 == 0), "site-walk chunk must be -1 (tuner), 0 or a multiple of 256");
   g_msb_walk = chunk;
   API_END
@@ -2759,6 +2802,7 @@ int qexhip_set_msb_walk(long chunk) {
 
 int qexhip_set_hop_multi_shape(int batches, long chunk) {
   API_BEGIN
+  g_cfg_gen++;
   require(batches >= 0 && batches <= 2, "vector batches must be 0 (tuner), 1 or 2");
   require(chunk == -1 || (chunk >= 0 && chunk % 256 == 0), "site-walk chunk must be -1 (tuner), 0 or a multiple of 256");
   g_multi_batches = batches;
@@ -2768,6 +2812,7 @@ int qexhip_set_hop_multi_shape(int batches, long chunk) {
 
 int qexhip_set_cg_graph(int on) {
   API_BEGIN
+  g_cfg_gen++;
   require(on >= 0 && on <= 2, "cg graph mode must be 0, 1 or 2");
   g_cg_graph = on != 0;
   g_cg_intervals = on == 1;
@@ -2776,6 +2821,7 @@ int qexhip_set_cg_graph(int on) {
 
 int qexhip_set_cg_form(int form) {
   API_BEGIN
+  g_cfg_gen++;
   require(form == 0 || (form >= 2 && form <= 4), "cg form must be 0 (tuned), 2, 3 or 4");
   g_cg_form = form;
   API_END
