@@ -522,3 +522,39 @@ def test_cg_block1_independent_of_block(oracle, gpu, form, block1):
     r = b - O.stagD2xx(L, g, 0, x, 0.05 ** 2)
     assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20
     assert rel(x[: lo.nEven], want[: lo.nEven]) < 1e-9
+
+
+@pytest.mark.parametrize("sloppy", [0, 1, 2])
+def test_graphs_kept_across_solves(oracle, gpu, sloppy):
+    """Solves on the same links, x and b replay the graphs the first one
+    captured: masses m1, m2, m1 (the mass is read from the device scalars,
+    not baked into the graph), then m1 again after a launch-shape setter
+    (which retires the kept graphs).  Each reaches the oracle's solution and
+    a true residual below the target; the repeated m1 solves are bitwise
+    equal to the first."""
+    O = oracle
+    L = (8, 8, 8, 8)
+    g = links(O, L, True)
+    b = O.source_gauss(L, 13, 0, 0)
+    lo = gpu.Layout(L)
+    s = gpu.Staggered(lo, g)
+    fb = lo.ColorVector().set(b)
+    fx = lo.ColorVector()
+    out = []
+    for k, m in enumerate((0.05, 0.2, 0.05, 0.05)):
+        if k == 3:
+            gpu.set_cg_graph(True)  # a setter: the kept graphs are captured again
+        fx.set(np.zeros_like(b))  # same x storage, zero start
+        sp = gpu.SolverParams()
+        sp.r2req, sp.verbosity, sp.sloppySolve = 1e-20, 0, sloppy
+        gpu.solveEE(s, fx, fb, m, sp)
+        got = fx.get(sub=0)[: lo.nEven].copy()
+        print(f"sloppy {sloppy} solve {k} m {m}: {sp.iterations} iterations", flush=True)
+        want, _, _ = O.solveXX(L, g, 0, b, m, 1e-24)
+        assert rel(got, want[: lo.nEven]) < (1e-9 if sloppy == 0 else 1e-6), (k, m)
+        r = b - O.stagD2xx(L, g, 0, fx.get(), m * m)
+        assert O.norm2(r[: lo.nEven]) / O.norm2(b) <= 1.0001e-20, (k, m)
+        out.append((got, sp.iterations))
+    for k in (2, 3):
+        assert out[k][1] == out[0][1]
+        assert np.array_equal(out[k][0], out[0][0]), k
