@@ -796,6 +796,7 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
   } else if (g_cg_graph && !Sh->done) {
     exec = cached_graph(w->cg[1], graph_key(1, kChunk), [&] { chunk(kChunk); });
   }
+  double r2last = Sh->r2true;  // true residual^2 at the last reliable update
   while (!Sh->done) {
     if (intervals) {
       const int its0 = Sh->its;
@@ -803,7 +804,17 @@ void cg_solve(qexhip_links *l, int par, double *x, const double *b, double m, do
       check_launch();
       pull(Sh, S);
       if (Sh->rollback) rollback();
-      if (!Sh->done && Sh->its - its0 < nint) {
+      bool to_chunks = !Sh->done && Sh->its - its0 < nint;
+      if (!Sh->done && !to_chunks && Sh->r2true > 0.0 && Sh->r2true < r2last) {
+        // the true residual's decay over this interval predicts the
+        // iterations left; with fewer than one interval to go, finish in
+        // chunks, whose idle tail after convergence is under kChunk
+        // iterations instead of under nint
+        const double per_it = std::log(Sh->r2true / r2last) / (double)(Sh->its - its0);
+        to_chunks = std::log(Sh->r2stop / Sh->r2true) / per_it < (double)nint;
+      }
+      r2last = Sh->r2true;
+      if (to_chunks) {
         intervals = false;
         exec = cached_graph(w->cg[1], graph_key(1, kChunk), [&] { chunk(kChunk); });
       }
